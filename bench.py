@@ -1,0 +1,146 @@
+"""Benchmark: one plain-MLEM iteration at 512^3 / 720 views / 768^2 detector.
+
+A step is one full MLEM iteration (batch = 1, i.e. one subset holding all 720
+views): the fused forward projection + L/(factor+1e-5) normalisation + MLEM
+ratio (one launch over all views) followed by the fused backprojection +
+multiplicative update (one launch).  Inputs are resident in HBM before timing.
+
+Prints one JSON line.  value = GUPS over the step = 2 * N^3 * P voxel-view
+updates (forward + back) per second / 1e9.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-to-residual 1e-3 (learned warm-start vs plain MLEM) 512³; GUPS fwd/back"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_VOXEL_VIEW = 4.0  # one FP32 volume stream per view (plain forward / back)
+
+
+def cpu_baseline(N, U, P, sdd, sod, det, vs, n_views=2):
+    """Time the oracle's FP32 restatement (the reference's operators on the CPU)
+    on a bounded sample: forward + back of n_views views of the same workload."""
+    import __graft_entry__ as entry
+
+    orc = entry.load_oracle()
+    xr = entry.load_package()
+    case = orc.Case(sdd, sod, det, N, U, P, vox_size=vs)
+    params = xr.ellipsoid_params(1234, 40, fov_radius=0.5 * N * vs)
+    vol = orc.phantom_ellipsoids(N, vs, params)
+    tmp = np.zeros(N ** 3)
+    fac = np.zeros(N ** 3)
+    img = np.ones((U, U), np.float32)
+    views = [int(round(i * P / n_views)) for i in range(n_views)]
+    t0 = time.perf_counter()
+    for v in views:
+        case.forward_view(vol, v, prec=orc.REF32)
+        case.backward_view(img, v, tmp, fac, prec=orc.REF32)
+    dt = time.perf_counter() - t0
+    gups = 2.0 * N ** 3 * len(views) / dt / 1e9
+    return {"value": gups, "unit": "GUPS", "cores": int(orc.num_threads()), "kind": "port",
+            "sample": "FP32 oracle forward+back of %d of %d views at %d^3, %dx%d detector (%.1f s)"
+                      % (len(views), P, N, U, U, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n-vox", type=int, default=512)
+    ap.add_argument("--n-det", type=int, default=768)
+    ap.add_argument("--n-proj", type=int, default=720)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-views", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (views would shard "
+              "across GPUs with an all-reduce of the backprojection; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import __graft_entry__ as entry
+
+    xr = entry.load_package()
+    dev = torch.device("cuda:0")
+    N, U, P = args.n_vox, args.n_det, args.n_proj
+    sdd, sod, det = 1000.0, 500.0, 0.4
+    vs = xr.default_vox_size(sdd, sod, det)  # 0.2 mm (geometry.h:13)
+    geom = xr.make_geometry(sdd, sod, det, N, U, P)
+    scan = xr.Scan()
+    ctx = xr.Context(0)
+    plan = ctx.plan(geom, scan)
+    N3 = N ** 3
+    lenD = P * U * U
+
+    # synthetic phantom (40 seeded ellipsoids) and its sinogram by the same operator
+    params = xr.ellipsoid_params(1234, 40, fov_radius=0.5 * N * vs)
+    vref = torch.zeros(N3, device=dev)
+    ctx.phantom_rasterise(params, N, vs, vref)
+    truth = torch.empty_like(vref)
+    ctx.to_columns(vref, truth, N)
+    del vref
+    codes = plan.codes(range(P))
+    sino = torch.zeros(lenD, device=dev)
+    plan.project(truth, sino, codes, out_mode=xr.OUT_NORMALISED)
+    del truth
+    vol = torch.full((N3,), 0.01, device=dev)
+    proj = torch.zeros(lenD, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        loss = plan.project(vol, proj, codes, out_mode=xr.OUT_RATIO, sino=sino)
+        plan.backproject_update(proj, codes, vol, method=xr.METHOD_MLEM, subset_size=P)
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    ctx.timing_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    fwd_ms, fwd_n = ctx.timing_read(xr.TIMING_FORWARD)
+    bwd_ms, bwd_n = ctx.timing_read(xr.TIMING_BACKWARD)
+    ctx.timing(False)
+
+    vv = float(N3) * P  # voxel-view updates per pass
+    ms_per_step = dt * 1e3 / args.steps
+    gups = 2.0 * vv * args.steps / dt / 1e9
+    fwd_avg, bwd_avg = fwd_ms / max(fwd_n, 1), bwd_ms / max(bwd_n, 1)
+    dom_name, dom_ms = ("backprojection+update", bwd_avg) if bwd_avg >= fwd_avg else ("forward+ratio", fwd_avg)
+    achieved = BYTES_PER_VOXEL_VIEW * vv / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
+                "kernel_ms": round(dom_ms, 3),
+                "algorithmic_bytes": "4 B per voxel-view x %d^3 voxels x %d views per launch" % (N, P)}
+    out = {
+        "metric": METRIC, "value": round(gups, 2), "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (40 seeded ellipsoids, sinogram from the projector)",
+        "config": {"workload": "plain MLEM iteration (batch 1), %d^3 volume, %d views, %dx%d detector, "
+                               "SOD 500 SDD 1000 det 0.4 mm" % (N, P, U, U),
+                   "n_vox": N, "n_proj": P, "n_det": U, "parallelism": "single GPU"},
+        "gups_forward": round(vv / (fwd_avg * 1e-3) / 1e9, 2), "gups_backward": round(vv / (bwd_avg * 1e-3) / 1e9, 2),
+        "forward_ms": round(fwd_avg, 3), "backward_ms": round(bwd_avg, 3),
+        "loss_first_last": [losses[0], losses[-1]],
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(N, U, P, sdd, sod, det, vs, args.cpu_views)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

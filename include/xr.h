@@ -1,0 +1,247 @@
+/*
+ * xr.h — C ABI of the MI355X cone-beam reconstruction library (libxr.so).
+ *
+ * This is the drop-in boundary for the reference's reconstruction hot path.
+ * The reference has no FFI; its interface is the C++ host API in
+ * reconstruct/include/reconstruct.cuh:14-73, direction/include/pca.cuh:9-12 and
+ * reconstruct/include/params.h:37.  Each entry point below names the reference
+ * function or kernel it replaces.  The reference-compatible C++ API
+ * (include/xr/reconstruct.hpp) is a thin header over these functions.
+ *
+ * Conventions
+ *  - Every function returns XR_OK (0) or a negative XR_ERR_* code; the message
+ *    of the last failure on a context is available from xr_last_error().
+ *  - Plain pointers and sizes only.  "host" buffers are ordinary CPU memory;
+ *    "device" buffers are HIP device memory of the context's device.
+ *  - Volumes at the host boundary use the reference layout (volume.h:65-66):
+ *    x fastest, then y, then z; channel k at k*N^3.  Device-level entry points
+ *    take volumes in the library's column layout (z fastest, then x, then y;
+ *    convert with xr_volume_to_columns / xr_volume_from_columns).
+ *  - Sinograms use the reference layout u + U*(v + V*view), one block of
+ *    U*V*P floats per condition (ir.cu:1518, reconstruct.cu:816).
+ *  - A context owns one HIP stream (the caller's, see xr_context_set_stream)
+ *    and is not thread-safe; use one context per host thread.
+ */
+#ifndef XR_H
+#define XR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XR_OK 0
+#define XR_ERR_INVALID (-1)     /* bad argument / unsupported size */
+#define XR_ERR_HIP (-2)         /* a HIP runtime call or kernel launch failed */
+#define XR_ERR_ALLOC (-3)       /* device or host allocation failed */
+#define XR_ERR_IO (-4)          /* settings / raw file could not be read */
+#define XR_ERR_NOT_FOUND (-5)   /* settings tag or key missing */
+
+/* Projector conventions (parity-critical, see the reference files cited). */
+#define XR_CONV_IR 0   /* source R(0,+sod,0), voxel centre x-(N+1)/2 (ir.cu:1874-1884) */
+#define XR_CONV_FDK 1  /* source R(0,-sod,0), voxel centre x-(N-1)/2 (fdk.cu:125-135) */
+
+/* Rotate (reconstruct.cuh:19-24): CW -> view n at +2 pi n/P, CCW -> -2 pi n/P. */
+#define XR_CW 0
+#define XR_CCW 1
+
+/* Method (reconstruct.cuh:14-18), same numbering. */
+#define XR_METHOD_XTT 0
+#define XR_METHOD_MLEM 1
+#define XR_METHOD_ART 2
+#define XR_METHOD_FDK 3
+
+/* FDK filters: the reference's kernel 1/(pi^2 d (1-4m^2)) (reconstruct.cu:1694-1699)
+ * or Ram-Lak in the same scaling convention. */
+#define XR_FILTER_REFERENCE 0
+#define XR_FILTER_RAMLAK 1
+
+/* Forward-projection output modes (xr_project). */
+#define XR_OUT_RAW 0         /* sum_c c*ratio*mu, un-normalised splat (ir.cu:1518-1525) */
+#define XR_OUT_NORMALISED 1  /* ... * L/(factor+1e-5) (ir.cu:1602) */
+#define XR_OUT_RATIO 2       /* MLEM ratio y/p clamped at 2 (projRatio, ir.cu:1616-1635) */
+#define XR_OUT_RESIDUAL 3    /* ART residual y-p (projSubtract, ir.cu:1656-1666) */
+
+/* Mirrors Geometry (reconstruct/include/geometry.h:8-25): same field order, the
+ * detector width and height split (the reference uses `detect` for both).
+ * vox_size is explicit; the reference derives sod*det_size/sdd (geometry.h:13),
+ * which xr_default_vox_size() returns. */
+typedef struct xr_geometry {
+    float sdd, sod, vox_size, det_size;
+    int32_t n_vox, n_det_u, n_det_v, n_proj;
+} xr_geometry;
+
+/* Per-condition scan tables the reference keeps in __constant__ / __managed__
+ * memory (params.cu:31-36): cond_R = elemR (9 per condition), cond_t = elemT
+ * (3 per condition, areaTrans already added), cond_offset = INIT_OFFSET
+ * (3 per condition), basis = basisVector (3 per basis vector).  grating is an
+ * optional per-condition grating vector (3 per condition; NULL -> z, as in
+ * ir.cu:1901).  No fixed caps: n_cond is unbounded, n_basis <= 16 (the reference
+ * caps 13 conditions and 7 basis vectors, params.h:26-30). */
+typedef struct xr_scan {
+    int32_t n_cond, n_basis;
+    const float* cond_R;
+    const float* cond_t;
+    const float* cond_offset;
+    const float* basis;
+    const float* grating;
+} xr_scan;
+
+typedef struct xr_context xr_context;
+typedef struct xr_plan xr_plan;
+
+/* ---- context and plans ------------------------------------------------- */
+
+/* Create a context on HIP device `device` (default null stream). */
+int xr_context_create(xr_context** out, int device);
+void xr_context_destroy(xr_context* ctx);
+/* Use `hip_stream` (a hipStream_t, NULL = null stream) for all further work. */
+int xr_context_set_stream(xr_context* ctx, void* hip_stream);
+int xr_context_synchronize(xr_context* ctx);
+const char* xr_last_error(const xr_context* ctx);
+
+/* Kernel timing: when enabled every launch is bracketed by HIP events on the
+ * context's stream.  kind: 0 forward projection, 1 backprojection, 2 FDK filter,
+ * 3 other.  xr_timing_read synchronises and returns the summed milliseconds and
+ * the launch count since the last reset. */
+int xr_timing_enable(xr_context* ctx, int enable);
+int xr_timing_reset(xr_context* ctx);
+int xr_timing_read(xr_context* ctx, int kind, double* total_ms, int64_t* count);
+
+/* A plan holds the per-(condition, view) projection tables for one geometry,
+ * scan, convention and direction; the L/(factor+1e-5) normalisation table is
+ * built on first use (forward projection's correlationProjByLength,
+ * ir.cu:1533-1603).  Replaces init_params' constant-memory upload
+ * (params.cu:80-95) and the per-thread geometry of rayCasting. */
+int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int convention, int direction,
+                   xr_plan** out);
+void xr_plan_destroy(xr_plan* plan);
+
+float xr_default_vox_size(float sdd, float sod, float det_size);
+
+/* ---- device-level operators ------------------------------------------- */
+
+/* Reference layout <-> column layout, `channels` volumes of n_vox^3 (device). */
+int xr_volume_to_columns(xr_context* ctx, const float* ref_dev, float* col_dev, int n_vox, int channels);
+int xr_volume_from_columns(xr_context* ctx, const float* col_dev, float* ref_dev, int n_vox, int channels);
+
+/* Forward projection of the listed views.  Replaces forwardProj /
+ * forwardProjXTT (ir.cu:1480-1531, 133-141, 1753-1804) followed by
+ * correlationProjByLength (ir.cu:1533-1603) and projRatio / projSubtract
+ * (ir.cu:1616-1666).
+ *   views: host array of n_views codes cond*n_proj + view (view = sinogram index).
+ *   accumulate: optional host array (n_views bytes); 1 -> the splat is added to
+ *     the image already in proj (the reference zeroes proj once per epoch,
+ *     reconstruct.cu:83), 0 -> the image is overwritten.
+ *   vol: column layout; plain: n_cond volumes of N^3 (volume cond*N^3 is used
+ *     for condition cond); xtt != 0: n_basis channels of N^3.
+ *   proj, sino: n_cond*U*V*P (device).  sino is read for RATIO / RESIDUAL.
+ *   loss: optional host out, sum |y - p| over the projected pixels (FP64). */
+int xr_project(xr_context* ctx, xr_plan* plan, const float* vol, float* proj, const float* sino,
+               const int32_t* views, int n_views, const uint8_t* accumulate, int xtt, int out_mode, double* loss);
+
+/* Raw backprojection of the listed views into column-layout accumulators:
+ * tmp += sum_c c*r, factor += 1 (backwardProj, ir.cu:1605-1614, 1720-1751) or,
+ * xtt != 0, per channel tmp_k += v_km^2 sum_c c*r, factor_k += v_km^2
+ * (backwardProjXTT, ir.cu:143-152, 1807-1850). */
+int xr_backproject(xr_context* ctx, xr_plan* plan, const float* proj, const int32_t* views, int n_views, int xtt,
+                   float* tmp, float* factor);
+
+/* Backprojection of the listed views fused with the volume update:
+ * MLEM mu = factor==0 ? 0 : mu*tmp/factor (voxelProduct, ir.cu:1637-1654) or ART
+ * mu += lambda/subset_size*tmp (voxelPlus, ir.cu:1682-1693); per channel for xtt. */
+int xr_backproject_update(xr_context* ctx, xr_plan* plan, const float* proj, const int32_t* views, int n_views,
+                          int xtt, int method, float lambda, int subset_size, float* vol);
+
+/* FDK pieces (fdk.cu:13-43, 51-170).  filter: XR_FILTER_*; the plan must use
+ * XR_CONV_FDK for the backprojection.  sino/filt: n_cond*U*V*P (device);
+ * vol: column layout N^3 of the condition of the listed views. */
+int xr_fdk_filter(xr_context* ctx, xr_plan* plan, int filter, const float* sino, float* filt, const int32_t* views,
+                  int n_views);
+int xr_fdk_backproject(xr_context* ctx, xr_plan* plan, const float* filt, const int32_t* views, int n_views,
+                       int overwrite, float* vol);
+/* The filter taps the FDK path uses (n_det_u floats, host out). */
+int xr_fdk_taps(const xr_geometry* g, int filter, float* taps_out);
+
+/* voxelSqrt (ir.cu:1707-1717): v = sqrt(|v|) for n floats (device). */
+int xr_volume_sqrt(xr_context* ctx, float* vol, int64_t n);
+
+/* calcEigenVector (direction/src/pca.cu:12-65) for nvox voxels: mu has n_basis
+ * channels of chan_stride floats; md and evalue get 3 channels each (device). */
+int xr_pca(xr_context* ctx, const float* mu, int n_basis, const float* basis_host, int64_t nvox, int64_t chan_stride,
+           float* md, float* evalue);
+
+/* ---- host-buffer solvers (the reference's host API) ------------------- */
+
+/* Per-epoch loss sums (raw sums of |y-p|; the reference writes them divided by
+ * U*V*P to loss.csv, reconstruct.cu:153-155). */
+typedef struct xr_stats {
+    double* losses; /* caller-provided array of `epochs` doubles, or NULL */
+} xr_stats;
+
+/* IR::reconstruct (reconstruct.cu:23-156): OS-EM / OS-ART.  sino[c], vol[c]
+ * are host arrays (reference layouts) for each condition; vol is the initial
+ * guess on entry and the result on exit. */
+int xr_ir_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, float* const* sino, float* const* vol,
+                      int epochs, int batch, int direction, int method, float lambda, xr_stats* stats);
+
+/* XTT::reconstruct (reconstruct.cu:799-954): vol[k] (n_basis channels) in/out;
+ * sino[c] is overwritten with the final ratio / residual images
+ * (reconstruct.cu:924-925); md[0..2] receives the principal orientation
+ * (calcEigenVector); evalue[0..2] (optional, may be NULL) the normalised
+ * eigenvalues. */
+int xr_xtt_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, float* const* sino, float* const* vol,
+                       float* const* md, float* const* evalue, int epochs, int batch, int direction, int method,
+                       float lambda, xr_stats* stats);
+
+/* FDK::reconstruct (reconstruct.cu:1651-1724): vol[c] out (zero-initialised,
+ * a deviation: the reference leaves its device buffers uninitialised); sino[c]
+ * is overwritten with the filtered sinogram, as in the reference. */
+int xr_fdk_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, float* const* sino, float* const* vol,
+                       int direction, int filter);
+
+/* ---- settings (init_params, params.cu:42-104) -------------------------- */
+
+/* Parsed settings.json entry for one tag, same schema as the reference's file
+ * (utility/settings.json).  String fields are NUL-terminated.  The per
+ * condition tables are already selected by `index` when `sorting` is true and
+ * have areaTrans added (params.cu:74-95). */
+typedef struct xr_settings {
+    char proj_path[512];
+    char vol_path[512];
+    char direc_path[512];
+    int32_t block_size;
+    int32_t n_basis;       /* recon_variable.vector */
+    int32_t n_cond;        /* recon_variable.condition */
+    float sod, sdd, det_size;
+    int32_t n_proj, n_det_u, n_det_v, n_vox;
+    int32_t load_index[64];  /* LOAD_INDEX (1-based file indices) */
+    float cond_R[9 * 64];
+    float cond_t[3 * 64];
+    float cond_offset[3 * 64];
+    float basis[3 * 16];
+} xr_settings;
+
+int xr_load_settings(const char* json_path, const char* tag, xr_settings* out);
+/* Geometry/scan views of a loaded settings entry (vox_size from geometry.h:13). */
+int xr_settings_geometry(const xr_settings* st, xr_geometry* g, xr_scan* s);
+
+/* ---- synthetic inputs ---------------------------------------------------- */
+
+/* Seeded random ellipsoids (16 doubles each: centre[3] mm, semi-axes[3] mm,
+ * rotation[9] row-major body<-world, value), from a SplitMix64 stream of `seed`:
+ * centres uniform in a ball of radius centre_frac*fov_radius, semi-axes
+ * uniform in [axis_lo, axis_hi]*fov_radius, uniform random rotation, value
+ * uniform in [value_lo, value_hi].  Host only. */
+int xr_phantom_ellipsoid_params(uint64_t seed, int count, double fov_radius, double centre_frac, double axis_lo,
+                                double axis_hi, double value_lo, double value_hi, double* params_out);
+/* Rasterise ellipsoids at voxel centres (x-(N+1)/2)*vs into a reference-layout
+ * device volume (values of overlapping ellipsoids add). */
+int xr_phantom_rasterise(xr_context* ctx, const double* params_host, int count, int n_vox, float vox_size,
+                         float* vol_ref_dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XR_H */

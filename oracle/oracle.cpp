@@ -1,0 +1,882 @@
+// CPU oracle for the reconstruction hot path — TEST INFRASTRUCTURE ONLY.
+//
+// This file restates, on the CPU, the reference's voxel-driven cone-beam
+// operators, their MLEM/ART/XTT/FDK orchestration and the per-voxel PCA, so
+// that the HIP product path can be checked against it.  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+// only as the checker or the timed CPU baseline.  The product never links it.
+//
+// Every function is templated on the working precision T:
+//   T = float  : "ref32" — the reference's FP32 arithmetic in its operation
+//                order (theta in FP32, cosf/sinf, Matrix3f products in the
+//                reference's loop order).  nvcc's FMA contraction and the
+//                last ulp of CUDA's cosf/sinf cannot be reproduced here, so
+//                ref32 is the reference up to that rounding.
+//   T = double : "f64"   — the same operator evaluated in FP64 (theta is the
+//                reference's FP32 angle, promoted), used as the exact target.
+//
+// Reference files followed (paths relative to the reference repository):
+//   reconstruct/src/ir.cu:1480-1531   forwardProj / forwardonDevice       (orc_forward_view)
+//   reconstruct/src/ir.cu:1533-1603   correlationProjByLength             (orc_chord_length)
+//   reconstruct/src/ir.cu:1616-1666   projRatio / projSubtract            (ratio_step)
+//   reconstruct/src/ir.cu:1637-1693   voxelProduct / voxelPlus            (update_step)
+//   reconstruct/src/ir.cu:1707-1717   voxelSqrt                           (orc_xtt_reconstruct)
+//   reconstruct/src/ir.cu:1720-1751   backwardonDevice                    (orc_backward_view)
+//   reconstruct/src/ir.cu:1753-1850   forward/backwardXTTonDevice         (orc_forward_view, orc_backward_view)
+//   reconstruct/src/ir.cu:1852-1902   rayCasting                          (raycast)
+//   reconstruct/src/fdk.cu:13-43      calcWeight / projConv               (orc_fdk_*)
+//   reconstruct/src/fdk.cu:62-170     FDK backwardonDevice                (orc_fdk_backproject_view)
+//   reconstruct/src/reconstruct.cu:23-156     IR::reconstruct             (orc_ir_reconstruct)
+//   reconstruct/src/reconstruct.cu:799-954    XTT::reconstruct            (orc_xtt_reconstruct)
+//   reconstruct/src/reconstruct.cu:1651-1724  FDK::reconstruct            (orc_fdk_reconstruct)
+//   reconstruct/include/geometry.h:11-14      voxSize = sod*detSize/sdd
+//   reconstruct/include/vec.h:93-109          normalize (eps 1e-8) / norm2
+//   direction/src/pca.cu:12-65                calcEigenVector             (orc_pca)
+//
+// Pinning: see tests/test_oracle_*.py.  Closed forms, invariants (splat/gather
+// adjointness, linearity, 90-degree rotation symmetry, MLEM fixed point) and
+// textbook routines (numpy eigh / convolve) pin each function.  Parity
+// unpinned: the degenerate chord branch of correlationProjByLength when a pixel
+// ray meets more than two box faces (the reference overruns a 6-float buffer
+// there; this oracle keeps the first two hits), and the eigenvector choice at
+// (near-)degenerate eigenvalues in calcEigenVector (Eigen3, unpinned version).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" {
+// Same field order as the reference Geometry (geometry.h:16-24), with the
+// detector width and height split (the reference uses one `detect` for both).
+typedef struct {
+    float sdd, sod, vox_size, det_size;
+    int32_t n_vox, n_det_u, n_det_v, n_proj;
+} orc_geometry;
+
+// Per-condition tables that the reference keeps in __constant__ memory
+// (params.cu:31-36): elemR (9/cond), elemT (3/cond, areaTrans already added),
+// INIT_OFFSET (3/cond) and basisVector (3/basis).  `grating` is an optional
+// per-condition grating vector (3/cond, NULL -> z axis as in ir.cu:1901).
+typedef struct {
+    int32_t n_cond, n_basis;
+    const float* cond_R;
+    const float* cond_t;
+    const float* cond_offset;
+    const float* basis;
+    const float* grating;
+} orc_scan;
+}
+
+namespace {
+
+constexpr float kPIf = (float)M_PI;
+
+// theta exactly as ir.cu:1860 evaluates it: 2.0f*(float)M_PI*(float)n/(float)P in FP32.
+inline float theta_f32(int n, int P) {
+    volatile float a = 2.0f * kPIf;
+    volatile float b = a * (float)n;
+    volatile float c = b / (float)P;
+    return c;
+}
+
+template <class T> inline T tcos(T a);
+template <class T> inline T tsin(T a);
+template <> inline float tcos<float>(float a) { return cosf(a); }
+template <> inline float tsin<float>(float a) { return sinf(a); }
+template <> inline double tcos<double>(double a) { return std::cos(a); }
+template <> inline double tsin<double>(double a) { return std::sin(a); }
+
+template <class T> struct V3 {
+    T x, y, z;
+    T operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+template <class T> inline T dot(const V3<T>& a, const V3<T>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> inline V3<T> operator-(const V3<T>& a, const V3<T>& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> inline V3<T> operator+(const V3<T>& a, const V3<T>& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> inline V3<T> scale(T s, const V3<T>& a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class T> inline V3<T> cross(const V3<T>& a, const V3<T>& b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T> inline T norm2(const V3<T>& a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+// Matrix3X * Vector3X (vec.h:192-200): w[i] accumulated from 0 over j.
+template <class T> inline V3<T> mv(const T m[9], const V3<T>& a) {
+    T w[3] = {0, 0, 0};
+    const T av[3] = {a.x, a.y, a.z};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) w[i] += m[3 * i + j] * av[j];
+    return {w[0], w[1], w[2]};
+}
+// Matrix3X * Matrix3X (vec.h:170-180).
+template <class T> inline void mm(const T a[9], const T b[9], T out[9]) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            T s = 0;
+            for (int k = 0; k < 3; k++) s += a[3 * i + k] * b[3 * k + j];
+            out[3 * i + j] = s;
+        }
+}
+// Vector3X::normalize (vec.h:93-105).
+template <class T> inline V3<T> normalized(const V3<T>& a) {
+    T value = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    if (value > (T)1e-8f) return {a.x / value, a.y / value, a.z / value};
+    return {0, 0, 0};
+}
+
+// Per-(cond, view) frame: R = condR * Rz(theta), offset rotated, source.
+template <class T> struct Frame {
+    T R[9];
+    V3<T> off, t, src, eu, ev, G;
+};
+
+template <class T>
+Frame<T> make_frame(const orc_geometry& g, const orc_scan& s, int cond, int n, int conv) {
+    Frame<T> f;
+    const T th = (T)theta_f32(n, g.n_proj);
+    const T c = tcos<T>(th), sn = tsin<T>(th);
+    const T Rz[9] = {c, -sn, 0, sn, c, 0, 0, 0, 1};
+    T cR[9];
+    for (int i = 0; i < 9; i++) cR[i] = (T)s.cond_R[9 * cond + i];
+    mm(cR, Rz, f.R);
+    const V3<T> off0 = {(T)s.cond_offset[3 * cond + 0], (T)s.cond_offset[3 * cond + 1], (T)s.cond_offset[3 * cond + 2]};
+    f.off = mv(f.R, off0);
+    f.t = {(T)s.cond_t[3 * cond + 0], (T)s.cond_t[3 * cond + 1], (T)s.cond_t[3 * cond + 2]};
+    // IR: source at R(0,+sod,0) (ir.cu:1874); FDK: R(0,-sod,0) (fdk.cu:125).
+    const V3<T> s0 = {0, conv == 0 ? (T)g.sod : -(T)g.sod, 0};
+    f.src = mv(f.R, s0);
+    f.eu = mv(f.R, V3<T>{1, 0, 0});
+    f.ev = mv(f.R, V3<T>{0, 0, 1});
+    V3<T> g0 = {0, 0, 1};
+    if (s.grating) g0 = {(T)s.grating[3 * cond + 0], (T)s.grating[3 * cond + 1], (T)s.grating[3 * cond + 2]};
+    f.G = mv(f.R, g0);
+    return f;
+}
+
+// Voxel centre relative to the rotation centre: ir.cu:1881-1884 (IR, (x-(N+1)/2)*vs)
+// and fdk.cu:132-135 (FDK, (x-(N-1)/2)*vs), both minus the rotated offset and t.
+template <class T> inline T centre_coord(int i, int N, T vs, int conv) {
+    if (conv == 0) return ((T)2.0f * (T)i - (T)N - (T)1.0f) * (T)0.5f * vs;
+    return ((T)2.0f * (T)i - (T)N + (T)1.0f) * (T)0.5f * vs;
+}
+
+template <class T>
+inline V3<T> voxel_pos(const orc_geometry& g, const Frame<T>& f, int x, int y, int z, int conv) {
+    const T vs = (T)g.vox_size;
+    return {centre_coord<T>(x, g.n_vox, vs, conv) - f.off.x - f.t.x,
+            centre_coord<T>(y, g.n_vox, vs, conv) - f.off.y - f.t.y,
+            centre_coord<T>(z, g.n_vox, vs, conv) - f.off.z - f.t.z};
+}
+
+// rayCasting (ir.cu:1852-1902).
+template <class T>
+inline void raycast(const orc_geometry& g, const Frame<T>& f, int x, int y, int z, int conv, T& u, T& v, V3<T>& B,
+                    V3<T>* X_out = nullptr) {
+    const V3<T> X = voxel_pos(g, f, x, y, z, conv);
+    if (X_out) *X_out = X;
+    const V3<T> d = X - f.src;
+    const T coeff = -dot(f.src, f.src) / dot(f.src, d);
+    const V3<T> p = f.src + scale(coeff, d);
+    const T mag = (T)g.sdd / (T)g.sod;
+    u = dot(p, f.eu) * mag / (T)g.det_size + (T)0.5f * (T)g.n_det_u;
+    v = dot(p, f.ev) * mag / (T)g.det_size + (T)0.5f * (T)g.n_det_v;
+    B = normalized(d);
+}
+
+// Acceptance window and bilinear weights (ir.cu:1501-1509).
+template <class T> struct Splat {
+    int64_t iu, iv;
+    T c1, c2, c3, c4;  // (iu,iv+1) (iu+1,iv+1) (iu+1,iv) (iu,iv)
+};
+template <class T> inline bool splat_weights(T u, T v, int U, int V, Splat<T>& s) {
+    if (!((T)0.55f < u && u < (T)U - (T)0.55f && (T)0.55f < v && v < (T)V - (T)0.55f)) return false;
+    const T ut = u - (T)0.5f, vt = v - (T)0.5f;
+    s.iu = (int64_t)std::floor(ut);
+    s.iv = (int64_t)std::floor(vt);
+    const T fu = ut - (T)s.iu, fv = vt - (T)s.iv;
+    s.c1 = ((T)1.0f - fu) * fv;
+    s.c2 = fu * fv;
+    s.c3 = fu * ((T)1.0f - fv);
+    s.c4 = ((T)1.0f - fu) * ((T)1.0f - fv);
+    return true;
+}
+
+template <class T> inline T proj_ratio(const orc_geometry& g) {
+    // ir.cu:1513-1514, left to right.
+    const T a = (T)g.vox_size * (T)g.vox_size;
+    const T m = (T)g.sod / (T)g.sdd;
+    return a / ((T)g.det_size * m * (T)g.det_size * m);
+}
+
+// XTT weight v_km = |B x S_k| * |S_k . G| (ir.cu:1786), returned squared.
+template <class T> inline T xtt_w2(const V3<T>& B, const V3<T>& S, const V3<T>& G) {
+    const T vkm = norm2(cross(B, S)) * std::fabs(dot(S, G));
+    return vkm * vkm;
+}
+
+template <class T> inline V3<T> basis_vec(const orc_scan& s, int k) {
+    return {(T)s.basis[3 * k + 0], (T)s.basis[3 * k + 1], (T)s.basis[3 * k + 2]};
+}
+
+int n_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+int thread_id() {
+#ifdef _OPENMP
+    return omp_get_thread_num();
+#else
+    return 0;
+#endif
+}
+
+// Raw voxel-driven splat of one view (forwardonDevice / forwardXTTonDevice,
+// ir.cu:1490-1531, 1753-1804): proj += c_k*ratio*mu (plain) or c_k*sum_k
+// vkm^2*vs*ratio*mu_k (XTT); factor += c_k.  Accumulates into proj/factor (T).
+template <class T, class VT>
+void forward_raw(const orc_geometry& g, const orc_scan& s, int cond, int n, const VT* vol, int xtt, T* proj,
+                 T* factor) {
+    const int N = g.n_vox, U = g.n_det_u, V = g.n_det_v;
+    const int64_t lenV = (int64_t)N * N * N;
+    const Frame<T> f = make_frame<T>(g, s, cond, n, 0);
+    const T ratio = proj_ratio<T>(g);
+    const int K = xtt ? s.n_basis : 1;
+    const int nt = n_threads();
+    std::vector<T> pbuf((size_t)nt * U * V, (T)0), fbuf((size_t)nt * U * V, (T)0);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < N; y++) {
+        T* pp = &pbuf[(size_t)thread_id() * U * V];
+        T* ff = &fbuf[(size_t)thread_id() * U * V];
+        for (int z = 0; z < N; z++)
+            for (int x = 0; x < N; x++) {
+                T u, v;
+                V3<T> B;
+                raycast(g, f, x, y, z, 0, u, v, B);
+                Splat<T> w;
+                if (!splat_weights(u, v, U, V, w)) continue;
+                const int64_t idx = x + (int64_t)N * y + (int64_t)N * N * z;
+                T val;
+                if (!xtt) {
+                    val = ratio * (T)vol[idx];
+                } else {
+                    val = 0;
+                    for (int k = 0; k < K; k++) {
+                        const T w2 = xtt_w2(B, basis_vec<T>(s, k), f.G);
+                        val += w2 * (T)g.vox_size * ratio * (T)vol[idx + k * lenV];
+                    }
+                }
+                const int64_t a1 = w.iu + (int64_t)U * (w.iv + 1), a2 = (w.iu + 1) + (int64_t)U * (w.iv + 1);
+                const int64_t a3 = (w.iu + 1) + (int64_t)U * w.iv, a4 = w.iu + (int64_t)U * w.iv;
+                if (!xtt) {
+                    pp[a1] += w.c1 * ratio * (T)vol[idx];
+                    pp[a2] += w.c2 * ratio * (T)vol[idx];
+                    pp[a3] += w.c3 * ratio * (T)vol[idx];
+                    pp[a4] += w.c4 * ratio * (T)vol[idx];
+                } else {
+                    pp[a1] += w.c1 * val;
+                    pp[a2] += w.c2 * val;
+                    pp[a3] += w.c3 * val;
+                    pp[a4] += w.c4 * val;
+                }
+                ff[a1] += w.c1;
+                ff[a2] += w.c2;
+                ff[a3] += w.c3;
+                ff[a4] += w.c4;
+            }
+    }
+    for (int t = 0; t < nt; t++)
+        for (int64_t i = 0; i < (int64_t)U * V; i++) {
+            proj[i] += pbuf[(size_t)t * U * V + i];
+            factor[i] += fbuf[(size_t)t * U * V + i];
+        }
+}
+
+// Chord length of the pixel-centre ray through the reconstruction box
+// (correlationProjByLength, ir.cu:1533-1597).
+template <class T> T chord_length(const orc_geometry& g, const Frame<T>& f, int u, int v) {
+    const int N = g.n_vox, U = g.n_det_u, V = g.n_det_v;
+    const T det = (T)g.det_size;
+    const V3<T> pix0 = {((T)u - ((T)U - (T)1.0f) * (T)0.5f) * det, -((T)g.sdd - (T)g.sod),
+                        ((T)v - ((T)V - (T)1.0f) * (T)0.5f) * det};
+    const V3<T> o2s = f.src;  // Rotate * (0, sod, 0)
+    const V3<T> o2p = mv(f.R, pix0);
+    const V3<T> s2p = o2s - o2p;
+    const T half = ((T)(N + 1) / (T)2.0f) * (T)g.vox_size;
+    const T plane[6] = {-half - f.off.x - f.t.x, half - f.off.x - f.t.x, -half - f.off.y - f.t.y,
+                        half - f.off.y - f.t.y,  -half - f.off.z - f.t.z, half - f.off.z - f.t.z};
+    int cnt = 0;
+    T coord[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = 0; p < 3; p++)
+        for (int sign = 0; sign < 2; sign++) {
+            const T tpar = (plane[2 * p + sign] - o2s[p]) / s2p[p];
+            const V3<T> pl = scale(tpar, s2p) + o2s;
+            const int p1 = (p + 1) % 3, p2 = (p + 2) % 3;
+            if (plane[2 * p1] < pl[p1] && pl[p1] < plane[2 * p1 + 1] && plane[2 * p2] < pl[p2] &&
+                pl[p2] < plane[2 * p2 + 1]) {
+                if (cnt < 2) {
+                    coord[3 * cnt + 0] = pl.x;
+                    coord[3 * cnt + 1] = pl.y;
+                    coord[3 * cnt + 2] = pl.z;
+                }
+                cnt++;
+            }
+        }
+    return std::sqrt((coord[0] - coord[3]) * (coord[0] - coord[3]) + (coord[1] - coord[4]) * (coord[1] - coord[4]) +
+                     (coord[2] - coord[5]) * (coord[2] - coord[5]));
+}
+
+// proj *= L / (factor + 1e-5)  (ir.cu:1602).
+template <class T> void normalise_view(const orc_geometry& g, const Frame<T>& f, T* proj, const T* factor) {
+    const int U = g.n_det_u, V = g.n_det_v;
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < V; v++)
+        for (int u = 0; u < U; u++) {
+            const T L = chord_length(g, f, u, v);
+            proj[u + (int64_t)U * v] *= L / (factor[u + (int64_t)U * v] + (T)1e-5f);
+        }
+}
+
+// Raw gather of one view (backwardonDevice ir.cu:1720-1751, backwardXTTonDevice
+// ir.cu:1807-1850): tmp += sum c_k p_k, factor += 1 (plain) or per channel
+// tmp_k += vkm^2 sum c p, factor_k += vkm^2 (XTT).  Volume-sized accumulators.
+template <class T>
+void backward_raw(const orc_geometry& g, const orc_scan& s, int cond, int n, const T* proj, int xtt, T* tmp, T* factor) {
+    const int N = g.n_vox, U = g.n_det_u, V = g.n_det_v;
+    const int64_t lenV = (int64_t)N * N * N;
+    const Frame<T> f = make_frame<T>(g, s, cond, n, 0);
+    const int K = xtt ? s.n_basis : 1;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; z++)
+        for (int y = 0; y < N; y++)
+            for (int x = 0; x < N; x++) {
+                T u, v;
+                V3<T> B;
+                raycast(g, f, x, y, z, 0, u, v, B);
+                Splat<T> w;
+                if (!splat_weights(u, v, U, V, w)) continue;
+                const T p1 = proj[w.iu + (int64_t)U * (w.iv + 1)], p2 = proj[(w.iu + 1) + (int64_t)U * (w.iv + 1)];
+                const T p3 = proj[(w.iu + 1) + (int64_t)U * w.iv], p4 = proj[w.iu + (int64_t)U * w.iv];
+                const int64_t idx = x + (int64_t)N * y + (int64_t)N * N * z;
+                if (!xtt) {
+                    const T numBack = w.c1 * p1 + w.c2 * p2 + w.c3 * p3 + w.c4 * p4;
+                    factor[idx] += (T)1.0f;
+                    tmp[idx] += numBack;
+                } else {
+                    for (int k = 0; k < K; k++) {
+                        const T w2 = xtt_w2(B, basis_vec<T>(s, k), f.G);
+                        const T bf = w2 * w.c1 * p1 + w2 * w.c2 * p2 + w2 * w.c3 * p3 + w2 * w.c4 * p4;
+                        factor[idx + k * lenV] += w2;
+                        tmp[idx + k * lenV] += bf;
+                    }
+                }
+            }
+}
+
+// projRatio (ir.cu:1616-1635) / projSubtract (ir.cu:1656-1666) on one view.
+template <class T> double ratio_step(int64_t npix, const float* sino, T* proj, int art) {
+    double loss = 0;
+    for (int64_t i = 0; i < npix; i++) {
+        const T y = (T)sino[i];
+        loss += (double)std::fabs(y - proj[i]);
+        if (art) {
+            proj[i] = y - proj[i];
+        } else {
+            if (proj[i] != (T)0) proj[i] = y / proj[i];
+            if (proj[i] > (T)2.0f) proj[i] = (T)2.0f;
+        }
+    }
+    return loss;
+}
+
+// voxelProduct (ir.cu:1637-1654) / voxelPlus (ir.cu:1682-1693).
+template <class T> void update_step(int64_t n, T* vol, const T* tmp, const T* factor, int art, T alpha) {
+    for (int64_t i = 0; i < n; i++) {
+        if (art)
+            vol[i] = vol[i] + alpha * tmp[i];
+        else
+            vol[i] = (factor[i] == (T)0) ? (T)0 : vol[i] * tmp[i] / factor[i];
+    }
+}
+
+// Subset order: std::shuffle with a default-seeded std::mt19937_64, one engine
+// per call (IR, reconstruct.cu:75) or re-created per epoch (XTT, :858).
+void subset_orders(int batch, int epochs, int per_epoch_engine, int32_t* out) {
+    std::vector<int> order(batch);
+    for (int i = 0; i < batch; i++) order[i] = i;
+    std::mt19937_64 rng;
+    for (int ep = 0; ep < epochs; ep++) {
+        // XTT re-creates the engine each epoch but shuffles the order left by
+        // the previous epoch (reconstruct.cu:843-859), so epoch k applies the
+        // same permutation k times.
+        if (per_epoch_engine) rng = std::mt19937_64();
+        std::shuffle(order.begin(), order.end(), rng);
+        for (int i = 0; i < batch; i++) out[ep * batch + i] = order[i];
+    }
+}
+
+// IR::reconstruct (reconstruct.cu:23-156): OS-EM / OS-ART, one volume per cond.
+template <class T>
+void ir_reconstruct(const orc_geometry& g, const orc_scan& s, const float* sino, const float* vol_in, int epochs,
+                    int batch, int dir, int art, float lambda, double* vol_out, double* losses) {
+    const int N = g.n_vox, U = g.n_det_u, V = g.n_det_v, P = g.n_proj;
+    const int64_t lenV = (int64_t)N * N * N, lenP = (int64_t)U * V, lenD = lenP * P;
+    const int rotation = dir == 0 ? 1 : -1;
+    const int subsetSize = (P + batch - 1) / batch;
+    std::vector<int> order(batch);
+    for (int i = 0; i < batch; i++) order[i] = i;
+    std::mt19937_64 rng;
+    std::vector<T> vol(lenV), proj(lenD), factorP(lenP), tmp(lenV), factorV(lenV);
+    for (int cond = 0; cond < s.n_cond; cond++) {
+        for (int64_t i = 0; i < lenV; i++) vol[i] = (T)vol_in[cond * lenV + i];
+        const float* y = sino + cond * lenD;
+        for (int ep = 0; ep < epochs; ep++) {
+            std::shuffle(order.begin(), order.end(), rng);
+            double loss = 0;
+            std::fill(proj.begin(), proj.end(), (T)0);
+            for (int sub : order) {
+                for (int so = 0; so < subsetSize; so++) {
+                    std::fill(factorP.begin(), factorP.end(), (T)0);
+                    const int n = rotation * ((sub + batch * so) % P);
+                    const int an = std::abs(n);
+                    T* pv = &proj[(int64_t)an * lenP];
+                    forward_raw<T, T>(g, s, cond, n, vol.data(), 0, pv, factorP.data());
+                    const Frame<T> f = make_frame<T>(g, s, cond, n, 0);
+                    normalise_view(g, f, pv, factorP.data());
+                    loss += ratio_step(lenP, y + (int64_t)an * lenP, pv, art);
+                }
+                std::fill(tmp.begin(), tmp.end(), (T)0);
+                std::fill(factorV.begin(), factorV.end(), (T)0);
+                for (int so = 0; so < subsetSize; so++) {
+                    const int n = rotation * ((sub + batch * so) % P);
+                    backward_raw<T>(g, s, cond, n, &proj[(int64_t)std::abs(n) * lenP], 0, tmp.data(), factorV.data());
+                }
+                update_step<T>(lenV, vol.data(), tmp.data(), factorV.data(), art, (T)(lambda / (float)subsetSize));
+            }
+            if (losses) losses[cond * epochs + ep] = loss;
+        }
+        for (int64_t i = 0; i < lenV; i++) vol_out[cond * lenV + i] = (double)vol[i];
+    }
+}
+
+// XTT::reconstruct (reconstruct.cu:799-954) up to the volume square root; the
+// final ratio/residual images are returned like the reference's sinogram
+// overwrite (reconstruct.cu:924-925).
+template <class T>
+void xtt_reconstruct(const orc_geometry& g, const orc_scan& s, const float* sino, const float* vol_in, int epochs,
+                     int batch, int dir, int art, float lambda, double* vol_out, double* proj_out, double* losses) {
+    const int N = g.n_vox, U = g.n_det_u, V = g.n_det_v, P = g.n_proj, K = s.n_basis, C = s.n_cond;
+    const int64_t lenV = (int64_t)N * N * N, lenP = (int64_t)U * V, lenD = lenP * P;
+    const int rotation = dir == 0 ? 1 : -1;
+    const int subsetSize = (P + batch - 1) / batch;
+    std::vector<int> order(batch);
+    for (int i = 0; i < batch; i++) order[i] = i;
+    std::vector<T> vol(lenV * K), proj(lenD * C), factorP(lenP), tmp(lenV * K), factorV(lenV * K);
+    for (int64_t i = 0; i < lenV * K; i++) vol[i] = (T)vol_in[i];
+    for (int ep = 0; ep < epochs; ep++) {
+        std::mt19937_64 rng;
+        std::shuffle(order.begin(), order.end(), rng);
+        double loss = 0;
+        std::fill(proj.begin(), proj.end(), (T)0);
+        for (int sub : order) {
+            for (int cond = 0; cond < C; cond++)
+                for (int so = 0; so < subsetSize; so++) {
+                    std::fill(factorP.begin(), factorP.end(), (T)0);
+                    const int n = rotation * ((sub + batch * so) % P);
+                    T* pv = &proj[cond * lenD + (int64_t)std::abs(n) * lenP];
+                    forward_raw<T, T>(g, s, cond, n, vol.data(), 1, pv, factorP.data());
+                    normalise_view(g, make_frame<T>(g, s, cond, n, 0), pv, factorP.data());
+                    loss += ratio_step(lenP, sino + cond * lenD + (int64_t)std::abs(n) * lenP, pv, art);
+                }
+            std::fill(tmp.begin(), tmp.end(), (T)0);
+            std::fill(factorV.begin(), factorV.end(), (T)0);
+            for (int cond = 0; cond < C; cond++)
+                for (int so = 0; so < subsetSize; so++) {
+                    const int n = rotation * ((sub + batch * so) % P);
+                    backward_raw<T>(g, s, cond, n, &proj[cond * lenD + (int64_t)std::abs(n) * lenP], 1, tmp.data(),
+                                    factorV.data());
+                }
+            update_step<T>(lenV * K, vol.data(), tmp.data(), factorV.data(), art, (T)(lambda / (float)subsetSize));
+        }
+        if (losses) losses[ep] = loss;
+    }
+    for (int64_t i = 0; i < lenV * K; i++) {
+        // voxelSqrt (ir.cu:1715)
+        const T m = vol[i];
+        vol_out[i] = (double)((m < (T)0) ? std::sqrt(-m) : std::sqrt(m));
+    }
+    if (proj_out)
+        for (int64_t i = 0; i < lenD * C; i++) proj_out[i] = (double)proj[i];
+}
+
+// calcWeight (fdk.cu:13-26).
+template <class T> void fdk_weight(const orc_geometry& g, T* w) {
+    const int U = g.n_det_u, V = g.n_det_v;
+    const T sdd = (T)g.sdd, det = (T)g.det_size;
+    for (int v = 0; v < V; v++)
+        for (int u = 0; u < U; u++) {
+            const T ur = ((T)u + (T)0.5f - (T)U / (T)2.0f) * det;
+            const T vr = ((T)v + (T)0.5f - (T)V / (T)2.0f) * det;
+            w[u + (int64_t)U * v] = sdd / std::sqrt(sdd * sdd + ur * ur + vr * vr);
+        }
+}
+
+// Filter taps.  kind 0: the reference's kernel (reconstruct.cu:1694-1699),
+// 1/(pi^2 d (1-4m^2)) with d = det*sod/sdd, evaluated as the reference does
+// (double product, rounded to float, FP32 reciprocal).  kind 1: Ram-Lak in the
+// same scaling convention (sample spacing d and the 1/2 of a full rotation
+// folded in): h[0] = 1/(8d), h[odd m] = -1/(2 pi^2 m^2 d), h[even m] = 0.
+template <class T> void fdk_taps(const orc_geometry& g, int kind, T* h) {
+    const float d = g.det_size * (g.sod / g.sdd);
+    for (int m = 0; m < g.n_det_u; m++) {
+        if (kind == 0) {
+            if (sizeof(T) == sizeof(float))
+                h[m] = (T)(1.0f / (float)(M_PI * M_PI * d * (1.0f - 4.0f * (float)(m * m))));
+            else
+                h[m] = (T)(1.0 / (M_PI * M_PI * (double)d * (1.0 - 4.0 * (double)m * (double)m)));
+        } else {
+            if (m == 0)
+                h[m] = (T)(1.0 / (8.0 * (double)d));
+            else if (m % 2 == 1)
+                h[m] = (T)(-1.0 / (2.0 * M_PI * M_PI * (double)m * (double)m * (double)d));
+            else
+                h[m] = (T)0;
+        }
+    }
+}
+
+// projConv (fdk.cu:28-43) for one view: dst[u,v] = sum_j h[|u-j|] src[j,v] w[j,v].
+template <class T> void fdk_filter_view(const orc_geometry& g, const T* h, const T* w, const float* src, T* dst) {
+    const int U = g.n_det_u, V = g.n_det_v;
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < V; v++)
+        for (int u = 0; u < U; u++) {
+            T acc = 0;
+            for (int j = 0; j < U; j++) {
+                const T hh = (u > j) ? h[u - j] : h[j - u];
+                acc += hh * (T)src[j + (int64_t)U * v] * w[j + (int64_t)U * v];
+            }
+            dst[u + (int64_t)U * v] = acc;
+        }
+}
+
+// FDK backwardonDevice (fdk.cu:62-170) for one view: vol += Ud^2 * (2 pi / P) * sum c p,
+// Ud = sod / (sod + y), y the un-rotated voxel y coordinate (fdk.cu:160).
+template <class T, class PT>
+void fdk_backproject_view(const orc_geometry& g, const orc_scan& s, int cond, int n, const PT* proj, T* vol) {
+    const int N = g.n_vox, U = g.n_det_u, V = g.n_det_v;
+    const Frame<T> f = make_frame<T>(g, s, cond, n, 1);
+    const T Cw = (T)2.0f * (T)kPIf / (T)g.n_proj;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; z++)
+        for (int y = 0; y < N; y++)
+            for (int x = 0; x < N; x++) {
+                T u, v;
+                V3<T> B, X;
+                raycast(g, f, x, y, z, 1, u, v, B, &X);
+                Splat<T> w;
+                if (!splat_weights(u, v, U, V, w)) continue;
+                const T Ud = (T)g.sod / ((T)g.sod + X.y);
+                const T numBack = w.c1 * (T)proj[w.iu + (int64_t)U * (w.iv + 1)] +
+                                  w.c2 * (T)proj[(w.iu + 1) + (int64_t)U * (w.iv + 1)] +
+                                  w.c3 * (T)proj[(w.iu + 1) + (int64_t)U * w.iv] +
+                                  w.c4 * (T)proj[w.iu + (int64_t)U * w.iv];
+                vol[x + (int64_t)N * y + (int64_t)N * N * z] += Ud * Ud * Cw * numBack;
+            }
+}
+
+// Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (textbook
+// method; the reference calls Eigen's SelfAdjointEigenSolver, pca.cu:33).
+// Returns eigenvalues ascending and the matching unit eigenvectors as columns.
+void jacobi3(const double A_in[9], double evals[3], double evecs[9]) {
+    double a[3][3], v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) a[i][j] = A_in[3 * i + j];
+    for (int sweep = 0; sweep < 50; sweep++) {
+        const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        const double diag = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+        if (off <= 1e-300 || off <= 1e-32 * diag) break;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                if (a[p][q] == 0.0) continue;
+                const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < 3; k++) {  // A <- A J
+                    const double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - sn * akq;
+                    a[k][q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < 3; k++) {  // A <- J^T A
+                    const double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - sn * aqk;
+                    a[q][k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; k++) {  // V <- V J
+                    const double vkp = v[k][p], vkq = v[k][q];
+                    v[k][p] = c * vkp - sn * vkq;
+                    v[k][q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    int idx[3] = {0, 1, 2};
+    std::sort(idx, idx + 3, [&](int i, int j) { return a[i][i] < a[j][j]; });
+    for (int c = 0; c < 3; c++) {
+        evals[c] = a[idx[c]][idx[c]];
+        for (int r = 0; r < 3; r++) evecs[3 * r + c] = v[r][idx[c]];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Reference-preset voxel size (geometry.h:13): sod * detSize / sdd in FP32.
+float orc_default_vox_size(float sdd, float sod, float det) { return sod * det / sdd; }
+
+float orc_theta_f32(int n, int P) { return theta_f32(n, P); }
+
+// a2: rayCasting for one voxel.  conv 0 = IR/XTT, 1 = FDK.  out: u, v, B[3], G[3].
+void orc_raycast(int prec, const orc_geometry* g, const orc_scan* s, int cond, int x, int y, int z, int n, int conv,
+                 double* out8) {
+    if (prec == 0) {
+        const Frame<float> f = make_frame<float>(*g, *s, cond, n, conv);
+        float u, v;
+        V3<float> B;
+        raycast(*g, f, x, y, z, conv, u, v, B);
+        const double o[8] = {u, v, B.x, B.y, B.z, f.G.x, f.G.y, f.G.z};
+        std::memcpy(out8, o, sizeof(o));
+    } else {
+        const Frame<double> f = make_frame<double>(*g, *s, cond, n, conv);
+        double u, v;
+        V3<double> B;
+        raycast(*g, f, x, y, z, conv, u, v, B);
+        const double o[8] = {u, v, B.x, B.y, B.z, f.G.x, f.G.y, f.G.z};
+        std::memcpy(out8, o, sizeof(o));
+    }
+}
+
+// a3/a11 (+a4 when normalise): one view n (signed) of cond.  vol: K*N^3 floats in
+// the reference layout (x fastest).  out_proj/out_factor: U*V doubles (overwritten).
+void orc_forward_view(int prec, const orc_geometry* g, const orc_scan* s, int cond, int n, const float* vol, int xtt,
+                      int normalise, double* out_proj, double* out_factor) {
+    const int64_t lenP = (int64_t)g->n_det_u * g->n_det_v;
+    if (prec == 0) {
+        std::vector<float> p(lenP, 0.f), f(lenP, 0.f);
+        forward_raw<float, float>(*g, *s, cond, n, vol, xtt, p.data(), f.data());
+        if (normalise) normalise_view(*g, make_frame<float>(*g, *s, cond, n, 0), p.data(), f.data());
+        for (int64_t i = 0; i < lenP; i++) {
+            out_proj[i] = p[i];
+            if (out_factor) out_factor[i] = f[i];
+        }
+    } else {
+        std::vector<double> p(lenP, 0.0), f(lenP, 0.0);
+        forward_raw<double, float>(*g, *s, cond, n, vol, xtt, p.data(), f.data());
+        if (normalise) normalise_view(*g, make_frame<double>(*g, *s, cond, n, 0), p.data(), f.data());
+        std::memcpy(out_proj, p.data(), lenP * sizeof(double));
+        if (out_factor) std::memcpy(out_factor, f.data(), lenP * sizeof(double));
+    }
+}
+
+// a4's L for every pixel of view n.
+void orc_chord_length(int prec, const orc_geometry* g, const orc_scan* s, int cond, int n, double* out) {
+    const int U = g->n_det_u, V = g->n_det_v;
+    if (prec == 0) {
+        const Frame<float> f = make_frame<float>(*g, *s, cond, n, 0);
+        for (int v = 0; v < V; v++)
+            for (int u = 0; u < U; u++) out[u + (int64_t)U * v] = chord_length(*g, f, u, v);
+    } else {
+        const Frame<double> f = make_frame<double>(*g, *s, cond, n, 0);
+        for (int v = 0; v < V; v++)
+            for (int u = 0; u < U; u++) out[u + (int64_t)U * v] = chord_length(*g, f, u, v);
+    }
+}
+
+// a7/a12 raw gather of one view into K*N^3 accumulators (reference layout, doubles, accumulated).
+void orc_backward_view(int prec, const orc_geometry* g, const orc_scan* s, int cond, int n, const float* proj_view,
+                       int xtt, double* tmp, double* factor) {
+    const int64_t lenP = (int64_t)g->n_det_u * g->n_det_v;
+    const int64_t lenV = (int64_t)g->n_vox * g->n_vox * g->n_vox * (xtt ? s->n_basis : 1);
+    if (prec == 0) {
+        std::vector<float> p(proj_view, proj_view + lenP), t(lenV, 0.f), f(lenV, 0.f);
+        backward_raw<float>(*g, *s, cond, n, p.data(), xtt, t.data(), f.data());
+        for (int64_t i = 0; i < lenV; i++) {
+            tmp[i] += t[i];
+            factor[i] += f[i];
+        }
+    } else {
+        std::vector<double> p(proj_view, proj_view + lenP);
+        backward_raw<double>(*g, *s, cond, n, p.data(), xtt, tmp, factor);
+    }
+}
+
+void orc_subset_orders(int batch, int epochs, int per_epoch_engine, int32_t* out) {
+    subset_orders(batch, epochs, per_epoch_engine, out);
+}
+
+// a10: IR::reconstruct.  sino: n_cond*U*V*P floats; vol_in: n_cond*N^3 floats (reference layout).
+// dir 0 = CW, 1 = CCW; art 0 = MLEM, 1 = ART.  vol_out: n_cond*N^3 doubles; losses: n_cond*epochs raw sums.
+void orc_ir_reconstruct(int prec, const orc_geometry* g, const orc_scan* s, const float* sino, const float* vol_in,
+                        int epochs, int batch, int dir, int art, float lambda, double* vol_out, double* losses) {
+    if (prec == 0)
+        ir_reconstruct<float>(*g, *s, sino, vol_in, epochs, batch, dir, art, lambda, vol_out, losses);
+    else
+        ir_reconstruct<double>(*g, *s, sino, vol_in, epochs, batch, dir, art, lambda, vol_out, losses);
+}
+
+// a14 (+a13): XTT::reconstruct.  sino: n_cond*U*V*P floats; vol_in: K*N^3 floats.
+// vol_out: K*N^3 doubles after voxelSqrt; proj_out (optional): n_cond*U*V*P final
+// ratio/residual images; losses: epochs raw sums.
+void orc_xtt_reconstruct(int prec, const orc_geometry* g, const orc_scan* s, const float* sino, const float* vol_in,
+                         int epochs, int batch, int dir, int art, float lambda, double* vol_out, double* proj_out,
+                         double* losses) {
+    if (prec == 0)
+        xtt_reconstruct<float>(*g, *s, sino, vol_in, epochs, batch, dir, art, lambda, vol_out, proj_out, losses);
+    else
+        xtt_reconstruct<double>(*g, *s, sino, vol_in, epochs, batch, dir, art, lambda, vol_out, proj_out, losses);
+}
+
+void orc_fdk_weight(int prec, const orc_geometry* g, double* out) {
+    const int64_t n = (int64_t)g->n_det_u * g->n_det_v;
+    if (prec == 0) {
+        std::vector<float> w(n);
+        fdk_weight<float>(*g, w.data());
+        for (int64_t i = 0; i < n; i++) out[i] = w[i];
+    } else {
+        fdk_weight<double>(*g, out);
+    }
+}
+
+void orc_fdk_taps(int prec, const orc_geometry* g, int kind, double* out) {
+    if (prec == 0) {
+        std::vector<float> h(g->n_det_u);
+        fdk_taps<float>(*g, kind, h.data());
+        for (int i = 0; i < g->n_det_u; i++) out[i] = h[i];
+    } else {
+        fdk_taps<double>(*g, kind, out);
+    }
+}
+
+// a16-a18 for one view: weighted, filtered detector image (U*V doubles).
+void orc_fdk_filter_view(int prec, const orc_geometry* g, int kind, const float* src_view, double* out) {
+    const int U = g->n_det_u;
+    const int64_t n = (int64_t)U * g->n_det_v;
+    if (prec == 0) {
+        std::vector<float> h(U), w(n), d(n);
+        fdk_taps<float>(*g, kind, h.data());
+        fdk_weight<float>(*g, w.data());
+        fdk_filter_view<float>(*g, h.data(), w.data(), src_view, d.data());
+        for (int64_t i = 0; i < n; i++) out[i] = d[i];
+    } else {
+        std::vector<double> h(U), w(n);
+        fdk_taps<double>(*g, kind, h.data());
+        fdk_weight<double>(*g, w.data());
+        fdk_filter_view<double>(*g, h.data(), w.data(), src_view, out);
+    }
+}
+
+// a19 for one view of an already filtered image; accumulates into vol (N^3 doubles).
+void orc_fdk_backproject_view(int prec, const orc_geometry* g, const orc_scan* s, int cond, int n,
+                              const double* filt_view, double* vol) {
+    const int64_t lenV = (int64_t)g->n_vox * g->n_vox * g->n_vox;
+    if (prec == 0) {
+        std::vector<float> acc(lenV, 0.f);
+        std::vector<float> pv(filt_view, filt_view + (int64_t)g->n_det_u * g->n_det_v);
+        fdk_backproject_view<float, float>(*g, *s, cond, n, pv.data(), acc.data());
+        for (int64_t i = 0; i < lenV; i++) vol[i] += acc[i];
+    } else {
+        fdk_backproject_view<double, double>(*g, *s, cond, n, filt_view, vol);
+    }
+}
+
+// a20: FDK::reconstruct with zero-initialised buffers and each condition reading
+// its own filtered sinogram.  vol_out: n_cond*N^3 doubles; filt_out (optional):
+// n_cond*U*V*P filtered sinograms (the reference returns them through `sinogram`).
+void orc_fdk_reconstruct(int prec, const orc_geometry* g, const orc_scan* s, const float* sino, int dir, int kind,
+                         double* vol_out, double* filt_out) {
+    const int U = g->n_det_u, V = g->n_det_v, P = g->n_proj, N = g->n_vox;
+    const int64_t lenP = (int64_t)U * V, lenD = lenP * P, lenV = (int64_t)N * N * N;
+    const int rotation = dir == 0 ? 1 : -1;
+    std::vector<double> fv(lenP);
+    for (int cond = 0; cond < s->n_cond; cond++) {
+        std::vector<double> acc(lenV, 0.0);
+        std::vector<float> accf(prec == 0 ? lenV : 0, 0.f);
+        for (int n = 0; n < P; n++) {
+            orc_fdk_filter_view(prec, g, kind, sino + cond * lenD + n * lenP, fv.data());
+            if (filt_out) std::memcpy(filt_out + cond * lenD + n * lenP, fv.data(), lenP * sizeof(double));
+            if (prec == 0) {
+                std::vector<float> pv(fv.begin(), fv.end());
+                fdk_backproject_view<float, float>(*g, *s, cond, rotation * n, pv.data(), accf.data());
+            } else {
+                fdk_backproject_view<double, double>(*g, *s, cond, rotation * n, fv.data(), acc.data());
+            }
+        }
+        for (int64_t i = 0; i < lenV; i++) vol_out[cond * lenV + i] = prec == 0 ? (double)accf[i] : acc[i];
+    }
+}
+
+// a15: calcEigenVector over nvox voxels.  mu: K blocks of nvox floats; basis: 3K floats.
+// md: 3 blocks of nvox; evalue: 3 blocks of nvox (doubles).
+void orc_pca(const float* mu, int K, const float* basis, int64_t nvox, double* md, double* evalue) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nvox; i++) {
+        double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double mean = 0;
+        for (int k = 0; k < K; k++) {
+            const double m = mu[k * nvox + i];
+            mean += m;
+            const double sc[3] = {m * basis[3 * k + 0], m * basis[3 * k + 1], m * basis[3 * k + 2]};
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) C[3 * r + c] += sc[r] * sc[c];
+        }
+        mean /= K;
+        for (int j = 0; j < 9; j++) C[j] /= K;
+        double ev[3], vec[9];
+        jacobi3(C, ev, vec);
+        double e[3] = {vec[0], vec[3], vec[6]};
+        const double nrm = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        for (int r = 0; r < 3; r++) e[r] /= nrm;
+        const double sign = (e[2] >= 0) ? 1.0 : -1.0;
+        for (int r = 0; r < 3; r++) md[r * nvox + i] = sign * mean * e[r];
+        const double tot = ev[0] + ev[1] + ev[2];
+        for (int r = 0; r < 3; r++) evalue[r * nvox + i] = ev[r] / tot;
+    }
+}
+
+// Symmetric 3x3 eigen solve (ascending eigenvalues, eigenvectors as columns).
+void orc_eig3(const double* A, double* evals, double* evecs) { jacobi3(A, evals, evecs); }
+
+// Ellipsoid phantom rasterised at voxel centres, voxel x at (x-(N+1)/2)*vs (the IR
+// convention, ir.cu:1882).  params: 16 doubles per ellipsoid: centre[3] (mm),
+// semi-axes[3] (mm), rotation[9] (row-major, body <- world), value.  Values add.
+void orc_phantom_ellipsoids(int N, double vs, const double* params, int count, float* vol) {
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; z++)
+        for (int y = 0; y < N; y++)
+            for (int x = 0; x < N; x++) {
+                const double p[3] = {((double)x - (N + 1) * 0.5) * vs, ((double)y - (N + 1) * 0.5) * vs,
+                                     ((double)z - (N + 1) * 0.5) * vs};
+                double acc = 0;
+                for (int e = 0; e < count; e++) {
+                    const double* q = params + 16 * e;
+                    const double d[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+                    double r2 = 0;
+                    for (int i = 0; i < 3; i++) {
+                        const double b = q[6 + 3 * i] * d[0] + q[7 + 3 * i] * d[1] + q[8 + 3 * i] * d[2];
+                        r2 += (b / q[3 + i]) * (b / q[3 + i]);
+                    }
+                    if (r2 <= 1.0) acc += q[15];
+                }
+                vol[x + (int64_t)N * y + (int64_t)N * N * z] = (float)acc;
+            }
+}
+
+int orc_num_threads(void) { return n_threads(); }
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, as the checker or the timed CPU baseline.  ``prec`` selects the
+reference's FP32 arithmetic (REF32) or the same operator in FP64 (F64).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF32, F64 = 0, 1
+
+_f = ctypes.POINTER(ctypes.c_float)
+_d = ctypes.POINTER(ctypes.c_double)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+
+
+class OrcGeometry(ctypes.Structure):
+    _fields_ = [("sdd", ctypes.c_float), ("sod", ctypes.c_float), ("vox_size", ctypes.c_float),
+                ("det_size", ctypes.c_float), ("n_vox", ctypes.c_int32), ("n_det_u", ctypes.c_int32),
+                ("n_det_v", ctypes.c_int32), ("n_proj", ctypes.c_int32)]
+
+
+class OrcScan(ctypes.Structure):
+    _fields_ = [("n_cond", ctypes.c_int32), ("n_basis", ctypes.c_int32), ("cond_R", _f), ("cond_t", _f),
+                ("cond_offset", _f), ("basis", _f), ("grating", _f)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    G, S, I, F = ctypes.POINTER(OrcGeometry), ctypes.POINTER(OrcScan), ctypes.c_int, ctypes.c_float
+    sig = {
+        "orc_default_vox_size": (F, [F, F, F]),
+        "orc_theta_f32": (F, [I, I]),
+        "orc_raycast": (None, [I, G, S, I, I, I, I, I, I, _d]),
+        "orc_forward_view": (None, [I, G, S, I, I, _f, I, I, _d, _d]),
+        "orc_chord_length": (None, [I, G, S, I, I, _d]),
+        "orc_backward_view": (None, [I, G, S, I, I, _f, I, _d, _d]),
+        "orc_subset_orders": (None, [I, I, I, _i32]),
+        "orc_ir_reconstruct": (None, [I, G, S, _f, _f, I, I, I, I, F, _d, _d]),
+        "orc_xtt_reconstruct": (None, [I, G, S, _f, _f, I, I, I, I, F, _d, _d, _d]),
+        "orc_fdk_weight": (None, [I, G, _d]),
+        "orc_fdk_taps": (None, [I, G, I, _d]),
+        "orc_fdk_filter_view": (None, [I, G, I, _f, _d]),
+        "orc_fdk_backproject_view": (None, [I, G, S, I, I, _d, _d]),
+        "orc_fdk_reconstruct": (None, [I, G, S, _f, I, I, _d, _d]),
+        "orc_pca": (None, [_f, I, _f, ctypes.c_int64, _d, _d]),
+        "orc_eig3": (None, [_d, _d, _d]),
+        "orc_phantom_ellipsoids": (None, [I, ctypes.c_double, _d, I, _f]),
+        "orc_num_threads": (I, []),
+    }
+    for n, (r, a) in sig.items():
+        fn = getattr(lib, n)
+        fn.restype, fn.argtypes = r, a
+    return lib
+
+
+_lib = _load()
+
+
+def _fp(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_f)
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_d)
+
+
+class Case:
+    """Geometry + scan tables for the oracle, from plain Python values."""
+
+    def __init__(self, sdd, sod, det, N, U, P, vox_size=None, V=None, cond_R=None, cond_t=None, cond_offset=None,
+                 basis=None, grating=None, n_cond=1):
+        self.g = OrcGeometry()
+        self.g.sdd, self.g.sod, self.g.det_size = sdd, sod, det
+        self.g.vox_size = _lib.orc_default_vox_size(sdd, sod, det) if vox_size is None else vox_size
+        self.g.n_vox, self.g.n_det_u = N, U
+        self.g.n_det_v = U if V is None else V
+        self.g.n_proj = P
+        if cond_R is not None:
+            n_cond = len(np.asarray(cond_R).reshape(-1)) // 9
+        self.n_cond = n_cond
+        self.cond_R = np.ascontiguousarray(np.tile(np.eye(3).reshape(-1), n_cond) if cond_R is None else cond_R,
+                                           dtype=np.float32).reshape(-1)
+        self.cond_t = np.ascontiguousarray(np.zeros(3 * n_cond) if cond_t is None else cond_t,
+                                           dtype=np.float32).reshape(-1)
+        self.cond_offset = np.ascontiguousarray(np.zeros(3 * n_cond) if cond_offset is None else cond_offset,
+                                                dtype=np.float32).reshape(-1)
+        self.basis = None if basis is None else np.ascontiguousarray(basis, dtype=np.float32).reshape(-1)
+        self.grating = None if grating is None else np.ascontiguousarray(grating, dtype=np.float32).reshape(-1)
+        self.s = OrcScan()
+        self.s.n_cond = n_cond
+        self.s.n_basis = 0 if self.basis is None else len(self.basis) // 3
+        self.s.cond_R, self.s.cond_t, self.s.cond_offset = _fp(self.cond_R), _fp(self.cond_t), _fp(self.cond_offset)
+        self.s.basis = _fp(self.basis) if self.basis is not None else None
+        self.s.grating = _fp(self.grating) if self.grating is not None else None
+
+    @property
+    def N(self):
+        return self.g.n_vox
+
+    @property
+    def U(self):
+        return self.g.n_det_u
+
+    @property
+    def V(self):
+        return self.g.n_det_v
+
+    @property
+    def P(self):
+        return self.g.n_proj
+
+    def raycast(self, x, y, z, n, cond=0, conv=0, prec=F64):
+        out = np.zeros(8)
+        _lib.orc_raycast(prec, self.g, self.s, cond, x, y, z, n, conv, _dp(out))
+        return out
+
+    def forward_view(self, vol, n, cond=0, xtt=False, normalise=True, prec=F64):
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
+        p = np.zeros(self.U * self.V)
+        f = np.zeros(self.U * self.V)
+        _lib.orc_forward_view(prec, self.g, self.s, cond, n, _fp(vol), int(xtt), int(normalise), _dp(p), _dp(f))
+        return p.reshape(self.V, self.U), f.reshape(self.V, self.U)
+
+    def chord_length(self, n, cond=0, prec=F64):
+        out = np.zeros(self.U * self.V)
+        _lib.orc_chord_length(prec, self.g, self.s, cond, n, _dp(out))
+        return out.reshape(self.V, self.U)
+
+    def backward_view(self, proj_view, n, tmp, factor, cond=0, xtt=False, prec=F64):
+        pv = np.ascontiguousarray(proj_view, dtype=np.float32).reshape(-1)
+        _lib.orc_backward_view(prec, self.g, self.s, cond, n, _fp(pv), int(xtt), _dp(tmp), _dp(factor))
+
+    def ir_reconstruct(self, sino, vol0, epochs, batch, direction=0, art=False, lam=1e-2, prec=F64):
+        sino = np.ascontiguousarray(sino, dtype=np.float32)
+        vol0 = np.ascontiguousarray(vol0, dtype=np.float32)
+        out = np.zeros(vol0.size)
+        losses = np.zeros(max(1, self.n_cond * epochs))
+        _lib.orc_ir_reconstruct(prec, self.g, self.s, _fp(sino), _fp(vol0), epochs, batch, direction, int(art), lam,
+                                _dp(out), _dp(losses))
+        return out.reshape(vol0.shape), losses[: self.n_cond * epochs]
+
+    def xtt_reconstruct(self, sino, vol0, epochs, batch, direction=0, art=False, lam=1e-2, prec=F64):
+        sino = np.ascontiguousarray(sino, dtype=np.float32)
+        vol0 = np.ascontiguousarray(vol0, dtype=np.float32)
+        out = np.zeros(vol0.size)
+        proj = np.zeros(sino.size)
+        losses = np.zeros(max(1, epochs))
+        _lib.orc_xtt_reconstruct(prec, self.g, self.s, _fp(sino), _fp(vol0), epochs, batch, direction, int(art), lam,
+                                 _dp(out), _dp(proj), _dp(losses))
+        return out.reshape(vol0.shape), proj.reshape(sino.shape), losses[:epochs]
+
+    def fdk_weight(self, prec=F64):
+        out = np.zeros(self.U * self.V)
+        _lib.orc_fdk_weight(prec, self.g, _dp(out))
+        return out.reshape(self.V, self.U)
+
+    def fdk_taps(self, kind=0, prec=F64):
+        out = np.zeros(self.U)
+        _lib.orc_fdk_taps(prec, self.g, kind, _dp(out))
+        return out
+
+    def fdk_filter_view(self, view, kind=0, prec=F64):
+        v = np.ascontiguousarray(view, dtype=np.float32).reshape(-1)
+        out = np.zeros(self.U * self.V)
+        _lib.orc_fdk_filter_view(prec, self.g, kind, _fp(v), _dp(out))
+        return out.reshape(self.V, self.U)
+
+    def fdk_backproject_view(self, filt_view, n, vol, cond=0, prec=F64):
+        fv = np.ascontiguousarray(filt_view, dtype=np.float64).reshape(-1)
+        _lib.orc_fdk_backproject_view(prec, self.g, self.s, cond, n, _dp(fv), _dp(vol))
+
+    def fdk_reconstruct(self, sino, direction=0, kind=0, prec=F64):
+        sino = np.ascontiguousarray(sino, dtype=np.float32)
+        vol = np.zeros(self.n_cond * self.N ** 3)
+        filt = np.zeros(sino.size)
+        _lib.orc_fdk_reconstruct(prec, self.g, self.s, _fp(sino), direction, kind, _dp(vol), _dp(filt))
+        return vol.reshape(self.n_cond, self.N, self.N, self.N), filt.reshape(sino.shape)
+
+
+def theta_f32(n, P):
+    return _lib.orc_theta_f32(n, P)
+
+
+def subset_orders(batch, epochs, per_epoch_engine):
+    out = np.zeros(batch * epochs, dtype=np.int32)
+    _lib.orc_subset_orders(batch, epochs, int(per_epoch_engine), out.ctypes.data_as(_i32))
+    return out.reshape(epochs, batch)
+
+
+def pca(mu, basis):
+    """mu: (K, nvox) float32; returns md (3, nvox), evalue (3, nvox) in FP64."""
+    mu = np.ascontiguousarray(mu, dtype=np.float32)
+    K, nvox = mu.shape
+    b = np.ascontiguousarray(basis, dtype=np.float32).reshape(-1)
+    md = np.zeros(3 * nvox)
+    ev = np.zeros(3 * nvox)
+    _lib.orc_pca(_fp(mu), K, _fp(b), nvox, _dp(md), _dp(ev))
+    return md.reshape(3, nvox), ev.reshape(3, nvox)
+
+
+def eig3(A):
+    A = np.ascontiguousarray(A, dtype=np.float64).reshape(9)
+    w = np.zeros(3)
+    v = np.zeros(9)
+    _lib.orc_eig3(_dp(A), _dp(w), _dp(v))
+    return w, v.reshape(3, 3)
+
+
+def phantom_ellipsoids(N, vs, params):
+    p = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+    vol = np.zeros(N ** 3, dtype=np.float32)
+    _lib.orc_phantom_ellipsoids(N, vs, _dp(p), len(p) // 16, _fp(vol))
+    return vol.reshape(N, N, N)
+
+
+def num_threads():
+    return _lib.orc_num_threads()

@@ -1,0 +1,387 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Tolerances (north star): projections <= 1e-5 relative L2 against the FP64
+oracle; reconstructions <= 1e-4 after the stated iterations.  The FP32
+"ref32" restatement of the reference's arithmetic is also compared; its own
+distance to FP64 is the reference's rounding noise, and the GPU must be no
+further from ref32 than ref32 is from FP64 plus the GPU's own error.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ellipsoid_phantom, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+MEASURED = {}
+
+
+def record(key, value):
+    MEASURED[key] = float(value)
+    os.makedirs(OUT, exist_ok=True)
+    with open(os.path.join(OUT, "parity_errors.json"), "w") as f:
+        json.dump(MEASURED, f, indent=1, sort_keys=True)
+
+
+def rot_x(deg):
+    a = np.deg2rad(deg)
+    return np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+
+
+def rot_y(deg):
+    a = np.deg2rad(deg)
+    return np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+
+
+class Setup:
+    def __init__(self, orc, xr, N, U, P, V=None, vs=0.5, sdd=600.0, sod=300.0, det=0.8, cond_R=None, cond_t=None,
+                 cond_offset=None, basis=None, grating=None, n_cond=1):
+        kw = dict(cond_R=None if cond_R is None else np.asarray(cond_R, np.float64).reshape(-1),
+                  cond_t=cond_t, cond_offset=cond_offset, basis=basis, grating=grating)
+        if cond_R is not None:
+            n_cond = len(kw["cond_R"]) // 9
+        self.case = orc.Case(sdd, sod, det, N, U, P, vox_size=vs, V=V, n_cond=n_cond, **kw)
+        self.geom = xr.make_geometry(sdd, sod, det, N, U, P, vox_size=vs, n_det_v=V)
+        self.scan = xr.Scan(n_cond=n_cond, **kw)
+        self.N, self.U, self.V, self.P, self.C = N, U, U if V is None else V, P, n_cond
+        self.vs = vs
+
+
+def to_dev(a, dev):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32).reshape(-1)).to(dev)
+
+
+def columns(ctx, vol_ref, N, channels, dev):
+    import torch
+
+    vref = to_dev(vol_ref, dev)
+    vcol = torch.empty_like(vref)
+    ctx.to_columns(vref, vcol, N, channels)
+    return vcol
+
+
+def gpu_forward(ctx, st, vol_ref, codes, dev, out_mode, xtt=False, channels=1, sino=None, accumulate=None,
+                proj_init=None):
+    import torch
+
+    plan = ctx.plan(st.geom, st.scan)
+    vcol = columns(ctx, vol_ref, st.N, channels, dev)
+    n = st.C * st.P * st.V * st.U
+    proj = torch.zeros(n, dtype=torch.float32, device=dev) if proj_init is None else to_dev(proj_init, dev)
+    y = None if sino is None else to_dev(sino, dev)
+    loss = plan.project(vcol, proj, codes, out_mode=out_mode, sino=y, accumulate=accumulate, xtt=xtt)
+    ctx.synchronize()
+    return proj.cpu().numpy().reshape(st.C, st.P, st.V, st.U), loss
+
+
+def test_layout_roundtrip(ctx, torch_dev):
+    import torch
+
+    rng = np.random.default_rng(0)
+    for N, ch in [(37, 2), (64, 1)]:
+        ref = rng.random((ch, N, N, N)).astype(np.float32)  # [k][z][y][x]
+        vref = to_dev(ref, torch_dev)
+        vcol = torch.empty_like(vref)
+        ctx.to_columns(vref, vcol, N, ch)
+        col = vcol.cpu().numpy().reshape(ch, N, N, N)  # [k][y][x][z]
+        assert np.array_equal(col, ref.transpose(0, 2, 3, 1))
+        back = torch.empty_like(vref)
+        ctx.from_columns(vcol, back, N, ch)
+        assert np.array_equal(back.cpu().numpy().reshape(ref.shape), ref)
+
+
+FWD_CASES = {
+    "c1_64": dict(N=64, U=96, P=90, vs=0.5),
+    "odd_37_U50_V44": dict(N=37, U=50, V=44, P=30, vs=0.6),
+    "offsets": dict(N=48, U=72, P=40, vs=0.5, cond_t=[0.7, -0.4, 0.3], cond_offset=[0.2, 0.5, -0.6]),
+    "tilt20": dict(N=40, U=64, P=24, vs=0.5, cond_R=rot_x(20.0)),
+    "tilt90": dict(N=40, U=64, P=24, vs=0.5, cond_R=rot_x(90.0) @ rot_y(5.0)),
+}
+
+
+@pytest.mark.parametrize("name", list(FWD_CASES))
+def test_forward_matches_oracle(orc, xr, ctx, torch_dev, name):
+    st = Setup(orc, xr, **FWD_CASES[name])
+    vol, _ = ellipsoid_phantom(orc, xr, st.N, st.vs, seed=101)
+    views = sorted({0, 1, st.P // 3, st.P // 2 + 1, st.P - 1})
+    for direction in (xr.CW, xr.CCW):
+        st.scan  # same tables
+        plan_dir = direction
+        import torch
+
+        plan = ctx.plan(st.geom, st.scan, direction=plan_dir)
+        vcol = columns(ctx, vol, st.N, 1, torch_dev)
+        proj = torch.zeros(st.P * st.V * st.U, device=torch_dev)
+        raw = torch.zeros_like(proj)
+        plan.project(vcol, proj, plan.codes(views), out_mode=xr.OUT_NORMALISED)
+        plan.project(vcol, raw, plan.codes(views), out_mode=xr.OUT_RAW)
+        ctx.synchronize()
+        got = proj.cpu().numpy().reshape(st.P, st.V, st.U)
+        got_raw = raw.cpu().numpy().reshape(st.P, st.V, st.U)
+        worst = worst32 = ref_noise = worst_raw = 0.0
+        sgn = 1 if direction == xr.CW else -1
+        for v in views:
+            want, _ = st.case.forward_view(vol, sgn * v)
+            want32, _ = st.case.forward_view(vol, sgn * v, prec=0)
+            want_raw, _ = st.case.forward_view(vol, sgn * v, normalise=False)
+            worst = max(worst, rel_l2(got[v], want))
+            worst32 = max(worst32, rel_l2(got[v], want32))
+            ref_noise = max(ref_noise, rel_l2(want32, want))
+            worst_raw = max(worst_raw, rel_l2(got_raw[v], want_raw))
+        record("forward_f64/%s/%d" % (name, direction), worst)
+        record("forward_ref32/%s/%d" % (name, direction), worst32)
+        record("ref32_vs_f64/%s/%d" % (name, direction), ref_noise)
+        assert worst <= 1e-5, (name, direction, worst)
+        assert worst_raw <= 1e-5, (name, direction, worst_raw)
+        assert worst32 <= ref_noise + worst + 1e-6
+
+
+def test_forward_accumulate_flag(orc, xr, ctx, torch_dev):
+    st = Setup(orc, xr, N=32, U=48, P=12)
+    vol, _ = ellipsoid_phantom(orc, xr, st.N, st.vs, seed=5)
+    codes = np.arange(12)
+    base = np.random.default_rng(3).random((1, 12, st.V, st.U)).astype(np.float32)
+    once, _ = gpu_forward(ctx, st, vol, codes, torch_dev, xr.OUT_RAW)
+    acc = np.zeros(12, np.uint8)
+    acc[::2] = 1
+    got, _ = gpu_forward(ctx, st, vol, codes, torch_dev, xr.OUT_RAW, accumulate=acc, proj_init=base)
+    want = once.copy()
+    want[:, ::2] += base[:, ::2]
+    assert rel_l2(got, want) < 1e-6
+
+
+@pytest.mark.parametrize("name", ["c1_64", "odd_37_U50_V44", "tilt90"])
+def test_backward_matches_oracle(orc, xr, ctx, torch_dev, name):
+    import torch
+
+    st = Setup(orc, xr, **FWD_CASES[name])
+    rng = np.random.default_rng(9)
+    views = sorted({0, 3, st.P // 2, st.P - 2})
+    img = rng.random((st.P, st.V, st.U)).astype(np.float32)
+    plan = ctx.plan(st.geom, st.scan)
+    N3 = st.N ** 3
+    tmp = torch.zeros(N3, device=torch_dev)
+    fac = torch.zeros(N3, device=torch_dev)
+    plan.backproject(to_dev(img, torch_dev), plan.codes(views), tmp, fac)
+    tref = torch.empty_like(tmp)
+    fref = torch.empty_like(fac)
+    ctx.from_columns(tmp, tref, st.N)
+    ctx.from_columns(fac, fref, st.N)
+    got_t = tref.cpu().numpy()
+    got_f = fref.cpu().numpy()
+    want_t = np.zeros(N3)
+    want_f = np.zeros(N3)
+    for v in views:
+        st.case.backward_view(img[v], v, want_t, want_f)
+    err = rel_l2(got_t, want_t)
+    record("backward_f64/%s" % name, err)
+    assert err <= 1e-5
+    mism = np.count_nonzero(got_f != want_f)
+    record("backward_count_mismatch/%s" % name, mism / N3)
+    assert mism <= 1e-4 * N3
+
+
+XTT_BASIS13 = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1], [1, -1, 1], [-1, 1, 1], [-1, -1, 1],
+                        [1, 1, 0], [1, -1, 0], [1, 0, 1], [-1, 0, 1], [0, 1, 1], [0, -1, 1]], dtype=np.float64)
+XTT_BASIS13 /= np.linalg.norm(XTT_BASIS13, axis=1, keepdims=True)
+
+
+def grating(phis_deg):
+    return np.concatenate([[np.cos(np.deg2rad(p)), 0.0, np.sin(np.deg2rad(p))] for p in phis_deg])
+
+
+@pytest.mark.parametrize("tilt", [False, True])
+def test_xtt_operators_match_oracle(orc, xr, ctx, torch_dev, tilt):
+    import torch
+
+    cond_R = np.concatenate([np.eye(3).reshape(-1), (rot_x(15.0) if tilt else np.eye(3)).reshape(-1)])
+    st = Setup(orc, xr, N=24, U=40, P=16, vs=0.6, cond_R=cond_R, basis=XTT_BASIS13, grating=grating([0, 45]))
+    K = 13
+    rng = np.random.default_rng(12)
+    vol = (rng.random((K, st.N, st.N, st.N)) * 0.02).astype(np.float32)
+    views = [0, 5, 11]
+    codes = np.concatenate([np.array(views) + c * st.P for c in range(2)])
+    got, _ = gpu_forward(ctx, st, vol, codes, torch_dev, xr.OUT_NORMALISED, xtt=True, channels=K)
+    worst = 0.0
+    for c in range(2):
+        for v in views:
+            want, _ = st.case.forward_view(vol, v, cond=c, xtt=True)
+            worst = max(worst, rel_l2(got[c, v], want))
+    record("xtt_forward_f64/tilt=%d" % tilt, worst)
+    assert worst <= 1e-5
+    # raw XTT backprojection: per channel tmp_k and factor_k
+    img = rng.random((2, st.P, st.V, st.U)).astype(np.float32)
+    plan = ctx.plan(st.geom, st.scan)
+    N3 = st.N ** 3
+    tmp = torch.zeros(K * N3, device=torch_dev)
+    fac = torch.zeros(K * N3, device=torch_dev)
+    plan.backproject(to_dev(img, torch_dev), codes, tmp, fac, xtt=True)
+    tref, fref = torch.empty_like(tmp), torch.empty_like(fac)
+    ctx.from_columns(tmp, tref, st.N, K)
+    ctx.from_columns(fac, fref, st.N, K)
+    wt = np.zeros(K * N3)
+    wf = np.zeros(K * N3)
+    for c in range(2):
+        for v in views:
+            st.case.backward_view(img[c, v], v, wt, wf, cond=c, xtt=True)
+    et, ef = rel_l2(tref.cpu().numpy(), wt), rel_l2(fref.cpu().numpy(), wf)
+    record("xtt_backward_tmp/tilt=%d" % tilt, et)
+    record("xtt_backward_factor/tilt=%d" % tilt, ef)
+    assert et <= 1e-5 and ef <= 1e-5
+
+
+@pytest.mark.parametrize("method,batch", [("mlem", 1), ("mlem", 5), ("art", 4)])
+def test_ir_solver_matches_oracle(orc, xr, ctx, method, batch):
+    st = Setup(orc, xr, N=32, U=48, P=24, vs=0.5)
+    truth, _ = ellipsoid_phantom(orc, xr, st.N, st.vs, seed=21)
+    sino = np.stack([st.case.forward_view(truth, n)[0] for n in range(st.P)]).astype(np.float32)
+    art = method == "art"
+    lam = 0.02
+    vol0 = np.full(st.N ** 3, 0.0 if art else 0.01, dtype=np.float32)
+    epochs = 3
+    want, wloss = st.case.ir_reconstruct(sino, vol0, epochs, batch, art=art, lam=lam)
+    v = [vol0.copy()]
+    losses = ctx.ir_reconstruct(st.geom, st.scan, [sino.reshape(-1).copy()], v, epochs, batch,
+                                method=xr.METHOD_ART if art else xr.METHOD_MLEM, lam=lam)
+    err = rel_l2(v[0], want.reshape(-1))
+    record("ir_%s_batch%d" % (method, batch), err)
+    assert err <= 1e-4
+    assert np.allclose(losses, wloss, rtol=1e-4)
+
+
+def test_ir_solver_two_conditions_ccw(orc, xr, ctx):
+    st = Setup(orc, xr, N=24, U=40, P=16, vs=0.6, n_cond=2, cond_t=[0, 0, 0, 0.4, -0.3, 0.2],
+               cond_offset=[0.1, 0.0, 0.0, 0.0, 0.2, -0.1])
+    sinos, vols0 = [], []
+    for c in range(2):
+        truth, _ = ellipsoid_phantom(orc, xr, st.N, st.vs, seed=30 + c)
+        sinos.append(np.stack([st.case.forward_view(truth, -n, cond=c)[0] for n in range(st.P)]).astype(np.float32))
+        vols0.append(np.full(st.N ** 3, 0.01, dtype=np.float32))
+    want, wloss = st.case.ir_reconstruct(np.stack(sinos), np.stack(vols0), 2, 4, direction=1)
+    v = [x.copy() for x in vols0]
+    losses = ctx.ir_reconstruct(st.geom, st.scan, [s.reshape(-1).copy() for s in sinos], v, 2, 4, direction=xr.CCW)
+    err = rel_l2(np.stack(v), want.reshape(2, -1))
+    record("ir_two_cond_ccw", err)
+    assert err <= 1e-4
+    assert np.allclose(losses, wloss, rtol=1e-4)
+
+
+def test_xtt_solver_matches_oracle(orc, xr, ctx):
+    K = 7
+    basis = XTT_BASIS13[[0, 1, 2, 3, 4, 9, 11]]
+    st = Setup(orc, xr, N=20, U=32, P=12, vs=0.6, n_cond=2, cond_R=np.concatenate([np.eye(3).reshape(-1)] * 2),
+               basis=basis, grating=grating([0, 90]))
+    rng = np.random.default_rng(40)
+    truth = (rng.random((K, st.N, st.N, st.N)) * 0.02).astype(np.float32)
+    sino = np.stack([np.stack([st.case.forward_view(truth, n, cond=c, xtt=True)[0] for n in range(st.P)])
+                     for c in range(2)]).astype(np.float32)
+    vol0 = np.full((K, st.N ** 3), 0.01, dtype=np.float32)
+    want, want_proj, wloss = st.case.xtt_reconstruct(sino, vol0, 2, 3)
+    v = [vol0[k].copy() for k in range(K)]
+    s = [sino[c].reshape(-1).copy() for c in range(2)]
+    md = [np.zeros(st.N ** 3, np.float32) for _ in range(3)]
+    ev = [np.zeros(st.N ** 3, np.float32) for _ in range(3)]
+    losses = ctx.xtt_reconstruct(st.geom, st.scan, s, v, md, 2, 3, evalue=ev)
+    err = rel_l2(np.stack(v), want.reshape(K, -1))
+    record("xtt_solver", err)
+    assert err <= 1e-4
+    assert rel_l2(np.stack(s), want_proj.reshape(2, -1)) <= 1e-4  # sinogram overwritten with final ratios
+    assert np.allclose(losses, wloss, rtol=1e-4)
+    omd, oev = orc.pca(np.stack(v), basis)
+    valid = np.isfinite(oev).all(axis=0) & (np.abs(omd).sum(axis=0) > 1e-6)
+    assert rel_l2(np.stack(md)[:, valid], omd[:, valid]) <= 1e-4
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_fdk_matches_oracle(orc, xr, ctx, torch_dev, kind):
+    st = Setup(orc, xr, N=40, U=64, P=48, vs=0.5)
+    truth, _ = ellipsoid_phantom(orc, xr, st.N, st.vs, seed=50)
+    sino = np.stack([st.case.forward_view(truth, n)[0] for n in range(st.P)]).astype(np.float32)
+    want_vol, want_filt = st.case.fdk_reconstruct(sino[None], kind=kind)
+    v = [np.zeros(st.N ** 3, np.float32)]
+    s = [sino.reshape(-1).copy()]
+    ctx.fdk_reconstruct(st.geom, st.scan, s, v, filter=kind)
+    ef = rel_l2(s[0], want_filt.reshape(-1))
+    ev = rel_l2(v[0], want_vol.reshape(-1))
+    record("fdk_filter/%d" % kind, ef)
+    record("fdk_volume/%d" % kind, ev)
+    assert ef <= 1e-5 and ev <= 1e-5
+
+
+def test_pca_kernel_matches_oracle(orc, xr, ctx, torch_dev):
+    import torch
+
+    rng = np.random.default_rng(60)
+    K, nv = 13, 20000
+    mu = rng.random((K, nv)).astype(np.float32)
+    mu[:, :50] = 0.0  # empty voxels: md = 0, eigenvalues 0/0
+    md = torch.zeros(3 * nv, device=torch_dev)
+    evd = torch.zeros(3 * nv, device=torch_dev)
+    ctx.pca(to_dev(mu, torch_dev), XTT_BASIS13, nv, nv, md, evd)
+    got_md = md.cpu().numpy().reshape(3, nv)
+    got_ev = evd.cpu().numpy().reshape(3, nv)
+    want_md, want_ev = orc.pca(mu, XTT_BASIS13)
+    assert np.all(got_md[:, :50] == 0) and np.isnan(got_ev[:, :50]).all() and np.isnan(want_ev[:, :50]).all()
+    e1, e2 = rel_l2(got_md[:, 50:], want_md[:, 50:]), rel_l2(got_ev[:, 50:], want_ev[:, 50:])
+    record("pca_md", e1)
+    record("pca_evalue", e2)
+    assert e1 <= 1e-5 and e2 <= 1e-5
+
+
+def test_phantom_rasterise_matches_oracle(orc, xr, ctx, torch_dev):
+    import torch
+
+    N, vs = 48, 0.5
+    params = xr.ellipsoid_params(77, 10, fov_radius=0.5 * N * vs)
+    want = orc.phantom_ellipsoids(N, vs, params)
+    out = torch.zeros(N ** 3, device=torch_dev)
+    ctx.phantom_rasterise(params, N, vs, out)
+    got = out.cpu().numpy().reshape(N, N, N)
+    assert np.count_nonzero(np.abs(got - want) > 1e-7) <= 2
+
+
+def test_full_size_adjoint_and_sampled_view(orc, xr, ctx, torch_dev):
+    """512^3 / 768^2 geometry (the benchmark's): splat/gather adjointness on the GPU
+    (FP64 reductions) and one full-size view against the FP64 oracle."""
+    import torch
+
+    st = Setup(orc, xr, N=512, U=768, P=720, vs=0.2, sdd=1000.0, sod=500.0, det=0.4)
+    N3 = st.N ** 3
+    g = torch.Generator(device=torch_dev).manual_seed(3)
+    x = torch.rand(N3, device=torch_dev, generator=g)
+    plan = ctx.plan(st.geom, st.scan)
+    views = [0, 241]
+    codes = plan.codes(views)
+    proj = torch.zeros(st.P * st.V * st.U, device=torch_dev)
+    plan.project(x, proj, codes, out_mode=xr.OUT_RAW)
+    y = torch.rand(st.P * st.V * st.U, device=torch_dev, generator=g)
+    tmp = torch.zeros(N3, device=torch_dev)
+    fac = torch.zeros(N3, device=torch_dev)
+    plan.backproject(y, codes, tmp, fac)
+    ctx.synchronize()
+    P2 = proj.view(st.P, -1)[views].double()
+    Y2 = y.view(st.P, -1)[views].double()
+    lhs = float((P2 * Y2).sum())
+    ratio = 1.0
+    rhs = ratio * float((x.double() * tmp.double()).sum())
+    record("full_size_adjoint", abs(lhs - rhs) / abs(lhs))
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    # one full-size view of an ellipsoid phantom against the FP64 oracle
+    params = xr.ellipsoid_params(2024, 40, fov_radius=0.5 * st.N * st.vs)
+    vref = torch.zeros(N3, device=torch_dev)
+    ctx.phantom_rasterise(params, st.N, st.vs, vref)
+    vcol = torch.empty_like(vref)
+    ctx.to_columns(vref, vcol, st.N)
+    proj.zero_()
+    plan.project(vcol, proj, plan.codes([97]), out_mode=xr.OUT_NORMALISED)
+    got = proj.view(st.P, st.V, st.U)[97].cpu().numpy()
+    want, _ = st.case.forward_view(vref.cpu().numpy(), 97)
+    err = rel_l2(got, want)
+    record("full_size_view_f64", err)
+    assert err <= 1e-5

@@ -1,0 +1,361 @@
+"""Python binding of libxr.so, the MI355X cone-beam reconstruction library.
+
+ctypes over the C ABI declared in include/xr.h.  Device buffers are torch
+tensors on the HIP device (torch is only used for memory and streams); host
+buffers are numpy arrays.  The functions mirror the reference's host API
+(reconstruct/include/reconstruct.cuh:14-73): ``ir_reconstruct``,
+``xtt_reconstruct``, ``fdk_reconstruct``, plus the device-level operators the
+solvers are built from.  There is no CPU fallback: if libxr.so is missing or
+fails to load, importing this package raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxr.so")
+
+XR_OK = 0
+CONV_IR, CONV_FDK = 0, 1
+CW, CCW = 0, 1
+METHOD_XTT, METHOD_MLEM, METHOD_ART, METHOD_FDK = 0, 1, 2, 3
+FILTER_REFERENCE, FILTER_RAMLAK = 0, 1
+OUT_RAW, OUT_NORMALISED, OUT_RATIO, OUT_RESIDUAL = 0, 1, 2, 3
+TIMING_FORWARD, TIMING_BACKWARD, TIMING_FILTER, TIMING_OTHER = 0, 1, 2, 3
+
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_int32_p = ctypes.POINTER(ctypes.c_int32)
+_c_uint8_p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class Geometry(ctypes.Structure):
+    """xr_geometry: the reference Geometry (geometry.h:8-25) with U and V split."""
+
+    _fields_ = [("sdd", ctypes.c_float), ("sod", ctypes.c_float), ("vox_size", ctypes.c_float),
+                ("det_size", ctypes.c_float), ("n_vox", ctypes.c_int32), ("n_det_u", ctypes.c_int32),
+                ("n_det_v", ctypes.c_int32), ("n_proj", ctypes.c_int32)]
+
+
+class ScanStruct(ctypes.Structure):
+    _fields_ = [("n_cond", ctypes.c_int32), ("n_basis", ctypes.c_int32), ("cond_R", _c_float_p),
+                ("cond_t", _c_float_p), ("cond_offset", _c_float_p), ("basis", _c_float_p),
+                ("grating", _c_float_p)]
+
+
+class Settings(ctypes.Structure):
+    _fields_ = [("proj_path", ctypes.c_char * 512), ("vol_path", ctypes.c_char * 512),
+                ("direc_path", ctypes.c_char * 512), ("block_size", ctypes.c_int32), ("n_basis", ctypes.c_int32),
+                ("n_cond", ctypes.c_int32), ("sod", ctypes.c_float), ("sdd", ctypes.c_float),
+                ("det_size", ctypes.c_float), ("n_proj", ctypes.c_int32), ("n_det_u", ctypes.c_int32),
+                ("n_det_v", ctypes.c_int32), ("n_vox", ctypes.c_int32), ("load_index", ctypes.c_int32 * 64),
+                ("cond_R", ctypes.c_float * (9 * 64)), ("cond_t", ctypes.c_float * (3 * 64)),
+                ("cond_offset", ctypes.c_float * (3 * 64)), ("basis", ctypes.c_float * (3 * 16))]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("losses", _c_double_p)]
+
+
+class XRError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libxr.so not built (%s); run __graft_entry__.build() or "
+            "python xray-directional-reconstruction_amd/build_lib.py" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I, F, D, V = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double, None
+    sig = {
+        "xr_context_create": (I, [ctypes.POINTER(ctypes.c_void_p), I]),
+        "xr_context_destroy": (V, [P]),
+        "xr_context_set_stream": (I, [P, P]),
+        "xr_context_synchronize": (I, [P]),
+        "xr_last_error": (ctypes.c_char_p, [P]),
+        "xr_timing_enable": (I, [P, I]),
+        "xr_timing_reset": (I, [P]),
+        "xr_timing_read": (I, [P, I, _c_double_p, ctypes.POINTER(ctypes.c_int64)]),
+        "xr_plan_create": (I, [P, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct), I, I,
+                               ctypes.POINTER(ctypes.c_void_p)]),
+        "xr_plan_destroy": (V, [P]),
+        "xr_default_vox_size": (F, [F, F, F]),
+        "xr_volume_to_columns": (I, [P, P, P, I, I]),
+        "xr_volume_from_columns": (I, [P, P, P, I, I]),
+        "xr_project": (I, [P, P, P, P, P, _c_int32_p, I, _c_uint8_p, I, I, _c_double_p]),
+        "xr_backproject": (I, [P, P, P, _c_int32_p, I, I, P, P]),
+        "xr_backproject_update": (I, [P, P, P, _c_int32_p, I, I, I, F, I, P]),
+        "xr_fdk_filter": (I, [P, P, I, P, P, _c_int32_p, I]),
+        "xr_fdk_backproject": (I, [P, P, P, _c_int32_p, I, I, P]),
+        "xr_fdk_taps": (I, [ctypes.POINTER(Geometry), I, _c_float_p]),
+        "xr_volume_sqrt": (I, [P, P, ctypes.c_int64]),
+        "xr_pca": (I, [P, P, I, _c_float_p, ctypes.c_int64, ctypes.c_int64, P, P]),
+        "xr_ir_reconstruct": (I, [P, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct),
+                                  ctypes.POINTER(_c_float_p), ctypes.POINTER(_c_float_p), I, I, I, I, F,
+                                  ctypes.POINTER(Stats)]),
+        "xr_xtt_reconstruct": (I, [P, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct),
+                                   ctypes.POINTER(_c_float_p), ctypes.POINTER(_c_float_p),
+                                   ctypes.POINTER(_c_float_p), ctypes.POINTER(_c_float_p), I, I, I, I, F,
+                                   ctypes.POINTER(Stats)]),
+        "xr_fdk_reconstruct": (I, [P, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct),
+                                   ctypes.POINTER(_c_float_p), ctypes.POINTER(_c_float_p), I, I]),
+        "xr_load_settings": (I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Settings)]),
+        "xr_settings_geometry": (I, [ctypes.POINTER(Settings), ctypes.POINTER(Geometry),
+                                     ctypes.POINTER(ScanStruct)]),
+        "xr_phantom_ellipsoid_params": (I, [ctypes.c_uint64, I, D, D, D, D, D, D, _c_double_p]),
+        "xr_phantom_rasterise": (I, [P, _c_double_p, I, I, F, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = _load()
+EXPORTED = tuple(sorted(n for n in dir(_lib) if n.startswith("xr_")))
+
+
+def lib():
+    return _lib
+
+
+# ---------------------------------------------------------------- helpers --
+
+def _fptr(a):
+    return a.ctypes.data_as(_c_float_p)
+
+
+def _dptr(x):
+    """Device pointer of a torch tensor (or an int address)."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return ctypes.c_void_p(x)
+    if not x.is_cuda:
+        raise XRError("expected a device tensor")
+    if not x.is_contiguous():
+        raise XRError("expected a contiguous tensor")
+    return ctypes.c_void_p(x.data_ptr())
+
+
+def default_vox_size(sdd, sod, det_size):
+    """voxSize = sod * detSize / sdd (geometry.h:13)."""
+    return float(_lib.xr_default_vox_size(sdd, sod, det_size))
+
+
+def make_geometry(sdd, sod, det_size, n_vox, n_det, n_proj, vox_size=None, n_det_v=None):
+    g = Geometry()
+    g.sdd, g.sod, g.det_size = sdd, sod, det_size
+    g.vox_size = default_vox_size(sdd, sod, det_size) if vox_size is None else vox_size
+    g.n_vox, g.n_det_u = n_vox, n_det
+    g.n_det_v = n_det if n_det_v is None else n_det_v
+    g.n_proj = n_proj
+    return g
+
+
+class Scan:
+    """Per-condition tables (rotation, translation, offset, grating) and basis vectors."""
+
+    def __init__(self, cond_R=None, cond_t=None, cond_offset=None, basis=None, grating=None, n_cond=1):
+        n_cond = len(cond_R) // 9 if cond_R is not None else n_cond
+        eye = np.tile(np.eye(3, dtype=np.float32).reshape(-1), n_cond)
+        self.cond_R = np.ascontiguousarray(eye if cond_R is None else cond_R, dtype=np.float32).reshape(-1)
+        self.cond_t = np.ascontiguousarray(np.zeros(3 * n_cond) if cond_t is None else cond_t,
+                                           dtype=np.float32).reshape(-1)
+        self.cond_offset = np.ascontiguousarray(np.zeros(3 * n_cond) if cond_offset is None else cond_offset,
+                                                dtype=np.float32).reshape(-1)
+        self.basis = None if basis is None else np.ascontiguousarray(basis, dtype=np.float32).reshape(-1)
+        self.grating = None if grating is None else np.ascontiguousarray(grating, dtype=np.float32).reshape(-1)
+        self.n_cond = n_cond
+        self.n_basis = 0 if self.basis is None else len(self.basis) // 3
+        s = ScanStruct()
+        s.n_cond, s.n_basis = self.n_cond, self.n_basis
+        s.cond_R, s.cond_t, s.cond_offset = _fptr(self.cond_R), _fptr(self.cond_t), _fptr(self.cond_offset)
+        s.basis = _fptr(self.basis) if self.basis is not None else None
+        s.grating = _fptr(self.grating) if self.grating is not None else None
+        self.struct = s
+
+
+class Context:
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        rc = _lib.xr_context_create(ctypes.byref(h), device)
+        if rc != XR_OK:
+            raise XRError("xr_context_create failed (%d)" % rc)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _lib.xr_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != XR_OK:
+            raise XRError("libxr error %d: %s" % (rc, _lib.xr_last_error(self.h).decode()))
+
+    def set_stream(self, stream_ptr):
+        self.check(_lib.xr_context_set_stream(self.h, ctypes.c_void_p(stream_ptr)))
+
+    def synchronize(self):
+        self.check(_lib.xr_context_synchronize(self.h))
+
+    def timing(self, enable=True):
+        self.check(_lib.xr_timing_enable(self.h, int(enable)))
+
+    def timing_reset(self):
+        self.check(_lib.xr_timing_reset(self.h))
+
+    def timing_read(self, kind):
+        t, n = ctypes.c_double(), ctypes.c_int64()
+        self.check(_lib.xr_timing_read(self.h, kind, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+    # ---- plans and device operators ----
+    def plan(self, geom, scan, convention=CONV_IR, direction=CW):
+        return Plan(self, geom, scan, convention, direction)
+
+    def to_columns(self, ref, col, n_vox, channels=1):
+        self.check(_lib.xr_volume_to_columns(self.h, _dptr(ref), _dptr(col), n_vox, channels))
+
+    def from_columns(self, col, ref, n_vox, channels=1):
+        self.check(_lib.xr_volume_from_columns(self.h, _dptr(col), _dptr(ref), n_vox, channels))
+
+    def volume_sqrt(self, vol):
+        self.check(_lib.xr_volume_sqrt(self.h, _dptr(vol), vol.numel()))
+
+    def pca(self, mu, basis, nvox, chan_stride, md, evalue):
+        b = np.ascontiguousarray(basis, dtype=np.float32).reshape(-1)
+        self.check(_lib.xr_pca(self.h, _dptr(mu), len(b) // 3, _fptr(b), nvox, chan_stride, _dptr(md), _dptr(evalue)))
+
+    def phantom_rasterise(self, params, n_vox, vox_size, vol_ref):
+        p = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        self.check(_lib.xr_phantom_rasterise(self.h, p.ctypes.data_as(_c_double_p), len(p) // 16, n_vox,
+                                             vox_size, _dptr(vol_ref)))
+
+    # ---- host-buffer solvers (reference API) ----
+    def ir_reconstruct(self, geom, scan, sino, vol, epochs, batch, direction=CW, method=METHOD_MLEM, lam=1e-2):
+        """IR::reconstruct: sino/vol are lists (one per condition) of float32 numpy arrays; vol in/out."""
+        sp, vp = _ptr_list(sino), _ptr_list(vol)
+        losses = np.zeros(max(1, scan.n_cond * epochs), dtype=np.float64)
+        st = Stats(losses.ctypes.data_as(_c_double_p))
+        self.check(_lib.xr_ir_reconstruct(self.h, ctypes.byref(geom), ctypes.byref(scan.struct), sp, vp, epochs,
+                                          batch, direction, method, lam, ctypes.byref(st)))
+        return losses[: scan.n_cond * epochs]
+
+    def xtt_reconstruct(self, geom, scan, sino, vol, md, epochs, batch, direction=CW, method=METHOD_MLEM,
+                        lam=1e-2, evalue=None):
+        sp, vp, mp = _ptr_list(sino), _ptr_list(vol), _ptr_list(md)
+        ep = _ptr_list(evalue) if evalue is not None else None
+        losses = np.zeros(max(1, epochs), dtype=np.float64)
+        st = Stats(losses.ctypes.data_as(_c_double_p))
+        self.check(_lib.xr_xtt_reconstruct(self.h, ctypes.byref(geom), ctypes.byref(scan.struct), sp, vp, mp, ep,
+                                           epochs, batch, direction, method, lam, ctypes.byref(st)))
+        return losses[:epochs]
+
+    def fdk_reconstruct(self, geom, scan, sino, vol, direction=CW, filter=FILTER_REFERENCE):
+        sp, vp = _ptr_list(sino), _ptr_list(vol)
+        self.check(_lib.xr_fdk_reconstruct(self.h, ctypes.byref(geom), ctypes.byref(scan.struct), sp, vp, direction,
+                                           filter))
+
+
+def _ptr_list(arrs):
+    for a in arrs:
+        if a.dtype != np.float32 or not a.flags.c_contiguous:
+            raise XRError("expected contiguous float32 arrays")
+    return (_c_float_p * len(arrs))(*[_fptr(a) for a in arrs])
+
+
+class Plan:
+    def __init__(self, ctx, geom, scan, convention=CONV_IR, direction=CW):
+        self.ctx, self.geom, self.scan = ctx, geom, scan
+        h = ctypes.c_void_p()
+        ctx.check(_lib.xr_plan_create(ctx.h, ctypes.byref(geom), ctypes.byref(scan.struct), convention, direction,
+                                      ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _lib.xr_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def codes(self, views, cond=0):
+        return np.ascontiguousarray(np.asarray(views, dtype=np.int64) + cond * self.geom.n_proj, dtype=np.int32)
+
+    def project(self, vol, proj, codes, out_mode=OUT_NORMALISED, sino=None, accumulate=None, xtt=False):
+        c = np.ascontiguousarray(codes, dtype=np.int32)
+        acc = None if accumulate is None else np.ascontiguousarray(accumulate, dtype=np.uint8)
+        loss = ctypes.c_double(0.0)
+        self.ctx.check(_lib.xr_project(self.ctx.h, self.h, _dptr(vol), _dptr(proj), _dptr(sino),
+                                       c.ctypes.data_as(_c_int32_p), len(c),
+                                       None if acc is None else acc.ctypes.data_as(_c_uint8_p), int(xtt), out_mode,
+                                       ctypes.byref(loss)))
+        return loss.value
+
+    def backproject(self, proj, codes, tmp, factor, xtt=False):
+        c = np.ascontiguousarray(codes, dtype=np.int32)
+        self.ctx.check(_lib.xr_backproject(self.ctx.h, self.h, _dptr(proj), c.ctypes.data_as(_c_int32_p), len(c),
+                                           int(xtt), _dptr(tmp), _dptr(factor)))
+
+    def backproject_update(self, proj, codes, vol, method=METHOD_MLEM, lam=0.0, subset_size=1, xtt=False):
+        c = np.ascontiguousarray(codes, dtype=np.int32)
+        self.ctx.check(_lib.xr_backproject_update(self.ctx.h, self.h, _dptr(proj), c.ctypes.data_as(_c_int32_p),
+                                                  len(c), int(xtt), method, lam, subset_size, _dptr(vol)))
+
+    def fdk_filter(self, sino, filt, codes, filter=FILTER_REFERENCE):
+        c = np.ascontiguousarray(codes, dtype=np.int32)
+        self.ctx.check(_lib.xr_fdk_filter(self.ctx.h, self.h, filter, _dptr(sino), _dptr(filt),
+                                          c.ctypes.data_as(_c_int32_p), len(c)))
+
+    def fdk_backproject(self, filt, codes, vol, overwrite=True):
+        c = np.ascontiguousarray(codes, dtype=np.int32)
+        self.ctx.check(_lib.xr_fdk_backproject(self.ctx.h, self.h, _dptr(filt), c.ctypes.data_as(_c_int32_p),
+                                               len(c), int(overwrite), _dptr(vol)))
+
+
+def fdk_taps(geom, filter=FILTER_REFERENCE):
+    out = np.zeros(geom.n_det_u, dtype=np.float32)
+    rc = _lib.xr_fdk_taps(ctypes.byref(geom), filter, _fptr(out))
+    if rc != XR_OK:
+        raise XRError("xr_fdk_taps failed (%d)" % rc)
+    return out
+
+
+def load_settings(path, tag):
+    """init_params(tag) equivalent: returns (Settings, Geometry, Scan)."""
+    st = Settings()
+    rc = _lib.xr_load_settings(path.encode(), tag.encode(), ctypes.byref(st))
+    if rc != XR_OK:
+        raise XRError("xr_load_settings(%s, %s) failed (%d)" % (path, tag, rc))
+    g = Geometry()
+    s = ScanStruct()
+    _lib.xr_settings_geometry(ctypes.byref(st), ctypes.byref(g), ctypes.byref(s))
+    scan = Scan(cond_R=np.ctypeslib.as_array(st.cond_R)[: 9 * st.n_cond].copy(),
+                cond_t=np.ctypeslib.as_array(st.cond_t)[: 3 * st.n_cond].copy(),
+                cond_offset=np.ctypeslib.as_array(st.cond_offset)[: 3 * st.n_cond].copy(),
+                basis=np.ctypeslib.as_array(st.basis)[: 3 * st.n_basis].copy() if st.n_basis else None)
+    return st, g, scan
+
+
+def ellipsoid_params(seed, count, fov_radius, centre_frac=0.6, axis_lo=0.1, axis_hi=0.35, value_lo=0.0,
+                     value_hi=0.05):
+    out = np.zeros(16 * max(count, 1), dtype=np.float64)
+    rc = _lib.xr_phantom_ellipsoid_params(seed, count, fov_radius, centre_frac, axis_lo, axis_hi, value_lo, value_hi,
+                                          out.ctypes.data_as(_c_double_p))
+    if rc != XR_OK:
+        raise XRError("xr_phantom_ellipsoid_params failed (%d)" % rc)
+    return out[: 16 * count].reshape(count, 16)

@@ -1,0 +1,130 @@
+// Internal declarations shared by the HIP kernels (xr_kernels.hip) and the
+// host runtime (xr_runtime.cpp).  Not part of the public C ABI (include/xr.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace xr {
+
+// Detector band width (pixels along u) owned by one forward-projection block.
+constexpr int kBandU = 16;
+// LDS row stride of a band accumulator: one guard column on each side plus
+// one pad column so the stride is odd (conflict-free rows on 32 banks).
+constexpr int kBandStride = kBandU + 3;
+// Max basis vectors (scattering directions) per voxel for the XTT operator.
+constexpr int kMaxBasis = 16;
+// LDS capacity (floats) of the detector footprint staged by a backprojection block.
+constexpr int kStageCap = 12288;
+
+// Per-(condition, view) projection table, computed on the host in FP64 and
+// rounded once to FP32.  With centred voxel coordinates x' = x - xc (same for
+// y, z) the reference's ray casting (ir.cu:1852-1902, fdk.cu:111-148) reduces to
+//   u = hu + nu/den,  v = hv + nv/den,
+//   nu = au.(x',y',z',1), nv = av.(x',y',z',1), den = ad.(x',y',z',1).
+// d0: source->voxel vector at x'=0 (d = vs*x' + d0, only its direction is used).
+// fy: un-rotated voxel y at y'=0 (FDK distance weight, fdk.cu:160).
+// G:  grating sensitivity vector (XTT, ir.cu:1901).  axis: 1 when au[2] and
+// ad[2] are exactly zero (u constant along z columns).
+struct ViewGeom {
+    float au[4];
+    float av[4];
+    float ad[4];
+    float d0[3];
+    float fy;
+    float G[3];
+    float axis;
+    float pad[4];
+};
+static_assert(sizeof(ViewGeom) == 96, "ViewGeom layout");
+
+// FP64 pixel-ray geometry for the chord length of correlationProjByLength
+// (ir.cu:1533-1597): source, detector axes, box faces.
+struct ChordGeom {
+    double det[2];  // detector pixel pitch, sdd - sod
+    double src[3];
+    double eu[3];
+    double ev[3];
+    double ey[3];
+    double lo[3];
+    double hi[3];
+};
+
+enum OutMode { OUT_RAW = 0, OUT_NORMALISED = 1, OUT_RATIO = 2, OUT_RESIDUAL = 3, OUT_NORM_TABLE = 4 };
+enum FwdKind { FWD_WEIGHT = 0, FWD_PLAIN = 1, FWD_XTT = 2 };
+enum BwdKind { BWD_RAW = 0, BWD_MLEM = 1, BWD_ART = 2, BWD_FDK = 3, BWD_XTT_RAW = 4, BWD_XTT_MLEM = 5, BWD_XTT_ART = 6 };
+
+struct Dims {
+    int N, U, V, P, K;
+    float xc;        // voxel index of the centred origin ((N+1)/2 IR, (N-1)/2 FDK)
+    float hu, hv;    // 0.5f*U, 0.5f*V
+    float umax, vmax;  // (float)U - 0.55f, (float)V - 0.55f
+    float vs;
+    float ratio;     // (vs*vs)/(det*(sod/sdd)*det*(sod/sdd))
+    float sod;
+    float cfdk;      // 2 pi / P (FP32, fdk.cu:161)
+};
+
+struct Basis {
+    float S[kMaxBasis][3];
+    float S2[kMaxBasis];
+};
+
+struct FwdArgs {
+    Dims d;
+    Basis b;
+    const ViewGeom* vg;      // [n_cond*P]
+    const ChordGeom* cg;     // [n_cond*P] (OUT_NORM_TABLE only)
+    const int32_t* views;    // launch slot -> cond*P + view
+    const uint8_t* accum;    // launch slot -> 1: add the splat to proj's current contents
+    const float* vol;        // internal layout; plain: + cond*vol_cond_stride; XTT: K channels of chan_stride
+    int64_t vol_cond_stride;
+    int64_t chan_stride;
+    float* proj;             // [n_cond][P][V][U]
+    const float* sino;       // measured data, same layout (RATIO / RESIDUAL)
+    const float* norm;       // L/(factor+1e-5), same layout (NORMALISED / RATIO / RESIDUAL)
+    const float* g2;         // XTT [n_cond*P][K]: (S_k.G)^2
+    double* loss;            // accumulated sum |y - p| (RATIO / RESIDUAL)
+    int code_base;           // proj/sino image of a view = code - code_base (norm uses code)
+    int out_mode;
+    int n_bands;
+    int n_slots;
+    int v0, vb;              // detector rows [v0, v0+vb) owned by this launch
+};
+
+struct BwdArgs {
+    Dims d;
+    Basis b;
+    const ViewGeom* vg;
+    const int32_t* views;    // cond*P + view, looped in order by every block
+    int n_views;
+    const float* proj;       // [n_cond][P][V][U] ratio / residual / filtered images
+    int code_base;           // image of a view = code - code_base
+    float* vol;              // internal layout (K channels of chan_stride for XTT)
+    float* tmp;              // BWD_RAW / BWD_XTT_RAW accumulators (internal layout)
+    float* factor;
+    int64_t chan_stride;
+    const float* g2;         // XTT [n_cond*P][K]: (S_k.G)^2
+    float alpha;             // ART step lambda/subsetSize
+    int overwrite;           // FDK: 1 -> vol = sum, 0 -> vol += sum
+    int zw, gx, gy;           // launch shape, set by launch_backward
+    int tiles_x, tiles_y, tiles_z, n_tiles, per_xcd;
+};
+
+// Launchers (xr_kernels.hip).  All enqueue on `stream` and return hipSuccess
+// or the launch error.
+hipError_t launch_forward(const FwdArgs& a, int kind, bool axis, hipStream_t stream);
+hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream);
+hipError_t launch_fdk_filter(const float* src, float* dst, const float* weight, const float* taps, int U, int V,
+                             const int32_t* views, int n_views, int code_base, hipStream_t stream);
+hipError_t launch_fdk_weight(float* weight, int U, int V, float sdd, float det, hipStream_t stream);
+hipError_t launch_to_internal(const float* ref, float* internal, int N, int channels, hipStream_t stream);
+hipError_t launch_from_internal(const float* internal, float* ref, int N, int channels, hipStream_t stream);
+hipError_t launch_sqrt_abs(float* vol, int64_t n, hipStream_t stream);
+hipError_t launch_pca(const float* mu, int K, const Basis& b, int64_t nvox, int64_t chan_stride, float* md,
+                      float* evalue, hipStream_t stream);
+hipError_t launch_fill(float* p, int64_t n, float value, hipStream_t stream);
+hipError_t launch_clamp_min(float* p, int64_t n, float floor_value, float replacement, hipStream_t stream);
+hipError_t launch_ellipsoids(float* vol, int N, float vs, const double* params_dev, int count, hipStream_t stream);
+
+}  // namespace xr

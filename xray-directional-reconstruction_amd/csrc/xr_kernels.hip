@@ -1,0 +1,875 @@
+// HIP kernels (gfx950) for the cone-beam reconstruction hot path.
+//
+// Volume layout in HBM ("column layout"): voxel (x, y, z) of channel k lives at
+//   k*chan_stride + z + N*(x + N*y),
+// i.e. z-columns are contiguous.  The rotation axis is z, so for an untilted
+// scan (condR = I) a z-column projects onto a single detector column pair with
+// v advancing ~1 pixel per voxel: 64 lanes walking 64 consecutive z of one
+// column read 256 contiguous bytes and hit 64 distinct detector rows.
+// Sinograms keep the reference layout u + U*(v + V*view) per condition.
+//
+// Forward projection (ir.cu:1490-1531, 1753-1804 + 1533-1603 + 1616-1666) is
+// voxel-driven, like the reference, but each block OWNS a band of kBandU
+// detector columns of one view: it accumulates the bilinear splat of every voxel
+// that touches the band in LDS (ds_add_f32), then writes the band once with the
+// L/(factor+1e-5) normalisation and the MLEM ratio / ART residual fused in.
+// No global atomics; the sinogram is written exactly once per view.
+//
+// Backprojection (ir.cu:1720-1751, 1807-1850, fdk.cu:62-170) is a gather: a
+// block owns a tile of voxels, loops over all views of the subset, stages the
+// tile's detector footprint in LDS and accumulates in registers; the MLEM/ART
+// update (ir.cu:1637-1693) is applied in the same kernel, so each voxel is read
+// and written once per subset.
+#include <hip/hip_runtime.h>
+
+#include "xr_internal.hpp"
+
+namespace xr {
+
+namespace {
+
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
+struct ColForms {
+    float nu, nv, nd;
+};
+
+// The projective forms at z' = 0 for column (x', y').  Every kernel evaluates
+// u, v through col_forms + voxel_uv so that the splat, the gather and the
+// normalisation table agree bit for bit on (u, v) and the acceptance window.
+__device__ __forceinline__ ColForms col_forms(const ViewGeom& g, float xp, float yp) {
+    ColForms c;
+    c.nu = fmaf(g.au[1], yp, fmaf(g.au[0], xp, g.au[3]));
+    c.nv = fmaf(g.av[1], yp, fmaf(g.av[0], xp, g.av[3]));
+    c.nd = fmaf(g.ad[1], yp, fmaf(g.ad[0], xp, g.ad[3]));
+    return c;
+}
+
+__device__ __forceinline__ void voxel_uv(const ViewGeom& g, const ColForms& c, float zp, const Dims& d, float& u,
+                                         float& v) {
+    const float nu = fmaf(g.au[2], zp, c.nu);
+    const float nv = fmaf(g.av[2], zp, c.nv);
+    const float nd = fmaf(g.ad[2], zp, c.nd);
+    const float inv = rcpf(nd);
+    u = fmaf(nu, inv, d.hu);
+    v = fmaf(nv, inv, d.hv);
+}
+
+// Acceptance window 0.55 < u < U-0.55 (ir.cu:1501) and bilinear split with
+// floor(u-0.5) (ir.cu:1504-1509).
+__device__ __forceinline__ bool accept_u(float u, const Dims& d) { return 0.55f < u && u < d.umax; }
+__device__ __forceinline__ bool accept_v(float v, const Dims& d) { return 0.55f < v && v < d.vmax; }
+__device__ __forceinline__ void split(float u, int& iu, float& fu) {
+    const float t = u - 0.5f;
+    const float f = floorf(t);
+    iu = (int)f;
+    fu = t - f;
+}
+
+typedef __attribute__((address_space(3))) float lds_float;
+
+__device__ __forceinline__ void lds_add(lds_float* p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Chord length of the pixel-centre ray through the reconstruction box, FP64,
+// following correlationProjByLength (ir.cu:1556-1597): the six faces in order
+// x-,x+,y-,y+,z-,z+; a face hit counts when the other two coordinates lie
+// strictly inside; L is the distance between the first two hits (one hit: its
+// distance from the origin, as the reference's zero-filled buffer gives).
+__device__ double chord_length(const ChordGeom& c, int u, int v, const Dims& d) {
+    const double cu = ((double)u - ((double)d.U - 1.0) * 0.5) * c.det[0];
+    const double cv = ((double)v - ((double)d.V - 1.0) * 0.5) * c.det[0];
+    double pix[3], s2p[3];
+    for (int i = 0; i < 3; i++) {
+        pix[i] = cu * c.eu[i] - c.det[1] * c.ey[i] + cv * c.ev[i];
+        s2p[i] = c.src[i] - pix[i];
+    }
+    int cnt = 0;
+    double h0x = 0, h0y = 0, h0z = 0, h1x = 0, h1y = 0, h1z = 0;
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+#pragma unroll
+        for (int sg = 0; sg < 2; sg++) {
+            const double plane = sg ? c.hi[p] : c.lo[p];
+            const double t = (plane - c.src[p]) / s2p[p];
+            double pl[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) pl[i] = t * s2p[i] + c.src[i];
+            const int p1 = (p + 1) % 3, p2 = (p + 2) % 3;
+            if (c.lo[p1] < pl[p1] && pl[p1] < c.hi[p1] && c.lo[p2] < pl[p2] && pl[p2] < c.hi[p2]) {
+                if (cnt == 0) {
+                    h0x = pl[0];
+                    h0y = pl[1];
+                    h0z = pl[2];
+                } else if (cnt == 1) {
+                    h1x = pl[0];
+                    h1y = pl[1];
+                    h1z = pl[2];
+                }
+                cnt++;
+            }
+        }
+    }
+    const double dx = h0x - h1x, dy = h0y - h1y, dz = h0z - h1z;
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// ---------------------------------------------------------------------------
+// Forward projection: one block = (detector band of kBandU columns, one view).
+// ---------------------------------------------------------------------------
+template <int KIND, bool AXIS>
+__global__ __launch_bounds__(256) void k_forward(const FwdArgs a) {
+    extern __shared__ float acc[];
+    const Dims& d = a.d;
+    const int nblk = a.n_bands * a.n_slots;
+    const int per_xcd = (nblk + 7) / 8;
+    // XCD-chunked order: each XCD walks a contiguous run of (band, view) pairs,
+    // band-major, so concurrent blocks on one L2 share a band at adjacent views.
+    const int o = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (o >= nblk) return;
+    const int band = o / a.n_slots, slot = o % a.n_slots;
+    const int code = a.views[slot];
+    const int cond = code / d.P;
+    const ViewGeom g = a.vg[code];
+    const int u0 = band * kBandU;
+    const int v0 = a.v0, vb = a.vb;
+    const int rows = vb + 2;
+    for (int i = threadIdx.x; i < rows * kBandStride; i += 256) acc[i] = 0.f;
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float* vol = a.vol + (KIND == FWD_PLAIN ? (int64_t)cond * a.vol_cond_stride : 0);
+    float g2[kMaxBasis];
+    if (KIND == FWD_XTT) {
+#pragma unroll
+        for (int k = 0; k < kMaxBasis; k++) g2[k] = (k < d.K) ? a.g2[(int64_t)code * d.K + k] : 0.f;
+    }
+    const float zlo = -d.xc, zhi = (float)(d.N - 1) - d.xc;
+    const float band_lo = (float)u0 - 0.5f, band_hi = (float)(u0 + kBandU) + 0.5f;
+
+    for (int y = wave; y < d.N; y += 4) {
+        const float yp = (float)y - d.xc;
+        for (int x0 = 0; x0 < d.N; x0 += 64) {
+            const int x = x0 + lane;
+            bool cand = false;
+            if (x < d.N) {
+                const ColForms c = col_forms(g, (float)x - d.xc, yp);
+                if (AXIS) {
+                    const float u = fmaf(c.nu, rcpf(c.nd), d.hu);
+                    if (accept_u(u, d)) {
+                        int iu;
+                        float fu;
+                        split(u, iu, fu);
+                        cand = (iu >= u0 - 1) && (iu < u0 + kBandU);
+                    }
+                } else {
+                    // u is linear-fractional (monotone) along the column while den > 0.
+                    const float nd0 = fmaf(g.ad[2], zlo, c.nd), nd1 = fmaf(g.ad[2], zhi, c.nd);
+                    if (!(nd0 > 0.f && nd1 > 0.f)) {
+                        cand = true;
+                    } else {
+                        const float ua = fmaf(fmaf(g.au[2], zlo, c.nu), rcpf(nd0), d.hu);
+                        const float ub = fmaf(fmaf(g.au[2], zhi, c.nu), rcpf(nd1), d.hu);
+                        const float lo = fminf(ua, ub) - 0.01f, hi = fmaxf(ua, ub) + 0.01f;
+                        cand = (hi > band_lo) && (lo < band_hi) && (hi > 0.55f) && (lo < d.umax);
+                        if (ua != ua || ub != ub) cand = true;
+                    }
+                }
+            }
+            unsigned long long m = __ballot(cand);
+            while (m) {
+                const int bit = __ffsll(m) - 1;
+                m &= m - 1;
+                const int xcol = x0 + bit;
+                const float xp = (float)xcol - d.xc;
+                const ColForms c = col_forms(g, xp, yp);
+                const int64_t colbase = ((int64_t)y * d.N + xcol) * d.N;
+                float inv_c = 0.f, u_c = 0.f;
+                if (AXIS) {
+                    inv_c = rcpf(c.nd);
+                    u_c = fmaf(c.nu, inv_c, d.hu);
+                }
+                float dx = 0.f, dy = 0.f;
+                if (KIND == FWD_XTT) {
+                    dx = fmaf(d.vs, xp, g.d0[0]);
+                    dy = fmaf(d.vs, yp, g.d0[1]);
+                }
+                for (int z0 = 0; z0 < d.N; z0 += 64) {
+                    const int z = z0 + lane;
+                    if (z >= d.N) break;
+                    const float zp = (float)z - d.xc;
+                    float u, v;
+                    if (AXIS) {
+                        u = u_c;
+                        v = fmaf(fmaf(g.av[2], zp, c.nv), inv_c, d.hv);
+                    } else {
+                        voxel_uv(g, c, zp, d, u, v);
+                    }
+                    if (!(accept_u(u, d) && accept_v(v, d))) continue;
+                    int iu, iv;
+                    float fu, fv;
+                    split(u, iu, fu);
+                    split(v, iv, fv);
+                    if (!AXIS && (iu < u0 - 1 || iu >= u0 + kBandU)) continue;
+                    if (iv < v0 - 1 || iv >= v0 + vb) continue;
+                    float val;
+                    if (KIND == FWD_WEIGHT) {
+                        val = 1.f;
+                    } else if (KIND == FWD_PLAIN) {
+                        val = vol[colbase + z] * d.ratio;
+                    } else {
+                        // XTT: sum_k |B x S_k|^2 (S_k.G)^2 mu_k, times vs*ratio (ir.cu:1780-1794).
+                        const float dz = fmaf(d.vs, zp, g.d0[2]);
+                        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        const float id2 = rcpf(d2);
+                        float s = 0.f;
+#pragma unroll
+                        for (int k = 0; k < kMaxBasis; k++) {
+                            if (k < d.K) {
+                                const float ds = fmaf(dz, a.b.S[k][2], fmaf(dy, a.b.S[k][1], dx * a.b.S[k][0]));
+                                const float cr2 = fmaxf(fmaf(-ds * ds, id2, a.b.S2[k]), 0.f);
+                                s = fmaf(cr2 * g2[k], vol[colbase + z + k * a.chan_stride], s);
+                            }
+                        }
+                        val = s * (d.vs * d.ratio);
+                    }
+                    const float c1 = (1.0f - fu) * fv, c2 = fu * fv;
+                    const float c3 = fu * (1.0f - fv), c4 = (1.0f - fu) * (1.0f - fv);
+                    lds_float* p = (lds_float*)acc + (iv - v0 + 1) * kBandStride + (iu - u0 + 1);
+                    lds_add(p, c4 * val);                    // (iu,   iv)
+                    lds_add(p + 1, c3 * val);                // (iu+1, iv)
+                    lds_add(p + kBandStride, c1 * val);      // (iu,   iv+1)
+                    lds_add(p + kBandStride + 1, c2 * val);  // (iu+1, iv+1)
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    double lsum = 0.0;
+    const int64_t img = (int64_t)(code - a.code_base) * d.V * d.U;
+    const int64_t nimg = (int64_t)code * d.V * d.U;
+    const bool acc_old = (KIND != FWD_WEIGHT) && a.accum && a.accum[slot];
+    for (int i = threadIdx.x; i < vb * kBandU; i += 256) {
+        const int r = i / kBandU, cc = i % kBandU;
+        const int uu = u0 + cc, vv = v0 + r;
+        if (uu >= d.U || vv >= d.V) continue;
+        const float s = acc[(r + 1) * kBandStride + cc + 1];
+        const int64_t pix = img + (int64_t)vv * d.U + uu;
+        const int64_t npix = nimg + (int64_t)vv * d.U + uu;
+        if (a.out_mode == OUT_NORM_TABLE) {
+            const float L = (float)chord_length(a.cg[code], uu, vv, d);
+            a.proj[pix] = L / (s + 1e-5f);
+            continue;
+        }
+        float p = (acc_old ? a.proj[pix] : 0.f) + s;
+        if (a.out_mode == OUT_RAW) {
+            a.proj[pix] = p;
+            continue;
+        }
+        p = p * a.norm[npix];
+        if (a.out_mode == OUT_NORMALISED) {
+            a.proj[pix] = p;
+        } else {
+            const float yv = a.sino[pix];
+            lsum += (double)fabsf(yv - p);
+            if (a.out_mode == OUT_RESIDUAL) {
+                a.proj[pix] = yv - p;
+            } else {
+                float q = p;
+                if (q != 0.0f) q = yv / q;
+                if (q > 2.0f) q = 2.0f;
+                a.proj[pix] = q;
+            }
+        }
+    }
+    if (a.out_mode == OUT_RATIO || a.out_mode == OUT_RESIDUAL) {
+        // Cross-wave sum through LDS past the band accumulator (8-byte aligned slot).
+        double* red = reinterpret_cast<double*>(acc + (((rows * kBandStride) + 1) & ~1));
+        lsum = wave_sum(lsum);
+        if (lane == 0) red[wave] = lsum;
+        __syncthreads();
+        if (threadIdx.x == 0) atomicAdd(a.loss, red[0] + red[1] + red[2] + red[3]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Backprojection gather with fused update.  Block = 4 waves; zw waves stack
+// along z (64 lanes = 64 consecutive z each), 4/zw column groups of TCxTC
+// columns along x.  Every thread owns one z and TC*TC columns.
+// ---------------------------------------------------------------------------
+template <int KIND>
+struct IsXtt {
+    static constexpr bool value = (KIND == BWD_XTT_RAW || KIND == BWD_XTT_MLEM || KIND == BWD_XTT_ART);
+};
+
+template <int KIND, bool AXIS, int TC, int KM>
+__global__ __launch_bounds__(256) void k_backward(const BwdArgs a) {
+    constexpr bool XTT = IsXtt<KIND>::value;
+    constexpr int NC = TC * TC;
+    constexpr int KA = XTT ? KM : 1;
+    extern __shared__ float stage[];
+    const Dims& d = a.d;
+    const int o = (blockIdx.x % 8) * a.per_xcd + blockIdx.x / 8;
+    if (o >= a.n_tiles) return;
+    // zw waves stacked along z; the remaining 4/zw waves form a gx x gy grid of
+    // TC x TC column groups.
+    const int zw = a.zw, gx = a.gx;
+    const int bcx = TC * gx, bcy = TC * a.gy;  // block columns along x, y
+    const int tx = o % a.tiles_x, ty = (o / a.tiles_x) % a.tiles_y, tz = o / (a.tiles_x * a.tiles_y);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int zsub = wave % zw, cg = wave / zw;
+    const int z = tz * 64 * zw + zsub * 64 + lane;
+    const bool zvalid = z < d.N;
+    const int xb = tx * bcx + (cg % gx) * TC, yb = ty * bcy + (cg / gx) * TC;
+    // Box of the whole block (centred coordinates), for the footprint.
+    const float bx0 = (float)(tx * bcx) - d.xc, bx1 = (float)min(tx * bcx + bcx - 1, d.N - 1) - d.xc;
+    const float by0 = (float)(ty * bcy) - d.xc, by1 = (float)min(ty * bcy + bcy - 1, d.N - 1) - d.xc;
+    const float bz0 = (float)(tz * 64 * zw) - d.xc, bz1 = (float)min(tz * 64 * zw + 64 * zw - 1, d.N - 1) - d.xc;
+    const float zp = (float)z - d.xc;
+
+    float tmp[NC][KA], fac[NC][KA];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int k = 0; k < KA; k++) {
+            tmp[c][k] = 0.f;
+            fac[c][k] = 0.f;
+        }
+
+    for (int s = 0; s < a.n_views; s++) {
+        const int code = a.views[s];
+        const ViewGeom g = a.vg[code];
+        const float* img = a.proj + (int64_t)(code - a.code_base) * d.V * d.U;
+        // Footprint of the block box: the projective map's extremes over a box are
+        // at its corners while den > 0 there.
+        const int ci = lane & 7;
+        float cu, cv, cd;
+        {
+            const ColForms c = col_forms(g, (ci & 1) ? bx1 : bx0, (ci & 2) ? by1 : by0);
+            const float czp = (ci & 4) ? bz1 : bz0;
+            cd = fmaf(g.ad[2], czp, c.nd);
+            voxel_uv(g, c, czp, d, cu, cv);
+        }
+        float umn = cu, umx = cu, vmn = cv, vmx = cv, dmn = cd;
+        bool bad = !(cd > 0.f) || !(fabsf(cu) < 1e7f) || !(fabsf(cv) < 1e7f);
+        for (int off = 1; off < 8; off <<= 1) {
+            umn = fminf(umn, __shfl_xor(umn, off));
+            umx = fmaxf(umx, __shfl_xor(umx, off));
+            vmn = fminf(vmn, __shfl_xor(vmn, off));
+            vmx = fmaxf(vmx, __shfl_xor(vmx, off));
+            dmn = fminf(dmn, __shfl_xor(dmn, off));
+            bad = bad || (__shfl_xor((int)bad, off) != 0);
+        }
+        int iu_lo = 0, iv_lo = 0, W = 0, H = 0, Ws = 0;
+        bool staged = false;
+        if (!bad) {
+            iu_lo = max((int)floorf(umn - 0.5f) - 1, 0);
+            const int iu_hi = min((int)floorf(umx - 0.5f) + 2, d.U - 1);
+            iv_lo = max((int)floorf(vmn - 0.5f) - 1, 0);
+            const int iv_hi = min((int)floorf(vmx - 0.5f) + 2, d.V - 1);
+            W = iu_hi - iu_lo + 1;
+            H = iv_hi - iv_lo + 1;
+            if (W <= 1 || H <= 1) continue;  // footprint entirely off the detector: nothing accepted
+            Ws = W | 1;
+            staged = (Ws * H <= kStageCap);
+        }
+        if (staged) {
+            __syncthreads();
+            const int c0 = threadIdx.x & 15, r0 = threadIdx.x >> 4;
+            for (int r = r0; r < H; r += 16) {
+                const float* src = img + (int64_t)(iv_lo + r) * d.U + iu_lo;
+                for (int c = c0; c < W; c += 16) stage[r * Ws + c] = src[c];
+            }
+            __syncthreads();
+        }
+        float g2[KA];
+        if (XTT) {
+#pragma unroll
+            for (int k = 0; k < KA; k++) g2[k] = (k < d.K) ? a.g2[(int64_t)code * d.K + k] : 0.f;
+        }
+        if (!zvalid) continue;
+#pragma unroll
+        for (int j = 0; j < TC; j++) {
+#pragma unroll
+            for (int i = 0; i < TC; i++) {
+                const int x = xb + i, y = yb + j;
+                if (x >= d.N || y >= d.N) continue;
+                const int cidx = j * TC + i;
+                const float xp = (float)x - d.xc, yp = (float)y - d.xc;
+                const ColForms c = col_forms(g, xp, yp);
+                float u, v;
+                if (AXIS) {
+                    const float inv = rcpf(c.nd);
+                    u = fmaf(c.nu, inv, d.hu);
+                    v = fmaf(fmaf(g.av[2], zp, c.nv), inv, d.hv);
+                } else {
+                    voxel_uv(g, c, zp, d, u, v);
+                }
+                if (!(accept_u(u, d) && accept_v(v, d))) continue;
+                int iu, iv;
+                float fu, fv;
+                split(u, iu, fu);
+                split(v, iv, fv);
+                float r00, r01, r10, r11;  // (iu,iv) (iu+1,iv) (iu,iv+1) (iu+1,iv+1)
+                if (staged) {
+                    const int lu = min(max(iu - iu_lo, 0), W - 2);
+                    const int lv = min(max(iv - iv_lo, 0), H - 2);
+                    const int q = lv * Ws + lu;
+                    r00 = stage[q];
+                    r01 = stage[q + 1];
+                    r10 = stage[q + Ws];
+                    r11 = stage[q + Ws + 1];
+                } else {
+                    // Footprint too large for LDS (tilted scans): read the image directly.
+                    // These loads must not be merged with the LDS reads above into one
+                    // flat access (gfx950 backend rejects that form), hence the distinct
+                    // nontemporal load.
+                    const int64_t q = (int64_t)iv * d.U + iu;
+                    r00 = __builtin_nontemporal_load(img + q);
+                    r01 = __builtin_nontemporal_load(img + q + 1);
+                    r10 = __builtin_nontemporal_load(img + q + d.U);
+                    r11 = __builtin_nontemporal_load(img + q + d.U + 1);
+                }
+                const float c1 = (1.0f - fu) * fv, c2 = fu * fv;
+                const float c3 = fu * (1.0f - fv), c4 = (1.0f - fu) * (1.0f - fv);
+                const float nb = c1 * r10 + c2 * r11 + c3 * r01 + c4 * r00;
+                if (KIND == BWD_FDK) {
+                    // vol += Ud^2 * (2 pi/P) * sum c p, Ud = sod/(sod + y_unrotated) (fdk.cu:160-169)
+                    const float Xy = fmaf(d.vs, yp, g.fy);
+                    const float Ud = d.sod / (d.sod + Xy);
+                    tmp[cidx][0] = fmaf(Ud * Ud * d.cfdk, nb, tmp[cidx][0]);
+                } else if (!XTT) {
+                    tmp[cidx][0] += nb;
+                    fac[cidx][0] += 1.0f;
+                } else {
+                    const float dx = fmaf(d.vs, xp, g.d0[0]), dy = fmaf(d.vs, yp, g.d0[1]);
+                    const float dz = fmaf(d.vs, zp, g.d0[2]);
+                    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float id2 = rcpf(d2);
+#pragma unroll
+                    for (int k = 0; k < KA; k++) {
+                        if (k < d.K) {
+                            const float ds = fmaf(dz, a.b.S[k][2], fmaf(dy, a.b.S[k][1], dx * a.b.S[k][0]));
+                            const float w = fmaxf(fmaf(-ds * ds, id2, a.b.S2[k]), 0.f) * g2[k];
+                            tmp[cidx][k] = fmaf(w, nb, tmp[cidx][k]);
+                            fac[cidx][k] += w;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (!zvalid) return;
+#pragma unroll
+    for (int j = 0; j < TC; j++) {
+#pragma unroll
+        for (int i = 0; i < TC; i++) {
+            const int x = xb + i, y = yb + j;
+            if (x >= d.N || y >= d.N) continue;
+            const int cidx = j * TC + i;
+            const int64_t idx = ((int64_t)y * d.N + x) * d.N + z;
+#pragma unroll
+            for (int k = 0; k < KA; k++) {
+                if (XTT && k >= d.K) continue;
+                const int64_t vi = idx + (int64_t)k * a.chan_stride;
+                const float t = tmp[cidx][k], f = fac[cidx][k];
+                if (KIND == BWD_RAW || KIND == BWD_XTT_RAW) {
+                    a.tmp[vi] += t;
+                    a.factor[vi] += f;
+                } else if (KIND == BWD_MLEM || KIND == BWD_XTT_MLEM) {
+                    const float mu = a.vol[vi];
+                    a.vol[vi] = (f == 0.0f) ? 0.0f : mu * t / f;
+                } else if (KIND == BWD_ART || KIND == BWD_XTT_ART) {
+                    a.vol[vi] = a.vol[vi] + a.alpha * t;
+                } else {  // BWD_FDK
+                    a.vol[vi] = a.overwrite ? t : a.vol[vi] + t;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FDK row filter (projConv, fdk.cu:28-43): dst[u] = sum_j h[|u-j|] src[j] w[j],
+// register-blocked Toeplitz product, R consecutive outputs per thread.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(1024) void k_fdk_filter(const float* __restrict__ src, float* __restrict__ dst,
+                                                    const float* __restrict__ weight, const float* __restrict__ taps,
+                                                    int U, int V, int Upad, const int32_t* views,
+                                                    int code_base) {
+    extern __shared__ float sm[];
+    float* s = sm;         // Upad weighted samples (zero padded)
+    float* hx = sm + Upad;  // 2*Upad taps: hx[m + Upad] = h[|m|]
+    const int v = blockIdx.x;
+    const int code = views[blockIdx.y];
+    const int64_t row = ((int64_t)(code - code_base) * V + v) * U;
+    for (int j = threadIdx.x; j < Upad; j += blockDim.x) s[j] = (j < U) ? src[row + j] * weight[(int64_t)v * U + j] : 0.f;
+    for (int m = threadIdx.x; m < 2 * Upad; m += blockDim.x) {
+        const int am = abs(m - Upad);
+        hx[m] = (am < U) ? taps[am] : 0.f;
+    }
+    __syncthreads();
+    const int u0 = threadIdx.x * R;
+    if (u0 >= Upad) return;
+    float acc[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) acc[i] = 0.f;
+    for (int j0 = 0; j0 < Upad; j0 += R) {
+        float sj[R], hw[2 * R - 1];
+#pragma unroll
+        for (int r = 0; r < R; r++) sj[r] = s[j0 + r];
+        const int base = u0 - j0 - (R - 1) + Upad;
+#pragma unroll
+        for (int q = 0; q < 2 * R - 1; q++) hw[q] = hx[base + q];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int i = 0; i < R; i++) acc[i] = fmaf(hw[i - r + R - 1], sj[r], acc[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++)
+        if (u0 + i < U) dst[row + u0 + i] = acc[i];
+}
+
+// calcWeight (fdk.cu:13-26).
+__global__ void k_fdk_weight(float* w, int U, int V, float sdd, float det) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x, v = blockIdx.y;
+    if (u >= U) return;
+    const float ur = ((float)u + 0.5f - (float)U / 2.0f) * det;
+    const float vr = ((float)v + 0.5f - (float)V / 2.0f) * det;
+    w[(int64_t)v * U + u] = sdd / sqrtf(sdd * sdd + ur * ur + vr * vr);
+}
+
+// Reference layout (x + N*y + N^2*z) <-> column layout (z + N*(x + N*y)):
+// per y it is a transpose of the (z, x) plane; 32x32 tiles through LDS.
+template <bool TO_INTERNAL>
+__global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in, float* __restrict__ out, int N) {
+    __shared__ float t[32][33];
+    const int64_t N2 = (int64_t)N * N, N3 = N2 * N;
+    const int y = blockIdx.z % N, ch = blockIdx.z / N;
+    const float* src = in + ch * N3;
+    float* dst = out + ch * N3;
+    const int a0 = blockIdx.x * 32, b0 = blockIdx.y * 32;  // TO_INTERNAL: a = x, b = z
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int a = a0 + threadIdx.x, b = b0 + r;
+        if (a < N && b < N) {
+            // TO_INTERNAL: read ref[x=a, y, z=b]; else read internal[z=a, x=b, y]
+            t[r][threadIdx.x] = TO_INTERNAL ? src[a + (int64_t)N * y + N2 * b] : src[a + (int64_t)N * b + N2 * y];
+        }
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int b = b0 + threadIdx.x, a = a0 + r;
+        if (a < N && b < N) {
+            if (TO_INTERNAL)
+                dst[b + (int64_t)N * a + N2 * y] = t[threadIdx.x][r];  // internal[z=b, x=a, y]
+            else
+                dst[b + (int64_t)N * y + N2 * a] = t[threadIdx.x][r];  // ref[x=b, y, z=a]
+        }
+    }
+}
+
+__global__ void k_sqrt_abs(float* v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float m = v[i];
+    v[i] = (m < 0.0f) ? sqrtf(-m) : sqrtf(m);  // voxelSqrt (ir.cu:1715)
+}
+
+__global__ void k_fill(float* p, int64_t n, float value) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+// value < floor -> replacement (main.cu:35 clamps the sinogram: <0 -> 1e-8).
+__global__ void k_clamp_min(float* p, int64_t n, float floor_value, float replacement) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && p[i] < floor_value) p[i] = replacement;
+}
+
+// Per-voxel principal orientation (calcEigenVector, pca.cu:12-65):
+// C = (1/K) sum_k (mu_k S_k)(mu_k S_k)^T, eigenvector of the smallest
+// eigenvalue, sign so that z >= 0, md = sign * mean(mu) * e_min, eigenvalues
+// normalised to sum 1.  Closed-form (trigonometric) eigenvalues in FP64 and the
+// eigenvector from cross products of the rows of C - lambda_min I.
+__global__ void k_pca(const float* __restrict__ mu, int K, const Basis b, int64_t nvox, int64_t cs,
+                      float* __restrict__ md, float* __restrict__ ev) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nvox) return;
+    double a00 = 0, a01 = 0, a02 = 0, a11 = 0, a12 = 0, a22 = 0, mean = 0;
+    for (int k = 0; k < K; k++) {
+        const double m = mu[i + k * cs];
+        mean += m;
+        const double s0 = m * b.S[k][0], s1 = m * b.S[k][1], s2 = m * b.S[k][2];
+        a00 += s0 * s0;
+        a01 += s0 * s1;
+        a02 += s0 * s2;
+        a11 += s1 * s1;
+        a12 += s1 * s2;
+        a22 += s2 * s2;
+    }
+    const double invK = 1.0 / K;
+    mean *= invK;
+    a00 *= invK;
+    a01 *= invK;
+    a02 *= invK;
+    a11 *= invK;
+    a12 *= invK;
+    a22 *= invK;
+    double l0, l1, l2;  // ascending
+    double e0 = 1, e1 = 0, e2 = 0;
+    const double p1 = a01 * a01 + a02 * a02 + a12 * a12;
+    const double q = (a00 + a11 + a22) / 3.0;
+    const double p2 = (a00 - q) * (a00 - q) + (a11 - q) * (a11 - q) + (a22 - q) * (a22 - q) + 2.0 * p1;
+    const double scale = fmax(fabs(a00), fmax(fabs(a11), fabs(a22))) + sqrt(p1);
+    if (p1 <= 1e-30 * scale * scale) {
+        // diagonal: sort the diagonal ascending, eigenvectors are the axes
+        double dv[3] = {a00, a11, a22};
+        int id[3] = {0, 1, 2};
+        for (int s = 0; s < 2; s++)
+            for (int t = 0; t < 2 - s; t++)
+                if (dv[id[t + 1]] < dv[id[t]]) {
+                    const int tt = id[t];
+                    id[t] = id[t + 1];
+                    id[t + 1] = tt;
+                }
+        l0 = dv[id[0]];
+        l1 = dv[id[1]];
+        l2 = dv[id[2]];
+        e0 = id[0] == 0;
+        e1 = id[0] == 1;
+        e2 = id[0] == 2;
+    } else {
+        const double p = sqrt(p2 / 6.0);
+        const double ip = 1.0 / p;
+        const double b00 = (a00 - q) * ip, b11 = (a11 - q) * ip, b22 = (a22 - q) * ip;
+        const double b01 = a01 * ip, b02 = a02 * ip, b12 = a12 * ip;
+        double r = 0.5 * (b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b12 * b02) + b02 * (b01 * b12 - b11 * b02));
+        r = fmin(1.0, fmax(-1.0, r));
+        const double phi = acos(r) / 3.0;
+        l2 = q + 2.0 * p * cos(phi);
+        l0 = q + 2.0 * p * cos(phi + 2.0943951023931954923);
+        l1 = 3.0 * q - l0 - l2;
+        // rows of C - l0 I; the eigenvector is orthogonal to all of them
+        const double r0[3] = {a00 - l0, a01, a02}, r1[3] = {a01, a11 - l0, a12}, r2[3] = {a02, a12, a22 - l0};
+        const double c01[3] = {r0[1] * r1[2] - r0[2] * r1[1], r0[2] * r1[0] - r0[0] * r1[2], r0[0] * r1[1] - r0[1] * r1[0]};
+        const double c02[3] = {r0[1] * r2[2] - r0[2] * r2[1], r0[2] * r2[0] - r0[0] * r2[2], r0[0] * r2[1] - r0[1] * r2[0]};
+        const double c12[3] = {r1[1] * r2[2] - r1[2] * r2[1], r1[2] * r2[0] - r1[0] * r2[2], r1[0] * r2[1] - r1[1] * r2[0]};
+        const double n01 = c01[0] * c01[0] + c01[1] * c01[1] + c01[2] * c01[2];
+        const double n02 = c02[0] * c02[0] + c02[1] * c02[1] + c02[2] * c02[2];
+        const double n12 = c12[0] * c12[0] + c12[1] * c12[1] + c12[2] * c12[2];
+        const double* best = c01;
+        double nb = n01;
+        if (n02 > nb) {
+            best = c02;
+            nb = n02;
+        }
+        if (n12 > nb) {
+            best = c12;
+            nb = n12;
+        }
+        if (nb > 1e-30 * scale * scale * scale * scale) {
+            const double in = 1.0 / sqrt(nb);
+            e0 = best[0] * in;
+            e1 = best[1] * in;
+            e2 = best[2] * in;
+        } else {
+            // lambda_min is (at least) double: any unit vector orthogonal to the
+            // largest row of C - l0 I spans the eigenspace.
+            const double n0 = r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2];
+            const double n1 = r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2];
+            const double n2 = r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2];
+            const double* rr = r0;
+            if (n1 > n0 && n1 >= n2) rr = r1;
+            if (n2 > n0 && n2 > n1) rr = r2;
+            // cross with the axis least aligned with rr
+            const double ax = fabs(rr[0]), ay = fabs(rr[1]), az = fabs(rr[2]);
+            double t[3] = {0, 0, 0};
+            if (ax <= ay && ax <= az)
+                t[0] = 1;
+            else if (ay <= az)
+                t[1] = 1;
+            else
+                t[2] = 1;
+            double w0 = rr[1] * t[2] - rr[2] * t[1], w1 = rr[2] * t[0] - rr[0] * t[2], w2 = rr[0] * t[1] - rr[1] * t[0];
+            const double wn = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+            if (wn > 0) {
+                e0 = w0 / wn;
+                e1 = w1 / wn;
+                e2 = w2 / wn;
+            }
+        }
+    }
+    const double sg = (e2 >= 0) ? 1.0 : -1.0;
+    md[i] = (float)(sg * mean * e0);
+    md[i + cs] = (float)(sg * mean * e1);
+    md[i + 2 * cs] = (float)(sg * mean * e2);
+    const float f0 = (float)l0, f1 = (float)l1, f2 = (float)l2;
+    const float tot = f0 + f1 + f2;
+    ev[i] = f0 / tot;
+    ev[i + cs] = f1 / tot;
+    ev[i + 2 * cs] = f2 / tot;
+}
+
+// Ellipsoid phantom in the reference layout, voxel x at (x-(N+1)/2)*vs.
+// params: 16 doubles per ellipsoid (centre[3], semi-axes[3], rotation[9], value).
+__global__ void k_ellipsoids(float* vol, int N, double vs, const double* __restrict__ prm, int count) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= N) return;
+    const double p0 = ((double)x - (N + 1) * 0.5) * vs, p1 = ((double)y - (N + 1) * 0.5) * vs,
+                 p2 = ((double)z - (N + 1) * 0.5) * vs;
+    double acc = 0;
+    for (int e = 0; e < count; e++) {
+        const double* q = prm + 16 * e;
+        const double d0 = p0 - q[0], d1 = p1 - q[1], d2 = p2 - q[2];
+        double r2 = 0;
+        for (int i = 0; i < 3; i++) {
+            const double bb = q[6 + 3 * i] * d0 + q[7 + 3 * i] * d1 + q[8 + 3 * i] * d2;
+            r2 += (bb / q[3 + i]) * (bb / q[3 + i]);
+        }
+        if (r2 <= 1.0) acc += q[15];
+    }
+    vol[x + (int64_t)N * y + (int64_t)N * N * z] = (float)acc;
+}
+
+inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+hipError_t launch_forward(const FwdArgs& a, int kind, bool axis, hipStream_t stream) {
+    const int nblk = a.n_bands * a.n_slots;
+    if (nblk == 0) return hipSuccess;
+    const unsigned grid = (unsigned)(8 * ((nblk + 7) / 8));
+    const size_t lds = ((size_t)(((a.vb + 2) * kBandStride + 1) & ~1) + 8) * sizeof(float);
+#define XR_FWD(KD, A) hipLaunchKernelGGL((k_forward<KD, A>), dim3(grid), dim3(256), lds, stream, a)
+    if (kind == FWD_WEIGHT) {
+        if (axis) XR_FWD(FWD_WEIGHT, true); else XR_FWD(FWD_WEIGHT, false);
+    } else if (kind == FWD_PLAIN) {
+        if (axis) XR_FWD(FWD_PLAIN, true); else XR_FWD(FWD_PLAIN, false);
+    } else {
+        if (axis) XR_FWD(FWD_XTT, true); else XR_FWD(FWD_XTT, false);
+    }
+#undef XR_FWD
+    return hipGetLastError();
+}
+
+hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream) {
+    const int N = a.d.N;
+    const bool xtt = (kind == BWD_XTT_RAW || kind == BWD_XTT_MLEM || kind == BWD_XTT_ART);
+    // Plain/FDK: 4x4 columns per thread, waves stacked along z (256 z per block
+    // when N > 128).  XTT (2K accumulators per voxel): one column per thread,
+    // waves as a 2x2 column tile, 64 z per block.
+    const int TC = xtt ? 1 : 4;
+    a.zw = xtt ? 1 : ((N > 128) ? 4 : ((N > 64) ? 2 : 1));
+    a.gx = xtt ? 2 : 4 / a.zw;
+    a.gy = xtt ? 2 : 1;
+    a.tiles_x = (N + TC * a.gx - 1) / (TC * a.gx);
+    a.tiles_y = (N + TC * a.gy - 1) / (TC * a.gy);
+    a.tiles_z = (N + 64 * a.zw - 1) / (64 * a.zw);
+    a.n_tiles = a.tiles_x * a.tiles_y * a.tiles_z;
+    a.per_xcd = (a.n_tiles + 7) / 8;
+    const unsigned grid = (unsigned)(8 * a.per_xcd);
+#define XR_BWD(KD, A, T, KM) hipLaunchKernelGGL((k_backward<KD, A, T, KM>), dim3(grid), dim3(256), kStageCap * sizeof(float), stream, a)
+#define XR_BWD_AX(KD, T, KM)         \
+    do {                              \
+        if (axis)                     \
+            XR_BWD(KD, true, T, KM);  \
+        else                          \
+            XR_BWD(KD, false, T, KM); \
+    } while (0)
+#define XR_BWD_XTT(KD)                             \
+    do {                                           \
+        if (a.d.K <= 4)                            \
+            XR_BWD_AX(KD, 1, 4);                   \
+        else if (a.d.K <= 8)                       \
+            XR_BWD_AX(KD, 1, 8);                   \
+        else if (a.d.K <= 13)                      \
+            XR_BWD_AX(KD, 1, 13);                  \
+        else                                       \
+            XR_BWD_AX(KD, 1, 16);                  \
+    } while (0)
+    switch (kind) {
+        case BWD_RAW: XR_BWD_AX(BWD_RAW, 4, 1); break;
+        case BWD_MLEM: XR_BWD_AX(BWD_MLEM, 4, 1); break;
+        case BWD_ART: XR_BWD_AX(BWD_ART, 4, 1); break;
+        case BWD_FDK: XR_BWD_AX(BWD_FDK, 4, 1); break;
+        case BWD_XTT_RAW: XR_BWD_XTT(BWD_XTT_RAW); break;
+        case BWD_XTT_MLEM: XR_BWD_XTT(BWD_XTT_MLEM); break;
+        case BWD_XTT_ART: XR_BWD_XTT(BWD_XTT_ART); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef XR_BWD_XTT
+#undef XR_BWD_AX
+#undef XR_BWD
+    return hipGetLastError();
+}
+
+hipError_t launch_fdk_filter(const float* src, float* dst, const float* weight, const float* taps, int U, int V,
+                             const int32_t* views, int n_views, int code_base, hipStream_t stream) {
+    constexpr int R = 8;
+    const int Upad = ((U + R - 1) / R) * R;
+    int threads = ((Upad / R + 63) / 64) * 64;
+    if (threads > 1024 || threads < 64) return hipErrorInvalidValue;
+    const size_t lds = (size_t)3 * Upad * sizeof(float);
+    hipLaunchKernelGGL((k_fdk_filter<R>), dim3(V, n_views), dim3(threads), lds, stream, src, dst, weight, taps, U, V,
+                       Upad, views, code_base);
+    return hipGetLastError();
+}
+
+hipError_t launch_fdk_weight(float* weight, int U, int V, float sdd, float det, hipStream_t stream) {
+    hipLaunchKernelGGL(k_fdk_weight, dim3(grid1(U, 256), V), dim3(256), 0, stream, weight, U, V, sdd, det);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_internal(const float* ref, float* internal, int N, int channels, hipStream_t stream) {
+    hipLaunchKernelGGL((k_transpose<true>), dim3(grid1(N, 32), grid1(N, 32), (unsigned)N * channels), dim3(32, 8), 0,
+                       stream, ref, internal, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_from_internal(const float* internal, float* ref, int N, int channels, hipStream_t stream) {
+    hipLaunchKernelGGL((k_transpose<false>), dim3(grid1(N, 32), grid1(N, 32), (unsigned)N * channels), dim3(32, 8), 0,
+                       stream, internal, ref, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_sqrt_abs(float* vol, int64_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sqrt_abs, dim3(grid1(n, 256)), dim3(256), 0, stream, vol, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pca(const float* mu, int K, const Basis& b, int64_t nvox, int64_t chan_stride, float* md,
+                      float* evalue, hipStream_t stream) {
+    if (nvox == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pca, dim3(grid1(nvox, 256)), dim3(256), 0, stream, mu, K, b, nvox, chan_stride, md, evalue);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(float* p, int64_t n, float value, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3(grid1(n, 256)), dim3(256), 0, stream, p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_clamp_min(float* p, int64_t n, float floor_value, float replacement, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clamp_min, dim3(grid1(n, 256)), dim3(256), 0, stream, p, n, floor_value, replacement);
+    return hipGetLastError();
+}
+
+hipError_t launch_ellipsoids(float* vol, int N, float vs, const double* params_dev, int count, hipStream_t stream) {
+    hipLaunchKernelGGL(k_ellipsoids, dim3(grid1(N, 64), N, N), dim3(64), 0, stream, vol, N, (double)vs, params_dev,
+                       count);
+    return hipGetLastError();
+}
+
+}  // namespace xr

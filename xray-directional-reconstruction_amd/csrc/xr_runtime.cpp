@@ -1,0 +1,1190 @@
+// Host runtime behind the C ABI (include/xr.h): contexts, plans (per-view
+// projection tables in FP64), device-level operators and the IR / XTT / FDK
+// solvers that mirror reconstruct/src/reconstruct.cu.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/xr.h"
+#include "xr_internal.hpp"
+
+using namespace xr;
+
+struct xr_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int32_t* d_codes = nullptr;  // per-call view lists
+    size_t codes_cap = 0;
+    uint8_t* d_flags = nullptr;  // per-call accumulate flags
+    size_t flags_cap = 0;
+    double* d_loss = nullptr;
+    bool timing = false;
+    struct Ev {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> events;
+    std::vector<hipEvent_t> pool;
+};
+
+struct xr_plan {
+    xr_context* ctx = nullptr;
+    xr_geometry g{};
+    int n_cond = 0, n_basis = 0, conv = 0, dir = 0;
+    std::vector<float> cond_R, cond_t, cond_offset, basis, grating;
+    std::vector<char> axis_cond;  // 1: every view of the condition has au[2] = ad[2] = 0
+    ViewGeom* d_vg = nullptr;
+    ChordGeom* d_cg = nullptr;
+    float* d_g2 = nullptr;    // [n_cond*P][K]: (S_k.G)^2
+    float* d_norm = nullptr;  // [n_cond][P][V][U]: L/(factor+1e-5)
+    Dims dims{};
+    Basis bas{};
+};
+
+namespace {
+
+int fail(xr_context* ctx, int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+#define XR_HIP(ctx, call)                                                                                      \
+    do {                                                                                                       \
+        hipError_t e_ = (call);                                                                                \
+        if (e_ != hipSuccess) return fail((ctx), XR_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define XR_TRY(expr)          \
+    do {                      \
+        int r_ = (expr);      \
+        if (r_ != XR_OK) return r_; \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t count) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void**)&p, count * sizeof(T));
+    }
+};
+
+float theta_f32(int n, int P) {
+    // ir.cu:1860: 2.0f * (float)M_PI * (float)n / (float)P, each step in FP32.
+    volatile float a = 2.0f * (float)M_PI;
+    volatile float b = a * (float)n;
+    volatile float c = b / (float)P;
+    return c;
+}
+
+void mat3mul(const double* a, const double* b, double* o) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) o[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+}
+void mat3vec(const double* m, const double* v, double* o) {
+    for (int i = 0; i < 3; i++) o[i] = m[3 * i] * v[0] + m[3 * i + 1] * v[1] + m[3 * i + 2] * v[2];
+}
+double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Per-view projective tables (see ViewGeom).  Derivation from rayCasting
+// (ir.cu:1852-1902): p = s + coeff*(X - s), coeff = -(s.s)/(s.(X - s)) and
+// s.eu = s.ev = 0 give u = U/2 + (sdd/det) (X.eu) / (sod - shat.X), with
+// X = vs*x' - (R*offset + t) and shat = +-R*y (IR +, FDK -).  Numerator and
+// denominator are both divided by sod so that den ~ 1.
+void build_view(const xr_plan& p, int cond, int view, ViewGeom& vg, ChordGeom& cg) {
+    const xr_geometry& g = p.g;
+    const int n = (p.dir == XR_CW ? 1 : -1) * view;
+    const double th = (double)theta_f32(n, g.n_proj);
+    const double c = std::cos(th), s = std::sin(th);
+    const double Rz[9] = {c, -s, 0, s, c, 0, 0, 0, 1};
+    double cR[9], R[9];
+    for (int i = 0; i < 9; i++) cR[i] = (double)p.cond_R[9 * cond + i];
+    mat3mul(cR, Rz, R);
+    const double off0[3] = {p.cond_offset[3 * cond + 0], p.cond_offset[3 * cond + 1], p.cond_offset[3 * cond + 2]};
+    double off[3];
+    mat3vec(R, off0, off);
+    const double t[3] = {p.cond_t[3 * cond + 0], p.cond_t[3 * cond + 1], p.cond_t[3 * cond + 2]};
+    const double eu[3] = {R[0], R[3], R[6]}, ey[3] = {R[1], R[4], R[7]}, ev[3] = {R[2], R[5], R[8]};
+    const double sgn = (p.conv == XR_CONV_IR) ? 1.0 : -1.0;
+    const double sh[3] = {sgn * ey[0], sgn * ey[1], sgn * ey[2]};
+    const double Xc[3] = {-(off[0] + t[0]), -(off[1] + t[1]), -(off[2] + t[2])};
+    const double sod = g.sod, vs = g.vox_size;
+    const double k = (double)g.sdd / (double)g.det_size / sod;
+    for (int i = 0; i < 3; i++) {
+        vg.au[i] = (float)(k * vs * eu[i]);
+        vg.av[i] = (float)(k * vs * ev[i]);
+        vg.ad[i] = (float)(-vs / sod * sh[i]);
+    }
+    vg.au[3] = (float)(k * dot3(eu, Xc));
+    vg.av[3] = (float)(k * dot3(ev, Xc));
+    vg.ad[3] = (float)(1.0 - dot3(sh, Xc) / sod);
+    for (int i = 0; i < 3; i++) vg.d0[i] = (float)(Xc[i] - sod * sh[i]);
+    vg.fy = (float)Xc[1];
+    double g0[3] = {0, 0, 1};
+    if (!p.grating.empty())
+        for (int i = 0; i < 3; i++) g0[i] = p.grating[3 * cond + i];
+    double G[3];
+    mat3vec(R, g0, G);
+    for (int i = 0; i < 3; i++) vg.G[i] = (float)G[i];
+    vg.axis = (vg.au[2] == 0.0f && vg.ad[2] == 0.0f) ? 1.0f : 0.0f;
+    for (int i = 0; i < 4; i++) vg.pad[i] = 0.0f;
+    // chord geometry of the IR source side (ir.cu:1556-1574)
+    cg.det[0] = g.det_size;
+    cg.det[1] = (double)g.sdd - (double)g.sod;
+    const double half = ((double)(g.n_vox + 1) / 2.0) * vs;
+    for (int i = 0; i < 3; i++) {
+        cg.src[i] = sod * ey[i];
+        cg.eu[i] = eu[i];
+        cg.ev[i] = ev[i];
+        cg.ey[i] = ey[i];
+        cg.lo[i] = -half - off[i] - t[i];
+        cg.hi[i] = half - off[i] - t[i];
+    }
+}
+
+int check_geometry(xr_context* ctx, const xr_geometry* g) {
+    if (!g) return fail(ctx, XR_ERR_INVALID, "geometry is NULL");
+    if (g->n_vox <= 0 || g->n_det_u <= 1 || g->n_det_v <= 1 || g->n_proj <= 0)
+        return fail(ctx, XR_ERR_INVALID, "invalid sizes N=%d U=%d V=%d P=%d", g->n_vox, g->n_det_u, g->n_det_v,
+                    g->n_proj);
+    if (g->n_vox > 4096 || g->n_det_u > 16384 || g->n_det_v > 16384)
+        return fail(ctx, XR_ERR_INVALID, "sizes beyond the supported range");
+    if (!(g->sod > 0) || !(g->sdd > 0) || !(g->det_size > 0) || !(g->vox_size > 0))
+        return fail(ctx, XR_ERR_INVALID, "distances and pitches must be positive");
+    return XR_OK;
+}
+
+int check_scan(xr_context* ctx, const xr_scan* s) {
+    if (!s) return fail(ctx, XR_ERR_INVALID, "scan is NULL");
+    if (s->n_cond <= 0) return fail(ctx, XR_ERR_INVALID, "n_cond must be positive");
+    if (!s->cond_R || !s->cond_t || !s->cond_offset) return fail(ctx, XR_ERR_INVALID, "scan tables missing");
+    if (s->n_basis < 0 || s->n_basis > kMaxBasis)
+        return fail(ctx, XR_ERR_INVALID, "n_basis must be in [0, %d]", kMaxBasis);
+    if (s->n_basis > 0 && !s->basis) return fail(ctx, XR_ERR_INVALID, "basis vectors missing");
+    return XR_OK;
+}
+
+int ensure_codes(xr_context* ctx, size_t n) {
+    if (n <= ctx->codes_cap) return XR_OK;
+    if (ctx->d_codes) (void)hipFree(ctx->d_codes);
+    ctx->d_codes = nullptr;
+    XR_HIP(ctx, hipMalloc((void**)&ctx->d_codes, n * sizeof(int32_t)));
+    ctx->codes_cap = n;
+    return XR_OK;
+}
+int ensure_flags(xr_context* ctx, size_t n) {
+    if (n <= ctx->flags_cap) return XR_OK;
+    if (ctx->d_flags) (void)hipFree(ctx->d_flags);
+    ctx->d_flags = nullptr;
+    XR_HIP(ctx, hipMalloc((void**)&ctx->d_flags, n));
+    ctx->flags_cap = n;
+    return XR_OK;
+}
+
+hipEvent_t get_event(xr_context* ctx) {
+    if (!ctx->pool.empty()) {
+        hipEvent_t e = ctx->pool.back();
+        ctx->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Launch wrapper: brackets the launch with events when timing is enabled.
+template <class F>
+int timed(xr_context* ctx, int kind, F&& launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->timing) {
+        a = get_event(ctx);
+        b = get_event(ctx);
+        if (a && b) XR_HIP(ctx, hipEventRecord(a, ctx->stream));
+    }
+    const hipError_t e = launch();
+    if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    if (ctx->timing && a && b) {
+        XR_HIP(ctx, hipEventRecord(b, ctx->stream));
+        ctx->events.push_back({kind, a, b});
+    }
+    return XR_OK;
+}
+
+int ensure_norm(xr_plan* p);
+
+// vol_cond_stride: plain volumes of several conditions side by side (N^3), or 0
+// for a single volume; code_base: proj/sino hold the images of codes >= code_base.
+int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stride, float* proj, const float* sino,
+                   int code_base, const int32_t* d_codes, const uint8_t* d_flags, int n_slots, int out_mode,
+                   double* d_loss) {
+    xr_context* ctx = p->ctx;
+    FwdArgs a{};
+    a.d = p->dims;
+    a.b = p->bas;
+    a.vg = p->d_vg;
+    a.cg = p->d_cg;
+    a.views = d_codes;
+    a.accum = d_flags;
+    a.vol = vol;
+    const int64_t N3 = (int64_t)p->g.n_vox * p->g.n_vox * p->g.n_vox;
+    a.vol_cond_stride = vol_cond_stride;
+    a.chan_stride = N3;
+    a.proj = proj;
+    a.sino = sino;
+    a.norm = p->d_norm;
+    a.code_base = code_base;
+    a.g2 = p->d_g2;
+    a.loss = d_loss;
+    a.out_mode = out_mode;
+    a.n_bands = (p->g.n_det_u + kBandU - 1) / kBandU;
+    a.n_slots = n_slots;
+    // All conditions in one launch share the axis specialisation only if all are axis-aligned.
+    bool axis = true;
+    for (int c = 0; c < p->n_cond; c++) axis = axis && p->axis_cond[c];
+    // Detector rows per launch bounded so the band accumulator (kBandStride floats
+    // per row, two guard rows) stays within 64 KB of LDS; taller detectors are
+    // processed in row chunks (each chunk re-walks the voxels it needs).
+    const int max_rows = (64 * 1024 / (int)sizeof(float) - 16) / kBandStride - 2;
+    const int V = p->g.n_det_v;
+    for (int v0 = 0; v0 < V; v0 += max_rows) {
+        a.v0 = v0;
+        a.vb = std::min(max_rows, V - v0);
+        XR_TRY(timed(ctx, 0, [&] { return launch_forward(a, kind, axis, ctx->stream); }));
+    }
+    return XR_OK;
+}
+
+int backward_launch(xr_plan* p, int kind, const float* proj, int code_base, const int32_t* d_codes, int n_views,
+                    float* vol, float* tmp, float* factor, float alpha, int overwrite) {
+    xr_context* ctx = p->ctx;
+    BwdArgs a{};
+    a.d = p->dims;
+    a.b = p->bas;
+    a.vg = p->d_vg;
+    a.views = d_codes;
+    a.n_views = n_views;
+    a.proj = proj;
+    a.code_base = code_base;
+    a.vol = vol;
+    a.tmp = tmp;
+    a.factor = factor;
+    a.chan_stride = (int64_t)p->g.n_vox * p->g.n_vox * p->g.n_vox;
+    a.g2 = p->d_g2;
+    a.alpha = alpha;
+    a.overwrite = overwrite;
+    bool axis = true;
+    for (int c = 0; c < p->n_cond; c++) axis = axis && p->axis_cond[c];
+    return timed(ctx, 1, [&] { return launch_backward(a, kind, axis, ctx->stream); });
+}
+
+int ensure_norm(xr_plan* p) {
+    if (p->d_norm) return XR_OK;
+    xr_context* ctx = p->ctx;
+    const int P = p->g.n_proj;
+    const int64_t lenD = (int64_t)p->g.n_det_u * p->g.n_det_v * P;
+    XR_HIP(ctx, hipMalloc((void**)&p->d_norm, sizeof(float) * lenD * p->n_cond));
+    std::vector<int32_t> codes((size_t)p->n_cond * P);
+    for (size_t i = 0; i < codes.size(); i++) codes[i] = (int32_t)i;
+    DevBuf<int32_t> dc;
+    XR_HIP(ctx, dc.alloc(codes.size()));
+    XR_HIP(ctx, hipMemcpyAsync(dc.p, codes.data(), codes.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    // Splat the bilinear weights alone (factor), then L/(factor + 1e-5) per pixel.
+    XR_TRY(forward_launch(p, FWD_WEIGHT, nullptr, 0, p->d_norm, nullptr, 0, dc.p, nullptr, (int)codes.size(),
+                          OUT_NORM_TABLE, nullptr));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XR_OK;
+}
+
+// ---------------------------------------------------------------- JSON ----
+// Minimal JSON reader for settings.json (objects, arrays, numbers, strings,
+// booleans, null).
+struct JVal {
+    enum Kind { NUL, BOOL, NUM, STR, ARR, OBJ } kind = NUL;
+    bool b = false;
+    double num = 0;
+    std::string str;
+    std::vector<JVal> arr;
+    std::vector<std::pair<std::string, JVal>> obj;
+    const JVal* get(const std::string& k) const {
+        if (kind != OBJ) return nullptr;
+        for (auto& kv : obj)
+            if (kv.first == k) return &kv.second;
+        return nullptr;
+    }
+};
+
+struct JParser {
+    const char* s;
+    const char* e;
+    bool ok = true;
+    void ws() {
+        while (s < e && (*s == ' ' || *s == '\n' || *s == '\r' || *s == '\t')) s++;
+    }
+    bool lit(const char* w) {
+        const size_t n = strlen(w);
+        if ((size_t)(e - s) >= n && strncmp(s, w, n) == 0) {
+            s += n;
+            return true;
+        }
+        return false;
+    }
+    std::string str() {
+        std::string out;
+        if (s >= e || *s != '"') {
+            ok = false;
+            return out;
+        }
+        s++;
+        while (s < e && *s != '"') {
+            if (*s == '\\' && s + 1 < e) {
+                s++;
+                switch (*s) {
+                    case 'n': out += '\n'; break;
+                    case 't': out += '\t'; break;
+                    case 'r': out += '\r'; break;
+                    case 'b': out += '\b'; break;
+                    case 'f': out += '\f'; break;
+                    case 'u':
+                        // keep escaped code points as '?': paths and keys here are ASCII
+                        out += '?';
+                        s += std::min<ptrdiff_t>(4, e - s - 1);
+                        break;
+                    default: out += *s;
+                }
+                s++;
+            } else {
+                out += *s++;
+            }
+        }
+        if (s >= e) ok = false;
+        s++;
+        return out;
+    }
+    JVal val() {
+        JVal v;
+        ws();
+        if (s >= e) {
+            ok = false;
+            return v;
+        }
+        if (*s == '{') {
+            v.kind = JVal::OBJ;
+            s++;
+            ws();
+            if (s < e && *s == '}') {
+                s++;
+                return v;
+            }
+            while (ok) {
+                ws();
+                std::string k = str();
+                ws();
+                if (s >= e || *s != ':') {
+                    ok = false;
+                    break;
+                }
+                s++;
+                JVal x = val();
+                v.obj.emplace_back(std::move(k), std::move(x));
+                ws();
+                if (s < e && *s == ',') {
+                    s++;
+                    continue;
+                }
+                if (s < e && *s == '}') {
+                    s++;
+                    break;
+                }
+                ok = false;
+            }
+        } else if (*s == '[') {
+            v.kind = JVal::ARR;
+            s++;
+            ws();
+            if (s < e && *s == ']') {
+                s++;
+                return v;
+            }
+            while (ok) {
+                v.arr.push_back(val());
+                ws();
+                if (s < e && *s == ',') {
+                    s++;
+                    continue;
+                }
+                if (s < e && *s == ']') {
+                    s++;
+                    break;
+                }
+                ok = false;
+            }
+        } else if (*s == '"') {
+            v.kind = JVal::STR;
+            v.str = str();
+        } else if (lit("true")) {
+            v.kind = JVal::BOOL;
+            v.b = true;
+        } else if (lit("false")) {
+            v.kind = JVal::BOOL;
+        } else if (lit("null")) {
+            v.kind = JVal::NUL;
+        } else {
+            char* end = nullptr;
+            v.kind = JVal::NUM;
+            v.num = strtod(s, &end);
+            if (end == s) ok = false;
+            s = end;
+        }
+        return v;
+    }
+};
+
+bool jnum(const JVal* v, double& out) {
+    if (!v || v->kind != JVal::NUM) return false;
+    out = v->num;
+    return true;
+}
+bool jarr(const JVal* v, std::vector<double>& out) {
+    if (!v || v->kind != JVal::ARR) return false;
+    out.clear();
+    for (auto& x : v->arr) {
+        if (x.kind != JVal::NUM) return false;
+        out.push_back(x.num);
+    }
+    return true;
+}
+
+// ------------------------------------------------------------- phantom ----
+struct SplitMix64 {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------ context ----
+
+int xr_context_create(xr_context** out, int device) {
+    if (!out) return XR_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return XR_ERR_HIP;
+    if (device < 0 || device >= n) return XR_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return XR_ERR_HIP;
+    auto* c = new (std::nothrow) xr_context();
+    if (!c) return XR_ERR_ALLOC;
+    c->device = device;
+    if (hipMalloc((void**)&c->d_loss, sizeof(double)) != hipSuccess) {
+        delete c;
+        return XR_ERR_ALLOC;
+    }
+    *out = c;
+    return XR_OK;
+}
+
+void xr_context_destroy(xr_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& e : ctx->events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    for (auto e : ctx->pool) (void)hipEventDestroy(e);
+    if (ctx->d_codes) (void)hipFree(ctx->d_codes);
+    if (ctx->d_flags) (void)hipFree(ctx->d_flags);
+    if (ctx->d_loss) (void)hipFree(ctx->d_loss);
+    delete ctx;
+}
+
+int xr_context_set_stream(xr_context* ctx, void* hip_stream) {
+    if (!ctx) return XR_ERR_INVALID;
+    ctx->stream = (hipStream_t)hip_stream;
+    return XR_OK;
+}
+
+int xr_context_synchronize(xr_context* ctx) {
+    if (!ctx) return XR_ERR_INVALID;
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XR_OK;
+}
+
+const char* xr_last_error(const xr_context* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int xr_timing_enable(xr_context* ctx, int enable) {
+    if (!ctx) return XR_ERR_INVALID;
+    ctx->timing = enable != 0;
+    return XR_OK;
+}
+
+int xr_timing_reset(xr_context* ctx) {
+    if (!ctx) return XR_ERR_INVALID;
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto& e : ctx->events) {
+        ctx->pool.push_back(e.a);
+        ctx->pool.push_back(e.b);
+    }
+    ctx->events.clear();
+    return XR_OK;
+}
+
+int xr_timing_read(xr_context* ctx, int kind, double* total_ms, int64_t* count) {
+    if (!ctx || !total_ms || !count) return XR_ERR_INVALID;
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double t = 0;
+    int64_t c = 0;
+    for (auto& e : ctx->events) {
+        if (e.kind != kind) continue;
+        float ms = 0;
+        XR_HIP(ctx, hipEventElapsedTime(&ms, e.a, e.b));
+        t += ms;
+        c++;
+    }
+    *total_ms = t;
+    *count = c;
+    return XR_OK;
+}
+
+float xr_default_vox_size(float sdd, float sod, float det_size) { return sod * det_size / sdd; }
+
+// --------------------------------------------------------------- plans ----
+
+int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int convention, int direction,
+                   xr_plan** out) {
+    if (!ctx || !out) return XR_ERR_INVALID;
+    *out = nullptr;
+    XR_TRY(check_geometry(ctx, g));
+    XR_TRY(check_scan(ctx, s));
+    if (convention != XR_CONV_IR && convention != XR_CONV_FDK) return fail(ctx, XR_ERR_INVALID, "bad convention");
+    if (direction != XR_CW && direction != XR_CCW) return fail(ctx, XR_ERR_INVALID, "bad direction");
+    std::unique_ptr<xr_plan> p(new (std::nothrow) xr_plan());
+    if (!p) return fail(ctx, XR_ERR_ALLOC, "plan allocation failed");
+    p->ctx = ctx;
+    p->g = *g;
+    p->n_cond = s->n_cond;
+    p->n_basis = s->n_basis;
+    p->conv = convention;
+    p->dir = direction;
+    p->cond_R.assign(s->cond_R, s->cond_R + 9 * s->n_cond);
+    p->cond_t.assign(s->cond_t, s->cond_t + 3 * s->n_cond);
+    p->cond_offset.assign(s->cond_offset, s->cond_offset + 3 * s->n_cond);
+    if (s->n_basis > 0) p->basis.assign(s->basis, s->basis + 3 * s->n_basis);
+    if (s->grating) p->grating.assign(s->grating, s->grating + 3 * s->n_cond);
+    const int P = g->n_proj, C = s->n_cond, K = std::max(1, s->n_basis);
+    std::vector<ViewGeom> vg((size_t)C * P);
+    std::vector<ChordGeom> cg((size_t)C * P);
+    std::vector<float> g2((size_t)C * P * K, 0.f);
+    p->axis_cond.assign(C, 1);
+    for (int c = 0; c < C; c++)
+        for (int v = 0; v < P; v++) {
+            ViewGeom& t = vg[(size_t)c * P + v];
+            build_view(*p, c, v, t, cg[(size_t)c * P + v]);
+            if (t.axis == 0.0f) p->axis_cond[c] = 0;
+            for (int k = 0; k < s->n_basis; k++) {
+                const double sg = (double)p->basis[3 * k] * t.G[0] + (double)p->basis[3 * k + 1] * t.G[1] +
+                                  (double)p->basis[3 * k + 2] * t.G[2];
+                g2[((size_t)c * P + v) * K + k] = (float)(sg * sg);
+            }
+        }
+    Dims& d = p->dims;
+    d.N = g->n_vox;
+    d.U = g->n_det_u;
+    d.V = g->n_det_v;
+    d.P = P;
+    d.K = s->n_basis;
+    d.xc = (convention == XR_CONV_IR) ? (float)(g->n_vox + 1) * 0.5f : (float)(g->n_vox - 1) * 0.5f;
+    d.hu = 0.5f * (float)g->n_det_u;
+    d.hv = 0.5f * (float)g->n_det_v;
+    d.umax = (float)g->n_det_u - 0.55f;
+    d.vmax = (float)g->n_det_v - 0.55f;
+    d.vs = g->vox_size;
+    {
+        // ir.cu:1513-1514, evaluated left to right in FP32
+        volatile float m = g->sod / g->sdd;
+        volatile float den = g->det_size * m * g->det_size * m;
+        d.ratio = (g->vox_size * g->vox_size) / den;
+    }
+    d.sod = g->sod;
+    d.cfdk = 2.0f * (float)M_PI / (float)P;
+    for (int k = 0; k < s->n_basis; k++) {
+        for (int i = 0; i < 3; i++) p->bas.S[k][i] = p->basis[3 * k + i];
+        p->bas.S2[k] = p->bas.S[k][0] * p->bas.S[k][0] + p->bas.S[k][1] * p->bas.S[k][1] +
+                       p->bas.S[k][2] * p->bas.S[k][2];
+    }
+    XR_HIP(ctx, hipSetDevice(ctx->device));
+    XR_HIP(ctx, hipMalloc((void**)&p->d_vg, vg.size() * sizeof(ViewGeom)));
+    XR_HIP(ctx, hipMalloc((void**)&p->d_cg, cg.size() * sizeof(ChordGeom)));
+    XR_HIP(ctx, hipMalloc((void**)&p->d_g2, g2.size() * sizeof(float)));
+    XR_HIP(ctx, hipMemcpy(p->d_vg, vg.data(), vg.size() * sizeof(ViewGeom), hipMemcpyHostToDevice));
+    XR_HIP(ctx, hipMemcpy(p->d_cg, cg.data(), cg.size() * sizeof(ChordGeom), hipMemcpyHostToDevice));
+    XR_HIP(ctx, hipMemcpy(p->d_g2, g2.data(), g2.size() * sizeof(float), hipMemcpyHostToDevice));
+    *out = p.release();
+    return XR_OK;
+}
+
+void xr_plan_destroy(xr_plan* p) {
+    if (!p) return;
+    (void)hipStreamSynchronize(p->ctx->stream);
+    if (p->d_vg) (void)hipFree(p->d_vg);
+    if (p->d_cg) (void)hipFree(p->d_cg);
+    if (p->d_g2) (void)hipFree(p->d_g2);
+    if (p->d_norm) (void)hipFree(p->d_norm);
+    delete p;
+}
+
+// ------------------------------------------------------ device operators ----
+
+int xr_volume_to_columns(xr_context* ctx, const float* ref_dev, float* col_dev, int n_vox, int channels) {
+    if (!ctx || !ref_dev || !col_dev || n_vox <= 0 || channels <= 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    return timed(ctx, 3, [&] { return launch_to_internal(ref_dev, col_dev, n_vox, channels, ctx->stream); });
+}
+
+int xr_volume_from_columns(xr_context* ctx, const float* col_dev, float* ref_dev, int n_vox, int channels) {
+    if (!ctx || !ref_dev || !col_dev || n_vox <= 0 || channels <= 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    return timed(ctx, 3, [&] { return launch_from_internal(col_dev, ref_dev, n_vox, channels, ctx->stream); });
+}
+
+static int upload_views(xr_plan* p, const int32_t* views, int n_views, const uint8_t* accumulate) {
+    xr_context* ctx = p->ctx;
+    const int total = p->n_cond * p->g.n_proj;
+    for (int i = 0; i < n_views; i++)
+        if (views[i] < 0 || views[i] >= total)
+            return fail(ctx, XR_ERR_INVALID, "view code %d out of range [0, %d)", views[i], total);
+    XR_TRY(ensure_codes(ctx, (size_t)n_views));
+    XR_HIP(ctx, hipMemcpyAsync(ctx->d_codes, views, sizeof(int32_t) * n_views, hipMemcpyHostToDevice, ctx->stream));
+    if (accumulate) {
+        XR_TRY(ensure_flags(ctx, (size_t)n_views));
+        XR_HIP(ctx, hipMemcpyAsync(ctx->d_flags, accumulate, (size_t)n_views, hipMemcpyHostToDevice, ctx->stream));
+    }
+    return XR_OK;
+}
+
+int xr_project(xr_context* ctx, xr_plan* plan, const float* vol, float* proj, const float* sino,
+               const int32_t* views, int n_views, const uint8_t* accumulate, int xtt, int out_mode, double* loss) {
+    if (!ctx || !plan || !vol || !proj || !views || n_views < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    if (out_mode < XR_OUT_RAW || out_mode > XR_OUT_RESIDUAL) return fail(ctx, XR_ERR_INVALID, "bad out_mode");
+    if ((out_mode == XR_OUT_RATIO || out_mode == XR_OUT_RESIDUAL) && !sino)
+        return fail(ctx, XR_ERR_INVALID, "sino required for ratio/residual");
+    if (xtt && plan->n_basis <= 0) return fail(ctx, XR_ERR_INVALID, "XTT projection needs basis vectors");
+    if (plan->conv != XR_CONV_IR) return fail(ctx, XR_ERR_INVALID, "forward projection uses the IR convention");
+    if (n_views == 0) {
+        if (loss) *loss = 0;
+        return XR_OK;
+    }
+    XR_TRY(upload_views(plan, views, n_views, accumulate));
+    if (out_mode != XR_OUT_RAW) XR_TRY(ensure_norm(plan));
+    if (loss) XR_HIP(ctx, hipMemsetAsync(ctx->d_loss, 0, sizeof(double), ctx->stream));
+    const int64_t N3 = (int64_t)plan->g.n_vox * plan->g.n_vox * plan->g.n_vox;
+    XR_TRY(forward_launch(plan, xtt ? FWD_XTT : FWD_PLAIN, vol, N3, proj, sino, 0, ctx->d_codes,
+                          accumulate ? ctx->d_flags : nullptr, n_views, out_mode, ctx->d_loss));
+    if (loss) {
+        XR_HIP(ctx, hipMemcpyAsync(loss, ctx->d_loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return XR_OK;
+}
+
+int xr_backproject(xr_context* ctx, xr_plan* plan, const float* proj, const int32_t* views, int n_views, int xtt,
+                   float* tmp, float* factor) {
+    if (!ctx || !plan || !proj || !views || !tmp || !factor || n_views < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    if (xtt && plan->n_basis <= 0) return fail(ctx, XR_ERR_INVALID, "XTT backprojection needs basis vectors");
+    if (plan->conv != XR_CONV_IR) return fail(ctx, XR_ERR_INVALID, "backprojection uses the IR convention");
+    if (n_views == 0) return XR_OK;
+    XR_TRY(upload_views(plan, views, n_views, nullptr));
+    return backward_launch(plan, xtt ? BWD_XTT_RAW : BWD_RAW, proj, 0, ctx->d_codes, n_views, nullptr, tmp, factor,
+                           0.f, 0);
+}
+
+int xr_backproject_update(xr_context* ctx, xr_plan* plan, const float* proj, const int32_t* views, int n_views,
+                          int xtt, int method, float lambda, int subset_size, float* vol) {
+    if (!ctx || !plan || !proj || !views || !vol || n_views < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    if (method != XR_METHOD_MLEM && method != XR_METHOD_ART) return fail(ctx, XR_ERR_INVALID, "method must be MLEM or ART");
+    if (xtt && plan->n_basis <= 0) return fail(ctx, XR_ERR_INVALID, "XTT update needs basis vectors");
+    if (plan->conv != XR_CONV_IR) return fail(ctx, XR_ERR_INVALID, "backprojection uses the IR convention");
+    if (subset_size <= 0) return fail(ctx, XR_ERR_INVALID, "subset_size must be positive");
+    XR_TRY(upload_views(plan, views, n_views, nullptr));
+    int kind;
+    if (xtt)
+        kind = (method == XR_METHOD_MLEM) ? BWD_XTT_MLEM : BWD_XTT_ART;
+    else
+        kind = (method == XR_METHOD_MLEM) ? BWD_MLEM : BWD_ART;
+    return backward_launch(plan, kind, proj, 0, ctx->d_codes, n_views, vol, nullptr, nullptr,
+                           lambda / (float)subset_size, 0);
+}
+
+int xr_fdk_taps(const xr_geometry* g, int filter, float* taps) {
+    if (!g || !taps || g->n_det_u <= 0) return XR_ERR_INVALID;
+    const float d = g->det_size * (g->sod / g->sdd);
+    for (int m = 0; m < g->n_det_u; m++) {
+        if (filter == XR_FILTER_REFERENCE) {
+            // reconstruct.cu:1697: double product rounded to float, FP32 reciprocal
+            taps[m] = 1.0f / (float)(M_PI * M_PI * d * (1.0f - 4.0f * (float)(m * m)));
+        } else if (filter == XR_FILTER_RAMLAK) {
+            if (m == 0)
+                taps[m] = (float)(1.0 / (8.0 * (double)d));
+            else if (m % 2 == 1)
+                taps[m] = (float)(-1.0 / (2.0 * M_PI * M_PI * (double)m * (double)m * (double)d));
+            else
+                taps[m] = 0.0f;
+        } else {
+            return XR_ERR_INVALID;
+        }
+    }
+    return XR_OK;
+}
+
+static int fdk_filter(xr_context* ctx, xr_plan* plan, int filter, const float* sino, float* filt,
+                      const int32_t* views, int n_views, int code_base) {
+    if (!ctx || !plan || !sino || !filt || !views || n_views < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    if (n_views == 0) return XR_OK;
+    const xr_geometry& g = plan->g;
+    std::vector<float> taps(g.n_det_u);
+    if (xr_fdk_taps(&g, filter, taps.data()) != XR_OK) return fail(ctx, XR_ERR_INVALID, "bad filter");
+    XR_TRY(upload_views(plan, views, n_views, nullptr));
+    DevBuf<float> dw, dt;
+    XR_HIP(ctx, dw.alloc((size_t)g.n_det_u * g.n_det_v));
+    XR_HIP(ctx, dt.alloc(taps.size()));
+    XR_HIP(ctx, hipMemcpyAsync(dt.p, taps.data(), taps.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    XR_TRY(timed(ctx, 3, [&] { return launch_fdk_weight(dw.p, g.n_det_u, g.n_det_v, g.sdd, g.det_size, ctx->stream); }));
+    XR_TRY(timed(ctx, 2, [&] {
+        return launch_fdk_filter(sino, filt, dw.p, dt.p, g.n_det_u, g.n_det_v, ctx->d_codes, n_views, code_base,
+                                 ctx->stream);
+    }));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));  // dw / dt are freed on return
+    return XR_OK;
+}
+
+int xr_fdk_filter(xr_context* ctx, xr_plan* plan, int filter, const float* sino, float* filt, const int32_t* views,
+                  int n_views) {
+    return fdk_filter(ctx, plan, filter, sino, filt, views, n_views, 0);
+}
+
+static int fdk_backproject(xr_context* ctx, xr_plan* plan, const float* filt, const int32_t* views, int n_views,
+                           int overwrite, float* vol, int code_base) {
+    if (!ctx || !plan || !filt || !views || !vol || n_views < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    if (plan->conv != XR_CONV_FDK) return fail(ctx, XR_ERR_INVALID, "FDK backprojection needs an XR_CONV_FDK plan");
+    if (n_views == 0) return XR_OK;
+    XR_TRY(upload_views(plan, views, n_views, nullptr));
+    return backward_launch(plan, BWD_FDK, filt, code_base, ctx->d_codes, n_views, vol, nullptr, nullptr, 0.f,
+                           overwrite);
+}
+
+int xr_fdk_backproject(xr_context* ctx, xr_plan* plan, const float* filt, const int32_t* views, int n_views,
+                       int overwrite, float* vol) {
+    return fdk_backproject(ctx, plan, filt, views, n_views, overwrite, vol, 0);
+}
+
+int xr_volume_sqrt(xr_context* ctx, float* vol, int64_t n) {
+    if (!ctx || !vol || n < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    return timed(ctx, 3, [&] { return launch_sqrt_abs(vol, n, ctx->stream); });
+}
+
+int xr_pca(xr_context* ctx, const float* mu, int n_basis, const float* basis_host, int64_t nvox, int64_t chan_stride,
+           float* md, float* evalue) {
+    if (!ctx || !mu || !basis_host || !md || !evalue || n_basis <= 0 || n_basis > kMaxBasis || nvox < 0 ||
+        chan_stride < nvox)
+        return fail(ctx, XR_ERR_INVALID, "bad args");
+    Basis b{};
+    for (int k = 0; k < n_basis; k++)
+        for (int i = 0; i < 3; i++) b.S[k][i] = basis_host[3 * k + i];
+    return timed(ctx, 3, [&] { return launch_pca(mu, n_basis, b, nvox, chan_stride, md, evalue, ctx->stream); });
+}
+
+// -------------------------------------------------------------- solvers ----
+
+// Subset structure of reconstruct.cu:61-65, 89: subset s holds views
+// (s + batch*j) % P, j < ceil(P/batch).
+static void subset_views(int sub, int batch, int P, std::vector<int>& out) {
+    const int ss = (P + batch - 1) / batch;
+    out.resize(ss);
+    for (int j = 0; j < ss; j++) out[j] = (sub + batch * j) % P;
+}
+
+int xr_ir_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, float* const* sino, float* const* vol,
+                      int epochs, int batch, int direction, int method, float lambda, xr_stats* stats) {
+    if (!ctx) return XR_ERR_INVALID;
+    XR_TRY(check_geometry(ctx, g));
+    XR_TRY(check_scan(ctx, s));
+    if (!sino || !vol) return fail(ctx, XR_ERR_INVALID, "sinogram / volume arrays missing");
+    if (epochs < 0 || batch <= 0 || batch > g->n_proj) return fail(ctx, XR_ERR_INVALID, "bad epochs / batch");
+    if (method != XR_METHOD_MLEM && method != XR_METHOD_ART) return fail(ctx, XR_ERR_INVALID, "method must be MLEM or ART");
+    xr_plan* plan = nullptr;
+    XR_TRY(xr_plan_create(ctx, g, s, XR_CONV_IR, direction, &plan));
+    std::unique_ptr<xr_plan, void (*)(xr_plan*)> guard(plan, xr_plan_destroy);
+    XR_TRY(ensure_norm(plan));
+    const int N = g->n_vox, P = g->n_proj, C = s->n_cond;
+    const int64_t N3 = (int64_t)N * N * N, lenD = (int64_t)g->n_det_u * g->n_det_v * P;
+    const int ss = (P + batch - 1) / batch;
+    DevBuf<float> dref, dvol, dsino, dproj;
+    DevBuf<int32_t> dcodes;
+    DevBuf<uint8_t> dflags;
+    XR_HIP(ctx, dref.alloc(N3));
+    XR_HIP(ctx, dvol.alloc(N3));
+    XR_HIP(ctx, dsino.alloc(lenD));
+    XR_HIP(ctx, dproj.alloc(lenD));
+    const size_t per_epoch = (size_t)batch * ss;
+    XR_HIP(ctx, dcodes.alloc(per_epoch * std::max(epochs, 1)));
+    XR_HIP(ctx, dflags.alloc(per_epoch * std::max(epochs, 1)));
+    // One engine per call, continuing across conditions (reconstruct.cu:75-81).
+    std::vector<int> order(batch);
+    for (int i = 0; i < batch; i++) order[i] = i;
+    std::mt19937_64 rng;
+    std::vector<int32_t> codes(per_epoch * std::max(epochs, 1));
+    std::vector<uint8_t> flags(codes.size());
+    std::vector<int> sv;
+    const float alpha = (method == XR_METHOD_ART) ? lambda / (float)ss : 0.f;
+    const int kind = (method == XR_METHOD_MLEM) ? BWD_MLEM : BWD_ART;
+    const int out_mode = (method == XR_METHOD_MLEM) ? OUT_RATIO : OUT_RESIDUAL;
+    for (int c = 0; c < C; c++) {
+        // Views visited earlier in an epoch keep their image (proj is zeroed once
+        // per epoch, reconstruct.cu:83), so a repeated view accumulates.
+        for (int ep = 0; ep < epochs; ep++) {
+            std::shuffle(order.begin(), order.end(), rng);
+            std::vector<char> visited(P, 0);
+            for (int pos = 0; pos < batch; pos++) {
+                subset_views(order[pos], batch, P, sv);
+                for (int j = 0; j < ss; j++) {
+                    const size_t i = (size_t)ep * per_epoch + (size_t)pos * ss + j;
+                    codes[i] = c * P + sv[j];
+                    flags[i] = visited[sv[j]];
+                    visited[sv[j]] = 1;
+                }
+            }
+        }
+        if (epochs > 0) {
+            XR_HIP(ctx, hipMemcpyAsync(dcodes.p, codes.data(), codes.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                       ctx->stream));
+            XR_HIP(ctx, hipMemcpyAsync(dflags.p, flags.data(), flags.size(), hipMemcpyHostToDevice, ctx->stream));
+        }
+        XR_HIP(ctx, hipMemcpyAsync(dref.p, vol[c], sizeof(float) * N3, hipMemcpyHostToDevice, ctx->stream));
+        XR_TRY(xr_volume_to_columns(ctx, dref.p, dvol.p, N, 1));
+        // One condition at a time: its images hold codes c*P .. c*P+P-1.
+        XR_HIP(ctx, hipMemcpyAsync(dsino.p, sino[c], sizeof(float) * lenD, hipMemcpyHostToDevice, ctx->stream));
+        for (int ep = 0; ep < epochs; ep++) {
+            XR_HIP(ctx, hipMemsetAsync(ctx->d_loss, 0, sizeof(double), ctx->stream));
+            for (int pos = 0; pos < batch; pos++) {
+                const size_t off = (size_t)ep * per_epoch + (size_t)pos * ss;
+                XR_TRY(forward_launch(plan, FWD_PLAIN, dvol.p, 0, dproj.p, dsino.p, c * P, dcodes.p + off,
+                                      dflags.p + off, ss, out_mode, ctx->d_loss));
+                XR_TRY(backward_launch(plan, kind, dproj.p, c * P, dcodes.p + off, ss, dvol.p, nullptr, nullptr, alpha,
+                                       0));
+            }
+            if (stats && stats->losses) {
+                XR_HIP(ctx, hipMemcpyAsync(&stats->losses[(size_t)c * epochs + ep], ctx->d_loss, sizeof(double),
+                                           hipMemcpyDeviceToHost, ctx->stream));
+            }
+        }
+        XR_TRY(xr_volume_from_columns(ctx, dvol.p, dref.p, N, 1));
+        XR_HIP(ctx, hipMemcpyAsync(vol[c], dref.p, sizeof(float) * N3, hipMemcpyDeviceToHost, ctx->stream));
+        XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return XR_OK;
+}
+
+int xr_xtt_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, float* const* sino, float* const* vol,
+                       float* const* md, float* const* evalue, int epochs, int batch, int direction, int method,
+                       float lambda, xr_stats* stats) {
+    if (!ctx) return XR_ERR_INVALID;
+    XR_TRY(check_geometry(ctx, g));
+    XR_TRY(check_scan(ctx, s));
+    if (s->n_basis <= 0) return fail(ctx, XR_ERR_INVALID, "XTT needs basis vectors");
+    if (!sino || !vol) return fail(ctx, XR_ERR_INVALID, "sinogram / volume arrays missing");
+    if (epochs < 0 || batch <= 0 || batch > g->n_proj) return fail(ctx, XR_ERR_INVALID, "bad epochs / batch");
+    if (method != XR_METHOD_MLEM && method != XR_METHOD_ART) return fail(ctx, XR_ERR_INVALID, "method must be MLEM or ART");
+    xr_plan* plan = nullptr;
+    XR_TRY(xr_plan_create(ctx, g, s, XR_CONV_IR, direction, &plan));
+    std::unique_ptr<xr_plan, void (*)(xr_plan*)> guard(plan, xr_plan_destroy);
+    XR_TRY(ensure_norm(plan));
+    const int N = g->n_vox, P = g->n_proj, C = s->n_cond, K = s->n_basis;
+    const int64_t N3 = (int64_t)N * N * N, lenD = (int64_t)g->n_det_u * g->n_det_v * P;
+    const int ss = (P + batch - 1) / batch;
+    DevBuf<float> dref, dvol, dsino, dproj;
+    DevBuf<int32_t> dcodes;
+    DevBuf<uint8_t> dflags;
+    XR_HIP(ctx, dref.alloc(N3 * std::max(K, 3)));
+    XR_HIP(ctx, dvol.alloc(N3 * std::max(K, 6)));
+    XR_HIP(ctx, dsino.alloc(lenD * C));
+    XR_HIP(ctx, dproj.alloc(lenD * C));
+    const size_t per_epoch = (size_t)batch * C * ss;
+    XR_HIP(ctx, dcodes.alloc(per_epoch * std::max(epochs, 1)));
+    XR_HIP(ctx, dflags.alloc(per_epoch * std::max(epochs, 1)));
+    std::vector<int> order(batch);
+    for (int i = 0; i < batch; i++) order[i] = i;
+    std::vector<int32_t> codes(per_epoch * std::max(epochs, 1));
+    std::vector<uint8_t> flags(codes.size());
+    std::vector<int> sv;
+    for (int ep = 0; ep < epochs; ep++) {
+        // XTT re-creates the engine every epoch but shuffles the order left by the
+        // previous epoch (reconstruct.cu:843-859).
+        std::mt19937_64 rng;
+        std::shuffle(order.begin(), order.end(), rng);
+        std::vector<char> visited((size_t)C * P, 0);
+        for (int pos = 0; pos < batch; pos++) {
+            subset_views(order[pos], batch, P, sv);
+            for (int c = 0; c < C; c++)
+                for (int j = 0; j < ss; j++) {
+                    const size_t i = (size_t)ep * per_epoch + (size_t)pos * C * ss + (size_t)c * ss + j;
+                    const int code = c * P + sv[j];
+                    codes[i] = code;
+                    flags[i] = visited[code];
+                    visited[code] = 1;
+                }
+        }
+    }
+    if (epochs > 0) {
+        XR_HIP(ctx, hipMemcpyAsync(dcodes.p, codes.data(), codes.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        XR_HIP(ctx, hipMemcpyAsync(dflags.p, flags.data(), flags.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    for (int k = 0; k < K; k++)
+        XR_HIP(ctx, hipMemcpyAsync(dref.p + k * N3, vol[k], sizeof(float) * N3, hipMemcpyHostToDevice, ctx->stream));
+    XR_TRY(xr_volume_to_columns(ctx, dref.p, dvol.p, N, K));
+    for (int c = 0; c < C; c++)
+        XR_HIP(ctx, hipMemcpyAsync(dsino.p + c * lenD, sino[c], sizeof(float) * lenD, hipMemcpyHostToDevice, ctx->stream));
+    const float alpha = (method == XR_METHOD_ART) ? lambda / (float)ss : 0.f;
+    const int kind = (method == XR_METHOD_MLEM) ? BWD_XTT_MLEM : BWD_XTT_ART;
+    const int out_mode = (method == XR_METHOD_MLEM) ? OUT_RATIO : OUT_RESIDUAL;
+    for (int ep = 0; ep < epochs; ep++) {
+        XR_HIP(ctx, hipMemsetAsync(ctx->d_loss, 0, sizeof(double), ctx->stream));
+        for (int pos = 0; pos < batch; pos++) {
+            const size_t off = (size_t)ep * per_epoch + (size_t)pos * C * ss;
+            XR_TRY(forward_launch(plan, FWD_XTT, dvol.p, 0, dproj.p, dsino.p, 0, dcodes.p + off, dflags.p + off,
+                                  C * ss, out_mode, ctx->d_loss));
+            XR_TRY(backward_launch(plan, kind, dproj.p, 0, dcodes.p + off, C * ss, dvol.p, nullptr, nullptr, alpha, 0));
+        }
+        if (stats && stats->losses)
+            XR_HIP(ctx, hipMemcpyAsync(&stats->losses[ep], ctx->d_loss, sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+    }
+    // voxelSqrt (reconstruct.cu:919-922), sinogram overwrite (:924-925), volumes (:926-927).
+    XR_TRY(xr_volume_sqrt(ctx, dvol.p, N3 * K));
+    for (int c = 0; c < C; c++)
+        XR_HIP(ctx, hipMemcpyAsync(sino[c], dproj.p + c * lenD, sizeof(float) * lenD, hipMemcpyDeviceToHost, ctx->stream));
+    XR_TRY(xr_volume_from_columns(ctx, dvol.p, dref.p, N, K));
+    for (int k = 0; k < K; k++)
+        XR_HIP(ctx, hipMemcpyAsync(vol[k], dref.p + k * N3, sizeof(float) * N3, hipMemcpyDeviceToHost, ctx->stream));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (md) {
+        // calcEigenVector over all voxels (reconstruct.cu:935-942), on the device,
+        // from the reference-layout volumes already in dref.
+        DevBuf<float> dmd, dev;
+        XR_HIP(ctx, dmd.alloc(N3 * 3));
+        XR_HIP(ctx, dev.alloc(N3 * 3));
+        XR_TRY(xr_pca(ctx, dref.p, K, s->basis, N3, N3, dmd.p, dev.p));
+        for (int i = 0; i < 3; i++) {
+            XR_HIP(ctx, hipMemcpyAsync(md[i], dmd.p + i * N3, sizeof(float) * N3, hipMemcpyDeviceToHost, ctx->stream));
+            if (evalue && evalue[i])
+                XR_HIP(ctx, hipMemcpyAsync(evalue[i], dev.p + i * N3, sizeof(float) * N3, hipMemcpyDeviceToHost,
+                                           ctx->stream));
+        }
+        XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return XR_OK;
+}
+
+int xr_fdk_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, float* const* sino, float* const* vol,
+                       int direction, int filter) {
+    if (!ctx) return XR_ERR_INVALID;
+    XR_TRY(check_geometry(ctx, g));
+    XR_TRY(check_scan(ctx, s));
+    if (!sino || !vol) return fail(ctx, XR_ERR_INVALID, "sinogram / volume arrays missing");
+    xr_plan* plan = nullptr;
+    XR_TRY(xr_plan_create(ctx, g, s, XR_CONV_FDK, direction, &plan));
+    std::unique_ptr<xr_plan, void (*)(xr_plan*)> guard(plan, xr_plan_destroy);
+    const int N = g->n_vox, P = g->n_proj, C = s->n_cond;
+    const int64_t N3 = (int64_t)N * N * N, lenD = (int64_t)g->n_det_u * g->n_det_v * P;
+    DevBuf<float> dsino, dfilt, dvol, dref;
+    XR_HIP(ctx, dsino.alloc(lenD));
+    XR_HIP(ctx, dfilt.alloc(lenD));
+    XR_HIP(ctx, dvol.alloc(N3));
+    XR_HIP(ctx, dref.alloc(N3));
+    std::vector<int32_t> codes(P);
+    for (int c = 0; c < C; c++) {
+        for (int n = 0; n < P; n++) codes[n] = c * P + n;
+        XR_HIP(ctx, hipMemcpyAsync(dsino.p, sino[c], sizeof(float) * lenD, hipMemcpyHostToDevice, ctx->stream));
+        XR_TRY(fdk_filter(ctx, plan, filter, dsino.p, dfilt.p, codes.data(), P, c * P));
+        XR_TRY(fdk_backproject(ctx, plan, dfilt.p, codes.data(), P, 1, dvol.p, c * P));
+        XR_TRY(xr_volume_from_columns(ctx, dvol.p, dref.p, N, 1));
+        XR_HIP(ctx, hipMemcpyAsync(vol[c], dref.p, sizeof(float) * N3, hipMemcpyDeviceToHost, ctx->stream));
+        XR_HIP(ctx, hipMemcpyAsync(sino[c], dfilt.p, sizeof(float) * lenD, hipMemcpyDeviceToHost, ctx->stream));
+        XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return XR_OK;
+}
+
+// ------------------------------------------------------------- settings ----
+
+int xr_load_settings(const char* json_path, const char* tag, xr_settings* out) {
+    if (!json_path || !tag || !out) return XR_ERR_INVALID;
+    FILE* f = fopen(json_path, "rb");
+    if (!f) return XR_ERR_IO;
+    std::string text;
+    char buf[65536];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
+    fclose(f);
+    JParser jp{text.data(), text.data() + text.size()};
+    const JVal root = jp.val();
+    if (!jp.ok || root.kind != JVal::OBJ) return XR_ERR_IO;
+    const JVal* rv = root.get("recon_variable");
+    const JVal* t = root.get(tag);
+    if (!rv || !t || t->kind != JVal::OBJ) return XR_ERR_NOT_FOUND;
+    std::memset(out, 0, sizeof(*out));
+    auto copy_str = [](const JVal* v, char* dst, size_t cap) {
+        if (v && v->kind == JVal::STR) snprintf(dst, cap, "%s", v->str.c_str());
+    };
+    copy_str(t->get("proj_path"), out->proj_path, sizeof(out->proj_path));
+    copy_str(t->get("vol_path"), out->vol_path, sizeof(out->vol_path));
+    copy_str(t->get("direc_path"), out->direc_path, sizeof(out->direc_path));
+    double x;
+    if (!jnum(rv->get("blockSize"), x)) return XR_ERR_NOT_FOUND;
+    out->block_size = (int)x;
+    if (!jnum(rv->get("vector"), x)) return XR_ERR_NOT_FOUND;
+    out->n_basis = (int)x;
+    if (!jnum(rv->get("condition"), x)) return XR_ERR_NOT_FOUND;
+    out->n_cond = (int)x;
+    if (out->n_basis < 0 || out->n_basis > 16 || out->n_cond <= 0 || out->n_cond > 64) return XR_ERR_INVALID;
+    struct {
+        const char* key;
+        float* dst;
+    } fl[] = {{"sod", &out->sod}, {"sdd", &out->sdd}, {"det_size", &out->det_size}};
+    for (auto& e : fl) {
+        if (!jnum(t->get(e.key), x)) return XR_ERR_NOT_FOUND;
+        *e.dst = (float)x;
+    }
+    struct {
+        const char* key;
+        int32_t* dst;
+    } in[] = {{"proj", &out->n_proj}, {"num_det_u", &out->n_det_u}, {"num_det_v", &out->n_det_v},
+              {"num_voxel", &out->n_vox}};
+    for (auto& e : in) {
+        if (!jnum(t->get(e.key), x)) return XR_ERR_NOT_FOUND;
+        *e.dst = (int32_t)x;
+    }
+    if (!jnum(t->get("rot"), x)) return XR_ERR_NOT_FOUND;
+    const int rot = (int)x;
+    std::vector<double> mR, vT, aT, off, base, index;
+    if (!jarr(t->get("matrixRot"), mR) || !jarr(t->get("vecTrans"), vT) || !jarr(t->get("areaTrans"), aT) ||
+        !jarr(t->get("offset"), off) || !jarr(rv->get("base"), base))
+        return XR_ERR_NOT_FOUND;
+    if ((int)base.size() < 3 * out->n_basis || aT.size() < 3 || (int)mR.size() < 9 * rot || (int)vT.size() < 3 * rot ||
+        (int)off.size() < 3 * rot)
+        return XR_ERR_INVALID;
+    for (int i = 0; i < 3 * out->n_basis; i++) out->basis[i] = (float)base[i];
+    // vecTrans += areaTrans for every rotation (params.cu:74-78), in FP32 as the reference does.
+    std::vector<float> vTf(vT.size());
+    for (size_t i = 0; i < vT.size(); i++) vTf[i] = (float)vT[i];
+    for (int i = 0; i < rot; i++)
+        for (int j = 0; j < 3; j++) vTf[3 * i + j] += (float)aT[j];
+    // A missing "sorting" key reads as false (the reference's json lookup would throw).
+    const JVal* so = t->get("sorting");
+    const bool sorting = so && so->kind == JVal::BOOL && so->b;
+    if (sorting) {
+        if (!jarr(t->get("index"), index) || (int)index.size() < out->n_cond) return XR_ERR_NOT_FOUND;
+    }
+    if (!sorting && out->n_cond > rot) return XR_ERR_INVALID;
+    for (int c = 0; c < out->n_cond; c++) {
+        const int src = sorting ? (int)index[c] : c;
+        if (src < 0 || src >= rot) return XR_ERR_INVALID;
+        out->load_index[c] = src + 1;  // params.cu:83, 90
+        for (int j = 0; j < 9; j++) out->cond_R[9 * c + j] = (float)mR[9 * src + j];
+        for (int j = 0; j < 3; j++) {
+            out->cond_t[3 * c + j] = vTf[3 * src + j];
+            out->cond_offset[3 * c + j] = (float)off[3 * src + j];
+        }
+    }
+    return XR_OK;
+}
+
+int xr_settings_geometry(const xr_settings* st, xr_geometry* g, xr_scan* s) {
+    if (!st || !g || !s) return XR_ERR_INVALID;
+    g->sdd = st->sdd;
+    g->sod = st->sod;
+    g->det_size = st->det_size;
+    g->vox_size = xr_default_vox_size(st->sdd, st->sod, st->det_size);
+    g->n_vox = st->n_vox;
+    g->n_det_u = st->n_det_u;
+    g->n_det_v = st->n_det_v;
+    g->n_proj = st->n_proj;
+    s->n_cond = st->n_cond;
+    s->n_basis = st->n_basis;
+    s->cond_R = st->cond_R;
+    s->cond_t = st->cond_t;
+    s->cond_offset = st->cond_offset;
+    s->basis = st->basis;
+    s->grating = nullptr;
+    return XR_OK;
+}
+
+// ------------------------------------------------------------- phantoms ----
+
+int xr_phantom_ellipsoid_params(uint64_t seed, int count, double fov_radius, double centre_frac, double axis_lo,
+                                double axis_hi, double value_lo, double value_hi, double* out) {
+    if (count < 0 || !out || !(fov_radius > 0) || !(axis_lo > 0) || axis_hi < axis_lo) return XR_ERR_INVALID;
+    SplitMix64 r{seed};
+    for (int e = 0; e < count; e++) {
+        double* q = out + 16 * e;
+        double c[3];
+        do {
+            for (int i = 0; i < 3; i++) c[i] = 2.0 * r.uniform() - 1.0;
+        } while (c[0] * c[0] + c[1] * c[1] + c[2] * c[2] > 1.0);
+        for (int i = 0; i < 3; i++) q[i] = c[i] * centre_frac * fov_radius;
+        for (int i = 0; i < 3; i++) q[3 + i] = (axis_lo + (axis_hi - axis_lo) * r.uniform()) * fov_radius;
+        // uniform rotation from a uniform unit quaternion (Shoemake)
+        const double u1 = r.uniform(), u2 = r.uniform(), u3 = r.uniform();
+        const double a = std::sqrt(1.0 - u1), b = std::sqrt(u1);
+        const double w = a * std::sin(2 * M_PI * u2), x = a * std::cos(2 * M_PI * u2);
+        const double y = b * std::sin(2 * M_PI * u3), z = b * std::cos(2 * M_PI * u3);
+        const double R[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - z * w),     2 * (x * z + y * w),
+                             2 * (x * y + z * w),     1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                             2 * (x * z - y * w),     2 * (y * z + x * w),     1 - 2 * (x * x + y * y)};
+        for (int i = 0; i < 9; i++) q[6 + i] = R[i];
+        q[15] = value_lo + (value_hi - value_lo) * r.uniform();
+    }
+    return XR_OK;
+}
+
+int xr_phantom_rasterise(xr_context* ctx, const double* params_host, int count, int n_vox, float vox_size,
+                         float* vol_ref_dev) {
+    if (!ctx || (!params_host && count > 0) || count < 0 || n_vox <= 0 || !vol_ref_dev)
+        return fail(ctx, XR_ERR_INVALID, "bad args");
+    DevBuf<double> dp;
+    XR_HIP(ctx, dp.alloc((size_t)16 * std::max(count, 1)));
+    if (count > 0)
+        XR_HIP(ctx, hipMemcpyAsync(dp.p, params_host, sizeof(double) * 16 * count, hipMemcpyHostToDevice, ctx->stream));
+    XR_TRY(timed(ctx, 3, [&] { return launch_ellipsoids(vol_ref_dev, n_vox, vox_size, dp.p, count, ctx->stream); }));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XR_OK;
+}
+
+}  // extern "C"
