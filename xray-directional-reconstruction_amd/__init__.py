@@ -63,6 +63,13 @@ class XRError(RuntimeError):
 
 
 def _load():
+    # torch ships its own HIP runtime (libamdhip64).  Both it and libxr.so need
+    # the same runtime instance in the process, so torch (when present) is
+    # imported first and libxr.so binds to the runtime torch already loaded.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             "libxr.so not built (%s); run __graft_entry__.build() or "

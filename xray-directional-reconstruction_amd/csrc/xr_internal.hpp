@@ -14,8 +14,6 @@ constexpr int kBandU = 16;
 constexpr int kBandStride = kBandU + 3;
 // Max basis vectors (scattering directions) per voxel for the XTT operator.
 constexpr int kMaxBasis = 16;
-// LDS capacity (floats) of the detector footprint staged by a backprojection block.
-constexpr int kStageCap = 12288;
 
 // Per-(condition, view) projection table, computed on the host in FP64 and
 // rounded once to FP32.  With centred voxel coordinates x' = x - xc (same for
@@ -85,6 +83,8 @@ struct FwdArgs {
     const float* norm;       // L/(factor+1e-5), same layout (NORMALISED / RATIO / RESIDUAL)
     const float* g2;         // XTT [n_cond*P][K]: (S_k.G)^2
     double* loss;            // accumulated sum |y - p| (RATIO / RESIDUAL)
+    const float* vmax;       // device scalar: max |voxel value| of vol (1 for the weight pass)
+    float wmul;              // bound of a voxel's splat value per unit |voxel value|
     int code_base;           // proj/sino image of a view = code - code_base (norm uses code)
     int out_mode;
     int n_bands;
@@ -124,6 +124,8 @@ hipError_t launch_sqrt_abs(float* vol, int64_t n, hipStream_t stream);
 hipError_t launch_pca(const float* mu, int K, const Basis& b, int64_t nvox, int64_t chan_stride, float* md,
                       float* evalue, hipStream_t stream);
 hipError_t launch_fill(float* p, int64_t n, float value, hipStream_t stream);
+// out[0] = max |x| over n floats (out must hold 0 beforehand).
+hipError_t launch_absmax(const float* x, int64_t n, float* out, hipStream_t stream);
 hipError_t launch_clamp_min(float* p, int64_t n, float floor_value, float replacement, hipStream_t stream);
 hipError_t launch_ellipsoids(float* vol, int N, float vs, const double* params_dev, int count, hipStream_t stream);
 

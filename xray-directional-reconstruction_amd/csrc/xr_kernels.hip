@@ -11,9 +11,15 @@
 // Forward projection (ir.cu:1490-1531, 1753-1804 + 1533-1603 + 1616-1666) is
 // voxel-driven, like the reference, but each block OWNS a band of kBandU
 // detector columns of one view: it accumulates the bilinear splat of every voxel
-// that touches the band in LDS (ds_add_f32), then writes the band once with the
+// that touches the band in LDS, then writes the band once with the
 // L/(factor+1e-5) normalisation and the MLEM ratio / ART residual fused in.
-// No global atomics; the sinogram is written exactly once per view.
+// No global atomics; the sinogram is written exactly once per view.  The LDS
+// accumulation is fixed point: each contribution is rounded to a 32-bit integer
+// on a per-launch power-of-two scale (from max |voxel value|) and summed with
+// 64-bit integer LDS atomics (ds_add_u64).  On gfx950 these run at bank rate
+// while FP32 LDS atomics (ds_add_f32) serialise per lane (~20x slower here), and
+// integer sums are exact and order independent: the forward projection is
+// bitwise deterministic.
 //
 // Backprojection (ir.cu:1720-1751, 1807-1850, fdk.cu:62-170) is a gather: a
 // block owns a tile of voxels, loops over all views of the subset, stages the
@@ -21,6 +27,8 @@
 // update (ir.cu:1637-1693) is applied in the same kernel, so each voxel is read
 // and written once per subset.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "xr_internal.hpp"
 
@@ -121,11 +129,120 @@ __device__ double chord_length(const ChordGeom& c, int u, int v, const Dims& d) 
 }
 
 // ---------------------------------------------------------------------------
-// Forward projection: one block = (detector band of kBandU columns, one view).
+// Forward projection: one block = (detector band of kBandU columns, one view,
+// a chunk of detector rows).  kFwdWaves waves walk the volume rows y (round
+// robin), find the z-columns whose footprint touches the band (ballot over 64 x
+// per step), and splat them with lanes = z.  Two columns are in flight per wave
+// and all z chunks of a column are loaded before any is used.
 // ---------------------------------------------------------------------------
+constexpr int kFwdWaves = 8;
+constexpr int kFwdChunks = 8;  // z chunks of 64 loaded per column per step (512 z)
+
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+
+struct FwdCol {
+    ColForms c;
+    float inv, u;
+    int64_t base;  // column base index in the column layout
+    float dx, dy;  // XTT source->voxel vector (x, y parts)
+    int z_beg, z_end;  // z range that can reach this launch's rows (AXIS), else [0, N)
+};
+
+template <bool AXIS, int KIND>
+__device__ __forceinline__ FwdCol fwd_col(const FwdArgs& a, const ViewGeom& g, int xcol, int y) {
+    const Dims& d = a.d;
+    FwdCol f;
+    const float xp = (float)xcol - d.xc, yp = (float)y - d.xc;
+    f.c = col_forms(g, xp, yp);
+    f.inv = AXIS ? rcpf(f.c.nd) : 0.f;
+    f.u = AXIS ? fmaf(f.c.nu, f.inv, d.hu) : 0.f;
+    f.base = ((int64_t)y * d.N + xcol) * d.N;
+    f.dx = (KIND == FWD_XTT) ? fmaf(d.vs, xp, g.d0[0]) : 0.f;
+    f.dy = (KIND == FWD_XTT) ? fmaf(d.vs, yp, g.d0[1]) : 0.f;
+    f.z_beg = 0;
+    f.z_end = d.N;
+    if (AXIS) {
+        // v(z') = hv + (nv + av2 z') inv is linear in z'; a voxel reaches rows
+        // [v0, v0+vb) only if v in [v0 - 0.5, v0 + vb + 0.5).  The range is widened
+        // by two voxels and every voxel is still tested exactly in fwd_splat.
+        const float slope = g.av[2] * f.inv;
+        if (fabsf(slope) > 1e-6f) {
+            const float za = ((float)a.v0 - 0.5f - d.hv - f.c.nv * f.inv) / slope;
+            const float zb = ((float)(a.v0 + a.vb) + 0.5f - d.hv - f.c.nv * f.inv) / slope;
+            const float lo = fminf(za, zb) + d.xc - 2.0f, hi = fmaxf(za, zb) + d.xc + 2.0f;
+            f.z_beg = (int)fmaxf(0.0f, floorf(lo));
+            f.z_end = (int)fminf((float)d.N, ceilf(hi) + 1.0f);
+            if (!(lo == lo) || !(hi == hi)) {
+                f.z_beg = 0;
+                f.z_end = d.N;
+            }
+        }
+    }
+    return f;
+}
+
+// Splat one voxel (z of column f, value val) into the band: fixed point with
+// scale 2^S, 64-bit integer LDS atomics.
+template <bool AXIS>
+__device__ __forceinline__ void fwd_splat(const FwdArgs& a, const ViewGeom& g, const FwdCol& f, int z, float val,
+                                          float scale, lds_u64* acc, int u0) {
+    const Dims& d = a.d;
+    const float zp = (float)z - d.xc;
+    float u, v;
+    if (AXIS) {
+        u = f.u;
+        v = fmaf(fmaf(g.av[2], zp, f.c.nv), f.inv, d.hv);
+    } else {
+        voxel_uv(g, f.c, zp, d, u, v);
+    }
+    int iu, iv;
+    float fu, fv;
+    split(u, iu, fu);
+    split(v, iv, fv);
+    bool ok = z < f.z_end && accept_u(u, d) && accept_v(v, d) && iv >= a.v0 - 1 && iv < a.v0 + a.vb;
+    if (!AXIS) ok = ok && iu >= u0 - 1 && iu < u0 + kBandU;
+    if (ok) {
+        const float vs = val * scale;
+        const float c1 = (1.0f - fu) * fv, c2 = fu * fv;
+        const float c3 = fu * (1.0f - fv), c4 = (1.0f - fu) * (1.0f - fv);
+        lds_u64* p = acc + (iv - a.v0 + 1) * kBandStride + (iu - u0 + 1);
+        const int q4 = __float2int_rn(c4 * vs), q3 = __float2int_rn(c3 * vs);
+        const int q1 = __float2int_rn(c1 * vs), q2 = __float2int_rn(c2 * vs);
+        __hip_atomic_fetch_add(p, (unsigned long long)(long long)q4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(p + 1, (unsigned long long)(long long)q3, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(p + kBandStride, (unsigned long long)(long long)q1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(p + kBandStride + 1, (unsigned long long)(long long)q2, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+// XTT voxel value: sum_k |B x S_k|^2 (S_k.G)^2 mu_k * vs * ratio (ir.cu:1780-1794),
+// with |B x S|^2 = |S|^2 - (d.S)^2/|d|^2 for the source->voxel vector d.
+__device__ __forceinline__ float xtt_value(const FwdArgs& a, const FwdCol& f, const ViewGeom& g, int z,
+                                           const float* g2, const float* mu) {
+    const Dims& d = a.d;
+    const float zp = (float)z - d.xc;
+    const float dz = fmaf(d.vs, zp, g.d0[2]);
+    const float d2 = fmaf(dz, dz, fmaf(f.dy, f.dy, f.dx * f.dx));
+    const float id2 = rcpf(d2);
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < kMaxBasis; k++) {
+        if (k < d.K) {
+            const float ds = fmaf(dz, a.b.S[k][2], fmaf(f.dy, a.b.S[k][1], f.dx * a.b.S[k][0]));
+            const float cr2 = fmaxf(fmaf(-ds * ds, id2, a.b.S2[k]), 0.f);
+            s = fmaf(cr2 * g2[k], mu[k], s);
+        }
+    }
+    return s * (d.vs * d.ratio);
+}
+
 template <int KIND, bool AXIS>
-__global__ __launch_bounds__(256) void k_forward(const FwdArgs a) {
-    extern __shared__ float acc[];
+__global__ __launch_bounds__(64 * kFwdWaves) void k_forward(const FwdArgs a) {
+    extern __shared__ unsigned long long acc_raw[];
+    lds_u64* const acc = (lds_u64*)acc_raw;
     const Dims& d = a.d;
     const int nblk = a.n_bands * a.n_slots;
     const int per_xcd = (nblk + 7) / 8;
@@ -138,9 +255,13 @@ __global__ __launch_bounds__(256) void k_forward(const FwdArgs a) {
     const int cond = code / d.P;
     const ViewGeom g = a.vg[code];
     const int u0 = band * kBandU;
-    const int v0 = a.v0, vb = a.vb;
-    const int rows = vb + 2;
-    for (int i = threadIdx.x; i < rows * kBandStride; i += 256) acc[i] = 0.f;
+    const int rows = a.vb + 2;
+    for (int i = threadIdx.x; i < rows * kBandStride; i += 64 * kFwdWaves) acc[i] = 0ull;
+    // Fixed-point scale: the largest splat term is < 2^30 * 2^-S.
+    int e = 0;
+    (void)frexpf(fmaxf(a.wmul * (*a.vmax), 1e-30f), &e);
+    const int S = min(max(30 - e, -100), 100);
+    const float scale = ldexpf(1.0f, S), unscale = ldexpf(1.0f, -S);
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -150,10 +271,11 @@ __global__ __launch_bounds__(256) void k_forward(const FwdArgs a) {
 #pragma unroll
         for (int k = 0; k < kMaxBasis; k++) g2[k] = (k < d.K) ? a.g2[(int64_t)code * d.K + k] : 0.f;
     }
+    const float scalev = (KIND == FWD_PLAIN) ? d.ratio : 1.f;
     const float zlo = -d.xc, zhi = (float)(d.N - 1) - d.xc;
     const float band_lo = (float)u0 - 0.5f, band_hi = (float)(u0 + kBandU) + 0.5f;
 
-    for (int y = wave; y < d.N; y += 4) {
+    for (int y = wave; y < d.N; y += kFwdWaves) {
         const float yp = (float)y - d.xc;
         for (int x0 = 0; x0 < d.N; x0 += 64) {
             const int x = x0 + lane;
@@ -183,69 +305,59 @@ __global__ __launch_bounds__(256) void k_forward(const FwdArgs a) {
                 }
             }
             unsigned long long m = __ballot(cand);
-            while (m) {
-                const int bit = __ffsll(m) - 1;
-                m &= m - 1;
-                const int xcol = x0 + bit;
-                const float xp = (float)xcol - d.xc;
-                const ColForms c = col_forms(g, xp, yp);
-                const int64_t colbase = ((int64_t)y * d.N + xcol) * d.N;
-                float inv_c = 0.f, u_c = 0.f;
-                if (AXIS) {
-                    inv_c = rcpf(c.nd);
-                    u_c = fmaf(c.nu, inv_c, d.hu);
-                }
-                float dx = 0.f, dy = 0.f;
-                if (KIND == FWD_XTT) {
-                    dx = fmaf(d.vs, xp, g.d0[0]);
-                    dy = fmaf(d.vs, yp, g.d0[1]);
-                }
-                for (int z0 = 0; z0 < d.N; z0 += 64) {
-                    const int z = z0 + lane;
-                    if (z >= d.N) break;
-                    const float zp = (float)z - d.xc;
-                    float u, v;
-                    if (AXIS) {
-                        u = u_c;
-                        v = fmaf(fmaf(g.av[2], zp, c.nv), inv_c, d.hv);
-                    } else {
-                        voxel_uv(g, c, zp, d, u, v);
-                    }
-                    if (!(accept_u(u, d) && accept_v(v, d))) continue;
-                    int iu, iv;
-                    float fu, fv;
-                    split(u, iu, fu);
-                    split(v, iv, fv);
-                    if (!AXIS && (iu < u0 - 1 || iu >= u0 + kBandU)) continue;
-                    if (iv < v0 - 1 || iv >= v0 + vb) continue;
-                    float val;
-                    if (KIND == FWD_WEIGHT) {
-                        val = 1.f;
-                    } else if (KIND == FWD_PLAIN) {
-                        val = vol[colbase + z] * d.ratio;
-                    } else {
-                        // XTT: sum_k |B x S_k|^2 (S_k.G)^2 mu_k, times vs*ratio (ir.cu:1780-1794).
-                        const float dz = fmaf(d.vs, zp, g.d0[2]);
-                        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        const float id2 = rcpf(d2);
-                        float s = 0.f;
+            if (KIND == FWD_XTT) {
+                while (m) {
+                    const int bit = __ffsll(m) - 1;
+                    m &= m - 1;
+                    const FwdCol f = fwd_col<AXIS, KIND>(a, g, x0 + bit, y);
+                    for (int z0 = f.z_beg; z0 < f.z_end; z0 += 64) {
+                        const int z = z0 + lane;
+                        float mu[kMaxBasis];
 #pragma unroll
-                        for (int k = 0; k < kMaxBasis; k++) {
-                            if (k < d.K) {
-                                const float ds = fmaf(dz, a.b.S[k][2], fmaf(dy, a.b.S[k][1], dx * a.b.S[k][0]));
-                                const float cr2 = fmaxf(fmaf(-ds * ds, id2, a.b.S2[k]), 0.f);
-                                s = fmaf(cr2 * g2[k], vol[colbase + z + k * a.chan_stride], s);
+                        for (int k = 0; k < kMaxBasis; k++)
+                            mu[k] = (k < d.K && z < f.z_end) ? vol[f.base + z + k * a.chan_stride] : 0.f;
+                        const float val = xtt_value(a, f, g, z, g2, mu);
+                        fwd_splat<AXIS>(a, g, f, z, val, scale, acc, u0);
+                    }
+                }
+            } else {
+                while (m) {
+                    const int b0 = __ffsll(m) - 1;
+                    m &= m - 1;
+                    const bool two = m != 0;
+                    const int b1 = two ? __ffsll(m) - 1 : b0;
+                    if (two) m &= m - 1;
+                    const FwdCol f0 = fwd_col<AXIS, KIND>(a, g, x0 + b0, y);
+                    const FwdCol f1 = fwd_col<AXIS, KIND>(a, g, x0 + b1, y);
+                    const int zb = min(f0.z_beg, f1.z_beg), ze = max(f0.z_end, f1.z_end);
+                    for (int zg = zb; zg < ze; zg += 64 * kFwdChunks) {
+                        float m0[kFwdChunks], m1[kFwdChunks];
+#pragma unroll
+                        for (int c = 0; c < kFwdChunks; c++) {
+                            const int z = zg + c * 64 + lane;
+                            if (KIND == FWD_WEIGHT) {
+                                m0[c] = 1.f;
+                                m1[c] = 1.f;
+                            } else {
+                                m0[c] = (z < f0.z_end && z >= f0.z_beg) ? vol[f0.base + z] : 0.f;
+                                m1[c] = (two && z < f1.z_end && z >= f1.z_beg) ? vol[f1.base + z] : 0.f;
                             }
                         }
-                        val = s * (d.vs * d.ratio);
+#pragma unroll
+                        for (int c = 0; c < kFwdChunks; c++) {
+                            if (zg + c * 64 >= ze) break;
+                            const int z = zg + c * 64 + lane;
+                            if (z >= f0.z_beg) fwd_splat<AXIS>(a, g, f0, z, m0[c] * scalev, scale, acc, u0);
+                        }
+                        if (two) {
+#pragma unroll
+                            for (int c = 0; c < kFwdChunks; c++) {
+                                if (zg + c * 64 >= ze) break;
+                                const int z = zg + c * 64 + lane;
+                                if (z >= f1.z_beg) fwd_splat<AXIS>(a, g, f1, z, m1[c] * scalev, scale, acc, u0);
+                            }
+                        }
                     }
-                    const float c1 = (1.0f - fu) * fv, c2 = fu * fv;
-                    const float c3 = fu * (1.0f - fv), c4 = (1.0f - fu) * (1.0f - fv);
-                    lds_float* p = (lds_float*)acc + (iv - v0 + 1) * kBandStride + (iu - u0 + 1);
-                    lds_add(p, c4 * val);                    // (iu,   iv)
-                    lds_add(p + 1, c3 * val);                // (iu+1, iv)
-                    lds_add(p + kBandStride, c1 * val);      // (iu,   iv+1)
-                    lds_add(p + kBandStride + 1, c2 * val);  // (iu+1, iv+1)
                 }
             }
         }
@@ -256,11 +368,11 @@ __global__ __launch_bounds__(256) void k_forward(const FwdArgs a) {
     const int64_t img = (int64_t)(code - a.code_base) * d.V * d.U;
     const int64_t nimg = (int64_t)code * d.V * d.U;
     const bool acc_old = (KIND != FWD_WEIGHT) && a.accum && a.accum[slot];
-    for (int i = threadIdx.x; i < vb * kBandU; i += 256) {
+    for (int i = threadIdx.x; i < a.vb * kBandU; i += 64 * kFwdWaves) {
         const int r = i / kBandU, cc = i % kBandU;
-        const int uu = u0 + cc, vv = v0 + r;
+        const int uu = u0 + cc, vv = a.v0 + r;
         if (uu >= d.U || vv >= d.V) continue;
-        const float s = acc[(r + 1) * kBandStride + cc + 1];
+        const float s = (float)(long long)acc[(r + 1) * kBandStride + cc + 1] * unscale;
         const int64_t pix = img + (int64_t)vv * d.U + uu;
         const int64_t npix = nimg + (int64_t)vv * d.U + uu;
         if (a.out_mode == OUT_NORM_TABLE) {
@@ -291,205 +403,296 @@ __global__ __launch_bounds__(256) void k_forward(const FwdArgs a) {
     }
     if (a.out_mode == OUT_RATIO || a.out_mode == OUT_RESIDUAL) {
         // Cross-wave sum through LDS past the band accumulator (8-byte aligned slot).
-        double* red = reinterpret_cast<double*>(acc + (((rows * kBandStride) + 1) & ~1));
+        double* red = reinterpret_cast<double*>(acc_raw + rows * kBandStride);
         lsum = wave_sum(lsum);
         if (lane == 0) red[wave] = lsum;
         __syncthreads();
-        if (threadIdx.x == 0) atomicAdd(a.loss, red[0] + red[1] + red[2] + red[3]);
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kFwdWaves; w++) t += red[w];
+            atomicAdd(a.loss, t);
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Backprojection gather with fused update.  Block = 4 waves; zw waves stack
-// along z (64 lanes = 64 consecutive z each), 4/zw column groups of TCxTC
-// columns along x.  Every thread owns one z and TC*TC columns.
+// Backprojection gather with fused update.  Block = kBwdWaves waves laid out
+// as a gx x gy grid of CPW x CPW column groups; every lane owns ZPL voxels
+// (z = z0 + lane + 64 q) in each of its CPW^2 columns, so the per-column
+// geometry of a view is computed once per ZPL voxels.  The block's detector
+// footprint for each view is staged in LDS, double buffered: the footprint of
+// view s+1 is fetched into registers while view s is accumulated, so there is
+// one barrier per view.  The update (MLEM / ART / FDK / raw) is applied at the
+// end: each voxel is read and written once per subset.
 // ---------------------------------------------------------------------------
+constexpr int kBwdWaves = 16;
+constexpr int kBwdThreads = 64 * kBwdWaves;
+constexpr int kStageBuf = 8192;                    // floats per staging buffer
+constexpr int kStagePer = kStageBuf / kBwdThreads;  // staged floats per thread
+
 template <int KIND>
 struct IsXtt {
     static constexpr bool value = (KIND == BWD_XTT_RAW || KIND == BWD_XTT_MLEM || KIND == BWD_XTT_ART);
 };
 
-template <int KIND, bool AXIS, int TC, int KM>
-__global__ __launch_bounds__(256) void k_backward(const BwdArgs a) {
+struct Foot {
+    int iu_lo, iv_lo, W, H, Ws;
+    bool staged, empty;
+};
+
+// Footprint of the block box on view g: the projective map's extremes over a box
+// are at its corners while den > 0 there (all lanes compute it identically).
+__device__ __forceinline__ Foot footprint(const ViewGeom& g, const Dims& d, float bx0, float bx1, float by0, float by1,
+                                          float bz0, float bz1, int lane) {
+    const int ci = lane & 7;
+    float cu, cv, cd;
+    {
+        const ColForms c = col_forms(g, (ci & 1) ? bx1 : bx0, (ci & 2) ? by1 : by0);
+        const float czp = (ci & 4) ? bz1 : bz0;
+        cd = fmaf(g.ad[2], czp, c.nd);
+        voxel_uv(g, c, czp, d, cu, cv);
+    }
+    float umn = cu, umx = cu, vmn = cv, vmx = cv;
+    int bad = (!(cd > 0.f) || !(fabsf(cu) < 1e7f) || !(fabsf(cv) < 1e7f)) ? 1 : 0;
+    for (int off = 1; off < 8; off <<= 1) {
+        umn = fminf(umn, __shfl_xor(umn, off));
+        umx = fmaxf(umx, __shfl_xor(umx, off));
+        vmn = fminf(vmn, __shfl_xor(vmn, off));
+        vmx = fmaxf(vmx, __shfl_xor(vmx, off));
+        bad |= __shfl_xor(bad, off);
+    }
+    Foot f{0, 0, 0, 0, 0, false, false};
+    if (bad) return f;  // direct (unstaged) reads
+    f.iu_lo = max((int)floorf(umn - 0.5f) - 1, 0);
+    const int iu_hi = min((int)floorf(umx - 0.5f) + 2, d.U - 1);
+    f.iv_lo = max((int)floorf(vmn - 0.5f) - 1, 0);
+    const int iv_hi = min((int)floorf(vmx - 0.5f) + 2, d.V - 1);
+    f.W = iu_hi - f.iu_lo + 1;
+    f.H = iv_hi - f.iv_lo + 1;
+    if (f.W <= 1 || f.H <= 1) {
+        f.empty = true;  // entirely off the detector: nothing is accepted
+        return f;
+    }
+    f.Ws = f.W | 1;
+    f.staged = (f.Ws * f.H <= kStageBuf);
+    return f;
+}
+
+template <int KIND, bool AXIS, int CPW, int ZPL, int KM>
+__global__ __launch_bounds__(kBwdThreads) void k_backward(const BwdArgs a) {
     constexpr bool XTT = IsXtt<KIND>::value;
-    constexpr int NC = TC * TC;
+    constexpr int NC = CPW * CPW;
     constexpr int KA = XTT ? KM : 1;
-    extern __shared__ float stage[];
+    extern __shared__ float stage[];  // 2 x kStageBuf
     const Dims& d = a.d;
     const int o = (blockIdx.x % 8) * a.per_xcd + blockIdx.x / 8;
     if (o >= a.n_tiles) return;
-    // zw waves stacked along z; the remaining 4/zw waves form a gx x gy grid of
-    // TC x TC column groups.
-    const int zw = a.zw, gx = a.gx;
-    const int bcx = TC * gx, bcy = TC * a.gy;  // block columns along x, y
+    const int gx = a.gx;
+    const int bcx = CPW * gx, bcy = CPW * a.gy;  // block columns along x, y
     const int tx = o % a.tiles_x, ty = (o / a.tiles_x) % a.tiles_y, tz = o / (a.tiles_x * a.tiles_y);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int zsub = wave % zw, cg = wave / zw;
-    const int z = tz * 64 * zw + zsub * 64 + lane;
-    const bool zvalid = z < d.N;
-    const int xb = tx * bcx + (cg % gx) * TC, yb = ty * bcy + (cg / gx) * TC;
-    // Box of the whole block (centred coordinates), for the footprint.
+    const int xb = tx * bcx + (wave % gx) * CPW, yb = ty * bcy + (wave / gx) * CPW;
+    const int z0 = tz * 64 * ZPL;
     const float bx0 = (float)(tx * bcx) - d.xc, bx1 = (float)min(tx * bcx + bcx - 1, d.N - 1) - d.xc;
     const float by0 = (float)(ty * bcy) - d.xc, by1 = (float)min(ty * bcy + bcy - 1, d.N - 1) - d.xc;
-    const float bz0 = (float)(tz * 64 * zw) - d.xc, bz1 = (float)min(tz * 64 * zw + 64 * zw - 1, d.N - 1) - d.xc;
-    const float zp = (float)z - d.xc;
+    const float bz0 = (float)z0 - d.xc, bz1 = (float)min(z0 + 64 * ZPL - 1, d.N - 1) - d.xc;
 
-    float tmp[NC][KA], fac[NC][KA];
+    float tmp[NC][ZPL][KA], fac[NC][ZPL][KA];
 #pragma unroll
     for (int c = 0; c < NC; c++)
 #pragma unroll
-        for (int k = 0; k < KA; k++) {
-            tmp[c][k] = 0.f;
-            fac[c][k] = 0.f;
+        for (int q = 0; q < ZPL; q++)
+#pragma unroll
+            for (int k = 0; k < KA; k++) {
+                tmp[c][q][k] = 0.f;
+                fac[c][q][k] = 0.f;
+            }
+
+    // staging of one view's footprint: prefetch into registers, store later
+    float pre[kStagePer];
+    int pidx[kStagePer];
+    auto fetch = [&](const Foot& f, int code) {
+        const float* img = a.proj + (int64_t)(code - a.code_base) * d.V * d.U;
+        const int n = f.W * f.H;
+        const float invW = 1.0f / (float)f.W;
+#pragma unroll
+        for (int k = 0; k < kStagePer; k++) {
+            const int i = threadIdx.x + k * kBwdThreads;
+            int r = (int)((float)i * invW);
+            int c = i - r * f.W;
+            if (c < 0) {
+                r--;
+                c += f.W;
+            } else if (c >= f.W) {
+                r++;
+                c -= f.W;
+            }
+            const bool in = i < n;
+            pidx[k] = in ? r * f.Ws + c : -1;
+            pre[k] = in ? img[(int64_t)(f.iv_lo + r) * d.U + f.iu_lo + c] : 0.f;
         }
+    };
+    auto store = [&](lds_float* buf) {
+#pragma unroll
+        for (int k = 0; k < kStagePer; k++)
+            if (pidx[k] >= 0) buf[pidx[k]] = pre[k];
+    };
+
+    if (a.n_views == 0) return;
+    int code = a.views[0];
+    ViewGeom g = a.vg[code];
+    Foot f = footprint(g, d, bx0, bx1, by0, by1, bz0, bz1, lane);
+    lds_float* const lds = (lds_float*)stage;
+    if (f.staged) {
+        fetch(f, code);
+        store(lds);
+    }
+    __syncthreads();
 
     for (int s = 0; s < a.n_views; s++) {
-        const int code = a.views[s];
-        const ViewGeom g = a.vg[code];
-        const float* img = a.proj + (int64_t)(code - a.code_base) * d.V * d.U;
-        // Footprint of the block box: the projective map's extremes over a box are
-        // at its corners while den > 0 there.
-        const int ci = lane & 7;
-        float cu, cv, cd;
-        {
-            const ColForms c = col_forms(g, (ci & 1) ? bx1 : bx0, (ci & 2) ? by1 : by0);
-            const float czp = (ci & 4) ? bz1 : bz0;
-            cd = fmaf(g.ad[2], czp, c.nd);
-            voxel_uv(g, c, czp, d, cu, cv);
+        lds_float* cur = lds + (s & 1) * kStageBuf;
+        lds_float* nxt = lds + ((s + 1) & 1) * kStageBuf;
+        // prefetch the next view's footprint
+        int ncode = 0;
+        ViewGeom ng;
+        Foot nf{0, 0, 0, 0, 0, false, false};
+        if (s + 1 < a.n_views) {
+            ncode = a.views[s + 1];
+            ng = a.vg[ncode];
+            nf = footprint(ng, d, bx0, bx1, by0, by1, bz0, bz1, lane);
+            if (nf.staged) fetch(nf, ncode);
         }
-        float umn = cu, umx = cu, vmn = cv, vmx = cv, dmn = cd;
-        bool bad = !(cd > 0.f) || !(fabsf(cu) < 1e7f) || !(fabsf(cv) < 1e7f);
-        for (int off = 1; off < 8; off <<= 1) {
-            umn = fminf(umn, __shfl_xor(umn, off));
-            umx = fmaxf(umx, __shfl_xor(umx, off));
-            vmn = fminf(vmn, __shfl_xor(vmn, off));
-            vmx = fmaxf(vmx, __shfl_xor(vmx, off));
-            dmn = fminf(dmn, __shfl_xor(dmn, off));
-            bad = bad || (__shfl_xor((int)bad, off) != 0);
-        }
-        int iu_lo = 0, iv_lo = 0, W = 0, H = 0, Ws = 0;
-        bool staged = false;
-        if (!bad) {
-            iu_lo = max((int)floorf(umn - 0.5f) - 1, 0);
-            const int iu_hi = min((int)floorf(umx - 0.5f) + 2, d.U - 1);
-            iv_lo = max((int)floorf(vmn - 0.5f) - 1, 0);
-            const int iv_hi = min((int)floorf(vmx - 0.5f) + 2, d.V - 1);
-            W = iu_hi - iu_lo + 1;
-            H = iv_hi - iv_lo + 1;
-            if (W <= 1 || H <= 1) continue;  // footprint entirely off the detector: nothing accepted
-            Ws = W | 1;
-            staged = (Ws * H <= kStageCap);
-        }
-        if (staged) {
-            __syncthreads();
-            const int c0 = threadIdx.x & 15, r0 = threadIdx.x >> 4;
-            for (int r = r0; r < H; r += 16) {
-                const float* src = img + (int64_t)(iv_lo + r) * d.U + iu_lo;
-                for (int c = c0; c < W; c += 16) stage[r * Ws + c] = src[c];
+        if (!f.empty) {
+            const float* img = a.proj + (int64_t)(code - a.code_base) * d.V * d.U;
+            float g2[KA];
+            if (XTT) {
+#pragma unroll
+                for (int k = 0; k < KA; k++) g2[k] = (k < d.K) ? a.g2[(int64_t)code * d.K + k] : 0.f;
             }
-            __syncthreads();
-        }
-        float g2[KA];
-        if (XTT) {
 #pragma unroll
-            for (int k = 0; k < KA; k++) g2[k] = (k < d.K) ? a.g2[(int64_t)code * d.K + k] : 0.f;
-        }
-        if (!zvalid) continue;
+            for (int j = 0; j < CPW; j++) {
 #pragma unroll
-        for (int j = 0; j < TC; j++) {
+                for (int i = 0; i < CPW; i++) {
+                    const int x = xb + i, y = yb + j;
+                    if (x >= d.N || y >= d.N) continue;
+                    const int cidx = j * CPW + i;
+                    const float xp = (float)x - d.xc, yp = (float)y - d.xc;
+                    const ColForms c = col_forms(g, xp, yp);
+                    float inv_c = 0.f, u_c = 0.f;
+                    if (AXIS) {
+                        inv_c = rcpf(c.nd);
+                        u_c = fmaf(c.nu, inv_c, d.hu);
+                    }
+                    float wcol = 0.f;
+                    if (KIND == BWD_FDK) {
+                        // vol += Ud^2 * (2 pi/P) * sum c p, Ud = sod/(sod + y_unrotated) (fdk.cu:160-169)
+                        const float Xy = fmaf(d.vs, yp, g.fy);
+                        const float Ud = d.sod / (d.sod + Xy);
+                        wcol = Ud * Ud * d.cfdk;
+                    }
+                    float dx = 0.f, dy = 0.f;
+                    if (XTT) {
+                        dx = fmaf(d.vs, xp, g.d0[0]);
+                        dy = fmaf(d.vs, yp, g.d0[1]);
+                    }
 #pragma unroll
-            for (int i = 0; i < TC; i++) {
-                const int x = xb + i, y = yb + j;
-                if (x >= d.N || y >= d.N) continue;
-                const int cidx = j * TC + i;
-                const float xp = (float)x - d.xc, yp = (float)y - d.xc;
-                const ColForms c = col_forms(g, xp, yp);
-                float u, v;
-                if (AXIS) {
-                    const float inv = rcpf(c.nd);
-                    u = fmaf(c.nu, inv, d.hu);
-                    v = fmaf(fmaf(g.av[2], zp, c.nv), inv, d.hv);
-                } else {
-                    voxel_uv(g, c, zp, d, u, v);
-                }
-                if (!(accept_u(u, d) && accept_v(v, d))) continue;
-                int iu, iv;
-                float fu, fv;
-                split(u, iu, fu);
-                split(v, iv, fv);
-                float r00, r01, r10, r11;  // (iu,iv) (iu+1,iv) (iu,iv+1) (iu+1,iv+1)
-                if (staged) {
-                    const int lu = min(max(iu - iu_lo, 0), W - 2);
-                    const int lv = min(max(iv - iv_lo, 0), H - 2);
-                    const int q = lv * Ws + lu;
-                    r00 = stage[q];
-                    r01 = stage[q + 1];
-                    r10 = stage[q + Ws];
-                    r11 = stage[q + Ws + 1];
-                } else {
-                    // Footprint too large for LDS (tilted scans): read the image directly.
-                    // These loads must not be merged with the LDS reads above into one
-                    // flat access (gfx950 backend rejects that form), hence the distinct
-                    // nontemporal load.
-                    const int64_t q = (int64_t)iv * d.U + iu;
-                    r00 = __builtin_nontemporal_load(img + q);
-                    r01 = __builtin_nontemporal_load(img + q + 1);
-                    r10 = __builtin_nontemporal_load(img + q + d.U);
-                    r11 = __builtin_nontemporal_load(img + q + d.U + 1);
-                }
-                const float c1 = (1.0f - fu) * fv, c2 = fu * fv;
-                const float c3 = fu * (1.0f - fv), c4 = (1.0f - fu) * (1.0f - fv);
-                const float nb = c1 * r10 + c2 * r11 + c3 * r01 + c4 * r00;
-                if (KIND == BWD_FDK) {
-                    // vol += Ud^2 * (2 pi/P) * sum c p, Ud = sod/(sod + y_unrotated) (fdk.cu:160-169)
-                    const float Xy = fmaf(d.vs, yp, g.fy);
-                    const float Ud = d.sod / (d.sod + Xy);
-                    tmp[cidx][0] = fmaf(Ud * Ud * d.cfdk, nb, tmp[cidx][0]);
-                } else if (!XTT) {
-                    tmp[cidx][0] += nb;
-                    fac[cidx][0] += 1.0f;
-                } else {
-                    const float dx = fmaf(d.vs, xp, g.d0[0]), dy = fmaf(d.vs, yp, g.d0[1]);
-                    const float dz = fmaf(d.vs, zp, g.d0[2]);
-                    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const float id2 = rcpf(d2);
+                    for (int q = 0; q < ZPL; q++) {
+                        const int z = z0 + lane + 64 * q;
+                        const float zp = (float)z - d.xc;
+                        float u, v;
+                        if (AXIS) {
+                            u = u_c;
+                            v = fmaf(fmaf(g.av[2], zp, c.nv), inv_c, d.hv);
+                        } else {
+                            voxel_uv(g, c, zp, d, u, v);
+                        }
+                        if (!(z < d.N && accept_u(u, d) && accept_v(v, d))) continue;
+                        int iu, iv;
+                        float fu, fv;
+                        split(u, iu, fu);
+                        split(v, iv, fv);
+                        float r00, r01, r10, r11;  // (iu,iv) (iu+1,iv) (iu,iv+1) (iu+1,iv+1)
+                        if (f.staged) {
+                            const int lu = min(max(iu - f.iu_lo, 0), f.W - 2);
+                            const int lv = min(max(iv - f.iv_lo, 0), f.H - 2);
+                            const int qi = lv * f.Ws + lu;
+                            r00 = cur[qi];
+                            r01 = cur[qi + 1];
+                            r10 = cur[qi + f.Ws];
+                            r11 = cur[qi + f.Ws + 1];
+                        } else {
+                            // Footprint too large for LDS: read the image directly.  These
+                            // loads must not be merged with the LDS reads above into one
+                            // flat access (the gfx950 backend rejects that form), hence the
+                            // distinct nontemporal load.
+                            const int64_t qi = (int64_t)iv * d.U + iu;
+                            r00 = __builtin_nontemporal_load(img + qi);
+                            r01 = __builtin_nontemporal_load(img + qi + 1);
+                            r10 = __builtin_nontemporal_load(img + qi + d.U);
+                            r11 = __builtin_nontemporal_load(img + qi + d.U + 1);
+                        }
+                        const float c1 = (1.0f - fu) * fv, c2 = fu * fv;
+                        const float c3 = fu * (1.0f - fv), c4 = (1.0f - fu) * (1.0f - fv);
+                        const float nb = c1 * r10 + c2 * r11 + c3 * r01 + c4 * r00;
+                        if (KIND == BWD_FDK) {
+                            tmp[cidx][q][0] = fmaf(wcol, nb, tmp[cidx][q][0]);
+                        } else if (!XTT) {
+                            tmp[cidx][q][0] += nb;
+                            fac[cidx][q][0] += 1.0f;
+                        } else {
+                            const float dz = fmaf(d.vs, zp, g.d0[2]);
+                            const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                            const float id2 = rcpf(d2);
 #pragma unroll
-                    for (int k = 0; k < KA; k++) {
-                        if (k < d.K) {
-                            const float ds = fmaf(dz, a.b.S[k][2], fmaf(dy, a.b.S[k][1], dx * a.b.S[k][0]));
-                            const float w = fmaxf(fmaf(-ds * ds, id2, a.b.S2[k]), 0.f) * g2[k];
-                            tmp[cidx][k] = fmaf(w, nb, tmp[cidx][k]);
-                            fac[cidx][k] += w;
+                            for (int k = 0; k < KA; k++) {
+                                if (k < d.K) {
+                                    const float ds = fmaf(dz, a.b.S[k][2], fmaf(dy, a.b.S[k][1], dx * a.b.S[k][0]));
+                                    const float w = fmaxf(fmaf(-ds * ds, id2, a.b.S2[k]), 0.f) * g2[k];
+                                    tmp[cidx][q][k] = fmaf(w, nb, tmp[cidx][q][k]);
+                                    fac[cidx][q][k] += w;
+                                }
+                            }
                         }
                     }
                 }
             }
         }
+        if (s + 1 < a.n_views) {
+            if (nf.staged) store(nxt);
+            code = ncode;
+            g = ng;
+            f = nf;
+        }
+        __syncthreads();
     }
-    if (!zvalid) return;
 #pragma unroll
-    for (int j = 0; j < TC; j++) {
+    for (int j = 0; j < CPW; j++) {
 #pragma unroll
-        for (int i = 0; i < TC; i++) {
+        for (int i = 0; i < CPW; i++) {
             const int x = xb + i, y = yb + j;
             if (x >= d.N || y >= d.N) continue;
-            const int cidx = j * TC + i;
-            const int64_t idx = ((int64_t)y * d.N + x) * d.N + z;
+            const int cidx = j * CPW + i;
 #pragma unroll
-            for (int k = 0; k < KA; k++) {
-                if (XTT && k >= d.K) continue;
-                const int64_t vi = idx + (int64_t)k * a.chan_stride;
-                const float t = tmp[cidx][k], f = fac[cidx][k];
-                if (KIND == BWD_RAW || KIND == BWD_XTT_RAW) {
-                    a.tmp[vi] += t;
-                    a.factor[vi] += f;
-                } else if (KIND == BWD_MLEM || KIND == BWD_XTT_MLEM) {
-                    const float mu = a.vol[vi];
-                    a.vol[vi] = (f == 0.0f) ? 0.0f : mu * t / f;
-                } else if (KIND == BWD_ART || KIND == BWD_XTT_ART) {
-                    a.vol[vi] = a.vol[vi] + a.alpha * t;
-                } else {  // BWD_FDK
-                    a.vol[vi] = a.overwrite ? t : a.vol[vi] + t;
+            for (int q = 0; q < ZPL; q++) {
+                const int z = z0 + lane + 64 * q;
+                if (z >= d.N) continue;
+                const int64_t idx = ((int64_t)y * d.N + x) * d.N + z;
+#pragma unroll
+                for (int k = 0; k < KA; k++) {
+                    if (XTT && k >= d.K) continue;
+                    const int64_t vi = idx + (int64_t)k * a.chan_stride;
+                    const float t = tmp[cidx][q][k], fc = fac[cidx][q][k];
+                    if (KIND == BWD_RAW || KIND == BWD_XTT_RAW) {
+                        a.tmp[vi] += t;
+                        a.factor[vi] += fc;
+                    } else if (KIND == BWD_MLEM || KIND == BWD_XTT_MLEM) {
+                        const float mu = a.vol[vi];
+                        a.vol[vi] = (fc == 0.0f) ? 0.0f : mu * t / fc;
+                    } else if (KIND == BWD_ART || KIND == BWD_XTT_ART) {
+                        a.vol[vi] = a.vol[vi] + a.alpha * t;
+                    } else {  // BWD_FDK
+                        a.vol[vi] = a.overwrite ? t : a.vol[vi] + t;
+                    }
                 }
             }
         }
@@ -582,6 +785,15 @@ __global__ void k_sqrt_abs(float* v, int64_t n) {
     if (i >= n) return;
     const float m = v[i];
     v[i] = (m < 0.0f) ? sqrtf(-m) : sqrtf(m);  // voxelSqrt (ir.cu:1715)
+}
+
+// max |x| over n floats into out[0] (non-negative floats order like their bit patterns).
+__global__ __launch_bounds__(256) void k_absmax(const float* __restrict__ x, int64_t n, float* out) {
+    float m = 0.f;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        m = fmaxf(m, fabsf(x[i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax((unsigned int*)out, __float_as_uint(m));
 }
 
 __global__ void k_fill(float* p, int64_t n, float value) {
@@ -748,8 +960,8 @@ hipError_t launch_forward(const FwdArgs& a, int kind, bool axis, hipStream_t str
     const int nblk = a.n_bands * a.n_slots;
     if (nblk == 0) return hipSuccess;
     const unsigned grid = (unsigned)(8 * ((nblk + 7) / 8));
-    const size_t lds = ((size_t)(((a.vb + 2) * kBandStride + 1) & ~1) + 8) * sizeof(float);
-#define XR_FWD(KD, A) hipLaunchKernelGGL((k_forward<KD, A>), dim3(grid), dim3(256), lds, stream, a)
+    const size_t lds = ((size_t)(a.vb + 2) * kBandStride + kFwdWaves) * sizeof(unsigned long long);
+#define XR_FWD(KD, A) hipLaunchKernelGGL((k_forward<KD, A>), dim3(grid), dim3(64 * kFwdWaves), lds, stream, a)
     if (kind == FWD_WEIGHT) {
         if (axis) XR_FWD(FWD_WEIGHT, true); else XR_FWD(FWD_WEIGHT, false);
     } else if (kind == FWD_PLAIN) {
@@ -764,49 +976,64 @@ hipError_t launch_forward(const FwdArgs& a, int kind, bool axis, hipStream_t str
 hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream) {
     const int N = a.d.N;
     const bool xtt = (kind == BWD_XTT_RAW || kind == BWD_XTT_MLEM || kind == BWD_XTT_ART);
-    // Plain/FDK: 4x4 columns per thread, waves stacked along z (256 z per block
-    // when N > 128).  XTT (2K accumulators per voxel): one column per thread,
-    // waves as a 2x2 column tile, 64 z per block.
-    const int TC = xtt ? 1 : 4;
-    a.zw = xtt ? 1 : ((N > 128) ? 4 : ((N > 64) ? 2 : 1));
-    a.gx = xtt ? 2 : 4 / a.zw;
-    a.gy = xtt ? 2 : 1;
-    a.tiles_x = (N + TC * a.gx - 1) / (TC * a.gx);
-    a.tiles_y = (N + TC * a.gy - 1) / (TC * a.gy);
-    a.tiles_z = (N + 64 * a.zw - 1) / (64 * a.zw);
+    // Plain / FDK: 2x2 columns x ZPL z per lane, 16 waves as 4x4 column groups
+    // (an 8x8-column tile, 64*ZPL z).  XTT (2K accumulators per voxel): one
+    // column x one z per lane, 4x4 columns x 64 z per block.
+    const int CPW = xtt ? 1 : 2;
+    const int zpl = xtt ? 1 : (N > 128 ? 4 : (N > 64 ? 2 : 1));
+    a.gx = 4;
+    a.gy = kBwdWaves / 4;
+    a.zw = 1;
+    a.tiles_x = (N + CPW * a.gx - 1) / (CPW * a.gx);
+    a.tiles_y = (N + CPW * a.gy - 1) / (CPW * a.gy);
+    a.tiles_z = (N + 64 * zpl - 1) / (64 * zpl);
     a.n_tiles = a.tiles_x * a.tiles_y * a.tiles_z;
     a.per_xcd = (a.n_tiles + 7) / 8;
     const unsigned grid = (unsigned)(8 * a.per_xcd);
-#define XR_BWD(KD, A, T, KM) hipLaunchKernelGGL((k_backward<KD, A, T, KM>), dim3(grid), dim3(256), kStageCap * sizeof(float), stream, a)
-#define XR_BWD_AX(KD, T, KM)         \
-    do {                              \
-        if (axis)                     \
-            XR_BWD(KD, true, T, KM);  \
-        else                          \
-            XR_BWD(KD, false, T, KM); \
+    const size_t lds = 2 * kStageBuf * sizeof(float);
+#define XR_BWD(KD, A, C, Z, KM) \
+    hipLaunchKernelGGL((k_backward<KD, A, C, Z, KM>), dim3(grid), dim3(kBwdThreads), lds, stream, a)
+#define XR_BWD_AX(KD, C, Z, KM)         \
+    do {                                \
+        if (axis)                       \
+            XR_BWD(KD, true, C, Z, KM); \
+        else                            \
+            XR_BWD(KD, false, C, Z, KM); \
     } while (0)
-#define XR_BWD_XTT(KD)                             \
-    do {                                           \
-        if (a.d.K <= 4)                            \
-            XR_BWD_AX(KD, 1, 4);                   \
-        else if (a.d.K <= 8)                       \
-            XR_BWD_AX(KD, 1, 8);                   \
-        else if (a.d.K <= 13)                      \
-            XR_BWD_AX(KD, 1, 13);                  \
-        else                                       \
-            XR_BWD_AX(KD, 1, 16);                  \
+#define XR_BWD_PLAIN(KD)             \
+    do {                             \
+        if (zpl == 4)                \
+            XR_BWD_AX(KD, 2, 4, 1);  \
+        else if (zpl == 2)           \
+            XR_BWD_AX(KD, 2, 2, 1);  \
+        else                         \
+            XR_BWD_AX(KD, 2, 1, 1);  \
     } while (0)
+#define XR_BWD_XTT_Z(KD, Z)              \
+    do {                                 \
+        if (a.d.K <= 4)                  \
+            XR_BWD_AX(KD, 1, Z, 4);      \
+        else if (a.d.K <= 8)             \
+            XR_BWD_AX(KD, 1, Z, 8);      \
+        else if (a.d.K <= 13)            \
+            XR_BWD_AX(KD, 1, Z, 13);     \
+        else                             \
+            XR_BWD_AX(KD, 1, Z, 16);     \
+    } while (0)
+#define XR_BWD_XTT(KD) XR_BWD_XTT_Z(KD, 1)
     switch (kind) {
-        case BWD_RAW: XR_BWD_AX(BWD_RAW, 4, 1); break;
-        case BWD_MLEM: XR_BWD_AX(BWD_MLEM, 4, 1); break;
-        case BWD_ART: XR_BWD_AX(BWD_ART, 4, 1); break;
-        case BWD_FDK: XR_BWD_AX(BWD_FDK, 4, 1); break;
+        case BWD_RAW: XR_BWD_PLAIN(BWD_RAW); break;
+        case BWD_MLEM: XR_BWD_PLAIN(BWD_MLEM); break;
+        case BWD_ART: XR_BWD_PLAIN(BWD_ART); break;
+        case BWD_FDK: XR_BWD_PLAIN(BWD_FDK); break;
         case BWD_XTT_RAW: XR_BWD_XTT(BWD_XTT_RAW); break;
         case BWD_XTT_MLEM: XR_BWD_XTT(BWD_XTT_MLEM); break;
         case BWD_XTT_ART: XR_BWD_XTT(BWD_XTT_ART); break;
         default: return hipErrorInvalidValue;
     }
 #undef XR_BWD_XTT
+#undef XR_BWD_XTT_Z
+#undef XR_BWD_PLAIN
 #undef XR_BWD_AX
 #undef XR_BWD
     return hipGetLastError();
@@ -851,6 +1078,13 @@ hipError_t launch_pca(const float* mu, int K, const Basis& b, int64_t nvox, int6
                       float* evalue, hipStream_t stream) {
     if (nvox == 0) return hipSuccess;
     hipLaunchKernelGGL(k_pca, dim3(grid1(nvox, 256)), dim3(256), 0, stream, mu, K, b, nvox, chan_stride, md, evalue);
+    return hipGetLastError();
+}
+
+hipError_t launch_absmax(const float* x, int64_t n, float* out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_absmax, dim3((unsigned)blocks), dim3(256), 0, stream, x, n, out);
     return hipGetLastError();
 }
 

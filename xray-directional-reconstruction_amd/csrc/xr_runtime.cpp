@@ -27,6 +27,7 @@ struct xr_context {
     uint8_t* d_flags = nullptr;  // per-call accumulate flags
     size_t flags_cap = 0;
     double* d_loss = nullptr;
+    float* d_scal = nullptr;  // [0] = 1.0f, [1] = max |voxel value| of the last forward launch
     bool timing = false;
     struct Ev {
         int kind;
@@ -48,6 +49,7 @@ struct xr_plan {
     float* d_norm = nullptr;  // [n_cond][P][V][U]: L/(factor+1e-5)
     Dims dims{};
     Basis bas{};
+    float xtt_wsum = 0.f;  // sum_k |S_k|^2 max_view (S_k.G)^2: XTT splat value per unit |mu|
 };
 
 namespace {
@@ -260,17 +262,31 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
     a.out_mode = out_mode;
     a.n_bands = (p->g.n_det_u + kBandU - 1) / kBandU;
     a.n_slots = n_slots;
+    // Fixed-point scale of the LDS accumulation: bound of a voxel's splat value.
+    if (kind == FWD_WEIGHT) {
+        a.vmax = ctx->d_scal;  // 1.0
+        a.wmul = 1.0f;
+    } else {
+        const int64_t nvals = (kind == FWD_XTT) ? N3 * p->n_basis : (vol_cond_stride ? N3 * p->n_cond : N3);
+        XR_HIP(ctx, hipMemsetAsync(ctx->d_scal + 1, 0, sizeof(float), ctx->stream));
+        XR_TRY(timed(ctx, 3, [&] { return launch_absmax(vol, nvals, ctx->d_scal + 1, ctx->stream); }));
+        a.vmax = ctx->d_scal + 1;
+        a.wmul = (kind == FWD_XTT) ? p->dims.vs * p->dims.ratio * p->xtt_wsum * 1.0001f : p->dims.ratio * 1.0001f;
+    }
     // All conditions in one launch share the axis specialisation only if all are axis-aligned.
     bool axis = true;
     for (int c = 0; c < p->n_cond; c++) axis = axis && p->axis_cond[c];
-    // Detector rows per launch bounded so the band accumulator (kBandStride floats
+    // Detector rows per launch: the int64 band accumulator (kBandStride entries
     // per row, two guard rows) stays within 64 KB of LDS; taller detectors are
-    // processed in row chunks (each chunk re-walks the voxels it needs).
-    const int max_rows = (64 * 1024 / (int)sizeof(float) - 16) / kBandStride - 2;
+    // split into equal row chunks (untilted scans walk only the z range that
+    // reaches a chunk's rows).
     const int V = p->g.n_det_v;
-    for (int v0 = 0; v0 < V; v0 += max_rows) {
+    const int max_rows = (64 * 1024 / 8 - 16) / kBandStride - 2;
+    const int n_chunks = (V + max_rows - 1) / max_rows;
+    const int rows = (V + n_chunks - 1) / n_chunks;
+    for (int v0 = 0; v0 < V; v0 += rows) {
         a.v0 = v0;
-        a.vb = std::min(max_rows, V - v0);
+        a.vb = std::min(rows, V - v0);
         XR_TRY(timed(ctx, 0, [&] { return launch_forward(a, kind, axis, ctx->stream); }));
     }
     return XR_OK;
@@ -504,9 +520,18 @@ int xr_context_create(xr_context** out, int device) {
     auto* c = new (std::nothrow) xr_context();
     if (!c) return XR_ERR_ALLOC;
     c->device = device;
-    if (hipMalloc((void**)&c->d_loss, sizeof(double)) != hipSuccess) {
+    if (hipMalloc((void**)&c->d_loss, sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&c->d_scal, 2 * sizeof(float)) != hipSuccess) {
+        if (c->d_loss) (void)hipFree(c->d_loss);
         delete c;
         return XR_ERR_ALLOC;
+    }
+    const float one[2] = {1.0f, 0.0f};
+    if (hipMemcpy(c->d_scal, one, sizeof(one), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(c->d_loss);
+        (void)hipFree(c->d_scal);
+        delete c;
+        return XR_ERR_HIP;
     }
     *out = c;
     return XR_OK;
@@ -524,6 +549,7 @@ void xr_context_destroy(xr_context* ctx) {
     if (ctx->d_codes) (void)hipFree(ctx->d_codes);
     if (ctx->d_flags) (void)hipFree(ctx->d_flags);
     if (ctx->d_loss) (void)hipFree(ctx->d_loss);
+    if (ctx->d_scal) (void)hipFree(ctx->d_scal);
     delete ctx;
 }
 
@@ -636,6 +662,13 @@ int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int 
     }
     d.sod = g->sod;
     d.cfdk = 2.0f * (float)M_PI / (float)P;
+    for (int k = 0; k < s->n_basis; k++) {
+        float g2max = 0.f;
+        for (size_t i = 0; i < (size_t)C * P; i++) g2max = std::max(g2max, g2[i * K + k]);
+        const double s2 = (double)p->basis[3 * k] * p->basis[3 * k] + (double)p->basis[3 * k + 1] * p->basis[3 * k + 1] +
+                          (double)p->basis[3 * k + 2] * p->basis[3 * k + 2];
+        p->xtt_wsum += (float)(s2 * g2max);
+    }
     for (int k = 0; k < s->n_basis; k++) {
         for (int i = 0; i < 3; i++) p->bas.S[k][i] = p->basis[3 * k + i];
         p->bas.S2[k] = p->bas.S[k][0] * p->bas.S[k][0] + p->bas.S[k][1] * p->bas.S[k][1] +
