@@ -172,6 +172,12 @@ int xr_volume_sqrt(xr_context* ctx, float* vol, int64_t n);
 int xr_pca(xr_context* ctx, const float* mu, int n_basis, const float* basis_host, int64_t nvox, int64_t chan_stride,
            float* md, float* evalue);
 
+/* calcEigenVector on host buffers (reference layout or any layout shared by the
+ * channels): mu_host has n_basis channels of nvox floats; md_host and
+ * evalue_host receive 3 channels of nvox floats each.  Computed on the GPU. */
+int xr_pca_host(xr_context* ctx, const float* mu_host, int n_basis, const float* basis_host, int64_t nvox,
+                float* md_host, float* evalue_host);
+
 /* ---- host-buffer solvers (the reference's host API) ------------------- */
 
 /* Per-epoch loss sums (raw sums of |y-p|; the reference writes them divided by

@@ -853,6 +853,23 @@ int xr_pca(xr_context* ctx, const float* mu, int n_basis, const float* basis_hos
     return timed(ctx, 3, [&] { return launch_pca(mu, n_basis, b, nvox, chan_stride, md, evalue, ctx->stream); });
 }
 
+int xr_pca_host(xr_context* ctx, const float* mu_host, int n_basis, const float* basis_host, int64_t nvox,
+                float* md_host, float* evalue_host) {
+    if (!ctx || !mu_host || !basis_host || !md_host || !evalue_host || n_basis <= 0 || n_basis > kMaxBasis || nvox < 0)
+        return fail(ctx, XR_ERR_INVALID, "bad args");
+    if (nvox == 0) return XR_OK;
+    DevBuf<float> dmu, dmd, dev;
+    XR_HIP(ctx, dmu.alloc((size_t)n_basis * nvox));
+    XR_HIP(ctx, dmd.alloc((size_t)3 * nvox));
+    XR_HIP(ctx, dev.alloc((size_t)3 * nvox));
+    XR_HIP(ctx, hipMemcpyAsync(dmu.p, mu_host, sizeof(float) * n_basis * nvox, hipMemcpyHostToDevice, ctx->stream));
+    XR_TRY(xr_pca(ctx, dmu.p, n_basis, basis_host, nvox, nvox, dmd.p, dev.p));
+    XR_HIP(ctx, hipMemcpyAsync(md_host, dmd.p, sizeof(float) * 3 * nvox, hipMemcpyDeviceToHost, ctx->stream));
+    XR_HIP(ctx, hipMemcpyAsync(evalue_host, dev.p, sizeof(float) * 3 * nvox, hipMemcpyDeviceToHost, ctx->stream));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XR_OK;
+}
+
 // -------------------------------------------------------------- solvers ----
 
 // Subset structure of reconstruct.cu:61-65, 89: subset s holds views
