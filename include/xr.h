@@ -247,6 +247,35 @@ int xr_phantom_ellipsoid_params(uint64_t seed, int count, double fov_radius, dou
 int xr_phantom_rasterise(xr_context* ctx, const double* params_host, int count, int n_vox, float vox_size,
                          float* vol_ref_dev);
 
+/* Hollow z-axis cylinder (all z) added into a reference-layout device volume:
+ * value where r_out - thickness < r <= r_out (mm), voxel centres (x-(N+1)/2)*vs. */
+int xr_phantom_cylinder_shell(xr_context* ctx, int n_vox, float vox_size, double r_out, double thickness,
+                              double value, float* vol_ref_dev);
+
+/* Seeded fibre bundles (8 doubles each: centre[3] mm, unit direction[3],
+ * radius mm, half-length mm): n_axes bundle axes uniform on the sphere
+ * (axes_out, 3 doubles each); fibre e follows axis e % n_axes with a direction
+ * drawn from a von Mises-Fisher distribution of concentration kappa (Wood's
+ * method); centres uniform in a ball of radius centre_frac*fov_radius; radius
+ * uniform in [r_lo, r_hi] mm.  SplitMix64 stream of `seed`.  Host only. */
+int xr_phantom_fibre_params(uint64_t seed, int count, int n_axes, double kappa, double fov_radius,
+                            double centre_frac, double r_lo, double r_hi, double half_length, double* params_out,
+                            double* axes_out);
+/* Rasterise fibres into n_basis reference-layout channels (device): inside a
+ * fibre of direction f, channel k gains |S_k x f|^2 (values add). */
+int xr_phantom_fibres(xr_context* ctx, const double* params_host, int count, int n_vox, float vox_size,
+                      const float* basis_host, int n_basis, float* vol_ref_dev);
+
+/* Noise on device arrays of n floats (y may alias the input):
+ * Poisson: y = -ln(max(c, 1)/I0), c ~ Poisson(I0 exp(-p));
+ * Gaussian visibility: y = x (1 + sigma N(0, 1)).
+ * Counter-based (Philox4x32-10 keyed by seed, counter = element index): the
+ * result does not depend on launch shape. */
+int xr_noise_poisson(xr_context* ctx, float* y_dev, const float* p_dev, int64_t n, double I0, uint64_t seed);
+int xr_noise_gaussian(xr_context* ctx, float* y_dev, const float* x_dev, int64_t n, double sigma, uint64_t seed);
+/* The Philox4x32-10 block function (host), for known-answer tests. */
+void xr_philox4x32(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -877,6 +877,124 @@ void orc_phantom_ellipsoids(int N, double vs, const double* params, int count, f
             }
 }
 
+// Philox4x32-10 block function (Salmon et al., SC 2011), restated.
+static void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; r++) {
+        const uint64_t a = (uint64_t)0xD2511F53u * c[0], b = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(b >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(a >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)b;
+        c[3] = (uint32_t)a;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Uniforms of element idx: pairs of 32-bit words -> 53-bit doubles in (0,1).
+struct OrcUniforms {
+    uint64_t seed, idx;
+    uint32_t draw = 0, w[4] = {0, 0, 0, 0};
+    int left = 0;
+    OrcUniforms(uint64_t s, uint64_t i) : seed(s), idx(i) {}
+    double next() {
+        if (!left) {
+            w[0] = (uint32_t)idx;
+            w[1] = (uint32_t)(idx >> 32);
+            w[2] = draw++;
+            w[3] = 0;
+            philox10(w, (uint32_t)seed, (uint32_t)(seed >> 32));
+            left = 2;
+        }
+        const uint32_t a = left == 2 ? w[0] : w[2], b = left == 2 ? w[1] : w[3];
+        left--;
+        const uint64_t m = (((uint64_t)a << 21) ^ (uint64_t)b) & ((1ull << 53) - 1);
+        return ((double)m + 0.5) / 9007199254740992.0;
+    }
+};
+
+// Poisson: multiplication method below 10, PTRS (Hoermann 1993) above.
+static double orc_poisson(double lam, OrcUniforms& u) {
+    if (!(lam > 0)) return 0;
+    if (lam < 10) {
+        const double L = std::exp(-lam);
+        double p = 1, k = 0;
+        for (;;) {
+            p *= u.next();
+            if (p <= L) return k;
+            k += 1;
+        }
+    }
+    const double sl = std::sqrt(lam), ll = std::log(lam), b = 0.931 + 2.53 * sl, a = -0.059 + 0.02483 * b;
+    const double ia = 1.1239 + 1.1328 / (b - 3.4), vr = 0.9277 - 3.6224 / (b - 2.0);
+    for (;;) {
+        const double U = u.next() - 0.5, V = u.next(), us = 0.5 - std::fabs(U);
+        const double k = std::floor((2 * a / us + b) * U + lam + 0.43);
+        if (us >= 0.07 && V <= vr) return k;
+        if (k < 0 || (us < 0.013 && V > us)) continue;
+        if (std::log(V) + std::log(ia) - std::log(a / (us * us) + b) <= -lam + k * ll - std::lgamma(k + 1)) return k;
+    }
+}
+
+void orc_noise_poisson(float* y, const float* p, int64_t n, double I0, uint64_t seed) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        OrcUniforms u{seed, (uint64_t)i};
+        const double c = orc_poisson(I0 * std::exp(-(double)p[i]), u);
+        y[i] = (float)(-std::log(std::max(c, 1.0) / I0));
+    }
+}
+
+void orc_noise_gaussian(float* y, const float* x, int64_t n, double sigma, uint64_t seed) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        OrcUniforms u{seed, (uint64_t)i};
+        const double u1 = u.next(), u2 = u.next();
+        const double g = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+        y[i] = (float)((double)x[i] * (1.0 + sigma * g));
+    }
+}
+
+// Hollow z cylinder added into vol (reference layout), voxel x at (x-(N+1)/2)*vs.
+void orc_phantom_cylinder_shell(int N, double vs, double r_out, double thick, double value, float* vol) {
+    for (int z = 0; z < N; z++)
+        for (int y = 0; y < N; y++)
+            for (int x = 0; x < N; x++) {
+                const double px = (x - (N + 1) * 0.5) * vs, py = (y - (N + 1) * 0.5) * vs;
+                const double r = std::sqrt(px * px + py * py);
+                if (r <= r_out && r > r_out - thick) vol[x + (int64_t)N * y + (int64_t)N * N * z] += (float)value;
+            }
+}
+
+// Fibres (8 doubles each: centre[3], unit dir[3], radius, half-length): channel k
+// gains |S_k x f|^2 inside each fibre.  vol: K channels of N^3 (reference layout).
+void orc_phantom_fibres(int N, double vs, const double* prm, int count, const float* basis, int K, float* vol) {
+    const int64_t N3 = (int64_t)N * N * N;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; z++)
+        for (int y = 0; y < N; y++)
+            for (int x = 0; x < N; x++) {
+                const double p[3] = {(x - (N + 1) * 0.5) * vs, (y - (N + 1) * 0.5) * vs, (z - (N + 1) * 0.5) * vs};
+                std::vector<float> acc(K, 0.f);
+                for (int e = 0; e < count; e++) {
+                    const double* q = prm + 8 * e;
+                    const double d[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+                    const double t = d[0] * q[3] + d[1] * q[4] + d[2] * q[5];
+                    if (std::fabs(t) > q[7]) continue;
+                    const double r[3] = {d[0] - t * q[3], d[1] - t * q[4], d[2] - t * q[5]};
+                    if (r[0] * r[0] + r[1] * r[1] + r[2] * r[2] > q[6] * q[6]) continue;
+                    for (int k = 0; k < K; k++) {
+                        const double S[3] = {basis[3 * k], basis[3 * k + 1], basis[3 * k + 2]};
+                        const double c[3] = {S[1] * q[5] - S[2] * q[4], S[2] * q[3] - S[0] * q[5],
+                                             S[0] * q[4] - S[1] * q[3]};
+                        acc[k] += (float)(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+                    }
+                }
+                const int64_t i = x + (int64_t)N * y + (int64_t)N * N * z;
+                for (int k = 0; k < K; k++) vol[i + k * N3] += acc[k];
+            }
+}
+
 int orc_num_threads(void) { return n_threads(); }
 
 }  // extern "C"

@@ -58,6 +58,10 @@ def _load():
         "orc_eig3": (None, [_d, _d, _d]),
         "orc_phantom_ellipsoids": (None, [I, ctypes.c_double, _d, I, _f]),
         "orc_num_threads": (I, []),
+        "orc_noise_poisson": (None, [_f, _f, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64]),
+        "orc_noise_gaussian": (None, [_f, _f, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64]),
+        "orc_phantom_cylinder_shell": (None, [I, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _f]),
+        "orc_phantom_fibres": (None, [I, ctypes.c_double, _d, I, _f, I, _f]),
     }
     for n, (r, a) in sig.items():
         fn = getattr(lib, n)
@@ -225,6 +229,35 @@ def phantom_ellipsoids(N, vs, params):
     vol = np.zeros(N ** 3, dtype=np.float32)
     _lib.orc_phantom_ellipsoids(N, vs, _dp(p), len(p) // 16, _fp(vol))
     return vol.reshape(N, N, N)
+
+
+def noise_poisson(p, I0, seed):
+    p = np.ascontiguousarray(p, dtype=np.float32).reshape(-1)
+    y = np.zeros_like(p)
+    _lib.orc_noise_poisson(_fp(y), _fp(p), p.size, I0, seed)
+    return y
+
+
+def noise_gaussian(x, sigma, seed):
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+    y = np.zeros_like(x)
+    _lib.orc_noise_gaussian(_fp(y), _fp(x), x.size, sigma, seed)
+    return y
+
+
+def phantom_cylinder_shell(N, vs, r_out, thick, value, vol=None):
+    vol = np.zeros(N ** 3, np.float32) if vol is None else np.ascontiguousarray(vol, np.float32).reshape(-1)
+    _lib.orc_phantom_cylinder_shell(N, vs, r_out, thick, value, _fp(vol))
+    return vol.reshape(N, N, N)
+
+
+def phantom_fibres(N, vs, params, basis):
+    p = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(basis, dtype=np.float32).reshape(-1)
+    K = len(b) // 3
+    vol = np.zeros(K * N ** 3, np.float32)
+    _lib.orc_phantom_fibres(N, vs, _dp(p), len(p) // 8, _fp(b), K, _fp(vol))
+    return vol.reshape(K, N, N, N)
 
 
 def num_threads():

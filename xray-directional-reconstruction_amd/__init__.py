@@ -113,6 +113,14 @@ def _load():
                                      ctypes.POINTER(ScanStruct)]),
         "xr_phantom_ellipsoid_params": (I, [ctypes.c_uint64, I, D, D, D, D, D, D, _c_double_p]),
         "xr_phantom_rasterise": (I, [P, _c_double_p, I, I, F, P]),
+        "xr_pca_host": (I, [P, _c_float_p, I, _c_float_p, ctypes.c_int64, _c_float_p, _c_float_p]),
+        "xr_phantom_cylinder_shell": (I, [P, I, F, D, D, D, P]),
+        "xr_phantom_fibre_params": (I, [ctypes.c_uint64, I, I, D, D, D, D, D, D, _c_double_p, _c_double_p]),
+        "xr_phantom_fibres": (I, [P, _c_double_p, I, I, F, _c_float_p, I, P]),
+        "xr_noise_poisson": (I, [P, P, P, ctypes.c_int64, D, ctypes.c_uint64]),
+        "xr_noise_gaussian": (I, [P, P, P, ctypes.c_int64, D, ctypes.c_uint64]),
+        "xr_philox4x32": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                 ctypes.POINTER(ctypes.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -243,6 +251,30 @@ class Context:
         b = np.ascontiguousarray(basis, dtype=np.float32).reshape(-1)
         self.check(_lib.xr_pca(self.h, _dptr(mu), len(b) // 3, _fptr(b), nvox, chan_stride, _dptr(md), _dptr(evalue)))
 
+    def phantom_cylinder_shell(self, n_vox, vox_size, r_out, thickness, value, vol_ref):
+        self.check(_lib.xr_phantom_cylinder_shell(self.h, n_vox, vox_size, r_out, thickness, value, _dptr(vol_ref)))
+
+    def phantom_fibres(self, params, n_vox, vox_size, basis, vol_ref):
+        p = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(basis, dtype=np.float32).reshape(-1)
+        self.check(_lib.xr_phantom_fibres(self.h, p.ctypes.data_as(_c_double_p), len(p) // 8, n_vox, vox_size,
+                                          _fptr(b), len(b) // 3, _dptr(vol_ref)))
+
+    def noise_poisson(self, y, p, I0, seed):
+        self.check(_lib.xr_noise_poisson(self.h, _dptr(y), _dptr(p), p.numel(), I0, seed))
+
+    def noise_gaussian(self, y, x, sigma, seed):
+        self.check(_lib.xr_noise_gaussian(self.h, _dptr(y), _dptr(x), x.numel(), sigma, seed))
+
+    def pca_host(self, mu, basis):
+        mu = np.ascontiguousarray(mu, dtype=np.float32)
+        K, nv = mu.shape
+        b = np.ascontiguousarray(basis, dtype=np.float32).reshape(-1)
+        md = np.zeros(3 * nv, np.float32)
+        ev = np.zeros(3 * nv, np.float32)
+        self.check(_lib.xr_pca_host(self.h, _fptr(mu), K, _fptr(b), nv, _fptr(md), _fptr(ev)))
+        return md.reshape(3, nv), ev.reshape(3, nv)
+
     def phantom_rasterise(self, params, n_vox, vox_size, vol_ref):
         p = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
         self.check(_lib.xr_phantom_rasterise(self.h, p.ctypes.data_as(_c_double_p), len(p) // 16, n_vox,
@@ -356,6 +388,27 @@ def load_settings(path, tag):
                 cond_offset=np.ctypeslib.as_array(st.cond_offset)[: 3 * st.n_cond].copy(),
                 basis=np.ctypeslib.as_array(st.basis)[: 3 * st.n_basis].copy() if st.n_basis else None)
     return st, g, scan
+
+
+def fibre_params(seed, count, n_axes=3, kappa=20.0, fov_radius=1.0, centre_frac=0.6, r_lo=0.4, r_hi=1.2,
+                 half_length=None):
+    """Fibre bundles: returns (params (count, 8), axes (n_axes, 3))."""
+    out = np.zeros(8 * max(count, 1), dtype=np.float64)
+    axes = np.zeros(3 * n_axes, dtype=np.float64)
+    hl = fov_radius if half_length is None else half_length
+    rc = _lib.xr_phantom_fibre_params(seed, count, n_axes, kappa, fov_radius, centre_frac, r_lo, r_hi, hl,
+                                      out.ctypes.data_as(_c_double_p), axes.ctypes.data_as(_c_double_p))
+    if rc != XR_OK:
+        raise XRError("xr_phantom_fibre_params failed (%d)" % rc)
+    return out[: 8 * count].reshape(count, 8), axes.reshape(n_axes, 3)
+
+
+def philox4x32(counter, key):
+    c = (ctypes.c_uint32 * 4)(*counter)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    _lib.xr_philox4x32(c, k, o)
+    return list(o)
 
 
 def ellipsoid_params(seed, count, fov_radius, centre_frac=0.6, axis_lo=0.1, axis_hi=0.35, value_lo=0.0,

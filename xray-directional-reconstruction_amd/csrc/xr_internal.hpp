@@ -128,5 +128,15 @@ hipError_t launch_fill(float* p, int64_t n, float value, hipStream_t stream);
 hipError_t launch_absmax(const float* x, int64_t n, float* out, hipStream_t stream);
 hipError_t launch_clamp_min(float* p, int64_t n, float floor_value, float replacement, hipStream_t stream);
 hipError_t launch_ellipsoids(float* vol, int N, float vs, const double* params_dev, int count, hipStream_t stream);
+hipError_t launch_cylinder_shell(float* vol, int N, float vs, double r_out, double thick, double value,
+                                 hipStream_t stream);
+// Fibre phantom: K channels of mu_k = s |S_k x f|^2 inside each fibre (values add).
+hipError_t launch_fibres(float* vol, int N, float vs, const double* params_dev, int count, const Basis& b, int K,
+                         hipStream_t stream);
+// y = -ln(max(c, 1)/I0), c ~ Poisson(I0 exp(-p)) (Philox4x32-10 stream of (seed, index)).
+hipError_t launch_poisson_noise(float* y, const float* p, int64_t n, double I0, uint64_t seed, hipStream_t stream);
+// y = x (1 + sigma N(0,1)) (Box-Muller on Philox4x32-10 of (seed, index)).
+hipError_t launch_gaussian_noise(float* y, const float* x, int64_t n, double sigma, uint64_t seed,
+                                 hipStream_t stream);
 
 }  // namespace xr

@@ -31,6 +31,7 @@
 #include <algorithm>
 
 #include "xr_internal.hpp"
+#include "xr_random.hpp"
 
 namespace xr {
 
@@ -952,6 +953,63 @@ __global__ void k_ellipsoids(float* vol, int N, double vs, const double* __restr
     vol[x + (int64_t)N * y + (int64_t)N * N * z] = (float)acc;
 }
 
+// Hollow cylinder about the z axis (all z): value where r_out - thick < r <= r_out,
+// voxel centres (x-(N+1)/2)*vs as for the ellipsoids.  Adds into vol (reference layout).
+__global__ void k_cylinder_shell(float* vol, int N, double vs, double r_out, double thick, double value) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= N) return;
+    const double px = ((double)x - (N + 1) * 0.5) * vs, py = ((double)y - (N + 1) * 0.5) * vs;
+    const double r = sqrt(px * px + py * py);
+    if (r <= r_out && r > r_out - thick) vol[x + (int64_t)N * y + (int64_t)N * N * z] += (float)value;
+}
+
+// Fibres: params 8 doubles each (centre[3] mm, unit direction f[3], radius mm,
+// half-length mm).  Inside a fibre every channel k gains s_k = |S_k x f|^2
+// (scattering strength 1 per unit direction-weight; values add).  Reference
+// layout, K channels of N^3.
+__global__ void k_fibres(float* vol, int N, double vs, const double* __restrict__ prm, int count, const Basis b,
+                         int K) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= N) return;
+    const double p0 = ((double)x - (N + 1) * 0.5) * vs, p1 = ((double)y - (N + 1) * 0.5) * vs,
+                 p2 = ((double)z - (N + 1) * 0.5) * vs;
+    float acc[kMaxBasis];
+    for (int k = 0; k < kMaxBasis; k++) acc[k] = 0.f;
+    for (int e = 0; e < count; e++) {
+        const double* q = prm + 8 * e;
+        const double d0 = p0 - q[0], d1 = p1 - q[1], d2 = p2 - q[2];
+        const double t = d0 * q[3] + d1 * q[4] + d2 * q[5];
+        if (fabs(t) > q[7]) continue;
+        const double r0 = d0 - t * q[3], r1 = d1 - t * q[4], r2 = d2 - t * q[5];
+        if (r0 * r0 + r1 * r1 + r2 * r2 > q[6] * q[6]) continue;
+        for (int k = 0; k < K; k++) {
+            const double c0 = b.S[k][1] * q[5] - b.S[k][2] * q[4];
+            const double c1 = b.S[k][2] * q[3] - b.S[k][0] * q[5];
+            const double c2 = b.S[k][0] * q[4] - b.S[k][1] * q[3];
+            acc[k] += (float)(c0 * c0 + c1 * c1 + c2 * c2);
+        }
+    }
+    const int64_t i = x + (int64_t)N * y + (int64_t)N * N * z, N3 = (int64_t)N * N * N;
+    for (int k = 0; k < K; k++) vol[i + k * N3] += acc[k];
+}
+
+__global__ void k_poisson_noise(float* y, const float* p, int64_t n, double I0, uint64_t seed) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    UniformStream us(seed, (uint64_t)i);
+    const double c = poisson_sample(I0 * exp(-(double)p[i]), us);
+    y[i] = (float)(-log(fmax(c, 1.0) / I0));
+}
+
+__global__ void k_gaussian_noise(float* y, const float* x, int64_t n, double sigma, uint64_t seed) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    UniformStream us(seed, (uint64_t)i);
+    const double u1 = us.next(), u2 = us.next();
+    const double g = sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);  // Box-Muller
+    y[i] = (float)((double)x[i] * (1.0 + sigma * g));
+}
+
 inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -1103,6 +1161,33 @@ hipError_t launch_clamp_min(float* p, int64_t n, float floor_value, float replac
 hipError_t launch_ellipsoids(float* vol, int N, float vs, const double* params_dev, int count, hipStream_t stream) {
     hipLaunchKernelGGL(k_ellipsoids, dim3(grid1(N, 64), N, N), dim3(64), 0, stream, vol, N, (double)vs, params_dev,
                        count);
+    return hipGetLastError();
+}
+
+hipError_t launch_cylinder_shell(float* vol, int N, float vs, double r_out, double thick, double value,
+                                 hipStream_t stream) {
+    hipLaunchKernelGGL(k_cylinder_shell, dim3(grid1(N, 64), N, N), dim3(64), 0, stream, vol, N, (double)vs, r_out,
+                       thick, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_fibres(float* vol, int N, float vs, const double* params_dev, int count, const Basis& b, int K,
+                         hipStream_t stream) {
+    hipLaunchKernelGGL(k_fibres, dim3(grid1(N, 64), N, N), dim3(64), 0, stream, vol, N, (double)vs, params_dev, count,
+                       b, K);
+    return hipGetLastError();
+}
+
+hipError_t launch_poisson_noise(float* y, const float* p, int64_t n, double I0, uint64_t seed, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_poisson_noise, dim3(grid1(n, 256)), dim3(256), 0, stream, y, p, n, I0, seed);
+    return hipGetLastError();
+}
+
+hipError_t launch_gaussian_noise(float* y, const float* x, int64_t n, double sigma, uint64_t seed,
+                                 hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gaussian_noise, dim3(grid1(n, 256)), dim3(256), 0, stream, y, x, n, sigma, seed);
     return hipGetLastError();
 }
 

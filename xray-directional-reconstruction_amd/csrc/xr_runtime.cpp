@@ -15,6 +15,7 @@
 
 #include "../../include/xr.h"
 #include "xr_internal.hpp"
+#include "xr_random.hpp"
 
 using namespace xr;
 
@@ -1235,6 +1236,100 @@ int xr_phantom_rasterise(xr_context* ctx, const double* params_host, int count, 
     XR_TRY(timed(ctx, 3, [&] { return launch_ellipsoids(vol_ref_dev, n_vox, vox_size, dp.p, count, ctx->stream); }));
     XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return XR_OK;
+}
+
+int xr_phantom_cylinder_shell(xr_context* ctx, int n_vox, float vox_size, double r_out, double thickness,
+                              double value, float* vol_ref_dev) {
+    if (!ctx || n_vox <= 0 || !vol_ref_dev || !(vox_size > 0)) return fail(ctx, XR_ERR_INVALID, "bad args");
+    XR_TRY(timed(ctx, 3, [&] {
+        return launch_cylinder_shell(vol_ref_dev, n_vox, vox_size, r_out, thickness, value, ctx->stream);
+    }));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XR_OK;
+}
+
+int xr_phantom_fibre_params(uint64_t seed, int count, int n_axes, double kappa, double fov_radius,
+                            double centre_frac, double r_lo, double r_hi, double half_length, double* out,
+                            double* axes_out) {
+    if (count < 0 || n_axes <= 0 || !out || !axes_out || !(kappa > 0) || !(fov_radius > 0) || !(r_lo > 0) ||
+        r_hi < r_lo || !(half_length > 0))
+        return XR_ERR_INVALID;
+    SplitMix64 r{seed};
+    for (int a = 0; a < n_axes; a++) {  // uniform on the sphere
+        const double z = 2.0 * r.uniform() - 1.0, ph = 2.0 * M_PI * r.uniform();
+        const double s = std::sqrt(std::max(0.0, 1.0 - z * z));
+        axes_out[3 * a] = s * std::cos(ph);
+        axes_out[3 * a + 1] = s * std::sin(ph);
+        axes_out[3 * a + 2] = z;
+    }
+    for (int e = 0; e < count; e++) {
+        double* q = out + 8 * e;
+        double c[3];
+        do {
+            for (int i = 0; i < 3; i++) c[i] = 2.0 * r.uniform() - 1.0;
+        } while (c[0] * c[0] + c[1] * c[1] + c[2] * c[2] > 1.0);
+        for (int i = 0; i < 3; i++) q[i] = c[i] * centre_frac * fov_radius;
+        // von Mises-Fisher on S^2 about mu (Wood 1994): w = 1 + ln(U + (1-U) e^{-2 kappa}) / kappa
+        const double* mu = axes_out + 3 * (e % n_axes);
+        const double U = r.uniform(), ph = 2.0 * M_PI * r.uniform();
+        const double w = 1.0 + std::log(U + (1.0 - U) * std::exp(-2.0 * kappa)) / kappa;
+        const double sw = std::sqrt(std::max(0.0, 1.0 - w * w));
+        // orthonormal e1, e2 perpendicular to mu
+        double t[3] = {0, 0, 0};
+        const int m = (std::fabs(mu[0]) <= std::fabs(mu[1]) && std::fabs(mu[0]) <= std::fabs(mu[2])) ? 0
+                      : (std::fabs(mu[1]) <= std::fabs(mu[2]))                                           ? 1
+                                                                                                          : 2;
+        t[m] = 1.0;
+        double e1[3] = {mu[1] * t[2] - mu[2] * t[1], mu[2] * t[0] - mu[0] * t[2], mu[0] * t[1] - mu[1] * t[0]};
+        const double n1 = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+        for (int i = 0; i < 3; i++) e1[i] /= n1;
+        const double e2[3] = {mu[1] * e1[2] - mu[2] * e1[1], mu[2] * e1[0] - mu[0] * e1[2],
+                              mu[0] * e1[1] - mu[1] * e1[0]};
+        for (int i = 0; i < 3; i++) q[3 + i] = w * mu[i] + sw * (std::cos(ph) * e1[i] + std::sin(ph) * e2[i]);
+        const double nf = std::sqrt(q[3] * q[3] + q[4] * q[4] + q[5] * q[5]);
+        for (int i = 0; i < 3; i++) q[3 + i] /= nf;
+        q[6] = r_lo + (r_hi - r_lo) * r.uniform();
+        q[7] = half_length;
+    }
+    return XR_OK;
+}
+
+int xr_phantom_fibres(xr_context* ctx, const double* params_host, int count, int n_vox, float vox_size,
+                      const float* basis_host, int n_basis, float* vol_ref_dev) {
+    if (!ctx || (!params_host && count > 0) || count < 0 || n_vox <= 0 || !basis_host || n_basis <= 0 ||
+        n_basis > kMaxBasis || !vol_ref_dev)
+        return fail(ctx, XR_ERR_INVALID, "bad args");
+    Basis b{};
+    for (int k = 0; k < n_basis; k++)
+        for (int i = 0; i < 3; i++) b.S[k][i] = basis_host[3 * k + i];
+    DevBuf<double> dp;
+    XR_HIP(ctx, dp.alloc((size_t)8 * std::max(count, 1)));
+    if (count > 0)
+        XR_HIP(ctx, hipMemcpyAsync(dp.p, params_host, sizeof(double) * 8 * count, hipMemcpyHostToDevice, ctx->stream));
+    XR_TRY(timed(ctx, 3, [&] { return launch_fibres(vol_ref_dev, n_vox, vox_size, dp.p, count, b, n_basis, ctx->stream); }));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XR_OK;
+}
+
+int xr_noise_poisson(xr_context* ctx, float* y_dev, const float* p_dev, int64_t n, double I0, uint64_t seed) {
+    if (!ctx || !y_dev || !p_dev || n < 0 || !(I0 > 0)) return fail(ctx, XR_ERR_INVALID, "bad args");
+    return timed(ctx, 3, [&] { return launch_poisson_noise(y_dev, p_dev, n, I0, seed, ctx->stream); });
+}
+
+int xr_noise_gaussian(xr_context* ctx, float* y_dev, const float* x_dev, int64_t n, double sigma, uint64_t seed) {
+    if (!ctx || !y_dev || !x_dev || n < 0 || sigma < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    return timed(ctx, 3, [&] { return launch_gaussian_noise(y_dev, x_dev, n, sigma, seed, ctx->stream); });
+}
+
+void xr_philox4x32(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c[4] = {counter[0], counter[1], counter[2], counter[3]};
+    uint32_t k[2] = {key[0], key[1]};
+    for (int r = 0; r < 10; r++) {
+        philox_round(c, k);
+        k[0] += 0x9E3779B9u;
+        k[1] += 0xBB67AE85u;
+    }
+    for (int i = 0; i < 4; i++) out[i] = c[i];
 }
 
 }  // extern "C"
