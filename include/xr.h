@@ -99,6 +99,10 @@ void xr_context_destroy(xr_context* ctx);
 /* Use `hip_stream` (a hipStream_t, NULL = null stream) for all further work. */
 int xr_context_set_stream(xr_context* ctx, void* hip_stream);
 int xr_context_synchronize(xr_context* ctx);
+/* Copy `bytes` between any two addresses (host or device) on the context's
+ * stream and wait for it (hipMemcpyDefault).  A convenience for callers that
+ * hold raw device addresses, e.g. the buffers of a batch. */
+int xr_memcpy(xr_context* ctx, void* dst, const void* src, size_t bytes);
 const char* xr_last_error(const xr_context* ctx);
 
 /* Kernel timing: when enabled every launch is bracketed by HIP events on the
@@ -119,6 +123,35 @@ int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int 
 void xr_plan_destroy(xr_plan* plan);
 
 float xr_default_vox_size(float sdd, float sod, float det_size);
+
+/* Projector tile shape of a plan.  No counterpart in the reference (its
+ * kernels launch one thread per voxel column with a fixed blockSize,
+ * params.cu:31).  Zero fields select the defaults.  The tile shape changes
+ * only the work split, never the arithmetic: every voxel still sums the views
+ * of a subset in order and the forward accumulation is exact fixed point, so
+ * results are bitwise identical for every setting.
+ *   bwd_zpl:  z voxels per lane in the backprojection (1, 2, 4; default by N)
+ *   bwd_gx:   column groups along x in a 16-wave backprojection block
+ *             (1, 2, 4, 8, 16; gy = 16/gx; default 4)
+ *   fwd_rows: cap on the detector rows of one forward launch (default: the
+ *             most that fit 64 KB of LDS) */
+typedef struct xr_tiles {
+    int32_t bwd_zpl;
+    int32_t bwd_gx;
+    int32_t fwd_rows;
+} xr_tiles;
+
+int xr_plan_set_tiles(xr_plan* plan, const xr_tiles* tiles);
+int xr_plan_get_tiles(const xr_plan* plan, xr_tiles* tiles);
+
+/* Autotuner: times every candidate tile shape of the plan's forward projection
+ * (OUT_RATIO) and MLEM backprojection on the given device data (column-layout
+ * volume of condition 0, its sinogram) and returns the fastest in `best`
+ * (also set on the plan).  vol_cols and sino are not modified.  best_ms
+ * (optional) receives the forward + backward milliseconds of the winner.
+ * Results are cached in the context per (N, U, V, P, axis). */
+int xr_autotune(xr_context* ctx, xr_plan* plan, const float* vol_cols, const float* sino, xr_tiles* best,
+                double* best_ms);
 
 /* ---- device-level operators ------------------------------------------- */
 
@@ -206,6 +239,46 @@ int xr_xtt_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, 
  * is overwritten with the filtered sinogram, as in the reference. */
 int xr_fdk_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, float* const* sino, float* const* vol,
                        int direction, int filter);
+
+/* ---- batched multi-problem solver ---------------------------------------- */
+
+/* Many independent IR problems (MLEM or ART, each one untilted or tilted
+ * condition with its own geometry, view count and subset count) solved
+ * together: every subset step is one persistent forward launch and one
+ * persistent backprojection launch over all problems, and with
+ * XR_BATCH_GRAPH the whole epoch loop is captured once as a HIP graph and
+ * replayed.  No counterpart in the reference (IR::reconstruct,
+ * reconstruct.cu:23-156, solves one volume per call); each problem's result
+ * equals its own xr_ir_reconstruct solve bit for bit (same subset order: one
+ * default-seeded mt19937_64 per problem, reconstruct.cu:75). */
+typedef struct xr_batch xr_batch;
+#define XR_BATCH_GRAPH 1    /* capture the epoch loop as a HIP graph */
+#define XR_BATCH_AUTOTUNE 2 /* tile shapes from xr_autotune per distinct problem shape */
+
+/* geoms[i], scans[i] (scans may be NULL: one untilted condition, no basis;
+ * otherwise scans[i].n_cond must be 1 and n_basis 0); batch[i]: subsets of
+ * problem i (NULL: 1).  Allocates every device buffer of the batch. */
+int xr_batch_create(xr_context* ctx, int n, const xr_geometry* geoms, const xr_scan* scans, const int32_t* batch,
+                    int direction, int method, float lambda, int flags, xr_batch** out);
+void xr_batch_destroy(xr_batch* b);
+/* Device buffers of problem i, owned by the batch: the column-layout volume
+ * (N^3, initial guess in / result out) and the measured sinogram (U*V*P). */
+int xr_batch_buffers(xr_batch* b, int i, float** vol_cols, float** sino);
+/* The plan of problem i (projection tables; use with xr_project to make data). */
+xr_plan* xr_batch_plan(xr_batch* b, int i);
+/* Tune tile shapes now (per distinct shape) on the current buffers. */
+int xr_batch_autotune(xr_batch* b);
+/* Run `epochs` epochs on every problem.  losses: optional host array of
+ * epochs*n doubles, [epoch][problem] raw sums of |y-p|. */
+int xr_batch_run(xr_batch* b, int epochs, double* losses);
+/* Milliseconds of the last xr_batch_run on the GPU (HIP events around the
+ * epoch loop, excluding the upload of the subset lists). */
+int xr_batch_elapsed_ms(const xr_batch* b, double* ms);
+/* Host-buffer convenience (reference layouts), IR::reconstruct for n problems:
+ * sino[i], vol[i] host arrays; vol[i] in/out. */
+int xr_batch_reconstruct(xr_context* ctx, int n, const xr_geometry* geoms, const xr_scan* scans,
+                         const int32_t* batch, float* const* sino, float* const* vol, int epochs, int direction,
+                         int method, float lambda, int flags, double* losses);
 
 /* ---- settings (init_params, params.cu:42-104) -------------------------- */
 

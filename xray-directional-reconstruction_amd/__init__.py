@@ -23,6 +23,7 @@ METHOD_XTT, METHOD_MLEM, METHOD_ART, METHOD_FDK = 0, 1, 2, 3
 FILTER_REFERENCE, FILTER_RAMLAK = 0, 1
 OUT_RAW, OUT_NORMALISED, OUT_RATIO, OUT_RESIDUAL = 0, 1, 2, 3
 TIMING_FORWARD, TIMING_BACKWARD, TIMING_FILTER, TIMING_OTHER = 0, 1, 2, 3
+BATCH_GRAPH, BATCH_AUTOTUNE = 1, 2
 
 _c_float_p = ctypes.POINTER(ctypes.c_float)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -58,6 +59,15 @@ class Stats(ctypes.Structure):
     _fields_ = [("losses", _c_double_p)]
 
 
+class Tiles(ctypes.Structure):
+    """xr_tiles: projector tile shape (zeros select the defaults)."""
+
+    _fields_ = [("bwd_zpl", ctypes.c_int32), ("bwd_gx", ctypes.c_int32), ("fwd_rows", ctypes.c_int32)]
+
+    def as_tuple(self):
+        return (self.bwd_zpl, self.bwd_gx, self.fwd_rows)
+
+
 class XRError(RuntimeError):
     pass
 
@@ -81,6 +91,7 @@ def _load():
         "xr_context_destroy": (V, [P]),
         "xr_context_set_stream": (I, [P, P]),
         "xr_context_synchronize": (I, [P]),
+        "xr_memcpy": (I, [P, P, P, ctypes.c_size_t]),
         "xr_last_error": (ctypes.c_char_p, [P]),
         "xr_timing_enable": (I, [P, I]),
         "xr_timing_reset": (I, [P]),
@@ -121,6 +132,20 @@ def _load():
         "xr_noise_gaussian": (I, [P, P, P, ctypes.c_int64, D, ctypes.c_uint64]),
         "xr_philox4x32": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
                                  ctypes.POINTER(ctypes.c_uint32)]),
+        "xr_plan_set_tiles": (I, [P, ctypes.POINTER(Tiles)]),
+        "xr_plan_get_tiles": (I, [P, ctypes.POINTER(Tiles)]),
+        "xr_autotune": (I, [P, P, P, P, ctypes.POINTER(Tiles), _c_double_p]),
+        "xr_batch_create": (I, [P, I, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct), _c_int32_p, I, I, F,
+                                I, ctypes.POINTER(ctypes.c_void_p)]),
+        "xr_batch_destroy": (V, [P]),
+        "xr_batch_buffers": (I, [P, I, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]),
+        "xr_batch_plan": (P, [P, I]),
+        "xr_batch_autotune": (I, [P]),
+        "xr_batch_run": (I, [P, I, _c_double_p]),
+        "xr_batch_elapsed_ms": (I, [P, _c_double_p]),
+        "xr_batch_reconstruct": (I, [P, I, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct), _c_int32_p,
+                                     ctypes.POINTER(_c_float_p), ctypes.POINTER(_c_float_p), I, I, I, F, I,
+                                     _c_double_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -223,6 +248,22 @@ class Context:
     def synchronize(self):
         self.check(_lib.xr_context_synchronize(self.h))
 
+    def copy(self, dst, src, nbytes=None):
+        """Synchronous copy between device tensors / raw device addresses / numpy arrays."""
+        def addr(x):
+            if isinstance(x, int):
+                return ctypes.c_void_p(x)
+            if isinstance(x, np.ndarray):
+                if not x.flags.c_contiguous:
+                    raise XRError("expected a contiguous array")
+                return ctypes.c_void_p(x.ctypes.data)
+            return _dptr(x)
+
+        if nbytes is None:
+            ref = src if not isinstance(src, int) else dst
+            nbytes = ref.nbytes if isinstance(ref, np.ndarray) else ref.numel() * ref.element_size()
+        self.check(_lib.xr_memcpy(self.h, addr(dst), addr(src), nbytes))
+
     def timing(self, enable=True):
         self.check(_lib.xr_timing_enable(self.h, int(enable)))
 
@@ -300,6 +341,23 @@ class Context:
                                            epochs, batch, direction, method, lam, ctypes.byref(st)))
         return losses[:epochs]
 
+    def batch(self, geoms, batch=None, direction=CW, method=METHOD_MLEM, lam=1e-2, flags=BATCH_GRAPH, scans=None):
+        return Batch(self, geoms, batch, direction, method, lam, flags, scans)
+
+    def batch_reconstruct(self, geoms, sino, vol, epochs, batch=None, direction=CW, method=METHOD_MLEM, lam=1e-2,
+                          flags=BATCH_GRAPH, scans=None):
+        """IR::reconstruct for many problems at once: sino[i], vol[i] float32 numpy arrays (vol in/out).
+        Returns the [epoch][problem] raw loss sums."""
+        n = len(geoms)
+        ga = (Geometry * n)(*geoms)
+        sa = None if scans is None else (ScanStruct * n)(*[s.struct for s in scans])
+        ba = None if batch is None else np.ascontiguousarray(batch, dtype=np.int32)
+        losses = np.zeros(max(1, epochs * n), dtype=np.float64)
+        self.check(_lib.xr_batch_reconstruct(self.h, n, ga, sa, None if ba is None else ba.ctypes.data_as(_c_int32_p),
+                                             _ptr_list(sino), _ptr_list(vol), epochs, direction, method, lam, flags,
+                                             losses.ctypes.data_as(_c_double_p)))
+        return losses[: epochs * n].reshape(epochs, n)
+
     def fdk_reconstruct(self, geom, scan, sino, vol, direction=CW, filter=FILTER_REFERENCE):
         sp, vp = _ptr_list(sino), _ptr_list(vol)
         self.check(_lib.xr_fdk_reconstruct(self.h, ctypes.byref(geom), ctypes.byref(scan.struct), sp, vp, direction,
@@ -355,6 +413,23 @@ class Plan:
         self.ctx.check(_lib.xr_backproject_update(self.ctx.h, self.h, _dptr(proj), c.ctypes.data_as(_c_int32_p),
                                                   len(c), int(xtt), method, lam, subset_size, _dptr(vol)))
 
+    def set_tiles(self, bwd_zpl=0, bwd_gx=0, fwd_rows=0):
+        t = Tiles(bwd_zpl, bwd_gx, fwd_rows)
+        self.ctx.check(_lib.xr_plan_set_tiles(self.h, ctypes.byref(t)))
+
+    def tiles(self):
+        """Effective tile shape (zpl, gx, rows)."""
+        t = Tiles()
+        self.ctx.check(_lib.xr_plan_get_tiles(self.h, ctypes.byref(t)))
+        return t.as_tuple()
+
+    def autotune(self, vol, sino):
+        t = Tiles()
+        ms = ctypes.c_double()
+        self.ctx.check(_lib.xr_autotune(self.ctx.h, self.h, _dptr(vol), _dptr(sino), ctypes.byref(t),
+                                        ctypes.byref(ms)))
+        return t.as_tuple(), ms.value
+
     def fdk_filter(self, sino, filt, codes, filter=FILTER_REFERENCE):
         c = np.ascontiguousarray(codes, dtype=np.int32)
         self.ctx.check(_lib.xr_fdk_filter(self.ctx.h, self.h, filter, _dptr(sino), _dptr(filt),
@@ -364,6 +439,69 @@ class Plan:
         c = np.ascontiguousarray(codes, dtype=np.int32)
         self.ctx.check(_lib.xr_fdk_backproject(self.ctx.h, self.h, _dptr(filt), c.ctypes.data_as(_c_int32_p),
                                                len(c), int(overwrite), _dptr(vol)))
+
+
+class _BorrowedPlan(Plan):
+    """A plan owned by a batch (not destroyed from Python)."""
+
+    def __init__(self, ctx, geom, handle):
+        self.ctx, self.geom, self.scan = ctx, geom, None
+        self.h = handle
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """Batched multi-problem IR solver (xr_batch_*): persistent launches over all problems,
+    epoch loop captured as a HIP graph with BATCH_GRAPH."""
+
+    def __init__(self, ctx, geoms, batch=None, direction=CW, method=METHOD_MLEM, lam=1e-2, flags=BATCH_GRAPH,
+                 scans=None):
+        self.ctx, self.geoms = ctx, list(geoms)
+        n = len(self.geoms)
+        ga = (Geometry * n)(*self.geoms)
+        sa = None if scans is None else (ScanStruct * n)(*[s.struct for s in scans])
+        self._scans = scans
+        ba = None if batch is None else np.ascontiguousarray(batch, dtype=np.int32)
+        h = ctypes.c_void_p()
+        ctx.check(_lib.xr_batch_create(ctx.h, n, ga, sa, None if ba is None else ba.ctypes.data_as(_c_int32_p),
+                                       direction, method, lam, flags, ctypes.byref(h)))
+        self.h = h
+        self.n = n
+
+    def close(self):
+        if self.h:
+            _lib.xr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def buffers(self, i):
+        """(vol_cols, sino) device addresses of problem i."""
+        v, s = ctypes.c_void_p(), ctypes.c_void_p()
+        self.ctx.check(_lib.xr_batch_buffers(self.h, i, ctypes.byref(v), ctypes.byref(s)))
+        return v.value, s.value
+
+    def plan(self, i):
+        return _BorrowedPlan(self.ctx, self.geoms[i], ctypes.c_void_p(_lib.xr_batch_plan(self.h, i)))
+
+    def autotune(self):
+        self.ctx.check(_lib.xr_batch_autotune(self.h))
+
+    def run(self, epochs):
+        losses = np.zeros(max(1, epochs * self.n), dtype=np.float64)
+        self.ctx.check(_lib.xr_batch_run(self.h, epochs, losses.ctypes.data_as(_c_double_p)))
+        return losses[: epochs * self.n].reshape(epochs, self.n)
+
+    def elapsed_ms(self):
+        ms = ctypes.c_double()
+        self.ctx.check(_lib.xr_batch_elapsed_ms(self.h, ctypes.byref(ms)))
+        return ms.value
 
 
 def fdk_taps(geom, filter=FILTER_REFERENCE):

@@ -1,0 +1,655 @@
+// Batched multi-problem IR solver and projector tile autotuner (C ABI in
+// include/xr.h).
+//
+// A batch holds n independent IR problems (one condition each, any geometry,
+// view count and subset count).  A subset step of every problem is two
+// persistent launches: k_forward_batch over all (problem, row chunk, band,
+// view) blocks and k_backward_batch over all (problem, voxel tile) blocks,
+// each drawing work from a self-resetting ticket queue in descending order of
+// problem size.  With XR_BATCH_GRAPH the whole epoch loop (zero fills, the
+// per-problem max |voxel| reduction, both launches per step) is captured once
+// as a HIP graph and replayed by later runs with the same epoch count; only
+// the subset lists are uploaded per run, outside the graph.
+//
+// Each problem follows IR::reconstruct (reconstruct.cu:23-156) exactly as
+// xr_ir_reconstruct does for one condition: its own default-seeded
+// std::mt19937_64 shuffles the subset order every epoch (reconstruct.cu:75,
+// 89), views revisited within an epoch accumulate onto their image
+// (reconstruct.cu:83), the forward pass writes the MLEM ratio / ART residual
+// and the backprojection applies the update in place.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "../../include/xr.h"
+#include "xr_host.hpp"
+#include "xr_internal.hpp"
+
+using namespace xr;
+using namespace xr::host;
+
+namespace {
+
+bool valid_tiles(const xr_tiles& t) {
+    const int z = t.bwd_zpl, g = t.bwd_gx;
+    return (z == 0 || z == 1 || z == 2 || z == 4) && (g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16) &&
+           t.fwd_rows >= 0;
+}
+
+std::array<int, 5> tune_key(const xr_plan* p) {
+    return {p->g.n_vox, p->g.n_det_u, p->g.n_det_v, p->g.n_proj, p->axis_cond.empty() ? 0 : (int)p->axis_cond[0]};
+}
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    hipError_t init() {
+        hipError_t e = hipEventCreate(&a);
+        if (e != hipSuccess) return e;
+        return hipEventCreate(&b);
+    }
+};
+
+// Subset lists of one problem for `epochs` epochs, laid out [epoch][pos][j]
+// with pos < maxb (positions >= batch unused): one engine per problem, as
+// xr_ir_reconstruct uses for one condition.
+void problem_codes(int P, int batch, int maxb, int epochs, std::vector<int32_t>& codes, std::vector<uint8_t>& flags,
+                   bool& any_flag) {
+    const int ss = (P + batch - 1) / batch;
+    codes.assign((size_t)epochs * maxb * ss, 0);
+    flags.assign(codes.size(), 0);
+    any_flag = false;
+    std::vector<int> order(batch);
+    std::iota(order.begin(), order.end(), 0);
+    std::mt19937_64 rng;
+    for (int ep = 0; ep < epochs; ep++) {
+        std::shuffle(order.begin(), order.end(), rng);
+        std::vector<char> visited(P, 0);
+        for (int pos = 0; pos < batch; pos++) {
+            for (int j = 0; j < ss; j++) {
+                const int v = (order[pos] + batch * j) % P;  // reconstruct.cu:61-65
+                const size_t i = ((size_t)ep * maxb + pos) * ss + j;
+                codes[i] = v;
+                flags[i] = visited[v];
+                any_flag = any_flag || visited[v];
+                visited[v] = 1;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+struct xr_batch {
+    xr_context* ctx = nullptr;
+    int n = 0, method = 0, flags = 0, maxb = 1;
+    float lambda = 0.f;
+    hipStream_t stream = nullptr;
+    struct Problem {
+        xr_plan* plan = nullptr;
+        int N = 0, U = 0, V = 0, P = 0, batch = 1, ss = 0;
+        float* vol = nullptr;
+        float* sino = nullptr;
+        float* proj = nullptr;
+        int32_t* codes = nullptr;
+        uint8_t* flags = nullptr;
+        bool any_flag = false;
+    };
+    std::vector<Problem> pr;
+    std::vector<int> order;  // problems by descending work
+    int epoch_cap = 0;       // epochs the code buffers hold
+    // per subset position: device item tables
+    struct Step {
+        int n_fwd = 0, n_bwd = 0, n_active = 0;
+        int bwd_first[3] = {0, 0, 0}, bwd_count[3] = {0, 0, 0};  // backprojection items per zpl class
+        FwdArgs* fwd = nullptr;
+        int* fwd_end = nullptr;
+        BwdArgs* bwd = nullptr;
+        int* bwd_end = nullptr;
+        const float** segs = nullptr;  // volumes of the active problems (d_vmax order)
+        int64_t* seg_len = nullptr;
+    };
+    std::vector<Step> steps;
+    bool tables_ok = false;
+    float* d_vmax = nullptr;   // [n]
+    double* d_loss = nullptr;  // [epoch_cap][n]
+    unsigned* d_queue = nullptr;  // [4]
+    int fwd_grid = 0, bwd_grid[3] = {0, 0, 0}, ticket_slot = 0;
+    size_t fwd_lds = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_epochs = -1;
+    double last_ms = 0.0;
+
+    ~xr_batch() { release(); }
+    void free_tables() {
+        for (auto& s : steps) {
+            if (s.fwd) (void)hipFree(s.fwd);
+            if (s.fwd_end) (void)hipFree(s.fwd_end);
+            if (s.bwd) (void)hipFree(s.bwd);
+            if (s.bwd_end) (void)hipFree(s.bwd_end);
+            if (s.segs) (void)hipFree(s.segs);
+            if (s.seg_len) (void)hipFree(s.seg_len);
+        }
+        steps.clear();
+        tables_ok = false;
+        drop_graph();
+    }
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_epochs = -1;
+    }
+    void release() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        free_tables();
+        for (auto& p : pr) {
+            if (p.vol) (void)hipFree(p.vol);
+            if (p.sino) (void)hipFree(p.sino);
+            if (p.proj) (void)hipFree(p.proj);
+            if (p.codes) (void)hipFree(p.codes);
+            if (p.flags) (void)hipFree(p.flags);
+            if (p.plan) xr_plan_destroy(p.plan);
+        }
+        pr.clear();
+        if (d_vmax) (void)hipFree(d_vmax);
+        if (d_loss) (void)hipFree(d_loss);
+        if (d_queue) (void)hipFree(d_queue);
+        d_vmax = nullptr;
+        d_loss = nullptr;
+        d_queue = nullptr;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+namespace {
+
+// Grow the subset-list buffers to `epochs` epochs (invalidates the tables).
+int ensure_epochs(xr_batch* b, int epochs) {
+    xr_context* ctx = b->ctx;
+    if (epochs <= b->epoch_cap) return XR_OK;
+    b->free_tables();
+    for (auto& p : b->pr) {
+        if (p.codes) (void)hipFree(p.codes);
+        if (p.flags) (void)hipFree(p.flags);
+        p.codes = nullptr;
+        p.flags = nullptr;
+        const size_t cnt = (size_t)epochs * b->maxb * p.ss;
+        XR_HIP(ctx, hipMalloc((void**)&p.codes, cnt * sizeof(int32_t)));
+        XR_HIP(ctx, hipMalloc((void**)&p.flags, cnt));
+    }
+    if (b->d_loss) (void)hipFree(b->d_loss);
+    b->d_loss = nullptr;
+    XR_HIP(ctx, hipMalloc((void**)&b->d_loss, sizeof(double) * (size_t)epochs * b->n));
+    b->epoch_cap = epochs;
+    return XR_OK;
+}
+
+// Device item tables of every subset position.  Needs the code buffers.
+int build_tables(xr_batch* b) {
+    xr_context* ctx = b->ctx;
+    if (b->tables_ok) return XR_OK;
+    b->free_tables();
+    const bool mlem = b->method == XR_METHOD_MLEM;
+    const int out_mode = mlem ? OUT_RATIO : OUT_RESIDUAL;
+    int max_vb = 1;
+    b->steps.resize(b->maxb);
+    for (int pos = 0; pos < b->maxb; pos++) {
+        std::vector<FwdArgs> fi;
+        std::vector<int> fe;
+        std::vector<BwdArgs> bz[3];  // items by z depth class (zpl 1, 2, 4)
+        std::vector<const float*> segs;
+        std::vector<int64_t> lens;
+        int fsum = 0;
+        for (int k : b->order) {
+            xr_batch::Problem& p = b->pr[k];
+            if (pos >= p.batch) continue;
+            xr_plan* pl = p.plan;
+            const int64_t N3 = (int64_t)p.N * p.N * p.N;
+            const int slot = (int)segs.size();  // max |voxel| of this step's active problem `slot`
+            FwdArgs a{};
+            a.d = pl->dims;
+            a.b = pl->bas;
+            a.vg = pl->d_vg;
+            a.cg = pl->d_cg;
+            a.views = p.codes;
+            a.accum = p.flags;  // all zero unless a subset wraps past P
+            a.vol = p.vol;
+            a.vol_cond_stride = 0;
+            a.chan_stride = N3;
+            a.proj = p.proj;
+            a.sino = p.sino;
+            a.norm = pl->d_norm;
+            a.g2 = pl->d_g2;
+            a.loss = b->d_loss + k;
+            a.vmax = b->d_vmax + slot;
+            a.wmul = pl->dims.ratio * 1.0001f;  // as forward_launch
+            a.code_base = 0;
+            a.out_mode = out_mode;
+            a.n_bands = (p.U + kBandU - 1) / kBandU;
+            a.n_slots = p.ss;
+            a.axis = pl->axis_cond[0] ? 1 : 0;
+            const int rows = forward_rows(pl);
+            for (int v0 = 0; v0 < p.V; v0 += rows) {
+                a.v0 = v0;
+                a.vb = std::min(rows, p.V - v0);
+                max_vb = std::max(max_vb, a.vb);
+                fi.push_back(a);
+                fsum += a.n_bands * a.n_slots;
+                fe.push_back(fsum);
+            }
+            BwdArgs w{};
+            w.d = pl->dims;
+            w.b = pl->bas;
+            w.vg = pl->d_vg;
+            w.views = p.codes;
+            w.n_views = p.ss;
+            w.proj = p.proj;
+            w.code_base = 0;
+            w.vol = p.vol;
+            w.chan_stride = N3;
+            w.g2 = pl->d_g2;
+            w.alpha = mlem ? 0.f : b->lambda / (float)p.ss;  // reconstruct.cu:124
+            w.axis = pl->axis_cond[0] ? 1 : 0;
+            if (!backward_shape(w, pl->tiles.bwd_zpl, pl->tiles.bwd_gx, false))
+                return fail(ctx, XR_ERR_INVALID, "bad tile shape for problem %d", k);
+            bz[w.zpl == 4 ? 2 : w.zpl - 1].push_back(w);
+            segs.push_back(p.vol);
+            lens.push_back(N3);
+        }
+        xr_batch::Step& s = b->steps[pos];
+        s.n_fwd = (int)fi.size();
+        s.n_active = (int)segs.size();
+        if (s.n_fwd == 0) continue;
+        // backprojection items by z depth class (zpl 4, 2, 1): one launch and
+        // one ticket range per class, so prefix sums restart in each class
+        std::vector<BwdArgs> bi;
+        std::vector<int> be;
+        for (int c = 2; c >= 0; c--) {
+            s.bwd_first[c] = (int)bi.size();
+            s.bwd_count[c] = (int)bz[c].size();
+            int run = 0;
+            for (const BwdArgs& w : bz[c]) {
+                bi.push_back(w);
+                run += w.n_tiles;
+                be.push_back(run);
+            }
+        }
+        s.n_bwd = (int)bi.size();
+        XR_HIP(ctx, hipMalloc((void**)&s.fwd, sizeof(FwdArgs) * fi.size()));
+        XR_HIP(ctx, hipMalloc((void**)&s.fwd_end, sizeof(int) * fe.size()));
+        XR_HIP(ctx, hipMalloc((void**)&s.bwd, sizeof(BwdArgs) * bi.size()));
+        XR_HIP(ctx, hipMalloc((void**)&s.bwd_end, sizeof(int) * be.size()));
+        XR_HIP(ctx, hipMalloc((void**)&s.segs, sizeof(float*) * segs.size()));
+        XR_HIP(ctx, hipMalloc((void**)&s.seg_len, sizeof(int64_t) * lens.size()));
+        XR_HIP(ctx, hipMemcpy(s.fwd, fi.data(), sizeof(FwdArgs) * fi.size(), hipMemcpyHostToDevice));
+        XR_HIP(ctx, hipMemcpy(s.fwd_end, fe.data(), sizeof(int) * fe.size(), hipMemcpyHostToDevice));
+        XR_HIP(ctx, hipMemcpy(s.bwd, bi.data(), sizeof(BwdArgs) * bi.size(), hipMemcpyHostToDevice));
+        XR_HIP(ctx, hipMemcpy(s.bwd_end, be.data(), sizeof(int) * be.size(), hipMemcpyHostToDevice));
+        XR_HIP(ctx, hipMemcpy(s.segs, segs.data(), sizeof(float*) * segs.size(), hipMemcpyHostToDevice));
+        XR_HIP(ctx, hipMemcpy(s.seg_len, lens.data(), sizeof(int64_t) * lens.size(), hipMemcpyHostToDevice));
+    }
+    // Persistent grids: every block resident at once.
+    hipDeviceProp_t prop;
+    XR_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
+    b->ticket_slot = (max_vb + 2) * kBandStride + 8;
+    b->fwd_lds = sizeof(unsigned long long) * ((size_t)b->ticket_slot + 1);
+    const int fpc = batch_blocks_per_cu_forward(FWD_PLAIN, b->fwd_lds);
+    if (fpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched forward kernel");
+    b->fwd_grid = fpc * prop.multiProcessorCount;
+    for (int c = 0; c < 3; c++) {
+        const int bpc = batch_blocks_per_cu_backward(mlem ? BWD_MLEM : BWD_ART, c == 2 ? 4 : c + 1);
+        if (bpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched backprojection");
+        b->bwd_grid[c] = bpc * prop.multiProcessorCount;
+    }
+    b->tables_ok = true;
+    return XR_OK;
+}
+
+// Enqueue the epoch loop on the batch stream (eagerly or under capture).
+int enqueue_epochs(xr_batch* b, int epochs) {
+    xr_context* ctx = b->ctx;
+    hipStream_t st = b->stream;
+    const bool mlem = b->method == XR_METHOD_MLEM;
+    hipError_t e = launch_fill(reinterpret_cast<float*>(b->d_loss), (int64_t)2 * epochs * b->n, 0.0f, st);
+    if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "fill failed: %s", hipGetErrorString(e));
+    for (int ep = 0; ep < epochs; ep++) {
+        for (int pos = 0; pos < b->maxb; pos++) {
+            const xr_batch::Step& s = b->steps[pos];
+            if (s.n_fwd == 0) continue;
+            const int step = ep * b->maxb + pos;
+            e = launch_fill(b->d_vmax, b->n, 0.0f, st);
+            if (e == hipSuccess) e = launch_absmax_seg(s.segs, s.seg_len, b->d_vmax, s.n_active, 64, st);
+            if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "absmax failed: %s", hipGetErrorString(e));
+            e = launch_forward_batch(s.fwd, s.fwd_end, s.n_fwd, b->d_queue, b->ticket_slot, b->fwd_lds, b->fwd_grid,
+                                     step, ep * b->n, FWD_PLAIN, st);
+            if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "batched forward failed: %s", hipGetErrorString(e));
+            // one persistent launch per z depth class present (in-order on the stream)
+            for (int c = 2; c >= 0; c--) {
+                if (s.bwd_count[c] == 0) continue;
+                e = launch_backward_batch(s.bwd + s.bwd_first[c], s.bwd_end + s.bwd_first[c], s.bwd_count[c],
+                                          b->d_queue + 2, b->bwd_grid[c], step, mlem ? BWD_MLEM : BWD_ART,
+                                          c == 2 ? 4 : c + 1, st);
+                if (e != hipSuccess)
+                    return fail(ctx, XR_ERR_HIP, "batched backprojection failed: %s", hipGetErrorString(e));
+            }
+        }
+    }
+    return XR_OK;
+}
+
+}  // namespace
+
+
+extern "C" {
+
+// ---------------------------------------------------------------- tiles ----
+
+int xr_plan_set_tiles(xr_plan* plan, const xr_tiles* tiles) {
+    if (!plan || !tiles) return XR_ERR_INVALID;
+    if (!valid_tiles(*tiles)) return fail(plan->ctx, XR_ERR_INVALID, "invalid tile shape (zpl %d, gx %d, rows %d)",
+                                          tiles->bwd_zpl, tiles->bwd_gx, tiles->fwd_rows);
+    plan->tiles = *tiles;
+    return XR_OK;
+}
+
+int xr_plan_get_tiles(const xr_plan* plan, xr_tiles* tiles) {
+    if (!plan || !tiles) return XR_ERR_INVALID;
+    BwdArgs a{};
+    a.d = plan->dims;
+    if (!backward_shape(a, plan->tiles.bwd_zpl, plan->tiles.bwd_gx, false)) return XR_ERR_INVALID;
+    tiles->bwd_zpl = a.zpl;
+    tiles->bwd_gx = a.gx;
+    tiles->fwd_rows = forward_rows(plan);
+    return XR_OK;
+}
+
+// ------------------------------------------------------------ autotuner ----
+
+int xr_autotune(xr_context* ctx, xr_plan* plan, const float* vol_cols, const float* sino, xr_tiles* best,
+                double* best_ms) {
+    if (!ctx || !plan || !vol_cols || !sino) return fail(ctx, XR_ERR_INVALID, "bad args");
+    const auto key = tune_key(plan);
+    for (auto& kv : ctx->tune_cache) {
+        if (kv.first == key) {
+            plan->tiles = kv.second;
+            if (best) *best = kv.second;
+            if (best_ms) *best_ms = -1.0;
+            return XR_OK;
+        }
+    }
+    XR_TRY(ensure_norm(plan));
+    const int N = plan->g.n_vox, P = plan->g.n_proj;
+    const int64_t N3 = (int64_t)N * N * N, lenD = (int64_t)plan->g.n_det_u * plan->g.n_det_v * P;
+    DevBuf<float> proj, vol2;
+    DevBuf<int32_t> codes;
+    XR_HIP(ctx, proj.alloc(lenD));
+    XR_HIP(ctx, vol2.alloc(N3));
+    XR_HIP(ctx, codes.alloc(P));
+    std::vector<int32_t> hc(P);
+    std::iota(hc.begin(), hc.end(), 0);
+    XR_HIP(ctx, hipMemcpyAsync(codes.p, hc.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice, ctx->stream));
+    XR_HIP(ctx, hipMemcpyAsync(vol2.p, vol_cols, sizeof(float) * N3, hipMemcpyDeviceToDevice, ctx->stream));
+    EventPair ev;
+    XR_HIP(ctx, ev.init());
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    const xr_tiles saved = plan->tiles;
+    auto time_it = [&](auto&& fn, double& ms) -> int {
+        XR_TRY(fn());  // warm-up
+        XR_HIP(ctx, hipEventRecord(ev.a, ctx->stream));
+        for (int r = 0; r < 2; r++) XR_TRY(fn());
+        XR_HIP(ctx, hipEventRecord(ev.b, ctx->stream));
+        XR_HIP(ctx, hipEventSynchronize(ev.b));
+        float t = 0.f;
+        XR_HIP(ctx, hipEventElapsedTime(&t, ev.a, ev.b));
+        ms = 0.5 * t;
+        return XR_OK;
+    };
+    int rc = XR_OK;
+    xr_tiles win = saved;
+    double fbest = 1e300, bbest = 1e300;
+    // forward: detector-row chunk heights (LDS footprint vs. column scans)
+    {
+        std::vector<int> seen;
+        for (int rows : {0, 96, 64, 32}) {
+            plan->tiles.fwd_rows = rows;
+            const int eff = forward_rows(plan);
+            if (std::find(seen.begin(), seen.end(), eff) != seen.end()) continue;
+            seen.push_back(eff);
+            double ms = 0;
+            rc = time_it([&] { return forward_launch(plan, FWD_PLAIN, vol_cols, 0, proj.p, sino, 0, codes.p, nullptr,
+                                                     P, OUT_RATIO, ctx->d_loss); },
+                         ms);
+            if (rc != XR_OK) break;
+            if (ms < fbest) {
+                fbest = ms;
+                win.fwd_rows = rows;
+            }
+        }
+    }
+    // backprojection: z depth per lane x column-group layout, MLEM on a copy
+    // of the volume with a ratio image of ones (the volume is unchanged).
+    if (rc == XR_OK) {
+        const hipError_t e = launch_fill(proj.p, lenD, 1.0f, ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, XR_ERR_HIP, "fill failed: %s", hipGetErrorString(e));
+    }
+    plan->tiles.fwd_rows = win.fwd_rows;
+    for (int zpl : {1, 2, 4}) {
+        if (rc != XR_OK) break;
+        if (zpl > 1 && 64 * zpl > 2 * std::max(N, 64)) continue;
+        for (int gx : {2, 4, 8}) {
+            plan->tiles.bwd_zpl = zpl;
+            plan->tiles.bwd_gx = gx;
+            double ms = 0;
+            rc = time_it([&] { return backward_launch(plan, BWD_MLEM, proj.p, 0, codes.p, P, vol2.p, nullptr,
+                                                      nullptr, 0.f, 0); },
+                         ms);
+            if (rc != XR_OK) break;
+            if (ms < bbest) {
+                bbest = ms;
+                win.bwd_zpl = zpl;
+                win.bwd_gx = gx;
+            }
+        }
+    }
+    ctx->timing = timing;
+    if (rc != XR_OK) {
+        plan->tiles = saved;
+        return rc;
+    }
+    plan->tiles = win;
+    ctx->tune_cache.push_back({key, win});
+    if (best) *best = win;
+    if (best_ms) *best_ms = fbest + bbest;
+    return XR_OK;
+}
+
+// -------------------------------------------------------------- batches ----
+
+int xr_batch_create(xr_context* ctx, int n, const xr_geometry* geoms, const xr_scan* scans, const int32_t* batch,
+                    int direction, int method, float lambda, int flags, xr_batch** out) {
+    if (!ctx || !out || !geoms || n <= 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    *out = nullptr;
+    if (method != XR_METHOD_MLEM && method != XR_METHOD_ART) return fail(ctx, XR_ERR_INVALID, "method must be MLEM or ART");
+    std::unique_ptr<xr_batch> b(new (std::nothrow) xr_batch());
+    if (!b) return fail(ctx, XR_ERR_ALLOC, "batch allocation failed");
+    b->ctx = ctx;
+    b->n = n;
+    b->method = method;
+    b->lambda = lambda;
+    b->flags = flags;
+    XR_HIP(ctx, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    const float I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Z[3] = {0, 0, 0};
+    xr_scan plain{};
+    plain.n_cond = 1;
+    plain.n_basis = 0;
+    plain.cond_R = I;
+    plain.cond_t = Z;
+    plain.cond_offset = Z;
+    b->pr.resize(n);
+    for (int i = 0; i < n; i++) {
+        xr_batch::Problem& p = b->pr[i];
+        const xr_scan* s = scans ? &scans[i] : &plain;
+        if (s->n_cond != 1 || s->n_basis != 0)
+            return fail(ctx, XR_ERR_INVALID, "problem %d: batched problems have one condition and no basis", i);
+        XR_TRY(check_geometry(ctx, &geoms[i]));
+        p.N = geoms[i].n_vox;
+        p.U = geoms[i].n_det_u;
+        p.V = geoms[i].n_det_v;
+        p.P = geoms[i].n_proj;
+        p.batch = batch ? batch[i] : 1;
+        if (p.batch <= 0 || p.batch > p.P) return fail(ctx, XR_ERR_INVALID, "problem %d: bad batch %d", i, p.batch);
+        p.ss = (p.P + p.batch - 1) / p.batch;
+        b->maxb = std::max(b->maxb, p.batch);
+        XR_TRY(xr_plan_create(ctx, &geoms[i], s, XR_CONV_IR, direction, &p.plan));
+        XR_TRY(ensure_norm(p.plan));
+        const int64_t N3 = (int64_t)p.N * p.N * p.N, lenD = (int64_t)p.U * p.V * p.P;
+        XR_HIP(ctx, hipMalloc((void**)&p.vol, sizeof(float) * N3));
+        XR_HIP(ctx, hipMalloc((void**)&p.sino, sizeof(float) * lenD));
+        XR_HIP(ctx, hipMalloc((void**)&p.proj, sizeof(float) * lenD));
+        XR_HIP(ctx, hipMemsetAsync(p.vol, 0, sizeof(float) * N3, ctx->stream));
+        XR_HIP(ctx, hipMemsetAsync(p.sino, 0, sizeof(float) * lenD, ctx->stream));
+    }
+    // Longest work first: backprojection cost ~ N^3 P, the dominant term.
+    b->order.resize(n);
+    std::iota(b->order.begin(), b->order.end(), 0);
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) {
+        const auto& a = b->pr[x];
+        const auto& c = b->pr[y];
+        return (double)a.N * a.N * a.N * a.ss > (double)c.N * c.N * c.N * c.ss;
+    });
+    XR_HIP(ctx, hipMalloc((void**)&b->d_vmax, sizeof(float) * n));
+    XR_HIP(ctx, hipMalloc((void**)&b->d_queue, sizeof(unsigned) * 4));
+    XR_HIP(ctx, hipMemsetAsync(b->d_queue, 0, sizeof(unsigned) * 4, ctx->stream));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = b.release();
+    return XR_OK;
+}
+
+void xr_batch_destroy(xr_batch* b) { delete b; }
+
+int xr_batch_buffers(xr_batch* b, int i, float** vol_cols, float** sino) {
+    if (!b || i < 0 || i >= b->n) return XR_ERR_INVALID;
+    if (vol_cols) *vol_cols = b->pr[i].vol;
+    if (sino) *sino = b->pr[i].sino;
+    return XR_OK;
+}
+
+xr_plan* xr_batch_plan(xr_batch* b, int i) {
+    if (!b || i < 0 || i >= b->n) return nullptr;
+    return b->pr[i].plan;
+}
+
+int xr_batch_autotune(xr_batch* b) {
+    if (!b) return XR_ERR_INVALID;
+    xr_context* ctx = b->ctx;
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto& p : b->pr) XR_TRY(xr_autotune(ctx, p.plan, p.vol, p.sino, nullptr, nullptr));
+    b->free_tables();
+    return XR_OK;
+}
+
+int xr_batch_run(xr_batch* b, int epochs, double* losses) {
+    if (!b) return XR_ERR_INVALID;
+    xr_context* ctx = b->ctx;
+    if (epochs < 0) return fail(ctx, XR_ERR_INVALID, "epochs must be >= 0");
+    b->last_ms = 0.0;
+    if (epochs == 0) return XR_OK;
+    if ((b->flags & XR_BATCH_AUTOTUNE) && !b->tables_ok) XR_TRY(xr_batch_autotune(b));
+    XR_TRY(ensure_epochs(b, epochs));
+    XR_TRY(build_tables(b));
+    // The caller's data (made on the context stream) must be complete.
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<int32_t> codes;
+    std::vector<uint8_t> fl;
+    for (auto& p : b->pr) {
+        problem_codes(p.P, p.batch, b->maxb, epochs, codes, fl, p.any_flag);
+        XR_HIP(ctx, hipMemcpyAsync(p.codes, codes.data(), sizeof(int32_t) * codes.size(), hipMemcpyHostToDevice,
+                                   b->stream));
+        XR_HIP(ctx, hipMemcpyAsync(p.flags, fl.data(), fl.size(), hipMemcpyHostToDevice, b->stream));
+        // host vectors are reused: wait before overwriting them
+        XR_HIP(ctx, hipStreamSynchronize(b->stream));
+    }
+    EventPair ev;
+    XR_HIP(ctx, ev.init());
+    const bool use_graph = (b->flags & XR_BATCH_GRAPH) != 0;
+    if (use_graph && b->graph_epochs != epochs) {
+        b->drop_graph();
+        XR_HIP(ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_epochs(b, epochs);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc != XR_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        b->graph = g;
+        XR_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graph_epochs = epochs;
+    }
+    XR_HIP(ctx, hipEventRecord(ev.a, b->stream));
+    if (use_graph) {
+        XR_HIP(ctx, hipGraphLaunch(b->exec, b->stream));
+    } else {
+        XR_TRY(enqueue_epochs(b, epochs));
+    }
+    XR_HIP(ctx, hipEventRecord(ev.b, b->stream));
+    XR_HIP(ctx, hipEventSynchronize(ev.b));
+    float ms = 0.f;
+    XR_HIP(ctx, hipEventElapsedTime(&ms, ev.a, ev.b));
+    b->last_ms = ms;
+    if (losses)
+        XR_HIP(ctx, hipMemcpy(losses, b->d_loss, sizeof(double) * (size_t)epochs * b->n, hipMemcpyDeviceToHost));
+    return XR_OK;
+}
+
+int xr_batch_elapsed_ms(const xr_batch* b, double* ms) {
+    if (!b || !ms) return XR_ERR_INVALID;
+    *ms = b->last_ms;
+    return XR_OK;
+}
+
+int xr_batch_reconstruct(xr_context* ctx, int n, const xr_geometry* geoms, const xr_scan* scans,
+                         const int32_t* batch, float* const* sino, float* const* vol, int epochs, int direction,
+                         int method, float lambda, int flags, double* losses) {
+    if (!ctx || !sino || !vol) return fail(ctx, XR_ERR_INVALID, "bad args");
+    xr_batch* b = nullptr;
+    XR_TRY(xr_batch_create(ctx, n, geoms, scans, batch, direction, method, lambda, flags, &b));
+    std::unique_ptr<xr_batch, void (*)(xr_batch*)> guard(b, xr_batch_destroy);
+    int64_t maxN3 = 0;
+    for (auto& p : b->pr) maxN3 = std::max(maxN3, (int64_t)p.N * p.N * p.N);
+    DevBuf<float> ref;
+    XR_HIP(ctx, ref.alloc(maxN3));
+    for (int i = 0; i < n; i++) {
+        auto& p = b->pr[i];
+        const int64_t N3 = (int64_t)p.N * p.N * p.N, lenD = (int64_t)p.U * p.V * p.P;
+        XR_HIP(ctx, hipMemcpyAsync(p.sino, sino[i], sizeof(float) * lenD, hipMemcpyHostToDevice, ctx->stream));
+        XR_HIP(ctx, hipMemcpyAsync(ref.p, vol[i], sizeof(float) * N3, hipMemcpyHostToDevice, ctx->stream));
+        XR_TRY(xr_volume_to_columns(ctx, ref.p, p.vol, p.N, 1));
+        XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    XR_TRY(xr_batch_run(b, epochs, losses));
+    for (int i = 0; i < n; i++) {
+        auto& p = b->pr[i];
+        const int64_t N3 = (int64_t)p.N * p.N * p.N;
+        XR_TRY(xr_volume_from_columns(ctx, p.vol, ref.p, p.N, 1));
+        XR_HIP(ctx, hipMemcpyAsync(vol[i], ref.p, sizeof(float) * N3, hipMemcpyDeviceToHost, ctx->stream));
+        XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return XR_OK;
+}
+
+}  // extern "C"

@@ -90,6 +90,7 @@ struct FwdArgs {
     int n_bands;
     int n_slots;
     int v0, vb;              // detector rows [v0, v0+vb) owned by this launch
+    int axis;                // batched launches: 1 -> every view is axis aligned (ViewGeom::axis)
 };
 
 struct BwdArgs {
@@ -107,14 +108,38 @@ struct BwdArgs {
     const float* g2;         // XTT [n_cond*P][K]: (S_k.G)^2
     float alpha;             // ART step lambda/subsetSize
     int overwrite;           // FDK: 1 -> vol = sum, 0 -> vol += sum
-    int zw, gx, gy;           // launch shape, set by launch_backward
+    int zpl, gx, gy;          // tile shape: z voxels per lane (1, 2, 4), column groups along x, y (gx*gy = 16)
     int tiles_x, tiles_y, tiles_z, n_tiles, per_xcd;
+    int axis;                 // batched launches: 1 -> every view is axis aligned
 };
+
+// Tile shape of the plain backprojection (CPW = 2 columns per wave side).
+// zpl / gx of 0 select the default for N; fills zpl, gx, gy and the tile
+// counts of a.  Returns false for an unsupported shape.
+bool backward_shape(BwdArgs& a, int zpl, int gx, bool xtt);
+// Default plain-kind tile shape for volume edge N.
+int default_zpl(int N);
 
 // Launchers (xr_kernels.hip).  All enqueue on `stream` and return hipSuccess
 // or the launch error.
 hipError_t launch_forward(const FwdArgs& a, int kind, bool axis, hipStream_t stream);
 hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream);
+// Persistent multi-problem launches (see k_forward_batch / k_backward_batch).
+// items / item_end live in device memory; queue: two zeroed words per launch
+// site; grid: resident blocks (batch_grid_*).
+hipError_t launch_forward_batch(const FwdArgs* items, const int* item_end, int n_items, unsigned* queue,
+                                int ticket_slot, size_t lds_bytes, int grid, int step, int loss_shift, int kind,
+                                hipStream_t stream);
+// One launch per z depth class: every item of the launch has a.zpl == zpl.
+hipError_t launch_backward_batch(const BwdArgs* items, const int* item_end, int n_items, unsigned* queue, int grid,
+                                 int step, int kind, int zpl, hipStream_t stream);
+// Resident blocks per CU for the batched kernels (0 on error).
+int batch_blocks_per_cu_forward(int kind, size_t lds_bytes);
+int batch_blocks_per_cu_backward(int kind, int zpl);
+size_t backward_batch_lds_bytes();
+// out[i] = max |segs[i][0, seg_len[i])| (out zeroed beforehand).
+hipError_t launch_absmax_seg(const float* const* segs, const int64_t* seg_len, float* out, int n_seg,
+                             int blocks_per_seg, hipStream_t stream);
 hipError_t launch_fdk_filter(const float* src, float* dst, const float* weight, const float* taps, int U, int V,
                              const int32_t* views, int n_views, int code_base, hipStream_t stream);
 hipError_t launch_fdk_weight(float* weight, int U, int V, float sdd, float det, hipStream_t stream);

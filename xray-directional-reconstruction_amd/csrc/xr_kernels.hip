@@ -77,6 +77,23 @@ __device__ __forceinline__ void split(float u, int& iu, float& fu) {
 
 typedef __attribute__((address_space(3))) float lds_float;
 
+// Global-memory view of a pointer.  Kernel-argument pointers are known to be
+// global, but pointers read from a work-item table in memory are not: without
+// this cast their loads are flat accesses (slower, and mixing them with LDS
+// accesses in one kernel is what the gfx950 backend handles worst).
+// (The host pass only parses device code; it sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+#else
+template <class T>
+using gptr = T*;
+#endif
+template <class T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+    return (gptr<T>)p;
+}
+
 __device__ __forceinline__ void lds_add(lds_float* p, float v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -240,37 +257,37 @@ __device__ __forceinline__ float xtt_value(const FwdArgs& a, const FwdCol& f, co
     return s * (d.vs * d.ratio);
 }
 
+// One (band, view) work item o = band * n_slots + slot of a forward launch.
+// step selects the view list views + step*n_slots (a solver's subset step);
+// loss receives the item's share of sum |y - p|.
 template <int KIND, bool AXIS>
-__global__ __launch_bounds__(64 * kFwdWaves) void k_forward(const FwdArgs a) {
-    extern __shared__ unsigned long long acc_raw[];
+__device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, double* loss,
+                                             unsigned long long* acc_raw) {
     lds_u64* const acc = (lds_u64*)acc_raw;
     const Dims& d = a.d;
-    const int nblk = a.n_bands * a.n_slots;
-    const int per_xcd = (nblk + 7) / 8;
-    // XCD-chunked order: each XCD walks a contiguous run of (band, view) pairs,
-    // band-major, so concurrent blocks on one L2 share a band at adjacent views.
-    const int o = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-    if (o >= nblk) return;
     const int band = o / a.n_slots, slot = o % a.n_slots;
-    const int code = a.views[slot];
+    const auto views = to_global(a.views + (int64_t)step * a.n_slots);
+    const auto accum = to_global(a.accum ? a.accum + (int64_t)step * a.n_slots : nullptr);
+    const auto proj = to_global(a.proj);
+    const int code = views[slot];
     const int cond = code / d.P;
-    const ViewGeom g = a.vg[code];
+    const ViewGeom g = to_global(a.vg)[code];
     const int u0 = band * kBandU;
     const int rows = a.vb + 2;
     for (int i = threadIdx.x; i < rows * kBandStride; i += 64 * kFwdWaves) acc[i] = 0ull;
     // Fixed-point scale: the largest splat term is < 2^30 * 2^-S.
     int e = 0;
-    (void)frexpf(fmaxf(a.wmul * (*a.vmax), 1e-30f), &e);
+    (void)frexpf(fmaxf(a.wmul * (*to_global(a.vmax)), 1e-30f), &e);
     const int S = min(max(30 - e, -100), 100);
     const float scale = ldexpf(1.0f, S), unscale = ldexpf(1.0f, -S);
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const float* vol = a.vol + (KIND == FWD_PLAIN ? (int64_t)cond * a.vol_cond_stride : 0);
+    const auto vol = to_global(a.vol + (KIND == FWD_PLAIN ? (int64_t)cond * a.vol_cond_stride : 0));
     float g2[kMaxBasis];
     if (KIND == FWD_XTT) {
 #pragma unroll
-        for (int k = 0; k < kMaxBasis; k++) g2[k] = (k < d.K) ? a.g2[(int64_t)code * d.K + k] : 0.f;
+        for (int k = 0; k < kMaxBasis; k++) g2[k] = (k < d.K) ? to_global(a.g2)[(int64_t)code * d.K + k] : 0.f;
     }
     const float scalev = (KIND == FWD_PLAIN) ? d.ratio : 1.f;
     const float zlo = -d.xc, zhi = (float)(d.N - 1) - d.xc;
@@ -368,7 +385,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_forward(const FwdArgs a) {
     double lsum = 0.0;
     const int64_t img = (int64_t)(code - a.code_base) * d.V * d.U;
     const int64_t nimg = (int64_t)code * d.V * d.U;
-    const bool acc_old = (KIND != FWD_WEIGHT) && a.accum && a.accum[slot];
+    const bool acc_old = (KIND != FWD_WEIGHT) && accum && accum[slot];
     for (int i = threadIdx.x; i < a.vb * kBandU; i += 64 * kFwdWaves) {
         const int r = i / kBandU, cc = i % kBandU;
         const int uu = u0 + cc, vv = a.v0 + r;
@@ -377,28 +394,28 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_forward(const FwdArgs a) {
         const int64_t pix = img + (int64_t)vv * d.U + uu;
         const int64_t npix = nimg + (int64_t)vv * d.U + uu;
         if (a.out_mode == OUT_NORM_TABLE) {
-            const float L = (float)chord_length(a.cg[code], uu, vv, d);
-            a.proj[pix] = L / (s + 1e-5f);
+            const float L = (float)chord_length(to_global(a.cg)[code], uu, vv, d);
+            proj[pix] = L / (s + 1e-5f);
             continue;
         }
-        float p = (acc_old ? a.proj[pix] : 0.f) + s;
+        float p = (acc_old ? proj[pix] : 0.f) + s;
         if (a.out_mode == OUT_RAW) {
-            a.proj[pix] = p;
+            proj[pix] = p;
             continue;
         }
-        p = p * a.norm[npix];
+        p = p * to_global(a.norm)[npix];
         if (a.out_mode == OUT_NORMALISED) {
-            a.proj[pix] = p;
+            proj[pix] = p;
         } else {
-            const float yv = a.sino[pix];
+            const float yv = to_global(a.sino)[pix];
             lsum += (double)fabsf(yv - p);
             if (a.out_mode == OUT_RESIDUAL) {
-                a.proj[pix] = yv - p;
+                proj[pix] = yv - p;
             } else {
                 float q = p;
                 if (q != 0.0f) q = yv / q;
                 if (q > 2.0f) q = 2.0f;
-                a.proj[pix] = q;
+                proj[pix] = q;
             }
         }
     }
@@ -411,8 +428,82 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_forward(const FwdArgs a) {
         if (threadIdx.x == 0) {
             double t = 0.0;
             for (int w = 0; w < kFwdWaves; w++) t += red[w];
-            atomicAdd(a.loss, t);
+            atomicAdd(loss, t);
         }
+    }
+}
+
+template <int KIND, bool AXIS>
+__global__ __launch_bounds__(64 * kFwdWaves) void k_forward(const FwdArgs a) {
+    extern __shared__ unsigned long long acc_raw[];
+    const int nblk = a.n_bands * a.n_slots;
+    const int per_xcd = (nblk + 7) / 8;
+    // XCD-chunked order: each XCD walks a contiguous run of (band, view) pairs,
+    // band-major, so concurrent blocks on one L2 share a band at adjacent views.
+    const int o = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (o >= nblk) return;
+    forward_item<KIND, AXIS>(a, o, 0, a.loss, acc_raw);
+}
+
+// Work queue of a persistent launch: q[0] hands out tickets 0, 1, 2, ...; a
+// block stops at its first ticket >= total and counts itself in q[1]; the last
+// block to stop resets both words, so the next launch (or graph replay) starts
+// from zero without a host-side clear.  Every block stops after one failed
+// ticket: there is no wait on other blocks.
+__device__ __forceinline__ unsigned take_ticket(unsigned* q, unsigned total) {
+    const unsigned t = __hip_atomic_fetch_add(q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t >= total) {
+        const unsigned done = __hip_atomic_fetch_add(q + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (done == gridDim.x - 1) {
+            __hip_atomic_store(q, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(q + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    return t;
+}
+
+// Item holding ticket t: the first i with item_end[i] > t (binary search).
+__device__ __forceinline__ int find_item(const int* __restrict__ item_end, int n_items, int t) {
+    int lo = 0, hi = n_items - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (item_end[mid] > t)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// Persistent forward launch over many problems: items[i] is one (problem,
+// detector-row chunk) launch description with item_end[i] - item_end[i-1]
+// (band, view) blocks.  A fixed grid of resident blocks pulls tickets in order,
+// so the long items listed first are started first and the short ones fill
+// the tail.  ticket_slot: index of a free 8-byte LDS word past every item's
+// band accumulator.
+template <int KIND>
+__global__ __launch_bounds__(64 * kFwdWaves) void k_forward_batch(const FwdArgs* __restrict__ items,
+                                                                  const int* __restrict__ item_end, int n_items,
+                                                                  unsigned* queue, int ticket_slot, int step,
+                                                                  int loss_shift) {
+    extern __shared__ unsigned long long acc_raw[];
+    lds_u64* const tk = (lds_u64*)acc_raw + ticket_slot;
+    const int total = item_end[n_items - 1];
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) *tk = take_ticket(queue, (unsigned)total);
+        __syncthreads();
+        // uniform across the block: keeps the item's arguments in scalar registers
+        const int t = __builtin_amdgcn_readfirstlane((int)*tk);
+        if (t >= total) break;
+        const int i = find_item(item_end, n_items, t);
+        const FwdArgs& a = items[i];
+        const int o = t - (i ? item_end[i - 1] : 0);
+        double* loss = a.loss ? a.loss + loss_shift : nullptr;
+        if (a.axis)
+            forward_item<KIND, true>(a, o, step, loss, acc_raw);
+        else
+            forward_item<KIND, false>(a, o, step, loss, acc_raw);
     }
 }
 
@@ -475,19 +566,23 @@ __device__ __forceinline__ Foot footprint(const ViewGeom& g, const Dims& d, floa
         return f;
     }
     f.Ws = f.W | 1;
-    f.staged = (f.Ws * f.H <= kStageBuf);
+    // the last word of the second buffer is reserved (work-queue ticket of the
+    // persistent launch), so a staged footprint holds at most kStageBuf-1 floats
+    f.staged = (f.Ws * f.H <= kStageBuf - 1);
     return f;
 }
 
+// One voxel tile o of a backprojection launch; step selects the view list
+// views + step*n_views (a solver's subset step).  stage: 2 x kStageBuf floats.
 template <int KIND, bool AXIS, int CPW, int ZPL, int KM>
-__global__ __launch_bounds__(kBwdThreads) void k_backward(const BwdArgs a) {
+__device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step, float* stage) {
     constexpr bool XTT = IsXtt<KIND>::value;
     constexpr int NC = CPW * CPW;
     constexpr int KA = XTT ? KM : 1;
-    extern __shared__ float stage[];  // 2 x kStageBuf
     const Dims& d = a.d;
-    const int o = (blockIdx.x % 8) * a.per_xcd + blockIdx.x / 8;
-    if (o >= a.n_tiles) return;
+    const auto views = to_global(a.views + (int64_t)step * a.n_views);
+    const auto vg = to_global(a.vg);
+    const auto vol = to_global(a.vol);
     const int gx = a.gx;
     const int bcx = CPW * gx, bcy = CPW * a.gy;  // block columns along x, y
     const int tx = o % a.tiles_x, ty = (o / a.tiles_x) % a.tiles_y, tz = o / (a.tiles_x * a.tiles_y);
@@ -513,7 +608,7 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward(const BwdArgs a) {
     float pre[kStagePer];
     int pidx[kStagePer];
     auto fetch = [&](const Foot& f, int code) {
-        const float* img = a.proj + (int64_t)(code - a.code_base) * d.V * d.U;
+        const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
         const int n = f.W * f.H;
         const float invW = 1.0f / (float)f.W;
 #pragma unroll
@@ -540,8 +635,8 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward(const BwdArgs a) {
     };
 
     if (a.n_views == 0) return;
-    int code = a.views[0];
-    ViewGeom g = a.vg[code];
+    int code = views[0];
+    ViewGeom g = vg[code];
     Foot f = footprint(g, d, bx0, bx1, by0, by1, bz0, bz1, lane);
     lds_float* const lds = (lds_float*)stage;
     if (f.staged) {
@@ -558,17 +653,17 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward(const BwdArgs a) {
         ViewGeom ng;
         Foot nf{0, 0, 0, 0, 0, false, false};
         if (s + 1 < a.n_views) {
-            ncode = a.views[s + 1];
-            ng = a.vg[ncode];
+            ncode = views[s + 1];
+            ng = vg[ncode];
             nf = footprint(ng, d, bx0, bx1, by0, by1, bz0, bz1, lane);
             if (nf.staged) fetch(nf, ncode);
         }
         if (!f.empty) {
-            const float* img = a.proj + (int64_t)(code - a.code_base) * d.V * d.U;
+            const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
             float g2[KA];
             if (XTT) {
 #pragma unroll
-                for (int k = 0; k < KA; k++) g2[k] = (k < d.K) ? a.g2[(int64_t)code * d.K + k] : 0.f;
+                for (int k = 0; k < KA; k++) g2[k] = (k < d.K) ? to_global(a.g2)[(int64_t)code * d.K + k] : 0.f;
             }
 #pragma unroll
             for (int j = 0; j < CPW; j++) {
@@ -684,20 +779,69 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward(const BwdArgs a) {
                     const int64_t vi = idx + (int64_t)k * a.chan_stride;
                     const float t = tmp[cidx][q][k], fc = fac[cidx][q][k];
                     if (KIND == BWD_RAW || KIND == BWD_XTT_RAW) {
-                        a.tmp[vi] += t;
-                        a.factor[vi] += fc;
+                        to_global(a.tmp)[vi] += t;
+                        to_global(a.factor)[vi] += fc;
                     } else if (KIND == BWD_MLEM || KIND == BWD_XTT_MLEM) {
-                        const float mu = a.vol[vi];
-                        a.vol[vi] = (fc == 0.0f) ? 0.0f : mu * t / fc;
+                        const float mu = vol[vi];
+                        vol[vi] = (fc == 0.0f) ? 0.0f : mu * t / fc;
                     } else if (KIND == BWD_ART || KIND == BWD_XTT_ART) {
-                        a.vol[vi] = a.vol[vi] + a.alpha * t;
+                        vol[vi] = vol[vi] + a.alpha * t;
                     } else {  // BWD_FDK
-                        a.vol[vi] = a.overwrite ? t : a.vol[vi] + t;
+                        vol[vi] = a.overwrite ? t : vol[vi] + t;
                     }
                 }
             }
         }
     }
+}
+
+template <int KIND, bool AXIS, int CPW, int ZPL, int KM>
+__global__ __launch_bounds__(kBwdThreads) void k_backward(const BwdArgs a) {
+    extern __shared__ float stage[];  // 2 x kStageBuf
+    const int o = (blockIdx.x % 8) * a.per_xcd + blockIdx.x / 8;
+    if (o >= a.n_tiles) return;
+    backward_item<KIND, AXIS, CPW, ZPL, KM>(a, o, 0, stage);
+}
+
+// Persistent backprojection over many problems (plain MLEM / ART kinds): the
+// same work queue as k_forward_batch; items[i] is one problem with
+// item_end[i] - item_end[i-1] tiles, and its own tile shape (zpl, gx).
+template <int KIND, int ZPL>
+__global__ __launch_bounds__(kBwdThreads) void k_backward_batch(const BwdArgs* __restrict__ items,
+                                                                const int* __restrict__ item_end, int n_items,
+                                                                unsigned* queue, int step) {
+    extern __shared__ float stage[];  // 2 x kStageBuf; the last word holds the ticket
+    typedef __attribute__((address_space(3))) unsigned lds_uint;
+    lds_uint* const tk = (lds_uint*)stage + (2 * kStageBuf - 1);
+    const int total = item_end[n_items - 1];
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) *tk = take_ticket(queue, (unsigned)total);
+        __syncthreads();
+        // uniform across the block: keeps the item's arguments in scalar registers
+        const int t = __builtin_amdgcn_readfirstlane((int)*tk);
+        if (t >= total) break;
+        const int i = find_item(item_end, n_items, t);
+        const BwdArgs& a = items[i];
+        const int o = t - (i ? item_end[i - 1] : 0);
+        if (a.axis)
+            backward_item<KIND, true, 2, ZPL, 1>(a, o, step, stage);
+        else
+            backward_item<KIND, false, 2, ZPL, 1>(a, o, step, stage);
+    }
+}
+
+// Per-segment max |x|: segment i is x[seg_off[i], seg_off[i+1]), its maximum
+// goes to out[i] (zeroed beforehand); grid (blocks per segment, segments).
+__global__ __launch_bounds__(256) void k_absmax_seg(const float* const* __restrict__ segs,
+                                                    const int64_t* __restrict__ seg_len, float* out) {
+    const float* x = segs[blockIdx.y];
+    const int64_t n = seg_len[blockIdx.y];
+    float m = 0.f;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        m = fmaxf(m, fabsf(x[i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax((unsigned int*)(out + blockIdx.y), __float_as_uint(m));
 }
 
 // ---------------------------------------------------------------------------
@@ -1031,22 +1175,34 @@ hipError_t launch_forward(const FwdArgs& a, int kind, bool axis, hipStream_t str
     return hipGetLastError();
 }
 
-hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream) {
+int default_zpl(int N) { return N > 128 ? 4 : (N > 64 ? 2 : 1); }
+
+bool backward_shape(BwdArgs& a, int zpl, int gx, bool xtt) {
     const int N = a.d.N;
-    const bool xtt = (kind == BWD_XTT_RAW || kind == BWD_XTT_MLEM || kind == BWD_XTT_ART);
-    // Plain / FDK: 2x2 columns x ZPL z per lane, 16 waves as 4x4 column groups
-    // (an 8x8-column tile, 64*ZPL z).  XTT (2K accumulators per voxel): one
-    // column x one z per lane, 4x4 columns x 64 z per block.
+    // Plain / FDK: 2x2 columns x zpl z per lane, 16 waves as gx x gy column
+    // groups (a (2gx) x (2gy)-column tile, 64*zpl z).  XTT (2K accumulators per
+    // voxel): one column x one z per lane.
     const int CPW = xtt ? 1 : 2;
-    const int zpl = xtt ? 1 : (N > 128 ? 4 : (N > 64 ? 2 : 1));
-    a.gx = 4;
-    a.gy = kBwdWaves / 4;
-    a.zw = 1;
+    if (xtt) zpl = 1;
+    if (zpl == 0) zpl = default_zpl(N);
+    if (gx == 0) gx = 4;
+    if (!(zpl == 1 || zpl == 2 || zpl == 4)) return false;
+    if (!(gx == 1 || gx == 2 || gx == 4 || gx == 8 || gx == 16)) return false;
+    a.zpl = zpl;
+    a.gx = gx;
+    a.gy = kBwdWaves / gx;
     a.tiles_x = (N + CPW * a.gx - 1) / (CPW * a.gx);
     a.tiles_y = (N + CPW * a.gy - 1) / (CPW * a.gy);
     a.tiles_z = (N + 64 * zpl - 1) / (64 * zpl);
     a.n_tiles = a.tiles_x * a.tiles_y * a.tiles_z;
     a.per_xcd = (a.n_tiles + 7) / 8;
+    return true;
+}
+
+hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream) {
+    const bool xtt = (kind == BWD_XTT_RAW || kind == BWD_XTT_MLEM || kind == BWD_XTT_ART);
+    if (!backward_shape(a, a.zpl, a.gx, xtt)) return hipErrorInvalidValue;
+    const int zpl = a.zpl;
     const unsigned grid = (unsigned)(8 * a.per_xcd);
     const size_t lds = 2 * kStageBuf * sizeof(float);
 #define XR_BWD(KD, A, C, Z, KM) \
@@ -1094,6 +1250,69 @@ hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream) {
 #undef XR_BWD_PLAIN
 #undef XR_BWD_AX
 #undef XR_BWD
+    return hipGetLastError();
+}
+
+hipError_t launch_forward_batch(const FwdArgs* items, const int* item_end, int n_items, unsigned* queue,
+                                int ticket_slot, size_t lds_bytes, int grid, int step, int loss_shift, int kind,
+                                hipStream_t stream) {
+    if (n_items <= 0 || grid <= 0) return hipSuccess;
+    if (kind == FWD_PLAIN)
+        hipLaunchKernelGGL((k_forward_batch<FWD_PLAIN>), dim3(grid), dim3(64 * kFwdWaves), lds_bytes, stream, items,
+                           item_end, n_items, queue, ticket_slot, step, loss_shift);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// Batched backprojection kernel of (kind, zpl), or nullptr.
+const void* backward_batch_kernel(int kind, int zpl) {
+#define XR_BB(KD)                                                        \
+    if (kind == KD) {                                                    \
+        if (zpl == 4) return (const void*)k_backward_batch<KD, 4>;       \
+        if (zpl == 2) return (const void*)k_backward_batch<KD, 2>;       \
+        if (zpl == 1) return (const void*)k_backward_batch<KD, 1>;       \
+    }
+    XR_BB(BWD_MLEM)
+    XR_BB(BWD_ART)
+#undef XR_BB
+    return nullptr;
+}
+
+hipError_t launch_backward_batch(const BwdArgs* items, const int* item_end, int n_items, unsigned* queue, int grid,
+                                 int step, int kind, int zpl, hipStream_t stream) {
+    if (n_items <= 0 || grid <= 0) return hipSuccess;
+    const void* k = backward_batch_kernel(kind, zpl);
+    if (!k) return hipErrorInvalidValue;
+    void* args[] = {(void*)&items, (void*)&item_end, (void*)&n_items, (void*)&queue, (void*)&step};
+    return hipLaunchKernel(k, dim3(grid), dim3(kBwdThreads), args, backward_batch_lds_bytes(), stream);
+}
+
+size_t backward_batch_lds_bytes() { return 2 * kStageBuf * sizeof(float); }
+
+int batch_blocks_per_cu_forward(int kind, size_t lds_bytes) {
+    int n = 0;
+    if (kind != FWD_PLAIN) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_forward_batch<FWD_PLAIN>, 64 * kFwdWaves,
+                                                     lds_bytes) != hipSuccess)
+        return 0;
+    return n;
+}
+
+int batch_blocks_per_cu_backward(int kind, int zpl) {
+    int n = 0;
+    const void* k = backward_batch_kernel(kind, zpl);
+    if (!k) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBwdThreads, backward_batch_lds_bytes()) != hipSuccess)
+        return 0;
+    return n;
+}
+
+hipError_t launch_absmax_seg(const float* const* segs, const int64_t* seg_len, float* out, int n_seg,
+                             int blocks_per_seg, hipStream_t stream) {
+    if (n_seg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_absmax_seg, dim3((unsigned)blocks_per_seg, (unsigned)n_seg), dim3(256), 0, stream, segs,
+                       seg_len, out);
     return hipGetLastError();
 }
 

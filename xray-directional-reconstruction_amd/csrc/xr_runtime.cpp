@@ -14,46 +14,14 @@
 #include <vector>
 
 #include "../../include/xr.h"
+#include "xr_host.hpp"
 #include "xr_internal.hpp"
 #include "xr_random.hpp"
 
 using namespace xr;
 
-struct xr_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    int32_t* d_codes = nullptr;  // per-call view lists
-    size_t codes_cap = 0;
-    uint8_t* d_flags = nullptr;  // per-call accumulate flags
-    size_t flags_cap = 0;
-    double* d_loss = nullptr;
-    float* d_scal = nullptr;  // [0] = 1.0f, [1] = max |voxel value| of the last forward launch
-    bool timing = false;
-    struct Ev {
-        int kind;
-        hipEvent_t a, b;
-    };
-    std::vector<Ev> events;
-    std::vector<hipEvent_t> pool;
-};
-
-struct xr_plan {
-    xr_context* ctx = nullptr;
-    xr_geometry g{};
-    int n_cond = 0, n_basis = 0, conv = 0, dir = 0;
-    std::vector<float> cond_R, cond_t, cond_offset, basis, grating;
-    std::vector<char> axis_cond;  // 1: every view of the condition has au[2] = ad[2] = 0
-    ViewGeom* d_vg = nullptr;
-    ChordGeom* d_cg = nullptr;
-    float* d_g2 = nullptr;    // [n_cond*P][K]: (S_k.G)^2
-    float* d_norm = nullptr;  // [n_cond][P][V][U]: L/(factor+1e-5)
-    Dims dims{};
-    Basis bas{};
-    float xtt_wsum = 0.f;  // sum_k |S_k|^2 max_view (S_k.G)^2: XTT splat value per unit |mu|
-};
-
-namespace {
+namespace xr {
+namespace host {
 
 int fail(xr_context* ctx, int code, const char* fmt, ...) {
     char buf[1024];
@@ -64,35 +32,6 @@ int fail(xr_context* ctx, int code, const char* fmt, ...) {
     if (ctx) ctx->err = buf;
     return code;
 }
-
-#define XR_HIP(ctx, call)                                                                                      \
-    do {                                                                                                       \
-        hipError_t e_ = (call);                                                                                \
-        if (e_ != hipSuccess) return fail((ctx), XR_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
-    } while (0)
-
-#define XR_TRY(expr)          \
-    do {                      \
-        int r_ = (expr);      \
-        if (r_ != XR_OK) return r_; \
-    } while (0)
-
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t alloc(size_t count) {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        if (count == 0) return hipSuccess;
-        return hipMalloc((void**)&p, count * sizeof(T));
-    }
-};
 
 float theta_f32(int n, int P) {
     // ir.cu:1860: 2.0f * (float)M_PI * (float)n / (float)P, each step in FP32.
@@ -217,25 +156,6 @@ hipEvent_t get_event(xr_context* ctx) {
     return e;
 }
 
-// Launch wrapper: brackets the launch with events when timing is enabled.
-template <class F>
-int timed(xr_context* ctx, int kind, F&& launch) {
-    hipEvent_t a = nullptr, b = nullptr;
-    if (ctx->timing) {
-        a = get_event(ctx);
-        b = get_event(ctx);
-        if (a && b) XR_HIP(ctx, hipEventRecord(a, ctx->stream));
-    }
-    const hipError_t e = launch();
-    if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
-    if (ctx->timing && a && b) {
-        XR_HIP(ctx, hipEventRecord(b, ctx->stream));
-        ctx->events.push_back({kind, a, b});
-    }
-    return XR_OK;
-}
-
-int ensure_norm(xr_plan* p);
 
 // vol_cond_stride: plain volumes of several conditions side by side (N^3), or 0
 // for a single volume; code_base: proj/sino hold the images of codes >= code_base.
@@ -277,20 +197,28 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
     // All conditions in one launch share the axis specialisation only if all are axis-aligned.
     bool axis = true;
     for (int c = 0; c < p->n_cond; c++) axis = axis && p->axis_cond[c];
-    // Detector rows per launch: the int64 band accumulator (kBandStride entries
-    // per row, two guard rows) stays within 64 KB of LDS; taller detectors are
-    // split into equal row chunks (untilted scans walk only the z range that
-    // reaches a chunk's rows).
+    // Detector rows in equal chunks, one launch each (untilted scans walk only
+    // the z range that reaches a chunk's rows).
     const int V = p->g.n_det_v;
-    const int max_rows = (64 * 1024 / 8 - 16) / kBandStride - 2;
-    const int n_chunks = (V + max_rows - 1) / max_rows;
-    const int rows = (V + n_chunks - 1) / n_chunks;
+    const int rows = forward_rows(p);
     for (int v0 = 0; v0 < V; v0 += rows) {
         a.v0 = v0;
         a.vb = std::min(rows, V - v0);
         XR_TRY(timed(ctx, 0, [&] { return launch_forward(a, kind, axis, ctx->stream); }));
     }
     return XR_OK;
+}
+
+int forward_rows(const xr_plan* p) {
+    // Detector rows per launch: the int64 band accumulator (kBandStride entries
+    // per row, two guard rows, a reduction slot) stays within 64 KB of LDS;
+    // taller detectors are split into equal row chunks.  tiles.fwd_rows caps
+    // the chunk height further (smaller LDS footprint, more resident blocks).
+    const int V = p->g.n_det_v;
+    int max_rows = (64 * 1024 / 8 - 16) / kBandStride - 2;
+    if (p->tiles.fwd_rows > 0) max_rows = std::min(max_rows, (int)p->tiles.fwd_rows);
+    const int n_chunks = (V + max_rows - 1) / max_rows;
+    return (V + n_chunks - 1) / n_chunks;
 }
 
 int backward_launch(xr_plan* p, int kind, const float* proj, int code_base, const int32_t* d_codes, int n_views,
@@ -311,6 +239,8 @@ int backward_launch(xr_plan* p, int kind, const float* proj, int code_base, cons
     a.g2 = p->d_g2;
     a.alpha = alpha;
     a.overwrite = overwrite;
+    a.zpl = p->tiles.bwd_zpl;
+    a.gx = p->tiles.bwd_gx;
     bool axis = true;
     for (int c = 0; c < p->n_cond; c++) axis = axis && p->axis_cond[c];
     return timed(ctx, 1, [&] { return launch_backward(a, kind, axis, ctx->stream); });
@@ -333,6 +263,13 @@ int ensure_norm(xr_plan* p) {
     XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return XR_OK;
 }
+
+}  // namespace host
+}  // namespace xr
+
+using namespace xr::host;
+
+namespace {
 
 // ---------------------------------------------------------------- JSON ----
 // Minimal JSON reader for settings.json (objects, arrays, numbers, strings,
@@ -562,6 +499,15 @@ int xr_context_set_stream(xr_context* ctx, void* hip_stream) {
 
 int xr_context_synchronize(xr_context* ctx) {
     if (!ctx) return XR_ERR_INVALID;
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XR_OK;
+}
+
+int xr_memcpy(xr_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return XR_ERR_INVALID;
+    if (bytes == 0) return XR_OK;
+    if (!dst || !src) return fail(ctx, XR_ERR_INVALID, "NULL copy address");
+    XR_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream));
     XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return XR_OK;
 }
