@@ -280,6 +280,39 @@ int xr_batch_reconstruct(xr_context* ctx, int n, const xr_geometry* geoms, const
                          const int32_t* batch, float* const* sino, float* const* vol, int epochs, int direction,
                          int method, float lambda, int flags, double* losses);
 
+/* ---- learned warm start: 3-level 3-D U-Net -------------------------------- */
+
+/* No counterpart in the reference.  A U-Net of base width C (8..32, multiple
+ * of 8): 3x3x3 convolutions + ReLU at widths C, 2C, 4C over three resolution
+ * levels (2x2x2 average pooling, nearest x2 upsampling, concatenated skips),
+ * BF16 weights and activations on MFMA with FP32 accumulation, and a residual
+ * 1x1x1 head: y = x + s*net(x/s), s = mean(x) unless a scale > 0 is given.
+ * Volumes are device FP32 in the column layout (any D0 x D1 x D2 with each a
+ * multiple of 4; D2 is the fastest axis).  Weights are He-initialised from a
+ * Philox stream of `seed`, with a zero head, so a new network is the identity. */
+typedef struct xr_unet xr_unet;
+int xr_unet_create(xr_context* ctx, int base_channels, uint64_t seed, xr_unet** out);
+void xr_unet_destroy(xr_unet* net);
+int64_t xr_unet_num_params(const xr_unet* net);
+/* Flat FP32 parameters: per layer W[Cout][27][Cin] then b[Cout] (layers 0-9),
+ * then the head's w[C] and b. */
+int xr_unet_get_params(xr_unet* net, float* host);
+int xr_unet_set_params(xr_unet* net, const float* host);
+int xr_unet_get_grads(xr_unet* net, float* host);
+/* Inference: y = x + s*net(x/s). */
+int xr_unet_forward(xr_unet* net, const float* x_dev, float* y_dev, int D0, int D1, int D2, float scale);
+/* Training: loss = mean((x/s + net(x/s)) - t/s)^2 (FP64 sum) and its gradient
+ * into the network's gradient buffer. */
+int xr_unet_loss_grad(xr_unet* net, const float* x_dev, const float* t_dev, int D0, int D1, int D2, float scale,
+                      double* loss);
+/* One Adam step on the current gradients (bias corrected). */
+int xr_unet_adam_step(xr_unet* net, float lr, float beta1, float beta2, float eps);
+/* Scale s used by the last forward / loss_grad call. */
+float xr_unet_last_scale(const xr_unet* net);
+/* Raw parameter file: 16-byte header ("XUNT", version 1, C, count) + FP32. */
+int xr_unet_save(xr_unet* net, const char* path);
+int xr_unet_load(xr_unet* net, const char* path);
+
 /* ---- settings (init_params, params.cu:42-104) -------------------------- */
 
 /* Parsed settings.json entry for one tag, same schema as the reference's file

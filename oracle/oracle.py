@@ -48,6 +48,10 @@ def _load():
         "orc_backward_view": (None, [I, G, S, I, I, _f, I, _d, _d]),
         "orc_subset_orders": (None, [I, I, I, _i32]),
         "orc_ir_reconstruct": (None, [I, G, S, _f, _f, I, I, I, I, F, _d, _d]),
+        "orc_unet_num_params": (ctypes.c_int64, [I]),
+        "orc_conv3": (None, [_f, I, I, I, I, _f, _f, I, I, _d]),
+        "orc_unet_forward": (None, [_f, I, _f, I, I, I, F, I, _f]),
+        "orc_unet_loss_grad": (ctypes.c_double, [_f, I, _f, _f, I, I, I, F, I, _d]),
         "orc_xtt_reconstruct": (None, [I, G, S, _f, _f, I, I, I, I, F, _d, _d, _d]),
         "orc_fdk_weight": (None, [I, G, _d]),
         "orc_fdk_taps": (None, [I, G, I, _d]),
@@ -262,3 +266,42 @@ def phantom_fibres(N, vs, params, basis):
 
 def num_threads():
     return _lib.orc_num_threads()
+
+
+# ---- learned warm-start network (restatement of the product's U-Net) ----
+
+def unet_num_params(C):
+    return int(_lib.orc_unet_num_params(C))
+
+
+def unet_forward(params, C, x, scale, emulate_bf16=True):
+    """x: (D0, D1, D2) float32 (D2 fastest); returns y = x + s*net(x/s) (float32)."""
+    p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    d0, d1, d2 = x.shape
+    y = np.zeros_like(x)
+    _lib.orc_unet_forward(_fp(p), C, _fp(x), d0, d1, d2, scale, int(emulate_bf16), _fp(y))
+    return y
+
+
+def unet_loss_grad(params, C, x, t, scale, emulate_bf16=True):
+    """Returns (loss, grads (FP64, same layout as params))."""
+    p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    d0, d1, d2 = x.shape
+    g = np.zeros(p.size, dtype=np.float64)
+    loss = _lib.orc_unet_loss_grad(_fp(p), C, _fp(x), _fp(t), d0, d1, d2, scale, int(emulate_bf16), _dp(g))
+    return float(loss), g
+
+
+def conv3(x, W, b, relu=False):
+    """x: (D0, D1, D2, Cin); W: (Cout, 27, Cin); returns (D0, D1, D2, Cout) FP64."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    W = np.ascontiguousarray(W, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    d0, d1, d2, cin = x.shape
+    cout = W.shape[0]
+    out = np.zeros((d0, d1, d2, cout))
+    _lib.orc_conv3(_fp(x), cin, d0, d1, d2, _fp(W), _fp(b), cout, int(relu), _dp(out))
+    return out

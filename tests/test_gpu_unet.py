@@ -1,0 +1,143 @@
+"""GPU parity of the warm-start U-Net (BF16 MFMA implicit-GEMM kernels) with
+the CPU oracle's restatement (oracle/oracle_unet.cpp, BF16 emulation: the same
+rounding points, FP64 sums).  The product sums in FP32 in another order, so a
+value at a BF16 rounding boundary can land one BF16 step (2^-8) away; the
+tolerances are relative L2 norms over whole tensors:
+  forward (network part y - x):   <= 1e-2
+  gradients, per layer:           <= 5e-2 (after masking and two BF16 roundings)
+  loss:                           <= 1e-3 relative
+  training curve (20 Adam steps): <= 3e-2 relative per step
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from test_oracle_unet import random_params
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def dev(a, torch_dev):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32).reshape(-1)).to(torch_dev)
+
+
+def layer_slices(C):
+    io = [(8, C), (C, C), (C, 2 * C), (2 * C, 2 * C), (2 * C, 4 * C), (4 * C, 4 * C), (6 * C, 2 * C),
+          (2 * C, 2 * C), (3 * C, C), (C, C)]
+    out, off = [], 0
+    for i, (cin, cout) in enumerate(io):
+        nw = cout * 27 * cin
+        out.append(("W%d" % i, slice(off, off + nw)))
+        out.append(("b%d" % i, slice(off + nw, off + nw + cout)))
+        off += nw + cout
+    out.append(("head", slice(off, off + C + 1)))
+    return out
+
+
+def test_new_network_is_identity(xr, ctx, torch_dev):
+    import torch
+
+    net = xr.UNet(ctx, 24, seed=5)
+    x = torch.rand(16 * 16 * 16, device=torch_dev) * 0.05
+    y = torch.empty_like(x)
+    net.forward(x, y, (16, 16, 16))
+    torch.cuda.synchronize()
+    assert torch.equal(x, y)
+    assert abs(net.last_scale() - float(x.mean())) <= 1e-6 * float(x.mean())
+    net.close()
+
+
+@pytest.mark.parametrize("C,dims", [(24, (16, 16, 16)), (8, (8, 12, 16)), (16, (12, 8, 8))])
+def test_forward_matches_oracle(orc, xr, ctx, torch_dev, C, dims):
+    import torch
+
+    net = xr.UNet(ctx, C, seed=1)
+    p = random_params(orc, C, 30 + C)
+    net.set_params(p)
+    rng = np.random.default_rng(C)
+    x = rng.uniform(0.0, 0.05, dims).astype(np.float32)
+    s = float(x.mean())
+    xd = dev(x, torch_dev)
+    yd = torch.empty_like(xd)
+    net.forward(xd, yd, dims, scale=s)
+    torch.cuda.synchronize()
+    got = yd.cpu().numpy().reshape(dims)
+    want = orc.unet_forward(p, C, x, s, emulate_bf16=True)
+    err = np.linalg.norm((got - x) - (want - x)) / np.linalg.norm(want - x)
+    print("C", C, dims, "forward rel err", err, flush=True)
+    assert err <= 1e-2
+    net.close()
+
+
+@pytest.mark.parametrize("C,dims", [(8, (8, 8, 8)), (24, (8, 12, 16))])
+def test_gradients_match_oracle(orc, xr, ctx, torch_dev, C, dims):
+    net = xr.UNet(ctx, C, seed=1)
+    p = random_params(orc, C, 40 + C)
+    net.set_params(p)
+    rng = np.random.default_rng(100 + C)
+    x = rng.uniform(0.0, 0.05, dims).astype(np.float32)
+    t = np.maximum(x + rng.normal(0, 0.005, dims), 0).astype(np.float32)
+    s = float(x.mean())
+    loss = net.loss_grad(dev(x, torch_dev), dev(t, torch_dev), dims, scale=s)
+    g = net.grads()
+    want_loss, want = orc.unet_loss_grad(p, C, x, t, s, emulate_bf16=True)
+    print("C", C, "loss", loss, want_loss, flush=True)
+    assert abs(loss - want_loss) <= 1e-3 * want_loss
+    for name, sl in layer_slices(C):
+        w = want[sl]
+        if name == "W0":  # only input channel 0 carries data
+            w = w.reshape(C, 27, 8)[:, :, 0].reshape(-1)
+            gg = g[sl].reshape(C, 27, 8)
+            assert np.all(gg[:, :, 1:] == 0)
+            gg = gg[:, :, 0].reshape(-1)
+        else:
+            gg = g[sl]
+        err = np.linalg.norm(gg - w) / max(np.linalg.norm(w), 1e-30)
+        print("  ", name, "rel err", err, flush=True)
+        assert err <= 5e-2, name
+    net.close()
+
+
+def test_training_curve_matches_oracle(xr, ctx, torch_dev):
+    sys.path.insert(0, GOLD)
+    import make_unet_curve as mk
+
+    fx = json.load(open(os.path.join(GOLD, "unet_loss_curve.json")))
+    cfg = fx["config"]
+    p, x, t = mk.tiny_problem(cfg)
+    net = xr.UNet(ctx, cfg["C"], seed=0)
+    net.set_params(p)
+    xd, td = dev(x, torch_dev), dev(t, torch_dev)
+    dims = tuple(cfg["dims"])
+    s = float(np.float32(x.mean()))
+    curve = []
+    for _ in range(cfg["steps"]):
+        curve.append(net.loss_grad(xd, td, dims, scale=s))
+        net.adam_step(cfg["lr"], cfg["beta1"], cfg["beta2"], cfg["eps"])
+    print("gpu", curve, flush=True)
+    print("fixture", fx["loss"], flush=True)
+    np.testing.assert_allclose(curve, fx["loss"], rtol=3e-2)
+    assert curve[-1] < 0.7 * curve[0]
+    net.close()
+
+
+def test_save_load_roundtrip(xr, ctx, tmp_path):
+    a = xr.UNet(ctx, 16, seed=3)
+    path = str(tmp_path / "net.xunt")
+    a.save(path)
+    b = xr.UNet(ctx, 16, seed=4)
+    assert not np.array_equal(a.params(), b.params())
+    b.load(path)
+    assert np.array_equal(a.params(), b.params())
+    c = xr.UNet(ctx, 24, seed=4)
+    with pytest.raises(xr.XRError):
+        c.load(path)
+    for n in (a, b, c):
+        n.close()

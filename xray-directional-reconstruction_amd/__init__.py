@@ -143,6 +143,18 @@ def _load():
         "xr_batch_autotune": (I, [P]),
         "xr_batch_run": (I, [P, I, _c_double_p]),
         "xr_batch_elapsed_ms": (I, [P, _c_double_p]),
+        "xr_unet_create": (I, [P, I, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+        "xr_unet_destroy": (V, [P]),
+        "xr_unet_num_params": (ctypes.c_int64, [P]),
+        "xr_unet_get_params": (I, [P, _c_float_p]),
+        "xr_unet_set_params": (I, [P, _c_float_p]),
+        "xr_unet_get_grads": (I, [P, _c_float_p]),
+        "xr_unet_forward": (I, [P, P, P, I, I, I, F]),
+        "xr_unet_loss_grad": (I, [P, P, P, I, I, I, F, _c_double_p]),
+        "xr_unet_adam_step": (I, [P, F, F, F, F]),
+        "xr_unet_last_scale": (F, [P]),
+        "xr_unet_save": (I, [P, ctypes.c_char_p]),
+        "xr_unet_load": (I, [P, ctypes.c_char_p]),
         "xr_batch_reconstruct": (I, [P, I, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct), _c_int32_p,
                                      ctypes.POINTER(_c_float_p), ctypes.POINTER(_c_float_p), I, I, I, F, I,
                                      _c_double_p]),
@@ -502,6 +514,66 @@ class Batch:
         ms = ctypes.c_double()
         self.ctx.check(_lib.xr_batch_elapsed_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+
+class UNet:
+    """The learned warm-start network (xr_unet_*): y = x + s*net(x/s) on device
+    volumes in the column layout (dims (D0, D1, D2), D2 fastest)."""
+
+    def __init__(self, ctx, base_channels=24, seed=0):
+        self.ctx, self.C = ctx, base_channels
+        h = ctypes.c_void_p()
+        ctx.check(_lib.xr_unet_create(ctx.h, base_channels, seed, ctypes.byref(h)))
+        self.h = h
+        self.n_params = int(_lib.xr_unet_num_params(h))
+
+    def close(self):
+        if self.h:
+            _lib.xr_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.n_params, dtype=np.float32)
+        self.ctx.check(_lib.xr_unet_get_params(self.h, _fptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.n_params:
+            raise XRError("expected %d parameters" % self.n_params)
+        self.ctx.check(_lib.xr_unet_set_params(self.h, _fptr(p)))
+
+    def grads(self):
+        out = np.zeros(self.n_params, dtype=np.float32)
+        self.ctx.check(_lib.xr_unet_get_grads(self.h, _fptr(out)))
+        return out
+
+    def forward(self, x, y, dims, scale=0.0):
+        self.ctx.check(_lib.xr_unet_forward(self.h, _dptr(x), _dptr(y), dims[0], dims[1], dims[2], scale))
+
+    def loss_grad(self, x, t, dims, scale=0.0):
+        loss = ctypes.c_double()
+        self.ctx.check(_lib.xr_unet_loss_grad(self.h, _dptr(x), _dptr(t), dims[0], dims[1], dims[2], scale,
+                                              ctypes.byref(loss)))
+        return loss.value
+
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.ctx.check(_lib.xr_unet_adam_step(self.h, lr, beta1, beta2, eps))
+
+    def last_scale(self):
+        return float(_lib.xr_unet_last_scale(self.h))
+
+    def save(self, path):
+        self.ctx.check(_lib.xr_unet_save(self.h, path.encode()))
+
+    def load(self, path):
+        self.ctx.check(_lib.xr_unet_load(self.h, path.encode()))
 
 
 def fdk_taps(geom, filter=FILTER_REFERENCE):

@@ -21,7 +21,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["xr_kernels.hip", "xr_runtime.cpp", "xr_batch.cpp"]
+SOURCES = ["xr_kernels.hip", "xr_runtime.cpp", "xr_batch.cpp", "xr_unet_kernels.hip", "xr_unet.cpp"]
 
 
 def _run(cmd):

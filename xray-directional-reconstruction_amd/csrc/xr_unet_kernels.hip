@@ -1,0 +1,604 @@
+// HIP kernels (gfx950) of the learned warm start: a 3-level 3-D U-Net on
+// channels-last BF16 activations with FP32 accumulation.  No counterpart in
+// the reference (it has no learned component).
+//
+// Activations: act[v * ld + off + c], v = voxel index of the column layout
+// (z fastest: v = z + N*(x + N*y)), c < C, BF16.  A tensor can be a channel
+// slice (off, ld) of a wider buffer, which is how skip connections are
+// concatenated without copies.
+//
+// 3x3x3 convolution, stride 1, zero padding 1, as an implicit GEMM on MFMA
+// (v_mfma_f32_16x16x32_bf16): M = voxels, N = output channels, K = 27*Cin with
+// k = tap*Cin + ci, tap = (d0+1)*9 + (d1+1)*3 + (d2+1).  Cin is a multiple of
+// 8 so the 8 consecutive k of one MFMA lane lie in one tap: one 16-byte load
+// of 8 channels of one neighbour voxel.  Weights are [Cout16][Kpad] BF16 (k
+// fastest, zero padded).  The data gradient is the same kernel on the output
+// gradient with mirrored taps and transposed channels (weights [Cin16][K'pad],
+// k' = (26 - tap)*Cout + co); the weight gradient contracts over voxels with
+// both operands transposed through LDS.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "xr_unet_internal.hpp"
+
+namespace xr {
+namespace nn {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) bf16x8 lds_bf16x8;
+typedef __attribute__((address_space(3))) __bf16 lds_bf16;
+
+__device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 zero8() {
+    bf16x8 z;
+#pragma unroll
+    for (int j = 0; j < 8; j++) z[j] = (__bf16)0.0f;
+    return z;
+}
+
+// Tap t -> offsets (d0, d1, d2) in {-1, 0, 1}.
+__device__ __forceinline__ void tap_offsets(int t, int& d0, int& d1, int& d2) {
+    d0 = t / 9 - 1;
+    d1 = (t / 3) % 3 - 1;
+    d2 = t % 3 - 1;
+}
+
+// ---------------------------------------------------------------------------
+// Forward / data-gradient convolution.  Block: 4 waves, 128 consecutive
+// voxels (32 per wave as two 16-row MFMA tiles) x all output channels (NT
+// tiles of 16).
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3(const ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t V = (int64_t)a.D0 * a.D1 * a.D2;
+    const int64_t vbase = (int64_t)blockIdx.x * 128 + wave * 32;
+    if (vbase >= V) return;
+    const int row = lane & 15, kg = lane >> 4;
+    int p0[2], p1[2], p2[2];
+    bool vin[2];
+#pragma unroll
+    for (int ms = 0; ms < 2; ms++) {
+        const int64_t v = vbase + ms * 16 + row;
+        vin[ms] = v < V;
+        const int64_t vv = vin[ms] ? v : 0;
+        p2[ms] = (int)(vv % a.D2);
+        p1[ms] = (int)((vv / a.D2) % a.D1);
+        p0[ms] = (int)(vv / ((int64_t)a.D2 * a.D1));
+    }
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int ms = 0; ms < 2; ms++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[ms][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int K = 27 * a.Cin;
+    const __bf16* __restrict__ in = a.in + a.in_off;
+    for (int kc = 0; kc < a.Kpad; kc += 32) {
+        const int k0 = kc + 8 * kg;
+        const int tap = k0 / a.Cin;
+        const int ci = k0 - tap * a.Cin;
+        int d0, d1, d2;
+        tap_offsets(tap, d0, d1, d2);
+        bf16x8 A[2];
+#pragma unroll
+        for (int ms = 0; ms < 2; ms++) {
+            const int n0 = p0[ms] + d0, n1 = p1[ms] + d1, n2 = p2[ms] + d2;
+            const bool ok = vin[ms] && k0 < K && (unsigned)n0 < (unsigned)a.D0 && (unsigned)n1 < (unsigned)a.D1 &&
+                            (unsigned)n2 < (unsigned)a.D2;
+            const int64_t nv = ((int64_t)n0 * a.D1 + n1) * a.D2 + n2;
+            A[ms] = ok ? *(const bf16x8*)(in + nv * a.in_ld + ci) : zero8();
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+            const bf16x8 B = *(const bf16x8*)(a.w + (int64_t)(nt * 16 + row) * a.Kpad + k0);
+#pragma unroll
+            for (int ms = 0; ms < 2; ms++) acc[ms][nt] = mfma(A[ms], B, acc[ms][nt]);
+        }
+    }
+    // D layout of the 16x16 MFMA: lane holds column (lane & 15), rows 4*kg + i.
+#pragma unroll
+    for (int ms = 0; ms < 2; ms++) {
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+            const int co = nt * 16 + row;
+            if (co >= a.Cout) continue;
+            const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int64_t v = vbase + ms * 16 + 4 * kg + i;
+                if (v >= V) continue;
+                float val = acc[ms][nt][i] + b;
+                if (a.relu) val = fmaxf(val, 0.f);
+                if (a.mask && !((float)a.mask[v * a.mask_ld + a.mask_off + co] > 0.f)) val = 0.f;
+                __bf16* o = a.out + v * a.out_ld + a.out_off + co;
+                if (a.accumulate) val += (float)*o;
+                *o = (__bf16)val;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: dw[co][k] += sum_v gout[v][co] * im2col(in)[v][k].
+// Block: 4 waves; all output channels (MT tiles of 16) x 64 k columns (16 per
+// wave) over a slab of voxels, 32 voxels per step.  Both operands are staged
+// transposed in LDS ([channel][voxel]) so that each MFMA lane reads 8
+// consecutive voxels with one 16-byte LDS load.  Partial sums are added to dw
+// with FP32 atomics (the summation order of the slabs is not fixed).
+// ---------------------------------------------------------------------------
+constexpr int kWgVox = 32;          // voxels per step
+constexpr int kWgRow = kWgVox + 8;  // LDS row stride (bf16), padded
+
+template <int MT>
+__global__ __launch_bounds__(256) void k_conv3_wgrad(const WgradArgs a) {
+    extern __shared__ __bf16 wg_smem[];
+    lds_bf16* const gT = (lds_bf16*)wg_smem;                    // [MT*16][kWgRow]
+    lds_bf16* const iT = (lds_bf16*)wg_smem + MT * 16 * kWgRow;  // [64][kWgRow]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int row = lane & 15, kg = lane >> 4;
+    const int kbase = blockIdx.x * 64;
+    const int64_t V = (int64_t)a.D0 * a.D1 * a.D2;
+    const int64_t v_beg = (int64_t)blockIdx.y * a.vox_per_block;
+    const int64_t v_end = v_beg + a.vox_per_block < V ? v_beg + a.vox_per_block : V;
+    const int K = 27 * a.Cin;
+    const int cg_out = a.Cout / 8;  // 8-channel groups of gout
+    // rows of gT beyond Cout stay zero
+    for (int i = tid; i < MT * 16 * kWgRow; i += 256) gT[i] = (__bf16)0.0f;
+    f32x4 acc[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // this thread's im2col column group (fixed over the slab): 32 voxels x 8 groups
+    const int iv = tid >> 3, ig = tid & 7;
+    const int k0 = kbase + 8 * ig;
+    const bool kin = k0 < K;
+    const int tap = kin ? k0 / a.Cin : 0;
+    const int ci = kin ? k0 - tap * a.Cin : 0;
+    int d0, d1, d2;
+    tap_offsets(tap, d0, d1, d2);
+    for (int64_t v0 = v_beg; v0 < v_end; v0 += kWgVox) {
+        __syncthreads();
+        // stage gout[v0 .. v0+31][0 .. Cout) transposed
+        for (int i = tid; i < kWgVox * cg_out; i += 256) {
+            const int vv = i / cg_out, g = i - vv * cg_out;
+            const int64_t v = v0 + vv;
+            const bf16x8 x = (v < v_end) ? *(const bf16x8*)(a.gout + v * a.g_ld + a.g_off + 8 * g) : zero8();
+#pragma unroll
+            for (int j = 0; j < 8; j++) gT[(8 * g + j) * kWgRow + vv] = x[j];
+        }
+        // stage im2col[v0 .. v0+31][kbase .. kbase+63] transposed
+        {
+            const int64_t v = v0 + iv;
+            bf16x8 x = zero8();
+            if (kin && v < v_end) {
+                const int p2 = (int)(v % a.D2), p1 = (int)((v / a.D2) % a.D1), p0 = (int)(v / ((int64_t)a.D2 * a.D1));
+                const int n0 = p0 + d0, n1 = p1 + d1, n2 = p2 + d2;
+                if ((unsigned)n0 < (unsigned)a.D0 && (unsigned)n1 < (unsigned)a.D1 && (unsigned)n2 < (unsigned)a.D2) {
+                    const int64_t nv = ((int64_t)n0 * a.D1 + n1) * a.D2 + n2;
+                    x = *(const bf16x8*)(a.in + nv * a.in_ld + a.in_off + ci);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) iT[(8 * ig + j) * kWgRow + iv] = x[j];
+        }
+        __syncthreads();
+        const bf16x8 B = *(const lds_bf16x8*)(iT + (16 * wave + row) * kWgRow + 8 * kg);
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) {
+            const bf16x8 A = *(const lds_bf16x8*)(gT + (mt * 16 + row) * kWgRow + 8 * kg);
+            acc[mt] = mfma(A, B, acc[mt]);
+        }
+    }
+    const int k = kbase + 16 * wave + row;
+    if (k >= K) return;
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int co = mt * 16 + 4 * kg + i;
+            if (co < a.Cout) atomicAdd(a.dw + (int64_t)co * a.ldw + k, acc[mt][i]);
+        }
+    }
+}
+
+// db[c] += sum_v g[v][c]; block = C threads (one per channel) over a voxel slab.
+__global__ void k_bias_grad(const __bf16* __restrict__ g, int ld, int off, int C, int64_t V, int64_t per_block,
+                            float* db) {
+    const int c = threadIdx.x;
+    if (c >= C) return;
+    const int64_t v0 = (int64_t)blockIdx.x * per_block;
+    const int64_t v1 = v0 + per_block < V ? v0 + per_block : V;
+    float s = 0.f;
+    for (int64_t v = v0; v < v1; v++) s += (float)g[v * ld + off + c];
+    atomicAdd(db + c, s);
+}
+
+// 2x2x2 average pooling: out[p][c] = mean of the 8 children (dims halve).
+__global__ void k_pool(const __bf16* __restrict__ in, int in_ld, int in_off, int C, int D0, int D1, int D2,
+                       __bf16* __restrict__ out) {
+    const int H0 = D0 / 2, H1 = D1 / 2, H2 = D2 / 2;
+    const int G = C / 8;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)H0 * H1 * H2 * G;
+    if (i >= n) return;
+    const int g = (int)(i % G);
+    const int64_t p = i / G;
+    const int q2 = (int)(p % H2), q1 = (int)((p / H2) % H1), q0 = (int)(p / ((int64_t)H2 * H1));
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < 8; c++) {
+        const int64_t v = ((int64_t)(2 * q0 + (c >> 2)) * D1 + (2 * q1 + ((c >> 1) & 1))) * D2 + (2 * q2 + (c & 1));
+        const bf16x8 x = *(const bf16x8*)(in + v * in_ld + in_off + 8 * g);
+#pragma unroll
+        for (int j = 0; j < 8; j++) s[j] += (float)x[j];
+    }
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; j++) o[j] = (__bf16)(0.125f * s[j]);
+    *(bf16x8*)(out + p * C + 8 * g) = o;
+}
+
+// Pooling backward into a (masked) slice: gin[child][c] (+)= gp[p][c]/8 * (mask[child][c] > 0).
+__global__ void k_pool_bwd(const __bf16* __restrict__ gp, int C, int D0, int D1, int D2, __bf16* gin, int g_ld,
+                           int g_off, const __bf16* __restrict__ mask, int m_ld, int m_off, int accumulate) {
+    const int G = C / 8;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)D0 * D1 * D2 * G;
+    if (i >= n) return;
+    const int g = (int)(i % G);
+    const int64_t v = i / G;
+    const int p2 = (int)(v % D2), p1 = (int)((v / D2) % D1), p0 = (int)(v / ((int64_t)D2 * D1));
+    const int64_t p = ((int64_t)(p0 / 2) * (D1 / 2) + p1 / 2) * (D2 / 2) + p2 / 2;
+    const bf16x8 x = *(const bf16x8*)(gp + p * C + 8 * g);
+    const bf16x8 m = *(const bf16x8*)(mask + v * m_ld + m_off + 8 * g);
+    bf16x8* o = (bf16x8*)(gin + v * g_ld + g_off + 8 * g);
+    bf16x8 old = accumulate ? *o : zero8();
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        float val = ((float)m[j] > 0.f) ? 0.125f * (float)x[j] : 0.f;
+        if (accumulate) val += (float)old[j];
+        r[j] = (__bf16)val;
+    }
+    *o = r;
+}
+
+// Nearest-neighbour upsampling x2 into a channel slice: out[child][off + c] = low[parent][c].
+__global__ void k_upsample(const __bf16* __restrict__ low, int C, int D0, int D1, int D2, __bf16* out, int o_ld,
+                           int o_off) {
+    const int G = C / 8;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)D0 * D1 * D2 * G;
+    if (i >= n) return;
+    const int g = (int)(i % G);
+    const int64_t v = i / G;
+    const int p2 = (int)(v % D2), p1 = (int)((v / D2) % D1), p0 = (int)(v / ((int64_t)D2 * D1));
+    const int64_t p = ((int64_t)(p0 / 2) * (D1 / 2) + p1 / 2) * (D2 / 2) + p2 / 2;
+    *(bf16x8*)(out + v * o_ld + o_off + 8 * g) = *(const bf16x8*)(low + p * C + 8 * g);
+}
+
+// Upsampling backward: glow[p][c] = sum of the 8 children's g[child][off + c].
+__global__ void k_upsample_bwd(const __bf16* __restrict__ g, int g_ld, int g_off, int C, int D0, int D1, int D2,
+                               __bf16* __restrict__ glow) {
+    const int H0 = D0 / 2, H1 = D1 / 2, H2 = D2 / 2;
+    const int G = C / 8;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)H0 * H1 * H2 * G;
+    if (i >= n) return;
+    const int gg = (int)(i % G);
+    const int64_t p = i / G;
+    const int q2 = (int)(p % H2), q1 = (int)((p / H2) % H1), q0 = (int)(p / ((int64_t)H2 * H1));
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < 8; c++) {
+        const int64_t v = ((int64_t)(2 * q0 + (c >> 2)) * D1 + (2 * q1 + ((c >> 1) & 1))) * D2 + (2 * q2 + (c & 1));
+        const bf16x8 x = *(const bf16x8*)(g + v * g_ld + g_off + 8 * gg);
+#pragma unroll
+        for (int j = 0; j < 8; j++) s[j] += (float)x[j];
+    }
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; j++) o[j] = (__bf16)s[j];
+    *(bf16x8*)(glow + p * C + 8 * gg) = o;
+}
+
+// Network input: x8[v] = (bf16(x[v] * inv_scale), 0, ..., 0).
+__global__ void k_input8(const float* __restrict__ x, int64_t V, float inv_scale, __bf16* __restrict__ x8) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    bf16x8 o = zero8();
+    o[0] = (__bf16)(x[v] * inv_scale);
+    *(bf16x8*)(x8 + v * 8) = o;
+}
+
+// Output head (1x1x1 conv, C -> 1) with the residual: y = x + scale*(w.e[v] + b),
+// w[C] = b.  Loss mode (t != null): r = (x/scale + w.e + b) - t/scale,
+// loss += r^2, g[v] = 2 r / V (normalised units).
+template <int C>
+__global__ __launch_bounds__(256) void k_head(const __bf16* __restrict__ e, int64_t V, const float* __restrict__ w,
+                                              const float* __restrict__ x, float scale, float* __restrict__ y,
+                                              const float* __restrict__ t, float* g, double* loss) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (v < V) {
+        float f = w[C];
+#pragma unroll
+        for (int c = 0; c < C; c += 8) {
+            const bf16x8 a = *(const bf16x8*)(e + v * C + c);
+#pragma unroll
+            for (int j = 0; j < 8; j++) f = fmaf(w[c + j], (float)a[j], f);
+        }
+        const float inv = 1.0f / scale;
+        if (y) y[v] = x[v] + scale * f;
+        if (t) {
+            const float r = (x[v] * inv + f) - t[v] * inv;
+            l = (double)r * (double)r;
+            g[v] = 2.0f * r / (float)V;
+        }
+    }
+    if (loss) {
+        for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+        if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
+    }
+}
+
+// Head backward: ge[v][c] = g[v] w[c] (e[v][c] > 0); dw[c] += sum_v g[v] e[v][c];
+// dw[C] (bias) += sum_v g[v].  Wave sums, then one atomic per value per block.
+template <int C>
+__global__ __launch_bounds__(256) void k_head_bwd(const __bf16* __restrict__ e, int64_t V, const float* __restrict__ w,
+                                                  const float* __restrict__ g, __bf16* __restrict__ ge, float* dw) {
+    __shared__ float red[C + 1][4];
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float part[C + 1];
+#pragma unroll
+    for (int c = 0; c <= C; c++) part[c] = 0.f;
+    if (v < V) {
+        const float gv = g[v];
+#pragma unroll
+        for (int c = 0; c < C; c += 8) {
+            const bf16x8 a = *(const bf16x8*)(e + v * C + c);
+            bf16x8 o;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                o[j] = (__bf16)(((float)a[j] > 0.f) ? gv * w[c + j] : 0.f);
+                part[c + j] = gv * (float)a[j];
+            }
+            *(bf16x8*)(ge + v * C + c) = o;
+        }
+        part[C] = gv;
+    }
+#pragma unroll
+    for (int c = 0; c <= C; c++) {
+        float s = part[c];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) red[c][wave] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x <= C) {
+        const int c = threadIdx.x;
+        atomicAdd(dw + c, red[c][0] + red[c][1] + red[c][2] + red[c][3]);
+    }
+}
+
+// Master FP32 weights [Cout][27][Cin] -> forward BF16 [Cout16][Kpad] and
+// data-gradient BF16 [Cin16][K'pad] (k' = (26 - tap)*Cout + co).
+__global__ void k_pack(const float* __restrict__ W, int Cin, int Cout, __bf16* wf, int Kpad, int cout16,
+                       __bf16* wd, int Kdpad, int cin16) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nf = (int64_t)cout16 * Kpad;
+    const int64_t nd = (int64_t)cin16 * Kdpad;
+    const int K = 27 * Cin, Kd = 27 * Cout;
+    if (i < nf) {
+        const int co = (int)(i / Kpad), k = (int)(i % Kpad);
+        wf[i] = (__bf16)((co < Cout && k < K) ? W[(int64_t)co * K + k] : 0.f);
+    } else if (wd && i < nf + nd) {
+        const int64_t j = i - nf;
+        const int ci = (int)(j / Kdpad), kd = (int)(j % Kdpad);
+        float val = 0.f;
+        if (ci < Cin && kd < Kd) {
+            const int tp = kd / Cout, co = kd % Cout;
+            val = W[(int64_t)co * K + (26 - tp) * Cin + ci];
+        }
+        wd[j] = (__bf16)val;
+    }
+}
+
+// Adam (bias-corrected step sizes precomputed on the host).
+__global__ void k_adam(float* p, const float* g, float* m, float* v, int64_t n, float b1, float b2, float lr_t,
+                       float eps_t) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr_t * mi / (sqrtf(vi) + eps_t);
+}
+
+// sum x over n floats (FP64 accumulation).
+__global__ __launch_bounds__(256) void k_sum(const float* __restrict__ x, int64_t n, double* out) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) s += x[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
+inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+hipError_t launch_conv3(const ConvArgs& a, hipStream_t s) {
+    const int64_t V = (int64_t)a.D0 * a.D1 * a.D2;
+    if (V == 0) return hipSuccess;
+    if (a.Cin % 8 != 0 || a.in_ld % 8 != 0 || a.in_off % 8 != 0 || a.Kpad % 32 != 0 || a.Kpad < 27 * a.Cin)
+        return hipErrorInvalidValue;
+    const int nt = (a.Cout + 15) / 16;
+    const dim3 grid(blocks(V, 128));
+    switch (nt) {
+#define XR_CONV(N) \
+    case N: hipLaunchKernelGGL(k_conv3<N>, grid, dim3(256), 0, s, a); break;
+        XR_CONV(1)
+        XR_CONV(2)
+        XR_CONV(3)
+        XR_CONV(4)
+        XR_CONV(5)
+        XR_CONV(6)
+        XR_CONV(7)
+        XR_CONV(8)
+        XR_CONV(9)
+#undef XR_CONV
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_conv3_wgrad(WgradArgs a, hipStream_t s) {
+    const int64_t V = (int64_t)a.D0 * a.D1 * a.D2;
+    if (V == 0) return hipSuccess;
+    if (a.Cin % 8 != 0 || a.Cout % 8 != 0 || a.in_ld % 8 != 0 || a.in_off % 8 != 0 || a.g_ld % 8 != 0 ||
+        a.g_off % 8 != 0 || a.ldw < 27 * a.Cin)
+        return hipErrorInvalidValue;
+    const int mt = (a.Cout + 15) / 16;
+    const int kblocks = (27 * a.Cin + 63) / 64;
+    // slabs: about 2048 blocks in all, whole 32-voxel steps
+    int64_t slabs = std::max<int64_t>(1, std::min<int64_t>((V + kWgVox - 1) / kWgVox, 2048 / kblocks + 1));
+    int64_t per = (V + slabs - 1) / slabs;
+    per = ((per + kWgVox - 1) / kWgVox) * kWgVox;
+    slabs = (V + per - 1) / per;
+    a.vox_per_block = per;
+    const dim3 grid((unsigned)kblocks, (unsigned)slabs);
+    const size_t lds = (size_t)(mt * 16 + 64) * kWgRow * sizeof(__bf16);
+    switch (mt) {
+#define XR_WG(M) \
+    case M: hipLaunchKernelGGL(k_conv3_wgrad<M>, grid, dim3(256), lds, s, a); break;
+        XR_WG(1)
+        XR_WG(2)
+        XR_WG(3)
+        XR_WG(4)
+        XR_WG(5)
+        XR_WG(6)
+        XR_WG(7)
+        XR_WG(8)
+        XR_WG(9)
+#undef XR_WG
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bias_grad(const __bf16* g, int ld, int off, int C, int64_t V, float* db, hipStream_t s) {
+    if (V == 0) return hipSuccess;
+    if (C > 1024) return hipErrorInvalidValue;
+    const int64_t nb = std::min<int64_t>(1024, (V + 255) / 256);
+    const int64_t per = (V + nb - 1) / nb;
+    hipLaunchKernelGGL(k_bias_grad, dim3(blocks(V, (int)per)), dim3(((C + 63) / 64) * 64), 0, s, g, ld, off, C, V,
+                       per, db);
+    return hipGetLastError();
+}
+
+hipError_t launch_pool(const __bf16* in, int in_ld, int in_off, int C, int D0, int D1, int D2, __bf16* out,
+                       hipStream_t s) {
+    if (C % 8 || in_ld % 8 || in_off % 8 || (D0 | D1 | D2) & 1) return hipErrorInvalidValue;
+    const int64_t n = (int64_t)(D0 / 2) * (D1 / 2) * (D2 / 2) * (C / 8);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pool, dim3(blocks(n, 256)), dim3(256), 0, s, in, in_ld, in_off, C, D0, D1, D2, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pool_bwd(const __bf16* gp, int C, int D0, int D1, int D2, __bf16* gin, int g_ld, int g_off,
+                           const __bf16* mask, int m_ld, int m_off, int accumulate, hipStream_t s) {
+    if (C % 8 || g_ld % 8 || g_off % 8 || m_ld % 8 || m_off % 8 || (D0 | D1 | D2) & 1) return hipErrorInvalidValue;
+    const int64_t n = (int64_t)D0 * D1 * D2 * (C / 8);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pool_bwd, dim3(blocks(n, 256)), dim3(256), 0, s, gp, C, D0, D1, D2, gin, g_ld, g_off, mask,
+                       m_ld, m_off, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_upsample(const __bf16* low, int C, int D0, int D1, int D2, __bf16* out, int o_ld, int o_off,
+                           hipStream_t s) {
+    if (C % 8 || o_ld % 8 || o_off % 8 || (D0 | D1 | D2) & 1) return hipErrorInvalidValue;
+    const int64_t n = (int64_t)D0 * D1 * D2 * (C / 8);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_upsample, dim3(blocks(n, 256)), dim3(256), 0, s, low, C, D0, D1, D2, out, o_ld, o_off);
+    return hipGetLastError();
+}
+
+hipError_t launch_upsample_bwd(const __bf16* g, int g_ld, int g_off, int C, int D0, int D1, int D2, __bf16* glow,
+                               hipStream_t s) {
+    if (C % 8 || g_ld % 8 || g_off % 8 || (D0 | D1 | D2) & 1) return hipErrorInvalidValue;
+    const int64_t n = (int64_t)(D0 / 2) * (D1 / 2) * (D2 / 2) * (C / 8);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_upsample_bwd, dim3(blocks(n, 256)), dim3(256), 0, s, g, g_ld, g_off, C, D0, D1, D2, glow);
+    return hipGetLastError();
+}
+
+hipError_t launch_input8(const float* x, int64_t V, float inv_scale, __bf16* x8, hipStream_t s) {
+    if (V == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_input8, dim3(blocks(V, 256)), dim3(256), 0, s, x, V, inv_scale, x8);
+    return hipGetLastError();
+}
+
+hipError_t launch_head(const __bf16* e, int C, int64_t V, const float* w, const float* x, float scale, float* y,
+                       const float* t, float* g, double* loss, hipStream_t s) {
+    if (V == 0) return hipSuccess;
+    switch (C) {
+#define XR_HEAD(N) \
+    case N: hipLaunchKernelGGL(k_head<N>, dim3(blocks(V, 256)), dim3(256), 0, s, e, V, w, x, scale, y, t, g, loss); break;
+        XR_HEAD(8)
+        XR_HEAD(16)
+        XR_HEAD(24)
+        XR_HEAD(32)
+#undef XR_HEAD
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_head_bwd(const __bf16* e, int C, int64_t V, const float* w, const float* g, __bf16* ge,
+                           float* dw, hipStream_t s) {
+    if (V == 0) return hipSuccess;
+    switch (C) {
+#define XR_HEADB(N) \
+    case N: hipLaunchKernelGGL(k_head_bwd<N>, dim3(blocks(V, 256)), dim3(256), 0, s, e, V, w, g, ge, dw); break;
+        XR_HEADB(8)
+        XR_HEADB(16)
+        XR_HEADB(24)
+        XR_HEADB(32)
+#undef XR_HEADB
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const float* W, int Cin, int Cout, __bf16* wf, int Kpad, __bf16* wd, int Kdpad,
+                       hipStream_t s) {
+    const int cout16 = (Cout + 15) / 16 * 16, cin16 = (Cin + 15) / 16 * 16;
+    const int64_t n = (int64_t)cout16 * Kpad + (wd ? (int64_t)cin16 * Kdpad : 0);
+    hipLaunchKernelGGL(k_pack, dim3(blocks(n, 256)), dim3(256), 0, s, W, Cin, Cout, wf, Kpad, cout16, wd, Kdpad,
+                       cin16);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, int64_t n, float b1, float b2, float lr_t,
+                       float eps_t, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adam, dim3(blocks(n, 256)), dim3(256), 0, s, p, g, m, v, n, b1, b2, lr_t, eps_t);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum(const float* x, int64_t n, double* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const int64_t nb = std::min<int64_t>(2048, (n + 255) / 256);
+    hipLaunchKernelGGL(k_sum, dim3((unsigned)nb), dim3(256), 0, s, x, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace nn
+}  // namespace xr
