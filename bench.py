@@ -49,6 +49,21 @@ def cpu_baseline(N, U, P, sdd, sod, det, vs, n_views=2):
                       % (len(views), P, N, U, U, dt)}
 
 
+def measured_traffic(kernel, N, P):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy), if one
+    matches this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_%d_%d.md" % (N, P))
+    if not os.path.exists(path):
+        return None
+    key = "traffic_bytes[%s]:" % kernel
+    with open(path) as f:
+        for line in f:
+            if line.startswith(key):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +74,8 @@ def main():
     ap.add_argument("--n-proj", type=int, default=720)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=2)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (views would shard "
@@ -115,15 +132,24 @@ def main():
     bwd_ms, bwd_n = ctx.timing_read(xr.TIMING_BACKWARD)
     ctx.timing(False)
 
-    vv = float(N3) * P  # voxel-view updates per pass
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    vv = float(N3) * P  # voxel-view updates per pass (one launch each way)
     ms_per_step = dt * 1e3 / args.steps
     gups = 2.0 * vv * args.steps / dt / 1e9
     fwd_avg, bwd_avg = fwd_ms / max(fwd_n, 1), bwd_ms / max(bwd_n, 1)
     dom_name, dom_ms = ("backprojection+update", bwd_avg) if bwd_avg >= fwd_avg else ("forward+ratio", fwd_avg)
     achieved = BYTES_PER_VOXEL_VIEW * vv / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic(dom_name, N, P)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
-                "kernel_ms": round(dom_ms, 3),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_name,
+                "kernel_ms": round(dom_ms, 3), "launches_per_pass": 1,
                 "algorithmic_bytes": "4 B per voxel-view x %d^3 voxels x %d views per launch" % (N, P)}
     out = {
         "metric": METRIC, "value": round(gups, 2), "unit": "GUPS", "n_gpus": 1, "steps": args.steps,

@@ -372,6 +372,10 @@ int xr_phantom_fibre_params(uint64_t seed, int count, int n_axes, double kappa, 
 int xr_phantom_fibres(xr_context* ctx, const double* params_host, int count, int n_vox, float vox_size,
                       const float* basis_host, int n_basis, float* vol_ref_dev);
 
+/* Scalar fibres (attenuation phantom): `value` added inside every fibre. */
+int xr_phantom_fibres_scalar(xr_context* ctx, const double* params_host, int count, int n_vox, float vox_size,
+                             float value, float* vol_ref_dev);
+
 /* Noise on device arrays of n floats (y may alias the input):
  * Poisson: y = -ln(max(c, 1)/I0), c ~ Poisson(I0 exp(-p));
  * Gaussian visibility: y = x (1 + sigma N(0, 1)).

@@ -128,6 +128,7 @@ def _load():
         "xr_phantom_cylinder_shell": (I, [P, I, F, D, D, D, P]),
         "xr_phantom_fibre_params": (I, [ctypes.c_uint64, I, I, D, D, D, D, D, D, _c_double_p, _c_double_p]),
         "xr_phantom_fibres": (I, [P, _c_double_p, I, I, F, _c_float_p, I, P]),
+        "xr_phantom_fibres_scalar": (I, [P, _c_double_p, I, I, F, F, P]),
         "xr_noise_poisson": (I, [P, P, P, ctypes.c_int64, D, ctypes.c_uint64]),
         "xr_noise_gaussian": (I, [P, P, P, ctypes.c_int64, D, ctypes.c_uint64]),
         "xr_philox4x32": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
@@ -312,6 +313,11 @@ class Context:
         b = np.ascontiguousarray(basis, dtype=np.float32).reshape(-1)
         self.check(_lib.xr_phantom_fibres(self.h, p.ctypes.data_as(_c_double_p), len(p) // 8, n_vox, vox_size,
                                           _fptr(b), len(b) // 3, _dptr(vol_ref)))
+
+    def phantom_fibres_scalar(self, params, n_vox, vox_size, value, vol_ref):
+        p = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        self.check(_lib.xr_phantom_fibres_scalar(self.h, p.ctypes.data_as(_c_double_p), len(p) // 8, n_vox,
+                                                 vox_size, value, _dptr(vol_ref)))
 
     def noise_poisson(self, y, p, I0, seed):
         self.check(_lib.xr_noise_poisson(self.h, _dptr(y), _dptr(p), p.numel(), I0, seed))

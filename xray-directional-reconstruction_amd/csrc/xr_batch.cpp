@@ -241,6 +241,7 @@ int build_tables(xr_batch* b) {
             a.n_slots = p.ss;
             a.axis = pl->axis_cond[0] ? 1 : 0;
             const int rows = forward_rows(pl);
+            a.n_chunks = 1;
             for (int v0 = 0; v0 < p.V; v0 += rows) {
                 a.v0 = v0;
                 a.vb = std::min(rows, p.V - v0);

@@ -89,7 +89,8 @@ struct FwdArgs {
     int out_mode;
     int n_bands;
     int n_slots;
-    int v0, vb;              // detector rows [v0, v0+vb) owned by this launch
+    int v0, vb;              // detector rows [v0, v0+vb) of the first row chunk (chunk height vb)
+    int n_chunks;            // row chunks in one launch: chunk c owns rows v0 + c*vb + [0, vb) (clipped to V)
     int axis;                // batched launches: 1 -> every view is axis aligned (ViewGeom::axis)
 };
 
@@ -158,6 +159,9 @@ hipError_t launch_cylinder_shell(float* vol, int N, float vs, double r_out, doub
 // Fibre phantom: K channels of mu_k = s |S_k x f|^2 inside each fibre (values add).
 hipError_t launch_fibres(float* vol, int N, float vs, const double* params_dev, int count, const Basis& b, int K,
                          hipStream_t stream);
+// Fibre phantom, scalar: `value` added inside each fibre (values add).
+hipError_t launch_fibres_scalar(float* vol, int N, float vs, const double* params_dev, int count, float value,
+                                hipStream_t stream);
 // y = -ln(max(c, 1)/I0), c ~ Poisson(I0 exp(-p)) (Philox4x32-10 stream of (seed, index)).
 hipError_t launch_poisson_noise(float* y, const float* p, int64_t n, double I0, uint64_t seed, hipStream_t stream);
 // y = x (1 + sigma N(0,1)) (Box-Muller on Philox4x32-10 of (seed, index)).

@@ -167,7 +167,7 @@ struct FwdCol {
 };
 
 template <bool AXIS, int KIND>
-__device__ __forceinline__ FwdCol fwd_col(const FwdArgs& a, const ViewGeom& g, int xcol, int y) {
+__device__ __forceinline__ FwdCol fwd_col(const FwdArgs& a, const ViewGeom& g, int xcol, int y, int v0, int vb) {
     const Dims& d = a.d;
     FwdCol f;
     const float xp = (float)xcol - d.xc, yp = (float)y - d.xc;
@@ -185,8 +185,8 @@ __device__ __forceinline__ FwdCol fwd_col(const FwdArgs& a, const ViewGeom& g, i
         // by two voxels and every voxel is still tested exactly in fwd_splat.
         const float slope = g.av[2] * f.inv;
         if (fabsf(slope) > 1e-6f) {
-            const float za = ((float)a.v0 - 0.5f - d.hv - f.c.nv * f.inv) / slope;
-            const float zb = ((float)(a.v0 + a.vb) + 0.5f - d.hv - f.c.nv * f.inv) / slope;
+            const float za = ((float)v0 - 0.5f - d.hv - f.c.nv * f.inv) / slope;
+            const float zb = ((float)(v0 + vb) + 0.5f - d.hv - f.c.nv * f.inv) / slope;
             const float lo = fminf(za, zb) + d.xc - 2.0f, hi = fmaxf(za, zb) + d.xc + 2.0f;
             f.z_beg = (int)fmaxf(0.0f, floorf(lo));
             f.z_end = (int)fminf((float)d.N, ceilf(hi) + 1.0f);
@@ -203,7 +203,7 @@ __device__ __forceinline__ FwdCol fwd_col(const FwdArgs& a, const ViewGeom& g, i
 // scale 2^S, 64-bit integer LDS atomics.
 template <bool AXIS>
 __device__ __forceinline__ void fwd_splat(const FwdArgs& a, const ViewGeom& g, const FwdCol& f, int z, float val,
-                                          float scale, lds_u64* acc, int u0) {
+                                          float scale, lds_u64* acc, int u0, int v0, int vb) {
     const Dims& d = a.d;
     const float zp = (float)z - d.xc;
     float u, v;
@@ -217,13 +217,13 @@ __device__ __forceinline__ void fwd_splat(const FwdArgs& a, const ViewGeom& g, c
     float fu, fv;
     split(u, iu, fu);
     split(v, iv, fv);
-    bool ok = z < f.z_end && accept_u(u, d) && accept_v(v, d) && iv >= a.v0 - 1 && iv < a.v0 + a.vb;
+    bool ok = z < f.z_end && accept_u(u, d) && accept_v(v, d) && iv >= v0 - 1 && iv < v0 + vb;
     if (!AXIS) ok = ok && iu >= u0 - 1 && iu < u0 + kBandU;
     if (ok) {
         const float vs = val * scale;
         const float c1 = (1.0f - fu) * fv, c2 = fu * fv;
         const float c3 = fu * (1.0f - fv), c4 = (1.0f - fu) * (1.0f - fv);
-        lds_u64* p = acc + (iv - a.v0 + 1) * kBandStride + (iu - u0 + 1);
+        lds_u64* p = acc + (iv - v0 + 1) * kBandStride + (iu - u0 + 1);
         const int q4 = __float2int_rn(c4 * vs), q3 = __float2int_rn(c3 * vs);
         const int q1 = __float2int_rn(c1 * vs), q2 = __float2int_rn(c2 * vs);
         __hip_atomic_fetch_add(p, (unsigned long long)(long long)q4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -262,7 +262,7 @@ __device__ __forceinline__ float xtt_value(const FwdArgs& a, const FwdCol& f, co
 // loss receives the item's share of sum |y - p|.
 template <int KIND, bool AXIS>
 __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, double* loss,
-                                             unsigned long long* acc_raw) {
+                                             unsigned long long* acc_raw, int v0, int vb) {
     lds_u64* const acc = (lds_u64*)acc_raw;
     const Dims& d = a.d;
     const int band = o / a.n_slots, slot = o % a.n_slots;
@@ -273,7 +273,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const int cond = code / d.P;
     const ViewGeom g = to_global(a.vg)[code];
     const int u0 = band * kBandU;
-    const int rows = a.vb + 2;
+    const int rows = vb + 2;
     for (int i = threadIdx.x; i < rows * kBandStride; i += 64 * kFwdWaves) acc[i] = 0ull;
     // Fixed-point scale: the largest splat term is < 2^30 * 2^-S.
     int e = 0;
@@ -327,7 +327,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                 while (m) {
                     const int bit = __ffsll(m) - 1;
                     m &= m - 1;
-                    const FwdCol f = fwd_col<AXIS, KIND>(a, g, x0 + bit, y);
+                    const FwdCol f = fwd_col<AXIS, KIND>(a, g, x0 + bit, y, v0, vb);
                     for (int z0 = f.z_beg; z0 < f.z_end; z0 += 64) {
                         const int z = z0 + lane;
                         float mu[kMaxBasis];
@@ -335,7 +335,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                         for (int k = 0; k < kMaxBasis; k++)
                             mu[k] = (k < d.K && z < f.z_end) ? vol[f.base + z + k * a.chan_stride] : 0.f;
                         const float val = xtt_value(a, f, g, z, g2, mu);
-                        fwd_splat<AXIS>(a, g, f, z, val, scale, acc, u0);
+                        fwd_splat<AXIS>(a, g, f, z, val, scale, acc, u0, v0, vb);
                     }
                 }
             } else {
@@ -345,8 +345,8 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                     const bool two = m != 0;
                     const int b1 = two ? __ffsll(m) - 1 : b0;
                     if (two) m &= m - 1;
-                    const FwdCol f0 = fwd_col<AXIS, KIND>(a, g, x0 + b0, y);
-                    const FwdCol f1 = fwd_col<AXIS, KIND>(a, g, x0 + b1, y);
+                    const FwdCol f0 = fwd_col<AXIS, KIND>(a, g, x0 + b0, y, v0, vb);
+                    const FwdCol f1 = fwd_col<AXIS, KIND>(a, g, x0 + b1, y, v0, vb);
                     const int zb = min(f0.z_beg, f1.z_beg), ze = max(f0.z_end, f1.z_end);
                     for (int zg = zb; zg < ze; zg += 64 * kFwdChunks) {
                         float m0[kFwdChunks], m1[kFwdChunks];
@@ -365,14 +365,14 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                         for (int c = 0; c < kFwdChunks; c++) {
                             if (zg + c * 64 >= ze) break;
                             const int z = zg + c * 64 + lane;
-                            if (z >= f0.z_beg) fwd_splat<AXIS>(a, g, f0, z, m0[c] * scalev, scale, acc, u0);
+                            if (z >= f0.z_beg) fwd_splat<AXIS>(a, g, f0, z, m0[c] * scalev, scale, acc, u0, v0, vb);
                         }
                         if (two) {
 #pragma unroll
                             for (int c = 0; c < kFwdChunks; c++) {
                                 if (zg + c * 64 >= ze) break;
                                 const int z = zg + c * 64 + lane;
-                                if (z >= f1.z_beg) fwd_splat<AXIS>(a, g, f1, z, m1[c] * scalev, scale, acc, u0);
+                                if (z >= f1.z_beg) fwd_splat<AXIS>(a, g, f1, z, m1[c] * scalev, scale, acc, u0, v0, vb);
                             }
                         }
                     }
@@ -386,9 +386,9 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const int64_t img = (int64_t)(code - a.code_base) * d.V * d.U;
     const int64_t nimg = (int64_t)code * d.V * d.U;
     const bool acc_old = (KIND != FWD_WEIGHT) && accum && accum[slot];
-    for (int i = threadIdx.x; i < a.vb * kBandU; i += 64 * kFwdWaves) {
+    for (int i = threadIdx.x; i < vb * kBandU; i += 64 * kFwdWaves) {
         const int r = i / kBandU, cc = i % kBandU;
-        const int uu = u0 + cc, vv = a.v0 + r;
+        const int uu = u0 + cc, vv = v0 + r;
         if (uu >= d.U || vv >= d.V) continue;
         const float s = (float)(long long)acc[(r + 1) * kBandStride + cc + 1] * unscale;
         const int64_t pix = img + (int64_t)vv * d.U + uu;
@@ -436,13 +436,18 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
 template <int KIND, bool AXIS>
 __global__ __launch_bounds__(64 * kFwdWaves) void k_forward(const FwdArgs a) {
     extern __shared__ unsigned long long acc_raw[];
-    const int nblk = a.n_bands * a.n_slots;
+    const int per_chunk = a.n_bands * a.n_slots;
+    const int nblk = per_chunk * a.n_chunks;
     const int per_xcd = (nblk + 7) / 8;
-    // XCD-chunked order: each XCD walks a contiguous run of (band, view) pairs,
-    // band-major, so concurrent blocks on one L2 share a band at adjacent views.
+    // XCD-chunked order: each XCD walks a contiguous run of (row chunk, band,
+    // view) triples, band-major within a chunk, so concurrent blocks on one L2
+    // share a band at adjacent views.
     const int o = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     if (o >= nblk) return;
-    forward_item<KIND, AXIS>(a, o, 0, a.loss, acc_raw);
+    const int chunk = o / per_chunk;
+    const int v0 = a.v0 + chunk * a.vb;
+    const int vb = min(a.vb, a.d.V - v0);
+    forward_item<KIND, AXIS>(a, o - chunk * per_chunk, 0, a.loss, acc_raw, v0, vb);
 }
 
 // Work queue of a persistent launch: q[0] hands out tickets 0, 1, 2, ...; a
@@ -501,9 +506,9 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_forward_batch(const FwdArgs*
         const int o = t - (i ? item_end[i - 1] : 0);
         double* loss = a.loss ? a.loss + loss_shift : nullptr;
         if (a.axis)
-            forward_item<KIND, true>(a, o, step, loss, acc_raw);
+            forward_item<KIND, true>(a, o, step, loss, acc_raw, a.v0, a.vb);
         else
-            forward_item<KIND, false>(a, o, step, loss, acc_raw);
+            forward_item<KIND, false>(a, o, step, loss, acc_raw, a.v0, a.vb);
     }
 }
 
@@ -1137,6 +1142,26 @@ __global__ void k_fibres(float* vol, int N, double vs, const double* __restrict_
     for (int k = 0; k < K; k++) vol[i + k * N3] += acc[k];
 }
 
+// Scalar fibres: `value` added inside every fibre (attenuation phantom).
+__global__ void k_fibres_scalar(float* vol, int N, double vs, const double* __restrict__ prm, int count,
+                                float value) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= N) return;
+    const double p0 = ((double)x - (N + 1) * 0.5) * vs, p1 = ((double)y - (N + 1) * 0.5) * vs,
+                 p2 = ((double)z - (N + 1) * 0.5) * vs;
+    float acc = 0.f;
+    for (int e = 0; e < count; e++) {
+        const double* q = prm + 8 * e;
+        const double d0 = p0 - q[0], d1 = p1 - q[1], d2 = p2 - q[2];
+        const double t = d0 * q[3] + d1 * q[4] + d2 * q[5];
+        if (fabs(t) > q[7]) continue;
+        const double r0 = d0 - t * q[3], r1 = d1 - t * q[4], r2 = d2 - t * q[5];
+        if (r0 * r0 + r1 * r1 + r2 * r2 > q[6] * q[6]) continue;
+        acc += value;
+    }
+    vol[x + (int64_t)N * y + (int64_t)N * N * z] += acc;
+}
+
 __global__ void k_poisson_noise(float* y, const float* p, int64_t n, double I0, uint64_t seed) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1159,7 +1184,7 @@ inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs);
 }  // namespace
 
 hipError_t launch_forward(const FwdArgs& a, int kind, bool axis, hipStream_t stream) {
-    const int nblk = a.n_bands * a.n_slots;
+    const int nblk = a.n_bands * a.n_slots * a.n_chunks;
     if (nblk == 0) return hipSuccess;
     const unsigned grid = (unsigned)(8 * ((nblk + 7) / 8));
     const size_t lds = ((size_t)(a.vb + 2) * kBandStride + kFwdWaves) * sizeof(unsigned long long);
@@ -1394,6 +1419,13 @@ hipError_t launch_fibres(float* vol, int N, float vs, const double* params_dev, 
                          hipStream_t stream) {
     hipLaunchKernelGGL(k_fibres, dim3(grid1(N, 64), N, N), dim3(64), 0, stream, vol, N, (double)vs, params_dev, count,
                        b, K);
+    return hipGetLastError();
+}
+
+hipError_t launch_fibres_scalar(float* vol, int N, float vs, const double* params_dev, int count, float value,
+                                hipStream_t stream) {
+    hipLaunchKernelGGL(k_fibres_scalar, dim3(grid1(N, 64), N, N), dim3(64), 0, stream, vol, N, (double)vs,
+                       params_dev, count, value);
     return hipGetLastError();
 }
 

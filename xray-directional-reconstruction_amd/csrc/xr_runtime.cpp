@@ -197,16 +197,14 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
     // All conditions in one launch share the axis specialisation only if all are axis-aligned.
     bool axis = true;
     for (int c = 0; c < p->n_cond; c++) axis = axis && p->axis_cond[c];
-    // Detector rows in equal chunks, one launch each (untilted scans walk only
-    // the z range that reaches a chunk's rows).
+    // Detector rows in equal chunks, all in one launch (untilted scans walk
+    // only the z range that reaches a chunk's rows).
     const int V = p->g.n_det_v;
     const int rows = forward_rows(p);
-    for (int v0 = 0; v0 < V; v0 += rows) {
-        a.v0 = v0;
-        a.vb = std::min(rows, V - v0);
-        XR_TRY(timed(ctx, 0, [&] { return launch_forward(a, kind, axis, ctx->stream); }));
-    }
-    return XR_OK;
+    a.v0 = 0;
+    a.vb = rows;
+    a.n_chunks = (V + rows - 1) / rows;
+    return timed(ctx, 0, [&] { return launch_forward(a, kind, axis, ctx->stream); });
 }
 
 int forward_rows(const xr_plan* p) {
@@ -1253,6 +1251,21 @@ int xr_phantom_fibres(xr_context* ctx, const double* params_host, int count, int
     if (count > 0)
         XR_HIP(ctx, hipMemcpyAsync(dp.p, params_host, sizeof(double) * 8 * count, hipMemcpyHostToDevice, ctx->stream));
     XR_TRY(timed(ctx, 3, [&] { return launch_fibres(vol_ref_dev, n_vox, vox_size, dp.p, count, b, n_basis, ctx->stream); }));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XR_OK;
+}
+
+int xr_phantom_fibres_scalar(xr_context* ctx, const double* params_host, int count, int n_vox, float vox_size,
+                             float value, float* vol_ref_dev) {
+    if (!ctx || (!params_host && count > 0) || count < 0 || n_vox <= 0 || !vol_ref_dev)
+        return fail(ctx, XR_ERR_INVALID, "bad args");
+    DevBuf<double> dp;
+    XR_HIP(ctx, dp.alloc((size_t)8 * std::max(count, 1)));
+    if (count > 0)
+        XR_HIP(ctx, hipMemcpyAsync(dp.p, params_host, sizeof(double) * 8 * count, hipMemcpyHostToDevice, ctx->stream));
+    XR_TRY(timed(ctx, 3, [&] {
+        return launch_fibres_scalar(vol_ref_dev, n_vox, vox_size, dp.p, count, value, ctx->stream);
+    }));
     XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return XR_OK;
 }

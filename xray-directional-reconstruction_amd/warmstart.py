@@ -1,0 +1,216 @@
+"""Learned warm start for plain MLEM (no counterpart in the reference).
+
+A small 3-D U-Net (``UNet``, BF16 MFMA kernels in libxr.so) maps an early
+MLEM volume (the plain method after a few iterations) to an estimate of a
+late one.  It is trained only on pairs the plain method itself generates
+(``generate_pairs``), and at run time it never replaces the method: MLEM
+resumes from the network's output, and the projection-domain data residual
+judges it — if the network's volume fits the data worse than the volume it
+was given, the driver falls back to that volume (``warm_start``).
+
+Data residual: r(x) = sum|y - A x| / sum|y| (relative L1 of the normalised
+forward projection A, accumulated in FP64 by the forward kernel), the
+reference's own MLEM loss (reconstruct.cu:131) divided by the data's L1 norm.
+
+Everything here orchestrates device work through the C ABI; volumes stay in
+HBM in the column layout.
+"""
+import time
+
+import numpy as np
+
+from . import BATCH_GRAPH, METHOD_MLEM, OUT_RATIO, Scan, UNet, XRError, make_geometry  # noqa: F401
+
+__all__ = ["Scene", "make_phantom", "generate_pairs", "train", "warm_start", "plain_mlem", "positivity_floor"]
+
+
+class Scene:
+    """One scan geometry (single untilted condition) with a device sinogram.
+
+    Wraps a one-problem batch (``Context.batch``) so MLEM runs on device
+    buffers with the epoch loop captured as a HIP graph."""
+
+    def __init__(self, ctx, geom, sino_dev):
+        import torch
+
+        self.ctx, self.geom = ctx, geom
+        self.batch = ctx.batch([geom], flags=BATCH_GRAPH)
+        self.vol_addr, self.sino_addr = self.batch.buffers(0)
+        self.plan = self.batch.plan(0)
+        self.N, self.P, self.U, self.V = geom.n_vox, geom.n_proj, geom.n_det_u, geom.n_det_v
+        ctx.copy(self.sino_addr, sino_dev)
+        self.y_l1 = float(sino_dev.abs().double().sum())
+        dev = sino_dev.device
+        self.proj = torch.empty(self.P * self.U * self.V, dtype=torch.float32, device=dev)
+        self.codes = self.plan.codes(range(self.P))
+        self.sino = sino_dev
+
+    def close(self):
+        self.batch.close()
+
+    def set_volume(self, vol_cols):
+        self.ctx.copy(self.vol_addr, vol_cols)
+
+    def get_volume(self, out):
+        self.ctx.copy(out, self.vol_addr, out.numel() * 4)
+
+    def residual(self, vol_cols):
+        """r(x) for a column-layout device volume (one forward projection)."""
+        loss = self.plan.project(vol_cols, self.proj, self.codes, out_mode=OUT_RATIO, sino=self.sino)
+        return loss / self.y_l1
+
+    def iterate(self, epochs):
+        """MLEM epochs on the scene's volume; returns (residuals before each
+        epoch, GPU milliseconds)."""
+        losses = self.batch.run(epochs)[:, 0]
+        return losses / self.y_l1, self.batch.elapsed_ms()
+
+
+def make_phantom(ctx, N, vox_size, seed, n_ellipsoids=12, n_fibres=24, fibre_value=0.02, vol_cols=None):
+    """Seeded ellipsoids (attenuation uniform in [0.005, 0.05]/mm) plus fibre
+    bundles (3 von Mises-Fisher axes, kappa 20, radius 2-6 voxels) as a
+    column-layout device volume."""
+    import torch
+
+    from . import ellipsoid_params, fibre_params
+
+    dev = torch.device("cuda")
+    fov = 0.5 * N * vox_size
+    ref = torch.zeros(N ** 3, dtype=torch.float32, device=dev)
+    ep = ellipsoid_params(seed, n_ellipsoids, fov_radius=fov, centre_frac=0.5, axis_lo=0.08, axis_hi=0.3,
+                          value_lo=0.005, value_hi=0.05)
+    ctx.phantom_rasterise(ep, N, vox_size, ref)
+    if n_fibres:
+        fp, _ = fibre_params(seed + 7919, n_fibres, n_axes=3, kappa=20.0, fov_radius=fov, centre_frac=0.5,
+                             r_lo=2.0 * vox_size, r_hi=6.0 * vox_size, half_length=0.6 * fov)
+        ctx.phantom_fibres_scalar(fp, N, vox_size, fibre_value, ref)
+    out = torch.empty_like(ref) if vol_cols is None else vol_cols
+    ctx.to_columns(ref, out, N)
+    return out
+
+
+def project(ctx, geom, vol_cols):
+    """Normalised forward projection of every view (the sinogram model)."""
+    import torch
+
+    plan = ctx.plan(geom, Scan())
+    proj = torch.zeros(geom.n_proj * geom.n_det_u * geom.n_det_v, dtype=torch.float32, device=vol_cols.device)
+    plan.project(vol_cols, proj, plan.codes(range(geom.n_proj)))
+    ctx.synchronize()
+    plan.close()
+    return proj
+
+
+def generate_pairs(ctx, count, N=128, P=180, U=192, sdd=1000.0, sod=500.0, det=0.4, pre=5, post=200, seed=0,
+                   per_batch=48, init=0.01, log=None):
+    """(inputs, targets): the plain MLEM volume after `pre` and after `post`
+    iterations for `count` seeded phantoms, all from the batched solver
+    (`per_batch` problems per persistent launch).  Lists of column-layout
+    device tensors."""
+    import torch
+
+    vs = sod * det / sdd  # one voxel per magnified detector pixel (geometry.h:13)
+    geom = make_geometry(sdd, sod, det, N, U, P, vox_size=vs)
+    inputs, targets = [], []
+    for b0 in range(0, count, per_batch):
+        n = min(per_batch, count - b0)
+        batch = ctx.batch([geom] * n, flags=BATCH_GRAPH)
+        for i in range(n):
+            vol = make_phantom(ctx, N, vs, seed + b0 + i)
+            sino = project(ctx, geom, vol)
+            va, sa = batch.buffers(i)
+            ctx.copy(sa, sino)
+            ctx.copy(va, torch.full((N ** 3,), init, dtype=torch.float32, device=sino.device))
+        t0 = time.time()
+        batch.run(pre)
+        for i in range(n):
+            x = torch.empty(N ** 3, dtype=torch.float32, device="cuda")
+            ctx.copy(x, batch.buffers(i)[0], x.numel() * 4)
+            inputs.append(x)
+        batch.run(post - pre)
+        for i in range(n):
+            t = torch.empty(N ** 3, dtype=torch.float32, device="cuda")
+            ctx.copy(t, batch.buffers(i)[0], t.numel() * 4)
+            targets.append(t)
+        batch.close()
+        if log:
+            log("pairs %d-%d: %.1f s" % (b0, b0 + n, time.time() - t0))
+    return inputs, targets, geom
+
+
+def train(net, inputs, targets, dims, steps, lr=3e-4, seed=0, log=None, log_every=50):
+    """Adam on single-volume steps in a seeded random order; returns the loss history."""
+    rng = np.random.default_rng(seed)
+    hist = []
+    order = []
+    for k in range(steps):
+        if not order:
+            order = list(rng.permutation(len(inputs)))
+        i = order.pop()
+        hist.append(net.loss_grad(inputs[i], targets[i], dims))
+        net.adam_step(lr)
+        if log and (k % log_every == 0 or k == steps - 1):
+            log("step %d loss %.5f" % (k, hist[-1]))
+    return hist
+
+
+def positivity_floor(vol_cols, floor):
+    """MLEM is multiplicative: a voxel at zero never moves again.  The
+    network's output is clamped to at least `floor`."""
+    vol_cols.clamp_(min=floor)
+    return vol_cols
+
+
+def plain_mlem(scene, tau, max_iters, init=0.01):
+    """Plain MLEM from a constant volume (main.cu:40-42) until r <= tau."""
+    import torch
+
+    scene.set_volume(torch.full((scene.N ** 3,), init, dtype=torch.float32, device=scene.sino.device))
+    return _resume(scene, tau, max_iters, 0, 0.0, [])
+
+
+def _resume(scene, tau, max_iters, iters, ms, hist):
+    while iters < max_iters:
+        r, t = scene.iterate(1)
+        ms += t
+        hist.append(float(r[0]))
+        if r[0] <= tau:
+            return {"iterations": iters, "ms": ms, "residuals": hist, "reached": True}
+        iters += 1
+    return {"iterations": iters, "ms": ms, "residuals": hist, "reached": False}
+
+
+def warm_start(scene, net, tau, max_iters, pre=5, init=0.01, floor_frac=1e-3, use_net=True):
+    """Plain MLEM for `pre` iterations, then the network (judged by the data
+    residual, with fallback), then MLEM until r <= tau.  Returns iterations
+    (MLEM iterations only), GPU milliseconds (everything: MLEM, network,
+    residual checks), the residual history and whether the network's volume
+    was kept."""
+    import torch
+
+    dev = scene.sino.device
+    N3 = scene.N ** 3
+    scene.set_volume(torch.full((N3,), init, dtype=torch.float32, device=dev))
+    r, ms = scene.iterate(pre)
+    hist = [float(v) for v in r]
+    used = False
+    if use_net:
+        x = torch.empty(N3, dtype=torch.float32, device=dev)
+        scene.get_volume(x)
+        y = torch.empty_like(x)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        scene.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        e0.record()
+        net.forward(x, y, (scene.N, scene.N, scene.N))
+        positivity_floor(y, floor_frac * float(net.last_scale()))
+        r_x = scene.residual(x)
+        r_y = scene.residual(y)
+        e1.record()
+        torch.cuda.synchronize()
+        scene.ctx.set_stream(0)
+        ms += e0.elapsed_time(e1)
+        used = r_y < r_x
+        scene.set_volume(y if used else x)
+    out = _resume(scene, tau, max_iters, pre, ms, hist)
+    out["used_net"] = used
+    return out
