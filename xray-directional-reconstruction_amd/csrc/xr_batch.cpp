@@ -235,12 +235,13 @@ int build_tables(xr_batch* b) {
             a.loss = b->d_loss + k;
             a.vmax = b->d_vmax + slot;
             a.wmul = pl->dims.ratio * 1.0001f;  // as forward_launch
+            a.fsum = pl->max_factor * 1.01f + 1.0f;
             a.code_base = 0;
             a.out_mode = out_mode;
             a.n_bands = (p.U + kBandU - 1) / kBandU;
             a.n_slots = p.ss;
             a.axis = pl->axis_cond[0] ? 1 : 0;
-            const int rows = forward_rows(pl);
+            const int rows = forward_rows(pl, FWD_PLAIN);
             a.n_chunks = 1;
             for (int v0 = 0; v0 < p.V; v0 += rows) {
                 a.v0 = v0;
@@ -304,7 +305,7 @@ int build_tables(xr_batch* b) {
     // Persistent grids: every block resident at once.
     hipDeviceProp_t prop;
     XR_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
-    b->ticket_slot = (max_vb + 2) * kBandStride + 8;
+    b->ticket_slot = fwd_red_slot(max_vb + 2, 4) + 8;  // past the 32-bit band and the reduction slots
     b->fwd_lds = sizeof(unsigned long long) * ((size_t)b->ticket_slot + 1);
     const int fpc = batch_blocks_per_cu_forward(FWD_PLAIN, b->fwd_lds);
     if (fpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched forward kernel");
@@ -372,7 +373,7 @@ int xr_plan_get_tiles(const xr_plan* plan, xr_tiles* tiles) {
     if (!backward_shape(a, plan->tiles.bwd_zpl, plan->tiles.bwd_gx, false)) return XR_ERR_INVALID;
     tiles->bwd_zpl = a.zpl;
     tiles->bwd_gx = a.gx;
-    tiles->fwd_rows = forward_rows(plan);
+    tiles->fwd_rows = forward_rows(plan, FWD_PLAIN);
     return XR_OK;
 }
 
@@ -426,7 +427,7 @@ int xr_autotune(xr_context* ctx, xr_plan* plan, const float* vol_cols, const flo
         std::vector<int> seen;
         for (int rows : {0, 96, 64, 32}) {
             plan->tiles.fwd_rows = rows;
-            const int eff = forward_rows(plan);
+            const int eff = forward_rows(plan, FWD_PLAIN);
             if (std::find(seen.begin(), seen.end(), eff) != seen.end()) continue;
             seen.push_back(eff);
             double ms = 0;

@@ -48,6 +48,8 @@ struct xr_plan {
     xr::Basis bas{};
     float xtt_wsum = 0.f;  // sum_k |S_k|^2 max_view (S_k.G)^2: XTT splat value per unit |mu|
     xr_tiles tiles{};      // projector tile shape (zeros: defaults)
+    float max_factor = 0.f;     // largest per-pixel splat weight sum (set with d_norm)
+    float* d_fmax = nullptr;    // device scalar receiving it during the weight pass
 };
 
 namespace xr {
@@ -115,8 +117,9 @@ int backward_launch(xr_plan* p, int kind, const float* proj, int code_base, cons
                     float* vol, float* tmp, float* factor, float alpha, int overwrite);
 // Per-pixel L/(factor + 1e-5) table of the plan (computed once).
 int ensure_norm(xr_plan* p);
-// Detector rows per forward launch for the plan's tile setting.
-int forward_rows(const xr_plan* p);
+// Detector rows per forward row chunk for the plan's tile setting and the
+// accumulator width of `kind` (FWD_WEIGHT: 64-bit words, else 32-bit).
+int forward_rows(const xr_plan* p, int kind);
 
 }  // namespace host
 }  // namespace xr

@@ -12,6 +12,11 @@ constexpr int kBandU = 16;
 // LDS row stride of a band accumulator: one guard column on each side plus
 // one pad column so the stride is odd (conflict-free rows on 32 banks).
 constexpr int kBandStride = kBandU + 3;
+// 8-byte slot index just past a forward band accumulator of `rows` rows of
+// `acc_bytes`-byte words (the per-wave reduction slots follow it).
+__host__ __device__ constexpr int fwd_red_slot(int rows, int acc_bytes) {
+    return (rows * kBandStride * acc_bytes + 7) / 8;
+}
 // Max basis vectors (scattering directions) per voxel for the XTT operator.
 constexpr int kMaxBasis = 16;
 
@@ -85,6 +90,8 @@ struct FwdArgs {
     double* loss;            // accumulated sum |y - p| (RATIO / RESIDUAL)
     const float* vmax;       // device scalar: max |voxel value| of vol (1 for the weight pass)
     float wmul;              // bound of a voxel's splat value per unit |voxel value|
+    float fsum;              // bound of a pixel's weight sum (1 for the weight pass)
+    float* fmax;             // OUT_NORM_TABLE: receives max pixel weight sum (zeroed beforehand), or null
     int code_base;           // proj/sino image of a view = code - code_base (norm uses code)
     int out_mode;
     int n_bands;

@@ -158,6 +158,22 @@ constexpr int kFwdChunks = 8;  // z chunks of 64 loaded per column per step (512
 
 typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 
+// Band accumulator word: 64-bit for the weight pass (its own sum bound is what
+// it computes), 32-bit otherwise, with the scale chosen from the largest
+// per-pixel weight sum so that no pixel sum can exceed 2^30 in magnitude.
+template <int KIND>
+struct FwdAcc {
+    typedef unsigned int T;
+    typedef int S;
+};
+template <>
+struct FwdAcc<FWD_WEIGHT> {
+    typedef unsigned long long T;
+    typedef long long S;
+};
+template <class T>
+using lds_ptr = __attribute__((address_space(3))) T*;
+
 struct FwdCol {
     ColForms c;
     float inv, u;
@@ -201,9 +217,9 @@ __device__ __forceinline__ FwdCol fwd_col(const FwdArgs& a, const ViewGeom& g, i
 
 // Splat one voxel (z of column f, value val) into the band: fixed point with
 // scale 2^S, 64-bit integer LDS atomics.
-template <bool AXIS>
+template <bool AXIS, class AccT, class AccS>
 __device__ __forceinline__ void fwd_splat(const FwdArgs& a, const ViewGeom& g, const FwdCol& f, int z, float val,
-                                          float scale, lds_u64* acc, int u0, int v0, int vb) {
+                                          float scale, lds_ptr<AccT> acc, int u0, int v0, int vb) {
     const Dims& d = a.d;
     const float zp = (float)z - d.xc;
     float u, v;
@@ -223,16 +239,36 @@ __device__ __forceinline__ void fwd_splat(const FwdArgs& a, const ViewGeom& g, c
         const float vs = val * scale;
         const float c1 = (1.0f - fu) * fv, c2 = fu * fv;
         const float c3 = fu * (1.0f - fv), c4 = (1.0f - fu) * (1.0f - fv);
-        lds_u64* p = acc + (iv - v0 + 1) * kBandStride + (iu - u0 + 1);
+        lds_ptr<AccT> p = acc + (iv - v0 + 1) * kBandStride + (iu - u0 + 1);
         const int q4 = __float2int_rn(c4 * vs), q3 = __float2int_rn(c3 * vs);
         const int q1 = __float2int_rn(c1 * vs), q2 = __float2int_rn(c2 * vs);
-        __hip_atomic_fetch_add(p, (unsigned long long)(long long)q4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(p + 1, (unsigned long long)(long long)q3, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(p + kBandStride, (unsigned long long)(long long)q1, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(p + kBandStride + 1, (unsigned long long)(long long)q2, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(p, (AccT)(AccS)q4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(p + 1, (AccT)(AccS)q3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(p + kBandStride, (AccT)(AccS)q1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(p + kBandStride + 1, (AccT)(AccS)q2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+// Untilted splat: u, hence (iu, fu), is constant along the column, so the
+// four weights factor into (1-fu, fu) x (1-fv, fv); wu0 = (1-fu)*s and
+// wu1 = fu*s carry the fixed-point scale.  colp = band column iu's row 0.
+template <class AccT, class AccS>
+__device__ __forceinline__ void fwd_splat_axis(const Dims& d, const ViewGeom& g, const FwdCol& f, int z, float val,
+                                               lds_ptr<AccT> colp, float wu0, float wu1, int v0, int vb) {
+    const float zp = (float)z - d.xc;
+    const float v = fmaf(fmaf(g.av[2], zp, f.c.nv), f.inv, d.hv);
+    int iv;
+    float fv;
+    split(v, iv, fv);
+    if (z >= f.z_beg && z < f.z_end && accept_v(v, d) && iv >= v0 - 1 && iv < v0 + vb) {
+        const float g1 = fv * val, g0 = (1.0f - fv) * val;
+        lds_ptr<AccT> p = colp + (iv - v0 + 1) * kBandStride;
+        const int q4 = __float2int_rn(wu0 * g0), q3 = __float2int_rn(wu1 * g0);
+        const int q1 = __float2int_rn(wu0 * g1), q2 = __float2int_rn(wu1 * g1);
+        __hip_atomic_fetch_add(p, (AccT)(AccS)q4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(p + 1, (AccT)(AccS)q3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(p + kBandStride, (AccT)(AccS)q1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(p + kBandStride + 1, (AccT)(AccS)q2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 }
 
@@ -263,7 +299,9 @@ __device__ __forceinline__ float xtt_value(const FwdArgs& a, const FwdCol& f, co
 template <int KIND, bool AXIS>
 __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, double* loss,
                                              unsigned long long* acc_raw, int v0, int vb) {
-    lds_u64* const acc = (lds_u64*)acc_raw;
+    typedef typename FwdAcc<KIND>::T AccT;
+    typedef typename FwdAcc<KIND>::S AccS;
+    const lds_ptr<AccT> acc = (lds_ptr<AccT>)acc_raw;
     const Dims& d = a.d;
     const int band = o / a.n_slots, slot = o % a.n_slots;
     const auto views = to_global(a.views + (int64_t)step * a.n_slots);
@@ -275,9 +313,10 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const int u0 = band * kBandU;
     const int rows = vb + 2;
     for (int i = threadIdx.x; i < rows * kBandStride; i += 64 * kFwdWaves) acc[i] = 0ull;
-    // Fixed-point scale: the largest splat term is < 2^30 * 2^-S.
+    // Fixed-point scale: the largest pixel sum (weight pass: the largest
+    // term) is < 2^30 * 2^-S.
     int e = 0;
-    (void)frexpf(fmaxf(a.wmul * (*to_global(a.vmax)), 1e-30f), &e);
+    (void)frexpf(fmaxf(a.wmul * (*to_global(a.vmax)) * a.fsum, 1e-30f), &e);
     const int S = min(max(30 - e, -100), 100);
     const float scale = ldexpf(1.0f, S), unscale = ldexpf(1.0f, -S);
     __syncthreads();
@@ -335,7 +374,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                         for (int k = 0; k < kMaxBasis; k++)
                             mu[k] = (k < d.K && z < f.z_end) ? vol[f.base + z + k * a.chan_stride] : 0.f;
                         const float val = xtt_value(a, f, g, z, g2, mu);
-                        fwd_splat<AXIS>(a, g, f, z, val, scale, acc, u0, v0, vb);
+                        fwd_splat<AXIS, AccT, AccS>(a, g, f, z, val, scale, acc, u0, v0, vb);
                     }
                 }
             } else {
@@ -348,6 +387,24 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                     const FwdCol f0 = fwd_col<AXIS, KIND>(a, g, x0 + b0, y, v0, vb);
                     const FwdCol f1 = fwd_col<AXIS, KIND>(a, g, x0 + b1, y, v0, vb);
                     const int zb = min(f0.z_beg, f1.z_beg), ze = max(f0.z_end, f1.z_end);
+                    // untilted: per-column u terms (uniform across the wave)
+                    lds_ptr<AccT> cp0 = acc, cp1 = acc;
+                    float w00 = 0.f, w01 = 0.f, w10 = 0.f, w11 = 0.f;
+                    bool ok0 = false, ok1 = false;
+                    if (AXIS) {
+                        int iu;
+                        float fu;
+                        split(f0.u, iu, fu);
+                        ok0 = accept_u(f0.u, d) && iu >= u0 - 1 && iu < u0 + kBandU;
+                        cp0 = acc + (ok0 ? iu - u0 + 1 : 0);
+                        w00 = (1.0f - fu) * (scale * scalev);
+                        w01 = fu * (scale * scalev);
+                        split(f1.u, iu, fu);
+                        ok1 = two && accept_u(f1.u, d) && iu >= u0 - 1 && iu < u0 + kBandU;
+                        cp1 = acc + (ok1 ? iu - u0 + 1 : 0);
+                        w10 = (1.0f - fu) * (scale * scalev);
+                        w11 = fu * (scale * scalev);
+                    }
                     for (int zg = zb; zg < ze; zg += 64 * kFwdChunks) {
                         float m0[kFwdChunks], m1[kFwdChunks];
 #pragma unroll
@@ -361,18 +418,28 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                                 m1[c] = (two && z < f1.z_end && z >= f1.z_beg) ? vol[f1.base + z] : 0.f;
                             }
                         }
+                        if (AXIS) {
+#pragma unroll
+                            for (int c = 0; c < kFwdChunks; c++) {
+                                if (zg + c * 64 >= ze) break;
+                                const int z = zg + c * 64 + lane;
+                                if (ok0) fwd_splat_axis<AccT, AccS>(d, g, f0, z, m0[c], cp0, w00, w01, v0, vb);
+                                if (ok1) fwd_splat_axis<AccT, AccS>(d, g, f1, z, m1[c], cp1, w10, w11, v0, vb);
+                            }
+                            continue;
+                        }
 #pragma unroll
                         for (int c = 0; c < kFwdChunks; c++) {
                             if (zg + c * 64 >= ze) break;
                             const int z = zg + c * 64 + lane;
-                            if (z >= f0.z_beg) fwd_splat<AXIS>(a, g, f0, z, m0[c] * scalev, scale, acc, u0, v0, vb);
+                            if (z >= f0.z_beg) fwd_splat<AXIS, AccT, AccS>(a, g, f0, z, m0[c] * scalev, scale, acc, u0, v0, vb);
                         }
                         if (two) {
 #pragma unroll
                             for (int c = 0; c < kFwdChunks; c++) {
                                 if (zg + c * 64 >= ze) break;
                                 const int z = zg + c * 64 + lane;
-                                if (z >= f1.z_beg) fwd_splat<AXIS>(a, g, f1, z, m1[c] * scalev, scale, acc, u0, v0, vb);
+                                if (z >= f1.z_beg) fwd_splat<AXIS, AccT, AccS>(a, g, f1, z, m1[c] * scalev, scale, acc, u0, v0, vb);
                             }
                         }
                     }
@@ -383,6 +450,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     __syncthreads();
 
     double lsum = 0.0;
+    float fmax_loc = 0.0f;  // weight pass: largest per-pixel weight sum
     const int64_t img = (int64_t)(code - a.code_base) * d.V * d.U;
     const int64_t nimg = (int64_t)code * d.V * d.U;
     const bool acc_old = (KIND != FWD_WEIGHT) && accum && accum[slot];
@@ -390,12 +458,13 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
         const int r = i / kBandU, cc = i % kBandU;
         const int uu = u0 + cc, vv = v0 + r;
         if (uu >= d.U || vv >= d.V) continue;
-        const float s = (float)(long long)acc[(r + 1) * kBandStride + cc + 1] * unscale;
+        const float s = (float)(AccS)acc[(r + 1) * kBandStride + cc + 1] * unscale;
         const int64_t pix = img + (int64_t)vv * d.U + uu;
         const int64_t npix = nimg + (int64_t)vv * d.U + uu;
         if (a.out_mode == OUT_NORM_TABLE) {
             const float L = (float)chord_length(to_global(a.cg)[code], uu, vv, d);
             proj[pix] = L / (s + 1e-5f);
+            fmax_loc = fmaxf(fmax_loc, s);
             continue;
         }
         float p = (acc_old ? proj[pix] : 0.f) + s;
@@ -419,9 +488,13 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
             }
         }
     }
+    if (a.out_mode == OUT_NORM_TABLE && a.fmax) {
+        for (int off = 32; off > 0; off >>= 1) fmax_loc = fmaxf(fmax_loc, __shfl_xor(fmax_loc, off));
+        if (lane == 0) atomicMax((unsigned int*)a.fmax, __float_as_uint(fmax_loc));
+    }
     if (a.out_mode == OUT_RATIO || a.out_mode == OUT_RESIDUAL) {
         // Cross-wave sum through LDS past the band accumulator (8-byte aligned slot).
-        double* red = reinterpret_cast<double*>(acc_raw + rows * kBandStride);
+        double* red = reinterpret_cast<double*>(acc_raw + fwd_red_slot(rows, sizeof(AccT)));
         lsum = wave_sum(lsum);
         if (lane == 0) red[wave] = lsum;
         __syncthreads();
@@ -533,7 +606,7 @@ struct IsXtt {
 };
 
 struct Foot {
-    int iu_lo, iv_lo, W, H, Ws;
+    int iu_lo, iv_lo, W, H, Ws, R;  // R: footprint rows staged per pass of the block's threads
     bool staged, empty;
 };
 
@@ -558,7 +631,7 @@ __device__ __forceinline__ Foot footprint(const ViewGeom& g, const Dims& d, floa
         vmx = fmaxf(vmx, __shfl_xor(vmx, off));
         bad |= __shfl_xor(bad, off);
     }
-    Foot f{0, 0, 0, 0, 0, false, false};
+    Foot f{0, 0, 0, 0, 0, 0, false, false};
     if (bad) return f;  // direct (unstaged) reads
     f.iu_lo = max((int)floorf(umn - 0.5f) - 1, 0);
     const int iu_hi = min((int)floorf(umx - 0.5f) + 2, d.U - 1);
@@ -571,9 +644,10 @@ __device__ __forceinline__ Foot footprint(const ViewGeom& g, const Dims& d, floa
         return f;
     }
     f.Ws = f.W | 1;
+    f.R = kBwdThreads / f.W;
     // the last word of the second buffer is reserved (work-queue ticket of the
     // persistent launch), so a staged footprint holds at most kStageBuf-1 floats
-    f.staged = (f.Ws * f.H <= kStageBuf - 1);
+    f.staged = (f.Ws * f.H <= kStageBuf - 1) && (f.H <= kStagePer * f.R);
     return f;
 }
 
@@ -613,24 +687,27 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
     float pre[kStagePer];
     int pidx[kStagePer];
     auto fetch = [&](const Foot& f, int code) {
+        // rows of the footprint in passes of R = kBwdThreads / W rows: thread t
+        // owns column t % W of rows t / W + k*R (one division per view)
         const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
-        const int n = f.W * f.H;
-        const float invW = 1.0f / (float)f.W;
+        int r0 = (int)((float)threadIdx.x * (1.0f / (float)f.W));
+        int c = (int)threadIdx.x - r0 * f.W;
+        if (c < 0) {
+            r0--;
+            c += f.W;
+        } else if (c >= f.W) {
+            r0++;
+            c -= f.W;
+        }
+        const bool act = r0 < f.R;
+        const auto src = img + (int64_t)(f.iv_lo + r0) * d.U + f.iu_lo + c;
+        const int64_t step = (int64_t)f.R * d.U;
 #pragma unroll
         for (int k = 0; k < kStagePer; k++) {
-            const int i = threadIdx.x + k * kBwdThreads;
-            int r = (int)((float)i * invW);
-            int c = i - r * f.W;
-            if (c < 0) {
-                r--;
-                c += f.W;
-            } else if (c >= f.W) {
-                r++;
-                c -= f.W;
-            }
-            const bool in = i < n;
+            const int r = r0 + k * f.R;
+            const bool in = act && r < f.H;
             pidx[k] = in ? r * f.Ws + c : -1;
-            pre[k] = in ? img[(int64_t)(f.iv_lo + r) * d.U + f.iu_lo + c] : 0.f;
+            pre[k] = in ? src[k * step] : 0.f;
         }
     };
     auto store = [&](lds_float* buf) {
@@ -640,6 +717,15 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
     };
 
     if (a.n_views == 0) return;
+    // view-independent per-lane z terms
+    float zpq[ZPL];
+    bool zin[ZPL];
+#pragma unroll
+    for (int q = 0; q < ZPL; q++) {
+        const int z = z0 + lane + 64 * q;
+        zpq[q] = (float)z - d.xc;
+        zin[q] = z < d.N;
+    }
     int code = views[0];
     ViewGeom g = vg[code];
     Foot f = footprint(g, d, bx0, bx1, by0, by1, bz0, bz1, lane);
@@ -656,7 +742,7 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
         // prefetch the next view's footprint
         int ncode = 0;
         ViewGeom ng;
-        Foot nf{0, 0, 0, 0, 0, false, false};
+        Foot nf{0, 0, 0, 0, 0, 0, false, false};
         if (s + 1 < a.n_views) {
             ncode = views[s + 1];
             ng = vg[ncode];
@@ -695,6 +781,51 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
                     if (XTT) {
                         dx = fmaf(d.vs, xp, g.d0[0]);
                         dy = fmaf(d.vs, yp, g.d0[1]);
+                    }
+                    if (AXIS && !XTT) {
+                        // Untilted: u, hence (iu, fu), is constant along the column and the
+                        // bilinear weights factor into (1-fu, fu) x (1-fv, fv).  Branch free:
+                        // a rejected voxel reads a clamped (valid) texel and adds zero.
+                        int iu;
+                        float fu;
+                        split(u_c, iu, fu);
+                        const bool uok = accept_u(u_c, d);
+                        const float a0 = 1.0f - fu;
+#pragma unroll
+                        for (int q = 0; q < ZPL; q++) {
+                            const float v = fmaf(fmaf(g.av[2], zpq[q], c.nv), inv_c, d.hv);
+                            const bool ok = zin[q] && uok && accept_v(v, d);
+                            int iv;
+                            float fv;
+                            split(v, iv, fv);
+                            float r00, r01, r10, r11;
+                            if (f.staged) {
+                                const int lu = min(max(iu - f.iu_lo, 0), f.W - 2);
+                                const int lv = min(max(iv - f.iv_lo, 0), f.H - 2);
+                                const int qi = lv * f.Ws + lu;
+                                r00 = cur[qi];
+                                r01 = cur[qi + 1];
+                                r10 = cur[qi + f.Ws];
+                                r11 = cur[qi + f.Ws + 1];
+                            } else {
+                                // direct reads (see the general path below)
+                                const int64_t qi = (int64_t)min(max(iv, 0), d.V - 2) * d.U + min(max(iu, 0), d.U - 2);
+                                r00 = __builtin_nontemporal_load(img + qi);
+                                r01 = __builtin_nontemporal_load(img + qi + 1);
+                                r10 = __builtin_nontemporal_load(img + qi + d.U);
+                                r11 = __builtin_nontemporal_load(img + qi + d.U + 1);
+                            }
+                            const float top = fmaf(fu, r01, a0 * r00);
+                            const float bot = fmaf(fu, r11, a0 * r10);
+                            const float nb = ok ? fmaf(fv, bot - top, top) : 0.0f;
+                            if (KIND == BWD_FDK) {
+                                tmp[cidx][q][0] = fmaf(wcol, nb, tmp[cidx][q][0]);
+                            } else {
+                                tmp[cidx][q][0] += nb;
+                                fac[cidx][q][0] += ok ? 1.0f : 0.0f;
+                            }
+                        }
+                        continue;
                     }
 #pragma unroll
                     for (int q = 0; q < ZPL; q++) {
@@ -1187,7 +1318,8 @@ hipError_t launch_forward(const FwdArgs& a, int kind, bool axis, hipStream_t str
     const int nblk = a.n_bands * a.n_slots * a.n_chunks;
     if (nblk == 0) return hipSuccess;
     const unsigned grid = (unsigned)(8 * ((nblk + 7) / 8));
-    const size_t lds = ((size_t)(a.vb + 2) * kBandStride + kFwdWaves) * sizeof(unsigned long long);
+    const size_t acc_bytes = (kind == FWD_WEIGHT) ? 8 : 4;
+    const size_t lds = ((size_t)fwd_red_slot(a.vb + 2, (int)acc_bytes) + kFwdWaves) * sizeof(unsigned long long);
 #define XR_FWD(KD, A) hipLaunchKernelGGL((k_forward<KD, A>), dim3(grid), dim3(64 * kFwdWaves), lds, stream, a)
     if (kind == FWD_WEIGHT) {
         if (axis) XR_FWD(FWD_WEIGHT, true); else XR_FWD(FWD_WEIGHT, false);

@@ -183,11 +183,16 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
     a.out_mode = out_mode;
     a.n_bands = (p->g.n_det_u + kBandU - 1) / kBandU;
     a.n_slots = n_slots;
-    // Fixed-point scale of the LDS accumulation: bound of a voxel's splat value.
+    // Fixed-point scale of the LDS accumulation: bound of a voxel's splat value
+    // (wmul * vmax) times the bound of a pixel's weight sum (fsum).
     if (kind == FWD_WEIGHT) {
         a.vmax = ctx->d_scal;  // 1.0
         a.wmul = 1.0f;
+        a.fsum = 1.0f;
+        a.fmax = (out_mode == OUT_NORM_TABLE) ? p->d_fmax : nullptr;
     } else {
+        XR_TRY(ensure_norm(p));
+        a.fsum = p->max_factor * 1.01f + 1.0f;
         const int64_t nvals = (kind == FWD_XTT) ? N3 * p->n_basis : (vol_cond_stride ? N3 * p->n_cond : N3);
         XR_HIP(ctx, hipMemsetAsync(ctx->d_scal + 1, 0, sizeof(float), ctx->stream));
         XR_TRY(timed(ctx, 3, [&] { return launch_absmax(vol, nvals, ctx->d_scal + 1, ctx->stream); }));
@@ -200,20 +205,21 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
     // Detector rows in equal chunks, all in one launch (untilted scans walk
     // only the z range that reaches a chunk's rows).
     const int V = p->g.n_det_v;
-    const int rows = forward_rows(p);
+    const int rows = forward_rows(p, kind);
     a.v0 = 0;
     a.vb = rows;
     a.n_chunks = (V + rows - 1) / rows;
     return timed(ctx, 0, [&] { return launch_forward(a, kind, axis, ctx->stream); });
 }
 
-int forward_rows(const xr_plan* p) {
-    // Detector rows per launch: the int64 band accumulator (kBandStride entries
-    // per row, two guard rows, a reduction slot) stays within 64 KB of LDS;
+int forward_rows(const xr_plan* p, int kind) {
+    // Detector rows per chunk: the band accumulator (kBandStride words per
+    // row, two guard rows, the reduction slots) stays within 64 KB of LDS;
     // taller detectors are split into equal row chunks.  tiles.fwd_rows caps
     // the chunk height further (smaller LDS footprint, more resident blocks).
     const int V = p->g.n_det_v;
-    int max_rows = (64 * 1024 / 8 - 16) / kBandStride - 2;
+    const int word = (kind == FWD_WEIGHT) ? 8 : 4;
+    int max_rows = (64 * 1024 - 8 * 8 - 16) / (kBandStride * word) - 2;
     if (p->tiles.fwd_rows > 0) max_rows = std::min(max_rows, (int)p->tiles.fwd_rows);
     const int n_chunks = (V + max_rows - 1) / max_rows;
     return (V + n_chunks - 1) / n_chunks;
@@ -255,10 +261,20 @@ int ensure_norm(xr_plan* p) {
     DevBuf<int32_t> dc;
     XR_HIP(ctx, dc.alloc(codes.size()));
     XR_HIP(ctx, hipMemcpyAsync(dc.p, codes.data(), codes.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    // Splat the bilinear weights alone (factor), then L/(factor + 1e-5) per pixel.
-    XR_TRY(forward_launch(p, FWD_WEIGHT, nullptr, 0, p->d_norm, nullptr, 0, dc.p, nullptr, (int)codes.size(),
-                          OUT_NORM_TABLE, nullptr));
+    // Splat the bilinear weights alone (factor), then L/(factor + 1e-5) per
+    // pixel; the largest factor bounds every pixel sum of later passes.
+    DevBuf<float> fmax;
+    XR_HIP(ctx, fmax.alloc(1));
+    XR_HIP(ctx, hipMemsetAsync(fmax.p, 0, sizeof(float), ctx->stream));
+    p->d_fmax = fmax.p;
+    const int rc = forward_launch(p, FWD_WEIGHT, nullptr, 0, p->d_norm, nullptr, 0, dc.p, nullptr, (int)codes.size(),
+                                  OUT_NORM_TABLE, nullptr);
+    p->d_fmax = nullptr;
+    XR_TRY(rc);
+    float mf = 0.f;
+    XR_HIP(ctx, hipMemcpyAsync(&mf, fmax.p, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    p->max_factor = mf;
     return XR_OK;
 }
 

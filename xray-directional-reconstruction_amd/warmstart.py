@@ -28,13 +28,14 @@ class Scene:
     """One scan geometry (single untilted condition) with a device sinogram.
 
     Wraps a one-problem batch (``Context.batch``) so MLEM runs on device
-    buffers with the epoch loop captured as a HIP graph."""
+    buffers with the epoch loop captured as a HIP graph.  `subsets` > 1 gives
+    ordered-subset MLEM (IR::reconstruct's batch, reconstruct.cu:61-65)."""
 
-    def __init__(self, ctx, geom, sino_dev):
+    def __init__(self, ctx, geom, sino_dev, subsets=1):
         import torch
 
         self.ctx, self.geom = ctx, geom
-        self.batch = ctx.batch([geom], flags=BATCH_GRAPH)
+        self.batch = ctx.batch([geom], batch=[subsets], flags=BATCH_GRAPH)
         self.vol_addr, self.sino_addr = self.batch.buffers(0)
         self.plan = self.batch.plan(0)
         self.N, self.P, self.U, self.V = geom.n_vox, geom.n_proj, geom.n_det_u, geom.n_det_v
