@@ -49,6 +49,34 @@ def cpu_baseline(N, U, P, sdd, sod, det, vs, n_views=2):
                       % (len(views), P, N, U, U, dt)}
 
 
+MODEL = os.path.join(ROOT, "xray-directional-reconstruction_amd", "models", "warmstart_c24.xunt")
+
+
+def time_to_residual(ctx, xr, geom, sino, tau, subsets, max_epochs=300, pre=5):
+    """Epochs and GPU seconds until the data residual r = sum|y - Ax|/sum|y|
+    reaches tau: the plain method (MLEM, `subsets` ordered subsets, x0 = 0.01)
+    against the learned warm start (plain for `pre` epochs, then the trained
+    network judged by the residual with fallback, then plain again)."""
+    import importlib
+
+    ws = importlib.import_module(xr.__name__ + ".warmstart")
+    net = xr.UNet(ctx, 24)
+    net.load(MODEL)
+    scene = ws.Scene(ctx, geom, sino, subsets=subsets)
+    plain = ws.plain_mlem(scene, tau, max_epochs)
+    warm = ws.warm_start(scene, net, tau, max_epochs, pre=pre)
+    scene.close()
+    net.close()
+    return {"tau": tau, "residual": "sum|y-Ax|/sum|y| (FP64)", "method": "MLEM, %d ordered subsets" % subsets,
+            "plain_epochs": plain["iterations"], "plain_s": round(plain["ms"] / 1e3, 3),
+            "plain_reached": plain["reached"],
+            "warm_epochs": warm["iterations"], "warm_s": round(warm["ms"] / 1e3, 3), "warm_reached": warm["reached"],
+            "network_kept": warm["used_net"],
+            "speedup_epochs": round(plain["iterations"] / max(warm["iterations"], 1), 3),
+            "speedup_time": round(plain["ms"] / max(warm["ms"], 1e-9), 3),
+            "network": "models/warmstart_c24.xunt (128^3, 180 views, 480 pairs)"}
+
+
 def measured_traffic(kernel, N, P):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy), if one
@@ -74,6 +102,10 @@ def main():
     ap.add_argument("--n-proj", type=int, default=720)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=2)
+    ap.add_argument("--no-warmstart", action="store_true",
+                    help="skip the time-to-residual comparison (learned warm start vs plain MLEM)")
+    ap.add_argument("--tau", type=float, default=1e-3)
+    ap.add_argument("--subsets", type=int, default=20)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -98,10 +130,12 @@ def main():
     N3 = N ** 3
     lenD = P * U * U
 
-    # synthetic phantom (40 seeded ellipsoids) and its sinogram by the same operator
+    # synthetic phantom (40 seeded ellipsoids plus a cylinder shell) and its
+    # sinogram by the same operator (noise free, so the residual can reach tau)
     params = xr.ellipsoid_params(1234, 40, fov_radius=0.5 * N * vs)
     vref = torch.zeros(N3, device=dev)
     ctx.phantom_rasterise(params, N, vs, vref)
+    ctx.phantom_cylinder_shell(N, vs, 0.45 * N * vs, 4 * vs, 0.02, vref)
     truth = torch.empty_like(vref)
     ctx.to_columns(vref, truth, N)
     del vref
@@ -163,6 +197,8 @@ def main():
         "loss_first_last": [losses[0], losses[-1]],
         "roofline": roofline,
     }
+    if not args.no_warmstart and os.path.exists(MODEL):
+        out["time_to_residual"] = time_to_residual(ctx, xr, geom, sino, args.tau, args.subsets)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, U, P, sdd, sod, det, vs, args.cpu_views)
     print(json.dumps(out))

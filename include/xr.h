@@ -269,8 +269,13 @@ xr_plan* xr_batch_plan(xr_batch* b, int i);
 /* Tune tile shapes now (per distinct shape) on the current buffers. */
 int xr_batch_autotune(xr_batch* b);
 /* Run `epochs` epochs on every problem.  losses: optional host array of
- * epochs*n doubles, [epoch][problem] raw sums of |y-p|. */
+ * epochs*n doubles, [epoch][problem] raw sums of |y-p|.  Each problem's
+ * subset order continues across runs (runs of a and b epochs visit subsets as
+ * one solve of a + b epochs does) until xr_batch_restart. */
 int xr_batch_run(xr_batch* b, int epochs, double* losses);
+/* Restart every problem's subset order (fresh default-seeded engine), as at
+ * the start of a new IR::reconstruct call. */
+int xr_batch_restart(xr_batch* b);
 /* Milliseconds of the last xr_batch_run on the GPU (HIP events around the
  * epoch loop, excluding the upload of the subset lists). */
 int xr_batch_elapsed_ms(const xr_batch* b, double* ms);

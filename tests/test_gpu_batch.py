@@ -104,6 +104,7 @@ def test_graph_replay_matches_eager(xr, ctx, problems, torch_dev):
                 ctx.copy(vptr, v)
                 ctx.copy(sptr, s)
             torch.cuda.synchronize()
+            b.restart()  # each replay is a fresh 2-epoch solve
             losses = b.run(2)
             outs.append(([read_back(ctx, b.buffers(i)[0], geoms[i].n_vox ** 3) for i in range(len(geoms))],
                          losses))
@@ -176,3 +177,32 @@ def test_batch_autotune_keeps_results(xr, ctx, problems):
                           flags=xr.BATCH_GRAPH | xr.BATCH_AUTOTUNE)
     for i in range(len(geoms)):
         assert np.array_equal(a[i], b[i])
+
+
+def test_runs_continue_the_subset_order(xr, ctx, problems):
+    """Runs of 1 + 2 epochs equal one 3-epoch run (and the standalone solve):
+    the subset order continues across runs until restart."""
+    geoms = [p[0] for p in problems]
+    batch = [c[2] for c in CASES]
+    b = ctx.batch(geoms, batch=batch)
+    init = [np.full(g.n_vox ** 3, 0.01, np.float32) for g in geoms]
+
+    def load():
+        import torch
+
+        for i, (g, sino, _) in enumerate(problems):
+            vptr, sptr = b.buffers(i)
+            ctx.copy(vptr, torch.from_numpy(init[i]).cuda())
+            ctx.copy(sptr, torch.from_numpy(sino).cuda())
+
+    load()
+    b.run(1)
+    b.run(2)
+    split = [read_back(ctx, b.buffers(i)[0], geoms[i].n_vox ** 3) for i in range(len(geoms))]
+    load()
+    b.restart()
+    b.run(3)
+    whole = [read_back(ctx, b.buffers(i)[0], geoms[i].n_vox ** 3) for i in range(len(geoms))]
+    for i in range(len(geoms)):
+        assert np.array_equal(split[i], whole[i]), i
+    b.close()

@@ -1,0 +1,113 @@
+"""Learned warm start: the residual judge with fallback, the positivity clamp,
+and what the trained network gains over the plain method.
+
+The plain method is IR::reconstruct's MLEM with ordered subsets
+(reconstruct.cu:23-156) from x0 = 0.01 (main.cu:40-42); the data residual is
+r = sum|y - A x| / sum|y| (FP64 sums in the forward kernel).  The committed
+network (models/warmstart_c24.xunt) was trained by train_warmstart.py on 480
+seeded 128^3 phantoms at 180 views, 20 subsets, 5 -> 200 epochs."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                     "xray-directional-reconstruction_amd", "models", "warmstart_c24.xunt")
+
+
+@pytest.fixture(scope="module")
+def ws(xr):
+    import importlib
+
+    return importlib.import_module(xr.__name__ + ".warmstart")
+
+
+def scene_for(xr, ctx, ws, N, P, S, seed, n_ellipsoids=12):
+    vs = 0.2
+    g = xr.make_geometry(1000.0, 500.0, 0.4, N, N * 3 // 2, P, vox_size=vs)
+    vol = ws.make_phantom(ctx, N, vs, seed, n_ellipsoids=n_ellipsoids)
+    sino = ws.project(ctx, g, vol)
+    return ws.Scene(ctx, g, sino, subsets=S)
+
+
+def test_positivity_floor(ws, torch_dev):
+    import torch
+
+    v = torch.tensor([-1.0, 0.0, 1e-9, 0.5], device=torch_dev)
+    ws.positivity_floor(v, 1e-6)
+    assert v.cpu().tolist() == pytest.approx([1e-6, 1e-6, 1e-6, 0.5])
+
+
+def test_fallback_when_network_fits_worse(xr, ctx, ws, orc):
+    """A network whose output fits the data worse than its input is rejected:
+    the run continues from the plain volume and matches the plain method."""
+    from test_oracle_unet import random_params
+
+    sc = scene_for(xr, ctx, ws, 48, 72, 6, 77)
+    bad = xr.UNet(ctx, 8, seed=1)
+    p = random_params(orc, 8, 3, head_scale=5.0)  # large random residual
+    bad.set_params(p)
+    tau = 5e-3
+    plain = ws.plain_mlem(sc, tau, 300)
+    warm = ws.warm_start(sc, bad, tau, 300, pre=5)
+    print("plain", plain["iterations"], "warm", warm["iterations"], "used", warm["used_net"], flush=True)
+    assert plain["reached"]
+    assert warm["used_net"] is False
+    assert warm["iterations"] == plain["iterations"]
+    np.testing.assert_allclose(warm["residuals"], plain["residuals"], rtol=1e-6)
+    sc.close()
+    bad.close()
+
+
+def test_identity_network_changes_nothing(xr, ctx, ws):
+    """An untrained network is the identity: its volume fits exactly as well as
+    its input, so the rule keeps the input (no improvement, no harm)."""
+    sc = scene_for(xr, ctx, ws, 48, 72, 6, 78)
+    net = xr.UNet(ctx, 8, seed=2)
+    plain = ws.plain_mlem(sc, 5e-3, 300)
+    warm = ws.warm_start(sc, net, 5e-3, 300, pre=5)
+    assert warm["iterations"] == plain["iterations"]
+    sc.close()
+    net.close()
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="trained network not present")
+def test_trained_network_gains_on_held_out_phantom(xr, ctx, ws):
+    """On a held-out 128^3 phantom (training distribution), MLEM resumed from the
+    trained network reaches r <= 1e-3 in far fewer epochs than the plain
+    method (measured 3.1-3.2x in epochs, 2.4-2.9x in GPU time)."""
+    net = xr.UNet(ctx, 24)
+    net.load(MODEL)
+    sc = scene_for(xr, ctx, ws, 128, 180, 20, 9002)
+    plain = ws.plain_mlem(sc, 1e-3, 400)
+    warm = ws.warm_start(sc, net, 1e-3, 400, pre=5)
+    print("plain %d epochs %.0f ms | warm %d epochs %.0f ms" % (plain["iterations"], plain["ms"], warm["iterations"],
+                                                                 warm["ms"]), flush=True)
+    assert plain["reached"] and warm["reached"] and warm["used_net"]
+    assert plain["iterations"] >= 2.0 * warm["iterations"]
+    sc.close()
+    net.close()
+
+
+def test_training_from_scratch_gains(xr, ctx, ws):
+    """Small end-to-end run: pairs from the plain method, training with the
+    library's kernels, and a gain on a held-out phantom."""
+    N, P, S = 48, 72, 8
+    inputs, targets, geom = ws.generate_pairs(ctx, 24, N=N, P=P, U=N * 3 // 2, pre=5, post=120, seed=500,
+                                              subsets=S)
+    net = xr.UNet(ctx, 16, seed=3)
+    hist = ws.train(net, inputs, targets, (N, N, N), 600, lr=1e-3, seed=3)
+    k = 50
+    assert np.mean(hist[-k:]) < 0.6 * np.mean(hist[:k])
+    sc = scene_for(xr, ctx, ws, N, P, S, 9500)
+    tau = 2e-3
+    plain = ws.plain_mlem(sc, tau, 400)
+    warm = ws.warm_start(sc, net, tau, 400, pre=5)
+    print("scratch: plain %d epochs | warm %d epochs used %s" % (plain["iterations"], warm["iterations"],
+                                                                warm["used_net"]), flush=True)
+    assert plain["reached"] and warm["reached"]
+    assert warm["iterations"] < plain["iterations"]
+    sc.close()
+    net.close()

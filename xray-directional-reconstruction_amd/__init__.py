@@ -143,6 +143,7 @@ def _load():
         "xr_batch_plan": (P, [P, I]),
         "xr_batch_autotune": (I, [P]),
         "xr_batch_run": (I, [P, I, _c_double_p]),
+        "xr_batch_restart": (I, [P]),
         "xr_batch_elapsed_ms": (I, [P, _c_double_p]),
         "xr_unet_create": (I, [P, I, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
         "xr_unet_destroy": (V, [P]),
@@ -515,6 +516,10 @@ class Batch:
         losses = np.zeros(max(1, epochs * self.n), dtype=np.float64)
         self.ctx.check(_lib.xr_batch_run(self.h, epochs, losses.ctypes.data_as(_c_double_p)))
         return losses[: epochs * self.n].reshape(epochs, self.n)
+
+    def restart(self):
+        """Restart the subset order (as a new IR::reconstruct call)."""
+        self.ctx.check(_lib.xr_batch_restart(self.h))
 
     def elapsed_ms(self):
         ms = ctypes.c_double()

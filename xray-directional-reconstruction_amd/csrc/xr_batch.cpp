@@ -59,17 +59,16 @@ struct EventPair {
 };
 
 // Subset lists of one problem for `epochs` epochs, laid out [epoch][pos][j]
-// with pos < maxb (positions >= batch unused): one engine per problem, as
-// xr_ir_reconstruct uses for one condition.
-void problem_codes(int P, int batch, int maxb, int epochs, std::vector<int32_t>& codes, std::vector<uint8_t>& flags,
-                   bool& any_flag) {
+// with pos < maxb (positions >= batch unused).  `rng` and `order` are the
+// problem's engine and subset order, continued from the previous run, so that
+// consecutive runs of a and b epochs visit subsets exactly as one
+// xr_ir_reconstruct solve of a + b epochs does (reconstruct.cu:75, 89).
+void problem_codes(int P, int batch, int maxb, int epochs, std::mt19937_64& rng, std::vector<int>& order,
+                   std::vector<int32_t>& codes, std::vector<uint8_t>& flags, bool& any_flag) {
     const int ss = (P + batch - 1) / batch;
     codes.assign((size_t)epochs * maxb * ss, 0);
     flags.assign(codes.size(), 0);
     any_flag = false;
-    std::vector<int> order(batch);
-    std::iota(order.begin(), order.end(), 0);
-    std::mt19937_64 rng;
     for (int ep = 0; ep < epochs; ep++) {
         std::shuffle(order.begin(), order.end(), rng);
         std::vector<char> visited(P, 0);
@@ -102,6 +101,13 @@ struct xr_batch {
         int32_t* codes = nullptr;
         uint8_t* flags = nullptr;
         bool any_flag = false;
+        std::mt19937_64 rng;     // subset-order engine (default seed), continued across runs
+        std::vector<int> order;  // current subset order
+        void restart() {
+            rng = std::mt19937_64();
+            order.resize(batch);
+            std::iota(order.begin(), order.end(), 0);
+        }
     };
     std::vector<Problem> pr;
     std::vector<int> order;  // problems by descending work
@@ -514,6 +520,7 @@ int xr_batch_create(xr_context* ctx, int n, const xr_geometry* geoms, const xr_s
         p.batch = batch ? batch[i] : 1;
         if (p.batch <= 0 || p.batch > p.P) return fail(ctx, XR_ERR_INVALID, "problem %d: bad batch %d", i, p.batch);
         p.ss = (p.P + p.batch - 1) / p.batch;
+        p.restart();
         b->maxb = std::max(b->maxb, p.batch);
         XR_TRY(xr_plan_create(ctx, &geoms[i], s, XR_CONV_IR, direction, &p.plan));
         XR_TRY(ensure_norm(p.plan));
@@ -577,7 +584,7 @@ int xr_batch_run(xr_batch* b, int epochs, double* losses) {
     std::vector<int32_t> codes;
     std::vector<uint8_t> fl;
     for (auto& p : b->pr) {
-        problem_codes(p.P, p.batch, b->maxb, epochs, codes, fl, p.any_flag);
+        problem_codes(p.P, p.batch, b->maxb, epochs, p.rng, p.order, codes, fl, p.any_flag);
         XR_HIP(ctx, hipMemcpyAsync(p.codes, codes.data(), sizeof(int32_t) * codes.size(), hipMemcpyHostToDevice,
                                    b->stream));
         XR_HIP(ctx, hipMemcpyAsync(p.flags, fl.data(), fl.size(), hipMemcpyHostToDevice, b->stream));
@@ -615,6 +622,12 @@ int xr_batch_run(xr_batch* b, int epochs, double* losses) {
     b->last_ms = ms;
     if (losses)
         XR_HIP(ctx, hipMemcpy(losses, b->d_loss, sizeof(double) * (size_t)epochs * b->n, hipMemcpyDeviceToHost));
+    return XR_OK;
+}
+
+int xr_batch_restart(xr_batch* b) {
+    if (!b) return XR_ERR_INVALID;
+    for (auto& p : b->pr) p.restart();
     return XR_OK;
 }
 

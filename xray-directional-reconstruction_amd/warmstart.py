@@ -52,6 +52,11 @@ class Scene:
     def set_volume(self, vol_cols):
         self.ctx.copy(self.vol_addr, vol_cols)
 
+    def restart(self, vol_cols):
+        """Start a new solve from vol_cols (subset order from the beginning)."""
+        self.batch.restart()
+        self.set_volume(vol_cols)
+
     def get_volume(self, out):
         self.ctx.copy(out, self.vol_addr, out.numel() * 4)
 
@@ -103,9 +108,10 @@ def project(ctx, geom, vol_cols):
 
 
 def generate_pairs(ctx, count, N=128, P=180, U=192, sdd=1000.0, sod=500.0, det=0.4, pre=5, post=200, seed=0,
-                   per_batch=48, init=0.01, log=None):
-    """(inputs, targets): the plain MLEM volume after `pre` and after `post`
-    iterations for `count` seeded phantoms, all from the batched solver
+                   per_batch=48, init=0.01, subsets=1, log=None):
+    """(inputs, targets, geometry): the plain method's volume (MLEM with
+    `subsets` ordered subsets, IR::reconstruct's batch) after `pre` and after
+    `post` epochs for `count` seeded phantoms, all from the batched solver
     (`per_batch` problems per persistent launch).  Lists of column-layout
     device tensors."""
     import torch
@@ -115,7 +121,7 @@ def generate_pairs(ctx, count, N=128, P=180, U=192, sdd=1000.0, sod=500.0, det=0
     inputs, targets = [], []
     for b0 in range(0, count, per_batch):
         n = min(per_batch, count - b0)
-        batch = ctx.batch([geom] * n, flags=BATCH_GRAPH)
+        batch = ctx.batch([geom] * n, batch=[subsets] * n, flags=BATCH_GRAPH)
         for i in range(n):
             vol = make_phantom(ctx, N, vs, seed + b0 + i)
             sino = project(ctx, geom, vol)
@@ -139,9 +145,12 @@ def generate_pairs(ctx, count, N=128, P=180, U=192, sdd=1000.0, sod=500.0, det=0
     return inputs, targets, geom
 
 
-def train(net, inputs, targets, dims, steps, lr=3e-4, seed=0, log=None, log_every=50):
-    """Adam on single-volume steps in a seeded random order; returns the loss history."""
+def train(net, inputs, targets, dims, steps, lr=3e-4, seed=0, log=None, log_every=50, lr_final=None):
+    """Adam on single-volume steps in a seeded random order, learning rate
+    cosine-decayed from `lr` to `lr_final` (default lr/10); returns the loss
+    history."""
     rng = np.random.default_rng(seed)
+    lr_final = lr / 10 if lr_final is None else lr_final
     hist = []
     order = []
     for k in range(steps):
@@ -149,7 +158,8 @@ def train(net, inputs, targets, dims, steps, lr=3e-4, seed=0, log=None, log_ever
             order = list(rng.permutation(len(inputs)))
         i = order.pop()
         hist.append(net.loss_grad(inputs[i], targets[i], dims))
-        net.adam_step(lr)
+        t = k / max(steps - 1, 1)
+        net.adam_step(lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * t)))
         if log and (k % log_every == 0 or k == steps - 1):
             log("step %d loss %.5f" % (k, hist[-1]))
     return hist
@@ -166,7 +176,7 @@ def plain_mlem(scene, tau, max_iters, init=0.01):
     """Plain MLEM from a constant volume (main.cu:40-42) until r <= tau."""
     import torch
 
-    scene.set_volume(torch.full((scene.N ** 3,), init, dtype=torch.float32, device=scene.sino.device))
+    scene.restart(torch.full((scene.N ** 3,), init, dtype=torch.float32, device=scene.sino.device))
     return _resume(scene, tau, max_iters, 0, 0.0, [])
 
 
@@ -191,7 +201,7 @@ def warm_start(scene, net, tau, max_iters, pre=5, init=0.01, floor_frac=1e-3, us
 
     dev = scene.sino.device
     N3 = scene.N ** 3
-    scene.set_volume(torch.full((N3,), init, dtype=torch.float32, device=dev))
+    scene.restart(torch.full((N3,), init, dtype=torch.float32, device=dev))
     r, ms = scene.iterate(pre)
     hist = [float(v) for v in r]
     used = False
