@@ -12,6 +12,8 @@ constexpr int kBandU = 16;
 // LDS row stride of a band accumulator: one guard column on each side plus
 // one pad column so the stride is odd (conflict-free rows on 32 banks).
 constexpr int kBandStride = kBandU + 3;
+// Waves per forward-projection block (they share one band accumulator).
+constexpr int kFwdWaves = 8;
 // 8-byte slot index just past a forward band accumulator of `rows` rows of
 // `acc_bytes`-byte words (the per-wave reduction slots follow it).
 __host__ __device__ constexpr int fwd_red_slot(int rows, int acc_bytes) {

@@ -219,7 +219,7 @@ int forward_rows(const xr_plan* p, int kind) {
     // the chunk height further (smaller LDS footprint, more resident blocks).
     const int V = p->g.n_det_v;
     const int word = (kind == FWD_WEIGHT) ? 8 : 4;
-    int max_rows = (64 * 1024 - 8 * 8 - 16) / (kBandStride * word) - 2;
+    int max_rows = (64 * 1024 - 8 * kFwdWaves - 16) / (kBandStride * word) - 2;
     if (p->tiles.fwd_rows > 0) max_rows = std::min(max_rows, (int)p->tiles.fwd_rows);
     const int n_chunks = (V + max_rows - 1) / max_rows;
     return (V + n_chunks - 1) / n_chunks;
