@@ -39,6 +39,14 @@ namespace {
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Wave-uniform copy of a value loaded through a vector load: the compiler
+// then keeps everything derived from it in scalar registers and scalar
+// branches (a view's table is the same for every lane).
+__device__ __forceinline__ float uniformf(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ int uniformi(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 struct ColForms {
     float nu, nv, nd;
 };
@@ -73,6 +81,25 @@ __device__ __forceinline__ void split(float u, int& iu, float& fu) {
     const float f = floorf(t);
     iu = (int)f;
     fu = t - f;
+}
+
+__device__ __forceinline__ ViewGeom uniform_view(const ViewGeom& v) {
+    ViewGeom u;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        u.au[i] = uniformf(v.au[i]);
+        u.av[i] = uniformf(v.av[i]);
+        u.ad[i] = uniformf(v.ad[i]);
+        u.pad[i] = 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        u.d0[i] = uniformf(v.d0[i]);
+        u.G[i] = uniformf(v.G[i]);
+    }
+    u.fy = uniformf(v.fy);
+    u.axis = uniformf(v.axis);
+    return u;
 }
 
 typedef __attribute__((address_space(3))) float lds_float;
@@ -211,6 +238,8 @@ __device__ __forceinline__ FwdCol fwd_col(const FwdArgs& a, const ViewGeom& g, i
             }
         }
     }
+    f.z_beg = uniformi(f.z_beg);
+    f.z_end = uniformi(f.z_end);
     return f;
 }
 
@@ -248,29 +277,38 @@ __device__ __forceinline__ void fwd_splat(const FwdArgs& a, const ViewGeom& g, c
     }
 }
 
-// Untilted splat: u, hence (iu, fu), is constant along the column, so the
-// four weights factor into (1-fu, fu) x (1-fv, fv); wu0 = (1-fu)*s and
-// wu1 = fu*s carry the fixed-point scale.  colp = band column iu's row 0.
-// (Pre-combining adjacent lanes' shares of a row with DPP halves the LDS
-// atomics but measured slower: 234 ms vs 194 ms per 512^3/720 pass.)
+// floor(x + 0.5) in one instruction (round half up; x is far inside int range).
+__device__ __forceinline__ int round_half_up(float x) {
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// Untilted splat of one 64-z chunk of one column.  u, hence (iu, fu), is
+// constant along the column, so the weights factor into (1-fu, fu) x
+// (1-fv, fv) with wu0 = (1-fu)*s, wu1 = fu*s carrying the fixed-point scale,
+// and v is affine in z: t = v - 0.5 = B*z_lane + A (B, A per column and chunk
+// group).  The acceptance window 0.55 < v < V-0.55 and the chunk's row range
+// reduce to lo < t < hi.  Branch free: a rejected voxel adds zero.  cp = band
+// column iu's first row.
 template <class AccT, class AccS>
-__device__ __forceinline__ void fwd_splat_axis(const Dims& d, const ViewGeom& g, const FwdCol& f, int z, float val,
-                                               lds_ptr<AccT> colp, float wu0, float wu1, int v0, int vb) {
-    const float zp = (float)z - d.xc;
-    const float v = fmaf(fmaf(g.av[2], zp, f.c.nv), f.inv, d.hv);
-    int iv;
-    float fv;
-    split(v, iv, fv);
-    if (z >= f.z_beg && z < f.z_end && accept_v(v, d) && iv >= v0 - 1 && iv < v0 + vb) {
-        const float g1 = fv * val, g0 = (1.0f - fv) * val;
-        lds_ptr<AccT> p = colp + (iv - v0 + 1) * kBandStride;
-        const int q4 = __float2int_rn(wu0 * g0), q3 = __float2int_rn(wu1 * g0);
-        const int q1 = __float2int_rn(wu0 * g1), q2 = __float2int_rn(wu1 * g1);
-        __hip_atomic_fetch_add(p, (AccT)(AccS)q4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(p + 1, (AccT)(AccS)q3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(p + kBandStride, (AccT)(AccS)q1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(p + kBandStride + 1, (AccT)(AccS)q2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
+__device__ __forceinline__ void splat_untilted(float t, float val, lds_ptr<AccT> cp, float wu0, float wu1, float lo,
+                                               float hi, int v0, int spare_row) {
+    const bool ok = t > lo && t < hi;
+    const float fl = floorf(t);
+    const float fv = t - fl;
+    const float w = ok ? val : 0.0f;
+    const float g1 = fv * w, g0 = w - g1;
+    // a rejected voxel adds zero at a lane-dependent row (any row in the band:
+    // one shared address would serialise the wave's atomics)
+    const int row = ok ? (int)fl - v0 + 1 : spare_row;
+    lds_ptr<AccT> p = cp + row * kBandStride;
+    const int q4 = round_half_up(wu0 * g0), q3 = round_half_up(wu1 * g0);
+    const int q1 = round_half_up(wu0 * g1), q2 = round_half_up(wu1 * g1);
+    __hip_atomic_fetch_add(p, (AccT)(AccS)q4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(p + 1, (AccT)(AccS)q3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(p + kBandStride, (AccT)(AccS)q1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(p + kBandStride + 1, (AccT)(AccS)q2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // XTT voxel value: sum_k |B x S_k|^2 (S_k.G)^2 mu_k * vs * ratio (ir.cu:1780-1794),
@@ -310,7 +348,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const auto proj = to_global(a.proj);
     const int code = views[slot];
     const int cond = code / d.P;
-    const ViewGeom g = to_global(a.vg)[code];
+    const ViewGeom g = uniform_view(to_global(a.vg)[code]);
     const int u0 = band * kBandU;
     const int rows = vb + 2;
     for (int i = threadIdx.x; i < rows * kBandStride; i += 64 * kFwdWaves) acc[i] = 0ull;
@@ -332,6 +370,14 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const float scalev = (KIND == FWD_PLAIN) ? d.ratio : 1.f;
     const float zlo = -d.xc, zhi = (float)(d.N - 1) - d.xc;
     const float band_lo = (float)u0 - 0.5f, band_hi = (float)(u0 + kBandU) + 0.5f;
+    // untilted path: lane offsets of the z chunks, and the window of t = v - 0.5:
+    // 0.55 < v < V - 0.55 (ir.cu:1501) and v0 - 1 <= floor(t) < v0 + vb
+    float lanef[kFwdChunks];
+#pragma unroll
+    for (int c = 0; c < kFwdChunks; c++) lanef[c] = (float)(c * 64 + lane);
+    const float t_lo = (v0 >= 2) ? __int_as_float(__float_as_int((float)(v0 - 1)) - 1) : 0.05f;
+    const float t_hi = fminf(d.vmax - 0.5f, (float)(v0 + vb));
+    const int spare_row = min(lane, vb);  // rows spare_row, spare_row + 1 lie in the band
 
     for (int y = wave; y < d.N; y += kFwdWaves) {
         const float yp = (float)y - d.xc;
@@ -415,20 +461,32 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                         for (int c = 0; c < kFwdChunks; c++) {
                             const int z = zg + c * 64 + lane;
                             if (KIND == FWD_WEIGHT) {
-                                m0[c] = 1.f;
-                                m1[c] = 1.f;
+                                // zero outside the column's z range: the untilted splat has no z test
+                                m0[c] = (z < f0.z_end && z >= f0.z_beg) ? 1.f : 0.f;
+                                m1[c] = (two && z < f1.z_end && z >= f1.z_beg) ? 1.f : 0.f;
                             } else {
                                 m0[c] = (z < f0.z_end && z >= f0.z_beg) ? vol[f0.base + z] : 0.f;
                                 m1[c] = (two && z < f1.z_end && z >= f1.z_beg) ? vol[f1.base + z] : 0.f;
                             }
                         }
                         if (AXIS) {
+                            // t = v - 0.5 = B*(z - xc) + (nv*inv + hv - 0.5); z - xc is exact in
+                            // FP32 (half-integers below 2^23), so t depends on z alone and not
+                            // on where the chunk group starts
+                            const float zoff = (float)zg - d.xc;
+                            const float B0 = g.av[2] * f0.inv, B1 = g.av[2] * f1.inv;
+                            const float A0 = fmaf(f0.c.nv, f0.inv, d.hv - 0.5f);
+                            const float A1 = fmaf(f1.c.nv, f1.inv, d.hv - 0.5f);
 #pragma unroll
                             for (int c = 0; c < kFwdChunks; c++) {
                                 if (zg + c * 64 >= ze) break;
-                                const int z = zg + c * 64 + lane;
-                                if (ok0) fwd_splat_axis<AccT, AccS>(d, g, f0, z, m0[c], cp0, w00, w01, v0, vb);
-                                if (ok1) fwd_splat_axis<AccT, AccS>(d, g, f1, z, m1[c], cp1, w10, w11, v0, vb);
+                                const float zp = lanef[c] + zoff;
+                                if (ok0)
+                                    splat_untilted<AccT, AccS>(fmaf(B0, zp, A0), m0[c], cp0, w00, w01, t_lo, t_hi, v0,
+                                                               spare_row);
+                                if (ok1)
+                                    splat_untilted<AccT, AccS>(fmaf(B1, zp, A1), m1[c], cp1, w10, w11, t_lo, t_hi, v0,
+                                                               spare_row);
                             }
                             continue;
                         }
@@ -731,7 +789,7 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
         zin[q] = z < d.N;
     }
     int code = views[0];
-    ViewGeom g = vg[code];
+    ViewGeom g = uniform_view(vg[code]);
     Foot f = footprint(g, d, bx0, bx1, by0, by1, bz0, bz1, lane);
     lds_float* const lds = (lds_float*)stage;
     if (f.staged) {
@@ -749,7 +807,7 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
         Foot nf{0, 0, 0, 0, 0, 0, false, false};
         if (s + 1 < a.n_views) {
             ncode = views[s + 1];
-            ng = vg[ncode];
+            ng = uniform_view(vg[ncode]);
             nf = footprint(ng, d, bx0, bx1, by0, by1, bz0, bz1, lane);
             if (nf.staged) fetch(nf, ncode);
         }
