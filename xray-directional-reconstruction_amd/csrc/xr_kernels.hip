@@ -781,13 +781,8 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
     if (a.n_views == 0) return;
     // view-independent per-lane z terms
     float zpq[ZPL];
-    bool zin[ZPL];
 #pragma unroll
-    for (int q = 0; q < ZPL; q++) {
-        const int z = z0 + lane + 64 * q;
-        zpq[q] = (float)z - d.xc;
-        zin[q] = z < d.N;
-    }
+    for (int q = 0; q < ZPL; q++) zpq[q] = (float)(z0 + lane + 64 * q) - d.xc;
     int code = views[0];
     ViewGeom g = uniform_view(vg[code]);
     Foot f = footprint(g, d, bx0, bx1, by0, by1, bz0, bz1, lane);
@@ -845,26 +840,30 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
                         dy = fmaf(d.vs, yp, g.d0[1]);
                     }
                     if (AXIS && !XTT) {
-                        // Untilted: u, hence (iu, fu), is constant along the column and the
-                        // bilinear weights factor into (1-fu, fu) x (1-fv, fv).  Branch free:
-                        // a rejected voxel reads a clamped (valid) texel and adds zero.
+                        // Untilted: u, hence (iu, fu), is constant along the column, the
+                        // bilinear weights factor into (1-fu, fu) x (1-fv, fv), and
+                        // t = v - 0.5 is affine in z.  The acceptance window 0.55 < u, v <
+                        // U, V - 0.55 becomes 0.05 < t < hi (hi < 0 rejects the column).
+                        // Branch free: a rejected voxel reads an in-bounds texel and adds
+                        // zero.  Lanes with z >= N accumulate but are never stored.
                         int iu;
                         float fu;
                         split(u_c, iu, fu);
-                        const bool uok = accept_u(u_c, d);
                         const float a0 = 1.0f - fu;
+                        const float Bc = g.av[2] * inv_c;
+                        const float Ac = fmaf(c.nv, inv_c, d.hv - 0.5f);
+                        const float hi = accept_u(u_c, d) ? d.vmax - 0.5f : -1.0f;
+                        const int lu = min(max(iu - f.iu_lo, 0), f.W - 2);
 #pragma unroll
                         for (int q = 0; q < ZPL; q++) {
-                            const float v = fmaf(fmaf(g.av[2], zpq[q], c.nv), inv_c, d.hv);
-                            const bool ok = zin[q] && uok && accept_v(v, d);
-                            int iv;
-                            float fv;
-                            split(v, iv, fv);
+                            const float t = fmaf(Bc, zpq[q], Ac);
+                            const bool ok = t > 0.05f && t < hi;
+                            const float fl = floorf(t);
+                            const float fv = t - fl;
+                            const int iv = (int)fl;
                             float r00, r01, r10, r11;
                             if (f.staged) {
-                                const int lu = min(max(iu - f.iu_lo, 0), f.W - 2);
-                                const int lv = min(max(iv - f.iv_lo, 0), f.H - 2);
-                                const int qi = lv * f.Ws + lu;
+                                const int qi = (ok ? iv - f.iv_lo : 0) * f.Ws + lu;
                                 r00 = cur[qi];
                                 r01 = cur[qi + 1];
                                 r10 = cur[qi + f.Ws];
