@@ -74,7 +74,8 @@ def time_to_residual(ctx, xr, geom, sino, tau, subsets, max_epochs=300, pre=5):
             "network_kept": warm["used_net"],
             "speedup_epochs": round(plain["iterations"] / max(warm["iterations"], 1), 3),
             "speedup_time": round(plain["ms"] / max(warm["ms"], 1e-9), 3),
-            "network": "models/warmstart_c24.xunt (128^3, 180 views in 20 subsets of 9, 480 pairs, epochs 5 -> 200)"}
+            "network": "models/warmstart_c24.xunt (128^3, 180 views in 20 subsets of 9, 960 mixed phantoms, "
+                       "inputs after 5/10/20 epochs, targets after 200)"}
 
 
 def batch_workload(xr, args, n=48, epochs=30, seed=4):
@@ -257,7 +258,8 @@ def main():
     out = {
         "metric": METRIC, "value": round(gups, 2), "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (40 seeded ellipsoids, sinogram from the projector)",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (40 seeded ellipsoids plus a cylinder shell, noise-free sinogram from the projector)",
         "config": {"workload": "plain MLEM iteration (batch 1), %d^3 volume, %d views, %dx%d detector, "
                                "SOD 500 SDD 1000 det 0.4 mm" % (N, P, U, U),
                    "n_vox": N, "n_proj": P, "n_det": U, "parallelism": "single GPU"},

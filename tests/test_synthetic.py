@@ -99,3 +99,29 @@ def test_phantoms_gpu_match_oracle(orc, xr, ctx, torch_dev):
     shell = torch.zeros(N ** 3, device=torch_dev)
     ctx.phantom_cylinder_shell(N, vs, 10.0, 1.5, 0.04, shell)
     assert np.count_nonzero(shell.cpu().numpy().reshape(N, N, N) != orc.phantom_cylinder_shell(N, vs, 10.0, 1.5, 0.04)) <= 2
+
+
+def test_training_phantom_draw(xr):
+    """The warm-start training phantoms: the draw depends only on the seed,
+    "fibres" is always the ellipsoid-and-fibre default, and "mixed" keeps its
+    stated proportions (0.4 default; 0.75 of the rest with a cylinder shell)
+    and parameter ranges."""
+    import importlib
+
+    ws = importlib.import_module(xr.__name__ + ".warmstart")
+    N, vs = 128, 0.2
+    specs = [ws.phantom_spec(s, N, vs) for s in range(2000)]
+    assert specs == [ws.phantom_spec(s, N, vs) for s in range(2000)]
+    assert all(ws.phantom_spec(s, N, vs, "fibres") == {} for s in range(50))
+    dense = [d for d in specs if d]
+    n = len(specs)
+    assert abs(len(dense) / n - 0.6) < 5 * np.sqrt(0.24 / n)
+    shells = [d["shell"] for d in dense if "shell" in d]
+    assert abs(len(shells) / len(dense) - 0.75) < 5 * np.sqrt(0.1875 / len(dense))
+    for d in dense:
+        assert 20 <= d["n_ellipsoids"] <= 48
+        assert d["n_fibres"] == 0 or 8 <= d["n_fibres"] <= 24
+    r, t, v = np.array(shells).T
+    assert r.min() >= 0.35 * N * vs and r.max() <= 0.48 * N * vs
+    assert t.min() >= vs and t.max() <= 5 * vs
+    assert v.min() >= 0.01 and v.max() <= 0.04

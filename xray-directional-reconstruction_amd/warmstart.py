@@ -21,7 +21,7 @@ import numpy as np
 
 from . import BATCH_GRAPH, METHOD_MLEM, OUT_RATIO, Scan, UNet, XRError, make_geometry  # noqa: F401
 
-__all__ = ["Scene", "make_phantom", "generate_pairs", "train", "warm_start", "plain_mlem", "positivity_floor"]
+__all__ = ["Scene", "make_phantom", "phantom_spec", "generate_pairs", "train", "warm_start", "plain_mlem", "positivity_floor"]
 
 
 class Scene:
@@ -72,10 +72,16 @@ class Scene:
         return losses / self.y_l1, self.batch.elapsed_ms()
 
 
-def make_phantom(ctx, N, vox_size, seed, n_ellipsoids=12, n_fibres=24, fibre_value=0.02, vol_cols=None):
-    """Seeded ellipsoids (attenuation uniform in [0.005, 0.05]/mm) plus fibre
-    bundles (3 von Mises-Fisher axes, kappa 20, radius 2-6 voxels) as a
-    column-layout device volume."""
+def make_phantom(ctx, N, vox_size, seed, n_ellipsoids=12, n_fibres=24, fibre_value=0.02, vol_cols=None,
+                 ellipsoid_kw=None, shell=None):
+    """Seeded ellipsoids plus fibre bundles (3 von Mises-Fisher axes, kappa
+    20, radius 2-6 voxels) and optionally a z-axis cylinder shell, as a
+    column-layout device volume.
+
+    `ellipsoid_kw` overrides the ellipsoid draw (default: attenuation uniform
+    in [0.005, 0.05]/mm, centres within half the field of view, semi-axes
+    0.08-0.3 of its radius); `shell` = (outer radius, wall thickness, value)
+    in mm, mm and 1/mm."""
     import torch
 
     from . import ellipsoid_params, fibre_params
@@ -83,9 +89,12 @@ def make_phantom(ctx, N, vox_size, seed, n_ellipsoids=12, n_fibres=24, fibre_val
     dev = torch.device("cuda")
     fov = 0.5 * N * vox_size
     ref = torch.zeros(N ** 3, dtype=torch.float32, device=dev)
-    ep = ellipsoid_params(seed, n_ellipsoids, fov_radius=fov, centre_frac=0.5, axis_lo=0.08, axis_hi=0.3,
-                          value_lo=0.005, value_hi=0.05)
+    kw = dict(centre_frac=0.5, axis_lo=0.08, axis_hi=0.3, value_lo=0.005, value_hi=0.05)
+    kw.update(ellipsoid_kw or {})
+    ep = ellipsoid_params(seed, n_ellipsoids, fov_radius=fov, **kw)
     ctx.phantom_rasterise(ep, N, vox_size, ref)
+    if shell is not None:
+        ctx.phantom_cylinder_shell(N, vox_size, shell[0], shell[1], shell[2], ref)
     if n_fibres:
         fp, _ = fibre_params(seed + 7919, n_fibres, n_axes=3, kappa=20.0, fov_radius=fov, centre_frac=0.5,
                              r_lo=2.0 * vox_size, r_hi=6.0 * vox_size, half_length=0.6 * fov)
@@ -93,6 +102,31 @@ def make_phantom(ctx, N, vox_size, seed, n_ellipsoids=12, n_fibres=24, fibre_val
     out = torch.empty_like(ref) if vol_cols is None else vol_cols
     ctx.to_columns(ref, out, N)
     return out
+
+
+def phantom_spec(seed, N, vox_size, family="mixed"):
+    """Keyword arguments of `make_phantom` for training phantom `seed`.
+
+    family "fibres": 12 ellipsoids and 24 fibre bundles.  family "mixed":
+    that with probability 0.4; otherwise a dense set of 20-48 overlapping
+    ellipsoids (attenuation 0-0.05/mm each, centres within 0.6 of the field
+    of view, semi-axes 0.1-0.35 of its radius), a cylinder shell with
+    probability 0.75 (outer radius 0.35-0.48 N voxels, wall 1-5 voxels,
+    0.01-0.04/mm) and 8-24 fibre bundles with probability 0.3.  The draw
+    depends only on `seed`."""
+    if family == "fibres":
+        return {}
+    rng = np.random.default_rng([seed, 0x5EED])
+    if rng.random() < 0.4:
+        return {}
+    spec = {"n_ellipsoids": int(rng.integers(20, 49)), "n_fibres": 0,
+            "ellipsoid_kw": dict(centre_frac=0.6, axis_lo=0.1, axis_hi=0.35, value_lo=0.0, value_hi=0.05)}
+    if rng.random() < 0.75:
+        spec["shell"] = (float(rng.uniform(0.35, 0.48)) * N * vox_size, float(rng.uniform(1.0, 5.0)) * vox_size,
+                         float(rng.uniform(0.01, 0.04)))
+    if rng.random() < 0.3:
+        spec["n_fibres"] = int(rng.integers(8, 25))
+    return spec
 
 
 def project(ctx, geom, vol_cols):
@@ -108,38 +142,53 @@ def project(ctx, geom, vol_cols):
 
 
 def generate_pairs(ctx, count, N=128, P=180, U=192, sdd=1000.0, sod=500.0, det=0.4, pre=5, post=200, seed=0,
-                   per_batch=48, init=0.01, subsets=1, log=None):
+                   per_batch=48, init=0.01, subsets=1, family="fibres", log=None):
     """(inputs, targets, geometry): the plain method's volume (MLEM with
     `subsets` ordered subsets, IR::reconstruct's batch) after `pre` and after
-    `post` epochs for `count` seeded phantoms, all from the batched solver
-    (`per_batch` problems per persistent launch).  Lists of column-layout
-    device tensors."""
+    `post` epochs for `count` seeded phantoms (`phantom_spec(seed + i,
+    family)`), all from the batched solver (`per_batch` problems per
+    persistent launch).  `pre` may be a list of epoch counts: each phantom
+    then gives one input per count, all paired with the same target (the same
+    tensor object).  Lists of column-layout device tensors."""
     import torch
 
+    pres = sorted(set([pre] if np.isscalar(pre) else [int(p) for p in pre]))
+    if pres[0] < 0 or pres[-1] >= post:
+        raise ValueError("pre epochs must lie in [0, post)")
     vs = sod * det / sdd  # one voxel per magnified detector pixel (geometry.h:13)
     geom = make_geometry(sdd, sod, det, N, U, P, vox_size=vs)
     inputs, targets = [], []
+
+    def snapshot(batch, n):
+        out = []
+        for i in range(n):
+            x = torch.empty(N ** 3, dtype=torch.float32, device="cuda")
+            ctx.copy(x, batch.buffers(i)[0], x.numel() * 4)
+            out.append(x)
+        return out
+
     for b0 in range(0, count, per_batch):
         n = min(per_batch, count - b0)
         batch = ctx.batch([geom] * n, batch=[subsets] * n, flags=BATCH_GRAPH)
         for i in range(n):
-            vol = make_phantom(ctx, N, vs, seed + b0 + i)
+            vol = make_phantom(ctx, N, vs, seed + b0 + i, **phantom_spec(seed + b0 + i, N, vs, family))
             sino = project(ctx, geom, vol)
             va, sa = batch.buffers(i)
             ctx.copy(sa, sino)
             ctx.copy(va, torch.full((N ** 3,), init, dtype=torch.float32, device=sino.device))
         t0 = time.time()
-        batch.run(pre)
-        for i in range(n):
-            x = torch.empty(N ** 3, dtype=torch.float32, device="cuda")
-            ctx.copy(x, batch.buffers(i)[0], x.numel() * 4)
-            inputs.append(x)
-        batch.run(post - pre)
-        for i in range(n):
-            t = torch.empty(N ** 3, dtype=torch.float32, device="cuda")
-            ctx.copy(t, batch.buffers(i)[0], t.numel() * 4)
-            targets.append(t)
+        done, states = 0, []
+        for p in pres:
+            if p > done:
+                batch.run(p - done)
+                done = p
+            states.append(snapshot(batch, n))
+        batch.run(post - done)
+        final = snapshot(batch, n)
         batch.close()
+        for st in states:
+            inputs.extend(st)
+            targets.extend(final)
         if log:
             log("pairs %d-%d: %.1f s" % (b0, b0 + n, time.time() - t0))
     return inputs, targets, geom
