@@ -24,7 +24,7 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_VOXEL_VIEW = 4.0  # one FP32 volume stream per view (plain forward / back)
 
 
-def cpu_baseline(N, U, P, sdd, sod, det, vs, n_views=2):
+def cpu_baseline(N, U, P, sdd, sod, det, vs, n_views=16):
     """Time the oracle's FP32 restatement (the reference's operators on the CPU)
     on a bounded sample: forward + back of n_views views of the same workload."""
     import __graft_entry__ as entry
@@ -164,7 +164,7 @@ def main():
     ap.add_argument("--n-det", type=int, default=768)
     ap.add_argument("--n-proj", type=int, default=720)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-views", type=int, default=2)
+    ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-warmstart", action="store_true",
                     help="skip the time-to-residual comparison (learned warm start vs plain MLEM)")
     ap.add_argument("--tau", type=float, default=1e-3)

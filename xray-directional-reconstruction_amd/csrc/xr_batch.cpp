@@ -122,6 +122,7 @@ struct xr_batch {
         int* bwd_end = nullptr;
         const float** segs = nullptr;  // volumes of the active problems (d_vmax order)
         int64_t* seg_len = nullptr;
+        int absmax_blocks = 1;  // blocks per segment of the max |voxel| launch
     };
     std::vector<Step> steps;
     bool tables_ok = false;
@@ -280,6 +281,7 @@ int build_tables(xr_batch* b) {
         s.n_fwd = (int)fi.size();
         s.n_active = (int)segs.size();
         if (s.n_fwd == 0) continue;
+        s.absmax_blocks = lens.empty() ? 1 : absmax_blocks(*std::max_element(lens.begin(), lens.end()), s.n_active);
         // backprojection items by z depth class (zpl 4, 2, 1): one launch and
         // one ticket range per class, so prefix sums restart in each class
         std::vector<BwdArgs> bi;
@@ -338,7 +340,7 @@ int enqueue_epochs(xr_batch* b, int epochs) {
             if (s.n_fwd == 0) continue;
             const int step = ep * b->maxb + pos;
             e = launch_fill(b->d_vmax, b->n, 0.0f, st);
-            if (e == hipSuccess) e = launch_absmax_seg(s.segs, s.seg_len, b->d_vmax, s.n_active, 64, st);
+            if (e == hipSuccess) e = launch_absmax_seg(s.segs, s.seg_len, b->d_vmax, s.n_active, s.absmax_blocks, st);
             if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "absmax failed: %s", hipGetErrorString(e));
             e = launch_forward_batch(s.fwd, s.fwd_end, s.n_fwd, b->d_queue, b->ticket_slot, b->fwd_lds, b->fwd_grid,
                                      step, ep * b->n, FWD_PLAIN, st);

@@ -148,6 +148,8 @@ int batch_blocks_per_cu_forward(int kind, size_t lds_bytes);
 int batch_blocks_per_cu_backward(int kind, int zpl);
 size_t backward_batch_lds_bytes();
 // out[i] = max |segs[i][0, seg_len[i])| (out zeroed beforehand).
+// blocks per segment for k_absmax_seg over n_seg segments of at most max_len floats
+int absmax_blocks(int64_t max_len, int n_seg);
 hipError_t launch_absmax_seg(const float* const* segs, const int64_t* seg_len, float* out, int n_seg,
                              int blocks_per_seg, hipStream_t stream);
 hipError_t launch_fdk_filter(const float* src, float* dst, const float* weight, const float* taps, int U, int V,

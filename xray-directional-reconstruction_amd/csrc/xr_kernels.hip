@@ -1028,17 +1028,44 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward_batch(const BwdArgs* _
     }
 }
 
-// Per-segment max |x|: segment i is x[seg_off[i], seg_off[i+1]), its maximum
-// goes to out[i] (zeroed beforehand); grid (blocks per segment, segments).
+// max |x| over blocks b, b + nb, ... of 256 threads: 16-byte loads, four in
+// flight per thread; scalar loads for an unaligned base and for the tail.
+__device__ __forceinline__ float absmax4(const float4 a) {
+    return fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w)));
+}
+
+__device__ __forceinline__ float absmax_range(const float* __restrict__ x, int64_t n, int64_t b, int64_t nb) {
+    float m = 0.f;
+    const int64_t n4 = (reinterpret_cast<uintptr_t>(x) & 15) == 0 ? n >> 2 : 0;
+    const float4* __restrict__ x4 = reinterpret_cast<const float4*>(x);
+    const int64_t stride = nb * 256;
+    int64_t i = b * 256 + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const float4 a0 = x4[i], a1 = x4[i + stride], a2 = x4[i + 2 * stride], a3 = x4[i + 3 * stride];
+        m = fmaxf(m, fmaxf(fmaxf(absmax4(a0), absmax4(a1)), fmaxf(absmax4(a2), absmax4(a3))));
+    }
+    for (; i < n4; i += stride) m = fmaxf(m, absmax4(x4[i]));
+    for (int64_t j = 4 * n4 + b * 256 + threadIdx.x; j < n; j += stride) m = fmaxf(m, fabsf(x[j]));
+    return m;
+}
+
+// Maximum over a block of 256 threads, valid in thread 0 (one atomic per
+// block: same-address atomics from thousands of waves serialise).
+__device__ __forceinline__ float block_max256(float m) {
+    __shared__ float part[4];
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    return fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+}
+
+// Per-segment max |x|: segment i is segs[i][0, seg_len[i]), its maximum goes
+// to out[i] (zeroed beforehand; non-negative floats order like their bit
+// patterns); grid (blocks per segment, segments).
 __global__ __launch_bounds__(256) void k_absmax_seg(const float* const* __restrict__ segs,
                                                     const int64_t* __restrict__ seg_len, float* out) {
-    const float* x = segs[blockIdx.y];
-    const int64_t n = seg_len[blockIdx.y];
-    float m = 0.f;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        m = fmaxf(m, fabsf(x[i]));
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax((unsigned int*)(out + blockIdx.y), __float_as_uint(m));
+    const float m = block_max256(absmax_range(segs[blockIdx.y], seg_len[blockIdx.y], blockIdx.x, gridDim.x));
+    if (threadIdx.x == 0) atomicMax((unsigned int*)(out + blockIdx.y), __float_as_uint(m));
 }
 
 // ---------------------------------------------------------------------------
@@ -1129,13 +1156,10 @@ __global__ void k_sqrt_abs(float* v, int64_t n) {
     v[i] = (m < 0.0f) ? sqrtf(-m) : sqrtf(m);  // voxelSqrt (ir.cu:1715)
 }
 
-// max |x| over n floats into out[0] (non-negative floats order like their bit patterns).
+// max |x| over n floats into out[0] (zeroed beforehand).
 __global__ __launch_bounds__(256) void k_absmax(const float* __restrict__ x, int64_t n, float* out) {
-    float m = 0.f;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        m = fmaxf(m, fabsf(x[i]));
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax((unsigned int*)out, __float_as_uint(m));
+    const float m = block_max256(absmax_range(x, n, blockIdx.x, gridDim.x));
+    if (threadIdx.x == 0) atomicMax((unsigned int*)out, __float_as_uint(m));
 }
 
 __global__ void k_fill(float* p, int64_t n, float value) {
@@ -1526,6 +1550,12 @@ int batch_blocks_per_cu_backward(int kind, int zpl) {
     return n;
 }
 
+int absmax_blocks(int64_t max_len, int n_seg) {
+    // at least 64 floats per thread in the largest segment; about 2048 blocks in all
+    const int64_t want = (max_len + 256 * 64 - 1) / (256 * 64);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(want, std::max(1, 2048 / std::max(n_seg, 1))));
+}
+
 hipError_t launch_absmax_seg(const float* const* segs, const int64_t* seg_len, float* out, int n_seg,
                              int blocks_per_seg, hipStream_t stream) {
     if (n_seg <= 0) return hipSuccess;
@@ -1578,8 +1608,7 @@ hipError_t launch_pca(const float* mu, int K, const Basis& b, int64_t nvox, int6
 
 hipError_t launch_absmax(const float* x, int64_t n, float* out, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_absmax, dim3((unsigned)blocks), dim3(256), 0, stream, x, n, out);
+    hipLaunchKernelGGL(k_absmax, dim3((unsigned)absmax_blocks(n, 1)), dim3(256), 0, stream, x, n, out);
     return hipGetLastError();
 }
 
