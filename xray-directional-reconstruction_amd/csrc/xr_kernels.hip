@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "xr_internal.hpp"
 #include "xr_random.hpp"
@@ -221,10 +222,12 @@ __device__ __forceinline__ FwdCol fwd_col(const FwdArgs& a, const ViewGeom& g, i
     f.dy = (KIND == FWD_XTT) ? fmaf(d.vs, yp, g.d0[1]) : 0.f;
     f.z_beg = 0;
     f.z_end = d.N;
-    if (AXIS) {
+    if (AXIS && (v0 > 0 || v0 + vb < d.V)) {
         // v(z') = hv + (nv + av2 z') inv is linear in z'; a voxel reaches rows
         // [v0, v0+vb) only if v in [v0 - 0.5, v0 + vb + 0.5).  The range is widened
         // by two voxels and every voxel is still tested exactly in fwd_splat.
+        // (A chunk of all rows keeps [0, N): the splat's window test rejects
+        // voxels that miss the detector.)
         const float slope = g.av[2] * f.inv;
         if (fabsf(slope) > 1e-6f) {
             const float za = ((float)v0 - 0.5f - d.hv - f.c.nv * f.inv) / slope;
@@ -284,25 +287,32 @@ __device__ __forceinline__ int round_half_up(float x) {
     return r;
 }
 
+// floor(x) as an integer in one instruction (x is far inside int range).
+__device__ __forceinline__ int floor_int(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // Untilted splat of one 64-z chunk of one column.  u, hence (iu, fu), is
 // constant along the column, so the weights factor into (1-fu, fu) x
 // (1-fv, fv) with wu0 = (1-fu)*s, wu1 = fu*s carrying the fixed-point scale,
-// and v is affine in z: t = v - 0.5 = B*z_lane + A (B, A per column and chunk
-// group).  The acceptance window 0.55 < v < V-0.55 and the chunk's row range
-// reduce to lo < t < hi.  Branch free: a rejected voxel adds zero.  cp = band
-// column iu's first row.
+// and v is affine in z: t = v - 0.5 = B*z_lane + A (B, A per column).  The
+// acceptance window 0.55 < v < V-0.55 and the chunk's row range reduce to
+// lo < t < hi (lo >= 0.05, so an accepted t is positive and fract(t) =
+// t - floor(t) exactly).  Branch free: a rejected voxel adds zero at `spare`,
+// a lane-dependent row of the band (one shared address would serialise the
+// wave's atomics).  row0 = the LDS byte address of band row floor(t) = 0 of
+// the column, so an accepted voxel's first row is row0 + floor(t) * row pitch.
 template <class AccT, class AccS>
-__device__ __forceinline__ void splat_untilted(float t, float val, lds_ptr<AccT> cp, float wu0, float wu1, float lo,
-                                               float hi, int v0, int spare_row) {
+__device__ __forceinline__ void splat_untilted(float t, float val, unsigned row0, lds_ptr<AccT> spare, float wu0,
+                                               float wu1, float lo, float hi) {
     const bool ok = t > lo && t < hi;
-    const float fl = floorf(t);
-    const float fv = t - fl;
+    const float fv = __builtin_amdgcn_fractf(t);
     const float w = ok ? val : 0.0f;
     const float g1 = fv * w, g0 = w - g1;
-    // a rejected voxel adds zero at a lane-dependent row (any row in the band:
-    // one shared address would serialise the wave's atomics)
-    const int row = ok ? (int)fl - v0 + 1 : spare_row;
-    lds_ptr<AccT> p = cp + row * kBandStride;
+    const unsigned at = __umul24((unsigned)floor_int(t), (unsigned)(kBandStride * sizeof(AccT))) + row0;
+    lds_ptr<AccT> p = ok ? (lds_ptr<AccT>)(uintptr_t)at : spare;
     const int q4 = round_half_up(wu0 * g0), q3 = round_half_up(wu1 * g0);
     const int q1 = round_half_up(wu0 * g1), q2 = round_half_up(wu1 * g1);
     __hip_atomic_fetch_add(p, (AccT)(AccS)q4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -434,61 +444,119 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                     const FwdCol f0 = fwd_col<AXIS, KIND>(a, g, x0 + b0, y, v0, vb);
                     const FwdCol f1 = fwd_col<AXIS, KIND>(a, g, x0 + b1, y, v0, vb);
                     const int zb = min(f0.z_beg, f1.z_beg), ze = max(f0.z_end, f1.z_end);
-                    // untilted: per-column u terms (uniform across the wave)
+                    // untilted: per-column u terms (uniform across the wave).  A candidate
+                    // column passed this same u test in the scan; should it fail here, its
+                    // weights are zero and it adds zeros inside the band.
                     lds_ptr<AccT> cp0 = acc, cp1 = acc;
                     float w00 = 0.f, w01 = 0.f, w10 = 0.f, w11 = 0.f;
-                    bool ok0 = false, ok1 = false;
                     if (AXIS) {
                         int iu;
                         float fu;
                         split(f0.u, iu, fu);
-                        ok0 = accept_u(f0.u, d) && iu >= u0 - 1 && iu < u0 + kBandU;
-                        cp0 = acc + (ok0 ? iu - u0 + 1 : 0);
-                        w00 = (1.0f - fu) * (scale * scalev);
-                        w01 = fu * (scale * scalev);
+                        bool ok = accept_u(f0.u, d) && iu >= u0 - 1 && iu < u0 + kBandU;
+                        cp0 = acc + (ok ? iu - u0 + 1 : 0);
+                        w00 = ok ? (1.0f - fu) * (scale * scalev) : 0.f;
+                        w01 = ok ? fu * (scale * scalev) : 0.f;
                         split(f1.u, iu, fu);
-                        ok1 = two && accept_u(f1.u, d) && iu >= u0 - 1 && iu < u0 + kBandU;
-                        cp1 = acc + (ok1 ? iu - u0 + 1 : 0);
-                        w10 = (1.0f - fu) * (scale * scalev);
-                        w11 = fu * (scale * scalev);
+                        ok = two && accept_u(f1.u, d) && iu >= u0 - 1 && iu < u0 + kBandU;
+                        cp1 = acc + (ok ? iu - u0 + 1 : 0);
+                        w10 = ok ? (1.0f - fu) * (scale * scalev) : 0.f;
+                        w11 = ok ? fu * (scale * scalev) : 0.f;
                     }
-                    // z chunks start at multiples of 64 so that a lane always holds the same
-                    // z residue: the untilted splat's lane pairing (hence its rounding) does
-                    // not depend on where the row chunk clips the column.
+                    if constexpr (AXIS) {
+                        // LDS byte address of band row floor(t) = 0 of each column, and each
+                        // lane's spare row for rejected voxels
+                        const unsigned row00 =
+                            (unsigned)(uintptr_t)cp0 + (unsigned)((1 - v0) * kBandStride * (int)sizeof(AccT));
+                        const unsigned row01 =
+                            (unsigned)(uintptr_t)cp1 + (unsigned)((1 - v0) * kBandStride * (int)sizeof(AccT));
+                        const lds_ptr<AccT> spare0 = cp0 + spare_row * kBandStride;
+                        const lds_ptr<AccT> spare1 = cp1 + spare_row * kBandStride;
+                        // t = v - 0.5 = B*(z - xc) + (nv*inv + hv - 0.5); z - xc is exact in
+                        // FP32 (half-integers below 2^23), so t depends on z alone and not on
+                        // where the chunk group starts
+                        const float B0 = g.av[2] * f0.inv, B1 = g.av[2] * f1.inv;
+                        const float A0 = fmaf(f0.c.nv, f0.inv, d.hv - 0.5f);
+                        const float A1 = fmaf(f1.c.nv, f1.inv, d.hv - 0.5f);
+                        // Whole columns (both z ranges are [0, N), N a multiple of 64): every
+                        // loaded z is in range, so the loads need no masks and share one
+                        // address per column.  Wave-uniform.
+                        const bool whole = KIND != FWD_WEIGHT && (d.N & 63) == 0 && f0.z_beg == 0 &&
+                                           f0.z_end == d.N && f1.z_beg == 0 && f1.z_end == d.N;
+                        const auto col0 = vol + f0.base + lane, col1 = vol + f1.base + lane;
+                        // one or two columns, whole or clipped: template constants, so that no
+                        // per-chunk test of either remains
+                        auto columns = [&](auto two_c, auto whole_c) {
+                            constexpr bool TWO = decltype(two_c)::value;
+                            constexpr bool WHOLE = decltype(whole_c)::value;
+                            // z chunks start at multiples of 64 so that a lane always holds the
+                            // same z residue: the splat's lane pairing (hence its rounding) does
+                            // not depend on where the row chunk clips the column.
+                            for (int zg = zb & ~63; zg < ze; zg += 64 * kFwdChunks) {
+                                float m0[kFwdChunks], m1[kFwdChunks];
+                                if (WHOLE && zg + 64 * kFwdChunks <= d.N) {
+                                    // all chunks in range: plain loads at immediate offsets
+#pragma unroll
+                                    for (int c = 0; c < kFwdChunks; c++) {
+                                        m0[c] = col0[zg + c * 64];
+                                        if (TWO) m1[c] = col1[zg + c * 64];
+                                    }
+                                } else {
+#pragma unroll
+                                for (int c = 0; c < kFwdChunks; c++) {
+                                    const int z = zg + c * 64 + lane;
+                                    if (WHOLE) {
+                                        const bool in = zg + c * 64 < d.N;
+                                        m0[c] = in ? col0[zg + c * 64] : 0.f;
+                                        if (TWO) m1[c] = in ? col1[zg + c * 64] : 0.f;
+                                    } else if (KIND == FWD_WEIGHT) {
+                                        // zero outside the column's z range: the splat has no z test
+                                        m0[c] = (z < f0.z_end && z >= f0.z_beg) ? 1.f : 0.f;
+                                        if (TWO) m1[c] = (z < f1.z_end && z >= f1.z_beg) ? 1.f : 0.f;
+                                    } else {
+                                        m0[c] = (z < f0.z_end && z >= f0.z_beg) ? vol[f0.base + z] : 0.f;
+                                        if (TWO) m1[c] = (z < f1.z_end && z >= f1.z_beg) ? vol[f1.base + z] : 0.f;
+                                    }
+                                }
+                                }
+                                const float zoff = (float)zg - d.xc;
+#pragma unroll
+                                for (int c = 0; c < kFwdChunks; c++) {
+                                    if (zg + c * 64 >= ze) break;
+                                    const float zp = lanef[c] + zoff;
+                                    splat_untilted<AccT, AccS>(fmaf(B0, zp, A0), m0[c], row00, spare0, w00, w01, t_lo,
+                                                               t_hi);
+                                    if (TWO)
+                                        splat_untilted<AccT, AccS>(fmaf(B1, zp, A1), m1[c], row01, spare1, w10, w11,
+                                                                   t_lo, t_hi);
+                                }
+                            }
+                        };
+                        if (two) {
+                            if (whole)
+                                columns(std::true_type(), std::true_type());
+                            else
+                                columns(std::true_type(), std::false_type());
+                        } else {
+                            if (whole)
+                                columns(std::false_type(), std::true_type());
+                            else
+                                columns(std::false_type(), std::false_type());
+                        }
+                        continue;
+                    }
                     for (int zg = zb & ~63; zg < ze; zg += 64 * kFwdChunks) {
                         float m0[kFwdChunks], m1[kFwdChunks];
 #pragma unroll
                         for (int c = 0; c < kFwdChunks; c++) {
                             const int z = zg + c * 64 + lane;
                             if (KIND == FWD_WEIGHT) {
-                                // zero outside the column's z range: the untilted splat has no z test
                                 m0[c] = (z < f0.z_end && z >= f0.z_beg) ? 1.f : 0.f;
                                 m1[c] = (two && z < f1.z_end && z >= f1.z_beg) ? 1.f : 0.f;
                             } else {
                                 m0[c] = (z < f0.z_end && z >= f0.z_beg) ? vol[f0.base + z] : 0.f;
                                 m1[c] = (two && z < f1.z_end && z >= f1.z_beg) ? vol[f1.base + z] : 0.f;
                             }
-                        }
-                        if (AXIS) {
-                            // t = v - 0.5 = B*(z - xc) + (nv*inv + hv - 0.5); z - xc is exact in
-                            // FP32 (half-integers below 2^23), so t depends on z alone and not
-                            // on where the chunk group starts
-                            const float zoff = (float)zg - d.xc;
-                            const float B0 = g.av[2] * f0.inv, B1 = g.av[2] * f1.inv;
-                            const float A0 = fmaf(f0.c.nv, f0.inv, d.hv - 0.5f);
-                            const float A1 = fmaf(f1.c.nv, f1.inv, d.hv - 0.5f);
-#pragma unroll
-                            for (int c = 0; c < kFwdChunks; c++) {
-                                if (zg + c * 64 >= ze) break;
-                                const float zp = lanef[c] + zoff;
-                                if (ok0)
-                                    splat_untilted<AccT, AccS>(fmaf(B0, zp, A0), m0[c], cp0, w00, w01, t_lo, t_hi, v0,
-                                                               spare_row);
-                                if (ok1)
-                                    splat_untilted<AccT, AccS>(fmaf(B1, zp, A1), m1[c], cp1, w10, w11, t_lo, t_hi, v0,
-                                                               spare_row);
-                            }
-                            continue;
                         }
 #pragma unroll
                         for (int c = 0; c < kFwdChunks; c++) {
