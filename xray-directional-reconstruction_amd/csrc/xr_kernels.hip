@@ -812,6 +812,12 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
                 tmp[c][q][k] = 0.f;
                 fac[c][q][k] = 0.f;
             }
+    // untilted: views that accept the whole z extent of a column (counted once
+    // per column, added to the per-voxel counts at the end; all integers, so
+    // the sums are exact in any order)
+    int nwhole[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) nwhole[c] = 0;
 
     // staging of one view's footprint: prefetch into registers, store later
     float pre[kStagePer];
@@ -922,6 +928,45 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
                         const float Ac = fmaf(c.nv, inv_c, d.hv - 0.5f);
                         const float hi = accept_u(u_c, d) ? d.vmax - 0.5f : -1.0f;
                         const int lu = min(max(iu - f.iu_lo, 0), f.W - 2);
+                        if (f.staged) {
+                            // Whole extent accepted: t at the tile's first and last z bound
+                            // every lane's t (fmaf is monotone in z), so one wave-uniform
+                            // test replaces the per-voxel window test and the select.
+                            const float ta = fmaf(Bc, (float)z0 - d.xc, Ac);
+                            const float tb = fmaf(Bc, (float)(z0 + 64 * ZPL - 1) - d.xc, Ac);
+                            if (uniformi(fminf(ta, tb) > 0.05f && fmaxf(ta, tb) < hi)) {
+                                // LDS byte address of texel (iu, iv): iv*Ws*4 + base
+                                const unsigned ws4 = (unsigned)f.Ws * 4u;
+                                const unsigned base =
+                                    (unsigned)(uintptr_t)cur + (unsigned)((lu - f.iv_lo * f.Ws) * 4);
+                                // all reads of the column first (LDS latency), then the sums
+                                float fv[ZPL], r[ZPL][4];
+#pragma unroll
+                                for (int q = 0; q < ZPL; q++) {
+                                    const float t = fmaf(Bc, zpq[q], Ac);
+                                    fv[q] = __builtin_amdgcn_fractf(t);
+                                    const unsigned at = __umul24((unsigned)floor_int(t), ws4) + base;
+                                    const lds_float* p0 = (const lds_float*)(uintptr_t)at;
+                                    const lds_float* p1 = (const lds_float*)(uintptr_t)(at + ws4);
+                                    r[q][0] = p0[0];
+                                    r[q][1] = p0[1];
+                                    r[q][2] = p1[0];
+                                    r[q][3] = p1[1];
+                                }
+#pragma unroll
+                                for (int q = 0; q < ZPL; q++) {
+                                    const float top = fmaf(fu, r[q][1], a0 * r[q][0]);
+                                    const float bot = fmaf(fu, r[q][3], a0 * r[q][2]);
+                                    const float nb = fmaf(fv[q], bot - top, top);
+                                    if (KIND == BWD_FDK)
+                                        tmp[cidx][q][0] = fmaf(wcol, nb, tmp[cidx][q][0]);
+                                    else
+                                        tmp[cidx][q][0] += nb;
+                                }
+                                if (KIND != BWD_FDK) nwhole[cidx]++;
+                                continue;
+                            }
+                        }
 #pragma unroll
                         for (int q = 0; q < ZPL; q++) {
                             const float t = fmaf(Bc, zpq[q], Ac);
@@ -1042,7 +1087,7 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
                 for (int k = 0; k < KA; k++) {
                     if (XTT && k >= d.K) continue;
                     const int64_t vi = idx + (int64_t)k * a.chan_stride;
-                    const float t = tmp[cidx][q][k], fc = fac[cidx][q][k];
+                    const float t = tmp[cidx][q][k], fc = fac[cidx][q][k] + (float)nwhole[cidx];
                     if (KIND == BWD_RAW || KIND == BWD_XTT_RAW) {
                         to_global(a.tmp)[vi] += t;
                         to_global(a.factor)[vi] += fc;
