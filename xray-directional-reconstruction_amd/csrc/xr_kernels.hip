@@ -740,6 +740,8 @@ struct Foot {
     bool staged, empty;
 };
 
+__device__ __forceinline__ Foot foot_from_extent(float umn, float umx, float vmn, float vmx, int bad, const Dims& d);
+
 // Footprint of the block box on view g: the projective map's extremes over a box
 // are at its corners while den > 0 there (all lanes compute it identically).
 __device__ __forceinline__ Foot footprint(const ViewGeom& g, const Dims& d, float bx0, float bx1, float by0, float by1,
@@ -761,6 +763,12 @@ __device__ __forceinline__ Foot footprint(const ViewGeom& g, const Dims& d, floa
         vmx = fmaxf(vmx, __shfl_xor(vmx, off));
         bad |= __shfl_xor(bad, off);
     }
+    return foot_from_extent(umn, umx, vmn, vmx, bad, d);
+}
+
+// Footprint from the extremes of the projected corners (bad: a corner behind
+// the source or far off the detector, so the extremes bound nothing).
+__device__ __forceinline__ Foot foot_from_extent(float umn, float umx, float vmn, float vmx, int bad, const Dims& d) {
     Foot f{0, 0, 0, 0, 0, 0, false, false};
     if (bad) return f;  // direct (unstaged) reads
     f.iu_lo = max((int)floorf(umn - 0.5f) - 1, 0);
@@ -781,6 +789,356 @@ __device__ __forceinline__ Foot footprint(const ViewGeom& g, const Dims& d, floa
     return f;
 }
 
+// Fused update of a backprojection tile: every voxel of the lane's columns is
+// read and written once (raw accumulate, MLEM ratio, ART step or FDK sum).
+template <int KIND, int CPW, int ZPL, int KA>
+__device__ __forceinline__ void backward_update(const BwdArgs& a, const float (&tmp)[CPW * CPW][ZPL][KA],
+                                                const float (&fac)[CPW * CPW][ZPL][KA], const int (&nwhole)[CPW * CPW],
+                                                int xb, int yb, int z0, int lane) {
+    constexpr bool XTT = IsXtt<KIND>::value;
+    const Dims& d = a.d;
+    const auto vol = to_global(a.vol);
+#pragma unroll
+    for (int j = 0; j < CPW; j++) {
+#pragma unroll
+        for (int i = 0; i < CPW; i++) {
+            const int x = xb + i, y = yb + j;
+            if (x >= d.N || y >= d.N) continue;
+            const int cidx = j * CPW + i;
+#pragma unroll
+            for (int q = 0; q < ZPL; q++) {
+                const int z = z0 + lane + 64 * q;
+                if (z >= d.N) continue;
+                const int64_t idx = ((int64_t)y * d.N + x) * d.N + z;
+#pragma unroll
+                for (int k = 0; k < KA; k++) {
+                    if (XTT && k >= d.K) continue;
+                    const int64_t vi = idx + (int64_t)k * a.chan_stride;
+                    const float t = tmp[cidx][q][k], fc = fac[cidx][q][k] + (float)nwhole[cidx];
+                    if (KIND == BWD_RAW || KIND == BWD_XTT_RAW) {
+                        to_global(a.tmp)[vi] += t;
+                        to_global(a.factor)[vi] += fc;
+                    } else if (KIND == BWD_MLEM || KIND == BWD_XTT_MLEM) {
+                        const float mu = vol[vi];
+                        vol[vi] = (fc == 0.0f) ? 0.0f : mu * t / fc;
+                    } else if (KIND == BWD_ART || KIND == BWD_XTT_ART) {
+                        vol[vi] = vol[vi] + a.alpha * t;
+                    } else {  // BWD_FDK
+                        vol[vi] = a.overwrite ? t : vol[vi] + t;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Untilted backprojection of the plain kinds (raw, MLEM, ART, FDK).  The
+// per-view quantities of a wave are the same in all its lanes: the block
+// footprint (8 projected corners) and, for each of the wave's 4 columns, the
+// column's u split, the affine t(z) = v - 0.5 and the acceptance window.  They
+// are computed lane-parallel, 16 views at a time, into LDS: the footprints of
+// a batch by wave 0 (lane = view x corner pair, published by the per-view
+// barrier two views before the batch starts), the column terms by every wave
+// for its own columns (lane = view x column).  The view loop reads them back
+// with broadcast LDS loads.  Arithmetic and accumulation order are those of
+// the general path, so results are bitwise the same.
+// ---------------------------------------------------------------------------
+constexpr int kBwdBatch = 16;    // views per batch of per-view terms
+constexpr int kFootWords = 8;    // iu_lo, iv_lo, W, H, Ws, R, staged | empty << 1, -
+constexpr int kTermWords = 8;    // Bc, Ac, fu, hi (+inf: whole z extent accepted), wcol, lu, iu, base
+constexpr int kTermCols = 4;     // columns per wave (CPW = 2)
+constexpr int kBwdFootArea = 2 * kBwdBatch * kFootWords;          // words, block-wide, two batches
+constexpr int kBwdTermArea = kBwdBatch * kTermCols * kTermWords;  // words per wave
+constexpr int kBwdExtraWords = kBwdFootArea + kBwdWaves * kBwdTermArea;
+static_assert(64 % kTermCols == 0 && 64 / kTermCols >= kBwdBatch, "one lane per (view, column) of a batch");
+
+typedef __attribute__((address_space(3))) int lds_int;
+
+template <int KIND, int ZPL>
+__device__ __forceinline__ void backward_item_untilted(const BwdArgs& a, int o, int step, float* stage) {
+    constexpr int CPW = 2, NC = CPW * CPW;
+    static_assert(NC == kTermCols, "term layout");
+    const Dims& d = a.d;
+    const auto views = to_global(a.views + (int64_t)step * a.n_views);
+    const auto vg = to_global(a.vg);
+    const int gx = a.gx;
+    const int bcx = CPW * gx, bcy = CPW * a.gy;  // block columns along x, y
+    const int tx = o % a.tiles_x, ty = (o / a.tiles_x) % a.tiles_y, tz = o / (a.tiles_x * a.tiles_y);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int xb = tx * bcx + (wave % gx) * CPW, yb = ty * bcy + (wave / gx) * CPW;
+    const int z0 = tz * 64 * ZPL;
+    const float bx0 = (float)(tx * bcx) - d.xc, bx1 = (float)min(tx * bcx + bcx - 1, d.N - 1) - d.xc;
+    const float by0 = (float)(ty * bcy) - d.xc, by1 = (float)min(ty * bcy + bcy - 1, d.N - 1) - d.xc;
+    const float bz0 = (float)z0 - d.xc, bz1 = (float)min(z0 + 64 * ZPL - 1, d.N - 1) - d.xc;
+    const int nv = a.n_views;
+    if (nv == 0) return;
+
+    float tmp[NC][ZPL][1], fac[NC][ZPL][1];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int q = 0; q < ZPL; q++) {
+            tmp[c][q][0] = 0.f;
+            fac[c][q][0] = 0.f;
+        }
+    int nwhole[NC];  // views accepting a column's whole z extent (added to the counts at the end)
+#pragma unroll
+    for (int c = 0; c < NC; c++) nwhole[c] = 0;
+    float zpq[ZPL];
+#pragma unroll
+    for (int q = 0; q < ZPL; q++) zpq[q] = (float)(z0 + lane + 64 * q) - d.xc;
+
+    lds_float* const lds = (lds_float*)stage;
+    lds_int* const foot = (lds_int*)(stage + 2 * kStageBuf);
+    lds_float* const term = (lds_float*)(stage + 2 * kStageBuf + kBwdFootArea + wave * kBwdTermArea);
+
+    // footprints of views b0 .. b0+15 into buffer buf: lane = 4 x view + corner
+    // pair (x, y corner; both z corners), extremes over the 4 lanes of a view
+    auto foot_batch = [&](int b0, int buf) {
+        const int j = lane >> 2, c = lane & 3;
+        const ViewGeom g = vg[views[min(b0 + j, nv - 1)]];
+        const ColForms cf = col_forms(g, (c & 1) ? bx1 : bx0, (c & 2) ? by1 : by0);
+        float umn = 0.f, umx = 0.f, vmn = 0.f, vmx = 0.f;
+        int bad = 0;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const float czp = k ? bz1 : bz0;
+            const float cd = fmaf(g.ad[2], czp, cf.nd);
+            float cu, cv;
+            voxel_uv(g, cf, czp, d, cu, cv);
+            umn = k ? fminf(umn, cu) : cu;
+            umx = k ? fmaxf(umx, cu) : cu;
+            vmn = k ? fminf(vmn, cv) : cv;
+            vmx = k ? fmaxf(vmx, cv) : cv;
+            bad |= (!(cd > 0.f) || !(fabsf(cu) < 1e7f) || !(fabsf(cv) < 1e7f)) ? 1 : 0;
+        }
+        for (int off = 1; off < 4; off <<= 1) {
+            umn = fminf(umn, __shfl_xor(umn, off));
+            umx = fmaxf(umx, __shfl_xor(umx, off));
+            vmn = fminf(vmn, __shfl_xor(vmn, off));
+            vmx = fmaxf(vmx, __shfl_xor(vmx, off));
+            bad |= __shfl_xor(bad, off);
+        }
+        const Foot f = foot_from_extent(umn, umx, vmn, vmx, bad, d);
+        if (c == 0) {
+            lds_int* p = foot + (buf * kBwdBatch + j) * kFootWords;
+            p[0] = f.iu_lo;
+            p[1] = f.iv_lo;
+            p[2] = f.W;
+            p[3] = f.H;
+            p[4] = f.Ws;
+            p[5] = f.R;
+            p[6] = (f.staged ? 1 : 0) | (f.empty ? 2 : 0);
+        }
+    };
+    auto load_foot = [&](int buf, int j) {
+        const lds_int* p = foot + (buf * kBwdBatch + j) * kFootWords;
+        Foot f;
+        f.iu_lo = uniformi(p[0]);
+        f.iv_lo = uniformi(p[1]);
+        f.W = uniformi(p[2]);
+        f.H = uniformi(p[3]);
+        f.Ws = uniformi(p[4]);
+        f.R = uniformi(p[5]);
+        const int fl = uniformi(p[6]);
+        f.staged = (fl & 1) != 0;
+        f.empty = (fl & 2) != 0;
+        return f;
+    };
+    // column terms of views b0 .. b0+15 for this wave's columns: lane = 4 x view + column
+    auto term_batch = [&](int b0, int buf) {
+        const int j = lane >> 2, cidx = lane & 3;
+        const ViewGeom g = vg[views[min(b0 + j, nv - 1)]];
+        const lds_int* fp = foot + (buf * kBwdBatch + j) * kFootWords;
+        const int iu_lo = fp[0], iv_lo = fp[1], W = fp[2], Ws = fp[4], fl = fp[6];
+        const float xp = (float)(xb + (cidx & 1)) - d.xc, yp = (float)(yb + (cidx >> 1)) - d.xc;
+        const ColForms c = col_forms(g, xp, yp);
+        const float inv_c = rcpf(c.nd);
+        const float u_c = fmaf(c.nu, inv_c, d.hu);
+        int iu;
+        float fu;
+        split(u_c, iu, fu);
+        const float Bc = g.av[2] * inv_c;
+        const float Ac = fmaf(c.nv, inv_c, d.hv - 0.5f);
+        const float hi = accept_u(u_c, d) ? d.vmax - 0.5f : -1.0f;
+        const int lu = min(max(iu - iu_lo, 0), W - 2);
+        float wcol = 0.f;
+        if (KIND == BWD_FDK) {
+            // vol += Ud^2 * (2 pi/P) * sum c p, Ud = sod/(sod + y_unrotated) (fdk.cu:160-169)
+            const float Xy = fmaf(d.vs, yp, g.fy);
+            const float Ud = d.sod / (d.sod + Xy);
+            wcol = Ud * Ud * d.cfdk;
+        }
+        // whole extent accepted: t at the tile's first and last z bound every
+        // lane's t (fmaf is monotone in z)
+        const float ta = fmaf(Bc, bz0, Ac);
+        const float tb = fmaf(Bc, (float)(z0 + 64 * ZPL - 1) - d.xc, Ac);
+        const bool whole = (fl & 1) && fminf(ta, tb) > 0.05f && fmaxf(ta, tb) < hi;
+        lds_float* tp = term + (j * kTermCols + cidx) * kTermWords;
+        lds_int* ti = (lds_int*)tp;
+        tp[0] = Bc;
+        tp[1] = Ac;
+        tp[2] = fu;
+        tp[3] = whole ? __int_as_float(0x7f800000) : hi;
+        tp[4] = wcol;
+        ti[5] = lu;
+        ti[6] = iu;
+        ti[7] = (lu - iv_lo * Ws) * 4;  // LDS byte offset of texel (iu, 0) in a staging buffer
+    };
+    // a wave's own LDS writes, then its lanes read each other's words
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+
+    // staging of one view's footprint: prefetch into registers, store later
+    float pre[kStagePer];
+    int pidx[kStagePer];
+    auto fetch = [&](const Foot& f, int code) {
+        const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
+        int r0 = (int)((float)threadIdx.x * (1.0f / (float)f.W));
+        int c = (int)threadIdx.x - r0 * f.W;
+        if (c < 0) {
+            r0--;
+            c += f.W;
+        } else if (c >= f.W) {
+            r0++;
+            c -= f.W;
+        }
+        const bool act = r0 < f.R;
+        const auto src = img + (int64_t)(f.iv_lo + r0) * d.U + f.iu_lo + c;
+        const int64_t rstep = (int64_t)f.R * d.U;
+#pragma unroll
+        for (int k = 0; k < kStagePer; k++) {
+            const int r = r0 + k * f.R;
+            const bool in = act && r < f.H;
+            pidx[k] = in ? r * f.Ws + c : -1;
+            pre[k] = in ? src[k * rstep] : 0.f;
+        }
+    };
+    auto store = [&](lds_float* buf) {
+#pragma unroll
+        for (int k = 0; k < kStagePer; k++)
+            if (pidx[k] >= 0) buf[pidx[k]] = pre[k];
+    };
+
+    if (wave == 0) foot_batch(0, 0);
+    __syncthreads();
+    term_batch(0, 0);
+    int code = views[0];
+    Foot f = load_foot(0, 0);
+    if (f.staged) {
+        fetch(f, code);
+        store(lds);
+    }
+    __syncthreads();
+
+    for (int s = 0; s < nv; s++) {
+        const int j = s & (kBwdBatch - 1);
+        if (j == 0 && s > 0) term_batch(s, (s / kBwdBatch) & 1);
+        wave_sync();
+        // the next batch's footprints, read from the next iteration on
+        if (wave == 0 && j == kBwdBatch - 2 && s + 2 < nv) foot_batch(s + 2, ((s + 2) / kBwdBatch) & 1);
+        lds_float* cur = lds + (s & 1) * kStageBuf;
+        lds_float* nxt = lds + ((s + 1) & 1) * kStageBuf;
+        int ncode = 0;
+        Foot nf{0, 0, 0, 0, 0, 0, false, false};
+        if (s + 1 < nv) {
+            ncode = views[s + 1];
+            nf = load_foot(((s + 1) / kBwdBatch) & 1, (s + 1) & (kBwdBatch - 1));
+            if (nf.staged) fetch(nf, ncode);
+        }
+        if (!f.empty) {
+            const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
+#pragma unroll
+            for (int cidx = 0; cidx < NC; cidx++) {
+                const int x = xb + (cidx & 1), y = yb + (cidx >> 1);
+                if (x >= d.N || y >= d.N) continue;
+                const lds_float* tp = term + (j * kTermCols + cidx) * kTermWords;
+                const lds_int* ti = (const lds_int*)tp;
+                const float Bc = tp[0], Ac = tp[1], fu = tp[2], hi = tp[3], wcol = tp[4];
+                const int lu = ti[5], iu = ti[6], boff = ti[7];
+                const float a0 = 1.0f - fu;
+                if (uniformi(__float_as_int(hi)) == 0x7f800000) {
+                    // all reads of the column first (LDS latency), then the sums
+                    const unsigned ws4 = (unsigned)f.Ws * 4u;
+                    const unsigned base = (unsigned)(uintptr_t)cur + (unsigned)boff;
+                    float fv[ZPL], r[ZPL][4];
+#pragma unroll
+                    for (int q = 0; q < ZPL; q++) {
+                        const float t = fmaf(Bc, zpq[q], Ac);
+                        fv[q] = __builtin_amdgcn_fractf(t);
+                        const unsigned at = __umul24((unsigned)floor_int(t), ws4) + base;
+                        const lds_float* p0 = (const lds_float*)(uintptr_t)at;
+                        const lds_float* p1 = (const lds_float*)(uintptr_t)(at + ws4);
+                        r[q][0] = p0[0];
+                        r[q][1] = p0[1];
+                        r[q][2] = p1[0];
+                        r[q][3] = p1[1];
+                    }
+#pragma unroll
+                    for (int q = 0; q < ZPL; q++) {
+                        const float top = fmaf(fu, r[q][1], a0 * r[q][0]);
+                        const float bot = fmaf(fu, r[q][3], a0 * r[q][2]);
+                        const float nb = fmaf(fv[q], bot - top, top);
+                        if (KIND == BWD_FDK)
+                            tmp[cidx][q][0] = fmaf(wcol, nb, tmp[cidx][q][0]);
+                        else
+                            tmp[cidx][q][0] += nb;
+                    }
+                    if (KIND != BWD_FDK) nwhole[cidx]++;
+                    continue;
+                }
+                // Window test per voxel: 0.55 < v < V - 0.55 becomes 0.05 < t < hi
+                // (hi < 0 rejects the column).  Branch free: a rejected voxel reads
+                // an in-bounds texel and adds zero.  Lanes with z >= N accumulate
+                // but are never stored.
+#pragma unroll
+                for (int q = 0; q < ZPL; q++) {
+                    const float t = fmaf(Bc, zpq[q], Ac);
+                    const bool ok = t > 0.05f && t < hi;
+                    const float fl = floorf(t);
+                    const float fv = t - fl;
+                    const int iv = (int)fl;
+                    float r00, r01, r10, r11;
+                    if (f.staged) {
+                        const int qi = (ok ? iv - f.iv_lo : 0) * f.Ws + lu;
+                        r00 = cur[qi];
+                        r01 = cur[qi + 1];
+                        r10 = cur[qi + f.Ws];
+                        r11 = cur[qi + f.Ws + 1];
+                    } else {
+                        // direct reads: a distinct (nontemporal) load form next to the LDS reads
+                        const int64_t qi = (int64_t)min(max(iv, 0), d.V - 2) * d.U + min(max(iu, 0), d.U - 2);
+                        r00 = __builtin_nontemporal_load(img + qi);
+                        r01 = __builtin_nontemporal_load(img + qi + 1);
+                        r10 = __builtin_nontemporal_load(img + qi + d.U);
+                        r11 = __builtin_nontemporal_load(img + qi + d.U + 1);
+                    }
+                    const float top = fmaf(fu, r01, a0 * r00);
+                    const float bot = fmaf(fu, r11, a0 * r10);
+                    const float nb = ok ? fmaf(fv, bot - top, top) : 0.0f;
+                    if (KIND == BWD_FDK) {
+                        tmp[cidx][q][0] = fmaf(wcol, nb, tmp[cidx][q][0]);
+                    } else {
+                        tmp[cidx][q][0] += nb;
+                        fac[cidx][q][0] += ok ? 1.0f : 0.0f;
+                    }
+                }
+            }
+        }
+        if (s + 1 < nv) {
+            if (nf.staged) store(nxt);
+            code = ncode;
+            f = nf;
+        }
+        __syncthreads();
+    }
+    backward_update<KIND, CPW, ZPL, 1>(a, tmp, fac, nwhole, xb, yb, z0, lane);
+}
+
 // One voxel tile o of a backprojection launch; step selects the view list
 // views + step*n_views (a solver's subset step).  stage: 2 x kStageBuf floats.
 template <int KIND, bool AXIS, int CPW, int ZPL, int KM>
@@ -788,6 +1146,10 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
     constexpr bool XTT = IsXtt<KIND>::value;
     constexpr int NC = CPW * CPW;
     constexpr int KA = XTT ? KM : 1;
+    if constexpr (AXIS && !XTT && CPW == 2) {
+        backward_item_untilted<KIND, ZPL>(a, o, step, stage);
+        return;
+    }
     const Dims& d = a.d;
     const auto views = to_global(a.views + (int64_t)step * a.n_views);
     const auto vg = to_global(a.vg);
@@ -1071,38 +1433,7 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
         }
         __syncthreads();
     }
-#pragma unroll
-    for (int j = 0; j < CPW; j++) {
-#pragma unroll
-        for (int i = 0; i < CPW; i++) {
-            const int x = xb + i, y = yb + j;
-            if (x >= d.N || y >= d.N) continue;
-            const int cidx = j * CPW + i;
-#pragma unroll
-            for (int q = 0; q < ZPL; q++) {
-                const int z = z0 + lane + 64 * q;
-                if (z >= d.N) continue;
-                const int64_t idx = ((int64_t)y * d.N + x) * d.N + z;
-#pragma unroll
-                for (int k = 0; k < KA; k++) {
-                    if (XTT && k >= d.K) continue;
-                    const int64_t vi = idx + (int64_t)k * a.chan_stride;
-                    const float t = tmp[cidx][q][k], fc = fac[cidx][q][k] + (float)nwhole[cidx];
-                    if (KIND == BWD_RAW || KIND == BWD_XTT_RAW) {
-                        to_global(a.tmp)[vi] += t;
-                        to_global(a.factor)[vi] += fc;
-                    } else if (KIND == BWD_MLEM || KIND == BWD_XTT_MLEM) {
-                        const float mu = vol[vi];
-                        vol[vi] = (fc == 0.0f) ? 0.0f : mu * t / fc;
-                    } else if (KIND == BWD_ART || KIND == BWD_XTT_ART) {
-                        vol[vi] = vol[vi] + a.alpha * t;
-                    } else {  // BWD_FDK
-                        vol[vi] = a.overwrite ? t : vol[vi] + t;
-                    }
-                }
-            }
-        }
-    }
+    backward_update<KIND, CPW, ZPL, KA>(a, tmp, fac, nwhole, xb, yb, z0, lane);
 }
 
 template <int KIND, bool AXIS, int CPW, int ZPL, int KM>
@@ -1559,7 +1890,8 @@ hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream) {
     if (!backward_shape(a, a.zpl, a.gx, xtt)) return hipErrorInvalidValue;
     const int zpl = a.zpl;
     const unsigned grid = (unsigned)(8 * a.per_xcd);
-    const size_t lds = 2 * kStageBuf * sizeof(float);
+    // plain kinds: staging buffers + per-view term areas; XTT: staging buffers only
+    const size_t lds = (2 * kStageBuf + (xtt ? 0 : kBwdExtraWords)) * sizeof(float);
 #define XR_BWD(KD, A, C, Z, KM) \
     hipLaunchKernelGGL((k_backward<KD, A, C, Z, KM>), dim3(grid), dim3(kBwdThreads), lds, stream, a)
 #define XR_BWD_AX(KD, C, Z, KM)         \
@@ -1643,7 +1975,7 @@ hipError_t launch_backward_batch(const BwdArgs* items, const int* item_end, int 
     return hipLaunchKernel(k, dim3(grid), dim3(kBwdThreads), args, backward_batch_lds_bytes(), stream);
 }
 
-size_t backward_batch_lds_bytes() { return 2 * kStageBuf * sizeof(float); }
+size_t backward_batch_lds_bytes() { return (2 * kStageBuf + kBwdExtraWords) * sizeof(float); }
 
 int batch_blocks_per_cu_forward(int kind, size_t lds_bytes) {
     int n = 0;
