@@ -209,6 +209,57 @@ struct FwdCol {
     int z_beg, z_end;  // z range that can reach this launch's rows (AXIS), else [0, N)
 };
 
+// Value of x in lane `l` (wave-uniform l), into a scalar register.
+__device__ __forceinline__ float lane_f(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ int lane_i(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+
+// z range of an untilted column that can reach detector rows [v0, v0+vb):
+// v(z') = hv + (nv + av2 z') inv is linear in z'; a voxel reaches those rows
+// only if v in [v0 - 0.5, v0 + vb + 0.5).  The range is widened by two voxels
+// and every voxel is still tested exactly in the splat.  A chunk of all rows
+// keeps [0, N): the splat's window test rejects voxels that miss the detector.
+__device__ __forceinline__ void col_zrange(const ViewGeom& g, const ColForms& c, float inv, const Dims& d, int v0,
+                                           int vb, int& z_beg, int& z_end) {
+    z_beg = 0;
+    z_end = d.N;
+    if (v0 <= 0 && v0 + vb >= d.V) return;
+    const float slope = g.av[2] * inv;
+    if (fabsf(slope) > 1e-6f) {
+        const float za = ((float)v0 - 0.5f - d.hv - c.nv * inv) / slope;
+        const float zb = ((float)(v0 + vb) + 0.5f - d.hv - c.nv * inv) / slope;
+        const float lo = fminf(za, zb) + d.xc - 2.0f, hi = fmaxf(za, zb) + d.xc + 2.0f;
+        z_beg = (int)fmaxf(0.0f, floorf(lo));
+        z_end = (int)fminf((float)d.N, ceilf(hi) + 1.0f);
+        if (!(lo == lo) || !(hi == hi)) {
+            z_beg = 0;
+            z_end = d.N;
+        }
+    }
+}
+
+// The same z range from t(z') = v - 0.5 = B z' + A, z' = z - xc: rows
+// [v0, v0+vb) need t in [v0 - 1, v0 + vb); widened by two voxels (the
+// rounding of A, B differs from the forms above by far less), tested exactly
+// in the splat.
+__device__ __forceinline__ void col_zrange_t(float B, float A, const Dims& d, int v0, int vb, int& z_beg,
+                                             int& z_end) {
+    z_beg = 0;
+    z_end = d.N;
+    if (fabsf(B) > 1e-6f) {
+        const float za = ((float)(v0 - 1) - A) / B;
+        const float zb = ((float)(v0 + vb) - A) / B;
+        const float lo = fminf(za, zb) + d.xc - 2.0f, hi = fmaxf(za, zb) + d.xc + 2.0f;
+        z_beg = (int)fmaxf(0.0f, floorf(lo));
+        z_end = (int)fminf((float)d.N, ceilf(hi) + 1.0f);
+        if (!(lo == lo) || !(hi == hi)) {
+            z_beg = 0;
+            z_end = d.N;
+        }
+    }
+}
+
 template <bool AXIS, int KIND>
 __device__ __forceinline__ FwdCol fwd_col(const FwdArgs& a, const ViewGeom& g, int xcol, int y, int v0, int vb) {
     const Dims& d = a.d;
@@ -222,25 +273,7 @@ __device__ __forceinline__ FwdCol fwd_col(const FwdArgs& a, const ViewGeom& g, i
     f.dy = (KIND == FWD_XTT) ? fmaf(d.vs, yp, g.d0[1]) : 0.f;
     f.z_beg = 0;
     f.z_end = d.N;
-    if (AXIS && (v0 > 0 || v0 + vb < d.V)) {
-        // v(z') = hv + (nv + av2 z') inv is linear in z'; a voxel reaches rows
-        // [v0, v0+vb) only if v in [v0 - 0.5, v0 + vb + 0.5).  The range is widened
-        // by two voxels and every voxel is still tested exactly in fwd_splat.
-        // (A chunk of all rows keeps [0, N): the splat's window test rejects
-        // voxels that miss the detector.)
-        const float slope = g.av[2] * f.inv;
-        if (fabsf(slope) > 1e-6f) {
-            const float za = ((float)v0 - 0.5f - d.hv - f.c.nv * f.inv) / slope;
-            const float zb = ((float)(v0 + vb) + 0.5f - d.hv - f.c.nv * f.inv) / slope;
-            const float lo = fminf(za, zb) + d.xc - 2.0f, hi = fmaxf(za, zb) + d.xc + 2.0f;
-            f.z_beg = (int)fmaxf(0.0f, floorf(lo));
-            f.z_end = (int)fminf((float)d.N, ceilf(hi) + 1.0f);
-            if (!(lo == lo) || !(hi == hi)) {
-                f.z_beg = 0;
-                f.z_end = d.N;
-            }
-        }
-    }
+    if (AXIS) col_zrange(g, f.c, f.inv, d, v0, vb, f.z_beg, f.z_end);
     f.z_beg = uniformi(f.z_beg);
     f.z_end = uniformi(f.z_end);
     return f;
@@ -370,7 +403,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const float scale = ldexpf(1.0f, S), unscale = ldexpf(1.0f, -S);
     __syncthreads();
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = uniformi(threadIdx.x >> 6);
     const auto vol = to_global(a.vol + (KIND == FWD_PLAIN ? (int64_t)cond * a.vol_cond_stride : 0));
     float g2[kMaxBasis];
     if (KIND == FWD_XTT) {
@@ -388,21 +421,33 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const float t_lo = (v0 >= 2) ? __int_as_float(__float_as_int((float)(v0 - 1)) - 1) : 0.05f;
     const float t_hi = fminf(d.vmax - 0.5f, (float)(v0 + vb));
     const int spare_row = min(lane, vb);  // rows spare_row, spare_row + 1 lie in the band
+    const bool rows_all = v0 <= 0 && v0 + vb >= d.V;  // this launch covers every detector row
 
     for (int y = wave; y < d.N; y += kFwdWaves) {
         const float yp = (float)y - d.xc;
         for (int x0 = 0; x0 < d.N; x0 += 64) {
             const int x = x0 + lane;
             bool cand = false;
+            // untilted: this lane's column terms, kept for the candidate loop (which
+            // reads a candidate's terms from its lane): t = v - 0.5 = B (z - xc) + A,
+            // the two u weights with the fixed-point scale, the band column, the z range
+            float sB = 0.f, sA = 0.f, sw0 = 0.f, sw1 = 0.f;
+            int scp = 0;
             if (x < d.N) {
                 const ColForms c = col_forms(g, (float)x - d.xc, yp);
                 if (AXIS) {
-                    const float u = fmaf(c.nu, rcpf(c.nd), d.hu);
+                    const float inv = rcpf(c.nd);
+                    const float u = fmaf(c.nu, inv, d.hu);
                     if (accept_u(u, d)) {
                         int iu;
                         float fu;
                         split(u, iu, fu);
                         cand = (iu >= u0 - 1) && (iu < u0 + kBandU);
+                        sB = g.av[2] * inv;
+                        sA = fmaf(c.nv, inv, d.hv - 0.5f);
+                        sw0 = (1.0f - fu) * (scale * scalev);
+                        sw1 = fu * (scale * scalev);
+                        scp = iu - u0 + 1;
                     }
                 } else {
                     // u is linear-fractional (monotone) along the column while den > 0.
@@ -441,29 +486,21 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                     const bool two = m != 0;
                     const int b1 = two ? __ffsll(m) - 1 : b0;
                     if (two) m &= m - 1;
-                    const FwdCol f0 = fwd_col<AXIS, KIND>(a, g, x0 + b0, y, v0, vb);
-                    const FwdCol f1 = fwd_col<AXIS, KIND>(a, g, x0 + b1, y, v0, vb);
-                    const int zb = min(f0.z_beg, f1.z_beg), ze = max(f0.z_end, f1.z_end);
-                    // untilted: per-column u terms (uniform across the wave).  A candidate
-                    // column passed this same u test in the scan; should it fail here, its
-                    // weights are zero and it adds zeros inside the band.
-                    lds_ptr<AccT> cp0 = acc, cp1 = acc;
-                    float w00 = 0.f, w01 = 0.f, w10 = 0.f, w11 = 0.f;
-                    if (AXIS) {
-                        int iu;
-                        float fu;
-                        split(f0.u, iu, fu);
-                        bool ok = accept_u(f0.u, d) && iu >= u0 - 1 && iu < u0 + kBandU;
-                        cp0 = acc + (ok ? iu - u0 + 1 : 0);
-                        w00 = ok ? (1.0f - fu) * (scale * scalev) : 0.f;
-                        w01 = ok ? fu * (scale * scalev) : 0.f;
-                        split(f1.u, iu, fu);
-                        ok = two && accept_u(f1.u, d) && iu >= u0 - 1 && iu < u0 + kBandU;
-                        cp1 = acc + (ok ? iu - u0 + 1 : 0);
-                        w10 = ok ? (1.0f - fu) * (scale * scalev) : 0.f;
-                        w11 = ok ? fu * (scale * scalev) : 0.f;
-                    }
                     if constexpr (AXIS) {
+                        // the candidates' terms, computed by their lanes in the scan
+                        const float B0 = lane_f(sB, b0), B1 = lane_f(sB, b1);
+                        const float A0 = lane_f(sA, b0), A1 = lane_f(sA, b1);
+                        const float w00 = lane_f(sw0, b0), w01 = lane_f(sw1, b0);
+                        const float w10 = lane_f(sw0, b1), w11 = lane_f(sw1, b1);
+                        const lds_ptr<AccT> cp0 = acc + lane_i(scp, b0), cp1 = acc + lane_i(scp, b1);
+                        int zb0 = 0, ze0 = d.N, zb1 = 0, ze1 = d.N;
+                        if (!rows_all) {
+                            col_zrange_t(B0, A0, d, v0, vb, zb0, ze0);
+                            col_zrange_t(B1, A1, d, v0, vb, zb1, ze1);
+                        }
+                        const int zb = min(zb0, zb1), ze = max(ze0, ze1);
+                        const int64_t base0 = ((int64_t)y * d.N + x0 + b0) * d.N;
+                        const int64_t base1 = ((int64_t)y * d.N + x0 + b1) * d.N;
                         // LDS byte address of band row floor(t) = 0 of each column, and each
                         // lane's spare row for rejected voxels
                         const unsigned row00 =
@@ -474,16 +511,13 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                         const lds_ptr<AccT> spare1 = cp1 + spare_row * kBandStride;
                         // t = v - 0.5 = B*(z - xc) + (nv*inv + hv - 0.5); z - xc is exact in
                         // FP32 (half-integers below 2^23), so t depends on z alone and not on
-                        // where the chunk group starts
-                        const float B0 = g.av[2] * f0.inv, B1 = g.av[2] * f1.inv;
-                        const float A0 = fmaf(f0.c.nv, f0.inv, d.hv - 0.5f);
-                        const float A1 = fmaf(f1.c.nv, f1.inv, d.hv - 0.5f);
+                        // where the chunk group starts.
                         // Whole columns (both z ranges are [0, N), N a multiple of 64): every
                         // loaded z is in range, so the loads need no masks and share one
                         // address per column.  Wave-uniform.
-                        const bool whole = KIND != FWD_WEIGHT && (d.N & 63) == 0 && f0.z_beg == 0 &&
-                                           f0.z_end == d.N && f1.z_beg == 0 && f1.z_end == d.N;
-                        const auto col0 = vol + f0.base + lane, col1 = vol + f1.base + lane;
+                        const bool whole = KIND != FWD_WEIGHT && (d.N & 63) == 0 && zb0 == 0 && ze0 == d.N &&
+                                           zb1 == 0 && ze1 == d.N;
+                        const auto col0 = vol + base0 + lane, col1 = vol + base1 + lane;
                         // one or two columns, whole or clipped: template constants, so that no
                         // per-chunk test of either remains
                         auto columns = [&](auto two_c, auto whole_c) {
@@ -511,11 +545,11 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                                         if (TWO) m1[c] = in ? col1[zg + c * 64] : 0.f;
                                     } else if (KIND == FWD_WEIGHT) {
                                         // zero outside the column's z range: the splat has no z test
-                                        m0[c] = (z < f0.z_end && z >= f0.z_beg) ? 1.f : 0.f;
-                                        if (TWO) m1[c] = (z < f1.z_end && z >= f1.z_beg) ? 1.f : 0.f;
+                                        m0[c] = (z < ze0 && z >= zb0) ? 1.f : 0.f;
+                                        if (TWO) m1[c] = (z < ze1 && z >= zb1) ? 1.f : 0.f;
                                     } else {
-                                        m0[c] = (z < f0.z_end && z >= f0.z_beg) ? vol[f0.base + z] : 0.f;
-                                        if (TWO) m1[c] = (z < f1.z_end && z >= f1.z_beg) ? vol[f1.base + z] : 0.f;
+                                        m0[c] = (z < ze0 && z >= zb0) ? vol[base0 + z] : 0.f;
+                                        if (TWO) m1[c] = (z < ze1 && z >= zb1) ? vol[base1 + z] : 0.f;
                                     }
                                 }
                                 }
@@ -545,6 +579,9 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                         }
                         continue;
                     }
+                    const FwdCol f0 = fwd_col<AXIS, KIND>(a, g, x0 + b0, y, v0, vb);
+                    const FwdCol f1 = fwd_col<AXIS, KIND>(a, g, x0 + b1, y, v0, vb);
+                    const int zb = min(f0.z_beg, f1.z_beg), ze = max(f0.z_end, f1.z_end);
                     for (int zg = zb & ~63; zg < ze; zg += 64 * kFwdChunks) {
                         float m0[kFwdChunks], m1[kFwdChunks];
 #pragma unroll
