@@ -313,7 +313,7 @@ int build_tables(xr_batch* b) {
     // Persistent grids: every block resident at once.
     hipDeviceProp_t prop;
     XR_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
-    b->ticket_slot = fwd_red_slot(max_vb + 2, 4) + kFwdWaves;  // past the 32-bit band and the reduction slots
+    b->ticket_slot = fwd_red_slot(max_vb + 2, 4) + kFwdBatchWaves;  // past the 32-bit band and the reduction slots
     b->fwd_lds = sizeof(unsigned long long) * ((size_t)b->ticket_slot + 1);
     const int fpc = batch_blocks_per_cu_forward(FWD_PLAIN, b->fwd_lds);
     if (fpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched forward kernel");

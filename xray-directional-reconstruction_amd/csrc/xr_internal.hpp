@@ -13,7 +13,11 @@ constexpr int kBandU = 16;
 // one pad column so the stride is odd (conflict-free rows on 32 banks).
 constexpr int kBandStride = kBandU + 3;
 // Waves per forward-projection block (they share one band accumulator).
-constexpr int kFwdWaves = 8;
+constexpr int kFwdWaves = 12;  // two 12-wave blocks per CU (58 KB band accumulator each): 6 waves per SIMD
+// The persistent multi-problem forward keeps more state per lane (about 100
+// VGPRs: 5 waves per SIMD), so 8-wave blocks fit two per CU where 12-wave ones
+// fit one.
+constexpr int kFwdBatchWaves = 8;
 // 8-byte slot index just past a forward band accumulator of `rows` rows of
 // `acc_bytes`-byte words (the per-wave reduction slots follow it).
 __host__ __device__ constexpr int fwd_red_slot(int rows, int acc_bytes) {

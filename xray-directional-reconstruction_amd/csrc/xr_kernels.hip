@@ -176,7 +176,8 @@ __device__ double chord_length(const ChordGeom& c, int u, int v, const Dims& d) 
 
 // ---------------------------------------------------------------------------
 // Forward projection: one block = (detector band of kBandU columns, one view,
-// a chunk of detector rows).  kFwdWaves waves walk the volume rows y (round
+// a chunk of detector rows).  The block's waves (kFwdWaves, or kFwdBatchWaves in
+// the persistent multi-problem launch) walk the volume rows y (round
 // robin), find the z-columns whose footprint touches the band (ballot over 64 x
 // per step), and splat them with lanes = z.  Two columns are in flight per wave
 // and all z chunks of a column are loaded before any is used.
@@ -378,7 +379,7 @@ __device__ __forceinline__ float xtt_value(const FwdArgs& a, const FwdCol& f, co
 // One (band, view) work item o = band * n_slots + slot of a forward launch.
 // step selects the view list views + step*n_slots (a solver's subset step);
 // loss receives the item's share of sum |y - p|.
-template <int KIND, bool AXIS>
+template <int KIND, bool AXIS, int WAVES>
 __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, double* loss,
                                              unsigned long long* acc_raw, int v0, int vb) {
     typedef typename FwdAcc<KIND>::T AccT;
@@ -394,7 +395,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const ViewGeom g = uniform_view(to_global(a.vg)[code]);
     const int u0 = band * kBandU;
     const int rows = vb + 2;
-    for (int i = threadIdx.x; i < rows * kBandStride; i += 64 * kFwdWaves) acc[i] = 0ull;
+    for (int i = threadIdx.x; i < rows * kBandStride; i += 64 * WAVES) acc[i] = 0ull;
     // Fixed-point scale: the largest pixel sum (weight pass: the largest
     // term) is < 2^30 * 2^-S.
     int e = 0;
@@ -423,7 +424,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const int spare_row = min(lane, vb);  // rows spare_row, spare_row + 1 lie in the band
     const bool rows_all = v0 <= 0 && v0 + vb >= d.V;  // this launch covers every detector row
 
-    for (int y = wave; y < d.N; y += kFwdWaves) {
+    for (int y = wave; y < d.N; y += WAVES) {
         const float yp = (float)y - d.xc;
         for (int x0 = 0; x0 < d.N; x0 += 64) {
             const int x = x0 + lane;
@@ -621,7 +622,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const int64_t img = (int64_t)(code - a.code_base) * d.V * d.U;
     const int64_t nimg = (int64_t)code * d.V * d.U;
     const bool acc_old = (KIND != FWD_WEIGHT) && accum && accum[slot];
-    for (int i = threadIdx.x; i < vb * kBandU; i += 64 * kFwdWaves) {
+    for (int i = threadIdx.x; i < vb * kBandU; i += 64 * WAVES) {
         const int r = i / kBandU, cc = i % kBandU;
         const int uu = u0 + cc, vv = v0 + r;
         if (uu >= d.U || vv >= d.V) continue;
@@ -667,7 +668,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
         __syncthreads();
         if (threadIdx.x == 0) {
             double t = 0.0;
-            for (int w = 0; w < kFwdWaves; w++) t += red[w];
+            for (int w = 0; w < WAVES; w++) t += red[w];
             atomicAdd(loss, t);
         }
     }
@@ -687,7 +688,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_forward(const FwdArgs a) {
     const int chunk = o / per_chunk;
     const int v0 = a.v0 + chunk * a.vb;
     const int vb = min(a.vb, a.d.V - v0);
-    forward_item<KIND, AXIS>(a, o - chunk * per_chunk, 0, a.loss, acc_raw, v0, vb);
+    forward_item<KIND, AXIS, kFwdWaves>(a, o - chunk * per_chunk, 0, a.loss, acc_raw, v0, vb);
 }
 
 // Work queue of a persistent launch: q[0] hands out tickets 0, 1, 2, ...; a
@@ -727,7 +728,7 @@ __device__ __forceinline__ int find_item(const int* __restrict__ item_end, int n
 // the tail.  ticket_slot: index of a free 8-byte LDS word past every item's
 // band accumulator.
 template <int KIND>
-__global__ __launch_bounds__(64 * kFwdWaves) void k_forward_batch(const FwdArgs* __restrict__ items,
+__global__ __launch_bounds__(64 * kFwdBatchWaves) void k_forward_batch(const FwdArgs* __restrict__ items,
                                                                   const int* __restrict__ item_end, int n_items,
                                                                   unsigned* queue, int ticket_slot, int step,
                                                                   int loss_shift) {
@@ -746,9 +747,9 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_forward_batch(const FwdArgs*
         const int o = t - (i ? item_end[i - 1] : 0);
         double* loss = a.loss ? a.loss + loss_shift : nullptr;
         if (a.axis)
-            forward_item<KIND, true>(a, o, step, loss, acc_raw, a.v0, a.vb);
+            forward_item<KIND, true, kFwdBatchWaves>(a, o, step, loss, acc_raw, a.v0, a.vb);
         else
-            forward_item<KIND, false>(a, o, step, loss, acc_raw, a.v0, a.vb);
+            forward_item<KIND, false, kFwdBatchWaves>(a, o, step, loss, acc_raw, a.v0, a.vb);
     }
 }
 
@@ -1982,7 +1983,7 @@ hipError_t launch_forward_batch(const FwdArgs* items, const int* item_end, int n
                                 hipStream_t stream) {
     if (n_items <= 0 || grid <= 0) return hipSuccess;
     if (kind == FWD_PLAIN)
-        hipLaunchKernelGGL((k_forward_batch<FWD_PLAIN>), dim3(grid), dim3(64 * kFwdWaves), lds_bytes, stream, items,
+        hipLaunchKernelGGL((k_forward_batch<FWD_PLAIN>), dim3(grid), dim3(64 * kFwdBatchWaves), lds_bytes, stream, items,
                            item_end, n_items, queue, ticket_slot, step, loss_shift);
     else
         return hipErrorInvalidValue;
@@ -2017,7 +2018,7 @@ size_t backward_batch_lds_bytes() { return (2 * kStageBuf + kBwdExtraWords) * si
 int batch_blocks_per_cu_forward(int kind, size_t lds_bytes) {
     int n = 0;
     if (kind != FWD_PLAIN) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_forward_batch<FWD_PLAIN>, 64 * kFwdWaves,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_forward_batch<FWD_PLAIN>, 64 * kFwdBatchWaves,
                                                      lds_bytes) != hipSuccess)
         return 0;
     return n;
