@@ -355,6 +355,23 @@ __device__ __forceinline__ void splat_untilted(float t, float val, unsigned row0
     __hip_atomic_fetch_add(p + kBandStride + 1, (AccT)(AccS)q2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// splat_untilted for a voxel known to lie in the window (0.05 < t, so fract(t)
+// = t - floor(t) exactly): the same terms at the same addresses, without the
+// test and the selects.
+template <class AccT, class AccS>
+__device__ __forceinline__ void splat_inside(float t, float val, unsigned row0, float wu0, float wu1) {
+    const float fv = __builtin_amdgcn_fractf(t);
+    const float g1 = fv * val, g0 = val - g1;
+    const unsigned at = __umul24((unsigned)floor_int(t), (unsigned)(kBandStride * sizeof(AccT))) + row0;
+    lds_ptr<AccT> p = (lds_ptr<AccT>)(uintptr_t)at;
+    const int q4 = round_half_up(wu0 * g0), q3 = round_half_up(wu1 * g0);
+    const int q1 = round_half_up(wu0 * g1), q2 = round_half_up(wu1 * g1);
+    __hip_atomic_fetch_add(p, (AccT)(AccS)q4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(p + 1, (AccT)(AccS)q3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(p + kBandStride, (AccT)(AccS)q1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(p + kBandStride + 1, (AccT)(AccS)q2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // XTT voxel value: sum_k |B x S_k|^2 (S_k.G)^2 mu_k * vs * ratio (ir.cu:1780-1794),
 // with |B x S|^2 = |S|^2 - (d.S)^2/|d|^2 for the source->voxel vector d.
 __device__ __forceinline__ float xtt_value(const FwdArgs& a, const FwdCol& f, const ViewGeom& g, int z,
@@ -519,11 +536,20 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                         const bool whole = KIND != FWD_WEIGHT && (d.N & 63) == 0 && zb0 == 0 && ze0 == d.N &&
                                            zb1 == 0 && ze1 == d.N;
                         const auto col0 = vol + base0 + lane, col1 = vol + base1 + lane;
-                        // one or two columns, whole or clipped: template constants, so that no
-                        // per-chunk test of either remains
-                        auto columns = [&](auto two_c, auto whole_c) {
+                        // Inside: every voxel of a whole column lies in the window (t at the
+                        // first and last z bound every lane's t: z - xc is exact and fmaf is
+                        // monotone in it), so the splat needs no window test.
+                        auto inside_col = [&](float B, float A) {
+                            const float ta = fmaf(B, -d.xc, A), tb = fmaf(B, (float)(d.N - 1) - d.xc, A);
+                            return fminf(ta, tb) > t_lo && fmaxf(ta, tb) < t_hi;
+                        };
+                        const bool inside = whole && inside_col(B0, A0) && (!two || inside_col(B1, A1));
+                        // one or two columns, whole or clipped, inside the window or not:
+                        // template constants, so that no per-chunk test of any remains
+                        auto columns = [&](auto two_c, auto whole_c, auto inside_c) {
                             constexpr bool TWO = decltype(two_c)::value;
                             constexpr bool WHOLE = decltype(whole_c)::value;
+                            constexpr bool INSIDE = decltype(inside_c)::value;
                             // z chunks start at multiples of 64 so that a lane always holds the
                             // same z residue: the splat's lane pairing (hence its rounding) does
                             // not depend on where the row chunk clips the column.
@@ -559,24 +585,35 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                                 for (int c = 0; c < kFwdChunks; c++) {
                                     if (zg + c * 64 >= ze) break;
                                     const float zp = lanef[c] + zoff;
-                                    splat_untilted<AccT, AccS>(fmaf(B0, zp, A0), m0[c], row00, spare0, w00, w01, t_lo,
-                                                               t_hi);
-                                    if (TWO)
-                                        splat_untilted<AccT, AccS>(fmaf(B1, zp, A1), m1[c], row01, spare1, w10, w11,
+                                    if (INSIDE) {
+                                        splat_inside<AccT, AccS>(fmaf(B0, zp, A0), m0[c], row00, w00, w01);
+                                        if (TWO) splat_inside<AccT, AccS>(fmaf(B1, zp, A1), m1[c], row01, w10, w11);
+                                    } else {
+                                        splat_untilted<AccT, AccS>(fmaf(B0, zp, A0), m0[c], row00, spare0, w00, w01,
                                                                    t_lo, t_hi);
+                                        if (TWO)
+                                            splat_untilted<AccT, AccS>(fmaf(B1, zp, A1), m1[c], row01, spare1, w10,
+                                                                       w11, t_lo, t_hi);
+                                    }
                                 }
                             }
                         };
+                        using T_ = std::true_type;
+                        using F_ = std::false_type;
                         if (two) {
-                            if (whole)
-                                columns(std::true_type(), std::true_type());
+                            if (inside)
+                                columns(T_(), T_(), T_());
+                            else if (whole)
+                                columns(T_(), T_(), F_());
                             else
-                                columns(std::true_type(), std::false_type());
+                                columns(T_(), F_(), F_());
                         } else {
-                            if (whole)
-                                columns(std::false_type(), std::true_type());
+                            if (inside)
+                                columns(F_(), T_(), T_());
+                            else if (whole)
+                                columns(F_(), T_(), F_());
                             else
-                                columns(std::false_type(), std::false_type());
+                                columns(F_(), F_(), F_());
                         }
                         continue;
                     }
