@@ -54,7 +54,10 @@ def test_new_network_is_identity(xr, ctx, torch_dev):
     net.close()
 
 
-@pytest.mark.parametrize("C,dims", [(24, (16, 16, 16)), (8, (8, 12, 16)), (16, (12, 8, 8))])
+# (4, 8, 32) and (8, 8, 64): dims that the LDS-tiled convolution (tiles of
+# 2 x 4 x 32 voxels) takes at the first level, and at the first two levels
+@pytest.mark.parametrize("C,dims", [(24, (16, 16, 16)), (8, (8, 12, 16)), (16, (12, 8, 8)), (24, (4, 8, 32)),
+                                    (16, (8, 8, 64))])
 def test_forward_matches_oracle(orc, xr, ctx, torch_dev, C, dims):
     import torch
 
@@ -76,7 +79,7 @@ def test_forward_matches_oracle(orc, xr, ctx, torch_dev, C, dims):
     net.close()
 
 
-@pytest.mark.parametrize("C,dims", [(8, (8, 8, 8)), (24, (8, 12, 16))])
+@pytest.mark.parametrize("C,dims", [(8, (8, 8, 8)), (24, (8, 12, 16)), (8, (4, 4, 32))])
 def test_gradients_match_oracle(orc, xr, ctx, torch_dev, C, dims):
     net = xr.UNet(ctx, C, seed=1)
     p = random_params(orc, C, 40 + C)

@@ -40,6 +40,8 @@ struct Layer {
     int Kpad, Kdpad;       // packed BF16 row lengths (forward, data gradient)
     __bf16* wf = nullptr;  // [round16(Cout)][Kpad]
     __bf16* wd = nullptr;  // [round16(Cin)][Kdpad]
+    int Ktpad;             // tiled-order forward row length: (Cin / 8) * 28 taps * 8
+    __bf16* wt = nullptr;  // [round16(Cout)][Ktpad]
 };
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
@@ -86,6 +88,7 @@ struct xr_unet {
         for (auto& l : L) {
             if (l.wf) (void)hipFree(l.wf);
             if (l.wd) (void)hipFree(l.wd);
+            if (l.wt) (void)hipFree(l.wt);
         }
         for (float* p : {d_p, d_g, d_m, d_v, d_ghead})
             if (p) (void)hipFree(p);
@@ -102,6 +105,8 @@ int pack_all(xr_unet* n) {
         const hipError_t e = launch_pack(n->d_p + l.w_off, l.Cin, l.Cout, l.wf, l.Kpad, i == 0 ? nullptr : l.wd,
                                          l.Kdpad, ctx->stream);
         if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "weight packing failed: %s", hipGetErrorString(e));
+        const hipError_t e2 = launch_pack_tiled(n->d_p + l.w_off, l.Cin, l.Cout, l.wt, l.Ktpad, ctx->stream);
+        if (e2 != hipSuccess) return fail(ctx, XR_ERR_HIP, "weight packing failed: %s", hipGetErrorString(e2));
     }
     return XR_OK;
 }
@@ -169,6 +174,8 @@ int conv_fwd(xr_unet* n, int i, const __bf16* in, int in_ld, int in_off, __bf16*
     a.Cin = l.Cin;
     a.w = l.wf;
     a.Kpad = l.Kpad;
+    a.wt = l.wt;
+    a.Ktpad = l.Ktpad;
     a.bias = n->d_p + l.b_off;
     a.out = out;
     a.out_ld = out_ld;
@@ -327,6 +334,7 @@ int xr_unet_create(xr_context* ctx, int base_channels, uint64_t seed, xr_unet** 
         off += l.Cout;
         l.Kpad = round_up(27 * l.Cin, 32);
         l.Kdpad = round_up(27 * l.Cout, 32);
+        l.Ktpad = l.Cin / 8 * 28 * 8;
     }
     n->head_off = off;
     off += C + 1;
@@ -335,6 +343,7 @@ int xr_unet_create(xr_context* ctx, int base_channels, uint64_t seed, xr_unet** 
         Layer& l = n->L[i];
         XR_HIP(ctx, hipMalloc((void**)&l.wf, sizeof(__bf16) * (size_t)round_up(l.Cout, 16) * l.Kpad));
         XR_HIP(ctx, hipMalloc((void**)&l.wd, sizeof(__bf16) * (size_t)round_up(l.Cin, 16) * l.Kdpad));
+        XR_HIP(ctx, hipMalloc((void**)&l.wt, sizeof(__bf16) * (size_t)round_up(l.Cout, 16) * l.Ktpad));
     }
     for (float** p : {&n->d_p, &n->d_g, &n->d_m, &n->d_v}) XR_HIP(ctx, hipMalloc((void**)p, sizeof(float) * off));
     XR_HIP(ctx, hipMalloc((void**)&n->d_red, sizeof(double) * 2));

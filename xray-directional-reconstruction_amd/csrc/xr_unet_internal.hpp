@@ -23,6 +23,11 @@ struct ConvArgs {
     int mask_ld, mask_off;
     int D0, D1, D2;  // spatial dims, D2 fastest
     int relu, accumulate;
+    // optional: the same weights as [round16(Cout)][Ktpad] with k = chunk*224 +
+    // tap*8 + c (chunk of 8 input channels, 28 taps with tap 27 zero), for the
+    // LDS-tiled kernel (used when the dims are multiples of its tile)
+    const __bf16* wt;
+    int Ktpad;
 };
 
 // dw[co * ldw + k] += sum_v gout[v][g_off + co] im2col(in)[v][k]  (k < 27*Cin).
@@ -55,6 +60,8 @@ hipError_t launch_head_bwd(const __bf16* e, int C, int64_t V, const float* w, co
                            float* dw, hipStream_t s);
 hipError_t launch_pack(const float* W, int Cin, int Cout, __bf16* wf, int Kpad, __bf16* wd, int Kdpad,
                        hipStream_t s);
+// Tiled-order forward weights (see ConvArgs::wt); Ktpad = (Cin / 8) * 224.
+hipError_t launch_pack_tiled(const float* W, int Cin, int Cout, __bf16* wt, int Ktpad, hipStream_t s);
 hipError_t launch_adam(float* p, const float* g, float* m, float* v, int64_t n, float b1, float b2, float lr_t,
                        float eps_t, hipStream_t s);
 hipError_t launch_sum(const float* x, int64_t n, double* out, hipStream_t s);

@@ -126,6 +126,106 @@ __global__ __launch_bounds__(256) void k_conv3(const ConvArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Forward convolution, LDS-tiled.  Block: 4 waves, an output tile of
+// 2 (D0) x 4 (D1) x 32 (D2) voxels; wave w owns D0 row w/2, D1 columns
+// 2(w%2) and 2(w%2)+1, and both 16-voxel halves of D2: four 16-row MFMA
+// tiles.  K runs over chunks of 8 input channels; for each chunk the input
+// halo (4 x 6 x 34 voxels x 8 channels, 13 KB) is staged in LDS once and read
+// by all 27 taps, instead of one global load per tap.  K within a chunk:
+// k = tap*8 + c, 28 taps with the last one zero (7 MFMA steps of 32: lane
+// group kg takes tap 4s + kg).
+// ---------------------------------------------------------------------------
+constexpr int kCT0 = 2, kCT1 = 4, kCT2 = 32;
+constexpr int kCH0 = kCT0 + 2, kCH1 = kCT1 + 2, kCH2 = kCT2 + 2;
+constexpr int kCHalo = kCH0 * kCH1 * kCH2;
+constexpr int kTapsPad = 28;
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3_tiled(const ConvArgs a) {
+    __shared__ bf16x8 halo[kCHalo];
+    const lds_bf16x8* const hl = (const lds_bf16x8*)halo;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kg = lane >> 4;
+    const int nb2 = a.D2 / kCT2, nb1 = a.D1 / kCT1;
+    const int b2 = blockIdx.x % nb2, b1 = (blockIdx.x / nb2) % nb1, b0 = blockIdx.x / (nb2 * nb1);
+    const int o0 = b0 * kCT0, o1 = b1 * kCT1, o2 = b2 * kCT2;
+    // halo coordinates (+1) of this lane's output voxels: tile m = 2*(x half) + (z half)
+    const int h0 = (wave >> 1) + 1;
+    int h1[4], h2[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        h1[m] = 2 * (wave & 1) + (m >> 1) + 1;
+        h2[m] = 16 * (m & 1) + row + 1;
+    }
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[m][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const __bf16* __restrict__ in = a.in + a.in_off;
+    const int nchunk = a.Cin / 8;
+    for (int cc = 0; cc < nchunk; cc++) {
+        __syncthreads();  // the previous chunk's reads are done
+        for (int i = threadIdx.x; i < kCHalo; i += 256) {
+            const int q2 = i % kCH2, q1 = (i / kCH2) % kCH1, q0 = i / (kCH2 * kCH1);
+            const int g0 = o0 + q0 - 1, g1 = o1 + q1 - 1, g2 = o2 + q2 - 1;
+            bf16x8 v = zero8();
+            if ((unsigned)g0 < (unsigned)a.D0 && (unsigned)g1 < (unsigned)a.D1 && (unsigned)g2 < (unsigned)a.D2)
+                v = *(const bf16x8*)(in + (((int64_t)g0 * a.D1 + g1) * a.D2 + g2) * a.in_ld + cc * 8);
+            halo[i] = v;
+        }
+        __syncthreads();
+        const __bf16* __restrict__ wc = a.wt + cc * (kTapsPad * 8);
+#pragma unroll
+        for (int s = 0; s < kTapsPad / 4; s++) {
+            const int tap = 4 * s + kg;
+            int d0, d1, d2;
+            tap_offsets(min(tap, 26), d0, d1, d2);  // tap 27 has zero weights
+            bf16x8 A[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) A[m] = hl[((h0 + d0) * kCH1 + h1[m] + d1) * kCH2 + h2[m] + d2];
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) {
+                const bf16x8 B = *(const bf16x8*)(wc + (int64_t)(nt * 16 + row) * a.Ktpad + tap * 8);
+#pragma unroll
+                for (int m = 0; m < 4; m++) acc[m][nt] = mfma(A[m], B, acc[m][nt]);
+            }
+        }
+    }
+    // D layout of the 16x16 MFMA: lane holds column (lane & 15), rows 4*kg + i.
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int64_t vrow = ((int64_t)(o0 + h0 - 1) * a.D1 + (o1 + h1[m] - 1)) * a.D2 + o2 + 16 * (m & 1);
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+            const int co = nt * 16 + row;
+            if (co >= a.Cout) continue;
+            const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int64_t v = vrow + 4 * kg + i;
+                float val = acc[m][nt][i] + b;
+                if (a.relu) val = fmaxf(val, 0.f);
+                if (a.mask && !((float)a.mask[v * a.mask_ld + a.mask_off + co] > 0.f)) val = 0.f;
+                __bf16* o = a.out + v * a.out_ld + a.out_off + co;
+                if (a.accumulate) val += (float)*o;
+                *o = (__bf16)val;
+            }
+        }
+    }
+}
+
+// Master FP32 weights [Cout][27][Cin] -> tiled-order BF16 [Cout16][Ktpad].
+__global__ void k_pack_tiled(const float* __restrict__ W, int Cin, int Cout, __bf16* wt, int Ktpad, int cout16) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)cout16 * Ktpad) return;
+    const int co = (int)(i / Ktpad), k = (int)(i % Ktpad);
+    const int cc = k / (kTapsPad * 8), tap = (k / 8) % kTapsPad, c = k % 8;
+    const int ci = cc * 8 + c;
+    wt[i] = (__bf16)((co < Cout && tap < 27 && ci < Cin) ? W[((int64_t)co * 27 + tap) * Cin + ci] : 0.f);
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient: dw[co][k] += sum_v gout[v][co] * im2col(in)[v][k].
 // Block: 4 waves; all output channels (MT tiles of 16) x 64 k columns (16 per
 // wave) over a slab of voxels, 32 voxels per step.  Both operands are staged
@@ -439,6 +539,22 @@ hipError_t launch_conv3(const ConvArgs& a, hipStream_t s) {
     if (a.Cin % 8 != 0 || a.in_ld % 8 != 0 || a.in_off % 8 != 0 || a.Kpad % 32 != 0 || a.Kpad < 27 * a.Cin)
         return hipErrorInvalidValue;
     const int nt = (a.Cout + 15) / 16;
+    if (a.wt && a.D0 % kCT0 == 0 && a.D1 % kCT1 == 0 && a.D2 % kCT2 == 0 && a.Ktpad == a.Cin / 8 * kTapsPad * 8 &&
+        nt <= 6) {
+        const dim3 tgrid((unsigned)(V / (kCT0 * kCT1 * kCT2)));
+        switch (nt) {
+#define XR_CONVT(N) \
+    case N: hipLaunchKernelGGL(k_conv3_tiled<N>, tgrid, dim3(256), 0, s, a); break;
+            XR_CONVT(1)
+            XR_CONVT(2)
+            XR_CONVT(3)
+            XR_CONVT(4)
+            XR_CONVT(5)
+            XR_CONVT(6)
+#undef XR_CONVT
+        }
+        return hipGetLastError();
+    }
     const dim3 grid(blocks(V, 128));
     switch (nt) {
 #define XR_CONV(N) \
@@ -583,6 +699,13 @@ hipError_t launch_pack(const float* W, int Cin, int Cout, __bf16* wf, int Kpad, 
     const int64_t n = (int64_t)cout16 * Kpad + (wd ? (int64_t)cin16 * Kdpad : 0);
     hipLaunchKernelGGL(k_pack, dim3(blocks(n, 256)), dim3(256), 0, s, W, Cin, Cout, wf, Kpad, cout16, wd, Kdpad,
                        cin16);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_tiled(const float* W, int Cin, int Cout, __bf16* wt, int Ktpad, hipStream_t s) {
+    const int cout16 = (Cout + 15) / 16 * 16;
+    const int64_t n = (int64_t)cout16 * Ktpad;
+    hipLaunchKernelGGL(k_pack_tiled, dim3(blocks(n, 256)), dim3(256), 0, s, W, Cin, Cout, wt, Ktpad, cout16);
     return hipGetLastError();
 }
 
