@@ -164,17 +164,35 @@ __global__ __launch_bounds__(256) void k_conv3_tiled(const ConvArgs a) {
         for (int nt = 0; nt < NT; nt++) acc[m][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     const __bf16* __restrict__ in = a.in + a.in_off;
     const int nchunk = a.Cin / 8;
+    // halo voxels of this thread: i = threadIdx.x + 256 j; their global offsets
+    // (-1: outside the volume, zero padding) are the same for every chunk
+    constexpr int kPer = (kCHalo + 255) / 256;
+    int64_t hoff[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        const int i = threadIdx.x + 256 * j;
+        const int q2 = i % kCH2, q1 = (i / kCH2) % kCH1, q0 = i / (kCH2 * kCH1);
+        const int g0 = o0 + q0 - 1, g1 = o1 + q1 - 1, g2 = o2 + q2 - 1;
+        const bool ok = i < kCHalo && (unsigned)g0 < (unsigned)a.D0 && (unsigned)g1 < (unsigned)a.D1 &&
+                        (unsigned)g2 < (unsigned)a.D2;
+        hoff[j] = ok ? (((int64_t)g0 * a.D1 + g1) * a.D2 + g2) * a.in_ld : -1;
+    }
+    bf16x8 pre[kPer];
+    auto fetch = [&](int cc) {
+#pragma unroll
+        for (int j = 0; j < kPer; j++) pre[j] = hoff[j] >= 0 ? *(const bf16x8*)(in + hoff[j] + cc * 8) : zero8();
+    };
+    auto store = [&] {
+#pragma unroll
+        for (int j = 0; j < kPer; j++)
+            if (threadIdx.x + 256 * j < kCHalo) halo[threadIdx.x + 256 * j] = pre[j];
+    };
+    fetch(0);
     for (int cc = 0; cc < nchunk; cc++) {
         __syncthreads();  // the previous chunk's reads are done
-        for (int i = threadIdx.x; i < kCHalo; i += 256) {
-            const int q2 = i % kCH2, q1 = (i / kCH2) % kCH1, q0 = i / (kCH2 * kCH1);
-            const int g0 = o0 + q0 - 1, g1 = o1 + q1 - 1, g2 = o2 + q2 - 1;
-            bf16x8 v = zero8();
-            if ((unsigned)g0 < (unsigned)a.D0 && (unsigned)g1 < (unsigned)a.D1 && (unsigned)g2 < (unsigned)a.D2)
-                v = *(const bf16x8*)(in + (((int64_t)g0 * a.D1 + g1) * a.D2 + g2) * a.in_ld + cc * 8);
-            halo[i] = v;
-        }
+        store();
         __syncthreads();
+        if (cc + 1 < nchunk) fetch(cc + 1);  // in flight during this chunk's MFMAs
         const __bf16* __restrict__ wc = a.wt + cc * (kTapsPad * 8);
 #pragma unroll
         for (int s = 0; s < kTapsPad / 4; s++) {
@@ -320,16 +338,20 @@ __global__ void k_bias_grad(const __bf16* __restrict__ g, int ld, int off, int C
 }
 
 // 2x2x2 average pooling: out[p][c] = mean of the 8 children (dims halve).
+// Index arithmetic of the elementwise kernels below: 32-bit (I = unsigned)
+// when the launch's element count fits, 64-bit otherwise (64-bit division is
+// several times the cost and these kernels are otherwise pure data movement).
+template <class I>
 __global__ void k_pool(const __bf16* __restrict__ in, int in_ld, int in_off, int C, int D0, int D1, int D2,
                        __bf16* __restrict__ out) {
     const int H0 = D0 / 2, H1 = D1 / 2, H2 = D2 / 2;
     const int G = C / 8;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = (int64_t)H0 * H1 * H2 * G;
+    const I i = (I)blockIdx.x * blockDim.x + threadIdx.x;
+    const I n = (I)H0 * H1 * H2 * G;
     if (i >= n) return;
     const int g = (int)(i % G);
     const int64_t p = i / G;
-    const int q2 = (int)(p % H2), q1 = (int)((p / H2) % H1), q0 = (int)(p / ((int64_t)H2 * H1));
+    const int q2 = (int)((I)p % H2), q1 = (int)(((I)p / H2) % H1), q0 = (int)((I)p / ((I)H2 * H1));
     float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int c = 0; c < 8; c++) {
         const int64_t v = ((int64_t)(2 * q0 + (c >> 2)) * D1 + (2 * q1 + ((c >> 1) & 1))) * D2 + (2 * q2 + (c & 1));
@@ -344,15 +366,16 @@ __global__ void k_pool(const __bf16* __restrict__ in, int in_ld, int in_off, int
 }
 
 // Pooling backward into a (masked) slice: gin[child][c] (+)= gp[p][c]/8 * (mask[child][c] > 0).
+template <class I>
 __global__ void k_pool_bwd(const __bf16* __restrict__ gp, int C, int D0, int D1, int D2, __bf16* gin, int g_ld,
                            int g_off, const __bf16* __restrict__ mask, int m_ld, int m_off, int accumulate) {
     const int G = C / 8;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = (int64_t)D0 * D1 * D2 * G;
+    const I i = (I)blockIdx.x * blockDim.x + threadIdx.x;
+    const I n = (I)D0 * D1 * D2 * G;
     if (i >= n) return;
     const int g = (int)(i % G);
     const int64_t v = i / G;
-    const int p2 = (int)(v % D2), p1 = (int)((v / D2) % D1), p0 = (int)(v / ((int64_t)D2 * D1));
+    const int p2 = (int)((I)v % D2), p1 = (int)(((I)v / D2) % D1), p0 = (int)((I)v / ((I)D2 * D1));
     const int64_t p = ((int64_t)(p0 / 2) * (D1 / 2) + p1 / 2) * (D2 / 2) + p2 / 2;
     const bf16x8 x = *(const bf16x8*)(gp + p * C + 8 * g);
     const bf16x8 m = *(const bf16x8*)(mask + v * m_ld + m_off + 8 * g);
@@ -369,30 +392,32 @@ __global__ void k_pool_bwd(const __bf16* __restrict__ gp, int C, int D0, int D1,
 }
 
 // Nearest-neighbour upsampling x2 into a channel slice: out[child][off + c] = low[parent][c].
+template <class I>
 __global__ void k_upsample(const __bf16* __restrict__ low, int C, int D0, int D1, int D2, __bf16* out, int o_ld,
                            int o_off) {
     const int G = C / 8;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = (int64_t)D0 * D1 * D2 * G;
+    const I i = (I)blockIdx.x * blockDim.x + threadIdx.x;
+    const I n = (I)D0 * D1 * D2 * G;
     if (i >= n) return;
     const int g = (int)(i % G);
     const int64_t v = i / G;
-    const int p2 = (int)(v % D2), p1 = (int)((v / D2) % D1), p0 = (int)(v / ((int64_t)D2 * D1));
+    const int p2 = (int)((I)v % D2), p1 = (int)(((I)v / D2) % D1), p0 = (int)((I)v / ((I)D2 * D1));
     const int64_t p = ((int64_t)(p0 / 2) * (D1 / 2) + p1 / 2) * (D2 / 2) + p2 / 2;
     *(bf16x8*)(out + v * o_ld + o_off + 8 * g) = *(const bf16x8*)(low + p * C + 8 * g);
 }
 
 // Upsampling backward: glow[p][c] = sum of the 8 children's g[child][off + c].
+template <class I>
 __global__ void k_upsample_bwd(const __bf16* __restrict__ g, int g_ld, int g_off, int C, int D0, int D1, int D2,
                                __bf16* __restrict__ glow) {
     const int H0 = D0 / 2, H1 = D1 / 2, H2 = D2 / 2;
     const int G = C / 8;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = (int64_t)H0 * H1 * H2 * G;
+    const I i = (I)blockIdx.x * blockDim.x + threadIdx.x;
+    const I n = (I)H0 * H1 * H2 * G;
     if (i >= n) return;
     const int gg = (int)(i % G);
     const int64_t p = i / G;
-    const int q2 = (int)(p % H2), q1 = (int)((p / H2) % H1), q0 = (int)(p / ((int64_t)H2 * H1));
+    const int q2 = (int)((I)p % H2), q1 = (int)(((I)p / H2) % H1), q0 = (int)((I)p / ((I)H2 * H1));
     float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int c = 0; c < 8; c++) {
         const int64_t v = ((int64_t)(2 * q0 + (c >> 2)) * D1 + (2 * q1 + ((c >> 1) & 1))) * D2 + (2 * q2 + (c & 1));
@@ -623,7 +648,12 @@ hipError_t launch_pool(const __bf16* in, int in_ld, int in_off, int C, int D0, i
     if (C % 8 || in_ld % 8 || in_off % 8 || (D0 | D1 | D2) & 1) return hipErrorInvalidValue;
     const int64_t n = (int64_t)(D0 / 2) * (D1 / 2) * (D2 / 2) * (C / 8);
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pool, dim3(blocks(n, 256)), dim3(256), 0, s, in, in_ld, in_off, C, D0, D1, D2, out);
+    if (n < (int64_t)1 << 31)
+        hipLaunchKernelGGL(k_pool<unsigned>, dim3(blocks(n, 256)), dim3(256), 0, s, in, in_ld, in_off, C, D0, D1, D2,
+                           out);
+    else
+        hipLaunchKernelGGL(k_pool<int64_t>, dim3(blocks(n, 256)), dim3(256), 0, s, in, in_ld, in_off, C, D0, D1, D2,
+                           out);
     return hipGetLastError();
 }
 
@@ -632,8 +662,12 @@ hipError_t launch_pool_bwd(const __bf16* gp, int C, int D0, int D1, int D2, __bf
     if (C % 8 || g_ld % 8 || g_off % 8 || m_ld % 8 || m_off % 8 || (D0 | D1 | D2) & 1) return hipErrorInvalidValue;
     const int64_t n = (int64_t)D0 * D1 * D2 * (C / 8);
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pool_bwd, dim3(blocks(n, 256)), dim3(256), 0, s, gp, C, D0, D1, D2, gin, g_ld, g_off, mask,
-                       m_ld, m_off, accumulate);
+    if (n < (int64_t)1 << 31)
+        hipLaunchKernelGGL(k_pool_bwd<unsigned>, dim3(blocks(n, 256)), dim3(256), 0, s, gp, C, D0, D1, D2, gin, g_ld,
+                           g_off, mask, m_ld, m_off, accumulate);
+    else
+        hipLaunchKernelGGL(k_pool_bwd<int64_t>, dim3(blocks(n, 256)), dim3(256), 0, s, gp, C, D0, D1, D2, gin, g_ld,
+                           g_off, mask, m_ld, m_off, accumulate);
     return hipGetLastError();
 }
 
@@ -642,7 +676,12 @@ hipError_t launch_upsample(const __bf16* low, int C, int D0, int D1, int D2, __b
     if (C % 8 || o_ld % 8 || o_off % 8 || (D0 | D1 | D2) & 1) return hipErrorInvalidValue;
     const int64_t n = (int64_t)D0 * D1 * D2 * (C / 8);
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_upsample, dim3(blocks(n, 256)), dim3(256), 0, s, low, C, D0, D1, D2, out, o_ld, o_off);
+    if (n < (int64_t)1 << 31)
+        hipLaunchKernelGGL(k_upsample<unsigned>, dim3(blocks(n, 256)), dim3(256), 0, s, low, C, D0, D1, D2, out, o_ld,
+                           o_off);
+    else
+        hipLaunchKernelGGL(k_upsample<int64_t>, dim3(blocks(n, 256)), dim3(256), 0, s, low, C, D0, D1, D2, out, o_ld,
+                           o_off);
     return hipGetLastError();
 }
 
@@ -651,7 +690,12 @@ hipError_t launch_upsample_bwd(const __bf16* g, int g_ld, int g_off, int C, int 
     if (C % 8 || g_ld % 8 || g_off % 8 || (D0 | D1 | D2) & 1) return hipErrorInvalidValue;
     const int64_t n = (int64_t)(D0 / 2) * (D1 / 2) * (D2 / 2) * (C / 8);
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_upsample_bwd, dim3(blocks(n, 256)), dim3(256), 0, s, g, g_ld, g_off, C, D0, D1, D2, glow);
+    if (n < (int64_t)1 << 31)
+        hipLaunchKernelGGL(k_upsample_bwd<unsigned>, dim3(blocks(n, 256)), dim3(256), 0, s, g, g_ld, g_off, C, D0, D1,
+                           D2, glow);
+    else
+        hipLaunchKernelGGL(k_upsample_bwd<int64_t>, dim3(blocks(n, 256)), dim3(256), 0, s, g, g_ld, g_off, C, D0, D1,
+                           D2, glow);
     return hipGetLastError();
 }
 
