@@ -64,6 +64,29 @@ def test_fallback_when_network_fits_worse(xr, ctx, ws, orc):
     bad.close()
 
 
+def test_judge_residual_over_view_subset(xr, ctx, ws):
+    """Scene.residual(x, every) is sum|y - Ax| / sum|y| over views 0, every,
+    2*every, ... exactly (FP64 sums of the same projected images)."""
+    import torch
+
+    N, P = 48, 72
+    sc = scene_for(xr, ctx, ws, N, P, 6, 81)
+    x = torch.full((N ** 3,), 0.01, dtype=torch.float32, device=sc.sino.device)
+    p = torch.zeros_like(sc.sino)
+    sc.plan.project(x, p, sc.codes)
+    y = sc.sino.view(P, -1).double()
+    p = p.view(P, -1).double()
+    for every in (1, 4, 5):
+        v = list(range(0, P, every))
+        want = float((y[v] - p[v]).abs().sum() / y[v].abs().sum())
+        got = sc.residual(x, every)
+        assert abs(got - want) <= 1e-6 * want, (every, got, want)
+    # the subset ranks volumes as the full set does here
+    x2 = x * 1.5
+    assert (sc.residual(x2, 4) < sc.residual(x, 4)) == (sc.residual(x2) < sc.residual(x))
+    sc.close()
+
+
 def test_identity_network_changes_nothing(xr, ctx, ws):
     """An untrained network is the identity: its volume fits exactly as well as
     its input, so the rule keeps the input (no improvement, no harm)."""
