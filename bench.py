@@ -72,7 +72,7 @@ def time_to_residual(ctx, xr, geom, sino, tau, subsets, max_epochs=300, pre=5):
             "plain_reached": plain["reached"],
             "warm_epochs": warm["iterations"], "warm_s": round(warm["ms"] / 1e3, 3), "warm_reached": warm["reached"],
             "network_kept": warm["used_net"],
-            "judge": "network kept if its residual over every 4th view is below the MLEM volume's",
+            "judge": "network kept unless its residual over every 4th view exceeds the MLEM volume's by > 10%",
             "speedup_epochs": round(plain["iterations"] / max(warm["iterations"], 1), 3),
             "speedup_time": round(plain["ms"] / max(warm["ms"], 1e-9), 3),
             "network": "models/warmstart_c24.xunt (128^3, 180 views in 20 subsets of 9, 960 mixed phantoms, "

@@ -56,6 +56,7 @@ def test_fallback_when_network_fits_worse(xr, ctx, ws, orc):
     plain = ws.plain_mlem(sc, tau, 300)
     warm = ws.warm_start(sc, bad, tau, 300, pre=5)
     print("plain", plain["iterations"], "warm", warm["iterations"], "used", warm["used_net"], flush=True)
+    # rejected by the default tolerance, not only by the strict rule
     assert plain["reached"]
     assert warm["used_net"] is False
     assert warm["iterations"] == plain["iterations"]
@@ -88,13 +89,16 @@ def test_judge_residual_over_view_subset(xr, ctx, ws):
 
 
 def test_identity_network_changes_nothing(xr, ctx, ws):
-    """An untrained network is the identity: its volume fits exactly as well as
-    its input, so the rule keeps the input (no improvement, no harm)."""
+    """An untrained network is the identity (up to the positivity floor): its
+    volume fits as well as its input, so either choice of the rule continues
+    the plain method (no improvement, no harm)."""
     sc = scene_for(xr, ctx, ws, 48, 72, 6, 78)
     net = xr.UNet(ctx, 8, seed=2)
     plain = ws.plain_mlem(sc, 5e-3, 300)
     warm = ws.warm_start(sc, net, 5e-3, 300, pre=5)
-    assert warm["iterations"] == plain["iterations"]
+    assert abs(warm["iterations"] - plain["iterations"]) <= 1
+    strict = ws.warm_start(sc, net, 5e-3, 300, pre=5, judge_tol=0.0)
+    assert strict["iterations"] == plain["iterations"]
     sc.close()
     net.close()
 

@@ -5,8 +5,9 @@ MLEM volume (the plain method after a few iterations) to an estimate of a
 late one.  It is trained only on pairs the plain method itself generates
 (``generate_pairs``), and at run time it never replaces the method: MLEM
 resumes from the network's output, and the projection-domain data residual
-judges it — if the network's volume fits the data worse than the volume it
-was given, the driver falls back to that volume (``warm_start``).
+judges it — if the network's volume fits the data clearly worse (beyond a
+relative tolerance) than the volume it was given, the driver falls back to
+that volume (``warm_start``).
 
 Data residual: r(x) = sum|y - A x| / sum|y| (relative L1 of the normalised
 forward projection A, accumulated in FP64 by the forward kernel), the
@@ -255,7 +256,8 @@ def _resume(scene, tau, max_iters, iters, ms, hist):
     return {"iterations": iters, "ms": ms, "residuals": hist, "reached": False}
 
 
-def warm_start(scene, net, tau, max_iters, pre=5, init=0.01, floor_frac=1e-3, use_net=True, judge_every=4):
+def warm_start(scene, net, tau, max_iters, pre=5, init=0.01, floor_frac=1e-3, use_net=True, judge_every=4,
+               judge_tol=0.1):
     """Plain MLEM for `pre` iterations, then the network (judged by the data
     residual, with fallback), then MLEM until r <= tau.  Returns iterations
     (MLEM iterations only), GPU milliseconds (everything: MLEM, network,
@@ -265,8 +267,12 @@ def warm_start(scene, net, tau, max_iters, pre=5, init=0.01, floor_frac=1e-3, us
     The judge compares the residuals of the two volumes over the same views,
     every `judge_every`-th one (both are sums over millions of detector
     pixels, so a quarter of the views ranks them as the full set does at a
-    quarter of the cost); judge_every=1 uses every view.  The stopping test
-    r <= tau always uses every view."""
+    quarter of the cost); judge_every=1 uses every view.  The network's volume
+    is kept unless its residual exceeds the MLEM volume's by more than
+    `judge_tol` (relative): a useful network changes the data residual little
+    (it corrects the components MLEM converges on slowly, which the residual
+    barely sees), while a harmful one raises it by far more.  The stopping
+    test r <= tau always uses every view."""
     import torch
 
     dev = scene.sino.device
@@ -290,7 +296,7 @@ def warm_start(scene, net, tau, max_iters, pre=5, init=0.01, floor_frac=1e-3, us
         torch.cuda.synchronize()
         scene.ctx.set_stream(0)
         ms += e0.elapsed_time(e1)
-        used = r_y < r_x
+        used = r_y <= (1.0 + judge_tol) * r_x
         scene.set_volume(y if used else x)
     out = _resume(scene, tau, max_iters, pre, ms, hist)
     out["used_net"] = used
