@@ -75,8 +75,8 @@ def time_to_residual(ctx, xr, geom, sino, tau, subsets, max_epochs=300, pre=5):
             "judge": "network kept unless its residual over every 4th view exceeds the MLEM volume's by > 10%",
             "speedup_epochs": round(plain["iterations"] / max(warm["iterations"], 1), 3),
             "speedup_time": round(plain["ms"] / max(warm["ms"], 1e-9), 3),
-            "network": "models/warmstart_c24.xunt (128^3, 180 views in 20 subsets of 9, 960 mixed phantoms, "
-                       "inputs after 5/10/20 epochs, targets after 200)"}
+            "network": "models/warmstart_c24.xunt (960 mixed phantoms at 128^3 / 180 views, then 240 at 256^3 / "
+                       "360 views, subsets of 9 views, inputs after 5/10/20 epochs, targets after 200)"}
 
 
 def batch_workload(xr, args, n=48, epochs=30, seed=4):

@@ -7,7 +7,8 @@ r = sum|y - A x| / sum|y| (FP64 sums in the forward kernel).  The committed
 network (models/warmstart_c24.xunt) was trained by train_warmstart.py on 960
 seeded 128^3 phantoms (ellipsoid-and-fibre volumes and dense ellipsoid sets
 with cylinder shells) at 180 views, 20 subsets, inputs after 5, 10 and 20
-epochs, targets after 200."""
+epochs, targets after 200, then fine-tuned on 240 phantoms at 256^3 (360
+views, 40 subsets)."""
 import os
 
 import numpy as np
@@ -109,7 +110,7 @@ def test_trained_network_gains_on_held_out_phantom(xr, ctx, ws, seed, family):
     """On held-out 128^3 phantoms (training distribution; seed 9003 of the
     mixed family is a dense ellipsoid set with a cylinder shell), MLEM resumed
     from the trained network reaches r <= 1e-3 in far fewer epochs than the
-    plain method (measured 1.9x and 2.3x in epochs)."""
+    plain method (measured 2.2x and 2.5x in epochs)."""
     net = xr.UNet(ctx, 24)
     net.load(MODEL)
     sc = scene_for(xr, ctx, ws, 128, 180, 20, seed, family=family)
