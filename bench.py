@@ -56,7 +56,7 @@ def time_to_residual(ctx, xr, geom, sino, tau, subsets, max_epochs=300, pre=5):
     """Epochs and GPU seconds until the data residual r = sum|y - Ax|/sum|y|
     reaches tau: the plain method (MLEM, `subsets` ordered subsets, x0 = 0.01)
     against the learned warm start (plain for `pre` epochs, then the trained
-    network judged by the residual with fallback, then plain again)."""
+    network checked by the residual with fallback, then plain again)."""
     import importlib
 
     ws = importlib.import_module(xr.__name__ + ".warmstart")
@@ -72,7 +72,7 @@ def time_to_residual(ctx, xr, geom, sino, tau, subsets, max_epochs=300, pre=5):
             "plain_reached": plain["reached"],
             "warm_epochs": warm["iterations"], "warm_s": round(warm["ms"] / 1e3, 3), "warm_reached": warm["reached"],
             "network_kept": warm["used_net"],
-            "judge": "network kept unless its residual over every 4th view exceeds the MLEM volume's by > 10%",
+            "acceptance": "network kept unless its residual over every 4th view exceeds the MLEM volume's by > 10%",
             "speedup_epochs": round(plain["iterations"] / max(warm["iterations"], 1), 3),
             "speedup_time": round(plain["ms"] / max(warm["ms"], 1e-9), 3),
             "network": "models/warmstart_c24.xunt (960 mixed phantoms at 128^3 / 180 views, then 240 at 256^3 / "

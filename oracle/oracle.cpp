@@ -4,7 +4,8 @@
 // operators, their MLEM/ART/XTT/FDK orchestration and the per-voxel PCA, so
 // that the HIP product path can be checked against it.  Only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
-// only as the checker or the timed CPU baseline.  The product never links it.
+// only as the reference results to compare against or the timed CPU baseline.
+// The product never links it.
 //
 // Every function is templated on the working precision T:
 //   T = float  : "ref32" — the reference's FP32 arithmetic in its operation

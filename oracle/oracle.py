@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so) — TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
-module, as the checker or the timed CPU baseline.  ``prec`` selects the
+module, as the reference results to compare against or the timed CPU baseline.  ``prec`` selects the
 reference's FP32 arithmetic (REF32) or the same operator in FP64 (F64).
 """
 import ctypes

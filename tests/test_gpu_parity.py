@@ -1,6 +1,6 @@
 """GPU parity of the HIP path against the CPU oracle, through the C ABI.
 
-Tolerances (north star): projections <= 1e-5 relative L2 against the FP64
+Tolerances: projections <= 1e-5 relative L2 against the FP64
 oracle; reconstructions <= 1e-4 after the stated iterations.  The FP32
 "ref32" restatement of the reference's arithmetic is also compared; its own
 distance to FP64 is the reference's rounding noise, and the GPU must be no
@@ -16,12 +16,15 @@ from conftest import ellipsoid_phantom, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+# measured errors are written to $XR_PARITY_OUT/parity_errors.json when set
+OUT = os.environ.get("XR_PARITY_OUT")
 MEASURED = {}
 
 
 def record(key, value):
     MEASURED[key] = float(value)
+    if not OUT:
+        return
     os.makedirs(OUT, exist_ok=True)
     with open(os.path.join(OUT, "parity_errors.json"), "w") as f:
         json.dump(MEASURED, f, indent=1, sort_keys=True)

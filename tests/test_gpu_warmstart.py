@@ -1,4 +1,4 @@
-"""Learned warm start: the residual judge with fallback, the positivity clamp,
+"""Learned warm start: the residual check with fallback, the positivity clamp,
 and what the trained network gains over the plain method.
 
 The plain method is IR::reconstruct's MLEM with ordered subsets
@@ -66,7 +66,7 @@ def test_fallback_when_network_fits_worse(xr, ctx, ws, orc):
     bad.close()
 
 
-def test_judge_residual_over_view_subset(xr, ctx, ws):
+def test_residual_over_view_subset(xr, ctx, ws):
     """Scene.residual(x, every) is sum|y - Ax| / sum|y| over views 0, every,
     2*every, ... exactly (FP64 sums of the same projected images)."""
     import torch
@@ -98,7 +98,7 @@ def test_identity_network_changes_nothing(xr, ctx, ws):
     plain = ws.plain_mlem(sc, 5e-3, 300)
     warm = ws.warm_start(sc, net, 5e-3, 300, pre=5)
     assert abs(warm["iterations"] - plain["iterations"]) <= 1
-    strict = ws.warm_start(sc, net, 5e-3, 300, pre=5, judge_tol=0.0)
+    strict = ws.warm_start(sc, net, 5e-3, 300, pre=5, accept_tol=0.0)
     assert strict["iterations"] == plain["iterations"]
     sc.close()
     net.close()

@@ -5,7 +5,7 @@ MLEM volume (the plain method after a few iterations) to an estimate of a
 late one.  It is trained only on pairs the plain method itself generates
 (``generate_pairs``), and at run time it never replaces the method: MLEM
 resumes from the network's output, and the projection-domain data residual
-judges it — if the network's volume fits the data clearly worse (beyond a
+checks it — if the network's volume fits the data clearly worse (beyond a
 relative tolerance) than the volume it was given, the driver falls back to
 that volume (``warm_start``).
 
@@ -46,7 +46,7 @@ class Scene:
         self.proj = torch.empty(self.P * self.U * self.V, dtype=torch.float32, device=dev)
         self.codes = self.plan.codes(range(self.P))
         self.sino = sino_dev
-        self._judge = {}
+        self._subset = {}
 
     def close(self):
         self.batch.close()
@@ -69,14 +69,14 @@ class Scene:
         if every <= 1:
             loss = self.plan.project(vol_cols, self.proj, self.codes, out_mode=OUT_RATIO, sino=self.sino)
             return loss / self.y_l1
-        if every not in self._judge:
+        if every not in self._subset:
             import torch
 
             views = list(range(0, self.P, every))
             idx = torch.tensor(views, device=self.sino.device)
             y_l1 = float(self.sino.view(self.P, -1)[idx].abs().double().sum())
-            self._judge[every] = (self.plan.codes(views), y_l1)
-        codes, y_l1 = self._judge[every]
+            self._subset[every] = (self.plan.codes(views), y_l1)
+        codes, y_l1 = self._subset[every]
         # images are addressed by view in the full sinogram and ratio buffers
         loss = self.plan.project(vol_cols, self.proj, codes, out_mode=OUT_RATIO, sino=self.sino)
         return loss / y_l1
@@ -256,20 +256,20 @@ def _resume(scene, tau, max_iters, iters, ms, hist):
     return {"iterations": iters, "ms": ms, "residuals": hist, "reached": False}
 
 
-def warm_start(scene, net, tau, max_iters, pre=5, init=0.01, floor_frac=1e-3, use_net=True, judge_every=4,
-               judge_tol=0.1):
-    """Plain MLEM for `pre` iterations, then the network (judged by the data
+def warm_start(scene, net, tau, max_iters, pre=5, init=0.01, floor_frac=1e-3, use_net=True, check_every=4,
+               accept_tol=0.1):
+    """Plain MLEM for `pre` iterations, then the network (checked by the data
     residual, with fallback), then MLEM until r <= tau.  Returns iterations
     (MLEM iterations only), GPU milliseconds (everything: MLEM, network,
     residual checks), the residual history and whether the network's volume
     was kept.
 
-    The judge compares the residuals of the two volumes over the same views,
-    every `judge_every`-th one (both are sums over millions of detector
+    The check compares the residuals of the two volumes over the same views,
+    every `check_every`-th one (both are sums over millions of detector
     pixels, so a quarter of the views ranks them as the full set does at a
-    quarter of the cost); judge_every=1 uses every view.  The network's volume
+    quarter of the cost); check_every=1 uses every view.  The network's volume
     is kept unless its residual exceeds the MLEM volume's by more than
-    `judge_tol` (relative): a useful network changes the data residual little
+    `accept_tol` (relative): a useful network changes the data residual little
     (it corrects the components MLEM converges on slowly, which the residual
     barely sees), while a harmful one raises it by far more.  The stopping
     test r <= tau always uses every view."""
@@ -290,13 +290,13 @@ def warm_start(scene, net, tau, max_iters, pre=5, init=0.01, floor_frac=1e-3, us
         e0.record()
         net.forward(x, y, (scene.N, scene.N, scene.N))
         positivity_floor(y, floor_frac * float(net.last_scale()))
-        r_x = scene.residual(x, judge_every)
-        r_y = scene.residual(y, judge_every)
+        r_x = scene.residual(x, check_every)
+        r_y = scene.residual(y, check_every)
         e1.record()
         torch.cuda.synchronize()
         scene.ctx.set_stream(0)
         ms += e0.elapsed_time(e1)
-        used = r_y <= (1.0 + judge_tol) * r_x
+        used = r_y <= (1.0 + accept_tol) * r_x
         scene.set_volume(y if used else x)
     out = _resume(scene, tau, max_iters, pre, ms, hist)
     out["used_net"] = used
