@@ -130,6 +130,7 @@ struct xr_batch {
     double* d_loss = nullptr;  // [epoch_cap][n]
     unsigned* d_queue = nullptr;  // [4]
     int fwd_grid = 0, bwd_grid[3] = {0, 0, 0}, ticket_slot = 0;
+    bool fwd_axis_all = false;  // every problem untilted: the untilted-only forward kernel
     size_t fwd_lds = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -313,9 +314,12 @@ int build_tables(xr_batch* b) {
     // Persistent grids: every block resident at once.
     hipDeviceProp_t prop;
     XR_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
-    b->ticket_slot = fwd_red_slot(max_vb + 2, 4) + kFwdBatchWaves;  // past the 32-bit band and the reduction slots
+    b->fwd_axis_all = true;
+    for (const auto& p : b->pr) b->fwd_axis_all = b->fwd_axis_all && p.plan->axis_cond[0];
+    // past the 32-bit band and the reduction slots (one per wave of either kernel)
+    b->ticket_slot = fwd_red_slot(max_vb + 2, 4) + std::max(kFwdWaves, kFwdBatchWaves);
     b->fwd_lds = sizeof(unsigned long long) * ((size_t)b->ticket_slot + 1);
-    const int fpc = batch_blocks_per_cu_forward(FWD_PLAIN, b->fwd_lds);
+    const int fpc = batch_blocks_per_cu_forward(FWD_PLAIN, b->fwd_lds, b->fwd_axis_all);
     if (fpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched forward kernel");
     b->fwd_grid = fpc * prop.multiProcessorCount;
     for (int c = 0; c < 3; c++) {
@@ -343,7 +347,7 @@ int enqueue_epochs(xr_batch* b, int epochs) {
             if (e == hipSuccess) e = launch_absmax_seg(s.segs, s.seg_len, b->d_vmax, s.n_active, s.absmax_blocks, st);
             if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "absmax failed: %s", hipGetErrorString(e));
             e = launch_forward_batch(s.fwd, s.fwd_end, s.n_fwd, b->d_queue, b->ticket_slot, b->fwd_lds, b->fwd_grid,
-                                     step, ep * b->n, FWD_PLAIN, st);
+                                     step, ep * b->n, FWD_PLAIN, b->fwd_axis_all, st);
             if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "batched forward failed: %s", hipGetErrorString(e));
             // one persistent launch per z depth class present (in-order on the stream)
             for (int c = 2; c >= 0; c--) {

@@ -143,12 +143,14 @@ hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream);
 // site; grid: resident blocks (batch_grid_*).
 hipError_t launch_forward_batch(const FwdArgs* items, const int* item_end, int n_items, unsigned* queue,
                                 int ticket_slot, size_t lds_bytes, int grid, int step, int loss_shift, int kind,
+                                bool axis_all,
                                 hipStream_t stream);
 // One launch per z depth class: every item of the launch has a.zpl == zpl.
 hipError_t launch_backward_batch(const BwdArgs* items, const int* item_end, int n_items, unsigned* queue, int grid,
                                  int step, int kind, int zpl, hipStream_t stream);
 // Resident blocks per CU for the batched kernels (0 on error).
-int batch_blocks_per_cu_forward(int kind, size_t lds_bytes);
+// axis_all: every item untilted (the untilted-only kernel, kFwdWaves waves per block)
+int batch_blocks_per_cu_forward(int kind, size_t lds_bytes, bool axis_all);
 int batch_blocks_per_cu_backward(int kind, int zpl);
 size_t backward_batch_lds_bytes();
 // out[i] = max |segs[i][0, seg_len[i])| (out zeroed beforehand).

@@ -764,11 +764,10 @@ __device__ __forceinline__ int find_item(const int* __restrict__ item_end, int n
 // so the long items listed first are started first and the short ones fill
 // the tail.  ticket_slot: index of a free 8-byte LDS word past every item's
 // band accumulator.
-template <int KIND>
-__global__ __launch_bounds__(64 * kFwdBatchWaves) void k_forward_batch(const FwdArgs* __restrict__ items,
-                                                                  const int* __restrict__ item_end, int n_items,
-                                                                  unsigned* queue, int ticket_slot, int step,
-                                                                  int loss_shift) {
+template <int KIND, int AXES, int WAVES>
+__device__ __forceinline__ void forward_batch_loop(const FwdArgs* __restrict__ items,
+                                                   const int* __restrict__ item_end, int n_items, unsigned* queue,
+                                                   int ticket_slot, int step, int loss_shift) {
     extern __shared__ unsigned long long acc_raw[];
     lds_u64* const tk = (lds_u64*)acc_raw + ticket_slot;
     const int total = item_end[n_items - 1];
@@ -783,11 +782,29 @@ __global__ __launch_bounds__(64 * kFwdBatchWaves) void k_forward_batch(const Fwd
         const FwdArgs& a = items[i];
         const int o = t - (i ? item_end[i - 1] : 0);
         double* loss = a.loss ? a.loss + loss_shift : nullptr;
-        if (a.axis)
-            forward_item<KIND, true, kFwdBatchWaves>(a, o, step, loss, acc_raw, a.v0, a.vb);
+        if (AXES == 1 || a.axis)
+            forward_item<KIND, true, WAVES>(a, o, step, loss, acc_raw, a.v0, a.vb);
         else
-            forward_item<KIND, false, kFwdBatchWaves>(a, o, step, loss, acc_raw, a.v0, a.vb);
+            forward_item<KIND, false, WAVES>(a, o, step, loss, acc_raw, a.v0, a.vb);
     }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64 * kFwdBatchWaves) void k_forward_batch(const FwdArgs* __restrict__ items,
+                                                                  const int* __restrict__ item_end, int n_items,
+                                                                  unsigned* queue, int ticket_slot, int step,
+                                                                  int loss_shift) {
+    forward_batch_loop<KIND, 0, kFwdBatchWaves>(items, item_end, n_items, queue, ticket_slot, step, loss_shift);
+}
+
+// The same loop when every item is untilted: without the tilted path the
+// kernel needs fewer registers, so 12-wave blocks fit two per CU.
+template <int KIND>
+__global__ __launch_bounds__(64 * kFwdWaves) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_forward_batch_axis(const FwdArgs* __restrict__ items,
+                                                                      const int* __restrict__ item_end, int n_items,
+                                                                      unsigned* queue, int ticket_slot, int step,
+                                                                      int loss_shift) {
+    forward_batch_loop<KIND, 1, kFwdWaves>(items, item_end, n_items, queue, ticket_slot, step, loss_shift);
 }
 
 // ---------------------------------------------------------------------------
@@ -2017,13 +2034,15 @@ hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream) {
 
 hipError_t launch_forward_batch(const FwdArgs* items, const int* item_end, int n_items, unsigned* queue,
                                 int ticket_slot, size_t lds_bytes, int grid, int step, int loss_shift, int kind,
-                                hipStream_t stream) {
+                                bool axis_all, hipStream_t stream) {
     if (n_items <= 0 || grid <= 0) return hipSuccess;
-    if (kind == FWD_PLAIN)
+    if (kind != FWD_PLAIN) return hipErrorInvalidValue;
+    if (axis_all)
+        hipLaunchKernelGGL((k_forward_batch_axis<FWD_PLAIN>), dim3(grid), dim3(64 * kFwdWaves), lds_bytes, stream,
+                           items, item_end, n_items, queue, ticket_slot, step, loss_shift);
+    else
         hipLaunchKernelGGL((k_forward_batch<FWD_PLAIN>), dim3(grid), dim3(64 * kFwdBatchWaves), lds_bytes, stream, items,
                            item_end, n_items, queue, ticket_slot, step, loss_shift);
-    else
-        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -2052,13 +2071,15 @@ hipError_t launch_backward_batch(const BwdArgs* items, const int* item_end, int 
 
 size_t backward_batch_lds_bytes() { return (2 * kStageBuf + kBwdExtraWords) * sizeof(float); }
 
-int batch_blocks_per_cu_forward(int kind, size_t lds_bytes) {
+int batch_blocks_per_cu_forward(int kind, size_t lds_bytes, bool axis_all) {
     int n = 0;
     if (kind != FWD_PLAIN) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_forward_batch<FWD_PLAIN>, 64 * kFwdBatchWaves,
-                                                     lds_bytes) != hipSuccess)
-        return 0;
-    return n;
+    const hipError_t e =
+        axis_all ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_forward_batch_axis<FWD_PLAIN>,
+                                                                64 * kFwdWaves, lds_bytes)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_forward_batch<FWD_PLAIN>,
+                                                                64 * kFwdBatchWaves, lds_bytes);
+    return e == hipSuccess ? n : 0;
 }
 
 int batch_blocks_per_cu_backward(int kind, int zpl) {
