@@ -134,11 +134,15 @@ float xr_default_vox_size(float sdd, float sod, float det_size);
  *   bwd_gx:   column groups along x in a 16-wave backprojection block
  *             (1, 2, 4, 8, 16; gy = 16/gx; default 4)
  *   fwd_rows: cap on the detector rows of one forward launch (default: the
- *             most that fit 64 KB of LDS) */
+ *             most that fit 64 KB of LDS)
+ *   fwd_scan: 0 (default): the forward of an untilted view scans, per volume
+ *             row, only the columns a per-plan table gives as possibly
+ *             touching the detector band; 1: every column of every row */
 typedef struct xr_tiles {
     int32_t bwd_zpl;
     int32_t bwd_gx;
     int32_t fwd_rows;
+    int32_t fwd_scan;
 } xr_tiles;
 
 int xr_plan_set_tiles(xr_plan* plan, const xr_tiles* tiles);

@@ -38,7 +38,7 @@ namespace {
 bool valid_tiles(const xr_tiles& t) {
     const int z = t.bwd_zpl, g = t.bwd_gx;
     return (z == 0 || z == 1 || z == 2 || z == 4) && (g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16) &&
-           t.fwd_rows >= 0;
+           t.fwd_rows >= 0 && (t.fwd_scan == 0 || t.fwd_scan == 1);
 }
 
 std::array<int, 5> tune_key(const xr_plan* p) {
@@ -239,6 +239,7 @@ int build_tables(xr_batch* b) {
             a.proj = p.proj;
             a.sino = p.sino;
             a.norm = pl->d_norm;
+            a.bands = pl->tiles.fwd_scan ? nullptr : pl->d_bands;
             a.g2 = pl->d_g2;
             a.loss = b->d_loss + k;
             a.vmax = b->d_vmax + slot;
@@ -372,8 +373,8 @@ extern "C" {
 
 int xr_plan_set_tiles(xr_plan* plan, const xr_tiles* tiles) {
     if (!plan || !tiles) return XR_ERR_INVALID;
-    if (!valid_tiles(*tiles)) return fail(plan->ctx, XR_ERR_INVALID, "invalid tile shape (zpl %d, gx %d, rows %d)",
-                                          tiles->bwd_zpl, tiles->bwd_gx, tiles->fwd_rows);
+    if (!valid_tiles(*tiles)) return fail(plan->ctx, XR_ERR_INVALID, "invalid tile shape (zpl %d, gx %d, rows %d, scan %d)",
+                                          tiles->bwd_zpl, tiles->bwd_gx, tiles->fwd_rows, tiles->fwd_scan);
     plan->tiles = *tiles;
     return XR_OK;
 }
@@ -386,6 +387,7 @@ int xr_plan_get_tiles(const xr_plan* plan, xr_tiles* tiles) {
     tiles->bwd_zpl = a.zpl;
     tiles->bwd_gx = a.gx;
     tiles->fwd_rows = forward_rows(plan, FWD_PLAIN);
+    tiles->fwd_scan = plan->tiles.fwd_scan;
     return XR_OK;
 }
 

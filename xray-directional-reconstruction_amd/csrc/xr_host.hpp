@@ -44,6 +44,7 @@ struct xr_plan {
     xr::ChordGeom* d_cg = nullptr;
     float* d_g2 = nullptr;    // [n_cond*P][K]: (S_k.G)^2
     float* d_norm = nullptr;  // [n_cond][P][V][U]: L/(factor+1e-5)
+    int32_t* d_bands = nullptr;  // [n_cond*P][n_bands][N]: candidate columns xs | xe << 16 of each volume row
     xr::Dims dims{};
     xr::Basis bas{};
     float xtt_wsum = 0.f;  // sum_k |S_k|^2 max_view (S_k.G)^2: XTT splat value per unit |mu|

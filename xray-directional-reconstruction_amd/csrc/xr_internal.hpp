@@ -105,6 +105,7 @@ struct FwdArgs {
     int v0, vb;              // detector rows [v0, v0+vb) of the first row chunk (chunk height vb)
     int n_chunks;            // row chunks in one launch: chunk c owns rows v0 + c*vb + [0, vb) (clipped to V)
     int axis;                // batched launches: 1 -> every view is axis aligned (ViewGeom::axis)
+    const int32_t* bands;    // untilted views: [code][band][y] columns xs | xe << 16 to scan, or null (all)
 };
 
 struct BwdArgs {
@@ -169,6 +170,11 @@ hipError_t launch_pca(const float* mu, int K, const Basis& b, int64_t nvox, int6
 hipError_t launch_fill(float* p, int64_t n, float value, hipStream_t stream);
 // out[0] = max |x| over n floats (out must hold 0 beforehand).
 hipError_t launch_absmax(const float* x, int64_t n, float* out, hipStream_t stream);
+// Candidate-column table of the forward scan: for every (code, band, volume
+// row) the columns [xs, xe) outside which no column can touch the band
+// (untilted views; whole rows otherwise), packed xs | xe << 16.
+hipError_t launch_band_rows(const ViewGeom* vg, const Dims& d, int n_codes, int n_bands, int32_t* out,
+                            hipStream_t stream);
 hipError_t launch_clamp_min(float* p, int64_t n, float floor_value, float replacement, hipStream_t stream);
 hipError_t launch_ellipsoids(float* vol, int N, float vs, const double* params_dev, int count, hipStream_t stream);
 hipError_t launch_cylinder_shell(float* vol, int N, float vs, double r_out, double thick, double value,

@@ -393,6 +393,82 @@ __device__ __forceinline__ float xtt_value(const FwdArgs& a, const FwdCol& f, co
     return s * (d.vs * d.ratio);
 }
 
+// Untilted view, volume row y' = yp: u(x') = hu + (au0 x' + cu) / (ad0 x' + cd)
+// is linear-fractional in x', hence monotone while the denominator is
+// positive, so the columns whose u falls in [ua, ub] form an interval.  The
+// u of a column is evaluated exactly as the scan's candidate test does; its
+// rounding error (a few ulp, far below kRowMargin) can break monotonicity by
+// that much, so every decision below keeps kRowMargin clear of the band:
+//  - u at both ends of the row below ua (or both above ub): no column can be
+//    a candidate, the row is empty;
+//  - otherwise the closed-form solution, widened by two columns, is a guess
+//    (it degenerates where u hardly varies along the row), accepted only if
+//    u lies beyond the band on the proper side at the first column left of it
+//    and at the first column right of it;
+//  - anything else (denominator not positive, non-finite u, a rejected guess)
+//    scans the whole row.
+// The scan still tests every column of [xs, xe) exactly, so the result does
+// not depend on the guess.
+constexpr float kRowMargin = 0.01f;
+
+__device__ __forceinline__ float row_u(const ViewGeom& g, const Dims& d, int x, float yp) {
+    const ColForms c = col_forms(g, (float)x - d.xc, yp);
+    return fmaf(c.nu, rcpf(c.nd), d.hu);
+}
+
+__device__ __forceinline__ void band_columns(const ViewGeom& g, const Dims& d, float yp, float ua, float ub, int& xs,
+                                             int& xe) {
+    xs = 0;
+    xe = d.N;
+    const float cu = fmaf(g.au[1], yp, g.au[3]), cd = fmaf(g.ad[1], yp, g.ad[3]);
+    const float x0 = -d.xc, x1 = (float)(d.N - 1) - d.xc;
+    if (!(fmaf(g.ad[0], x0, cd) > 0.f && fmaf(g.ad[0], x1, cd) > 0.f)) return;
+    const float u_first = row_u(g, d, 0, yp), u_last = row_u(g, d, d.N - 1, yp);
+    if (!(fabsf(u_first) < 1e30f && fabsf(u_last) < 1e30f)) return;
+    const float lo_u = ua - kRowMargin, hi_u = ub + kRowMargin;
+    if (fmaxf(u_first, u_last) < lo_u || fminf(u_first, u_last) > hi_u) {
+        xe = 0;  // the whole row misses the band
+        return;
+    }
+    if (!(u_first != u_last)) return;
+    const bool up = u_last > u_first;
+    // x' with u(x') = u: (u - hu)(ad0 x' + cd) = au0 x' + cu
+    const float ka = ua - d.hu, kb = ub - d.hu;
+    const float xa = (cu - ka * cd) / (ka * g.ad[0] - g.au[0]);
+    const float xb = (cu - kb * cd) / (kb * g.ad[0] - g.au[0]);
+    const float lo = fminf(xa, xb) + d.xc - 2.0f, hi = fmaxf(xa, xb) + d.xc + 2.0f;
+    if (!(lo == lo) || !(hi == hi)) return;
+    const int s = (int)fminf(fmaxf(floorf(lo), 0.0f), (float)d.N);
+    const int e = (int)fminf(fmaxf(ceilf(hi) + 1.0f, (float)s), (float)d.N);
+    if (s > 0) {
+        const float u = row_u(g, d, s - 1, yp);
+        if (!(up ? u < lo_u : u > hi_u)) return;
+    }
+    if (e < d.N) {
+        const float u = row_u(g, d, e, yp);
+        if (!(up ? u > hi_u : u < lo_u)) return;
+    }
+    xs = s;
+    xe = e;
+}
+
+// One thread per (code, band, volume row): the scan bounds of forward_item.
+// A candidate of band b has iu in [16b - 1, 16b + 16), i.e. u in
+// [16b - 0.5, 16b + 15.5), inside [ua, ub] below.
+__global__ void k_band_rows(const ViewGeom* __restrict__ vg, const Dims d, int n_codes, int n_bands,
+                            int32_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n_codes * n_bands * d.N) return;
+    const int y = (int)(i % d.N), band = (int)((i / d.N) % n_bands), code = (int)(i / ((int64_t)d.N * n_bands));
+    const ViewGeom g = vg[code];
+    int xs = 0, xe = d.N;
+    if (g.axis != 0.f) {
+        const float u0 = (float)(band * kBandU);
+        band_columns(g, d, (float)y - d.xc, u0 - 0.5f, u0 + (float)kBandU + 0.5f, xs, xe);
+    }
+    out[i] = xs | (xe << 16);
+}
+
 // One (band, view) work item o = band * n_slots + slot of a forward launch.
 // step selects the view list views + step*n_slots (a solver's subset step);
 // loss receives the item's share of sum |y - p|.
@@ -441,9 +517,16 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const int spare_row = min(lane, vb);  // rows spare_row, spare_row + 1 lie in the band
     const bool rows_all = v0 <= 0 && v0 + vb >= d.V;  // this launch covers every detector row
 
+    const auto bands = to_global(a.bands ? a.bands + ((int64_t)code * a.n_bands + band) * d.N : nullptr);
     for (int y = wave; y < d.N; y += WAVES) {
         const float yp = (float)y - d.xc;
-        for (int x0 = 0; x0 < d.N; x0 += 64) {
+        int xs = 0, xe = d.N;  // columns of this row to scan (the test below is exact)
+        if (AXIS && a.bands) {
+            const int v = uniformi(bands[y]);
+            xs = v & 0xffff;
+            xe = v >> 16;
+        }
+        for (int x0 = xs & ~63; x0 < xe; x0 += 64) {
             const int x = x0 + lane;
             bool cand = false;
             // untilted: this lane's column terms, kept for the candidate loop (which
@@ -2144,6 +2227,14 @@ hipError_t launch_pca(const float* mu, int K, const Basis& b, int64_t nvox, int6
                       float* evalue, hipStream_t stream) {
     if (nvox == 0) return hipSuccess;
     hipLaunchKernelGGL(k_pca, dim3(grid1(nvox, 256)), dim3(256), 0, stream, mu, K, b, nvox, chan_stride, md, evalue);
+    return hipGetLastError();
+}
+
+hipError_t launch_band_rows(const ViewGeom* vg, const Dims& d, int n_codes, int n_bands, int32_t* out,
+                            hipStream_t stream) {
+    const int64_t n = (int64_t)n_codes * n_bands * d.N;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_band_rows, dim3(grid1(n, 256)), dim3(256), 0, stream, vg, d, n_codes, n_bands, out);
     return hipGetLastError();
 }
 

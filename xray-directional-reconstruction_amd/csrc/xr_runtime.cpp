@@ -177,6 +177,7 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
     a.proj = proj;
     a.sino = sino;
     a.norm = p->d_norm;
+    a.bands = p->tiles.fwd_scan ? nullptr : p->d_bands;
     a.code_base = code_base;
     a.g2 = p->d_g2;
     a.loss = d_loss;
@@ -255,6 +256,12 @@ int ensure_norm(xr_plan* p) {
     xr_context* ctx = p->ctx;
     const int P = p->g.n_proj;
     const int64_t lenD = (int64_t)p->g.n_det_u * p->g.n_det_v * P;
+    if (!p->d_bands && p->g.n_vox <= 32767) {  // column bounds packed in 16 bits
+        const int n_bands = (p->g.n_det_u + kBandU - 1) / kBandU;
+        XR_HIP(ctx, hipMalloc((void**)&p->d_bands,
+                              sizeof(int32_t) * (size_t)p->n_cond * P * n_bands * p->g.n_vox));
+        XR_HIP(ctx, launch_band_rows(p->d_vg, p->dims, p->n_cond * P, n_bands, p->d_bands, ctx->stream));
+    }
     XR_HIP(ctx, hipMalloc((void**)&p->d_norm, sizeof(float) * lenD * p->n_cond));
     std::vector<int32_t> codes((size_t)p->n_cond * P);
     for (size_t i = 0; i < codes.size(); i++) codes[i] = (int32_t)i;
@@ -653,6 +660,7 @@ void xr_plan_destroy(xr_plan* p) {
     if (p->d_cg) (void)hipFree(p->d_cg);
     if (p->d_g2) (void)hipFree(p->d_g2);
     if (p->d_norm) (void)hipFree(p->d_norm);
+    if (p->d_bands) (void)hipFree(p->d_bands);
     delete p;
 }
 
