@@ -137,12 +137,16 @@ float xr_default_vox_size(float sdd, float sod, float det_size);
  *             most that fit 64 KB of LDS)
  *   fwd_scan: 0 (default): the forward of an untilted view scans, per volume
  *             row, only the columns a per-plan table gives as possibly
- *             touching the detector band; 1: every column of every row */
+ *             touching the detector band; 1: every column of every row
+ *   fwd_split: detector columns of one forward work item: 1, 2, 4 split each
+ *             16-column band into that many items; 0 (default) splits only
+ *             launches with too few items to fill the device (few views) */
 typedef struct xr_tiles {
     int32_t bwd_zpl;
     int32_t bwd_gx;
     int32_t fwd_rows;
     int32_t fwd_scan;
+    int32_t fwd_split;
 } xr_tiles;
 
 int xr_plan_set_tiles(xr_plan* plan, const xr_tiles* tiles);

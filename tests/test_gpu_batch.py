@@ -138,14 +138,16 @@ def test_tiles_do_not_change_results(orc, xr, ctx, torch_dev, problems):
     y = torch.from_numpy(sino).to(torch_dev)
     codes = plan.codes(range(P))
     base_fwd = base_bwd = None
-    for rows, scan in ((0, 1), (0, 0), (64, 0), (40, 0), (17, 0), (17, 1)):
-        plan.set_tiles(fwd_rows=rows, fwd_scan=scan)
+    for rows, scan, split in ((0, 1, 1), (0, 0, 1), (0, 0, 2), (0, 0, 4), (0, 0, 0), (64, 0, 2), (40, 0, 4),
+                              (17, 0, 1), (17, 1, 4)):
+        plan.set_tiles(fwd_rows=rows, fwd_scan=scan, fwd_split=split)
         p = torch.zeros(P * U * U, dtype=torch.float32, device=torch_dev)
         plan.project(col, p, codes, out_mode=xr.OUT_RATIO, sino=y)
         ctx.synchronize()
         if base_fwd is None:
             base_fwd = p.clone()
-        assert torch.equal(p, base_fwd), "fwd_rows=%d fwd_scan=%d changed the forward projection" % (rows, scan)
+        assert torch.equal(p, base_fwd), "fwd_rows=%d fwd_scan=%d fwd_split=%d changed the forward projection" % (
+            rows, scan, split)
     for zpl in (1, 2, 4):
         for gx in (1, 2, 4, 8, 16):
             plan.set_tiles(bwd_zpl=zpl, bwd_gx=gx)

@@ -63,7 +63,7 @@ class Tiles(ctypes.Structure):
     """xr_tiles: projector tile shape (zeros select the defaults)."""
 
     _fields_ = [("bwd_zpl", ctypes.c_int32), ("bwd_gx", ctypes.c_int32), ("fwd_rows", ctypes.c_int32),
-                ("fwd_scan", ctypes.c_int32)]
+                ("fwd_scan", ctypes.c_int32), ("fwd_split", ctypes.c_int32)]
 
     def as_tuple(self):
         return (self.bwd_zpl, self.bwd_gx, self.fwd_rows)
@@ -433,8 +433,8 @@ class Plan:
         self.ctx.check(_lib.xr_backproject_update(self.ctx.h, self.h, _dptr(proj), c.ctypes.data_as(_c_int32_p),
                                                   len(c), int(xtt), method, lam, subset_size, _dptr(vol)))
 
-    def set_tiles(self, bwd_zpl=0, bwd_gx=0, fwd_rows=0, fwd_scan=0):
-        t = Tiles(bwd_zpl, bwd_gx, fwd_rows, fwd_scan)
+    def set_tiles(self, bwd_zpl=0, bwd_gx=0, fwd_rows=0, fwd_scan=0, fwd_split=0):
+        t = Tiles(bwd_zpl, bwd_gx, fwd_rows, fwd_scan, fwd_split)
         self.ctx.check(_lib.xr_plan_set_tiles(self.h, ctypes.byref(t)))
 
     def tiles(self):

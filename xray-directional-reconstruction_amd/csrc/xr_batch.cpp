@@ -38,7 +38,8 @@ namespace {
 bool valid_tiles(const xr_tiles& t) {
     const int z = t.bwd_zpl, g = t.bwd_gx;
     return (z == 0 || z == 1 || z == 2 || z == 4) && (g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16) &&
-           t.fwd_rows >= 0 && (t.fwd_scan == 0 || t.fwd_scan == 1);
+           t.fwd_rows >= 0 && (t.fwd_scan == 0 || t.fwd_scan == 1) &&
+           (t.fwd_split == 0 || t.fwd_split == 1 || t.fwd_split == 2 || t.fwd_split == 4);
 }
 
 std::array<int, 5> tune_key(const xr_plan* p) {
@@ -211,7 +212,28 @@ int build_tables(xr_batch* b) {
     b->free_tables();
     const bool mlem = b->method == XR_METHOD_MLEM;
     const int out_mode = mlem ? OUT_RATIO : OUT_RESIDUAL;
-    int max_vb = 1;
+    // Persistent grids: every block resident at once.  The forward block's LDS
+    // holds the tallest row chunk of any problem.
+    int max_vb = 1, split_tile = 0;
+    for (const auto& p : b->pr) {
+        max_vb = std::max(max_vb, std::min(forward_rows(p.plan, FWD_PLAIN), p.V));
+        if (split_tile == 0) split_tile = p.plan->tiles.fwd_split;
+    }
+    hipDeviceProp_t prop;
+    XR_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
+    b->fwd_axis_all = true;
+    for (const auto& p : b->pr) b->fwd_axis_all = b->fwd_axis_all && p.plan->axis_cond[0];
+    // past the 32-bit band and the reduction slots (one per wave of either kernel)
+    b->ticket_slot = fwd_red_slot(max_vb + 2, 4) + std::max(kFwdWaves, kFwdBatchWaves);
+    b->fwd_lds = sizeof(unsigned long long) * ((size_t)b->ticket_slot + 1);
+    const int fpc = batch_blocks_per_cu_forward(FWD_PLAIN, b->fwd_lds, b->fwd_axis_all);
+    if (fpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched forward kernel");
+    b->fwd_grid = fpc * prop.multiProcessorCount;
+    for (int c = 0; c < 3; c++) {
+        const int bpc = batch_blocks_per_cu_backward(mlem ? BWD_MLEM : BWD_ART, c == 2 ? 4 : c + 1);
+        if (bpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched backprojection");
+        b->bwd_grid[c] = bpc * prop.multiProcessorCount;
+    }
     b->steps.resize(b->maxb);
     for (int pos = 0; pos < b->maxb; pos++) {
         std::vector<FwdArgs> fi;
@@ -255,10 +277,8 @@ int build_tables(xr_batch* b) {
             for (int v0 = 0; v0 < p.V; v0 += rows) {
                 a.v0 = v0;
                 a.vb = std::min(rows, p.V - v0);
-                max_vb = std::max(max_vb, a.vb);
                 fi.push_back(a);
                 fsum += a.n_bands * a.n_slots;
-                fe.push_back(fsum);
             }
             BwdArgs w{};
             w.d = pl->dims;
@@ -278,6 +298,15 @@ int build_tables(xr_batch* b) {
             bz[w.zpl == 4 ? 2 : w.zpl - 1].push_back(w);
             segs.push_back(p.vol);
             lens.push_back(N3);
+        }
+        // Few items in this step (small subsets): split the bands so that the
+        // persistent grid is filled.
+        const int split = forward_split(fsum, b->fwd_grid, split_tile);
+        fsum = 0;
+        for (FwdArgs& a : fi) {
+            a.n_sub = split;
+            fsum += a.n_bands * split * a.n_slots;
+            fe.push_back(fsum);
         }
         xr_batch::Step& s = b->steps[pos];
         s.n_fwd = (int)fi.size();
@@ -311,22 +340,6 @@ int build_tables(xr_batch* b) {
         XR_HIP(ctx, hipMemcpy(s.bwd_end, be.data(), sizeof(int) * be.size(), hipMemcpyHostToDevice));
         XR_HIP(ctx, hipMemcpy(s.segs, segs.data(), sizeof(float*) * segs.size(), hipMemcpyHostToDevice));
         XR_HIP(ctx, hipMemcpy(s.seg_len, lens.data(), sizeof(int64_t) * lens.size(), hipMemcpyHostToDevice));
-    }
-    // Persistent grids: every block resident at once.
-    hipDeviceProp_t prop;
-    XR_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
-    b->fwd_axis_all = true;
-    for (const auto& p : b->pr) b->fwd_axis_all = b->fwd_axis_all && p.plan->axis_cond[0];
-    // past the 32-bit band and the reduction slots (one per wave of either kernel)
-    b->ticket_slot = fwd_red_slot(max_vb + 2, 4) + std::max(kFwdWaves, kFwdBatchWaves);
-    b->fwd_lds = sizeof(unsigned long long) * ((size_t)b->ticket_slot + 1);
-    const int fpc = batch_blocks_per_cu_forward(FWD_PLAIN, b->fwd_lds, b->fwd_axis_all);
-    if (fpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched forward kernel");
-    b->fwd_grid = fpc * prop.multiProcessorCount;
-    for (int c = 0; c < 3; c++) {
-        const int bpc = batch_blocks_per_cu_backward(mlem ? BWD_MLEM : BWD_ART, c == 2 ? 4 : c + 1);
-        if (bpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched backprojection");
-        b->bwd_grid[c] = bpc * prop.multiProcessorCount;
     }
     b->tables_ok = true;
     return XR_OK;
@@ -373,8 +386,9 @@ extern "C" {
 
 int xr_plan_set_tiles(xr_plan* plan, const xr_tiles* tiles) {
     if (!plan || !tiles) return XR_ERR_INVALID;
-    if (!valid_tiles(*tiles)) return fail(plan->ctx, XR_ERR_INVALID, "invalid tile shape (zpl %d, gx %d, rows %d, scan %d)",
-                                          tiles->bwd_zpl, tiles->bwd_gx, tiles->fwd_rows, tiles->fwd_scan);
+    if (!valid_tiles(*tiles)) return fail(plan->ctx, XR_ERR_INVALID, "invalid tile shape (zpl %d, gx %d, rows %d, scan %d, split %d)",
+                                          tiles->bwd_zpl, tiles->bwd_gx, tiles->fwd_rows, tiles->fwd_scan,
+                                          tiles->fwd_split);
     plan->tiles = *tiles;
     return XR_OK;
 }
@@ -388,6 +402,7 @@ int xr_plan_get_tiles(const xr_plan* plan, xr_tiles* tiles) {
     tiles->bwd_gx = a.gx;
     tiles->fwd_rows = forward_rows(plan, FWD_PLAIN);
     tiles->fwd_scan = plan->tiles.fwd_scan;
+    tiles->fwd_split = plan->tiles.fwd_split;
     return XR_OK;
 }
 

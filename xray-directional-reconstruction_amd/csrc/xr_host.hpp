@@ -15,6 +15,7 @@
 
 struct xr_context {
     int device = 0;
+    int n_cu = 0;  // compute units of the device
     hipStream_t stream = nullptr;
     std::string err;
     int32_t* d_codes = nullptr;  // per-call view lists

@@ -104,6 +104,7 @@ struct FwdArgs {
     int n_slots;
     int v0, vb;              // detector rows [v0, v0+vb) of the first row chunk (chunk height vb)
     int n_chunks;            // row chunks in one launch: chunk c owns rows v0 + c*vb + [0, vb) (clipped to V)
+    int n_sub;               // 1, 2, 4: a work item owns kBandU / n_sub detector columns of its band
     int axis;                // batched launches: 1 -> every view is axis aligned (ViewGeom::axis)
     const int32_t* bands;    // untilted views: [code][band][y] columns xs | xe << 16 to scan, or null (all)
 };
@@ -152,6 +153,19 @@ hipError_t launch_backward_batch(const BwdArgs* items, const int* item_end, int 
 // Resident blocks per CU for the batched kernels (0 on error).
 // axis_all: every item untilted (the untilted-only kernel, kFwdWaves waves per block)
 int batch_blocks_per_cu_forward(int kind, size_t lds_bytes, bool axis_all);
+// Dynamic LDS of a standalone forward launch of vb detector rows.
+size_t forward_lds_bytes(int kind, int vb);
+// Resident blocks per CU of the standalone forward kernel.
+int forward_blocks_per_cu(int kind, bool axis, size_t lds_bytes);
+// Sub-bands per detector band (1, 2, 4) for a launch of `items` band items on
+// `slots` resident blocks: split while the items would leave the chip under-
+// filled (a forced value `tile` > 0 wins).
+inline int forward_split(int64_t items, int64_t slots, int tile) {
+    if (tile > 0) return tile;
+    int s = 1;
+    while (s < 4 && items * s * 2 < slots * 3) s *= 2;
+    return s;
+}
 int batch_blocks_per_cu_backward(int kind, int zpl);
 size_t backward_batch_lds_bytes();
 // out[i] = max |segs[i][0, seg_len[i])| (out zeroed beforehand).

@@ -479,7 +479,20 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     typedef typename FwdAcc<KIND>::S AccS;
     const lds_ptr<AccT> acc = (lds_ptr<AccT>)acc_raw;
     const Dims& d = a.d;
-    const int band = o / a.n_slots, slot = o % a.n_slots;
+    // Item o = (sub-band j, slot).  Sub-band j = (band, sub) owns the kBandU/n_sub
+    // detector columns us + [0, W) of the band's accumulator; a split launch
+    // hands out the central (heaviest) sub-bands first.
+    static_assert(kBandU == 16, "sub-band shifts assume 16-column bands");
+    const int nsub = a.n_sub > 1 ? a.n_sub : 1;
+    const int lw = nsub == 4 ? 2 : (nsub == 2 ? 3 : 4);  // log2 of W
+    int j = o / a.n_slots;
+    const int slot = o % a.n_slots;
+    if (nsub > 1) {
+        const int mid = (a.n_bands * nsub) >> 1;
+        j = (j & 1) ? mid - ((j + 1) >> 1) : mid + (j >> 1);
+    }
+    const int band = j >> (4 - lw), sub = j & (nsub - 1);
+    const int W = 1 << lw, cs = sub << lw;  // the item's columns cs + [0, W) of the band
     const auto views = to_global(a.views + (int64_t)step * a.n_slots);
     const auto accum = to_global(a.accum ? a.accum + (int64_t)step * a.n_slots : nullptr);
     const auto proj = to_global(a.proj);
@@ -506,7 +519,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     }
     const float scalev = (KIND == FWD_PLAIN) ? d.ratio : 1.f;
     const float zlo = -d.xc, zhi = (float)(d.N - 1) - d.xc;
-    const float band_lo = (float)u0 - 0.5f, band_hi = (float)(u0 + kBandU) + 0.5f;
+    const float band_lo = (float)(u0 + cs) - 0.5f, band_hi = (float)(u0 + cs + W) + 0.5f;
     // untilted path: lane offsets of the z chunks, and the window of t = v - 0.5:
     // 0.55 < v < V - 0.55 (ir.cu:1501) and v0 - 1 <= floor(t) < v0 + vb
     float lanef[kFwdChunks];
@@ -543,7 +556,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                         int iu;
                         float fu;
                         split(u, iu, fu);
-                        cand = (iu >= u0 - 1) && (iu < u0 + kBandU);
+                        cand = (unsigned)(iu - (u0 + cs - 1)) <= (unsigned)W;
                         sB = g.av[2] * inv;
                         sA = fmaf(c.nv, inv, d.hv - 0.5f);
                         sw0 = (1.0f - fu) * (scale * scalev);
@@ -742,8 +755,8 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const int64_t img = (int64_t)(code - a.code_base) * d.V * d.U;
     const int64_t nimg = (int64_t)code * d.V * d.U;
     const bool acc_old = (KIND != FWD_WEIGHT) && accum && accum[slot];
-    for (int i = threadIdx.x; i < vb * kBandU; i += 64 * WAVES) {
-        const int r = i / kBandU, cc = i % kBandU;
+    for (int i = threadIdx.x; i < (vb << lw); i += 64 * WAVES) {
+        const int r = i >> lw, cc = cs + (i & (W - 1));
         const int uu = u0 + cc, vv = v0 + r;
         if (uu >= d.U || vv >= d.V) continue;
         const float s = (float)(AccS)acc[(r + 1) * kBandStride + cc + 1] * unscale;
@@ -797,7 +810,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
 template <int KIND, bool AXIS>
 __global__ __launch_bounds__(64 * kFwdWaves) void k_forward(const FwdArgs a) {
     extern __shared__ unsigned long long acc_raw[];
-    const int per_chunk = a.n_bands * a.n_slots;
+    const int per_chunk = a.n_bands * max(a.n_sub, 1) * a.n_slots;
     const int nblk = per_chunk * a.n_chunks;
     const int per_xcd = (nblk + 7) / 8;
     // XCD-chunked order: each XCD walks a contiguous run of (row chunk, band,
@@ -2019,11 +2032,10 @@ inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs);
 }  // namespace
 
 hipError_t launch_forward(const FwdArgs& a, int kind, bool axis, hipStream_t stream) {
-    const int nblk = a.n_bands * a.n_slots * a.n_chunks;
+    const int nblk = a.n_bands * std::max(a.n_sub, 1) * a.n_slots * a.n_chunks;
     if (nblk == 0) return hipSuccess;
     const unsigned grid = (unsigned)(8 * ((nblk + 7) / 8));
-    const size_t acc_bytes = (kind == FWD_WEIGHT) ? 8 : 4;
-    const size_t lds = ((size_t)fwd_red_slot(a.vb + 2, (int)acc_bytes) + kFwdWaves) * sizeof(unsigned long long);
+    const size_t lds = forward_lds_bytes(kind, a.vb);
 #define XR_FWD(KD, A) hipLaunchKernelGGL((k_forward<KD, A>), dim3(grid), dim3(64 * kFwdWaves), lds, stream, a)
     if (kind == FWD_WEIGHT) {
         if (axis) XR_FWD(FWD_WEIGHT, true); else XR_FWD(FWD_WEIGHT, false);
@@ -2162,6 +2174,21 @@ int batch_blocks_per_cu_forward(int kind, size_t lds_bytes, bool axis_all) {
                                                                 64 * kFwdWaves, lds_bytes)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_forward_batch<FWD_PLAIN>,
                                                                 64 * kFwdBatchWaves, lds_bytes);
+    return e == hipSuccess ? n : 0;
+}
+
+size_t forward_lds_bytes(int kind, int vb) {
+    const int acc_bytes = (kind == FWD_WEIGHT) ? 8 : 4;
+    return ((size_t)fwd_red_slot(vb + 2, acc_bytes) + kFwdWaves) * sizeof(unsigned long long);
+}
+
+int forward_blocks_per_cu(int kind, bool axis, size_t lds_bytes) {
+    int n = 0;
+    const void* k = nullptr;
+    if (kind == FWD_WEIGHT) k = axis ? (const void*)k_forward<FWD_WEIGHT, true> : (const void*)k_forward<FWD_WEIGHT, false>;
+    else if (kind == FWD_PLAIN) k = axis ? (const void*)k_forward<FWD_PLAIN, true> : (const void*)k_forward<FWD_PLAIN, false>;
+    else k = axis ? (const void*)k_forward<FWD_XTT, true> : (const void*)k_forward<FWD_XTT, false>;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 64 * kFwdWaves, lds_bytes);
     return e == hipSuccess ? n : 0;
 }
 

@@ -210,6 +210,16 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
     a.v0 = 0;
     a.vb = rows;
     a.n_chunks = (V + rows - 1) / rows;
+    // Few views (an ordered subset) leave most CUs idle: split the bands.
+    const int64_t items = (int64_t)a.n_bands * a.n_slots * a.n_chunks;
+    a.n_sub = p->tiles.fwd_split;
+    if (a.n_sub == 0 && items < 4096) {
+        if (ctx->n_cu == 0)
+            XR_HIP(ctx, hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        const int64_t slots = (int64_t)forward_blocks_per_cu(kind, axis, forward_lds_bytes(kind, rows)) * ctx->n_cu;
+        a.n_sub = forward_split(items, slots, 0);
+    }
+    a.n_sub = std::max(a.n_sub, 1);
     return timed(ctx, 0, [&] { return launch_forward(a, kind, axis, ctx->stream); });
 }
 
