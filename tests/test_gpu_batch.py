@@ -255,3 +255,40 @@ def test_mixed_tilt_batch_equals_standalone(orc, xr, ctx, torch_dev, method):
                           scans=scans)
     for i in range(len(specs)):
         assert np.array_equal(got[i], want[i]), "problem %d differs from its standalone solve" % i
+
+
+@pytest.mark.parametrize("tilt", [0.0, 20.0])
+def test_forward_work_split_is_exact(orc, xr, ctx, torch_dev, tilt):
+    """The forward's band split (fwd_split 1, 2, 4 and the automatic choice for
+    few views) and its per-row candidate table (fwd_scan 0) change only the work
+    split: every view set projects bitwise as the unsplit whole-row scan, for
+    untilted views (table + split) and tilted ones (split)."""
+    import torch
+
+    N, P = 40, 24
+    U = (3 * N) // 2
+    vs = 0.5 * 64.0 / N
+    g = xr.make_geometry(SDD, SOD, DET, N, U, P, vox_size=vs)
+    scan = xr.Scan(cond_R=rot_x(tilt).reshape(-1) if tilt else None)
+    vol, _ = ellipsoid_phantom(orc, xr, N, vs, 77)
+    plan = ctx.plan(g, scan)
+    ref = torch.from_numpy(vol.reshape(-1)).to(torch_dev)
+    col = torch.empty_like(ref)
+    ctx.to_columns(ref, col, N)
+    y = torch.zeros(P * U * U, dtype=torch.float32, device=torch_dev)
+    plan.project(col, y, plan.codes(range(P)), out_mode=xr.OUT_NORMALISED)
+    for views in (list(range(P)), [3, 11, 17], [5]):
+        codes = plan.codes(views)
+        base = None
+        for scan_mode, split in ((1, 1), (0, 1), (0, 2), (0, 4), (0, 0), (1, 4)):
+            plan.set_tiles(fwd_scan=scan_mode, fwd_split=split)
+            p = torch.zeros(P * U * U, dtype=torch.float32, device=torch_dev)
+            loss = plan.project(col, p, codes, out_mode=xr.OUT_RATIO, sino=y)
+            ctx.synchronize()
+            if base is None:
+                base, base_loss = p.clone(), loss
+                assert float(p.abs().sum()) > 0
+            assert torch.equal(p, base), "views %s: fwd_scan=%d fwd_split=%d changed the projection" % (
+                views, scan_mode, split)
+            assert abs(loss - base_loss) <= 1e-9 * max(1.0, abs(base_loss))
+    plan.close()

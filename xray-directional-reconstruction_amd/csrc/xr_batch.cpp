@@ -242,12 +242,12 @@ int build_tables(xr_batch* b) {
         std::vector<const float*> segs;
         std::vector<int64_t> lens;
         int fsum = 0;
-        for (int k : b->order) {
+        for (int oi = 0; oi < (int)b->order.size(); oi++) {
+            const int k = b->order[oi];
             xr_batch::Problem& p = b->pr[k];
             if (pos >= p.batch) continue;
             xr_plan* pl = p.plan;
             const int64_t N3 = (int64_t)p.N * p.N * p.N;
-            const int slot = (int)segs.size();  // max |voxel| of this step's active problem `slot`
             FwdArgs a{};
             a.d = pl->dims;
             a.b = pl->bas;
@@ -264,7 +264,10 @@ int build_tables(xr_batch* b) {
             a.bands = pl->tiles.fwd_scan ? nullptr : pl->d_bands;
             a.g2 = pl->d_g2;
             a.loss = b->d_loss + k;
-            a.vmax = b->d_vmax + slot;
+            // max |voxel| of problem oi: measured by k_absmax_seg at the first
+            // subset step (every problem is active there, slot == oi), then by
+            // the previous step's backprojection (which updates every voxel)
+            a.vmax = b->d_vmax + oi;
             a.wmul = pl->dims.ratio * 1.0001f;  // as forward_launch
             a.fsum = pl->max_factor * 1.01f + 1.0f;
             a.code_base = 0;
@@ -293,6 +296,7 @@ int build_tables(xr_batch* b) {
             w.g2 = pl->d_g2;
             w.alpha = mlem ? 0.f : b->lambda / (float)p.ss;  // reconstruct.cu:124
             w.axis = pl->axis_cond[0] ? 1 : 0;
+            w.vmax_out = (pos + 1 < p.batch) ? reinterpret_cast<unsigned*>(b->d_vmax + oi) : nullptr;
             if (!backward_shape(w, pl->tiles.bwd_zpl, pl->tiles.bwd_gx, false))
                 return fail(ctx, XR_ERR_INVALID, "bad tile shape for problem %d", k);
             bz[w.zpl == 4 ? 2 : w.zpl - 1].push_back(w);
@@ -357,12 +361,23 @@ int enqueue_epochs(xr_batch* b, int epochs) {
             const xr_batch::Step& s = b->steps[pos];
             if (s.n_fwd == 0) continue;
             const int step = ep * b->maxb + pos;
-            e = launch_fill(b->d_vmax, b->n, 0.0f, st);
-            if (e == hipSuccess) e = launch_absmax_seg(s.segs, s.seg_len, b->d_vmax, s.n_active, s.absmax_blocks, st);
-            if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "absmax failed: %s", hipGetErrorString(e));
+            // max |voxel| of each problem: read from the volumes at the first
+            // step (the caller may have replaced them between runs), then left
+            // by each step's backprojection for the next (cleared once the
+            // forward has read it)
+            if (pos == 0) {
+                e = launch_fill(b->d_vmax, b->n, 0.0f, st);
+                if (e == hipSuccess)
+                    e = launch_absmax_seg(s.segs, s.seg_len, b->d_vmax, s.n_active, s.absmax_blocks, st);
+                if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "absmax failed: %s", hipGetErrorString(e));
+            }
             e = launch_forward_batch(s.fwd, s.fwd_end, s.n_fwd, b->d_queue, b->ticket_slot, b->fwd_lds, b->fwd_grid,
                                      step, ep * b->n, FWD_PLAIN, b->fwd_axis_all, st);
             if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "batched forward failed: %s", hipGetErrorString(e));
+            if (pos + 1 < b->maxb) {
+                e = launch_fill(b->d_vmax, b->n, 0.0f, st);
+                if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "fill failed: %s", hipGetErrorString(e));
+            }
             // one persistent launch per z depth class present (in-order on the stream)
             for (int c = 2; c >= 0; c--) {
                 if (s.bwd_count[c] == 0) continue;

@@ -127,6 +127,7 @@ struct BwdArgs {
     int zpl, gx, gy;          // tile shape: z voxels per lane (1, 2, 4), column groups along x, y (gx*gy = 16)
     int tiles_x, tiles_y, tiles_z, n_tiles, per_xcd;
     int axis;                 // batched launches: 1 -> every view is axis aligned
+    unsigned* vmax_out;       // batched MLEM / ART: atomic max of |updated voxel| (float bits), or null
 };
 
 // Tile shape of the plain backprojection (CPW = 2 columns per wave side).

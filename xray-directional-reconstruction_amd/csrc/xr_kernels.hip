@@ -980,12 +980,14 @@ __device__ __forceinline__ Foot foot_from_extent(float umn, float umx, float vmn
 // Fused update of a backprojection tile: every voxel of the lane's columns is
 // read and written once (raw accumulate, MLEM ratio, ART step or FDK sum).
 template <int KIND, int CPW, int ZPL, int KA>
-__device__ __forceinline__ void backward_update(const BwdArgs& a, const float (&tmp)[CPW * CPW][ZPL][KA],
-                                                const float (&fac)[CPW * CPW][ZPL][KA], const int (&nwhole)[CPW * CPW],
-                                                int xb, int yb, int z0, int lane) {
+__device__ __forceinline__ float backward_update(const BwdArgs& a, const float (&tmp)[CPW * CPW][ZPL][KA],
+                                                 const float (&fac)[CPW * CPW][ZPL][KA], const int (&nwhole)[CPW * CPW],
+                                                 int xb, int yb, int z0, int lane) {
+    // returns max |updated voxel| of this lane (MLEM / ART kinds)
     constexpr bool XTT = IsXtt<KIND>::value;
     const Dims& d = a.d;
     const auto vol = to_global(a.vol);
+    float vmx = 0.f;
 #pragma unroll
     for (int j = 0; j < CPW; j++) {
 #pragma unroll
@@ -1008,9 +1010,13 @@ __device__ __forceinline__ void backward_update(const BwdArgs& a, const float (&
                         to_global(a.factor)[vi] += fc;
                     } else if (KIND == BWD_MLEM || KIND == BWD_XTT_MLEM) {
                         const float mu = vol[vi];
-                        vol[vi] = (fc == 0.0f) ? 0.0f : mu * t / fc;
+                        const float nv = (fc == 0.0f) ? 0.0f : mu * t / fc;
+                        vol[vi] = nv;
+                        vmx = fmaxf(vmx, fabsf(nv));
                     } else if (KIND == BWD_ART || KIND == BWD_XTT_ART) {
-                        vol[vi] = vol[vi] + a.alpha * t;
+                        const float nv = vol[vi] + a.alpha * t;
+                        vol[vi] = nv;
+                        vmx = fmaxf(vmx, fabsf(nv));
                     } else {  // BWD_FDK
                         vol[vi] = a.overwrite ? t : vol[vi] + t;
                     }
@@ -1018,6 +1024,7 @@ __device__ __forceinline__ void backward_update(const BwdArgs& a, const float (&
             }
         }
     }
+    return vmx;
 }
 
 // ---------------------------------------------------------------------------
@@ -1044,7 +1051,7 @@ static_assert(64 % kTermCols == 0 && 64 / kTermCols >= kBwdBatch, "one lane per 
 typedef __attribute__((address_space(3))) int lds_int;
 
 template <int KIND, int ZPL>
-__device__ __forceinline__ void backward_item_untilted(const BwdArgs& a, int o, int step, float* stage) {
+__device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o, int step, float* stage) {
     constexpr int CPW = 2, NC = CPW * CPW;
     static_assert(NC == kTermCols, "term layout");
     const Dims& d = a.d;
@@ -1060,7 +1067,7 @@ __device__ __forceinline__ void backward_item_untilted(const BwdArgs& a, int o, 
     const float by0 = (float)(ty * bcy) - d.xc, by1 = (float)min(ty * bcy + bcy - 1, d.N - 1) - d.xc;
     const float bz0 = (float)z0 - d.xc, bz1 = (float)min(z0 + 64 * ZPL - 1, d.N - 1) - d.xc;
     const int nv = a.n_views;
-    if (nv == 0) return;
+    if (nv == 0) return 0.f;
 
     float tmp[NC][ZPL][1], fac[NC][ZPL][1];
 #pragma unroll
@@ -1324,19 +1331,19 @@ __device__ __forceinline__ void backward_item_untilted(const BwdArgs& a, int o, 
         }
         __syncthreads();
     }
-    backward_update<KIND, CPW, ZPL, 1>(a, tmp, fac, nwhole, xb, yb, z0, lane);
+    return backward_update<KIND, CPW, ZPL, 1>(a, tmp, fac, nwhole, xb, yb, z0, lane);
 }
 
 // One voxel tile o of a backprojection launch; step selects the view list
 // views + step*n_views (a solver's subset step).  stage: 2 x kStageBuf floats.
+// Returns the lane's max |updated voxel| (MLEM / ART kinds).
 template <int KIND, bool AXIS, int CPW, int ZPL, int KM>
-__device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step, float* stage) {
+__device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step, float* stage) {
     constexpr bool XTT = IsXtt<KIND>::value;
     constexpr int NC = CPW * CPW;
     constexpr int KA = XTT ? KM : 1;
     if constexpr (AXIS && !XTT && CPW == 2) {
-        backward_item_untilted<KIND, ZPL>(a, o, step, stage);
-        return;
+        return backward_item_untilted<KIND, ZPL>(a, o, step, stage);
     }
     const Dims& d = a.d;
     const auto views = to_global(a.views + (int64_t)step * a.n_views);
@@ -1402,7 +1409,7 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
             if (pidx[k] >= 0) buf[pidx[k]] = pre[k];
     };
 
-    if (a.n_views == 0) return;
+    if (a.n_views == 0) return 0.f;
     // view-independent per-lane z terms
     float zpq[ZPL];
 #pragma unroll
@@ -1621,7 +1628,7 @@ __device__ __forceinline__ void backward_item(const BwdArgs& a, int o, int step,
         }
         __syncthreads();
     }
-    backward_update<KIND, CPW, ZPL, KA>(a, tmp, fac, nwhole, xb, yb, z0, lane);
+    return backward_update<KIND, CPW, ZPL, KA>(a, tmp, fac, nwhole, xb, yb, z0, lane);
 }
 
 template <int KIND, bool AXIS, int CPW, int ZPL, int KM>
@@ -1629,7 +1636,7 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward(const BwdArgs a) {
     extern __shared__ float stage[];  // 2 x kStageBuf
     const int o = (blockIdx.x % 8) * a.per_xcd + blockIdx.x / 8;
     if (o >= a.n_tiles) return;
-    backward_item<KIND, AXIS, CPW, ZPL, KM>(a, o, 0, stage);
+    (void)backward_item<KIND, AXIS, CPW, ZPL, KM>(a, o, 0, stage);
 }
 
 // Persistent backprojection over many problems (plain MLEM / ART kinds): the
@@ -1642,10 +1649,24 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward_batch(const BwdArgs* _
     extern __shared__ float stage[];  // 2 x kStageBuf; the last word holds the ticket
     typedef __attribute__((address_space(3))) unsigned lds_uint;
     lds_uint* const tk = (lds_uint*)stage + (2 * kStageBuf - 1);
+    // past the staging and term areas: the last tile's per-wave max |voxel| and its item
+    lds_float* const wmax = (lds_float*)(stage + 2 * kStageBuf + kBwdExtraWords);
+    lds_int* const witem = (lds_int*)(wmax + kBwdWaves);
     const int total = item_end[n_items - 1];
+    if (threadIdx.x == 0) *witem = -1;
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) *tk = take_ticket(queue, (unsigned)total);
+        if (threadIdx.x == 0) {
+            // the previous tile's max |updated voxel| -> its problem's next-step bound
+            const int pi = *witem;
+            unsigned* const dst = pi >= 0 ? items[pi].vmax_out : nullptr;
+            if (dst) {
+                float m = 0.f;
+                for (int w = 0; w < kBwdWaves; w++) m = fmaxf(m, wmax[w]);
+                __hip_atomic_fetch_max(dst, __float_as_uint(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            *tk = take_ticket(queue, (unsigned)total);
+        }
         __syncthreads();
         // uniform across the block: keeps the item's arguments in scalar registers
         const int t = __builtin_amdgcn_readfirstlane((int)*tk);
@@ -1653,10 +1674,13 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward_batch(const BwdArgs* _
         const int i = find_item(item_end, n_items, t);
         const BwdArgs& a = items[i];
         const int o = t - (i ? item_end[i - 1] : 0);
-        if (a.axis)
-            backward_item<KIND, true, 2, ZPL, 1>(a, o, step, stage);
-        else
-            backward_item<KIND, false, 2, ZPL, 1>(a, o, step, stage);
+        float m = a.axis ? backward_item<KIND, true, 2, ZPL, 1>(a, o, step, stage)
+                         : backward_item<KIND, false, 2, ZPL, 1>(a, o, step, stage);
+        if (a.vmax_out) {
+            for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+            if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+            if (threadIdx.x == 0) *witem = i;
+        }
     }
 }
 
@@ -2164,7 +2188,7 @@ hipError_t launch_backward_batch(const BwdArgs* items, const int* item_end, int 
     return hipLaunchKernel(k, dim3(grid), dim3(kBwdThreads), args, backward_batch_lds_bytes(), stream);
 }
 
-size_t backward_batch_lds_bytes() { return (2 * kStageBuf + kBwdExtraWords) * sizeof(float); }
+size_t backward_batch_lds_bytes() { return (2 * kStageBuf + kBwdExtraWords + kBwdWaves + 1) * sizeof(float); }
 
 int batch_blocks_per_cu_forward(int kind, size_t lds_bytes, bool axis_all) {
     int n = 0;
