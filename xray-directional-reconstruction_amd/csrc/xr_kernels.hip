@@ -815,8 +815,10 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_forward(const FwdArgs a) {
     const int per_xcd = (nblk + 7) / 8;
     // XCD-chunked order: each XCD walks a contiguous run of (row chunk, band,
     // view) triples, band-major within a chunk, so concurrent blocks on one L2
-    // share a band at adjacent views.
-    const int o = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    // share a band at adjacent views.  A split launch (few views) lists its
+    // heaviest items first; there blocks take items in launch order, so that
+    // consecutive (round-robin dispatched) blocks spread them over the XCDs.
+    const int o = a.n_sub > 1 ? (int)blockIdx.x : (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (o >= nblk) return;
     const int chunk = o / per_chunk;
     const int v0 = a.v0 + chunk * a.vb;
