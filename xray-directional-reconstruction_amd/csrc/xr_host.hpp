@@ -112,11 +112,15 @@ int timed(xr_context* ctx, int kind, F&& launch) {
 
 // vol_cond_stride: plain volumes of several conditions side by side (N^3), or 0
 // for a single volume; code_base: proj/sino hold the images of codes >= code_base.
+// vmax_known: device max |voxel| of vol already measured (the solver's
+// previous backprojection), or null to measure it here.
 int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stride, float* proj, const float* sino,
                    int code_base, const int32_t* d_codes, const uint8_t* d_flags, int n_slots, int out_mode,
-                   double* d_loss);
+                   double* d_loss, const float* vmax_known = nullptr);
+// vmax_out (MLEM / ART kinds): receives the atomic max of |updated voxel| as
+// float bits; the caller clears it first.
 int backward_launch(xr_plan* p, int kind, const float* proj, int code_base, const int32_t* d_codes, int n_views,
-                    float* vol, float* tmp, float* factor, float alpha, int overwrite);
+                    float* vol, float* tmp, float* factor, float alpha, int overwrite, unsigned* vmax_out = nullptr);
 // Per-pixel L/(factor + 1e-5) table of the plan (computed once).
 int ensure_norm(xr_plan* p);
 // Detector rows per forward row chunk for the plan's tile setting and the

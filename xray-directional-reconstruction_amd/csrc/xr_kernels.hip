@@ -1638,7 +1638,21 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward(const BwdArgs a) {
     extern __shared__ float stage[];  // 2 x kStageBuf
     const int o = (blockIdx.x % 8) * a.per_xcd + blockIdx.x / 8;
     if (o >= a.n_tiles) return;
-    (void)backward_item<KIND, AXIS, CPW, ZPL, KM>(a, o, 0, stage);
+    float m = backward_item<KIND, AXIS, CPW, ZPL, KM>(a, o, 0, stage);
+    if constexpr (KIND == BWD_MLEM || KIND == BWD_ART) {
+        if (a.vmax_out) {
+            // the tile's max |updated voxel|: one atomic per block
+            lds_float* const wmax = (lds_float*)(stage + 2 * kStageBuf + kBwdExtraWords);
+            for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+            if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                float t = 0.f;
+                for (int w = 0; w < kBwdWaves; w++) t = fmaxf(t, wmax[w]);
+                __hip_atomic_fetch_max(a.vmax_out, __float_as_uint(t), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
 }
 
 // Persistent backprojection over many problems (plain MLEM / ART kinds): the
@@ -2103,8 +2117,9 @@ hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream) {
     if (!backward_shape(a, a.zpl, a.gx, xtt)) return hipErrorInvalidValue;
     const int zpl = a.zpl;
     const unsigned grid = (unsigned)(8 * a.per_xcd);
-    // plain kinds: staging buffers + per-view term areas; XTT: staging buffers only
-    const size_t lds = (2 * kStageBuf + (xtt ? 0 : kBwdExtraWords)) * sizeof(float);
+    // plain kinds: staging buffers, per-view term areas and the per-wave max
+    // slots; XTT: staging buffers only
+    const size_t lds = (2 * kStageBuf + (xtt ? 0 : kBwdExtraWords + kBwdWaves)) * sizeof(float);
 #define XR_BWD(KD, A, C, Z, KM) \
     hipLaunchKernelGGL((k_backward<KD, A, C, Z, KM>), dim3(grid), dim3(kBwdThreads), lds, stream, a)
 #define XR_BWD_AX(KD, C, Z, KM)         \

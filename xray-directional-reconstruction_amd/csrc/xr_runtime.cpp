@@ -161,7 +161,7 @@ hipEvent_t get_event(xr_context* ctx) {
 // for a single volume; code_base: proj/sino hold the images of codes >= code_base.
 int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stride, float* proj, const float* sino,
                    int code_base, const int32_t* d_codes, const uint8_t* d_flags, int n_slots, int out_mode,
-                   double* d_loss) {
+                   double* d_loss, const float* vmax_known) {
     xr_context* ctx = p->ctx;
     FwdArgs a{};
     a.d = p->dims;
@@ -194,10 +194,14 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
     } else {
         XR_TRY(ensure_norm(p));
         a.fsum = p->max_factor * 1.01f + 1.0f;
-        const int64_t nvals = (kind == FWD_XTT) ? N3 * p->n_basis : (vol_cond_stride ? N3 * p->n_cond : N3);
-        XR_HIP(ctx, hipMemsetAsync(ctx->d_scal + 1, 0, sizeof(float), ctx->stream));
-        XR_TRY(timed(ctx, 3, [&] { return launch_absmax(vol, nvals, ctx->d_scal + 1, ctx->stream); }));
-        a.vmax = ctx->d_scal + 1;
+        if (vmax_known) {
+            a.vmax = vmax_known;
+        } else {
+            const int64_t nvals = (kind == FWD_XTT) ? N3 * p->n_basis : (vol_cond_stride ? N3 * p->n_cond : N3);
+            XR_HIP(ctx, hipMemsetAsync(ctx->d_scal + 1, 0, sizeof(float), ctx->stream));
+            XR_TRY(timed(ctx, 3, [&] { return launch_absmax(vol, nvals, ctx->d_scal + 1, ctx->stream); }));
+            a.vmax = ctx->d_scal + 1;
+        }
         a.wmul = (kind == FWD_XTT) ? p->dims.vs * p->dims.ratio * p->xtt_wsum * 1.0001f : p->dims.ratio * 1.0001f;
     }
     // All conditions in one launch share the axis specialisation only if all are axis-aligned.
@@ -237,7 +241,7 @@ int forward_rows(const xr_plan* p, int kind) {
 }
 
 int backward_launch(xr_plan* p, int kind, const float* proj, int code_base, const int32_t* d_codes, int n_views,
-                    float* vol, float* tmp, float* factor, float alpha, int overwrite) {
+                    float* vol, float* tmp, float* factor, float alpha, int overwrite, unsigned* vmax_out) {
     xr_context* ctx = p->ctx;
     BwdArgs a{};
     a.d = p->dims;
@@ -254,6 +258,7 @@ int backward_launch(xr_plan* p, int kind, const float* proj, int code_base, cons
     a.g2 = p->d_g2;
     a.alpha = alpha;
     a.overwrite = overwrite;
+    a.vmax_out = (kind == BWD_MLEM || kind == BWD_ART) ? vmax_out : nullptr;
     a.zpl = p->tiles.bwd_zpl;
     a.gx = p->tiles.bwd_gx;
     bool axis = true;
@@ -881,6 +886,8 @@ int xr_ir_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, f
     XR_HIP(ctx, dvol.alloc(N3));
     XR_HIP(ctx, dsino.alloc(lenD));
     XR_HIP(ctx, dproj.alloc(lenD));
+    DevBuf<float> dvmax;
+    XR_HIP(ctx, dvmax.alloc(1));
     const size_t per_epoch = (size_t)batch * ss;
     XR_HIP(ctx, dcodes.alloc(per_epoch * std::max(epochs, 1)));
     XR_HIP(ctx, dflags.alloc(per_epoch * std::max(epochs, 1)));
@@ -919,14 +926,20 @@ int xr_ir_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, f
         XR_TRY(xr_volume_to_columns(ctx, dref.p, dvol.p, N, 1));
         // One condition at a time: its images hold codes c*P .. c*P+P-1.
         XR_HIP(ctx, hipMemcpyAsync(dsino.p, sino[c], sizeof(float) * lenD, hipMemcpyHostToDevice, ctx->stream));
+        // max |voxel| for the forward's fixed-point scale: measured from the
+        // uploaded volume, then left by each backprojection for the next
+        // subset (the update writes every voxel once)
+        bool vmax_known = false;
         for (int ep = 0; ep < epochs; ep++) {
             XR_HIP(ctx, hipMemsetAsync(ctx->d_loss, 0, sizeof(double), ctx->stream));
             for (int pos = 0; pos < batch; pos++) {
                 const size_t off = (size_t)ep * per_epoch + (size_t)pos * ss;
                 XR_TRY(forward_launch(plan, FWD_PLAIN, dvol.p, 0, dproj.p, dsino.p, c * P, dcodes.p + off,
-                                      dflags.p + off, ss, out_mode, ctx->d_loss));
+                                      dflags.p + off, ss, out_mode, ctx->d_loss, vmax_known ? dvmax.p : nullptr));
+                XR_HIP(ctx, launch_fill(dvmax.p, 1, 0.0f, ctx->stream));
                 XR_TRY(backward_launch(plan, kind, dproj.p, c * P, dcodes.p + off, ss, dvol.p, nullptr, nullptr, alpha,
-                                       0));
+                                       0, reinterpret_cast<unsigned*>(dvmax.p)));
+                vmax_known = true;
             }
             if (stats && stats->losses) {
                 XR_HIP(ctx, hipMemcpyAsync(&stats->losses[(size_t)c * epochs + ep], ctx->d_loss, sizeof(double),
