@@ -257,16 +257,17 @@ def test_mixed_tilt_batch_equals_standalone(orc, xr, ctx, torch_dev, method):
         assert np.array_equal(got[i], want[i]), "problem %d differs from its standalone solve" % i
 
 
-@pytest.mark.parametrize("tilt", [0.0, 20.0])
-def test_forward_work_split_is_exact(orc, xr, ctx, torch_dev, tilt):
+@pytest.mark.parametrize("tilt,det", [(0.0, 1.5), (20.0, 1.5), (0.0, 0.75)])
+def test_forward_work_split_is_exact(orc, xr, ctx, torch_dev, tilt, det):
     """The forward's band split (fwd_split 1, 2, 4 and the automatic choice for
     few views) and its per-row candidate table (fwd_scan 0) change only the work
     split: every view set projects bitwise as the unsplit whole-row scan, for
-    untilted views (table + split) and tilted ones (split)."""
+    untilted views (table + split), tilted ones (split) and a detector narrower
+    than the volume's shadow (truncated rows)."""
     import torch
 
     N, P = 40, 24
-    U = (3 * N) // 2
+    U = int(det * N)
     vs = 0.5 * 64.0 / N
     g = xr.make_geometry(SDD, SOD, DET, N, U, P, vox_size=vs)
     scan = xr.Scan(cond_R=rot_x(tilt).reshape(-1) if tilt else None)
