@@ -293,3 +293,46 @@ def test_forward_work_split_is_exact(orc, xr, ctx, torch_dev, tilt, det):
                 views, scan_mode, split)
             assert abs(loss - base_loss) <= 1e-9 * max(1.0, abs(base_loss))
     plan.close()
+
+
+@pytest.mark.parametrize("method", ["mlem", "art"])
+def test_solver_max_bound_equals_measured(orc, xr, ctx, torch_dev, method):
+    """The solver takes max |voxel| for the forward's fixed-point scale from its
+    previous backprojection instead of a pass over the volume.  Three epochs of
+    xr_ir_reconstruct with one subset (every view, in order) equal the same
+    epochs driven through xr_project / xr_backproject_update, which measure
+    the maximum from the volume at every call, bit for bit."""
+    import torch
+
+    m = xr.METHOD_MLEM if method == "mlem" else xr.METHOD_ART
+    out_mode = xr.OUT_RATIO if method == "mlem" else xr.OUT_RESIDUAL
+    N, P = 40, 24
+    U = (3 * N) // 2
+    vs = 0.5 * 64.0 / N
+    g = xr.make_geometry(SDD, SOD, DET, N, U, P, vox_size=vs)
+    vol, _ = ellipsoid_phantom(orc, xr, N, vs, 91)
+    plan = ctx.plan(g, xr.Scan())
+    codes = plan.codes(range(P))
+    ref = torch.from_numpy(vol.reshape(-1)).to(torch_dev)
+    col = torch.empty_like(ref)
+    ctx.to_columns(ref, col, N)
+    y = torch.zeros(P * U * U, dtype=torch.float32, device=torch_dev)
+    plan.project(col, y, codes, out_mode=xr.OUT_NORMALISED)
+    ctx.synchronize()
+    epochs = 3
+    v0 = np.full(N ** 3, 0.01, np.float32)
+    solved = [v0.copy()]
+    ctx.ir_reconstruct(g, xr.Scan(), [y.cpu().numpy()], solved, epochs, 1, method=m, lam=0.5)
+    cur = torch.empty_like(ref)
+    ctx.to_columns(torch.from_numpy(v0).to(torch_dev), cur, N)
+    for _ in range(epochs):
+        p = torch.zeros(P * U * U, dtype=torch.float32, device=torch_dev)
+        plan.project(cur, p, codes, out_mode=out_mode, sino=y)
+        plan.backproject_update(p, codes, cur, method=m, lam=0.5, subset_size=P)
+    out = torch.empty_like(cur)
+    ctx.from_columns(cur, out, N)
+    ctx.synchronize()
+    got = out.cpu().numpy()
+    assert float(np.abs(got).max()) > 0
+    assert np.array_equal(got, solved[0]), "max |diff| %g" % float(np.max(np.abs(got - solved[0])))
+    plan.close()
