@@ -140,9 +140,13 @@ constexpr int kCH0 = kCT0 + 2, kCH1 = kCT1 + 2, kCH2 = kCT2 + 2;
 constexpr int kCHalo = kCH0 * kCH1 * kCH2;
 constexpr int kTapsPad = 28;
 
-template <int NT>
+// CPS: input-channel chunks staged per barrier.  Wide inputs (the decoder's
+// concatenations, 96-byte voxel records) stage two chunks at once, so that
+// both 16-byte pieces of a record are fetched together while its cache line
+// is resident, instead of the line being fetched again for the next chunk.
+template <int NT, int CPS>
 __global__ __launch_bounds__(256) void k_conv3_tiled(const ConvArgs a) {
-    __shared__ bf16x8 halo[kCHalo];
+    __shared__ bf16x8 halo[CPS * kCHalo];
     const lds_bf16x8* const hl = (const lds_bf16x8*)halo;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row = lane & 15, kg = lane >> 4;
@@ -177,36 +181,51 @@ __global__ __launch_bounds__(256) void k_conv3_tiled(const ConvArgs a) {
                         (unsigned)g2 < (unsigned)a.D2;
         hoff[j] = ok ? (((int64_t)g0 * a.D1 + g1) * a.D2 + g2) * a.in_ld : -1;
     }
-    bf16x8 pre[kPer];
-    auto fetch = [&](int cc) {
+    // stage st holds chunks st*CPS .. st*CPS + CPS-1 (chunks past the last are zero)
+    bf16x8 pre[CPS][kPer];
+    auto fetch = [&](int st) {
 #pragma unroll
-        for (int j = 0; j < kPer; j++) pre[j] = hoff[j] >= 0 ? *(const bf16x8*)(in + hoff[j] + cc * 8) : zero8();
+        for (int c = 0; c < CPS; c++) {
+            const int cc = st * CPS + c;
+#pragma unroll
+            for (int j = 0; j < kPer; j++)
+                pre[c][j] = (cc < nchunk && hoff[j] >= 0) ? *(const bf16x8*)(in + hoff[j] + cc * 8) : zero8();
+        }
     };
     auto store = [&] {
 #pragma unroll
-        for (int j = 0; j < kPer; j++)
-            if (threadIdx.x + 256 * j < kCHalo) halo[threadIdx.x + 256 * j] = pre[j];
+        for (int c = 0; c < CPS; c++)
+#pragma unroll
+            for (int j = 0; j < kPer; j++)
+                if (threadIdx.x + 256 * j < kCHalo) halo[c * kCHalo + threadIdx.x + 256 * j] = pre[c][j];
     };
+    const int nstage = (nchunk + CPS - 1) / CPS;
     fetch(0);
-    for (int cc = 0; cc < nchunk; cc++) {
-        __syncthreads();  // the previous chunk's reads are done
+    for (int st = 0; st < nstage; st++) {
+        __syncthreads();  // the previous stage's reads are done
         store();
         __syncthreads();
-        if (cc + 1 < nchunk) fetch(cc + 1);  // in flight during this chunk's MFMAs
-        const __bf16* __restrict__ wc = a.wt + cc * (kTapsPad * 8);
+        if (st + 1 < nstage) fetch(st + 1);  // in flight during this stage's MFMAs
+#pragma unroll 1
+        for (int c = 0; c < CPS; c++) {
+            const int cc = st * CPS + c;
+            if (cc >= nchunk) break;
+            const __bf16* __restrict__ wc = a.wt + cc * (kTapsPad * 8);
+            const lds_bf16x8* const hc = hl + c * kCHalo;
 #pragma unroll
-        for (int s = 0; s < kTapsPad / 4; s++) {
-            const int tap = 4 * s + kg;
-            int d0, d1, d2;
-            tap_offsets(min(tap, 26), d0, d1, d2);  // tap 27 has zero weights
-            bf16x8 A[4];
+            for (int s = 0; s < kTapsPad / 4; s++) {
+                const int tap = 4 * s + kg;
+                int d0, d1, d2;
+                tap_offsets(min(tap, 26), d0, d1, d2);  // tap 27 has zero weights
+                bf16x8 A[4];
 #pragma unroll
-            for (int m = 0; m < 4; m++) A[m] = hl[((h0 + d0) * kCH1 + h1[m] + d1) * kCH2 + h2[m] + d2];
+                for (int m = 0; m < 4; m++) A[m] = hc[((h0 + d0) * kCH1 + h1[m] + d1) * kCH2 + h2[m] + d2];
 #pragma unroll
-            for (int nt = 0; nt < NT; nt++) {
-                const bf16x8 B = *(const bf16x8*)(wc + (int64_t)(nt * 16 + row) * a.Ktpad + tap * 8);
+                for (int nt = 0; nt < NT; nt++) {
+                    const bf16x8 B = *(const bf16x8*)(wc + (int64_t)(nt * 16 + row) * a.Ktpad + tap * 8);
 #pragma unroll
-                for (int m = 0; m < 4; m++) acc[m][nt] = mfma(A[m], B, acc[m][nt]);
+                    for (int m = 0; m < 4; m++) acc[m][nt] = mfma(A[m], B, acc[m][nt]);
+                }
             }
         }
     }
@@ -567,9 +586,15 @@ hipError_t launch_conv3(const ConvArgs& a, hipStream_t s) {
     if (a.wt && a.D0 % kCT0 == 0 && a.D1 % kCT1 == 0 && a.D2 % kCT2 == 0 && a.Ktpad == a.Cin / 8 * kTapsPad * 8 &&
         nt <= 6) {
         const dim3 tgrid((unsigned)(V / (kCT0 * kCT1 * kCT2)));
+        // several chunks per stage for inputs of 32 or more channels
+        const int cps = (a.Cin >= 32 && nt <= 3) ? (nt <= 2 ? 3 : 2) : 1;
         switch (nt) {
-#define XR_CONVT(N) \
-    case N: hipLaunchKernelGGL(k_conv3_tiled<N>, tgrid, dim3(256), 0, s, a); break;
+#define XR_CONVT(N)                                                                          \
+    case N:                                                                                  \
+        if (cps == 3) hipLaunchKernelGGL((k_conv3_tiled<N, 3>), tgrid, dim3(256), 0, s, a);     \
+        else if (cps == 2) hipLaunchKernelGGL((k_conv3_tiled<N, 2>), tgrid, dim3(256), 0, s, a); \
+        else hipLaunchKernelGGL((k_conv3_tiled<N, 1>), tgrid, dim3(256), 0, s, a);              \
+        break;
             XR_CONVT(1)
             XR_CONVT(2)
             XR_CONVT(3)
