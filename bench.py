@@ -79,6 +79,41 @@ def time_to_residual(ctx, xr, geom, sino, tau, subsets, max_epochs=300, pre=5):
                        "360 views, subsets of 9 views, inputs after 5/10/20 epochs, targets after 200)"}
 
 
+HELDOUT = ((9101, "mixed"), (9102, "fibres"), (9103, "mixed"))
+
+
+def heldout_workload(xr, args):
+    """time_to_residual on seeded phantoms the network was not trained on
+    (warmstart.phantom_spec draws; the committed network saw seeds 1000-1959,
+    5000-5095 and 6000-6143), at the bench geometry."""
+    import importlib
+
+    import torch
+
+    ws = importlib.import_module(xr.__name__ + ".warmstart")
+    N, U, P = args.n_vox, args.n_det, args.n_proj
+    sdd, sod, det = 1000.0, 500.0, 0.4
+    vs = xr.default_vox_size(sdd, sod, det)
+    geom = xr.make_geometry(sdd, sod, det, N, U, P)
+    ctx = xr.Context(0)
+    out = []
+    for seed, family in HELDOUT:
+        spec = ws.phantom_spec(seed, N, vs, family)
+        vol = ws.make_phantom(ctx, N, vs, seed, **spec)
+        sino = ws.project(ctx, geom, vol)
+        del vol
+        r = time_to_residual(ctx, xr, geom, sino, args.tau, args.subsets or max(1, P // 9))
+        r.update({"seed": seed, "family": family})
+        for k in ("residual", "acceptance", "network"):
+            r.pop(k)
+        out.append(r)
+        del sino
+        torch.cuda.empty_cache()
+    return {"metric": "time-to-residual %g, plain MLEM vs learned warm start, held-out phantoms" % args.tau,
+            "unit": "s", "n_gpus": 1, "data": "synthetic (warmstart.phantom_spec seeds %s)" % [s for s, _ in HELDOUT],
+            "config": {"workload": "%d^3 volume, %d views, %dx%d detector" % (N, P, U, U)}, "results": out}
+
+
 def batch_workload(xr, args, n=48, epochs=30, seed=4):
     """48 independent MLEM problems with volumes drawn from {64, 96, 128, 192,
     256}^3 and {90, 180, 360} views, detector 1.5x the volume edge, seeded
@@ -171,15 +206,16 @@ def main():
     ap.add_argument("--tau", type=float, default=1e-3)
     ap.add_argument("--subsets", type=int, default=0,
                     help="ordered subsets of the plain method (default: views/9, the training pairs' subset size)")
-    ap.add_argument("--workload", choices=["mlem", "batch"], default="mlem",
+    ap.add_argument("--workload", choices=["mlem", "batch", "heldout"], default="mlem",
                     help="mlem: one MLEM iteration at 512^3/720 views (default); batch: 48 uneven MLEM problems, "
-                         "30 epochs in persistent launches (HIP graph, autotuned tiles)")
+                         "30 epochs in persistent launches (HIP graph, autotuned tiles); heldout: time to "
+                         "residual, plain vs warm start, on three phantoms outside the training draws")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (views would shard "
-              "across GPUs with an all-reduce of the backprojection; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (views would shard "
+              "across GPUs with an all-reduce of the backprojection).", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -189,6 +225,9 @@ def main():
     xr = entry.load_package()
     if args.workload == "batch":
         print(json.dumps(batch_workload(xr, args)))
+        return
+    if args.workload == "heldout":
+        print(json.dumps(heldout_workload(xr, args)))
         return
     dev = torch.device("cuda:0")
     N, U, P = args.n_vox, args.n_det, args.n_proj
