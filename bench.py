@@ -24,9 +24,24 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_VOXEL_VIEW = 4.0  # one FP32 volume stream per view (plain forward / back)
 
 
-def cpu_baseline(N, U, P, sdd, sod, det, vs, n_views=16):
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(N, U, P, sdd, sod, det, vs, n_views=16, extras=True):
     """Time the oracle's FP32 restatement (the reference's operators on the CPU)
-    on a bounded sample: forward + back of n_views views of the same workload."""
+    on a bounded sample: forward + back of n_views views of the same workload
+    (value, GUPS).  With `extras`: the 64^3 / 90-view / 96^2 configuration
+    timed in full (20 MLEM iterations, batch 1) and the per-voxel PCA at
+    256^3 x 13 coefficients."""
     import __graft_entry__ as entry
 
     orc = entry.load_oracle()
@@ -44,9 +59,29 @@ def cpu_baseline(N, U, P, sdd, sod, det, vs, n_views=16):
         case.backward_view(img, v, tmp, fac, prec=orc.REF32)
     dt = time.perf_counter() - t0
     gups = 2.0 * N ** 3 * len(views) / dt / 1e9
-    return {"value": gups, "unit": "GUPS", "cores": int(orc.num_threads()), "kind": "port",
-            "sample": "FP32 oracle forward+back of %d of %d views at %d^3, %dx%d detector (%.1f s)"
-                      % (len(views), P, N, U, U, dt)}
+    out = {"value": gups, "unit": "GUPS", "cores": int(orc.num_threads()), "kind": "port",
+           "cpu_model": cpu_model(),
+           "sample": "FP32 oracle forward+back of %d of %d views at %d^3, %dx%d detector (%.1f s)"
+                     % (len(views), P, N, U, U, dt)}
+    if extras:
+        del vol, tmp, fac
+        # 64^3 / 90 views / 96^2, 20 MLEM iterations (batch 1), timed in full
+        c1 = orc.Case(600.0, 300.0, 0.8, 64, 96, 90, vox_size=0.5)
+        truth = orc.phantom_ellipsoids(64, 0.5, xr.ellipsoid_params(1001, 10, fov_radius=16.0, value_lo=0.0,
+                                                                   value_hi=0.05))
+        sino = np.stack([c1.forward_view(truth, n, prec=orc.REF32)[0] for n in range(90)]).astype(np.float32)
+        t0 = time.perf_counter()
+        c1.ir_reconstruct(sino, np.full(64 ** 3, 0.01, np.float32), 20, 1, prec=orc.REF32)
+        out["c1_mlem20_s"] = round(time.perf_counter() - t0, 3)
+        # per-voxel PCA over 256^3 voxels x 13 coefficients
+        import importlib
+
+        wl = importlib.import_module(xr.__name__ + ".workloads")
+        mu = np.random.default_rng(5).random((13, 256 ** 3), dtype=np.float32)
+        t0 = time.perf_counter()
+        orc.pca(mu, wl.XTT_BASIS13)
+        out["pca_256_13_s"] = round(time.perf_counter() - t0, 3)
+    return out
 
 
 MODEL = os.path.join(ROOT, "xray-directional-reconstruction_amd", "models", "warmstart_c24.xunt")
@@ -77,6 +112,82 @@ def time_to_residual(ctx, xr, geom, sino, tau, subsets, max_epochs=300, pre=5):
             "speedup_time": round(plain["ms"] / max(warm["ms"], 1e-9), 3),
             "network": "models/warmstart_c24.xunt (960 mixed phantoms at 128^3 / 180 views, then 240 at 256^3 / "
                        "360 views, subsets of 9 views, inputs after 5/10/20 epochs, targets after 200)"}
+
+
+def c2_noisy_sinogram(ctx, xr, geom, plan, N, vs, photons=1e4, seed=2002):
+    """The 512^3 / 720-view workload's data: 40 seeded ellipsoids plus a
+    cylinder shell, the normalised forward projection, Poisson noise at
+    `photons` per ray and the driver's clamp (main.cu:35)."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    params = xr.ellipsoid_params(1234, 40, fov_radius=0.5 * N * vs)
+    vref = torch.zeros(N ** 3, device=dev)
+    ctx.phantom_rasterise(params, N, vs, vref)
+    ctx.phantom_cylinder_shell(N, vs, 0.45 * N * vs, 4 * vs, 0.02, vref)
+    truth = torch.empty_like(vref)
+    ctx.to_columns(vref, truth, N)
+    del vref
+    P, U, V = geom.n_proj, geom.n_det_u, geom.n_det_v
+    p = torch.zeros(P * U * V, device=dev)
+    plan.project(truth, p, plan.codes(range(P)), out_mode=xr.OUT_NORMALISED)
+    del truth
+    y = torch.empty_like(p)
+    ctx.noise_poisson(y, p, photons, seed)
+    ctx.sino_clamp(y)
+    ctx.synchronize()
+    return y
+
+
+def ttr_frozen(ctx, xr, geom, sino, tau=1e-3, subsets=80, long_epochs=60, pres=(2,), model=None):
+    """Time to the frozen relative residual criterion (warmstart.plain_curve /
+    warm_curve): R_inf from `long_epochs` plain epochs, plain epochs to
+    (R_k - R_inf)/(R_0 - R_inf) <= tau, then the warm start for each pre."""
+    import importlib
+
+    ws = importlib.import_module(xr.__name__ + ".warmstart")
+    scene = ws.Scene(ctx, geom, sino, subsets=subsets)
+    R, ms = ws.plain_curve(scene, long_epochs)
+    R0, Rinf = R[0], R[-1]
+    k = ws.first_within(R, R0, Rinf, tau)
+    out = {"tau": tau, "criterion": "(R_k - R_inf)/(R_0 - R_inf), R = ||y - Ax||^2/||y||^2 (FP64), "
+                                    "R_inf after %d plain epochs" % long_epochs,
+           "method": "MLEM, %d ordered subsets" % subsets, "R0": R0, "Rinf": Rinf,
+           "plain_epochs": k, "plain_s": None if k is None else round(ms[k] / 1e3, 3),
+           "plain_R": [round(r, 9) for r in R[:40]], "epoch_ms": round(ms[-1] / long_epochs, 2)}
+    if model and os.path.exists(model):
+        net = xr.UNet(ctx, 24)
+        net.load(model)
+        out["warm"] = []
+        for pre in pres:
+            w = ws.warm_curve(scene, net, R0, Rinf, tau, long_epochs, pre=pre)
+            w["R"] = [round(r, 9) for r in w["R"]]
+            w["pre"] = pre
+            w["s"] = round(w.pop("ms") / 1e3, 3)
+            if k is not None and w["reached"]:
+                w["speedup_time"] = round(ms[k] / 1e3 / w["s"], 3)
+                w["speedup_epochs"] = round(k / max(w["epochs"], 1), 3)
+            out["warm"].append(w)
+        net.close()
+    scene.close()
+    return out
+
+
+def ttr_workload(xr, args):
+    """Time to the frozen residual criterion on the noisy 512^3 / 720-view data."""
+    N, U, P = args.n_vox, args.n_det, args.n_proj
+    sdd, sod, det = 1000.0, 500.0, 0.4
+    vs = xr.default_vox_size(sdd, sod, det)
+    geom = xr.make_geometry(sdd, sod, det, N, U, P)
+    ctx = xr.Context(0)
+    plan = ctx.plan(geom, xr.Scan())
+    sino = c2_noisy_sinogram(ctx, xr, geom, plan, N, vs, photons=args.photons)
+    plan.close()
+    pres = tuple(int(v) for v in args.pres.split(","))
+    r = ttr_frozen(ctx, xr, geom, sino, tau=args.tau, subsets=args.subsets or max(1, P // 9),
+                   long_epochs=args.long_epochs, pres=pres, model=args.model or MODEL)
+    r["data"] = "synthetic (40 seeded ellipsoids plus a cylinder shell, Poisson noise at %g photons)" % args.photons
+    return r
 
 
 HELDOUT = ((9101, "mixed"), (9102, "fibres"), (9103, "mixed"))
@@ -176,6 +287,125 @@ def batch_workload(xr, args, n=48, epochs=30, seed=4):
                                    "30 epochs, batch 1, persistent launches + HIP graph + autotuned tiles"}}
 
 
+def xtt_measure(xr, steps=3, warmup=1, N=256, U=512, P=360):
+    """Directional (XTT) operator pair at 256^3 x 13 coefficients, 4 grating
+    orientations x 360 views, 512^2 detector: the fused forward + ratio launch
+    over all 1440 (condition, view) images and the fused backprojection +
+    per-channel MLEM update launch, plus the per-voxel PCA (calcEigenVector)
+    over the whole volume.  G voxel-views/s = N^3 x conditions x views per pass
+    / s; roofline on 52 B per voxel-view (13 FP32 coefficients per view)."""
+    import importlib
+
+    import torch
+
+    wl = importlib.import_module(xr.__name__ + ".workloads")
+    ctx = xr.Context(0)
+    geom, scan, plan, truth, sino, _ = wl.xtt_workload(xr, ctx, N=N, U=U, P=P)
+    K, C = scan.n_basis, scan.n_cond
+    N3 = N ** 3
+    vol = torch.full((K * N3,), 0.01, device="cuda")
+    proj = torch.empty_like(sino)
+    codes = np.arange(C * P, dtype=np.int32)
+
+    def step():
+        loss = plan.project(vol, proj, codes, out_mode=xr.OUT_RATIO, sino=sino, xtt=True)
+        plan.backproject_update(proj, codes, vol, method=xr.METHOD_MLEM, subset_size=P, xtt=True)
+        return loss
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    ctx.timing_reset()
+    losses = [step() for _ in range(steps)]
+    torch.cuda.synchronize()
+    fwd_ms, fwd_n = ctx.timing_read(xr.TIMING_FORWARD)
+    bwd_ms, bwd_n = ctx.timing_read(xr.TIMING_BACKWARD)
+    # per-voxel principal orientation over the whole volume
+    md = torch.empty(3 * N3, device="cuda")
+    ev = torch.empty(3 * N3, device="cuda")
+    ctx.timing_reset()
+    for _ in range(steps):
+        ctx.pca(vol, scan.basis.reshape(-1, 3), N3, N3, md, ev)
+    torch.cuda.synchronize()
+    pca_ms, pca_n = ctx.timing_read(xr.TIMING_OTHER)
+    ctx.timing(False)
+    vv = float(N3) * C * P
+    fwd_avg, bwd_avg = fwd_ms / max(fwd_n, 1), bwd_ms / max(bwd_n, 1)
+    dom_name, dom_ms = ("xtt backprojection+update", bwd_avg) if bwd_avg >= fwd_avg else ("xtt forward+ratio", fwd_avg)
+    achieved = 4.0 * K * vv / (dom_ms * 1e-3) / 1e9
+    pca_avg = pca_ms / max(pca_n, 1)
+    out = {"workload": "XTT MLEM iteration, %d^3 x %d coefficients, %d conditions x %d views, %dx%d detector"
+                       % (N, K, C, P, U, U),
+           "gvv_forward": round(vv / (fwd_avg * 1e-3) / 1e9, 2), "gvv_backward": round(vv / (bwd_avg * 1e-3) / 1e9, 2),
+           "unit": "G voxel-views/s", "forward_ms": round(fwd_avg, 3), "backward_ms": round(bwd_avg, 3),
+           "loss_first_last": [losses[0], losses[-1]],
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": dom_name, "kernel_ms": round(dom_ms, 3),
+                        "algorithmic_bytes": "%d B per voxel-view x %d^3 voxels x %d views per launch"
+                                             % (4 * K, N, C * P)},
+           "pca_ms": round(pca_avg, 3),
+           "pca_gbs": round((4 * K + 24) * N3 / (pca_avg * 1e-3) / 1e9, 1)}
+    plan.close()
+    ctx.close()
+    return out
+
+
+def fdk_art_measure(ctx, xr, geom, sino, art_iters=50, lam=0.2):
+    """FDK with the Ram-Lak filter (calcWeight + projConv + weighted
+    backprojection, fdk.cu:13-170) over every view, and `art_iters` ART
+    iterations (projSubtract + voxelPlus, ir.cu:1656-1693; batch 1, so the step
+    is lambda / views, reconstruct.cu:124) on the same data, from zero.
+    Roofline models: 8 B per filtered detector sample, 4 B per voxel-view."""
+    import torch
+
+    N, U, V, P = geom.n_vox, geom.n_det_u, geom.n_det_v, geom.n_proj
+    codes = np.arange(P, dtype=np.int32)
+    fplan = ctx.plan(geom, xr.Scan(), convention=xr.CONV_FDK)
+    filt = torch.empty_like(sino)
+    vol = torch.empty(N ** 3, device=sino.device)
+    fplan.fdk_filter(sino, filt, codes, filter=xr.FILTER_RAMLAK)  # warm-up
+    fplan.fdk_backproject(filt, codes, vol)
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    ctx.timing_reset()
+    fplan.fdk_filter(sino, filt, codes, filter=xr.FILTER_RAMLAK)
+    fplan.fdk_backproject(filt, codes, vol)
+    torch.cuda.synchronize()
+    f_ms, _ = ctx.timing_read(xr.TIMING_FILTER)
+    b_ms, _ = ctx.timing_read(xr.TIMING_BACKWARD)
+    fplan.close()
+    del filt
+    # ART from zero
+    plan = ctx.plan(geom, xr.Scan())
+    vol.zero_()
+    resid = torch.empty_like(sino)
+    plan.project(vol, resid, codes, out_mode=xr.OUT_RESIDUAL, sino=sino)  # tables, warm-up
+    torch.cuda.synchronize()
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    losses = []
+    for _ in range(art_iters):
+        losses.append(plan.project(vol, resid, codes, out_mode=xr.OUT_RESIDUAL, sino=sino))
+        plan.backproject_update(resid, codes, vol, method=xr.METHOD_ART, lam=lam, subset_size=P)
+    torch.cuda.synchronize()
+    art_s = time.perf_counter() - t0
+    af_ms, af_n = ctx.timing_read(xr.TIMING_FORWARD)
+    ab_ms, ab_n = ctx.timing_read(xr.TIMING_BACKWARD)
+    ctx.timing(False)
+    plan.close()
+    vv = float(N) ** 3 * P
+    return {"fdk": {"filter": "ram-lak", "filter_ms": round(f_ms, 3), "backprojection_ms": round(b_ms, 3),
+                    "total_ms": round(f_ms + b_ms, 3),
+                    "filter_gbs": round(8.0 * U * V * P / (f_ms * 1e-3) / 1e9, 1),
+                    "backprojection_gups": round(vv / (b_ms * 1e-3) / 1e9, 2),
+                    "backprojection_roofline_frac": round(4.0 * vv / (b_ms * 1e-3) / 8e12, 4)},
+            "art": {"iterations": art_iters, "lambda": lam, "s": round(art_s, 3),
+                    "forward_ms": round(af_ms / max(af_n, 1), 3), "backward_ms": round(ab_ms / max(ab_n, 1), 3),
+                    "gups": round(2.0 * vv * art_iters / art_s / 1e9, 2),
+                    "loss_first_last": [losses[0], losses[-1]]}}
+
+
 def measured_traffic(kernel, N, P):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy), if one
@@ -206,7 +436,11 @@ def main():
     ap.add_argument("--tau", type=float, default=1e-3)
     ap.add_argument("--subsets", type=int, default=0,
                     help="ordered subsets of the plain method (default: views/9, the training pairs' subset size)")
-    ap.add_argument("--workload", choices=["mlem", "batch", "heldout"], default="mlem",
+    ap.add_argument("--photons", type=float, default=1e4)
+    ap.add_argument("--long-epochs", type=int, default=60)
+    ap.add_argument("--pres", default="2")
+    ap.add_argument("--model", default="")
+    ap.add_argument("--workload", choices=["mlem", "batch", "heldout", "xtt", "ttr", "fdkart"], default="mlem",
                     help="mlem: one MLEM iteration at 512^3/720 views (default); batch: 48 uneven MLEM problems, "
                          "30 epochs in persistent launches (HIP graph, autotuned tiles); heldout: time to "
                          "residual, plain vs warm start, on three phantoms outside the training draws")
@@ -228,6 +462,21 @@ def main():
         return
     if args.workload == "heldout":
         print(json.dumps(heldout_workload(xr, args)))
+        return
+    if args.workload == "fdkart":
+        N, U, P = args.n_vox, args.n_det, args.n_proj
+        geom = xr.make_geometry(1000.0, 500.0, 0.4, N, U, P)
+        ctx = xr.Context(0)
+        plan = ctx.plan(geom, xr.Scan())
+        sino = c2_noisy_sinogram(ctx, xr, geom, plan, N, geom.vox_size, photons=args.photons)
+        plan.close()
+        print(json.dumps(fdk_art_measure(ctx, xr, geom, sino)))
+        return
+    if args.workload == "ttr":
+        print(json.dumps(ttr_workload(xr, args)))
+        return
+    if args.workload == "xtt":
+        print(json.dumps(xtt_measure(xr, steps=args.steps, warmup=args.warmup)))
         return
     dev = torch.device("cuda:0")
     N, U, P = args.n_vox, args.n_det, args.n_proj

@@ -182,6 +182,16 @@ int xr_volume_from_columns(xr_context* ctx, const float* col_dev, float* ref_dev
 int xr_project(xr_context* ctx, xr_plan* plan, const float* vol, float* proj, const float* sino,
                const int32_t* views, int n_views, const uint8_t* accumulate, int xtt, int out_mode, double* loss);
 
+/* Data residual of a volume: the normalised forward projection p of the
+ * listed views (as xr_project's XR_OUT_NORMALISED), the residual images
+ * y - p written to `resid` (the layout of proj in xr_project), and the FP64
+ * sums l1 = sum |y - p| (the reference's loss, reconstruct.cu:131) and
+ * l2 = sum (y - p)^2 over the listed views (either may be NULL).  No
+ * counterpart in the reference beyond the loss: used for the data-residual
+ * criterion of the learned warm start. */
+int xr_residual(xr_context* ctx, xr_plan* plan, const float* vol, float* resid, const float* sino,
+                const int32_t* views, int n_views, int xtt, double* l1, double* l2);
+
 /* Raw backprojection of the listed views into column-layout accumulators:
  * tmp += sum_c c*r, factor += 1 (backwardProj, ir.cu:1605-1614, 1720-1751) or,
  * xtt != 0, per channel tmp_k += v_km^2 sum_c c*r, factor_k += v_km^2
@@ -219,6 +229,18 @@ int xr_pca(xr_context* ctx, const float* mu, int n_basis, const float* basis_hos
 int xr_pca_host(xr_context* ctx, const float* mu_host, int n_basis, const float* basis_host, int64_t nvox,
                 float* md_host, float* evalue_host);
 
+/* Direction post-processing, the consumers of md (direction/src/pca.cu:67-106).
+ * md: 3 channels of nvox floats (x, y, z components of the main direction).
+ * xr_md_angles (calcAngleFromMD): angle gets 2 channels, phi = atan2(y, x) and
+ * theta = atan2(sqrt(x^2 + y^2), z), with the direction flipped when y < 0.
+ * xr_md_normalise (directionNormalization): md / |md| where |md| > eps, else 0
+ * (the reference uses eps = 0.65).  Device buffers, or host buffers for the
+ * _host variants (computed on the GPU). */
+int xr_md_angles(xr_context* ctx, const float* md, float* angle, int64_t nvox);
+int xr_md_normalise(xr_context* ctx, float* md, int64_t nvox, float eps);
+int xr_md_angles_host(xr_context* ctx, const float* md_host, float* angle_host, int64_t nvox);
+int xr_md_normalise_host(xr_context* ctx, float* md_host, int64_t nvox, float eps);
+
 /* ---- host-buffer solvers (the reference's host API) ------------------- */
 
 /* Per-epoch loss sums (raw sums of |y-p|; the reference writes them divided by
@@ -250,27 +272,32 @@ int xr_fdk_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, 
 
 /* ---- batched multi-problem solver ---------------------------------------- */
 
-/* Many independent IR problems (MLEM or ART, each one untilted or tilted
- * condition with its own geometry, view count and subset count) solved
- * together: every subset step is one persistent forward launch and one
- * persistent backprojection launch over all problems, and with
- * XR_BATCH_GRAPH the whole epoch loop is captured once as a HIP graph and
- * replayed.  No counterpart in the reference (IR::reconstruct,
- * reconstruct.cu:23-156, solves one volume per call); each problem's result
- * equals its own xr_ir_reconstruct solve bit for bit (same subset order: one
- * default-seeded mt19937_64 per problem, reconstruct.cu:75). */
+/* Many independent problems of one method (MLEM or ART) solved together:
+ * IR problems (one untilted or tilted condition, no basis) and XTT problems
+ * (n_cond >= 1 grating conditions and n_basis scattering directions), each
+ * with its own geometry, view count and subset count.  Every subset step is
+ * one persistent forward launch and one persistent backprojection launch per
+ * kind (IR, XTT) over all problems of that kind, and with XR_BATCH_GRAPH the
+ * whole epoch loop is captured once as a HIP graph and replayed.  No
+ * counterpart in the reference (IR::reconstruct, reconstruct.cu:23-156, and
+ * XTT::reconstruct, reconstruct.cu:799-954, solve one problem per call); each
+ * problem's result equals its own xr_ir_reconstruct / xr_xtt_reconstruct
+ * solve bit for bit (same subset order: one default-seeded mt19937_64 per IR
+ * problem, reconstruct.cu:75; a fresh one each epoch for XTT, :858). */
 typedef struct xr_batch xr_batch;
 #define XR_BATCH_GRAPH 1    /* capture the epoch loop as a HIP graph */
 #define XR_BATCH_AUTOTUNE 2 /* tile shapes from xr_autotune per distinct problem shape */
 
 /* geoms[i], scans[i] (scans may be NULL: one untilted condition, no basis;
- * otherwise scans[i].n_cond must be 1 and n_basis 0); batch[i]: subsets of
- * problem i (NULL: 1).  Allocates every device buffer of the batch. */
+ * an IR problem has n_cond 1 and n_basis 0, an XTT problem n_basis > 0);
+ * batch[i]: subsets of problem i (NULL: 1).  Allocates every device buffer of
+ * the batch.  Autotuning (XR_BATCH_AUTOTUNE) applies to the IR problems. */
 int xr_batch_create(xr_context* ctx, int n, const xr_geometry* geoms, const xr_scan* scans, const int32_t* batch,
                     int direction, int method, float lambda, int flags, xr_batch** out);
 void xr_batch_destroy(xr_batch* b);
 /* Device buffers of problem i, owned by the batch: the column-layout volume
- * (N^3, initial guess in / result out) and the measured sinogram (U*V*P). */
+ * (N^3, or n_basis channels of N^3 for XTT; initial guess in / iterate out,
+ * before any square root) and the measured sinogram (U*V*P per condition). */
 int xr_batch_buffers(xr_batch* b, int i, float** vol_cols, float** sino);
 /* The plan of problem i (projection tables; use with xr_project to make data). */
 xr_plan* xr_batch_plan(xr_batch* b, int i);
@@ -287,8 +314,12 @@ int xr_batch_restart(xr_batch* b);
 /* Milliseconds of the last xr_batch_run on the GPU (HIP events around the
  * epoch loop, excluding the upload of the subset lists). */
 int xr_batch_elapsed_ms(const xr_batch* b, double* ms);
-/* Host-buffer convenience (reference layouts), IR::reconstruct for n problems:
- * sino[i], vol[i] host arrays; vol[i] in/out. */
+/* Host-buffer convenience (reference layouts), IR::reconstruct / XTT::reconstruct
+ * for n problems: sino[i], vol[i] host arrays (XTT: the conditions' sinograms
+ * and the channels' volumes each one after another); vol[i] in/out.  For XTT
+ * problems, as XTT::reconstruct (reconstruct.cu:919-927), the result is
+ * sqrt(|v|) and sino[i] is overwritten with the final ratio / residual images;
+ * the principal orientation is left to xr_pca. */
 int xr_batch_reconstruct(xr_context* ctx, int n, const xr_geometry* geoms, const xr_scan* scans,
                          const int32_t* batch, float* const* sino, float* const* vol, int epochs, int direction,
                          int method, float lambda, int flags, double* losses);
@@ -396,6 +427,10 @@ int xr_phantom_fibres_scalar(xr_context* ctx, const double* params_host, int cou
  * result does not depend on launch shape. */
 int xr_noise_poisson(xr_context* ctx, float* y_dev, const float* p_dev, int64_t n, double I0, uint64_t seed);
 int xr_noise_gaussian(xr_context* ctx, float* y_dev, const float* x_dev, int64_t n, double sigma, uint64_t seed);
+/* The driver's sinogram clamp (main.cu:35): values below 0 become 1e-8, for n
+ * floats on the device (e.g. after Poisson noise, where -ln(c/I0) < 0 when a
+ * count exceeds I0). */
+int xr_sino_clamp(xr_context* ctx, float* sino_dev, int64_t n);
 /* The Philox4x32-10 block function (host), for known-answer tests. */
 void xr_philox4x32(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
 

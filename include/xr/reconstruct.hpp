@@ -16,7 +16,8 @@
 //   - Volume's copy constructor/assignment allocate before copying (the
 //     reference copies into an unallocated buffer, volume.h:34-46);
 //   - K (basis vectors) and the number of conditions have no fixed caps;
-//   - the loss file is written only when XR_LOSS_CSV names a path.
+//   - XR_LOSS_CSV can move (or, empty, turn off) the loss file the reference
+//     always writes to ../python/loss.csv.
 // Header only; C++17.
 #pragma once
 
@@ -198,13 +199,52 @@ inline ScanTables scan_tables(int n_cond, int n_basis) {
     st.scan.grating = nullptr;
     return st;
 }
+// The reference writes the per-epoch loss / (U V P) to ../python/loss.csv,
+// relative to the working directory, after every IR / XTT solve
+// (reconstruct.cu:153-155, 951-953); an ofstream it cannot open (no such
+// directory) writes nothing.  XR_LOSS_CSV names another path; an empty
+// XR_LOSS_CSV turns the file off.
+inline std::string loss_path() {
+    const char* env = std::getenv("XR_LOSS_CSV");
+    return env ? std::string(env) : std::string("../python/loss.csv");
+}
 inline void write_loss(const std::vector<double>& losses, double norm) {
-    const char* path = std::getenv("XR_LOSS_CSV");
-    if (!path) return;
+    const std::string path = loss_path();
+    if (path.empty()) return;
     std::ofstream f(path);
     for (double l : losses) f << (float)(l / norm) << ",";
 }
+inline std::string raw_suffix(int64_t a, int64_t b, int64_t c) {
+    return "_" + std::to_string(a) + "x" + std::to_string(b) + "x" + std::to_string(c) + ".raw";
+}
 }  // namespace xr_detail
+
+// ---- raw file names of the reference's driver (main.cu:31-32, 83-107) ------
+// Raw float32 files without header, named after init_params' paths and sizes.
+namespace xr {
+// Sinogram of condition `cond` as loaded: PROJ_PATH + LOAD_INDEX + "_UxVxP.raw".
+inline std::string sinogram_load_path(int cond) {
+    return PROJ_PATH + std::to_string(LOAD_INDEX[cond]) + xr_detail::raw_suffix(NUM_DETECT_U, NUM_DETECT_V, NUM_PROJ);
+}
+// Sinogram of condition `cond` as saved: VOLUME_PATH + "_proj" + (cond+1) + "_UxVxP.raw".
+inline std::string sinogram_save_path(int cond) {
+    return VOLUME_PATH + "_proj" + std::to_string(cond + 1) +
+           xr_detail::raw_suffix(NUM_DETECT_U, NUM_DETECT_V, NUM_PROJ);
+}
+// Volume (coefficient) k: VOLUME_PATH + (k+1) + "_NxNxN.raw".
+inline std::string volume_save_path(int k) {
+    return VOLUME_PATH + std::to_string(k + 1) + xr_detail::raw_suffix(NUM_VOXEL, NUM_VOXEL, NUM_VOXEL);
+}
+// Main-direction component i (0, 1, 2): DIRECTION_PATH + (i+1) + "_NxNxN.raw".
+inline std::string direction_save_path(int i) {
+    return DIRECTION_PATH + std::to_string(i + 1) + xr_detail::raw_suffix(NUM_VOXEL, NUM_VOXEL, NUM_VOXEL);
+}
+// Angle volume i (0: phi, 1: theta): DIRECTION_PATH + "_phi_" / "_theta_" + "NxNxN.raw".
+inline std::string angle_save_path(int i) {
+    const std::string n = std::to_string(NUM_VOXEL);
+    return DIRECTION_PATH + (i == 0 ? "_phi_" : "_theta_") + n + "x" + n + "x" + n + ".raw";
+}
+}  // namespace xr
 
 inline void init_params(const std::string& tag, const std::string& path) {
     xr_settings& s = xr_detail::settings();
@@ -333,35 +373,27 @@ inline void calcEigenVectorVolume(const Volume<float>* ct, Volume<float>* md, Vo
     }
 }
 
-// calcAngleFromMD (pca.cu:67-83): azimuth/polar angles of the main direction.
+// calcAngleFromMD (pca.cu:67-83): azimuth/polar angles of the main direction
+// (computed on the GPU, xr_md_angles).
 inline void calcAngleFromMD(const Volume<float> md[3], Volume<float> angle[2], int size_x, int size_y, int size_z) {
-    for (int x = 0; x < size_x; x++)
-        for (int y = 0; y < size_y; y++)
-            for (int z = 0; z < size_z; z++) {
-                const float a = md[0](x, y, z), b = md[1](x, y, z), c = md[2](x, y, z);
-                if (b < 0.f) {
-                    angle[0](x, y, z) = std::atan2(-b, -a);
-                    angle[1](x, y, z) = std::atan2(std::sqrt(b * b + a * a), -c);
-                } else {
-                    angle[0](x, y, z) = std::atan2(b, a);
-                    angle[1](x, y, z) = std::atan2(std::sqrt(b * b + a * a), c);
-                }
-            }
+    const int64_t n = (int64_t)size_x * size_y * size_z;
+    if (md[0].x() * md[0].y() * md[0].z() != n)
+        throw std::runtime_error("calcAngleFromMD: sizes do not match the direction volumes");
+    std::vector<float> in((size_t)3 * n), out((size_t)2 * n);
+    for (int i = 0; i < 3; i++) std::memcpy(in.data() + (size_t)i * n, md[i].get(), sizeof(float) * n);
+    xr_detail::check(xr_md_angles_host(xr_detail::context(), in.data(), out.data(), n), "calcAngleFromMD");
+    for (int i = 0; i < 2; i++) {
+        if (angle[i].x() * angle[i].y() * angle[i].z() != n) angle[i] = Volume<float>(size_x, size_y, size_z);
+        std::memcpy(angle[i].get(), out.data() + (size_t)i * n, sizeof(float) * n);
+    }
 }
 
-// directionNormalization (pca.cu:85-106): unit main direction where |md| > 0.65.
+// directionNormalization (pca.cu:85-106): unit main direction where |md| > 0.65,
+// zero elsewhere (computed on the GPU, xr_md_normalise).
 inline void directionNormalization(Volume<float> md[3]) {
-    const float eps = 0.65f;
     const int64_t n = md[0].x() * md[0].y() * md[0].z();
-    float *a = md[0].get(), *b = md[1].get(), *c = md[2].get();
-    for (int64_t i = 0; i < n; i++) {
-        const float nrm = std::sqrt(a[i] * a[i] + b[i] * b[i] + c[i] * c[i]);
-        if (nrm > eps) {
-            a[i] /= nrm;
-            b[i] /= nrm;
-            c[i] /= nrm;
-        } else {
-            a[i] = b[i] = c[i] = 0.f;
-        }
-    }
+    std::vector<float> buf((size_t)3 * n);
+    for (int i = 0; i < 3; i++) std::memcpy(buf.data() + (size_t)i * n, md[i].get(), sizeof(float) * n);
+    xr_detail::check(xr_md_normalise_host(xr_detail::context(), buf.data(), n, 0.65f), "directionNormalization");
+    for (int i = 0; i < 3; i++) std::memcpy(md[i].get(), buf.data() + (size_t)i * n, sizeof(float) * n);
 }

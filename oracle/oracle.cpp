@@ -721,6 +721,19 @@ void orc_subset_orders(int batch, int epochs, int per_epoch_engine, int32_t* out
     subset_orders(batch, epochs, per_epoch_engine, out);
 }
 
+// a5/a6: projRatio / projSubtract on npix pixels.  proj: in = forward
+// projection, out = ratio (MLEM) or residual (ART), in the precision `prec`.
+// Returns sum |y - p| (the epoch loss term).
+double orc_ratio_step(int prec, int64_t npix, const float* sino, double* proj, int art) {
+    if (prec == 0) {
+        std::vector<float> p(proj, proj + npix);
+        const double loss = ratio_step<float>(npix, sino, p.data(), art);
+        for (int64_t i = 0; i < npix; i++) proj[i] = p[i];
+        return loss;
+    }
+    return ratio_step<double>(npix, sino, proj, art);
+}
+
 // a10: IR::reconstruct.  sino: n_cond*U*V*P floats; vol_in: n_cond*N^3 floats (reference layout).
 // dir 0 = CW, 1 = CCW; art 0 = MLEM, 1 = ART.  vol_out: n_cond*N^3 doubles; losses: n_cond*epochs raw sums.
 void orc_ir_reconstruct(int prec, const orc_geometry* g, const orc_scan* s, const float* sino, const float* vol_in,
@@ -852,6 +865,36 @@ void orc_pca(const float* mu, int K, const float* basis, int64_t nvox, double* m
 
 // Symmetric 3x3 eigen solve (ascending eigenvalues, eigenvectors as columns).
 void orc_eig3(const double* A, double* evals, double* evecs) { jacobi3(A, evals, evecs); }
+
+// calcAngleFromMD (direction/src/pca.cu:67-83) in the reference's FP32 host
+// arithmetic: md = 3 channels of n floats, angle = 2 channels (phi, theta).
+void orc_md_angles(const float* md, float* angle, int64_t n) {
+    for (int64_t i = 0; i < n; i++) {
+        const float a = md[i], b = md[n + i], c = md[2 * n + i];
+        if (b < 0.f) {
+            angle[i] = std::atan2(-b, -a);
+            angle[n + i] = std::atan2(std::sqrt(b * b + a * a), -c);
+        } else {
+            angle[i] = std::atan2(b, a);
+            angle[n + i] = std::atan2(std::sqrt(b * b + a * a), c);
+        }
+    }
+}
+
+// directionNormalization (direction/src/pca.cu:85-106), eps = 0.65 there.
+void orc_md_normalise(float* md, int64_t n, float eps) {
+    for (int64_t i = 0; i < n; i++) {
+        const float x = md[i], y = md[n + i], z = md[2 * n + i];
+        const float nrm = std::sqrt(x * x + y * y + z * z);
+        if (nrm > eps) {
+            md[i] = x / nrm;
+            md[n + i] = y / nrm;
+            md[2 * n + i] = z / nrm;
+        } else {
+            md[i] = md[n + i] = md[2 * n + i] = 0.f;
+        }
+    }
+}
 
 // Ellipsoid phantom rasterised at voxel centres, voxel x at (x-(N+1)/2)*vs (the IR
 // convention, ir.cu:1882).  params: 16 doubles per ellipsoid: centre[3] (mm),

@@ -47,6 +47,7 @@ def _load():
         "orc_chord_length": (None, [I, G, S, I, I, _d]),
         "orc_backward_view": (None, [I, G, S, I, I, _f, I, _d, _d]),
         "orc_subset_orders": (None, [I, I, I, _i32]),
+        "orc_ratio_step": (ctypes.c_double, [I, ctypes.c_int64, _f, _d, I]),
         "orc_ir_reconstruct": (None, [I, G, S, _f, _f, I, I, I, I, F, _d, _d]),
         "orc_unet_num_params": (ctypes.c_int64, [I]),
         "orc_conv3": (None, [_f, I, I, I, I, _f, _f, I, I, _d]),
@@ -60,6 +61,8 @@ def _load():
         "orc_fdk_reconstruct": (None, [I, G, S, _f, I, I, _d, _d]),
         "orc_pca": (None, [_f, I, _f, ctypes.c_int64, _d, _d]),
         "orc_eig3": (None, [_d, _d, _d]),
+        "orc_md_angles": (None, [_f, _f, ctypes.c_int64]),
+        "orc_md_normalise": (None, [_f, ctypes.c_int64, ctypes.c_float]),
         "orc_phantom_ellipsoids": (None, [I, ctypes.c_double, _d, I, _f]),
         "orc_num_threads": (I, []),
         "orc_noise_poisson": (None, [_f, _f, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64]),
@@ -209,6 +212,16 @@ def subset_orders(batch, epochs, per_epoch_engine):
     return out.reshape(epochs, batch)
 
 
+def ratio_step(sino, proj, art=False, prec=F64):
+    """projRatio / projSubtract (ir.cu:1616-1666) on arrays of pixels: returns
+    (ratio or residual image, sum |y - p|)."""
+    y = np.ascontiguousarray(sino, dtype=np.float32).reshape(-1)
+    p = np.array(proj, dtype=np.float64).reshape(-1)
+    assert y.size == p.size
+    loss = _lib.orc_ratio_step(prec, y.size, _fp(y), _dp(p), int(art))
+    return p.reshape(np.shape(proj)), float(loss)
+
+
 def pca(mu, basis):
     """mu: (K, nvox) float32; returns md (3, nvox), evalue (3, nvox) in FP64."""
     mu = np.ascontiguousarray(mu, dtype=np.float32)
@@ -218,6 +231,22 @@ def pca(mu, basis):
     ev = np.zeros(3 * nvox)
     _lib.orc_pca(_fp(mu), K, _fp(b), nvox, _dp(md), _dp(ev))
     return md.reshape(3, nvox), ev.reshape(3, nvox)
+
+
+def md_angles(md):
+    """calcAngleFromMD (pca.cu:67-83): md (3, n) float32 -> (phi, theta) (2, n)."""
+    md = np.ascontiguousarray(md, dtype=np.float32)
+    n = md.shape[1]
+    out = np.zeros((2, n), np.float32)
+    _lib.orc_md_angles(_fp(md), _fp(out), n)
+    return out
+
+
+def md_normalise(md, eps=0.65):
+    """directionNormalization (pca.cu:85-106): returns the normalised copy."""
+    out = np.array(md, dtype=np.float32, order="C")
+    _lib.orc_md_normalise(_fp(out), out.shape[1], eps)
+    return out
 
 
 def eig3(A):

@@ -95,3 +95,63 @@ def test_ellipsoid_parameters(xr):
         R = q[6:15].reshape(3, 3)
         assert np.allclose(R @ R.T, np.eye(3), atol=1e-12) and abs(np.linalg.det(R) - 1) < 1e-12
     assert ((a[:, 15] >= 0.0) & (a[:, 15] <= 0.05)).all()
+
+
+REF_SETTINGS = "/root/reference/utility/settings.json"
+
+
+def restate_init_params(data, tag):
+    """params.cu:42-104 restated with Python's json reader (independent of the
+    library's own reader): FP32 vecTrans += areaTrans for every rotation, then
+    the per-condition tables selected by `index` when `sorting` is true (a
+    missing key reads as false; the reference's lookup would throw there)."""
+    t, rv = data[tag], data["recon_variable"]
+    C, K = rv["condition"], rv["vector"]
+    vT = np.array(t["vecTrans"], np.float32).reshape(-1, 3)
+    vT[: t["rot"]] += np.array(t["areaTrans"], np.float32)
+    mR = np.array(t["matrixRot"], np.float32).reshape(-1, 9)
+    off = np.array(t["offset"], np.float32).reshape(-1, 3)
+    idx = [int(i) for i in t["index"][:C]] if t.get("sorting", False) else list(range(C))
+    return dict(n_cond=C, n_basis=K, base=np.array(rv["base"][: 3 * K], np.float32), load_index=[i + 1 for i in idx],
+                R=mR[idx].reshape(-1), T=vT[idx].reshape(-1), O=off[idx].reshape(-1),
+                geom=(t["sod"], t["sdd"], t["proj"], t["num_det_u"], t["num_det_v"], t["det_size"], t["num_voxel"]),
+                paths=(t["proj_path"], t["vol_path"], t["direc_path"]))
+
+
+@pytest.mark.parametrize("tag", ["oilpan", "phaseCT", "cfrp_7d_13rot", "cfrp_7d_6rot", "simulation"])
+def test_reference_settings_every_tag(xr, tag, tmp_path):
+    """Every tag of the reference's utility/settings.json loads through
+    xr_load_settings exactly as init_params would set it up."""
+    import json
+
+    if not os.path.exists(REF_SETTINGS):
+        pytest.skip("reference tree not present")
+    with open(REF_SETTINGS) as f:
+        data = json.load(f)
+    path = REF_SETTINGS
+    t = data[tag]
+    if not t.get("sorting", False) and data["recon_variable"]["condition"] > t["rot"]:
+        # recon_variable.condition (7) exceeds this tag's rotations without an
+        # index: the reference would run the extra conditions on zero-filled
+        # constant tables (params.cu:88-94); the library refuses instead ...
+        lib = xr.lib()
+        assert lib.xr_load_settings(path.encode(), tag.encode(), ctypes.byref(xr.Settings())) == -1
+        # ... and loads the tag once the condition count matches its rotations
+        data["recon_variable"]["condition"] = t["rot"]
+        path = os.path.join(tmp_path, "settings.json")
+        with open(path, "w") as f:
+            json.dump(data, f)
+    want = restate_init_params(data, tag)
+    st, g, scan = xr.load_settings(path, tag)
+    assert (st.n_cond, st.n_basis) == (want["n_cond"], want["n_basis"])
+    assert list(st.load_index[: st.n_cond]) == want["load_index"]
+    assert np.array_equal(scan.cond_R, want["R"])
+    assert np.array_equal(scan.cond_t, want["T"])
+    assert np.array_equal(scan.cond_offset, want["O"])
+    assert np.array_equal(np.ctypeslib.as_array(st.basis)[: 3 * st.n_basis], want["base"])
+    sod, sdd, P, U, V, det, N = want["geom"]
+    assert (st.n_proj, st.n_det_u, st.n_det_v, st.n_vox) == (P, U, V, N)
+    assert (st.sod, st.sdd, st.det_size) == (np.float32(sod), np.float32(sdd), np.float32(det))
+    assert (g.n_vox, g.n_det_u, g.n_det_v, g.n_proj) == (N, U, V, P)
+    assert g.vox_size == np.float32(np.float32(sod) * np.float32(det)) / np.float32(sdd)
+    assert (st.proj_path.decode(), st.vol_path.decode(), st.direc_path.decode()) == want["paths"]

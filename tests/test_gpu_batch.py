@@ -295,13 +295,15 @@ def test_forward_work_split_is_exact(orc, xr, ctx, torch_dev, tilt, det):
     plan.close()
 
 
-@pytest.mark.parametrize("method", ["mlem", "art"])
-def test_solver_max_bound_equals_measured(orc, xr, ctx, torch_dev, method):
+@pytest.mark.parametrize("method,subsets", [("mlem", 1), ("art", 1), ("mlem", 4), ("art", 3)])
+def test_solver_max_bound_equals_measured(orc, xr, ctx, torch_dev, method, subsets):
     """The solver takes max |voxel| for the forward's fixed-point scale from its
     previous backprojection instead of a pass over the volume.  Three epochs of
-    xr_ir_reconstruct with one subset (every view, in order) equal the same
-    epochs driven through xr_project / xr_backproject_update, which measure
-    the maximum from the volume at every call, bit for bit."""
+    xr_ir_reconstruct with `subsets` ordered subsets equal the same subset
+    steps (same order: libstdc++ std::shuffle with one default-seeded
+    mt19937_64 per call, reconstruct.cu:61-89) driven through xr_project /
+    xr_backproject_update, which measure the maximum from the volume at every
+    call, bit for bit."""
     import torch
 
     m = xr.METHOD_MLEM if method == "mlem" else xr.METHOD_ART
@@ -320,15 +322,20 @@ def test_solver_max_bound_equals_measured(orc, xr, ctx, torch_dev, method):
     plan.project(col, y, codes, out_mode=xr.OUT_NORMALISED)
     ctx.synchronize()
     epochs = 3
+    assert P % subsets == 0  # no view repeats within an epoch
+    ss = P // subsets
     v0 = np.full(N ** 3, 0.01, np.float32)
     solved = [v0.copy()]
-    ctx.ir_reconstruct(g, xr.Scan(), [y.cpu().numpy()], solved, epochs, 1, method=m, lam=0.5)
+    ctx.ir_reconstruct(g, xr.Scan(), [y.cpu().numpy()], solved, epochs, subsets, method=m, lam=0.5)
     cur = torch.empty_like(ref)
     ctx.to_columns(torch.from_numpy(v0).to(torch_dev), cur, N)
-    for _ in range(epochs):
+    orders = orc.subset_orders(subsets, epochs, False)
+    for ep in range(epochs):
         p = torch.zeros(P * U * U, dtype=torch.float32, device=torch_dev)
-        plan.project(cur, p, codes, out_mode=out_mode, sino=y)
-        plan.backproject_update(p, codes, cur, method=m, lam=0.5, subset_size=P)
+        for sub in orders[ep]:
+            sc = plan.codes([(int(sub) + subsets * j) % P for j in range(ss)])
+            plan.project(cur, p, sc, out_mode=out_mode, sino=y)
+            plan.backproject_update(p, sc, cur, method=m, lam=0.5, subset_size=ss)
     out = torch.empty_like(cur)
     ctx.from_columns(cur, out, N)
     ctx.synchronize()

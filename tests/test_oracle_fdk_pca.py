@@ -118,3 +118,34 @@ def test_pca_random_against_numpy(orc):
         e = V[:, 0] * (1 if V[2, 0] >= 0 else -1)
         assert np.allclose(md[:, i], m.mean() * e, atol=1e-9)
         assert np.allclose(ev[:, i], w / w.sum(), atol=1e-12)
+
+
+def test_fdk_distance_weight_closed_form(orc):
+    # FDK backprojection of a constant filtered image c over one view n adds
+    # c * (sum of bilinear weights = 1) * Ud^2 * 2 pi / P to every accepted
+    # voxel, Ud = sod / (sod + y_v) with y_v the voxel's UN-rotated y
+    # coordinate, (y - (N-1)/2) vs - offset_y - t_y (fdk.cu:129-135, 160-169).
+    # At 90 degrees the rotated y would be the voxel's x: a variant using it,
+    # a sign slip (sod - y) or a missing weight all fail below.
+    N, U, P = 12, 64, 8
+    ty = 0.37
+    case = orc.Case(600.0, 300.0, 0.8, N, U, P, cond_t=[0.0, ty, 0.0])
+    vs = case.g.vox_size
+    sod = case.g.sod
+    c = 0.75
+    img = np.full((U, U), c)
+    idx = np.arange(N) - (N - 1) / 2.0
+    yv = idx[None, :, None] * vs - ty  # [z][y][x] -> depends on y only
+    xv = idx[None, None, :] * vs
+    for n in (0, 2):  # 0 and 90 degrees
+        vol = np.zeros(N ** 3)
+        case.fdk_backproject_view(img, n, vol)
+        vol = vol.reshape(N, N, N)
+        want = c * (sod / (sod + yv)) ** 2 * 2 * np.pi / P * np.ones((N, N, N))
+        assert np.allclose(vol, want, rtol=1e-6), n  # FP32 geometry inputs
+        if n == 2:
+            rotated = c * (sod / (sod + xv - ty)) ** 2 * 2 * np.pi / P * np.ones((N, N, N))
+            assert np.abs(vol - rotated).max() > 1e-3 * np.abs(want).max()
+        flipped = c * (sod / (sod - yv)) ** 2 * 2 * np.pi / P
+        assert np.abs(vol - flipped).max() > 1e-3 * np.abs(want).max()
+        assert np.abs(vol - c * 2 * np.pi / P).max() > 1e-3 * np.abs(want).max()

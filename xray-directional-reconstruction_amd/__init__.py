@@ -104,6 +104,7 @@ def _load():
         "xr_volume_to_columns": (I, [P, P, P, I, I]),
         "xr_volume_from_columns": (I, [P, P, P, I, I]),
         "xr_project": (I, [P, P, P, P, P, _c_int32_p, I, _c_uint8_p, I, I, _c_double_p]),
+        "xr_residual": (I, [P, P, P, P, P, _c_int32_p, I, I, _c_double_p, _c_double_p]),
         "xr_backproject": (I, [P, P, P, _c_int32_p, I, I, P, P]),
         "xr_backproject_update": (I, [P, P, P, _c_int32_p, I, I, I, F, I, P]),
         "xr_fdk_filter": (I, [P, P, I, P, P, _c_int32_p, I]),
@@ -132,6 +133,11 @@ def _load():
         "xr_phantom_fibres_scalar": (I, [P, _c_double_p, I, I, F, F, P]),
         "xr_noise_poisson": (I, [P, P, P, ctypes.c_int64, D, ctypes.c_uint64]),
         "xr_noise_gaussian": (I, [P, P, P, ctypes.c_int64, D, ctypes.c_uint64]),
+        "xr_sino_clamp": (I, [P, P, ctypes.c_int64]),
+        "xr_md_angles": (I, [P, P, P, ctypes.c_int64]),
+        "xr_md_normalise": (I, [P, P, ctypes.c_int64, F]),
+        "xr_md_angles_host": (I, [P, _c_float_p, _c_float_p, ctypes.c_int64]),
+        "xr_md_normalise_host": (I, [P, _c_float_p, ctypes.c_int64, F]),
         "xr_philox4x32": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
                                  ctypes.POINTER(ctypes.c_uint32)]),
         "xr_plan_set_tiles": (I, [P, ctypes.POINTER(Tiles)]),
@@ -327,6 +333,29 @@ class Context:
     def noise_gaussian(self, y, x, sigma, seed):
         self.check(_lib.xr_noise_gaussian(self.h, _dptr(y), _dptr(x), x.numel(), sigma, seed))
 
+    def md_angles(self, md, angle, nvox):
+        """calcAngleFromMD on device buffers: md 3 x nvox, angle 2 x nvox."""
+        self.check(_lib.xr_md_angles(self.h, _dptr(md), _dptr(angle), nvox))
+
+    def md_normalise(self, md, nvox, eps=0.65):
+        """directionNormalization on a device buffer of 3 x nvox (in place)."""
+        self.check(_lib.xr_md_normalise(self.h, _dptr(md), nvox, eps))
+
+    def md_angles_host(self, md):
+        md = np.ascontiguousarray(md, dtype=np.float32)
+        out = np.zeros((2, md.shape[1]), np.float32)
+        self.check(_lib.xr_md_angles_host(self.h, _fptr(md), _fptr(out), md.shape[1]))
+        return out
+
+    def md_normalise_host(self, md, eps=0.65):
+        out = np.array(md, dtype=np.float32, order="C")
+        self.check(_lib.xr_md_normalise_host(self.h, _fptr(out), out.shape[1], eps))
+        return out
+
+    def sino_clamp(self, sino):
+        """main.cu:35: sinogram values below 0 become 1e-8 (in place)."""
+        self.check(_lib.xr_sino_clamp(self.h, _dptr(sino), sino.numel()))
+
     def pca_host(self, mu, basis):
         mu = np.ascontiguousarray(mu, dtype=np.float32)
         K, nv = mu.shape
@@ -400,9 +429,12 @@ class Plan:
         self.h = h
 
     def close(self):
-        if self.h:
+        # objects of a context must go before it; once the context is gone
+        # the handle is dropped (its device memory stays until the process
+        # ends) rather than destroyed through a dangling context
+        if self.h and self.ctx.h:
             _lib.xr_plan_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -422,6 +454,15 @@ class Plan:
                                        None if acc is None else acc.ctypes.data_as(_c_uint8_p), int(xtt), out_mode,
                                        ctypes.byref(loss)))
         return loss.value
+
+    def residual(self, vol, resid, sino, codes, xtt=False):
+        """(sum |y - p|, sum (y - p)^2) over the listed views; resid gets y - p."""
+        c = np.ascontiguousarray(codes, dtype=np.int32)
+        l1, l2 = ctypes.c_double(), ctypes.c_double()
+        self.ctx.check(_lib.xr_residual(self.ctx.h, self.h, _dptr(vol), _dptr(resid), _dptr(sino),
+                                        c.ctypes.data_as(_c_int32_p), len(c), int(xtt), ctypes.byref(l1),
+                                        ctypes.byref(l2)))
+        return l1.value, l2.value
 
     def backproject(self, proj, codes, tmp, factor, xtt=False):
         c = np.ascontiguousarray(codes, dtype=np.int32)
@@ -491,9 +532,12 @@ class Batch:
         self.n = n
 
     def close(self):
-        if self.h:
+        # objects of a context must go before it; once the context is gone
+        # the handle is dropped (its device memory stays until the process
+        # ends) rather than destroyed through a dangling context
+        if self.h and self.ctx.h:
             _lib.xr_batch_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -540,9 +584,12 @@ class UNet:
         self.n_params = int(_lib.xr_unet_num_params(h))
 
     def close(self):
-        if self.h:
+        # objects of a context must go before it; once the context is gone
+        # the handle is dropped (its device memory stays until the process
+        # ends) rather than destroyed through a dangling context
+        if self.h and self.ctx.h:
             _lib.xr_unet_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -1,8 +1,11 @@
 // Batched multi-problem IR solver and projector tile autotuner (C ABI in
 // include/xr.h).
 //
-// A batch holds n independent IR problems (one condition each, any geometry,
-// view count and subset count).  A subset step of every problem is two
+// A batch holds n independent problems of one method (MLEM or ART): IR
+// problems (one condition each) and XTT problems (any number of conditions,
+// K scattering-direction coefficients), each with any geometry, view count and
+// subset count.  IR problems and XTT problems each run in their own persistent
+// launches of a subset step.  A subset step of every problem is two
 // persistent launches: k_forward_batch over all (problem, row chunk, band,
 // view) blocks and k_backward_batch over all (problem, voxel tile) blocks,
 // each drawing work from a self-resetting ticket queue in descending order of
@@ -24,6 +27,7 @@
 #include <memory>
 #include <numeric>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/xr.h"
@@ -86,6 +90,33 @@ void problem_codes(int P, int batch, int maxb, int epochs, std::mt19937_64& rng,
     }
 }
 
+// XTT::reconstruct's subset lists (reconstruct.cu:843-870): a default-seeded
+// engine created every epoch shuffles the order left by the previous epoch;
+// step pos holds, condition after condition, the views of subset order[pos]
+// (codes cond * P + view), laid out [epoch][pos][cond][j].
+void problem_codes_xtt(int P, int C, int batch, int maxb, int epochs, std::vector<int>& order,
+                       std::vector<int32_t>& codes, std::vector<uint8_t>& flags, bool& any_flag) {
+    const int ss = (P + batch - 1) / batch;
+    codes.assign((size_t)epochs * maxb * C * ss, 0);
+    flags.assign(codes.size(), 0);
+    any_flag = false;
+    for (int ep = 0; ep < epochs; ep++) {
+        std::mt19937_64 rng;
+        std::shuffle(order.begin(), order.end(), rng);
+        std::vector<char> visited((size_t)C * P, 0);
+        for (int pos = 0; pos < batch; pos++)
+            for (int c = 0; c < C; c++)
+                for (int j = 0; j < ss; j++) {
+                    const int code = c * P + (order[pos] + batch * j) % P;
+                    const size_t i = (((size_t)ep * maxb + pos) * C + c) * ss + j;
+                    codes[i] = code;
+                    flags[i] = visited[code];
+                    any_flag = any_flag || visited[code];
+                    visited[code] = 1;
+                }
+    }
+}
+
 }  // namespace
 
 struct xr_batch {
@@ -96,6 +127,9 @@ struct xr_batch {
     struct Problem {
         xr_plan* plan = nullptr;
         int N = 0, U = 0, V = 0, P = 0, batch = 1, ss = 0;
+        int C = 1, K = 0;  // conditions, basis vectors (K > 0: an XTT problem)
+        bool xtt() const { return K > 0; }
+        int64_t vol_len() const { return (int64_t)N * N * N * (K > 0 ? K : 1); }
         float* vol = nullptr;
         float* sino = nullptr;
         float* proj = nullptr;
@@ -121,6 +155,12 @@ struct xr_batch {
         int* fwd_end = nullptr;
         BwdArgs* bwd = nullptr;
         int* bwd_end = nullptr;
+        // XTT problems: their own forward and backprojection launches
+        int n_fwd_x = 0, n_bwd_x = 0;
+        FwdArgs* fwd_x = nullptr;
+        int* fwd_x_end = nullptr;
+        BwdArgs* bwd_x = nullptr;
+        int* bwd_x_end = nullptr;
         const float** segs = nullptr;  // volumes of the active problems (d_vmax order)
         int64_t* seg_len = nullptr;
         int absmax_blocks = 1;  // blocks per segment of the max |voxel| launch
@@ -131,6 +171,7 @@ struct xr_batch {
     double* d_loss = nullptr;  // [epoch_cap][n]
     unsigned* d_queue = nullptr;  // [4]
     int fwd_grid = 0, bwd_grid[3] = {0, 0, 0}, ticket_slot = 0;
+    int fwd_grid_x = 0, bwd_grid_x = 0, kmax = 0;  // XTT launches; largest basis count
     bool fwd_axis_all = false;  // every problem untilted: the untilted-only forward kernel
     size_t fwd_lds = 0;
     hipGraph_t graph = nullptr;
@@ -147,6 +188,10 @@ struct xr_batch {
             if (s.bwd_end) (void)hipFree(s.bwd_end);
             if (s.segs) (void)hipFree(s.segs);
             if (s.seg_len) (void)hipFree(s.seg_len);
+            if (s.fwd_x) (void)hipFree(s.fwd_x);
+            if (s.fwd_x_end) (void)hipFree(s.fwd_x_end);
+            if (s.bwd_x) (void)hipFree(s.bwd_x);
+            if (s.bwd_x_end) (void)hipFree(s.bwd_x_end);
         }
         steps.clear();
         tables_ok = false;
@@ -194,7 +239,7 @@ int ensure_epochs(xr_batch* b, int epochs) {
         if (p.flags) (void)hipFree(p.flags);
         p.codes = nullptr;
         p.flags = nullptr;
-        const size_t cnt = (size_t)epochs * b->maxb * p.ss;
+        const size_t cnt = (size_t)epochs * b->maxb * p.C * p.ss;
         XR_HIP(ctx, hipMalloc((void**)&p.codes, cnt * sizeof(int32_t)));
         XR_HIP(ctx, hipMalloc((void**)&p.flags, cnt));
     }
@@ -214,39 +259,56 @@ int build_tables(xr_batch* b) {
     const int out_mode = mlem ? OUT_RATIO : OUT_RESIDUAL;
     // Persistent grids: every block resident at once.  The forward block's LDS
     // holds the tallest row chunk of any problem.
-    int max_vb = 1, split_tile = 0;
+    int max_vb = 1;
+    bool any_plain = false, any_xtt = false;
+    b->kmax = 0;
     for (const auto& p : b->pr) {
-        max_vb = std::max(max_vb, std::min(forward_rows(p.plan, FWD_PLAIN), p.V));
-        if (split_tile == 0) split_tile = p.plan->tiles.fwd_split;
+        max_vb = std::max(max_vb, std::min(forward_rows(p.plan, p.xtt() ? FWD_XTT : FWD_PLAIN), p.V));
+        any_plain = any_plain || !p.xtt();
+        any_xtt = any_xtt || p.xtt();
+        b->kmax = std::max(b->kmax, p.K);
     }
     hipDeviceProp_t prop;
     XR_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
     b->fwd_axis_all = true;
-    for (const auto& p : b->pr) b->fwd_axis_all = b->fwd_axis_all && p.plan->axis_cond[0];
-    // past the 32-bit band and the reduction slots (one per wave of either kernel)
-    b->ticket_slot = fwd_red_slot(max_vb + 2, 4) + std::max(kFwdWaves, kFwdBatchWaves);
+    for (const auto& p : b->pr)
+        if (!p.xtt()) b->fwd_axis_all = b->fwd_axis_all && p.plan->axis_cond[0];
+    // past the 32-bit band and the reduction slots (two per wave of any kernel)
+    b->ticket_slot = fwd_red_slot(max_vb + 2, 4) + 2 * std::max(kFwdWaves, kFwdBatchWaves);
     b->fwd_lds = sizeof(unsigned long long) * ((size_t)b->ticket_slot + 1);
-    const int fpc = batch_blocks_per_cu_forward(FWD_PLAIN, b->fwd_lds, b->fwd_axis_all);
-    if (fpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched forward kernel");
-    b->fwd_grid = fpc * prop.multiProcessorCount;
-    for (int c = 0; c < 3; c++) {
-        const int bpc = batch_blocks_per_cu_backward(mlem ? BWD_MLEM : BWD_ART, c == 2 ? 4 : c + 1);
-        if (bpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched backprojection");
-        b->bwd_grid[c] = bpc * prop.multiProcessorCount;
+    b->fwd_grid = b->fwd_grid_x = b->bwd_grid_x = 0;
+    if (any_plain) {
+        const int fpc = batch_blocks_per_cu_forward(FWD_PLAIN, b->fwd_lds, b->fwd_axis_all);
+        if (fpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched forward kernel");
+        b->fwd_grid = fpc * prop.multiProcessorCount;
+        for (int c = 0; c < 3; c++) {
+            const int bpc = batch_blocks_per_cu_backward(mlem ? BWD_MLEM : BWD_ART, c == 2 ? 4 : c + 1);
+            if (bpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched backprojection");
+            b->bwd_grid[c] = bpc * prop.multiProcessorCount;
+        }
+    }
+    if (any_xtt) {
+        const int fpc = batch_blocks_per_cu_forward(FWD_XTT, b->fwd_lds, false, b->kmax);
+        const int bpc = batch_blocks_per_cu_backward(mlem ? BWD_XTT_MLEM : BWD_XTT_ART, 1, b->kmax);
+        if (fpc <= 0 || bpc <= 0) return fail(ctx, XR_ERR_HIP, "occupancy query failed for the batched XTT kernels");
+        b->fwd_grid_x = fpc * prop.multiProcessorCount;
+        b->bwd_grid_x = bpc * prop.multiProcessorCount;
     }
     b->steps.resize(b->maxb);
     for (int pos = 0; pos < b->maxb; pos++) {
-        std::vector<FwdArgs> fi;
-        std::vector<int> fe;
-        std::vector<BwdArgs> bz[3];  // items by z depth class (zpl 1, 2, 4)
+        std::vector<FwdArgs> fi, fx;
+        std::vector<int> fsplit, fxsplit;  // each item's own plan setting of the band split (0: automatic)
+        std::vector<int> fe, fxe;
+        std::vector<BwdArgs> bz[3], bx;  // plain items by z depth class (zpl 1, 2, 4); XTT items
         std::vector<const float*> segs;
         std::vector<int64_t> lens;
-        int fsum = 0;
+        int fsum = 0, fxsum = 0;
         for (int oi = 0; oi < (int)b->order.size(); oi++) {
             const int k = b->order[oi];
             xr_batch::Problem& p = b->pr[k];
             if (pos >= p.batch) continue;
             xr_plan* pl = p.plan;
+            const bool xtt = p.xtt();
             const int64_t N3 = (int64_t)p.N * p.N * p.N;
             FwdArgs a{};
             a.d = pl->dims;
@@ -263,64 +325,89 @@ int build_tables(xr_batch* b) {
             a.norm = pl->d_norm;
             a.bands = pl->tiles.fwd_scan ? nullptr : pl->d_bands;
             a.g2 = pl->d_g2;
+            a.xw = pl->d_xw;
             a.loss = b->d_loss + k;
             // max |voxel| of problem oi: measured by k_absmax_seg at the first
             // subset step (every problem is active there, slot == oi), then by
             // the previous step's backprojection (which updates every voxel)
             a.vmax = b->d_vmax + oi;
-            a.wmul = pl->dims.ratio * 1.0001f;  // as forward_launch
+            // as forward_launch
+            a.wmul = xtt ? pl->dims.vs * pl->dims.ratio * pl->xtt_wsum * 1.0001f : pl->dims.ratio * 1.0001f;
             a.fsum = pl->max_factor * 1.01f + 1.0f;
             a.code_base = 0;
             a.out_mode = out_mode;
             a.n_bands = (p.U + kBandU - 1) / kBandU;
-            a.n_slots = p.ss;
-            a.axis = pl->axis_cond[0] ? 1 : 0;
-            const int rows = forward_rows(pl, FWD_PLAIN);
+            a.n_slots = p.C * p.ss;
+            a.axis = 1;
+            for (int c = 0; c < p.C; c++) a.axis = a.axis && pl->axis_cond[c];
+            const int rows = forward_rows(pl, xtt ? FWD_XTT : FWD_PLAIN);
             a.n_chunks = 1;
             for (int v0 = 0; v0 < p.V; v0 += rows) {
                 a.v0 = v0;
                 a.vb = std::min(rows, p.V - v0);
-                fi.push_back(a);
-                fsum += a.n_bands * a.n_slots;
+                if (xtt) {
+                    fx.push_back(a);
+                    fxsplit.push_back(pl->tiles.fwd_split);
+                    fxsum += a.n_bands * a.n_slots;
+                } else {
+                    fi.push_back(a);
+                    fsplit.push_back(pl->tiles.fwd_split);
+                    fsum += a.n_bands * a.n_slots;
+                }
             }
             BwdArgs w{};
             w.d = pl->dims;
             w.b = pl->bas;
             w.vg = pl->d_vg;
             w.views = p.codes;
-            w.n_views = p.ss;
+            w.n_views = p.C * p.ss;
             w.proj = p.proj;
             w.code_base = 0;
             w.vol = p.vol;
             w.chan_stride = N3;
             w.g2 = pl->d_g2;
-            w.alpha = mlem ? 0.f : b->lambda / (float)p.ss;  // reconstruct.cu:124
-            w.axis = pl->axis_cond[0] ? 1 : 0;
+            w.xw = pl->d_xw;
+            w.alpha = mlem ? 0.f : b->lambda / (float)p.ss;  // reconstruct.cu:124, 911
+            w.axis = a.axis;
             w.vmax_out = (pos + 1 < p.batch) ? reinterpret_cast<unsigned*>(b->d_vmax + oi) : nullptr;
-            if (!backward_shape(w, pl->tiles.bwd_zpl, pl->tiles.bwd_gx, false))
+            if (!backward_shape(w, xtt ? 0 : pl->tiles.bwd_zpl, pl->tiles.bwd_gx, xtt) || (xtt && w.zpl != 1))
                 return fail(ctx, XR_ERR_INVALID, "bad tile shape for problem %d", k);
-            bz[w.zpl == 4 ? 2 : w.zpl - 1].push_back(w);
+            if (xtt)
+                bx.push_back(w);
+            else
+                bz[w.zpl == 4 ? 2 : w.zpl - 1].push_back(w);
             segs.push_back(p.vol);
-            lens.push_back(N3);
+            lens.push_back(p.vol_len());
         }
         // Few items in this step (small subsets): split the bands so that the
-        // persistent grid is filled.
-        const int split = forward_split(fsum, b->fwd_grid, split_tile);
+        // persistent grid is filled.  A problem whose plan sets the split
+        // (xr_plan_set_tiles, autotune) keeps its own.
+        const int split = forward_split(fsum, std::max(b->fwd_grid, 1), 0);
         fsum = 0;
-        for (FwdArgs& a : fi) {
-            a.n_sub = split;
-            fsum += a.n_bands * split * a.n_slots;
+        for (size_t i = 0; i < fi.size(); i++) {
+            FwdArgs& a = fi[i];
+            a.n_sub = fsplit[i] > 0 ? fsplit[i] : split;
+            fsum += a.n_bands * a.n_sub * a.n_slots;
             fe.push_back(fsum);
+        }
+        const int xsplit = forward_split(fxsum, std::max(b->fwd_grid_x, 1), 0);
+        fxsum = 0;
+        for (size_t i = 0; i < fx.size(); i++) {
+            FwdArgs& a = fx[i];
+            a.n_sub = fxsplit[i] > 0 ? fxsplit[i] : xsplit;
+            fxsum += a.n_bands * a.n_sub * a.n_slots;
+            fxe.push_back(fxsum);
         }
         xr_batch::Step& s = b->steps[pos];
         s.n_fwd = (int)fi.size();
+        s.n_fwd_x = (int)fx.size();
         s.n_active = (int)segs.size();
-        if (s.n_fwd == 0) continue;
-        s.absmax_blocks = lens.empty() ? 1 : absmax_blocks(*std::max_element(lens.begin(), lens.end()), s.n_active);
+        if (s.n_active == 0) continue;
+        s.absmax_blocks = absmax_blocks(*std::max_element(lens.begin(), lens.end()), s.n_active);
         // backprojection items by z depth class (zpl 4, 2, 1): one launch and
         // one ticket range per class, so prefix sums restart in each class
         std::vector<BwdArgs> bi;
-        std::vector<int> be;
+        std::vector<int> be, bxe;
         for (int c = 2; c >= 0; c--) {
             s.bwd_first[c] = (int)bi.size();
             s.bwd_count[c] = (int)bz[c].size();
@@ -332,18 +419,31 @@ int build_tables(xr_batch* b) {
             }
         }
         s.n_bwd = (int)bi.size();
-        XR_HIP(ctx, hipMalloc((void**)&s.fwd, sizeof(FwdArgs) * fi.size()));
-        XR_HIP(ctx, hipMalloc((void**)&s.fwd_end, sizeof(int) * fe.size()));
-        XR_HIP(ctx, hipMalloc((void**)&s.bwd, sizeof(BwdArgs) * bi.size()));
-        XR_HIP(ctx, hipMalloc((void**)&s.bwd_end, sizeof(int) * be.size()));
-        XR_HIP(ctx, hipMalloc((void**)&s.segs, sizeof(float*) * segs.size()));
-        XR_HIP(ctx, hipMalloc((void**)&s.seg_len, sizeof(int64_t) * lens.size()));
-        XR_HIP(ctx, hipMemcpy(s.fwd, fi.data(), sizeof(FwdArgs) * fi.size(), hipMemcpyHostToDevice));
-        XR_HIP(ctx, hipMemcpy(s.fwd_end, fe.data(), sizeof(int) * fe.size(), hipMemcpyHostToDevice));
-        XR_HIP(ctx, hipMemcpy(s.bwd, bi.data(), sizeof(BwdArgs) * bi.size(), hipMemcpyHostToDevice));
-        XR_HIP(ctx, hipMemcpy(s.bwd_end, be.data(), sizeof(int) * be.size(), hipMemcpyHostToDevice));
-        XR_HIP(ctx, hipMemcpy(s.segs, segs.data(), sizeof(float*) * segs.size(), hipMemcpyHostToDevice));
-        XR_HIP(ctx, hipMemcpy(s.seg_len, lens.data(), sizeof(int64_t) * lens.size(), hipMemcpyHostToDevice));
+        s.n_bwd_x = (int)bx.size();
+        {
+            int run = 0;
+            for (const BwdArgs& w : bx) {
+                run += w.n_tiles;
+                bxe.push_back(run);
+            }
+        }
+        auto upload = [&](auto*& dst, const auto& src) -> int {
+            using T = typename std::remove_reference<decltype(src)>::type::value_type;
+            if (src.empty()) return XR_OK;
+            XR_HIP(ctx, hipMalloc((void**)&dst, sizeof(T) * src.size()));
+            XR_HIP(ctx, hipMemcpy(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+            return XR_OK;
+        };
+        XR_TRY(upload(s.fwd, fi));
+        XR_TRY(upload(s.fwd_end, fe));
+        XR_TRY(upload(s.bwd, bi));
+        XR_TRY(upload(s.bwd_end, be));
+        XR_TRY(upload(s.fwd_x, fx));
+        XR_TRY(upload(s.fwd_x_end, fxe));
+        XR_TRY(upload(s.bwd_x, bx));
+        XR_TRY(upload(s.bwd_x_end, bxe));
+        XR_TRY(upload(s.segs, segs));
+        XR_TRY(upload(s.seg_len, lens));
     }
     b->tables_ok = true;
     return XR_OK;
@@ -359,7 +459,7 @@ int enqueue_epochs(xr_batch* b, int epochs) {
     for (int ep = 0; ep < epochs; ep++) {
         for (int pos = 0; pos < b->maxb; pos++) {
             const xr_batch::Step& s = b->steps[pos];
-            if (s.n_fwd == 0) continue;
+            if (s.n_active == 0) continue;
             const int step = ep * b->maxb + pos;
             // max |voxel| of each problem: read from the volumes at the first
             // step (the caller may have replaced them between runs), then left
@@ -373,6 +473,9 @@ int enqueue_epochs(xr_batch* b, int epochs) {
             }
             e = launch_forward_batch(s.fwd, s.fwd_end, s.n_fwd, b->d_queue, b->ticket_slot, b->fwd_lds, b->fwd_grid,
                                      step, ep * b->n, FWD_PLAIN, b->fwd_axis_all, st);
+            if (e == hipSuccess)
+                e = launch_forward_batch(s.fwd_x, s.fwd_x_end, s.n_fwd_x, b->d_queue, b->ticket_slot, b->fwd_lds,
+                                         b->fwd_grid_x, step, ep * b->n, FWD_XTT, false, st, b->kmax);
             if (e != hipSuccess) return fail(ctx, XR_ERR_HIP, "batched forward failed: %s", hipGetErrorString(e));
             if (pos + 1 < b->maxb) {
                 e = launch_fill(b->d_vmax, b->n, 0.0f, st);
@@ -387,6 +490,10 @@ int enqueue_epochs(xr_batch* b, int epochs) {
                 if (e != hipSuccess)
                     return fail(ctx, XR_ERR_HIP, "batched backprojection failed: %s", hipGetErrorString(e));
             }
+            e = launch_backward_batch(s.bwd_x, s.bwd_x_end, s.n_bwd_x, b->d_queue + 2, b->bwd_grid_x, step,
+                                      mlem ? BWD_XTT_MLEM : BWD_XTT_ART, 1, st, b->kmax);
+            if (e != hipSuccess)
+                return fail(ctx, XR_ERR_HIP, "batched XTT backprojection failed: %s", hipGetErrorString(e));
         }
     }
     return XR_OK;
@@ -548,13 +655,16 @@ int xr_batch_create(xr_context* ctx, int n, const xr_geometry* geoms, const xr_s
     for (int i = 0; i < n; i++) {
         xr_batch::Problem& p = b->pr[i];
         const xr_scan* s = scans ? &scans[i] : &plain;
-        if (s->n_cond != 1 || s->n_basis != 0)
-            return fail(ctx, XR_ERR_INVALID, "problem %d: batched problems have one condition and no basis", i);
+        XR_TRY(check_scan(ctx, s));
+        if (s->n_basis == 0 && s->n_cond != 1)
+            return fail(ctx, XR_ERR_INVALID, "problem %d: an IR problem of a batch has one condition", i);
         XR_TRY(check_geometry(ctx, &geoms[i]));
         p.N = geoms[i].n_vox;
         p.U = geoms[i].n_det_u;
         p.V = geoms[i].n_det_v;
         p.P = geoms[i].n_proj;
+        p.C = s->n_cond;
+        p.K = s->n_basis;
         p.batch = batch ? batch[i] : 1;
         if (p.batch <= 0 || p.batch > p.P) return fail(ctx, XR_ERR_INVALID, "problem %d: bad batch %d", i, p.batch);
         p.ss = (p.P + p.batch - 1) / p.batch;
@@ -562,11 +672,11 @@ int xr_batch_create(xr_context* ctx, int n, const xr_geometry* geoms, const xr_s
         b->maxb = std::max(b->maxb, p.batch);
         XR_TRY(xr_plan_create(ctx, &geoms[i], s, XR_CONV_IR, direction, &p.plan));
         XR_TRY(ensure_norm(p.plan));
-        const int64_t N3 = (int64_t)p.N * p.N * p.N, lenD = (int64_t)p.U * p.V * p.P;
-        XR_HIP(ctx, hipMalloc((void**)&p.vol, sizeof(float) * N3));
+        const int64_t nv = p.vol_len(), lenD = (int64_t)p.U * p.V * p.P * p.C;
+        XR_HIP(ctx, hipMalloc((void**)&p.vol, sizeof(float) * nv));
         XR_HIP(ctx, hipMalloc((void**)&p.sino, sizeof(float) * lenD));
         XR_HIP(ctx, hipMalloc((void**)&p.proj, sizeof(float) * lenD));
-        XR_HIP(ctx, hipMemsetAsync(p.vol, 0, sizeof(float) * N3, ctx->stream));
+        XR_HIP(ctx, hipMemsetAsync(p.vol, 0, sizeof(float) * nv, ctx->stream));
         XR_HIP(ctx, hipMemsetAsync(p.sino, 0, sizeof(float) * lenD, ctx->stream));
     }
     // Longest work first: backprojection cost ~ N^3 P, the dominant term.
@@ -575,7 +685,7 @@ int xr_batch_create(xr_context* ctx, int n, const xr_geometry* geoms, const xr_s
     std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) {
         const auto& a = b->pr[x];
         const auto& c = b->pr[y];
-        return (double)a.N * a.N * a.N * a.ss > (double)c.N * c.N * c.N * c.ss;
+        return (double)a.vol_len() * a.C * a.ss > (double)c.vol_len() * c.C * c.ss;
     });
     XR_HIP(ctx, hipMalloc((void**)&b->d_vmax, sizeof(float) * n));
     XR_HIP(ctx, hipMalloc((void**)&b->d_queue, sizeof(unsigned) * 4));
@@ -603,7 +713,8 @@ int xr_batch_autotune(xr_batch* b) {
     if (!b) return XR_ERR_INVALID;
     xr_context* ctx = b->ctx;
     XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (auto& p : b->pr) XR_TRY(xr_autotune(ctx, p.plan, p.vol, p.sino, nullptr, nullptr));
+    for (auto& p : b->pr)
+        if (!p.xtt()) XR_TRY(xr_autotune(ctx, p.plan, p.vol, p.sino, nullptr, nullptr));
     b->free_tables();
     return XR_OK;
 }
@@ -622,7 +733,10 @@ int xr_batch_run(xr_batch* b, int epochs, double* losses) {
     std::vector<int32_t> codes;
     std::vector<uint8_t> fl;
     for (auto& p : b->pr) {
-        problem_codes(p.P, p.batch, b->maxb, epochs, p.rng, p.order, codes, fl, p.any_flag);
+        if (p.xtt())
+            problem_codes_xtt(p.P, p.C, p.batch, b->maxb, epochs, p.order, codes, fl, p.any_flag);
+        else
+            problem_codes(p.P, p.batch, b->maxb, epochs, p.rng, p.order, codes, fl, p.any_flag);
         XR_HIP(ctx, hipMemcpyAsync(p.codes, codes.data(), sizeof(int32_t) * codes.size(), hipMemcpyHostToDevice,
                                    b->stream));
         XR_HIP(ctx, hipMemcpyAsync(p.flags, fl.data(), fl.size(), hipMemcpyHostToDevice, b->stream));
@@ -682,24 +796,31 @@ int xr_batch_reconstruct(xr_context* ctx, int n, const xr_geometry* geoms, const
     xr_batch* b = nullptr;
     XR_TRY(xr_batch_create(ctx, n, geoms, scans, batch, direction, method, lambda, flags, &b));
     std::unique_ptr<xr_batch, void (*)(xr_batch*)> guard(b, xr_batch_destroy);
-    int64_t maxN3 = 0;
-    for (auto& p : b->pr) maxN3 = std::max(maxN3, (int64_t)p.N * p.N * p.N);
+    int64_t maxlen = 0;
+    for (auto& p : b->pr) maxlen = std::max(maxlen, p.vol_len());
     DevBuf<float> ref;
-    XR_HIP(ctx, ref.alloc(maxN3));
+    XR_HIP(ctx, ref.alloc(maxlen));
     for (int i = 0; i < n; i++) {
         auto& p = b->pr[i];
-        const int64_t N3 = (int64_t)p.N * p.N * p.N, lenD = (int64_t)p.U * p.V * p.P;
+        const int64_t nv = p.vol_len(), lenD = (int64_t)p.U * p.V * p.P * p.C;
         XR_HIP(ctx, hipMemcpyAsync(p.sino, sino[i], sizeof(float) * lenD, hipMemcpyHostToDevice, ctx->stream));
-        XR_HIP(ctx, hipMemcpyAsync(ref.p, vol[i], sizeof(float) * N3, hipMemcpyHostToDevice, ctx->stream));
-        XR_TRY(xr_volume_to_columns(ctx, ref.p, p.vol, p.N, 1));
+        XR_HIP(ctx, hipMemcpyAsync(ref.p, vol[i], sizeof(float) * nv, hipMemcpyHostToDevice, ctx->stream));
+        XR_TRY(xr_volume_to_columns(ctx, ref.p, p.vol, p.N, std::max(p.K, 1)));
         XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     XR_TRY(xr_batch_run(b, epochs, losses));
     for (int i = 0; i < n; i++) {
         auto& p = b->pr[i];
-        const int64_t N3 = (int64_t)p.N * p.N * p.N;
-        XR_TRY(xr_volume_from_columns(ctx, p.vol, ref.p, p.N, 1));
-        XR_HIP(ctx, hipMemcpyAsync(vol[i], ref.p, sizeof(float) * N3, hipMemcpyDeviceToHost, ctx->stream));
+        const int64_t nv = p.vol_len();
+        if (p.xtt()) {
+            // XTT::reconstruct's ending (reconstruct.cu:919-927): voxelSqrt and the
+            // sinogram overwritten with the final ratio / residual images
+            XR_TRY(xr_volume_sqrt(ctx, p.vol, nv));
+            const int64_t lenD = (int64_t)p.U * p.V * p.P * p.C;
+            XR_HIP(ctx, hipMemcpyAsync(sino[i], p.proj, sizeof(float) * lenD, hipMemcpyDeviceToHost, ctx->stream));
+        }
+        XR_TRY(xr_volume_from_columns(ctx, p.vol, ref.p, p.N, std::max(p.K, 1)));
+        XR_HIP(ctx, hipMemcpyAsync(vol[i], ref.p, sizeof(float) * nv, hipMemcpyDeviceToHost, ctx->stream));
         XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     return XR_OK;

@@ -44,6 +44,7 @@ struct xr_plan {
     xr::ViewGeom* d_vg = nullptr;
     xr::ChordGeom* d_cg = nullptr;
     float* d_g2 = nullptr;    // [n_cond*P][K]: (S_k.G)^2
+    float* d_xw = nullptr;    // [n_cond*P][4][K]: XTT weight constants of each view (BwdArgs::xw)
     float* d_norm = nullptr;  // [n_cond][P][V][U]: L/(factor+1e-5)
     int32_t* d_bands = nullptr;  // [n_cond*P][n_bands][N]: candidate columns xs | xe << 16 of each volume row
     xr::Dims dims{};
@@ -114,9 +115,13 @@ int timed(xr_context* ctx, int kind, F&& launch) {
 // for a single volume; code_base: proj/sino hold the images of codes >= code_base.
 // vmax_known: device max |voxel| of vol already measured (the solver's
 // previous backprojection), or null to measure it here.
+// no_alloc: the call makes no allocation of any kind (the plan's tables must
+// already exist; max |voxel| goes to vmax_buf, cleared by a fill kernel; no
+// timing events).
 int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stride, float* proj, const float* sino,
                    int code_base, const int32_t* d_codes, const uint8_t* d_flags, int n_slots, int out_mode,
-                   double* d_loss, const float* vmax_known = nullptr);
+                   double* d_loss, const float* vmax_known = nullptr, float* vmax_buf = nullptr,
+                   bool no_alloc = false);
 // vmax_out (MLEM / ART kinds): receives the atomic max of |updated voxel| as
 // float bits; the caller clears it first.
 int backward_launch(xr_plan* p, int kind, const float* proj, int code_base, const int32_t* d_codes, int n_views,

@@ -18,6 +18,9 @@ constexpr int kFwdWaves = 12;  // two 12-wave blocks per CU (58 KB band accumula
 // VGPRs: 5 waves per SIMD), so 8-wave blocks fit two per CU where 12-wave ones
 // fit one.
 constexpr int kFwdBatchWaves = 8;
+// The untilted XTT forward keeps K column terms and K coefficient loads per
+// lane (about 120 VGPRs: 4 waves per SIMD): 8-wave blocks, two per CU.
+constexpr int kFwdXttWaves = 8;
 // 8-byte slot index just past a forward band accumulator of `rows` rows of
 // `acc_bytes`-byte words (the per-wave reduction slots follow it).
 __host__ __device__ constexpr int fwd_red_slot(int rows, int acc_bytes) {
@@ -59,7 +62,10 @@ struct ChordGeom {
     double hi[3];
 };
 
-enum OutMode { OUT_RAW = 0, OUT_NORMALISED = 1, OUT_RATIO = 2, OUT_RESIDUAL = 3, OUT_NORM_TABLE = 4 };
+// OUT_RESIDUAL_L2: the residual image of OUT_RESIDUAL, with loss[0] += sum |y - p|
+// and loss[1] += sum (y - p)^2 (two FP64 sums).
+enum OutMode { OUT_RAW = 0, OUT_NORMALISED = 1, OUT_RATIO = 2, OUT_RESIDUAL = 3, OUT_NORM_TABLE = 4,
+               OUT_RESIDUAL_L2 = 5 };
 enum FwdKind { FWD_WEIGHT = 0, FWD_PLAIN = 1, FWD_XTT = 2 };
 enum BwdKind { BWD_RAW = 0, BWD_MLEM = 1, BWD_ART = 2, BWD_FDK = 3, BWD_XTT_RAW = 4, BWD_XTT_MLEM = 5, BWD_XTT_ART = 6 };
 
@@ -93,6 +99,7 @@ struct FwdArgs {
     const float* sino;       // measured data, same layout (RATIO / RESIDUAL)
     const float* norm;       // L/(factor+1e-5), same layout (NORMALISED / RATIO / RESIDUAL)
     const float* g2;         // XTT [n_cond*P][K]: (S_k.G)^2
+    const float* xw;         // untilted XTT [n_cond*P][5][K]: weight constants (see BwdArgs::xw)
     double* loss;            // accumulated sum |y - p| (RATIO / RESIDUAL)
     const float* vmax;       // device scalar: max |voxel value| of vol (1 for the weight pass)
     float wmul;              // bound of a voxel's splat value per unit |voxel value|
@@ -122,6 +129,9 @@ struct BwdArgs {
     float* factor;
     int64_t chan_stride;
     const float* g2;         // XTT [n_cond*P][K]: (S_k.G)^2
+    // untilted XTT [n_cond*P][5][K], with sig_k = |S_k.G|: sig_k S_k,z, (S_k.G)^2 |S_k|^2,
+    // sig_k vs S_k,x, sig_k vs S_k,y, sig_k (d0_x S_k,x + d0_y S_k,y)
+    const float* xw;
     float alpha;             // ART step lambda/subsetSize
     int overwrite;           // FDK: 1 -> vol = sum, 0 -> vol += sum
     int zpl, gx, gy;          // tile shape: z voxels per lane (1, 2, 4), column groups along x, y (gx*gy = 16)
@@ -144,16 +154,17 @@ hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream);
 // Persistent multi-problem launches (see k_forward_batch / k_backward_batch).
 // items / item_end live in device memory; queue: two zeroed words per launch
 // site; grid: resident blocks (batch_grid_*).
+// kind FWD_PLAIN or FWD_XTT (kmax: the largest basis count among the items).
 hipError_t launch_forward_batch(const FwdArgs* items, const int* item_end, int n_items, unsigned* queue,
                                 int ticket_slot, size_t lds_bytes, int grid, int step, int loss_shift, int kind,
-                                bool axis_all,
-                                hipStream_t stream);
+                                bool axis_all, hipStream_t stream, int kmax = 0);
 // One launch per z depth class: every item of the launch has a.zpl == zpl.
+// kind BWD_MLEM / BWD_ART (zpl 1, 2, 4) or BWD_XTT_MLEM / BWD_XTT_ART (zpl 1).
 hipError_t launch_backward_batch(const BwdArgs* items, const int* item_end, int n_items, unsigned* queue, int grid,
-                                 int step, int kind, int zpl, hipStream_t stream);
+                                 int step, int kind, int zpl, hipStream_t stream, int kmax = 0);
 // Resident blocks per CU for the batched kernels (0 on error).
 // axis_all: every item untilted (the untilted-only kernel, kFwdWaves waves per block)
-int batch_blocks_per_cu_forward(int kind, size_t lds_bytes, bool axis_all);
+int batch_blocks_per_cu_forward(int kind, size_t lds_bytes, bool axis_all, int kmax = 0);
 // Dynamic LDS of a standalone forward launch of vb detector rows.
 size_t forward_lds_bytes(int kind, int vb);
 // Resident blocks per CU of the standalone forward kernel.
@@ -167,7 +178,7 @@ inline int forward_split(int64_t items, int64_t slots, int tile) {
     while (s < 4 && items * s * 2 < slots * 3) s *= 2;
     return s;
 }
-int batch_blocks_per_cu_backward(int kind, int zpl);
+int batch_blocks_per_cu_backward(int kind, int zpl, int kmax = 0);
 size_t backward_batch_lds_bytes();
 // out[i] = max |segs[i][0, seg_len[i])| (out zeroed beforehand).
 // blocks per segment for k_absmax_seg over n_seg segments of at most max_len floats
@@ -183,6 +194,9 @@ hipError_t launch_sqrt_abs(float* vol, int64_t n, hipStream_t stream);
 hipError_t launch_pca(const float* mu, int K, const Basis& b, int64_t nvox, int64_t chan_stride, float* md,
                       float* evalue, hipStream_t stream);
 hipError_t launch_fill(float* p, int64_t n, float value, hipStream_t stream);
+// Direction post-processing on 3 channels of n floats (pca.cu:67-106).
+hipError_t launch_md_angles(const float* md, float* angle, int64_t n, hipStream_t stream);
+hipError_t launch_md_normalise(float* md, int64_t n, float eps, hipStream_t stream);
 // out[0] = max |x| over n floats (out must hold 0 beforehand).
 hipError_t launch_absmax(const float* x, int64_t n, float* out, hipStream_t stream);
 // Candidate-column table of the forward scan: for every (code, band, volume

@@ -122,6 +122,22 @@ __device__ __forceinline__ gptr<T> to_global(T* p) {
     return (gptr<T>)p;
 }
 
+// Read-only view of a table that no kernel writes (view lists, per-view
+// geometry and weight tables): loads through the constant address space at a
+// wave-uniform address become scalar loads into SGPRs, instead of vector loads
+// plus a readfirstlane (a VALU instruction) per value.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
+#else
+template <class T>
+using cptr = const T*;
+#endif
+template <class T>
+__device__ __forceinline__ cptr<T> to_const(const T* p) {
+    return (cptr<T>)p;
+}
+
 __device__ __forceinline__ void lds_add(lds_float* p, float v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -183,12 +199,13 @@ __device__ double chord_length(const ChordGeom& c, int u, int v, const Dims& d) 
 // and all z chunks of a column are loaded before any is used.
 // ---------------------------------------------------------------------------
 constexpr int kFwdChunks = 8;  // z chunks of 64 loaded per column per step (512 z)
+constexpr int kXttChunks = 2;  // untilted XTT: z chunks of 64 (x K channels) loaded per step
 
 typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 
 // Band accumulator word: 64-bit for the weight pass (its own sum bound is what
 // it computes), 32-bit otherwise, with the scale chosen from the largest
-// per-pixel weight sum so that no pixel sum can exceed 2^30 in magnitude.
+// per-pixel weight sum so that no pixel sum can reach 2^31 in magnitude.
 template <int KIND>
 struct FwdAcc {
     typedef unsigned int T;
@@ -472,7 +489,7 @@ __global__ void k_band_rows(const ViewGeom* __restrict__ vg, const Dims d, int n
 // One (band, view) work item o = band * n_slots + slot of a forward launch.
 // step selects the view list views + step*n_slots (a solver's subset step);
 // loss receives the item's share of sum |y - p|.
-template <int KIND, bool AXIS, int WAVES>
+template <int KIND, bool AXIS, int WAVES, int KM = kMaxBasis>
 __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, double* loss,
                                              unsigned long long* acc_raw, int v0, int vb) {
     typedef typename FwdAcc<KIND>::T AccT;
@@ -493,31 +510,47 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     }
     const int band = j >> (4 - lw), sub = j & (nsub - 1);
     const int W = 1 << lw, cs = sub << lw;  // the item's columns cs + [0, W) of the band
-    const auto views = to_global(a.views + (int64_t)step * a.n_slots);
+    const auto views = to_const(a.views + (int64_t)step * a.n_slots);
     const auto accum = to_global(a.accum ? a.accum + (int64_t)step * a.n_slots : nullptr);
     const auto proj = to_global(a.proj);
     const int code = views[slot];
     const int cond = code / d.P;
-    const ViewGeom g = uniform_view(to_global(a.vg)[code]);
+    const ViewGeom g = to_const(a.vg)[code];
     const int u0 = band * kBandU;
     const int rows = vb + 2;
     for (int i = threadIdx.x; i < rows * kBandStride; i += 64 * WAVES) acc[i] = 0ull;
-    // Fixed-point scale: the largest pixel sum (weight pass: the largest
-    // term) is < 2^30 * 2^-S.
-    int e = 0;
-    (void)frexpf(fmaxf(a.wmul * (*to_global(a.vmax)) * a.fsum, 1e-30f), &e);
-    const int S = min(max(30 - e, -100), 100);
-    const float scale = ldexpf(1.0f, S), unscale = ldexpf(1.0f, -S);
+    // Fixed-point scale.  Weight pass (64-bit words): a power of two 2^S with
+    // the largest term < 2^30 * 2^-S.  Other passes (32-bit words): the bound
+    // of a pixel sum (wmul * max|voxel| * fsum, each factor an upper bound
+    // with its own margin) maps to 2^31 (1 - 2^-10), so every sum of rounded
+    // terms stays inside the signed 32-bit range while using all but the
+    // last fraction of a bit of it.
+    const float bound = fmaxf(a.wmul * (*to_global(a.vmax)) * a.fsum, 1e-30f);
+    float scale, unscale;
+    if (KIND == FWD_WEIGHT) {
+        int e = 0;
+        (void)frexpf(bound, &e);
+        const int S = min(max(30 - e, -100), 100);
+        scale = ldexpf(1.0f, S);
+        unscale = ldexpf(1.0f, -S);
+    } else {
+        scale = fminf(fmaxf(2145386496.0f / bound, 0x1p-100f), 0x1p100f);
+        unscale = 1.0f / scale;
+    }
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = uniformi(threadIdx.x >> 6);
     const auto vol = to_global(a.vol + (KIND == FWD_PLAIN ? (int64_t)cond * a.vol_cond_stride : 0));
     float g2[kMaxBasis];
-    if (KIND == FWD_XTT) {
+    // untilted XTT: the view's row of the per-plan weight table (BwdArgs::xw)
+    const auto xrow = to_const(a.xw ? a.xw + (int64_t)code * 5 * d.K : nullptr);
+    if (KIND == FWD_XTT && !AXIS) {
 #pragma unroll
-        for (int k = 0; k < kMaxBasis; k++) g2[k] = (k < d.K) ? to_global(a.g2)[(int64_t)code * d.K + k] : 0.f;
+        for (int k = 0; k < kMaxBasis; k++) g2[k] = (k < d.K) ? to_const(a.g2)[(int64_t)code * d.K + k] : 0.f;
     }
-    const float scalev = (KIND == FWD_PLAIN) ? d.ratio : 1.f;
+    // value scale folded into the untilted splat weights: ratio (ir.cu:1513-1518),
+    // XTT vs * ratio (ir.cu:1791)
+    const float scalev = (KIND == FWD_PLAIN) ? d.ratio : (KIND == FWD_XTT ? d.vs * d.ratio : 1.f);
     const float zlo = -d.xc, zhi = (float)(d.N - 1) - d.xc;
     const float band_lo = (float)(u0 + cs) - 0.5f, band_hi = (float)(u0 + cs + W) + 0.5f;
     // untilted path: lane offsets of the z chunks, and the window of t = v - 0.5:
@@ -547,6 +580,18 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
             // the two u weights with the fixed-point scale, the band column, the z range
             float sB = 0.f, sA = 0.f, sw0 = 0.f, sw1 = 0.f;
             int scp = 0;
+            // untilted XTT: |d|^2 - dz^2 of the column's source->voxel vector d and
+            // sig_k (d_x S_k,x + d_y S_k,y), so that sig_k (d.S_k) = dz sig_k S_k,z + xck_k
+            float xcxy = 0.f, xck[KM];
+            if (KIND == FWD_XTT && AXIS) {
+                const float xp = (float)x - d.xc;
+                const float dx = fmaf(d.vs, xp, g.d0[0]), dy = fmaf(d.vs, yp, g.d0[1]);
+                xcxy = fmaf(dy, dy, dx * dx);
+#pragma unroll
+                for (int k = 0; k < KM; k++)
+                    xck[k] = (k < d.K) ? fmaf(xp, xrow[2 * d.K + k], fmaf(yp, xrow[3 * d.K + k], xrow[4 * d.K + k]))
+                                       : 0.f;
+            }
             if (x < d.N) {
                 const ColForms c = col_forms(g, (float)x - d.xc, yp);
                 if (AXIS) {
@@ -578,7 +623,80 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                 }
             }
             unsigned long long m = __ballot(cand);
-            if (KIND == FWD_XTT) {
+            if (KIND == FWD_XTT && AXIS) {
+                // Untilted XTT: a candidate's column terms come from its lane; per voxel
+                // the value sum_k mu_k (S_k.G)^2 |B x S_k|^2 (ir.cu:1780-1794) is
+                // sum_k mu_k (g2s_k - q_k^2 / |d|^2), q_k = sig_k (d.S_k) affine in z;
+                // vs * ratio rides in the splat weights.
+                // EXACT: d.K == KM (the k loops need no bound).  k is the outer loop of a
+                // chunk pair, so only one k's constants are live at a time.
+                auto candidates = [&](auto exact_c) {
+                    constexpr bool EXACT = decltype(exact_c)::value;
+                    while (m) {
+                        const int b = __ffsll(m) - 1;
+                        m &= m - 1;
+                        const float Bc = lane_f(sB, b), Ac = lane_f(sA, b);
+                        const float w0 = lane_f(sw0, b), w1 = lane_f(sw1, b);
+                        const float cxy = lane_f(xcxy, b);
+                        const lds_ptr<AccT> cp = acc + lane_i(scp, b);
+                        int zb = 0, ze = d.N;
+                        if (!rows_all) col_zrange_t(Bc, Ac, d, v0, vb, zb, ze);
+                        const int64_t cbase = ((int64_t)y * d.N + x0 + b) * d.N;
+                        const unsigned row0 =
+                            (unsigned)(uintptr_t)cp + (unsigned)((1 - v0) * kBandStride * (int)sizeof(AccT));
+                        const lds_ptr<AccT> spare = cp + spare_row * kBandStride;
+                        const bool whole = (d.N & 63) == 0 && zb == 0 && ze == d.N;
+                        // every voxel of the column inside the window (t monotone in z)
+                        const float ta = fmaf(Bc, -d.xc, Ac), tb = fmaf(Bc, (float)(d.N - 1) - d.xc, Ac);
+                        const bool inside = whole && fminf(ta, tb) > t_lo && fmaxf(ta, tb) < t_hi;
+                        const auto col = vol + cbase + lane;
+                        for (int zg = zb & ~63; zg < ze; zg += 64 * kXttChunks) {
+                            float mu[kXttChunks][KM];
+#pragma unroll
+                            for (int c = 0; c < kXttChunks; c++) {
+                                const int z = zg + c * 64 + lane;
+                                const bool in = whole ? (zg + c * 64 < d.N) : (z >= zb && z < ze);
+#pragma unroll
+                                for (int k = 0; k < KM; k++)
+                                    mu[c][k] = (in && (EXACT || k < d.K)) ? col[zg + c * 64 + (int64_t)k * a.chan_stride]
+                                                                          : 0.f;
+                            }
+                            const float zoff = (float)zg - d.xc;
+                            float dz[kXttChunks], id2[kXttChunks], val[kXttChunks];
+#pragma unroll
+                            for (int c = 0; c < kXttChunks; c++) {
+                                dz[c] = fmaf(d.vs, lanef[c] + zoff, g.d0[2]);
+                                id2[c] = rcpf(fmaf(dz[c], dz[c], cxy));
+                                val[c] = 0.f;
+                            }
+#pragma unroll
+                            for (int k = 0; k < KM; k++) {
+                                if (EXACT || k < d.K) {
+                                    const float ck = lane_f(xck[k], b), sS2 = xrow[k], g2s = xrow[d.K + k];
+#pragma unroll
+                                    for (int c = 0; c < kXttChunks; c++) {
+                                        const float q = fmaf(dz[c], sS2, ck);
+                                        val[c] = fmaf(fmaf(-q * q, id2[c], g2s), mu[c][k], val[c]);
+                                    }
+                                }
+                            }
+#pragma unroll
+                            for (int c = 0; c < kXttChunks; c++) {
+                                if (zg + c * 64 >= ze) break;
+                                const float t = fmaf(Bc, lanef[c] + zoff, Ac);
+                                if (inside)
+                                    splat_inside<AccT, AccS>(t, val[c], row0, w0, w1);
+                                else
+                                    splat_untilted<AccT, AccS>(t, val[c], row0, spare, w0, w1, t_lo, t_hi);
+                            }
+                        }
+                    }
+                };
+                if (d.K == KM)
+                    candidates(std::true_type());
+                else
+                    candidates(std::false_type());
+            } else if (KIND == FWD_XTT) {
                 while (m) {
                     const int bit = __ffsll(m) - 1;
                     m &= m - 1;
@@ -750,7 +868,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     }
     __syncthreads();
 
-    double lsum = 0.0;
+    double lsum = 0.0, l2sum = 0.0;  // sum |y - p|, sum (y - p)^2 (OUT_RESIDUAL_L2)
     float fmax_loc = 0.0f;  // weight pass: largest per-pixel weight sum
     const int64_t img = (int64_t)(code - a.code_base) * d.V * d.U;
     const int64_t nimg = (int64_t)code * d.V * d.U;
@@ -779,8 +897,10 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
         } else {
             const float yv = to_global(a.sino)[pix];
             lsum += (double)fabsf(yv - p);
-            if (a.out_mode == OUT_RESIDUAL) {
-                proj[pix] = yv - p;
+            if (a.out_mode == OUT_RESIDUAL || a.out_mode == OUT_RESIDUAL_L2) {
+                const float rv = yv - p;
+                proj[pix] = rv;
+                l2sum = fma((double)rv, (double)rv, l2sum);
             } else {
                 float q = p;
                 if (q != 0.0f) q = yv / q;
@@ -793,22 +913,32 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
         for (int off = 32; off > 0; off >>= 1) fmax_loc = fmaxf(fmax_loc, __shfl_xor(fmax_loc, off));
         if (lane == 0) atomicMax((unsigned int*)a.fmax, __float_as_uint(fmax_loc));
     }
-    if (a.out_mode == OUT_RATIO || a.out_mode == OUT_RESIDUAL) {
-        // Cross-wave sum through LDS past the band accumulator (8-byte aligned slot).
+    if (a.out_mode == OUT_RATIO || a.out_mode == OUT_RESIDUAL || a.out_mode == OUT_RESIDUAL_L2) {
+        // Cross-wave sums through LDS past the band accumulator (8-byte aligned
+        // slots: WAVES for the L1 sum, then WAVES for the squared sum).
         double* red = reinterpret_cast<double*>(acc_raw + fwd_red_slot(rows, sizeof(AccT)));
+        const bool l2 = a.out_mode == OUT_RESIDUAL_L2;
         lsum = wave_sum(lsum);
-        if (lane == 0) red[wave] = lsum;
+        if (l2) l2sum = wave_sum(l2sum);
+        if (lane == 0) {
+            red[wave] = lsum;
+            if (l2) red[WAVES + wave] = l2sum;
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
-            double t = 0.0;
+            double t = 0.0, t2 = 0.0;
             for (int w = 0; w < WAVES; w++) t += red[w];
             atomicAdd(loss, t);
+            if (l2) {
+                for (int w = 0; w < WAVES; w++) t2 += red[WAVES + w];
+                atomicAdd(loss + 1, t2);
+            }
         }
     }
 }
 
-template <int KIND, bool AXIS>
-__global__ __launch_bounds__(64 * kFwdWaves) void k_forward(const FwdArgs a) {
+template <int KIND, bool AXIS, int KM = kMaxBasis, int WAVES = kFwdWaves>
+__global__ __launch_bounds__(64 * WAVES) void k_forward(const FwdArgs a) {
     extern __shared__ unsigned long long acc_raw[];
     const int per_chunk = a.n_bands * max(a.n_sub, 1) * a.n_slots;
     const int nblk = per_chunk * a.n_chunks;
@@ -823,7 +953,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_forward(const FwdArgs a) {
     const int chunk = o / per_chunk;
     const int v0 = a.v0 + chunk * a.vb;
     const int vb = min(a.vb, a.d.V - v0);
-    forward_item<KIND, AXIS, kFwdWaves>(a, o - chunk * per_chunk, 0, a.loss, acc_raw, v0, vb);
+    forward_item<KIND, AXIS, WAVES, KM>(a, o - chunk * per_chunk, 0, a.loss, acc_raw, v0, vb);
 }
 
 // Work queue of a persistent launch: q[0] hands out tickets 0, 1, 2, ...; a
@@ -862,7 +992,7 @@ __device__ __forceinline__ int find_item(const int* __restrict__ item_end, int n
 // so the long items listed first are started first and the short ones fill
 // the tail.  ticket_slot: index of a free 8-byte LDS word past every item's
 // band accumulator.
-template <int KIND, int AXES, int WAVES>
+template <int KIND, int AXES, int WAVES, int KM = kMaxBasis>
 __device__ __forceinline__ void forward_batch_loop(const FwdArgs* __restrict__ items,
                                                    const int* __restrict__ item_end, int n_items, unsigned* queue,
                                                    int ticket_slot, int step, int loss_shift) {
@@ -881,10 +1011,20 @@ __device__ __forceinline__ void forward_batch_loop(const FwdArgs* __restrict__ i
         const int o = t - (i ? item_end[i - 1] : 0);
         double* loss = a.loss ? a.loss + loss_shift : nullptr;
         if (AXES == 1 || a.axis)
-            forward_item<KIND, true, WAVES>(a, o, step, loss, acc_raw, a.v0, a.vb);
+            forward_item<KIND, true, WAVES, KM>(a, o, step, loss, acc_raw, a.v0, a.vb);
         else
-            forward_item<KIND, false, WAVES>(a, o, step, loss, acc_raw, a.v0, a.vb);
+            forward_item<KIND, false, WAVES, KM>(a, o, step, loss, acc_raw, a.v0, a.vb);
     }
+}
+
+// Persistent forward of XTT problems (forwardProjXTT, ir.cu:1753-1804): the
+// same loop with the directional splat, KM >= every item's basis count.
+template <int KM>
+__global__ __launch_bounds__(64 * kFwdXttWaves) void k_forward_batch_xtt(const FwdArgs* __restrict__ items,
+                                                                       const int* __restrict__ item_end,
+                                                                       int n_items, unsigned* queue, int ticket_slot,
+                                                                       int step, int loss_shift) {
+    forward_batch_loop<FWD_XTT, 0, kFwdXttWaves, KM>(items, item_end, n_items, queue, ticket_slot, step, loss_shift);
 }
 
 template <int KIND>
@@ -981,6 +1121,11 @@ __device__ __forceinline__ Foot foot_from_extent(float umn, float umx, float vmn
 
 // Fused update of a backprojection tile: every voxel of the lane's columns is
 // read and written once (raw accumulate, MLEM ratio, ART step or FDK sum).
+// Invariant relied on by the solvers: the MLEM / ART tiles of a launch cover
+// and write every voxel of the volume, so the maximum of the values written
+// here is the volume's max |voxel| exactly, and the next forward takes its
+// fixed-point scale from it.  A tile skipped for any reason (say, outside the
+// field of view) would leave that bound too small.
 template <int KIND, int CPW, int ZPL, int KA>
 __device__ __forceinline__ float backward_update(const BwdArgs& a, const float (&tmp)[CPW * CPW][ZPL][KA],
                                                  const float (&fac)[CPW * CPW][ZPL][KA], const int (&nwhole)[CPW * CPW],
@@ -1348,8 +1493,8 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
         return backward_item_untilted<KIND, ZPL>(a, o, step, stage);
     }
     const Dims& d = a.d;
-    const auto views = to_global(a.views + (int64_t)step * a.n_views);
-    const auto vg = to_global(a.vg);
+    const auto views = to_const(a.views + (int64_t)step * a.n_views);
+    const auto vg = to_const(a.vg);
     const auto vol = to_global(a.vol);
     const int gx = a.gx;
     const int bcx = CPW * gx, bcy = CPW * a.gy;  // block columns along x, y
@@ -1417,7 +1562,7 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
 #pragma unroll
     for (int q = 0; q < ZPL; q++) zpq[q] = (float)(z0 + lane + 64 * q) - d.xc;
     int code = views[0];
-    ViewGeom g = uniform_view(vg[code]);
+    ViewGeom g = vg[code];
     Foot f = footprint(g, d, bx0, bx1, by0, by1, bz0, bz1, lane);
     lds_float* const lds = (lds_float*)stage;
     if (f.staged) {
@@ -1435,16 +1580,16 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
         Foot nf{0, 0, 0, 0, 0, 0, false, false};
         if (s + 1 < a.n_views) {
             ncode = views[s + 1];
-            ng = uniform_view(vg[ncode]);
+            ng = vg[ncode];
             nf = footprint(ng, d, bx0, bx1, by0, by1, bz0, bz1, lane);
             if (nf.staged) fetch(nf, ncode);
         }
         if (!f.empty) {
             const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
             float g2[KA];
-            if (XTT) {
+            if (XTT && !AXIS) {
 #pragma unroll
-                for (int k = 0; k < KA; k++) g2[k] = (k < d.K) ? to_global(a.g2)[(int64_t)code * d.K + k] : 0.f;
+                for (int k = 0; k < KA; k++) g2[k] = (k < d.K) ? to_const(a.g2)[(int64_t)code * d.K + k] : 0.f;
             }
 #pragma unroll
             for (int j = 0; j < CPW; j++) {
@@ -1471,6 +1616,98 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
                     if (XTT) {
                         dx = fmaf(d.vs, xp, g.d0[0]);
                         dy = fmaf(d.vs, yp, g.d0[1]);
+                    }
+                    if (AXIS && XTT) {
+                        // Untilted XTT: the column's u split and t(z) = v - 0.5 as in the
+                        // plain path; per voxel the weights w_k = (S_k.G)^2 |B x S_k|^2
+                        // (ir.cu:1832-1840) are g2s_k - q_k^2 / |d|^2 with
+                        // q_k = sig_k (d.S_k) = dz sig_k S_k,z + ca_k, ca_k affine in the
+                        // column's (x', y').  The k loop is outermost so that only the
+                        // per-voxel gather, dz and 1/|d|^2 stay live across it.
+                        int iu;
+                        float fu;
+                        split(u_c, iu, fu);
+                        const float a0 = 1.0f - fu;
+                        const float Bc = g.av[2] * inv_c;
+                        const float Ac = fmaf(c.nv, inv_c, d.hv - 0.5f);
+                        const float hi = accept_u(u_c, d) ? d.vmax - 0.5f : -1.0f;
+                        const int lu = min(max(iu - f.iu_lo, 0), f.W - 2);
+                        const float cxy = fmaf(dy, dy, dx * dx);
+                        const float ta = fmaf(Bc, (float)z0 - d.xc, Ac);
+                        const float tb = fmaf(Bc, (float)(z0 + 64 * ZPL - 1) - d.xc, Ac);
+                        const auto row = to_const(a.xw + (int64_t)code * 5 * d.K);
+                        // MASK false: the tile's whole z extent is accepted (t at its first
+                        // and last z bound every lane's t), no per-voxel window test.
+                        // EXACT: d.K == KA, the k loops need no bound.
+                        // Lanes with z >= N accumulate but are never stored.
+                        auto voxels = [&](auto mask_c, auto exact_c) {
+                            constexpr bool MASK = decltype(mask_c)::value;
+                            constexpr bool EXACT = decltype(exact_c)::value;
+                            // the view's constants of the column: scalar loads, ca in VGPRs
+                            float ca[KA], sS2[KA], g2s[KA];
+#pragma unroll
+                            for (int k = 0; k < KA; k++) {
+                                const bool kin = EXACT || k < d.K;
+                                sS2[k] = kin ? row[k] : 0.f;
+                                g2s[k] = kin ? row[d.K + k] : 0.f;
+                                ca[k] = kin ? fmaf(xp, row[2 * d.K + k], fmaf(yp, row[3 * d.K + k], row[4 * d.K + k]))
+                                            : 0.f;
+                            }
+                            float nb[ZPL], dzq[ZPL], id2[ZPL];
+                            bool okq[ZPL];
+#pragma unroll
+                            for (int q = 0; q < ZPL; q++) {
+                                const float t = fmaf(Bc, zpq[q], Ac);
+                                const bool ok = !MASK || (t > 0.05f && t < hi);
+                                okq[q] = ok;
+                                const float fl = floorf(t);
+                                const float fv = t - fl;
+                                const int iv = (int)fl;
+                                float r00, r01, r10, r11;
+                                if (f.staged) {
+                                    const int qi = (ok ? iv - f.iv_lo : 0) * f.Ws + lu;
+                                    r00 = cur[qi];
+                                    r01 = cur[qi + 1];
+                                    r10 = cur[qi + f.Ws];
+                                    r11 = cur[qi + f.Ws + 1];
+                                } else {
+                                    const int64_t qi =
+                                        (int64_t)min(max(iv, 0), d.V - 2) * d.U + min(max(iu, 0), d.U - 2);
+                                    r00 = __builtin_nontemporal_load(img + qi);
+                                    r01 = __builtin_nontemporal_load(img + qi + 1);
+                                    r10 = __builtin_nontemporal_load(img + qi + d.U);
+                                    r11 = __builtin_nontemporal_load(img + qi + d.U + 1);
+                                }
+                                const float top = fmaf(fu, r01, a0 * r00);
+                                const float bot = fmaf(fu, r11, a0 * r10);
+                                nb[q] = fmaf(fv, bot - top, top);
+                                dzq[q] = fmaf(d.vs, zpq[q], g.d0[2]);
+                                id2[q] = rcpf(fmaf(dzq[q], dzq[q], cxy));
+                            }
+#pragma unroll
+                            for (int k = 0; k < KA; k++) {
+                                if (EXACT || k < d.K) {
+#pragma unroll
+                                    for (int q = 0; q < ZPL; q++) {
+                                        const float qk = fmaf(dzq[q], sS2[k], ca[k]);
+                                        float w = fmaf(-qk * qk, id2[q], g2s[k]);
+                                        if (MASK) w = okq[q] ? w : 0.f;  // a rejected voxel adds nothing
+                                        tmp[cidx][q][k] = fmaf(w, nb[q], tmp[cidx][q][k]);
+                                        fac[cidx][q][k] += w;
+                                    }
+                                }
+                            }
+                        };
+                        const bool all_in = f.staged && uniformi(fminf(ta, tb) > 0.05f && fmaxf(ta, tb) < hi);
+                        if (d.K == KA) {
+                            if (all_in)
+                                voxels(std::false_type(), std::true_type());
+                            else
+                                voxels(std::true_type(), std::true_type());
+                        } else {
+                            voxels(std::true_type(), std::false_type());
+                        }
+                        continue;
                     }
                     if (AXIS && !XTT) {
                         // Untilted: u, hence (iu, fu), is constant along the column, the
@@ -1658,7 +1895,7 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward(const BwdArgs a) {
 // Persistent backprojection over many problems (plain MLEM / ART kinds): the
 // same work queue as k_forward_batch; items[i] is one problem with
 // item_end[i] - item_end[i-1] tiles, and its own tile shape (zpl, gx).
-template <int KIND, int ZPL>
+template <int KIND, int ZPL, int CPW = 2, int KM = 1>
 __global__ __launch_bounds__(kBwdThreads) void k_backward_batch(const BwdArgs* __restrict__ items,
                                                                 const int* __restrict__ item_end, int n_items,
                                                                 unsigned* queue, int step) {
@@ -1681,6 +1918,7 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward_batch(const BwdArgs* _
                 for (int w = 0; w < kBwdWaves; w++) m = fmaxf(m, wmax[w]);
                 __hip_atomic_fetch_max(dst, __float_as_uint(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
+            *witem = -1;  // flushed: only a tile that sets it again sends a max
             *tk = take_ticket(queue, (unsigned)total);
         }
         __syncthreads();
@@ -1690,8 +1928,8 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward_batch(const BwdArgs* _
         const int i = find_item(item_end, n_items, t);
         const BwdArgs& a = items[i];
         const int o = t - (i ? item_end[i - 1] : 0);
-        float m = a.axis ? backward_item<KIND, true, 2, ZPL, 1>(a, o, step, stage)
-                         : backward_item<KIND, false, 2, ZPL, 1>(a, o, step, stage);
+        float m = a.axis ? backward_item<KIND, true, CPW, ZPL, KM>(a, o, step, stage)
+                         : backward_item<KIND, false, CPW, ZPL, KM>(a, o, step, stage);
         if (a.vmax_out) {
             for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
             if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
@@ -1826,6 +2064,40 @@ __global__ void k_sqrt_abs(float* v, int64_t n) {
     if (i >= n) return;
     const float m = v[i];
     v[i] = (m < 0.0f) ? sqrtf(-m) : sqrtf(m);  // voxelSqrt (ir.cu:1715)
+}
+
+// calcAngleFromMD (pca.cu:67-83): azimuth phi and polar angle theta of the
+// main direction, the direction flipped into y >= 0 first.  The squares and the
+// sums are rounded separately (no contraction into FMAs), as the reference's
+// host code evaluates them.
+__global__ __launch_bounds__(256) void k_md_angles(const float* __restrict__ md, float* __restrict__ ang,
+                                                   int64_t n) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float a = md[i], b = md[n + i], c = md[2 * n + i];
+    const float r = sqrtf(b * b + a * a);
+    if (b < 0.f) {
+        ang[i] = atan2f(-b, -a);
+        ang[n + i] = atan2f(r, -c);
+    } else {
+        ang[i] = atan2f(b, a);
+        ang[n + i] = atan2f(r, c);
+    }
+}
+
+// directionNormalization (pca.cu:85-106): unit main direction where its norm
+// exceeds eps (0.65 in the reference), zero elsewhere.
+__global__ __launch_bounds__(256) void k_md_normalise(float* __restrict__ md, int64_t n, float eps) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float x = md[i], y = md[n + i], z = md[2 * n + i];
+    const float nrm = sqrtf(x * x + y * y + z * z);
+    const bool keep = nrm > eps;
+    md[i] = keep ? x / nrm : 0.f;
+    md[n + i] = keep ? y / nrm : 0.f;
+    md[2 * n + i] = keep ? z / nrm : 0.f;
 }
 
 // max |x| over n floats into out[0] (zeroed beforehand).
@@ -2082,7 +2354,21 @@ hipError_t launch_forward(const FwdArgs& a, int kind, bool axis, hipStream_t str
     } else if (kind == FWD_PLAIN) {
         if (axis) XR_FWD(FWD_PLAIN, true); else XR_FWD(FWD_PLAIN, false);
     } else {
-        if (axis) XR_FWD(FWD_XTT, true); else XR_FWD(FWD_XTT, false);
+        // basis count rounded up to a compiled size (registers of the untilted path)
+#define XR_FWD_K(KM)                                                                                     \
+    hipLaunchKernelGGL((k_forward<FWD_XTT, true, KM, kFwdXttWaves>), dim3(grid), dim3(64 * kFwdXttWaves), lds, \
+                       stream, a)
+        if (!axis)
+            XR_FWD(FWD_XTT, false);
+        else if (a.d.K <= 4)
+            XR_FWD_K(4);
+        else if (a.d.K <= 8)
+            XR_FWD_K(8);
+        else if (a.d.K <= 13)
+            XR_FWD_K(13);
+        else
+            XR_FWD_K(16);
+#undef XR_FWD_K
     }
 #undef XR_FWD
     return hipGetLastError();
@@ -2094,9 +2380,10 @@ bool backward_shape(BwdArgs& a, int zpl, int gx, bool xtt) {
     const int N = a.d.N;
     // Plain / FDK: 2x2 columns x zpl z per lane, 16 waves as gx x gy column
     // groups (a (2gx) x (2gy)-column tile, 64*zpl z).  XTT (2K accumulators per
-    // voxel): one column x one z per lane.
+    // voxel): one column x zpl (1 or 2) z per lane.
     const int CPW = xtt ? 1 : 2;
-    if (xtt) zpl = 1;
+    if (xtt && zpl == 0) zpl = 1;
+    if (xtt && zpl > 2) return false;
     if (zpl == 0) zpl = default_zpl(N);
     if (gx == 0) gx = 4;
     if (!(zpl == 1 || zpl == 2 || zpl == 4)) return false;
@@ -2149,7 +2436,13 @@ hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream) {
         else                             \
             XR_BWD_AX(KD, 1, Z, 16);     \
     } while (0)
-#define XR_BWD_XTT(KD) XR_BWD_XTT_Z(KD, 1)
+#define XR_BWD_XTT(KD)             \
+    do {                           \
+        if (zpl == 2)              \
+            XR_BWD_XTT_Z(KD, 2);   \
+        else                       \
+            XR_BWD_XTT_Z(KD, 1);   \
+    } while (0)
     switch (kind) {
         case BWD_RAW: XR_BWD_PLAIN(BWD_RAW); break;
         case BWD_MLEM: XR_BWD_PLAIN(BWD_MLEM); break;
@@ -2168,10 +2461,28 @@ hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream) {
     return hipGetLastError();
 }
 
+// Basis count rounded up to a compiled size of the XTT kernels.
+inline int xtt_km(int kmax) { return kmax <= 4 ? 4 : (kmax <= 8 ? 8 : (kmax <= 13 ? 13 : 16)); }
+
+const void* forward_batch_xtt_kernel(int kmax) {
+    switch (xtt_km(kmax)) {
+        case 4: return (const void*)k_forward_batch_xtt<4>;
+        case 8: return (const void*)k_forward_batch_xtt<8>;
+        case 13: return (const void*)k_forward_batch_xtt<13>;
+        default: return (const void*)k_forward_batch_xtt<16>;
+    }
+}
+
 hipError_t launch_forward_batch(const FwdArgs* items, const int* item_end, int n_items, unsigned* queue,
                                 int ticket_slot, size_t lds_bytes, int grid, int step, int loss_shift, int kind,
-                                bool axis_all, hipStream_t stream) {
+                                bool axis_all, hipStream_t stream, int kmax) {
     if (n_items <= 0 || grid <= 0) return hipSuccess;
+    if (kind == FWD_XTT) {
+        void* args[] = {(void*)&items, (void*)&item_end, (void*)&n_items, (void*)&queue, (void*)&ticket_slot,
+                        (void*)&step, (void*)&loss_shift};
+        return hipLaunchKernel(forward_batch_xtt_kernel(kmax), dim3(grid), dim3(64 * kFwdXttWaves), args, lds_bytes,
+                               stream);
+    }
     if (kind != FWD_PLAIN) return hipErrorInvalidValue;
     if (axis_all)
         hipLaunchKernelGGL((k_forward_batch_axis<FWD_PLAIN>), dim3(grid), dim3(64 * kFwdWaves), lds_bytes, stream,
@@ -2182,8 +2493,9 @@ hipError_t launch_forward_batch(const FwdArgs* items, const int* item_end, int n
     return hipGetLastError();
 }
 
-// Batched backprojection kernel of (kind, zpl), or nullptr.
-const void* backward_batch_kernel(int kind, int zpl) {
+// Batched backprojection kernel of (kind, zpl) (XTT kinds: zpl 1, one column
+// per wave, kmax basis vectors at most), or nullptr.
+const void* backward_batch_kernel(int kind, int zpl, int kmax) {
 #define XR_BB(KD)                                                        \
     if (kind == KD) {                                                    \
         if (zpl == 4) return (const void*)k_backward_batch<KD, 4>;       \
@@ -2193,13 +2505,25 @@ const void* backward_batch_kernel(int kind, int zpl) {
     XR_BB(BWD_MLEM)
     XR_BB(BWD_ART)
 #undef XR_BB
+#define XR_BBX(KD)                                                                \
+    if (kind == KD && zpl == 1) {                                                 \
+        switch (xtt_km(kmax)) {                                                   \
+            case 4: return (const void*)k_backward_batch<KD, 1, 1, 4>;            \
+            case 8: return (const void*)k_backward_batch<KD, 1, 1, 8>;            \
+            case 13: return (const void*)k_backward_batch<KD, 1, 1, 13>;          \
+            default: return (const void*)k_backward_batch<KD, 1, 1, 16>;          \
+        }                                                                         \
+    }
+    XR_BBX(BWD_XTT_MLEM)
+    XR_BBX(BWD_XTT_ART)
+#undef XR_BBX
     return nullptr;
 }
 
 hipError_t launch_backward_batch(const BwdArgs* items, const int* item_end, int n_items, unsigned* queue, int grid,
-                                 int step, int kind, int zpl, hipStream_t stream) {
+                                 int step, int kind, int zpl, hipStream_t stream, int kmax) {
     if (n_items <= 0 || grid <= 0) return hipSuccess;
-    const void* k = backward_batch_kernel(kind, zpl);
+    const void* k = backward_batch_kernel(kind, zpl, kmax);
     if (!k) return hipErrorInvalidValue;
     void* args[] = {(void*)&items, (void*)&item_end, (void*)&n_items, (void*)&queue, (void*)&step};
     return hipLaunchKernel(k, dim3(grid), dim3(kBwdThreads), args, backward_batch_lds_bytes(), stream);
@@ -2207,8 +2531,13 @@ hipError_t launch_backward_batch(const BwdArgs* items, const int* item_end, int 
 
 size_t backward_batch_lds_bytes() { return (2 * kStageBuf + kBwdExtraWords + kBwdWaves + 1) * sizeof(float); }
 
-int batch_blocks_per_cu_forward(int kind, size_t lds_bytes, bool axis_all) {
+int batch_blocks_per_cu_forward(int kind, size_t lds_bytes, bool axis_all, int kmax) {
     int n = 0;
+    if (kind == FWD_XTT) {
+        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, forward_batch_xtt_kernel(kmax),
+                                                                          64 * kFwdXttWaves, lds_bytes);
+        return e == hipSuccess ? n : 0;
+    }
     if (kind != FWD_PLAIN) return 0;
     const hipError_t e =
         axis_all ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)k_forward_batch_axis<FWD_PLAIN>,
@@ -2220,7 +2549,7 @@ int batch_blocks_per_cu_forward(int kind, size_t lds_bytes, bool axis_all) {
 
 size_t forward_lds_bytes(int kind, int vb) {
     const int acc_bytes = (kind == FWD_WEIGHT) ? 8 : 4;
-    return ((size_t)fwd_red_slot(vb + 2, acc_bytes) + kFwdWaves) * sizeof(unsigned long long);
+    return ((size_t)fwd_red_slot(vb + 2, acc_bytes) + 2 * kFwdWaves) * sizeof(unsigned long long);
 }
 
 int forward_blocks_per_cu(int kind, bool axis, size_t lds_bytes) {
@@ -2228,14 +2557,15 @@ int forward_blocks_per_cu(int kind, bool axis, size_t lds_bytes) {
     const void* k = nullptr;
     if (kind == FWD_WEIGHT) k = axis ? (const void*)k_forward<FWD_WEIGHT, true> : (const void*)k_forward<FWD_WEIGHT, false>;
     else if (kind == FWD_PLAIN) k = axis ? (const void*)k_forward<FWD_PLAIN, true> : (const void*)k_forward<FWD_PLAIN, false>;
-    else k = axis ? (const void*)k_forward<FWD_XTT, true> : (const void*)k_forward<FWD_XTT, false>;
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 64 * kFwdWaves, lds_bytes);
+    else k = axis ? (const void*)k_forward<FWD_XTT, true, 13, kFwdXttWaves> : (const void*)k_forward<FWD_XTT, false>;
+    const int threads = (kind == FWD_XTT && axis) ? 64 * kFwdXttWaves : 64 * kFwdWaves;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, lds_bytes);
     return e == hipSuccess ? n : 0;
 }
 
-int batch_blocks_per_cu_backward(int kind, int zpl) {
+int batch_blocks_per_cu_backward(int kind, int zpl, int kmax) {
     int n = 0;
-    const void* k = backward_batch_kernel(kind, zpl);
+    const void* k = backward_batch_kernel(kind, zpl, kmax);
     if (!k) return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBwdThreads, backward_batch_lds_bytes()) != hipSuccess)
         return 0;
@@ -2288,6 +2618,18 @@ hipError_t launch_from_internal(const float* internal, float* ref, int N, int ch
 hipError_t launch_sqrt_abs(float* vol, int64_t n, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_sqrt_abs, dim3(grid1(n, 256)), dim3(256), 0, stream, vol, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_md_angles(const float* md, float* angle, int64_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_md_angles, dim3(grid1(n, 256)), dim3(256), 0, stream, md, angle, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_md_normalise(float* md, int64_t n, float eps, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_md_normalise, dim3(grid1(n, 256)), dim3(256), 0, stream, md, n, eps);
     return hipGetLastError();
 }
 

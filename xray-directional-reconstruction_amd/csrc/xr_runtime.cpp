@@ -161,7 +161,7 @@ hipEvent_t get_event(xr_context* ctx) {
 // for a single volume; code_base: proj/sino hold the images of codes >= code_base.
 int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stride, float* proj, const float* sino,
                    int code_base, const int32_t* d_codes, const uint8_t* d_flags, int n_slots, int out_mode,
-                   double* d_loss, const float* vmax_known) {
+                   double* d_loss, const float* vmax_known, float* vmax_buf, bool no_alloc) {
     xr_context* ctx = p->ctx;
     FwdArgs a{};
     a.d = p->dims;
@@ -180,6 +180,7 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
     a.bands = p->tiles.fwd_scan ? nullptr : p->d_bands;
     a.code_base = code_base;
     a.g2 = p->d_g2;
+    a.xw = p->d_xw;
     a.loss = d_loss;
     a.out_mode = out_mode;
     a.n_bands = (p->g.n_det_u + kBandU - 1) / kBandU;
@@ -192,15 +193,27 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
         a.fsum = 1.0f;
         a.fmax = (out_mode == OUT_NORM_TABLE) ? p->d_fmax : nullptr;
     } else {
-        XR_TRY(ensure_norm(p));
+        if (no_alloc) {
+            if (!p->d_norm)
+                return fail(ctx, XR_ERR_INVALID,
+                            "the plan's normalisation table does not exist yet (a first xr_project call builds it)");
+        } else {
+            XR_TRY(ensure_norm(p));
+        }
         a.fsum = p->max_factor * 1.01f + 1.0f;
         if (vmax_known) {
             a.vmax = vmax_known;
         } else {
             const int64_t nvals = (kind == FWD_XTT) ? N3 * p->n_basis : (vol_cond_stride ? N3 * p->n_cond : N3);
-            XR_HIP(ctx, hipMemsetAsync(ctx->d_scal + 1, 0, sizeof(float), ctx->stream));
-            XR_TRY(timed(ctx, 3, [&] { return launch_absmax(vol, nvals, ctx->d_scal + 1, ctx->stream); }));
-            a.vmax = ctx->d_scal + 1;
+            float* vb = vmax_buf ? vmax_buf : ctx->d_scal + 1;
+            if (no_alloc) {
+                XR_HIP(ctx, launch_fill(vb, 1, 0.0f, ctx->stream));
+                XR_HIP(ctx, launch_absmax(vol, nvals, vb, ctx->stream));
+            } else {
+                XR_HIP(ctx, hipMemsetAsync(vb, 0, sizeof(float), ctx->stream));
+                XR_TRY(timed(ctx, 3, [&] { return launch_absmax(vol, nvals, vb, ctx->stream); }));
+            }
+            a.vmax = vb;
         }
         a.wmul = (kind == FWD_XTT) ? p->dims.vs * p->dims.ratio * p->xtt_wsum * 1.0001f : p->dims.ratio * 1.0001f;
     }
@@ -224,6 +237,10 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
         a.n_sub = forward_split(items, slots, 0);
     }
     a.n_sub = std::max(a.n_sub, 1);
+    if (no_alloc) {
+        XR_HIP(ctx, launch_forward(a, kind, axis, ctx->stream));
+        return XR_OK;
+    }
     return timed(ctx, 0, [&] { return launch_forward(a, kind, axis, ctx->stream); });
 }
 
@@ -234,7 +251,7 @@ int forward_rows(const xr_plan* p, int kind) {
     // the chunk height further (smaller LDS footprint, more resident blocks).
     const int V = p->g.n_det_v;
     const int word = (kind == FWD_WEIGHT) ? 8 : 4;
-    int max_rows = (64 * 1024 - 8 * kFwdWaves - 16) / (kBandStride * word) - 2;
+    int max_rows = (64 * 1024 - 16 * kFwdWaves - 16) / (kBandStride * word) - 2;
     if (p->tiles.fwd_rows > 0) max_rows = std::min(max_rows, (int)p->tiles.fwd_rows);
     const int n_chunks = (V + max_rows - 1) / max_rows;
     return (V + n_chunks - 1) / n_chunks;
@@ -256,6 +273,7 @@ int backward_launch(xr_plan* p, int kind, const float* proj, int code_base, cons
     a.factor = factor;
     a.chan_stride = (int64_t)p->g.n_vox * p->g.n_vox * p->g.n_vox;
     a.g2 = p->d_g2;
+    a.xw = p->d_xw;
     a.alpha = alpha;
     a.overwrite = overwrite;
     a.vmax_out = (kind == BWD_MLEM || kind == BWD_ART) ? vmax_out : nullptr;
@@ -494,7 +512,7 @@ int xr_context_create(xr_context** out, int device) {
     auto* c = new (std::nothrow) xr_context();
     if (!c) return XR_ERR_ALLOC;
     c->device = device;
-    if (hipMalloc((void**)&c->d_loss, sizeof(double)) != hipSuccess ||
+    if (hipMalloc((void**)&c->d_loss, 2 * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&c->d_scal, 2 * sizeof(float)) != hipSuccess) {
         if (c->d_loss) (void)hipFree(c->d_loss);
         delete c;
@@ -613,6 +631,7 @@ int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int 
     std::vector<ViewGeom> vg((size_t)C * P);
     std::vector<ChordGeom> cg((size_t)C * P);
     std::vector<float> g2((size_t)C * P * K, 0.f);
+    std::vector<float> xw((size_t)C * P * 5 * K, 0.f);
     p->axis_cond.assign(C, 1);
     for (int c = 0; c < C; c++)
         for (int v = 0; v < P; v++) {
@@ -623,6 +642,16 @@ int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int 
                 const double sg = (double)p->basis[3 * k] * t.G[0] + (double)p->basis[3 * k + 1] * t.G[1] +
                                   (double)p->basis[3 * k + 2] * t.G[2];
                 g2[((size_t)c * P + v) * K + k] = (float)(sg * sg);
+                // untilted XTT weight constants (see BwdArgs::xw)
+                const double Sx = p->basis[3 * k], Sy = p->basis[3 * k + 1], Sz = p->basis[3 * k + 2];
+                const double sig = std::fabs(sg);
+                float* row = &xw[((size_t)c * P + v) * 5 * K];
+                const double vs = g->vox_size;
+                row[k] = (float)(sig * Sz);
+                row[K + k] = (float)((double)(float)(sg * sg) * (double)(float)(Sx * Sx + Sy * Sy + Sz * Sz));
+                row[2 * K + k] = (float)(sig * vs * Sx);
+                row[3 * K + k] = (float)(sig * vs * Sy);
+                row[4 * K + k] = (float)(sig * ((double)t.d0[0] * Sx + (double)t.d0[1] * Sy));
             }
         }
     Dims& d = p->dims;
@@ -661,6 +690,8 @@ int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int 
     XR_HIP(ctx, hipMalloc((void**)&p->d_vg, vg.size() * sizeof(ViewGeom)));
     XR_HIP(ctx, hipMalloc((void**)&p->d_cg, cg.size() * sizeof(ChordGeom)));
     XR_HIP(ctx, hipMalloc((void**)&p->d_g2, g2.size() * sizeof(float)));
+    XR_HIP(ctx, hipMalloc((void**)&p->d_xw, xw.size() * sizeof(float)));
+    XR_HIP(ctx, hipMemcpy(p->d_xw, xw.data(), xw.size() * sizeof(float), hipMemcpyHostToDevice));
     XR_HIP(ctx, hipMemcpy(p->d_vg, vg.data(), vg.size() * sizeof(ViewGeom), hipMemcpyHostToDevice));
     XR_HIP(ctx, hipMemcpy(p->d_cg, cg.data(), cg.size() * sizeof(ChordGeom), hipMemcpyHostToDevice));
     XR_HIP(ctx, hipMemcpy(p->d_g2, g2.data(), g2.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -674,6 +705,7 @@ void xr_plan_destroy(xr_plan* p) {
     if (p->d_vg) (void)hipFree(p->d_vg);
     if (p->d_cg) (void)hipFree(p->d_cg);
     if (p->d_g2) (void)hipFree(p->d_g2);
+    if (p->d_xw) (void)hipFree(p->d_xw);
     if (p->d_norm) (void)hipFree(p->d_norm);
     if (p->d_bands) (void)hipFree(p->d_bands);
     delete p;
@@ -728,6 +760,28 @@ int xr_project(xr_context* ctx, xr_plan* plan, const float* vol, float* proj, co
         XR_HIP(ctx, hipMemcpyAsync(loss, ctx->d_loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
+    return XR_OK;
+}
+
+int xr_residual(xr_context* ctx, xr_plan* plan, const float* vol, float* resid, const float* sino,
+                const int32_t* views, int n_views, int xtt, double* l1, double* l2) {
+    if (!ctx || !plan || !vol || !resid || !sino || !views || n_views < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    if (xtt && plan->n_basis <= 0) return fail(ctx, XR_ERR_INVALID, "XTT projection needs basis vectors");
+    if (plan->conv != XR_CONV_IR) return fail(ctx, XR_ERR_INVALID, "forward projection uses the IR convention");
+    if (l1) *l1 = 0;
+    if (l2) *l2 = 0;
+    if (n_views == 0) return XR_OK;
+    XR_TRY(upload_views(plan, views, n_views, nullptr));
+    XR_TRY(ensure_norm(plan));
+    XR_HIP(ctx, hipMemsetAsync(ctx->d_loss, 0, 2 * sizeof(double), ctx->stream));
+    const int64_t N3 = (int64_t)plan->g.n_vox * plan->g.n_vox * plan->g.n_vox;
+    XR_TRY(forward_launch(plan, xtt ? FWD_XTT : FWD_PLAIN, vol, N3, resid, sino, 0, ctx->d_codes, nullptr, n_views,
+                          OUT_RESIDUAL_L2, ctx->d_loss));
+    double sums[2] = {0, 0};
+    XR_HIP(ctx, hipMemcpyAsync(sums, ctx->d_loss, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (l1) *l1 = sums[0];
+    if (l2) *l2 = sums[1];
     return XR_OK;
 }
 
@@ -850,6 +904,41 @@ int xr_pca_host(xr_context* ctx, const float* mu_host, int n_basis, const float*
     XR_TRY(xr_pca(ctx, dmu.p, n_basis, basis_host, nvox, nvox, dmd.p, dev.p));
     XR_HIP(ctx, hipMemcpyAsync(md_host, dmd.p, sizeof(float) * 3 * nvox, hipMemcpyDeviceToHost, ctx->stream));
     XR_HIP(ctx, hipMemcpyAsync(evalue_host, dev.p, sizeof(float) * 3 * nvox, hipMemcpyDeviceToHost, ctx->stream));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XR_OK;
+}
+
+int xr_md_angles(xr_context* ctx, const float* md, float* angle, int64_t nvox) {
+    if (!ctx || !md || !angle || nvox < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    return timed(ctx, 3, [&] { return launch_md_angles(md, angle, nvox, ctx->stream); });
+}
+
+int xr_md_normalise(xr_context* ctx, float* md, int64_t nvox, float eps) {
+    if (!ctx || !md || nvox < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    return timed(ctx, 3, [&] { return launch_md_normalise(md, nvox, eps, ctx->stream); });
+}
+
+int xr_md_angles_host(xr_context* ctx, const float* md_host, float* angle_host, int64_t nvox) {
+    if (!ctx || !md_host || !angle_host || nvox < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    if (nvox == 0) return XR_OK;
+    DevBuf<float> dmd, dang;
+    XR_HIP(ctx, dmd.alloc((size_t)3 * nvox));
+    XR_HIP(ctx, dang.alloc((size_t)2 * nvox));
+    XR_HIP(ctx, hipMemcpyAsync(dmd.p, md_host, sizeof(float) * 3 * nvox, hipMemcpyHostToDevice, ctx->stream));
+    XR_TRY(xr_md_angles(ctx, dmd.p, dang.p, nvox));
+    XR_HIP(ctx, hipMemcpyAsync(angle_host, dang.p, sizeof(float) * 2 * nvox, hipMemcpyDeviceToHost, ctx->stream));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XR_OK;
+}
+
+int xr_md_normalise_host(xr_context* ctx, float* md_host, int64_t nvox, float eps) {
+    if (!ctx || !md_host || nvox < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    if (nvox == 0) return XR_OK;
+    DevBuf<float> dmd;
+    XR_HIP(ctx, dmd.alloc((size_t)3 * nvox));
+    XR_HIP(ctx, hipMemcpyAsync(dmd.p, md_host, sizeof(float) * 3 * nvox, hipMemcpyHostToDevice, ctx->stream));
+    XR_TRY(xr_md_normalise(ctx, dmd.p, nvox, eps));
+    XR_HIP(ctx, hipMemcpyAsync(md_host, dmd.p, sizeof(float) * 3 * nvox, hipMemcpyDeviceToHost, ctx->stream));
     XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return XR_OK;
 }
@@ -1325,6 +1414,11 @@ int xr_noise_poisson(xr_context* ctx, float* y_dev, const float* p_dev, int64_t 
 int xr_noise_gaussian(xr_context* ctx, float* y_dev, const float* x_dev, int64_t n, double sigma, uint64_t seed) {
     if (!ctx || !y_dev || !x_dev || n < 0 || sigma < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
     return timed(ctx, 3, [&] { return launch_gaussian_noise(y_dev, x_dev, n, sigma, seed, ctx->stream); });
+}
+
+int xr_sino_clamp(xr_context* ctx, float* sino_dev, int64_t n) {
+    if (!ctx || !sino_dev || n < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    return timed(ctx, 3, [&] { return launch_clamp_min(sino_dev, n, 0.0f, 1e-8f, ctx->stream); });
 }
 
 void xr_philox4x32(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]) {
