@@ -87,33 +87,6 @@ def cpu_baseline(N, U, P, sdd, sod, det, vs, n_views=16, extras=True):
 MODEL = os.path.join(ROOT, "xray-directional-reconstruction_amd", "models", "warmstart_c24.xunt")
 
 
-def time_to_residual(ctx, xr, geom, sino, tau, subsets, max_epochs=300, pre=5):
-    """Epochs and GPU seconds until the data residual r = sum|y - Ax|/sum|y|
-    reaches tau: the plain method (MLEM, `subsets` ordered subsets, x0 = 0.01)
-    against the learned warm start (plain for `pre` epochs, then the trained
-    network checked by the residual with fallback, then plain again)."""
-    import importlib
-
-    ws = importlib.import_module(xr.__name__ + ".warmstart")
-    net = xr.UNet(ctx, 24)
-    net.load(MODEL)
-    scene = ws.Scene(ctx, geom, sino, subsets=subsets)
-    plain = ws.plain_mlem(scene, tau, max_epochs)
-    warm = ws.warm_start(scene, net, tau, max_epochs, pre=pre)
-    scene.close()
-    net.close()
-    return {"tau": tau, "residual": "sum|y-Ax|/sum|y| (FP64)", "method": "MLEM, %d ordered subsets" % subsets,
-            "plain_epochs": plain["iterations"], "plain_s": round(plain["ms"] / 1e3, 3),
-            "plain_reached": plain["reached"],
-            "warm_epochs": warm["iterations"], "warm_s": round(warm["ms"] / 1e3, 3), "warm_reached": warm["reached"],
-            "network_kept": warm["used_net"],
-            "acceptance": "network kept unless its residual over every 4th view exceeds the MLEM volume's by > 10%",
-            "speedup_epochs": round(plain["iterations"] / max(warm["iterations"], 1), 3),
-            "speedup_time": round(plain["ms"] / max(warm["ms"], 1e-9), 3),
-            "network": "models/warmstart_c24.xunt (960 mixed phantoms at 128^3 / 180 views, then 240 at 256^3 / "
-                       "360 views, subsets of 9 views, inputs after 5/10/20 epochs, targets after 200)"}
-
-
 def c2_noisy_sinogram(ctx, xr, geom, plan, N, vs, photons=1e4, seed=2002):
     """The 512^3 / 720-view workload's data: 40 seeded ellipsoids plus a
     cylinder shell, the normalised forward projection, Poisson noise at
@@ -184,7 +157,7 @@ def ttr_workload(xr, args):
     sino = c2_noisy_sinogram(ctx, xr, geom, plan, N, vs, photons=args.photons)
     plan.close()
     pres = tuple(int(v) for v in args.pres.split(","))
-    r = ttr_frozen(ctx, xr, geom, sino, tau=args.tau, subsets=args.subsets or max(1, P // 9),
+    r = ttr_frozen(ctx, xr, geom, sino, tau=args.tau, subsets=args.subsets or 1,
                    long_epochs=args.long_epochs, pres=pres, model=args.model or MODEL)
     r["data"] = "synthetic (40 seeded ellipsoids plus a cylinder shell, Poisson noise at %g photons)" % args.photons
     return r
@@ -194,9 +167,10 @@ HELDOUT = ((9101, "mixed"), (9102, "fibres"), (9103, "mixed"))
 
 
 def heldout_workload(xr, args):
-    """time_to_residual on seeded phantoms the network was not trained on
-    (warmstart.phantom_spec draws; the committed network saw seeds 1000-1959,
-    5000-5095 and 6000-6143), at the bench geometry."""
+    """The frozen-criterion comparison (ttr_frozen) on seeded phantoms the
+    network was not trained on (warmstart.phantom_spec draws; the committed
+    network saw seeds 1000-2999), at the bench geometry, with Poisson noise
+    at --photons per ray."""
     import importlib
 
     import torch
@@ -207,21 +181,28 @@ def heldout_workload(xr, args):
     vs = xr.default_vox_size(sdd, sod, det)
     geom = xr.make_geometry(sdd, sod, det, N, U, P)
     ctx = xr.Context(0)
+    pres = tuple(int(v) for v in args.pres.split(","))
     out = []
     for seed, family in HELDOUT:
         spec = ws.phantom_spec(seed, N, vs, family)
         vol = ws.make_phantom(ctx, N, vs, seed, **spec)
-        sino = ws.project(ctx, geom, vol)
+        clean = ws.project(ctx, geom, vol)
         del vol
-        r = time_to_residual(ctx, xr, geom, sino, args.tau, args.subsets or max(1, P // 9))
+        sino = torch.empty_like(clean)
+        ctx.noise_poisson(sino, clean, args.photons, seed)
+        ctx.sino_clamp(sino)
+        del clean
+        r = ttr_frozen(ctx, xr, geom, sino, tau=args.tau, subsets=args.subsets or 1, long_epochs=args.long_epochs,
+                       pres=pres, model=args.model or MODEL)
+        r.pop("plain_R", None)
         r.update({"seed": seed, "family": family})
-        for k in ("residual", "acceptance", "network"):
-            r.pop(k)
         out.append(r)
         del sino
         torch.cuda.empty_cache()
     return {"metric": "time-to-residual %g, plain MLEM vs learned warm start, held-out phantoms" % args.tau,
-            "unit": "s", "n_gpus": 1, "data": "synthetic (warmstart.phantom_spec seeds %s)" % [s for s, _ in HELDOUT],
+            "unit": "s", "n_gpus": 1,
+            "data": "synthetic (warmstart.phantom_spec seeds %s, Poisson noise at %g photons)"
+                    % ([s for s, _ in HELDOUT], args.photons),
             "config": {"workload": "%d^3 volume, %d views, %dx%d detector" % (N, P, U, U)}, "results": out}
 
 
@@ -351,11 +332,14 @@ def xtt_measure(xr, steps=3, warmup=1, N=256, U=512, P=360):
     return out
 
 
-def fdk_art_measure(ctx, xr, geom, sino, art_iters=50, lam=0.2):
+def fdk_art_measure(ctx, xr, geom, sino, art_iters=50, lam=1e-3):
     """FDK with the Ram-Lak filter (calcWeight + projConv + weighted
     backprojection, fdk.cu:13-170) over every view, and `art_iters` ART
     iterations (projSubtract + voxelPlus, ir.cu:1656-1693; batch 1, so the step
     is lambda / views, reconstruct.cu:124) on the same data, from zero.
+    lambda = 1e-3 is the driver's value (main.cu:60): the update adds
+    lambda/views times an unnormalised backprojection, which grows with the
+    volume edge, and at 512^3 with one subset lambda = 0.2 already diverges.
     Roofline models: 8 B per filtered detector sample, 4 B per voxel-view."""
     import torch
 
@@ -433,12 +417,14 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-warmstart", action="store_true",
                     help="skip the time-to-residual comparison (learned warm start vs plain MLEM)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the FDK / ART, XTT and 48-problem batch measurements of the default line")
     ap.add_argument("--tau", type=float, default=1e-3)
     ap.add_argument("--subsets", type=int, default=0,
-                    help="ordered subsets of the plain method (default: views/9, the training pairs' subset size)")
+                    help="ordered subsets of the plain method (default 1: plain MLEM, every view per update)")
     ap.add_argument("--photons", type=float, default=1e4)
-    ap.add_argument("--long-epochs", type=int, default=60)
-    ap.add_argument("--pres", default="2")
+    ap.add_argument("--long-epochs", type=int, default=200)
+    ap.add_argument("--pres", default="5")
     ap.add_argument("--model", default="")
     ap.add_argument("--workload", choices=["mlem", "batch", "heldout", "xtt", "ttr", "fdkart"], default="mlem",
                     help="mlem: one MLEM iteration at 512^3/720 views (default); batch: 48 uneven MLEM problems, "
@@ -489,19 +475,10 @@ def main():
     N3 = N ** 3
     lenD = P * U * U
 
-    # synthetic phantom (40 seeded ellipsoids plus a cylinder shell) and its
-    # sinogram by the same operator (noise free, so the residual can reach tau)
-    params = xr.ellipsoid_params(1234, 40, fov_radius=0.5 * N * vs)
-    vref = torch.zeros(N3, device=dev)
-    ctx.phantom_rasterise(params, N, vs, vref)
-    ctx.phantom_cylinder_shell(N, vs, 0.45 * N * vs, 4 * vs, 0.02, vref)
-    truth = torch.empty_like(vref)
-    ctx.to_columns(vref, truth, N)
-    del vref
+    # synthetic phantom (40 seeded ellipsoids plus a cylinder shell), its
+    # sinogram by the same operator, Poisson noise at --photons per ray
+    sino = c2_noisy_sinogram(ctx, xr, geom, plan, N, vs, photons=args.photons)
     codes = plan.codes(range(P))
-    sino = torch.zeros(lenD, device=dev)
-    plan.project(truth, sino, codes, out_mode=xr.OUT_NORMALISED)
-    del truth
     vol = torch.full((N3,), 0.01, device=dev)
     proj = torch.zeros(lenD, device=dev)
     torch.cuda.synchronize()
@@ -548,7 +525,8 @@ def main():
         "metric": METRIC, "value": round(gups, 2), "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (40 seeded ellipsoids plus a cylinder shell, noise-free sinogram from the projector)",
+        "data": "synthetic (40 seeded ellipsoids plus a cylinder shell, sinogram from the projector with Poisson "
+                "noise at %g photons per ray)" % args.photons,
         "config": {"workload": "plain MLEM iteration (batch 1), %d^3 volume, %d views, %dx%d detector, "
                                "SOD 500 SDD 1000 det 0.4 mm" % (N, P, U, U),
                    "n_vox": N, "n_proj": P, "n_det": U, "parallelism": "single GPU"},
@@ -557,8 +535,21 @@ def main():
         "loss_first_last": [losses[0], losses[-1]],
         "roofline": roofline,
     }
-    if not args.no_warmstart and os.path.exists(MODEL):
-        out["time_to_residual"] = time_to_residual(ctx, xr, geom, sino, args.tau, args.subsets or max(1, P // 9))
+    plan.close()
+    del proj, vol
+    torch.cuda.empty_cache()
+    if not args.no_warmstart and os.path.exists(args.model or MODEL):
+        pres = tuple(int(v) for v in args.pres.split(","))
+        out["time_to_residual"] = ttr_frozen(ctx, xr, geom, sino, tau=args.tau, subsets=args.subsets or 1,
+                                             long_epochs=args.long_epochs, pres=pres, model=args.model or MODEL)
+    if not args.no_extras:
+        out["fdk_art"] = fdk_art_measure(ctx, xr, geom, sino)
+        del sino
+        torch.cuda.empty_cache()
+        out["xtt"] = xtt_measure(xr)
+        b = batch_workload(xr, args)
+        out["batch48"] = {k: b[k] for k in ("value", "unit", "batched_s", "sequential_s", "speedup_vs_sequential")}
+        out["batch48"]["config"] = b["config"]["workload"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, U, P, sdd, sod, det, vs, args.cpu_views)
     print(json.dumps(out))

@@ -1,14 +1,16 @@
 """Learned warm start: the residual check with fallback, the positivity clamp,
 and what the trained network gains over the plain method.
 
-The plain method is IR::reconstruct's MLEM with ordered subsets
-(reconstruct.cu:23-156) from x0 = 0.01 (main.cu:40-42); the data residual is
-r = sum|y - A x| / sum|y| (FP64 sums in the forward kernel).  The committed
-network (models/warmstart_c24.xunt) was trained by train_warmstart.py on 960
-seeded 128^3 phantoms (ellipsoid-and-fibre volumes and dense ellipsoid sets
-with cylinder shells) at 180 views, 20 subsets, inputs after 5, 10 and 20
-epochs, targets after 200, then fine-tuned on 240 phantoms at 256^3 (360
-views, 40 subsets)."""
+The plain method is IR::reconstruct's MLEM (reconstruct.cu:23-156) from
+x0 = 0.01 (main.cu:40-42).  Two residuals: r = sum|y - A x| / sum|y| (the
+reference's loss over the data's L1 norm, FP64 sums in the forward kernel)
+for the driver's own stopping rule, and the frozen relative criterion
+(R_k - R_inf) / (R_0 - R_inf) <= tau with R = ||y - A x||^2 / ||y||^2 and
+R_inf after 200 plain epochs for the trained network.  The committed network
+(models/warmstart_c24.xunt) was trained by train_warmstart.py on 2000 seeded
+128^3 phantoms (ellipsoid-and-fibre volumes and dense ellipsoid sets with
+cylinder shells) at 180 views on a 192^2 detector: input the plain method
+(one subset) after 5 epochs, target after 200."""
 import os
 
 import numpy as np
@@ -107,19 +109,19 @@ def test_identity_network_changes_nothing(xr, ctx, ws):
 @pytest.mark.skipif(not os.path.exists(MODEL), reason="trained network not present")
 @pytest.mark.parametrize("seed,family", [(9002, "fibres"), (9003, "mixed")])
 def test_trained_network_gains_on_held_out_phantom(xr, ctx, ws, seed, family):
-    """On held-out 128^3 phantoms (training distribution; seed 9003 of the
-    mixed family is a dense ellipsoid set with a cylinder shell), MLEM resumed
-    from the trained network reaches r <= 1e-3 in far fewer epochs than the
-    plain method (measured 2.2x and 2.5x in epochs)."""
+    """On held-out 128^3 phantoms of the training distribution (seed 9003 of
+    the mixed family is a dense ellipsoid set with a cylinder shell), plain
+    MLEM resumed from the trained network meets the frozen criterion in far
+    fewer epochs than the plain method."""
     net = xr.UNet(ctx, 24)
     net.load(MODEL)
-    sc = scene_for(xr, ctx, ws, 128, 180, 20, seed, family=family)
-    plain = ws.plain_mlem(sc, 1e-3, 400)
-    warm = ws.warm_start(sc, net, 1e-3, 400, pre=5)
-    print("plain %d epochs %.0f ms | warm %d epochs %.0f ms" % (plain["iterations"], plain["ms"], warm["iterations"],
-                                                                 warm["ms"]), flush=True)
-    assert plain["reached"] and warm["reached"] and warm["used_net"]
-    assert plain["iterations"] >= 1.7 * warm["iterations"]
+    sc = scene_for(xr, ctx, ws, 128, 180, 1, seed, family=family)
+    R, _ = ws.plain_curve(sc, 200)
+    k = ws.first_within(R, R[0], R[-1], 1e-3)
+    warm = ws.warm_curve(sc, net, R[0], R[-1], 1e-3, 200, pre=5)
+    print("plain %s epochs | warm %d epochs (network kept: %s)" % (k, warm["epochs"], warm["used_net"]), flush=True)
+    assert k is not None and warm["reached"] and warm["used_net"]
+    assert k >= 2.5 * warm["epochs"]
     sc.close()
     net.close()
 

@@ -44,7 +44,7 @@ struct xr_plan {
     xr::ViewGeom* d_vg = nullptr;
     xr::ChordGeom* d_cg = nullptr;
     float* d_g2 = nullptr;    // [n_cond*P][K]: (S_k.G)^2
-    float* d_xw = nullptr;    // [n_cond*P][4][K]: XTT weight constants of each view (BwdArgs::xw)
+    float* d_xw = nullptr;    // [n_cond*P][5][kXwStride]: XTT weight constants of each view (BwdArgs::xw)
     float* d_norm = nullptr;  // [n_cond][P][V][U]: L/(factor+1e-5)
     int32_t* d_bands = nullptr;  // [n_cond*P][n_bands][N]: candidate columns xs | xe << 16 of each volume row
     xr::Dims dims{};

@@ -85,6 +85,12 @@ struct Basis {
     float S2[kMaxBasis];
 };
 
+// Row stride of the untilted XTT weight table: the largest basis count, so a
+// kernel compiled for any KM reads whole rows without bounds tests (entries
+// k >= K are zero).  A batch's kernels are compiled for its largest K, which
+// may exceed a problem's own.
+constexpr int kXwStride = kMaxBasis;
+
 struct FwdArgs {
     Dims d;
     Basis b;
@@ -99,7 +105,7 @@ struct FwdArgs {
     const float* sino;       // measured data, same layout (RATIO / RESIDUAL)
     const float* norm;       // L/(factor+1e-5), same layout (NORMALISED / RATIO / RESIDUAL)
     const float* g2;         // XTT [n_cond*P][K]: (S_k.G)^2
-    const float* xw;         // untilted XTT [n_cond*P][5][K]: weight constants (see BwdArgs::xw)
+    const float* xw;         // untilted XTT [n_cond*P][5][kXwStride]: weight constants (see BwdArgs::xw)
     double* loss;            // accumulated sum |y - p| (RATIO / RESIDUAL)
     const float* vmax;       // device scalar: max |voxel value| of vol (1 for the weight pass)
     float wmul;              // bound of a voxel's splat value per unit |voxel value|
@@ -129,8 +135,9 @@ struct BwdArgs {
     float* factor;
     int64_t chan_stride;
     const float* g2;         // XTT [n_cond*P][K]: (S_k.G)^2
-    // untilted XTT [n_cond*P][5][K], with sig_k = |S_k.G|: sig_k S_k,z, (S_k.G)^2 |S_k|^2,
-    // sig_k vs S_k,x, sig_k vs S_k,y, sig_k (d0_x S_k,x + d0_y S_k,y)
+    // untilted XTT [n_cond*P][5][kXwStride], with sig_k = |S_k.G|: sig_k S_k,z,
+    // (S_k.G)^2 |S_k|^2, sig_k vs S_k,x, sig_k vs S_k,y, sig_k (d0_x S_k,x + d0_y S_k,y);
+    // entries k >= K are zero
     const float* xw;
     float alpha;             // ART step lambda/subsetSize
     int overwrite;           // FDK: 1 -> vol = sum, 0 -> vol += sum

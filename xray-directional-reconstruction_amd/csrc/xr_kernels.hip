@@ -543,7 +543,8 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
     const auto vol = to_global(a.vol + (KIND == FWD_PLAIN ? (int64_t)cond * a.vol_cond_stride : 0));
     float g2[kMaxBasis];
     // untilted XTT: the view's row of the per-plan weight table (BwdArgs::xw)
-    const auto xrow = to_const(a.xw ? a.xw + (int64_t)code * 5 * d.K : nullptr);
+    constexpr int xws = kXwStride;
+    const auto xrow = to_const(a.xw ? a.xw + (int64_t)code * 5 * xws : nullptr);
     if (KIND == FWD_XTT && !AXIS) {
 #pragma unroll
         for (int k = 0; k < kMaxBasis; k++) g2[k] = (k < d.K) ? to_const(a.g2)[(int64_t)code * d.K + k] : 0.f;
@@ -589,8 +590,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                 xcxy = fmaf(dy, dy, dx * dx);
 #pragma unroll
                 for (int k = 0; k < KM; k++)
-                    xck[k] = (k < d.K) ? fmaf(xp, xrow[2 * d.K + k], fmaf(yp, xrow[3 * d.K + k], xrow[4 * d.K + k]))
-                                       : 0.f;
+                    xck[k] = fmaf(xp, xrow[2 * xws + k], fmaf(yp, xrow[3 * xws + k], xrow[4 * xws + k]));
             }
             if (x < d.N) {
                 const ColForms c = col_forms(g, (float)x - d.xc, yp);
@@ -652,14 +652,25 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                         const auto col = vol + cbase + lane;
                         for (int zg = zb & ~63; zg < ze; zg += 64 * kXttChunks) {
                             float mu[kXttChunks][KM];
-#pragma unroll
-                            for (int c = 0; c < kXttChunks; c++) {
-                                const int z = zg + c * 64 + lane;
-                                const bool in = whole ? (zg + c * 64 < d.N) : (z >= zb && z < ze);
+                            if (EXACT && whole && zg + 64 * kXttChunks <= d.N) {
+                                // every lane in range: unmasked loads, one address per channel
+                                // walked from the previous one (the chunks are immediate offsets)
 #pragma unroll
                                 for (int k = 0; k < KM; k++)
-                                    mu[c][k] = (in && (EXACT || k < d.K)) ? col[zg + c * 64 + (int64_t)k * a.chan_stride]
-                                                                          : 0.f;
+#pragma unroll
+                                    for (int c = 0; c < kXttChunks; c++)
+                                        mu[c][k] = col[zg + c * 64 + (int64_t)k * a.chan_stride];
+                            } else {
+#pragma unroll
+                                for (int c = 0; c < kXttChunks; c++) {
+                                    const int z = zg + c * 64 + lane;
+                                    const bool in = whole ? (zg + c * 64 < d.N) : (z >= zb && z < ze);
+#pragma unroll
+                                    for (int k = 0; k < KM; k++)
+                                        mu[c][k] = (in && (EXACT || k < d.K))
+                                                       ? col[zg + c * 64 + (int64_t)k * a.chan_stride]
+                                                       : 0.f;
+                                }
                             }
                             const float zoff = (float)zg - d.xc;
                             float dz[kXttChunks], id2[kXttChunks], val[kXttChunks];
@@ -672,7 +683,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
 #pragma unroll
                             for (int k = 0; k < KM; k++) {
                                 if (EXACT || k < d.K) {
-                                    const float ck = lane_f(xck[k], b), sS2 = xrow[k], g2s = xrow[d.K + k];
+                                    const float ck = lane_f(xck[k], b), sS2 = xrow[k], g2s = xrow[xws + k];
 #pragma unroll
                                     for (int c = 0; c < kXttChunks; c++) {
                                         const float q = fmaf(dz[c], sS2, ck);
@@ -1481,6 +1492,314 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
     return backward_update<KIND, CPW, ZPL, 1>(a, tmp, fac, nwhole, xb, yb, z0, lane);
 }
 
+// ---------------------------------------------------------------------------
+// Untilted XTT backprojection.  A wave owns one column and ZPL voxels per lane
+// in each of its K channels.  Per view and voxel the work is the bilinear
+// gather, dz and 1/|d|^2, then per channel w_k = g2s_k - q_k^2/|d|^2 with
+// q_k = dz sig_k S_k,z + ca_k (ir.cu:1832-1840) and two accumulations.
+// Everything else is the same for all lanes of a wave and is computed
+// lane-parallel 16 views at a time, as in the plain untilted path: the block
+// footprints by wave 0 (lane = view x corner pair), the column terms and the
+// K column constants ca_k = sig_k (d_x S_k,x + d_y S_k,y) by every wave for
+// its own column (lane = view x channel quarter), all into LDS; the view loop
+// reads them back with broadcast LDS loads.  The view's sig_k S_k,z and g2s_k
+// are scalar loads of the plan's weight table.  Footprints are staged in
+// LDS, double buffered, with as many fetch passes as the footprint needs
+// (at most kXttStagePer; a taller one is read from global memory directly).
+// ---------------------------------------------------------------------------
+constexpr int kXttStagePer = 4;
+
+template <int KIND, int ZPL, int KM>
+__device__ __forceinline__ float backward_item_untilted_xtt(const BwdArgs& a, int o, int step, float* stage) {
+    constexpr int KP = (KM + 3) & ~3;  // channel constants per view, padded for 16-byte LDS loads
+    constexpr int TW = 8 + KP;         // term words per view: Bc, Ac, fu, hi, cxy, lu, iu, boff, ca[KP]
+    static_assert(kBwdBatch * TW <= kBwdTermArea, "term layout");
+    const Dims& d = a.d;
+    constexpr int xws = kXwStride;  // >= KM: whole rows, zero padded
+    const auto views = to_global(a.views + (int64_t)step * a.n_views);
+    const auto vg = to_global(a.vg);
+    const auto vgc = to_const(a.vg);
+    const int gx = a.gx;
+    const int tx = o % a.tiles_x, ty = (o / a.tiles_x) % a.tiles_y, tz = o / (a.tiles_x * a.tiles_y);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int xb = tx * gx + (wave % gx), yb = ty * a.gy + (wave / gx);
+    const int z0 = tz * 64 * ZPL;
+    const float bx0 = (float)(tx * gx) - d.xc, bx1 = (float)min(tx * gx + gx - 1, d.N - 1) - d.xc;
+    const float by0 = (float)(ty * a.gy) - d.xc, by1 = (float)min(ty * a.gy + a.gy - 1, d.N - 1) - d.xc;
+    const float bz0 = (float)z0 - d.xc, bz1 = (float)min(z0 + 64 * ZPL - 1, d.N - 1) - d.xc;
+    const float xp = (float)xb - d.xc, yp = (float)yb - d.xc;
+    const bool col_in = xb < d.N && yb < d.N;
+    const int nv = a.n_views;
+    if (nv == 0) return 0.f;
+
+    float tmp[1][ZPL][KM], fac[1][ZPL][KM];
+#pragma unroll
+    for (int q = 0; q < ZPL; q++)
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            tmp[0][q][k] = 0.f;
+            fac[0][q][k] = 0.f;
+        }
+    const int nwhole[1] = {0};
+    float zpq[ZPL];
+#pragma unroll
+    for (int q = 0; q < ZPL; q++) zpq[q] = (float)(z0 + lane + 64 * q) - d.xc;
+
+    lds_float* const lds = (lds_float*)stage;
+    lds_int* const foot = (lds_int*)(stage + 2 * kStageBuf);
+    lds_float* const term = (lds_float*)(stage + 2 * kStageBuf + kBwdFootArea + wave * kBwdTermArea);
+    // staged only when the footprint fits the fetch passes of this path
+    auto staged_here = [](int fl, int H, int R) { return (fl & 1) != 0 && H <= kXttStagePer * R; };
+
+    auto foot_batch = [&](int b0, int buf) {
+        const int j = lane >> 2, c = lane & 3;
+        const ViewGeom g = vg[views[min(b0 + j, nv - 1)]];
+        const ColForms cf = col_forms(g, (c & 1) ? bx1 : bx0, (c & 2) ? by1 : by0);
+        float umn = 0.f, umx = 0.f, vmn = 0.f, vmx = 0.f;
+        int bad = 0;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const float czp = k ? bz1 : bz0;
+            const float cd = fmaf(g.ad[2], czp, cf.nd);
+            float cu, cv;
+            voxel_uv(g, cf, czp, d, cu, cv);
+            umn = k ? fminf(umn, cu) : cu;
+            umx = k ? fmaxf(umx, cu) : cu;
+            vmn = k ? fminf(vmn, cv) : cv;
+            vmx = k ? fmaxf(vmx, cv) : cv;
+            bad |= (!(cd > 0.f) || !(fabsf(cu) < 1e7f) || !(fabsf(cv) < 1e7f)) ? 1 : 0;
+        }
+        for (int off = 1; off < 4; off <<= 1) {
+            umn = fminf(umn, __shfl_xor(umn, off));
+            umx = fmaxf(umx, __shfl_xor(umx, off));
+            vmn = fminf(vmn, __shfl_xor(vmn, off));
+            vmx = fmaxf(vmx, __shfl_xor(vmx, off));
+            bad |= __shfl_xor(bad, off);
+        }
+        const Foot f = foot_from_extent(umn, umx, vmn, vmx, bad, d);
+        if (c == 0) {
+            lds_int* p = foot + (buf * kBwdBatch + j) * kFootWords;
+            p[0] = f.iu_lo;
+            p[1] = f.iv_lo;
+            p[2] = f.W;
+            p[3] = f.H;
+            p[4] = f.Ws;
+            p[5] = f.R;
+            p[6] = (f.staged ? 1 : 0) | (f.empty ? 2 : 0);
+        }
+    };
+    auto load_foot = [&](int buf, int j) {
+        const lds_int* p = foot + (buf * kBwdBatch + j) * kFootWords;
+        Foot f;
+        f.iu_lo = uniformi(p[0]);
+        f.iv_lo = uniformi(p[1]);
+        f.W = uniformi(p[2]);
+        f.H = uniformi(p[3]);
+        f.Ws = uniformi(p[4]);
+        f.R = uniformi(p[5]);
+        const int fl = uniformi(p[6]);
+        f.staged = staged_here(fl, f.H, f.R);
+        f.empty = (fl & 2) != 0;
+        return f;
+    };
+    // column terms of views b0 .. b0+15: lane = 4 x view + channel quarter
+    auto term_batch = [&](int b0, int buf) {
+        const int j = lane >> 2, p = lane & 3;
+        const int code = views[min(b0 + j, nv - 1)];
+        const ViewGeom g = vg[code];
+        const lds_int* fp = foot + (buf * kBwdBatch + j) * kFootWords;
+        const int iu_lo = fp[0], iv_lo = fp[1], W = fp[2], H = fp[3], Ws = fp[4], R = fp[5], fl = fp[6];
+        const ColForms c = col_forms(g, xp, yp);
+        const float inv_c = rcpf(c.nd);
+        const float u_c = fmaf(c.nu, inv_c, d.hu);
+        int iu;
+        float fu;
+        split(u_c, iu, fu);
+        const float Bc = g.av[2] * inv_c;
+        const float Ac = fmaf(c.nv, inv_c, d.hv - 0.5f);
+        const float hi = accept_u(u_c, d) ? d.vmax - 0.5f : -1.0f;
+        const int lu = min(max(iu - iu_lo, 0), W - 2);
+        const float dx = fmaf(d.vs, xp, g.d0[0]);
+        const float dy = fmaf(d.vs, yp, g.d0[1]);
+        const float ta = fmaf(Bc, bz0, Ac);
+        const float tb = fmaf(Bc, (float)(z0 + 64 * ZPL - 1) - d.xc, Ac);
+        const bool whole = staged_here(fl, H, R) && fminf(ta, tb) > 0.05f && fmaxf(ta, tb) < hi;
+        lds_float* tp = term + j * TW;
+        lds_int* ti = (lds_int*)tp;
+        if (p == 0) {
+            tp[0] = Bc;
+            tp[1] = Ac;
+            tp[2] = fu;
+            tp[3] = whole ? __int_as_float(0x7f800000) : hi;
+            tp[4] = fmaf(dy, dy, dx * dx);
+            ti[5] = lu;
+            ti[6] = iu;
+            ti[7] = (lu - iv_lo * Ws) * 4;  // LDS byte offset of texel (iu, 0) in a staging buffer
+        }
+        const auto row = to_global(a.xw + (int64_t)code * 5 * xws);
+#pragma unroll
+        for (int m = 0; m < KP / 4; m++) {
+            const int k = p + 4 * m;
+            tp[8 + k] = fmaf(xp, row[2 * xws + k], fmaf(yp, row[3 * xws + k], row[4 * xws + k]));  // 0 for k >= K
+        }
+    };
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+
+    float pre[kXttStagePer];
+    int pidx[kXttStagePer];
+    auto fetch = [&](const Foot& f, int code) {
+        const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
+        int r0 = (int)((float)threadIdx.x * (1.0f / (float)f.W));
+        int c = (int)threadIdx.x - r0 * f.W;
+        if (c < 0) {
+            r0--;
+            c += f.W;
+        } else if (c >= f.W) {
+            r0++;
+            c -= f.W;
+        }
+        const bool act = r0 < f.R;
+        const auto src = img + (int64_t)(f.iv_lo + r0) * d.U + f.iu_lo + c;
+        const int64_t rstep = (int64_t)f.R * d.U;
+#pragma unroll
+        for (int k = 0; k < kXttStagePer; k++) {
+            pidx[k] = -1;
+            if (k * f.R < f.H) {  // uniform: only the passes the footprint needs
+                const int r = r0 + k * f.R;
+                const bool in = act && r < f.H;
+                pidx[k] = in ? r * f.Ws + c : -1;
+                pre[k] = in ? src[k * rstep] : 0.f;
+            }
+        }
+    };
+    auto store = [&](const Foot& f, lds_float* buf) {
+#pragma unroll
+        for (int k = 0; k < kXttStagePer; k++)
+            if (k * f.R < f.H && pidx[k] >= 0) buf[pidx[k]] = pre[k];
+    };
+
+    if (wave == 0) foot_batch(0, 0);
+    __syncthreads();
+    term_batch(0, 0);
+    int code = views[0];
+    Foot f = load_foot(0, 0);
+    if (f.staged) {
+        fetch(f, code);
+        store(f, lds);
+    }
+    __syncthreads();
+
+    for (int s = 0; s < nv; s++) {
+        const int j = s & (kBwdBatch - 1);
+        if (j == 0 && s > 0) term_batch(s, (s / kBwdBatch) & 1);
+        wave_sync();
+        if (wave == 0 && j == kBwdBatch - 2 && s + 2 < nv) foot_batch(s + 2, ((s + 2) / kBwdBatch) & 1);
+        lds_float* cur = lds + (s & 1) * kStageBuf;
+        lds_float* nxt = lds + ((s + 1) & 1) * kStageBuf;
+        int ncode = 0;
+        Foot nf{0, 0, 0, 0, 0, 0, false, false};
+        if (s + 1 < nv) {
+            ncode = views[s + 1];
+            nf = load_foot(((s + 1) / kBwdBatch) & 1, (s + 1) & (kBwdBatch - 1));
+            if (nf.staged) fetch(nf, ncode);
+        }
+        if (!f.empty && col_in) {
+            const lds_float* tp = term + j * TW;
+            const lds_int* ti = (const lds_int*)tp;
+            const float Bc = tp[0], Ac = tp[1], fu = tp[2], hi = tp[3], cxy = tp[4];
+            const int lu = ti[5], iu = ti[6], boff = ti[7];
+            const float a0 = 1.0f - fu;
+            const float d0z = vgc[code].d0[2];
+            const auto row = to_const(a.xw + (int64_t)code * 5 * xws);
+            float ca[KM], sz[KM], g2s[KM];
+#pragma unroll
+            for (int k = 0; k < KM; k++) {
+                ca[k] = tp[8 + k];
+                sz[k] = row[k];
+                g2s[k] = row[xws + k];
+            }
+            // channels k >= K have zero constants (table padding): they add zero and
+            // are never stored
+            auto accumulate = [&](int q, float nb) {
+                const float dz = fmaf(d.vs, zpq[q], d0z);
+                const float id2 = rcpf(fmaf(dz, dz, cxy));
+#pragma unroll
+                for (int k = 0; k < KM; k++) {
+                    const float qk = fmaf(dz, sz[k], ca[k]);
+                    const float w = fmaf(-qk * qk, id2, g2s[k]);
+                    tmp[0][q][k] = fmaf(w, nb, tmp[0][q][k]);
+                    fac[0][q][k] += w;
+                }
+            };
+            if (uniformi(__float_as_int(hi)) == 0x7f800000) {
+                // the whole z extent accepted: all reads first (LDS latency), then the sums
+                const unsigned ws4 = (unsigned)f.Ws * 4u;
+                const unsigned base = (unsigned)(uintptr_t)cur + (unsigned)boff;
+                float fv[ZPL], r[ZPL][4];
+#pragma unroll
+                for (int q = 0; q < ZPL; q++) {
+                    const float t = fmaf(Bc, zpq[q], Ac);
+                    fv[q] = __builtin_amdgcn_fractf(t);
+                    const unsigned at = __umul24((unsigned)floor_int(t), ws4) + base;
+                    const lds_float* p0 = (const lds_float*)(uintptr_t)at;
+                    const lds_float* p1 = (const lds_float*)(uintptr_t)(at + ws4);
+                    r[q][0] = p0[0];
+                    r[q][1] = p0[1];
+                    r[q][2] = p1[0];
+                    r[q][3] = p1[1];
+                }
+#pragma unroll
+                for (int q = 0; q < ZPL; q++) {
+                    const float top = fmaf(fu, r[q][1], a0 * r[q][0]);
+                    const float bot = fmaf(fu, r[q][3], a0 * r[q][2]);
+                    accumulate(q, fmaf(fv[q], bot - top, top));
+                }
+            } else {
+                // window test per voxel, 0.05 < t < hi (hi < 0 rejects the column); a
+                // rejected voxel reads an in-bounds texel and skips the sums.  Lanes
+                // with z >= N accumulate but are never stored.
+                const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
+#pragma unroll
+                for (int q = 0; q < ZPL; q++) {
+                    const float t = fmaf(Bc, zpq[q], Ac);
+                    const bool ok = t > 0.05f && t < hi;
+                    const float fl = floorf(t);
+                    const float fv = t - fl;
+                    const int iv = (int)fl;
+                    float r00, r01, r10, r11;
+                    if (f.staged) {
+                        const int qi = (ok ? iv - f.iv_lo : 0) * f.Ws + lu;
+                        r00 = cur[qi];
+                        r01 = cur[qi + 1];
+                        r10 = cur[qi + f.Ws];
+                        r11 = cur[qi + f.Ws + 1];
+                    } else {
+                        const int64_t qi = (int64_t)min(max(iv, 0), d.V - 2) * d.U + min(max(iu, 0), d.U - 2);
+                        r00 = __builtin_nontemporal_load(img + qi);
+                        r01 = __builtin_nontemporal_load(img + qi + 1);
+                        r10 = __builtin_nontemporal_load(img + qi + d.U);
+                        r11 = __builtin_nontemporal_load(img + qi + d.U + 1);
+                    }
+                    const float top = fmaf(fu, r01, a0 * r00);
+                    const float bot = fmaf(fu, r11, a0 * r10);
+                    if (ok) accumulate(q, fmaf(fv, bot - top, top));
+                }
+            }
+        }
+        if (s + 1 < nv) {
+            if (nf.staged) store(nf, nxt);
+            code = ncode;
+            f = nf;
+        }
+        __syncthreads();
+    }
+    return backward_update<KIND, 1, ZPL, KM>(a, tmp, fac, nwhole, xb, yb, z0, lane);
+}
+
 // One voxel tile o of a backprojection launch; step selects the view list
 // views + step*n_views (a solver's subset step).  stage: 2 x kStageBuf floats.
 // Returns the lane's max |updated voxel| (MLEM / ART kinds).
@@ -1491,6 +1810,9 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
     constexpr int KA = XTT ? KM : 1;
     if constexpr (AXIS && !XTT && CPW == 2) {
         return backward_item_untilted<KIND, ZPL>(a, o, step, stage);
+    }
+    if constexpr (AXIS && XTT && CPW == 1) {
+        return backward_item_untilted_xtt<KIND, ZPL, KM>(a, o, step, stage);
     }
     const Dims& d = a.d;
     const auto views = to_const(a.views + (int64_t)step * a.n_views);
@@ -1616,98 +1938,6 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
                     if (XTT) {
                         dx = fmaf(d.vs, xp, g.d0[0]);
                         dy = fmaf(d.vs, yp, g.d0[1]);
-                    }
-                    if (AXIS && XTT) {
-                        // Untilted XTT: the column's u split and t(z) = v - 0.5 as in the
-                        // plain path; per voxel the weights w_k = (S_k.G)^2 |B x S_k|^2
-                        // (ir.cu:1832-1840) are g2s_k - q_k^2 / |d|^2 with
-                        // q_k = sig_k (d.S_k) = dz sig_k S_k,z + ca_k, ca_k affine in the
-                        // column's (x', y').  The k loop is outermost so that only the
-                        // per-voxel gather, dz and 1/|d|^2 stay live across it.
-                        int iu;
-                        float fu;
-                        split(u_c, iu, fu);
-                        const float a0 = 1.0f - fu;
-                        const float Bc = g.av[2] * inv_c;
-                        const float Ac = fmaf(c.nv, inv_c, d.hv - 0.5f);
-                        const float hi = accept_u(u_c, d) ? d.vmax - 0.5f : -1.0f;
-                        const int lu = min(max(iu - f.iu_lo, 0), f.W - 2);
-                        const float cxy = fmaf(dy, dy, dx * dx);
-                        const float ta = fmaf(Bc, (float)z0 - d.xc, Ac);
-                        const float tb = fmaf(Bc, (float)(z0 + 64 * ZPL - 1) - d.xc, Ac);
-                        const auto row = to_const(a.xw + (int64_t)code * 5 * d.K);
-                        // MASK false: the tile's whole z extent is accepted (t at its first
-                        // and last z bound every lane's t), no per-voxel window test.
-                        // EXACT: d.K == KA, the k loops need no bound.
-                        // Lanes with z >= N accumulate but are never stored.
-                        auto voxels = [&](auto mask_c, auto exact_c) {
-                            constexpr bool MASK = decltype(mask_c)::value;
-                            constexpr bool EXACT = decltype(exact_c)::value;
-                            // the view's constants of the column: scalar loads, ca in VGPRs
-                            float ca[KA], sS2[KA], g2s[KA];
-#pragma unroll
-                            for (int k = 0; k < KA; k++) {
-                                const bool kin = EXACT || k < d.K;
-                                sS2[k] = kin ? row[k] : 0.f;
-                                g2s[k] = kin ? row[d.K + k] : 0.f;
-                                ca[k] = kin ? fmaf(xp, row[2 * d.K + k], fmaf(yp, row[3 * d.K + k], row[4 * d.K + k]))
-                                            : 0.f;
-                            }
-                            float nb[ZPL], dzq[ZPL], id2[ZPL];
-                            bool okq[ZPL];
-#pragma unroll
-                            for (int q = 0; q < ZPL; q++) {
-                                const float t = fmaf(Bc, zpq[q], Ac);
-                                const bool ok = !MASK || (t > 0.05f && t < hi);
-                                okq[q] = ok;
-                                const float fl = floorf(t);
-                                const float fv = t - fl;
-                                const int iv = (int)fl;
-                                float r00, r01, r10, r11;
-                                if (f.staged) {
-                                    const int qi = (ok ? iv - f.iv_lo : 0) * f.Ws + lu;
-                                    r00 = cur[qi];
-                                    r01 = cur[qi + 1];
-                                    r10 = cur[qi + f.Ws];
-                                    r11 = cur[qi + f.Ws + 1];
-                                } else {
-                                    const int64_t qi =
-                                        (int64_t)min(max(iv, 0), d.V - 2) * d.U + min(max(iu, 0), d.U - 2);
-                                    r00 = __builtin_nontemporal_load(img + qi);
-                                    r01 = __builtin_nontemporal_load(img + qi + 1);
-                                    r10 = __builtin_nontemporal_load(img + qi + d.U);
-                                    r11 = __builtin_nontemporal_load(img + qi + d.U + 1);
-                                }
-                                const float top = fmaf(fu, r01, a0 * r00);
-                                const float bot = fmaf(fu, r11, a0 * r10);
-                                nb[q] = fmaf(fv, bot - top, top);
-                                dzq[q] = fmaf(d.vs, zpq[q], g.d0[2]);
-                                id2[q] = rcpf(fmaf(dzq[q], dzq[q], cxy));
-                            }
-#pragma unroll
-                            for (int k = 0; k < KA; k++) {
-                                if (EXACT || k < d.K) {
-#pragma unroll
-                                    for (int q = 0; q < ZPL; q++) {
-                                        const float qk = fmaf(dzq[q], sS2[k], ca[k]);
-                                        float w = fmaf(-qk * qk, id2[q], g2s[k]);
-                                        if (MASK) w = okq[q] ? w : 0.f;  // a rejected voxel adds nothing
-                                        tmp[cidx][q][k] = fmaf(w, nb[q], tmp[cidx][q][k]);
-                                        fac[cidx][q][k] += w;
-                                    }
-                                }
-                            }
-                        };
-                        const bool all_in = f.staged && uniformi(fminf(ta, tb) > 0.05f && fmaxf(ta, tb) < hi);
-                        if (d.K == KA) {
-                            if (all_in)
-                                voxels(std::false_type(), std::true_type());
-                            else
-                                voxels(std::true_type(), std::true_type());
-                        } else {
-                            voxels(std::true_type(), std::false_type());
-                        }
-                        continue;
                     }
                     if (AXIS && !XTT) {
                         // Untilted: u, hence (iu, fu), is constant along the column, the
@@ -2404,9 +2634,8 @@ hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream) {
     if (!backward_shape(a, a.zpl, a.gx, xtt)) return hipErrorInvalidValue;
     const int zpl = a.zpl;
     const unsigned grid = (unsigned)(8 * a.per_xcd);
-    // plain kinds: staging buffers, per-view term areas and the per-wave max
-    // slots; XTT: staging buffers only
-    const size_t lds = (2 * kStageBuf + (xtt ? 0 : kBwdExtraWords + kBwdWaves)) * sizeof(float);
+    // staging buffers, per-view footprint and term areas, per-wave max slots
+    const size_t lds = (2 * kStageBuf + kBwdExtraWords + kBwdWaves) * sizeof(float);
 #define XR_BWD(KD, A, C, Z, KM) \
     hipLaunchKernelGGL((k_backward<KD, A, C, Z, KM>), dim3(grid), dim3(kBwdThreads), lds, stream, a)
 #define XR_BWD_AX(KD, C, Z, KM)         \

@@ -631,7 +631,8 @@ int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int 
     std::vector<ViewGeom> vg((size_t)C * P);
     std::vector<ChordGeom> cg((size_t)C * P);
     std::vector<float> g2((size_t)C * P * K, 0.f);
-    std::vector<float> xw((size_t)C * P * 5 * K, 0.f);
+    const int KS = kXwStride;
+    std::vector<float> xw((size_t)C * P * 5 * KS, 0.f);
     p->axis_cond.assign(C, 1);
     for (int c = 0; c < C; c++)
         for (int v = 0; v < P; v++) {
@@ -645,13 +646,13 @@ int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int 
                 // untilted XTT weight constants (see BwdArgs::xw)
                 const double Sx = p->basis[3 * k], Sy = p->basis[3 * k + 1], Sz = p->basis[3 * k + 2];
                 const double sig = std::fabs(sg);
-                float* row = &xw[((size_t)c * P + v) * 5 * K];
+                float* row = &xw[((size_t)c * P + v) * 5 * KS];
                 const double vs = g->vox_size;
                 row[k] = (float)(sig * Sz);
-                row[K + k] = (float)((double)(float)(sg * sg) * (double)(float)(Sx * Sx + Sy * Sy + Sz * Sz));
-                row[2 * K + k] = (float)(sig * vs * Sx);
-                row[3 * K + k] = (float)(sig * vs * Sy);
-                row[4 * K + k] = (float)(sig * ((double)t.d0[0] * Sx + (double)t.d0[1] * Sy));
+                row[KS + k] = (float)((double)(float)(sg * sg) * (double)(float)(Sx * Sx + Sy * Sy + Sz * Sz));
+                row[2 * KS + k] = (float)(sig * vs * Sx);
+                row[3 * KS + k] = (float)(sig * vs * Sy);
+                row[4 * KS + k] = (float)(sig * ((double)t.d0[0] * Sx + (double)t.d0[1] * Sy));
             }
         }
     Dims& d = p->dims;

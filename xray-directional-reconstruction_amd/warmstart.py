@@ -166,14 +166,17 @@ def project(ctx, geom, vol_cols):
 
 
 def generate_pairs(ctx, count, N=128, P=180, U=192, sdd=1000.0, sod=500.0, det=0.4, pre=5, post=200, seed=0,
-                   per_batch=48, init=0.01, subsets=1, family="fibres", log=None):
+                   per_batch=48, init=0.01, subsets=1, family="fibres", photons=None, log=None):
     """(inputs, targets, geometry): the plain method's volume (MLEM with
     `subsets` ordered subsets, IR::reconstruct's batch) after `pre` and after
     `post` epochs for `count` seeded phantoms (`phantom_spec(seed + i,
     family)`), all from the batched solver (`per_batch` problems per
     persistent launch).  `pre` may be a list of epoch counts: each phantom
     then gives one input per count, all paired with the same target (the same
-    tensor object).  Lists of column-layout device tensors."""
+    tensor object).  `photons`: Poisson noise at that many photons per ray
+    (seeded per phantom) and the driver's clamp of the noisy sinogram
+    (main.cu:35); None leaves the sinogram noise free.  Lists of column-layout
+    device tensors."""
     import torch
 
     pres = sorted(set([pre] if np.isscalar(pre) else [int(p) for p in pre]))
@@ -197,6 +200,12 @@ def generate_pairs(ctx, count, N=128, P=180, U=192, sdd=1000.0, sod=500.0, det=0
         for i in range(n):
             vol = make_phantom(ctx, N, vs, seed + b0 + i, **phantom_spec(seed + b0 + i, N, vs, family))
             sino = project(ctx, geom, vol)
+            if photons:
+                clean = sino
+                sino = torch.empty_like(clean)
+                ctx.noise_poisson(sino, clean, photons, seed + b0 + i)
+                ctx.sino_clamp(sino)
+                del clean
             va, sa = batch.buffers(i)
             ctx.copy(sa, sino)
             ctx.copy(va, torch.full((N ** 3,), init, dtype=torch.float32, device=sino.device))
