@@ -192,6 +192,22 @@ int xr_project(xr_context* ctx, xr_plan* plan, const float* vol, float* proj, co
 int xr_residual(xr_context* ctx, xr_plan* plan, const float* vol, float* resid, const float* sino,
                 const int32_t* views, int n_views, int xtt, double* l1, double* l2);
 
+/* xr_project without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and results as xr_project; the view list, accumulate flags, FP64 loss sum
+ * and max |voxel| of the call live in a fixed 80.25 KiB device array per GPU
+ * (the library's `scratch`), held until the process ends, instead of buffers
+ * the library allocates.  Limits: at most 16384 views per call (larger lists
+ * return XR_ERR_INVALID); the plan's normalisation table must already exist
+ * (any earlier xr_project call on the plan builds it, else XR_ERR_INVALID);
+ * no kernel timing.  Every call on one GPU shares that GPU's array, which
+ * keeps the last call's contents: calls that overlap on one GPU (other
+ * streams, other host threads, other contexts) race. */
+int xr_project_noalloc(xr_context* ctx, xr_plan* plan, const float* vol, float* proj, const float* sino,
+                       const int32_t* views, int n_views, const uint8_t* accumulate, int xtt, int out_mode,
+                       double* loss);
+
 /* Raw backprojection of the listed views into column-layout accumulators:
  * tmp += sum_c c*r, factor += 1 (backwardProj, ir.cu:1605-1614, 1720-1751) or,
  * xtt != 0, per channel tmp_k += v_km^2 sum_c c*r, factor_k += v_km^2

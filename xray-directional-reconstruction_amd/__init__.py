@@ -104,6 +104,7 @@ def _load():
         "xr_volume_to_columns": (I, [P, P, P, I, I]),
         "xr_volume_from_columns": (I, [P, P, P, I, I]),
         "xr_project": (I, [P, P, P, P, P, _c_int32_p, I, _c_uint8_p, I, I, _c_double_p]),
+        "xr_project_noalloc": (I, [P, P, P, P, P, _c_int32_p, I, _c_uint8_p, I, I, _c_double_p]),
         "xr_residual": (I, [P, P, P, P, P, _c_int32_p, I, I, _c_double_p, _c_double_p]),
         "xr_backproject": (I, [P, P, P, _c_int32_p, I, I, P, P]),
         "xr_backproject_update": (I, [P, P, P, _c_int32_p, I, I, I, F, I, P]),
@@ -463,6 +464,18 @@ class Plan:
                                         c.ctypes.data_as(_c_int32_p), len(c), int(xtt), ctypes.byref(l1),
                                         ctypes.byref(l2)))
         return l1.value, l2.value
+
+    def project_noalloc(self, vol, proj, codes, out_mode=OUT_NORMALISED, sino=None, accumulate=None, xtt=False):
+        """project() through xr_project_noalloc (no allocation; the plan's
+        tables must exist, i.e. after a first project() call)."""
+        c = np.ascontiguousarray(codes, dtype=np.int32)
+        acc = None if accumulate is None else np.ascontiguousarray(accumulate, dtype=np.uint8)
+        loss = ctypes.c_double(0.0)
+        self.ctx.check(_lib.xr_project_noalloc(self.ctx.h, self.h, _dptr(vol), _dptr(proj), _dptr(sino),
+                                               c.ctypes.data_as(_c_int32_p), len(c),
+                                               None if acc is None else acc.ctypes.data_as(_c_uint8_p), int(xtt),
+                                               out_mode, ctypes.byref(loss)))
+        return loss.value
 
     def backproject(self, proj, codes, tmp, factor, xtt=False):
         c = np.ascontiguousarray(codes, dtype=np.int32)

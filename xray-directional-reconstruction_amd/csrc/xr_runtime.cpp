@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -19,6 +20,44 @@
 #include "xr_random.hpp"
 
 using namespace xr;
+
+// Scratch of xr_project_noalloc: the view list, the accumulate flags, the
+// FP64 loss sum and max |voxel| of one call live in this array instead of
+// buffers the library allocates.  Each device holds its own copy (found with
+// hipGetSymbolAddress at context creation); all calls on one device share it.
+namespace {
+constexpr int kNoAllocMaxViews = 16384;
+constexpr size_t kScratchLoss = 0;                                    // double
+constexpr size_t kScratchVmax = 8;                                    // float
+constexpr size_t kScratchCodes = 256;                                 // int32 x kNoAllocMaxViews
+constexpr size_t kScratchFlags = kScratchCodes + 4 * kNoAllocMaxViews;  // uint8 x kNoAllocMaxViews
+constexpr size_t kScratchBytes = kScratchFlags + kNoAllocMaxViews;     // 80.25 KiB
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch[kMaxDevices] = {};  // device address of `scratch` per device
+
+// Device address of this device's scratch; the device must be current.
+hipError_t find_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<unsigned char*>(ptr);
+    return hipSuccess;
+}
+
+unsigned char* scratch_of(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return (device >= 0 && device < kMaxDevices) ? g_scratch[device] : nullptr;
+}
+}  // namespace
 
 namespace xr {
 namespace host {
@@ -509,6 +548,7 @@ int xr_context_create(xr_context** out, int device) {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return XR_ERR_HIP;
     if (device < 0 || device >= n) return XR_ERR_INVALID;
     if (hipSetDevice(device) != hipSuccess) return XR_ERR_HIP;
+    if (find_scratch(device) != hipSuccess) return XR_ERR_HIP;
     auto* c = new (std::nothrow) xr_context();
     if (!c) return XR_ERR_ALLOC;
     c->device = device;
@@ -759,6 +799,47 @@ int xr_project(xr_context* ctx, xr_plan* plan, const float* vol, float* proj, co
                           accumulate ? ctx->d_flags : nullptr, n_views, out_mode, ctx->d_loss));
     if (loss) {
         XR_HIP(ctx, hipMemcpyAsync(loss, ctx->d_loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return XR_OK;
+}
+
+int xr_project_noalloc(xr_context* ctx, xr_plan* plan, const float* vol, float* proj, const float* sino,
+                       const int32_t* views, int n_views, const uint8_t* accumulate, int xtt, int out_mode,
+                       double* loss) {
+    if (!ctx || !plan || !vol || !proj || !views || n_views < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    if (out_mode < XR_OUT_RAW || out_mode > XR_OUT_RESIDUAL) return fail(ctx, XR_ERR_INVALID, "bad out_mode");
+    if ((out_mode == XR_OUT_RATIO || out_mode == XR_OUT_RESIDUAL) && !sino)
+        return fail(ctx, XR_ERR_INVALID, "sino required for ratio/residual");
+    if (xtt && plan->n_basis <= 0) return fail(ctx, XR_ERR_INVALID, "XTT projection needs basis vectors");
+    if (plan->conv != XR_CONV_IR) return fail(ctx, XR_ERR_INVALID, "forward projection uses the IR convention");
+    if (n_views > kNoAllocMaxViews)
+        return fail(ctx, XR_ERR_INVALID, "xr_project_noalloc takes at most %d views per call (got %d)",
+                    kNoAllocMaxViews, n_views);
+    if (n_views == 0) {
+        if (loss) *loss = 0;
+        return XR_OK;
+    }
+    const int total = plan->n_cond * plan->g.n_proj;
+    for (int i = 0; i < n_views; i++)
+        if (views[i] < 0 || views[i] >= total)
+            return fail(ctx, XR_ERR_INVALID, "view code %d out of range [0, %d)", views[i], total);
+    unsigned char* base = scratch_of(ctx->device);
+    if (!base) return fail(ctx, XR_ERR_HIP, "no scratch address for device %d", ctx->device);
+    int32_t* d_codes = reinterpret_cast<int32_t*>(base + kScratchCodes);
+    uint8_t* d_flags = base + kScratchFlags;
+    double* d_loss = reinterpret_cast<double*>(base + kScratchLoss);
+    float* d_vmax = reinterpret_cast<float*>(base + kScratchVmax);
+    XR_HIP(ctx, hipMemcpyAsync(d_codes, views, sizeof(int32_t) * n_views, hipMemcpyHostToDevice, ctx->stream));
+    if (accumulate)
+        XR_HIP(ctx, hipMemcpyAsync(d_flags, accumulate, (size_t)n_views, hipMemcpyHostToDevice, ctx->stream));
+    // the FP64 loss starts at zero: two zero floats are a zero double
+    XR_HIP(ctx, launch_fill(reinterpret_cast<float*>(d_loss), 2, 0.0f, ctx->stream));
+    const int64_t N3 = (int64_t)plan->g.n_vox * plan->g.n_vox * plan->g.n_vox;
+    XR_TRY(forward_launch(plan, xtt ? FWD_XTT : FWD_PLAIN, vol, N3, proj, sino, 0, d_codes,
+                          accumulate ? d_flags : nullptr, n_views, out_mode, d_loss, nullptr, d_vmax, true));
+    if (loss) {
+        XR_HIP(ctx, hipMemcpyAsync(loss, d_loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     return XR_OK;
