@@ -131,9 +131,14 @@ def ttr_frozen(ctx, xr, geom, sino, tau=1e-3, subsets=80, long_epochs=60, pres=(
     if model and os.path.exists(model):
         net = xr.UNet(ctx, 24)
         net.load(model)
+        train_n = ws.trained_edge(model)
+        if train_n and (train_n > geom.n_vox or geom.n_vox % train_n):
+            train_n = None
+        out["network"] = {"file": os.path.relpath(model, ROOT), "trained_at": train_n,
+                          "applied_at": train_n or geom.n_vox}
         out["warm"] = []
         for pre in pres:
-            w = ws.warm_curve(scene, net, R0, Rinf, tau, long_epochs, pre=pre)
+            w = ws.warm_curve(scene, net, R0, Rinf, tau, long_epochs, pre=pre, train_n=train_n)
             w["R"] = [round(r, 9) for r in w["R"]]
             w["pre"] = pre
             w["s"] = round(w.pop("ms") / 1e3, 3)

@@ -231,6 +231,17 @@ int xr_fdk_backproject(xr_context* ctx, xr_plan* plan, const float* filt, const 
 /* The filter taps the FDK path uses (n_det_u floats, host out). */
 int xr_fdk_taps(const xr_geometry* g, int filter, float* taps_out);
 
+/* Multi-resolution helpers of the learned warm start (no counterpart in the
+ * reference), single-channel column-layout volumes on the device.
+ * xr_volume_downsample: coarse (edge n_vox/factor) = mean of each
+ * factor^3 block of fine (edge n_vox, a multiple of factor).
+ * xr_volume_upsample_add: fine (edge n_coarse*factor) = max(fine +
+ * trilinear(coarse_a - coarse_b), floor_value), fine voxel i at coarse
+ * coordinate (i + 0.5)/factor - 0.5 clamped to the coarse grid. */
+int xr_volume_downsample(xr_context* ctx, const float* fine, int n_vox, int factor, float* coarse);
+int xr_volume_upsample_add(xr_context* ctx, const float* coarse_a, const float* coarse_b, int n_coarse, int factor,
+                           float* fine, float floor_value);
+
 /* voxelSqrt (ir.cu:1707-1717): v = sqrt(|v|) for n floats (device). */
 int xr_volume_sqrt(xr_context* ctx, float* vol, int64_t n);
 

@@ -106,6 +106,66 @@ def test_identity_network_changes_nothing(xr, ctx, ws):
     net.close()
 
 
+def interp_matrix(Nc, f):
+    """Rows: fine index i of an axis, weights of trilinear interpolation at
+    coarse coordinate (i + 0.5)/f - 0.5 clamped to the coarse grid."""
+    M = np.zeros((Nc * f, Nc))
+    for i in range(Nc * f):
+        c = max((i + 0.5) / f - 0.5, 0.0)
+        j0 = min(int(c), Nc - 1)
+        j1 = min(j0 + 1, Nc - 1)
+        w = 0.0 if j0 == Nc - 1 else c - j0
+        M[i, j0] += 1.0 - w
+        M[i, j1] += w
+    return M
+
+
+@pytest.mark.parametrize("Nc,f", [(8, 2), (6, 4), (5, 1)])
+def test_volume_resampling_matches_numpy(xr, ctx, torch_dev, Nc, f):
+    """The warm start's block mean and trilinear correction against numpy
+    (column layout: index z + N (x + N y))."""
+    import torch
+
+    rng = np.random.default_rng(11 + Nc)
+    N = Nc * f
+    fine = rng.random(N ** 3).astype(np.float32)
+    coarse = torch.empty(Nc ** 3, device=torch_dev)
+    ctx.volume_downsample(torch.tensor(fine, device=torch_dev), N, f, coarse)
+    ctx.synchronize()
+    want = fine.astype(np.float64).reshape(Nc, f, Nc, f, Nc, f).mean(axis=(1, 3, 5)).reshape(-1)
+    assert np.allclose(coarse.cpu().numpy(), want, rtol=1e-5, atol=1e-6)
+    a = rng.random(Nc ** 3).astype(np.float32)
+    b = rng.random(Nc ** 3).astype(np.float32)
+    base = rng.random(N ** 3).astype(np.float32)
+    out = torch.tensor(base, device=torch_dev)
+    ctx.volume_upsample_add(torch.tensor(a, device=torch_dev), torch.tensor(b, device=torch_dev), Nc, f, out, 0.3)
+    ctx.synchronize()
+    M = interp_matrix(Nc, f)
+    d = (a.astype(np.float64) - b).reshape(Nc, Nc, Nc)
+    up = np.einsum("ya,xb,zc,abc->yxz", M, M, M, d)
+    want = np.maximum(base.reshape(N, N, N) + up, 0.3)
+    assert np.allclose(out.cpu().numpy().reshape(N, N, N), want, rtol=1e-5, atol=1e-5)
+    with pytest.raises(xr.XRError):
+        ctx.volume_downsample(torch.tensor(fine, device=torch_dev), N, N + 1, coarse)
+
+
+def test_network_applied_at_its_training_scale(xr, ctx, ws, torch_dev):
+    """An identity network applied at a coarser scale adds a zero correction:
+    the result is the input, floored."""
+    import torch
+
+    net = xr.UNet(ctx, 8, seed=2)
+    rng = np.random.default_rng(5)
+    x = torch.tensor((rng.random(64 ** 3) * 0.05).astype(np.float32), device=torch_dev)
+    y = ws.apply_network(ctx, net, x, 64, floor_frac=1e-3, train_n=32)
+    ctx.synchronize()
+    floor = 1e-3 * float(net.last_scale())
+    assert torch.equal(y, torch.clamp(x, min=floor))
+    with pytest.raises(ValueError):
+        ws.apply_network(ctx, net, x[: 60 ** 3], 60, train_n=32)
+    net.close()
+
+
 @pytest.mark.skipif(not os.path.exists(MODEL), reason="trained network not present")
 @pytest.mark.parametrize("seed,family", [(9002, "fibres"), (9003, "mixed")])
 def test_trained_network_gains_on_held_out_phantom(xr, ctx, ws, seed, family):

@@ -112,6 +112,8 @@ def _load():
         "xr_fdk_backproject": (I, [P, P, P, _c_int32_p, I, I, P]),
         "xr_fdk_taps": (I, [ctypes.POINTER(Geometry), I, _c_float_p]),
         "xr_volume_sqrt": (I, [P, P, ctypes.c_int64]),
+        "xr_volume_downsample": (I, [P, P, I, I, P]),
+        "xr_volume_upsample_add": (I, [P, P, P, I, I, P, ctypes.c_float]),
         "xr_pca": (I, [P, P, I, _c_float_p, ctypes.c_int64, ctypes.c_int64, P, P]),
         "xr_ir_reconstruct": (I, [P, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct),
                                   ctypes.POINTER(_c_float_p), ctypes.POINTER(_c_float_p), I, I, I, I, F,
@@ -306,6 +308,15 @@ class Context:
 
     def from_columns(self, col, ref, n_vox, channels=1):
         self.check(_lib.xr_volume_from_columns(self.h, _dptr(col), _dptr(ref), n_vox, channels))
+
+    def volume_downsample(self, fine, n_vox, factor, coarse):
+        """coarse = mean of each factor^3 block of fine (column layout)."""
+        self.check(_lib.xr_volume_downsample(self.h, _dptr(fine), n_vox, factor, _dptr(coarse)))
+
+    def volume_upsample_add(self, coarse_a, coarse_b, n_coarse, factor, fine, floor=-3.0e38):
+        """fine = max(fine + trilinear(coarse_a - coarse_b), floor) (column layout)."""
+        self.check(_lib.xr_volume_upsample_add(self.h, _dptr(coarse_a), _dptr(coarse_b), n_coarse, factor,
+                                               _dptr(fine), floor))
 
     def volume_sqrt(self, vol):
         self.check(_lib.xr_volume_sqrt(self.h, _dptr(vol), vol.numel()))

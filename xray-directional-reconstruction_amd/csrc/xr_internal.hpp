@@ -198,6 +198,8 @@ hipError_t launch_fdk_weight(float* weight, int U, int V, float sdd, float det, 
 hipError_t launch_to_internal(const float* ref, float* internal, int N, int channels, hipStream_t stream);
 hipError_t launch_from_internal(const float* internal, float* ref, int N, int channels, hipStream_t stream);
 hipError_t launch_sqrt_abs(float* vol, int64_t n, hipStream_t stream);
+hipError_t launch_vol_down(const float* fine, float* coarse, int Nc, int f, hipStream_t stream);
+hipError_t launch_vol_up_add(const float* a, const float* b, int Nc, int f, float* fine, float lo, hipStream_t stream);
 hipError_t launch_pca(const float* mu, int K, const Basis& b, int64_t nvox, int64_t chan_stride, float* md,
                       float* evalue, hipStream_t stream);
 hipError_t launch_fill(float* p, int64_t n, float value, hipStream_t stream);

@@ -2289,6 +2289,58 @@ __global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in,
     }
 }
 
+// Mean over f x f x f blocks of a column-layout volume of edge N = Nc * f
+// (one thread per coarse voxel, z fastest: a wave reads 64 f consecutive
+// floats of each fine column it visits).
+__global__ __launch_bounds__(256) void k_vol_down(const float* __restrict__ fine, float* __restrict__ coarse, int Nc,
+                                                  int f) {
+    const int64_t n = (int64_t)Nc * Nc * Nc;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int zc = (int)(i % Nc), xc = (int)((i / Nc) % Nc), yc = (int)(i / ((int64_t)Nc * Nc));
+    const int64_t N = (int64_t)Nc * f;
+    float s = 0.f;
+    for (int dy = 0; dy < f; dy++)
+        for (int dx = 0; dx < f; dx++) {
+            const float* col = fine + ((int64_t)(yc * f + dy) * N + (xc * f + dx)) * N + (int64_t)zc * f;
+            for (int dz = 0; dz < f; dz++) s += col[dz];
+        }
+    coarse[i] = s / (float)(f * f * f);
+}
+
+// fine = max(fine + up(a - b), lo): the difference of two coarse column-layout
+// volumes of edge Nc, interpolated trilinearly onto the fine grid of edge
+// Nc * f (fine index i sits at coarse coordinate (i + 0.5) / f - 0.5, clamped
+// to the coarse grid), added to the fine volume and floored at lo.
+__global__ __launch_bounds__(256) void k_vol_up_add(const float* __restrict__ a, const float* __restrict__ b, int Nc,
+                                                    int f, float* __restrict__ fine, float lo) {
+    const int64_t N = (int64_t)Nc * f;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N * N * N) return;
+    const int z = (int)(i % N), x = (int)((i / N) % N), y = (int)(i / (N * N));
+    auto axis = [&](int j, int& j0, int& j1, float& w) {
+        const float c = fmaxf(((float)j + 0.5f) / (float)f - 0.5f, 0.f);
+        j0 = min((int)c, Nc - 1);
+        j1 = min(j0 + 1, Nc - 1);
+        w = c - (float)j0;
+        if (j0 == Nc - 1) w = 0.f;
+    };
+    int z0, z1, x0, x1, y0, y1;
+    float wz, wx, wy;
+    axis(z, z0, z1, wz);
+    axis(x, x0, x1, wx);
+    axis(y, y0, y1, wy);
+    auto at = [&](int yy, int xx, int zz) {
+        const int64_t k = ((int64_t)yy * Nc + xx) * Nc + zz;
+        return a[k] - b[k];
+    };
+    auto lerp = [](float u, float v, float w) { return fmaf(w, v - u, u); };
+    const float c00 = lerp(at(y0, x0, z0), at(y0, x0, z1), wz), c01 = lerp(at(y0, x1, z0), at(y0, x1, z1), wz);
+    const float c10 = lerp(at(y1, x0, z0), at(y1, x0, z1), wz), c11 = lerp(at(y1, x1, z0), at(y1, x1, z1), wz);
+    const float d = lerp(lerp(c00, c01, wx), lerp(c10, c11, wx), wy);
+    fine[i] = fmaxf(fine[i] + d, lo);
+}
+
 __global__ void k_sqrt_abs(float* v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -2841,6 +2893,19 @@ hipError_t launch_to_internal(const float* ref, float* internal, int N, int chan
 hipError_t launch_from_internal(const float* internal, float* ref, int N, int channels, hipStream_t stream) {
     hipLaunchKernelGGL((k_transpose<false>), dim3(grid1(N, 32), grid1(N, 32), (unsigned)N * channels), dim3(32, 8), 0,
                        stream, internal, ref, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_vol_down(const float* fine, float* coarse, int Nc, int f, hipStream_t stream) {
+    const int64_t n = (int64_t)Nc * Nc * Nc;
+    hipLaunchKernelGGL(k_vol_down, dim3(grid1(n, 256)), dim3(256), 0, stream, fine, coarse, Nc, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_vol_up_add(const float* a, const float* b, int Nc, int f, float* fine, float lo,
+                             hipStream_t stream) {
+    const int64_t N = (int64_t)Nc * f;
+    hipLaunchKernelGGL(k_vol_up_add, dim3(grid1(N * N * N, 256)), dim3(256), 0, stream, a, b, Nc, f, fine, lo);
     return hipGetLastError();
 }
 

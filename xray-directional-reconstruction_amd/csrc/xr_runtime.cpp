@@ -764,6 +764,21 @@ int xr_volume_from_columns(xr_context* ctx, const float* col_dev, float* ref_dev
     return timed(ctx, 3, [&] { return launch_from_internal(col_dev, ref_dev, n_vox, channels, ctx->stream); });
 }
 
+int xr_volume_downsample(xr_context* ctx, const float* fine, int n_vox, int factor, float* coarse) {
+    if (!ctx || !fine || !coarse || factor <= 0 || n_vox <= 0 || n_vox % factor != 0)
+        return fail(ctx, XR_ERR_INVALID, "downsample: n_vox %d must be a positive multiple of factor %d", n_vox, factor);
+    return timed(ctx, 3, [&] { return launch_vol_down(fine, coarse, n_vox / factor, factor, ctx->stream); });
+}
+
+int xr_volume_upsample_add(xr_context* ctx, const float* coarse_a, const float* coarse_b, int n_coarse, int factor,
+                           float* fine, float floor_value) {
+    if (!ctx || !coarse_a || !coarse_b || !fine || factor <= 0 || n_coarse <= 0 ||
+        (int64_t)n_coarse * factor > 32768)
+        return fail(ctx, XR_ERR_INVALID, "upsample_add: bad sizes (n_coarse %d, factor %d)", n_coarse, factor);
+    return timed(ctx, 3,
+                 [&] { return launch_vol_up_add(coarse_a, coarse_b, n_coarse, factor, fine, floor_value, ctx->stream); });
+}
+
 static int upload_views(xr_plan* p, const int32_t* views, int n_views, const uint8_t* accumulate) {
     xr_context* ctx = p->ctx;
     const int total = p->n_cond * p->g.n_proj;

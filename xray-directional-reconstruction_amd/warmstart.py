@@ -22,7 +22,8 @@ import numpy as np
 
 from . import BATCH_GRAPH, METHOD_MLEM, OUT_RATIO, Scan, UNet, XRError, make_geometry  # noqa: F401
 
-__all__ = ["Scene", "make_phantom", "phantom_spec", "generate_pairs", "train", "warm_start", "plain_mlem", "positivity_floor"]
+__all__ = ["Scene", "make_phantom", "phantom_spec", "generate_pairs", "train", "warm_start", "plain_mlem", "positivity_floor",
+           "apply_network", "trained_edge", "plain_curve", "warm_curve", "first_within"]
 
 
 class Scene:
@@ -254,6 +255,54 @@ def positivity_floor(vol_cols, floor):
     return vol_cols
 
 
+def trained_edge(model_path):
+    """The volume edge a saved network was trained at (its training summary,
+    model_path + ".json", written by train_warmstart.py), or None."""
+    import json
+    import os
+
+    meta = model_path + ".json"
+    if not os.path.exists(meta):
+        return None
+    with open(meta) as f:
+        d = json.load(f)
+    if "n_vox" in d:
+        return int(d["n_vox"])
+    stages = d.get("stages") or []
+    return int(stages[-1]["n_vox"]) if stages else None
+
+
+def apply_network(ctx, net, x, N, floor_frac=1e-3, train_n=None):
+    """The network's estimate from the column-layout volume x (edge N), floored
+    at floor_frac x the mean of its input.
+
+    MLEM recovers an object's spatial frequencies in an order set by the
+    field of view, not by the voxel size (the backprojection of a projection
+    is a 1/r blur over the whole field), so the early iterate of an N^3 solve
+    is, in field-of-view units, the early iterate of a coarser solve.  A
+    network trained at edge train_n < N (N a multiple of it) is therefore
+    applied at that scale: x is averaged over f^3 blocks (f = N / train_n),
+    the network's correction y - x is computed there and added to x after
+    trilinear interpolation back to N^3.  train_n None or N: applied as is."""
+    import torch
+
+    y = torch.empty_like(x)
+    if train_n and N != train_n:
+        if N % train_n:
+            raise ValueError("volume edge %d is not a multiple of the network's %d" % (N, train_n))
+        f = N // train_n
+        xd = torch.empty(train_n ** 3, dtype=torch.float32, device=x.device)
+        ctx.volume_downsample(x, N, f, xd)
+        yd = torch.empty_like(xd)
+        net.forward(xd, yd, (train_n, train_n, train_n))
+        y.copy_(x)
+        ctx.volume_upsample_add(yd, xd, train_n, f, y, floor_frac * float(net.last_scale()))
+        return y
+    net.forward(x, y, (N, N, N))
+    positivity_floor(y, floor_frac * float(net.last_scale()))
+    return y
+
+
 def plain_mlem(scene, tau, max_iters, init=0.01):
     """Plain MLEM from a constant volume (main.cu:40-42) until r <= tau."""
     import torch
@@ -274,7 +323,7 @@ def _resume(scene, tau, max_iters, iters, ms, hist):
 
 
 def warm_start(scene, net, tau, max_iters, pre=5, init=0.01, floor_frac=1e-3, use_net=True, check_every=4,
-               accept_tol=0.1):
+               accept_tol=0.1, train_n=None):
     """Plain MLEM for `pre` iterations, then the network (checked by the data
     residual, with fallback), then MLEM until r <= tau.  Returns iterations
     (MLEM iterations only), GPU milliseconds (everything: MLEM, network,
@@ -301,12 +350,10 @@ def warm_start(scene, net, tau, max_iters, pre=5, init=0.01, floor_frac=1e-3, us
     if use_net:
         x = torch.empty(N3, dtype=torch.float32, device=dev)
         scene.get_volume(x)
-        y = torch.empty_like(x)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         scene.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
         e0.record()
-        net.forward(x, y, (scene.N, scene.N, scene.N))
-        positivity_floor(y, floor_frac * float(net.last_scale()))
+        y = apply_network(scene.ctx, net, x, scene.N, floor_frac, train_n)
         r_x = scene.residual(x, check_every)
         r_y = scene.residual(y, check_every)
         e1.record()
@@ -356,8 +403,8 @@ def first_within(R, R0, Rinf, tau):
     return None
 
 
-def warm_curve(scene, net, R0, Rinf, tau, max_epochs, pre=2, init=0.01, floor_frac=1e-3, check_every=4,
-               accept_tol=0.1):
+def warm_curve(scene, net, R0, Rinf, tau, max_epochs, pre=5, init=0.01, floor_frac=1e-3, check_every=4,
+               accept_tol=0.1, train_n=None):
     """Warm start against the frozen criterion: `pre` plain epochs from
     x0 = init, the network (its output clamped to a positivity floor), the
     acceptance check (the network's volume is kept unless its data residual
@@ -373,12 +420,10 @@ def warm_curve(scene, net, R0, Rinf, tau, max_epochs, pre=2, init=0.01, floor_fr
     _, ms = scene.iterate(pre) if pre > 0 else (None, 0.0)
     x = torch.empty(N3, dtype=torch.float32, device=dev)
     scene.get_volume(x)
-    y = torch.empty_like(x)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     scene.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     e0.record()
-    net.forward(x, y, (scene.N, scene.N, scene.N))
-    positivity_floor(y, floor_frac * float(net.last_scale()))
+    y = apply_network(scene.ctx, net, x, scene.N, floor_frac, train_n)
     r_x = scene.residual(x, check_every)
     r_y = scene.residual(y, check_every)
     e1.record()
