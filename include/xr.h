@@ -53,9 +53,13 @@ extern "C" {
 #define XR_METHOD_FDK 3
 
 /* FDK filters: the reference's kernel 1/(pi^2 d (1-4m^2)) (reconstruct.cu:1694-1699)
- * or Ram-Lak in the same scaling convention. */
+ * or Ram-Lak in the same scaling convention, applied to cosine-weighted rows;
+ * or the Hilbert filter of FDK::hilbertReconstruct (fdk.cu:298-375: R2C FFT of
+ * each raw row, every half-spectrum bin times -i/(U/2+1), unnormalised C2R),
+ * a circular convolution with U taps. */
 #define XR_FILTER_REFERENCE 0
 #define XR_FILTER_RAMLAK 1
+#define XR_FILTER_HILBERT 2
 
 /* Forward-projection output modes (xr_project). */
 #define XR_OUT_RAW 0         /* sum_c c*ratio*mu, un-normalised splat (ir.cu:1518-1525) */
@@ -228,7 +232,8 @@ int xr_fdk_filter(xr_context* ctx, xr_plan* plan, int filter, const float* sino,
                   int n_views);
 int xr_fdk_backproject(xr_context* ctx, xr_plan* plan, const float* filt, const int32_t* views, int n_views,
                        int overwrite, float* vol);
-/* The filter taps the FDK path uses (n_det_u floats, host out). */
+/* The filter taps the FDK path uses (n_det_u floats, host out): h[|m|] of the
+ * symmetric filters, h[m mod U] of the circular Hilbert filter. */
 int xr_fdk_taps(const xr_geometry* g, int filter, float* taps_out);
 
 /* Multi-resolution helpers of the learned warm start (no counterpart in the

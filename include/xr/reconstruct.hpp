@@ -334,6 +334,25 @@ inline void reconstruct(Volume<float>* sinogram, Volume<float>* voxel, const Geo
                                         dir == Rotate::CW ? XR_CW : XR_CCW, XR_FILTER_REFERENCE),
                      "FDK::reconstruct");
 }
+// FDK::hilbertReconstruct (reconstruct.cu:1726-1793): every condition's raw
+// rows through the Hilbert filter of fdk.cu:298-375 (XR_FILTER_HILBERT), then
+// the FDK weighted backprojection.  The filtered rows are returned through the
+// sinograms.  The reference filters only condition 0 (the others are
+// backprojected unfiltered) and copies into a device buffer it never
+// allocates (reconstruct.cu:1749); here every condition is filtered.
+inline void hilbertReconstruct(Volume<float>* sinogram, Volume<float>* voxel, const Geometry& geom, Rotate dir) {
+    const int C = NUM_PROJ_COND;
+    auto st = xr_detail::scan_tables(C, 0);
+    xr_geometry g = xr_detail::geometry(geom, sinogram[0].x(), sinogram[0].y(), sinogram[0].z(), voxel[0].x());
+    std::vector<float*> s(C), v(C);
+    for (int c = 0; c < C; c++) {
+        s[c] = sinogram[c].get();
+        v[c] = voxel[c].get();
+    }
+    xr_detail::check(xr_fdk_reconstruct(xr_detail::context(), &g, &st.scan, s.data(), v.data(),
+                                        dir == Rotate::CW ? XR_CW : XR_CCW, XR_FILTER_HILBERT),
+                     "FDK::hilbertReconstruct");
+}
 }  // namespace FDK
 
 // ---- direction utilities (direction/include/pca.cuh:9-12) ------------------

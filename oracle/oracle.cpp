@@ -30,6 +30,8 @@
 //   reconstruct/src/reconstruct.cu:23-156     IR::reconstruct             (orc_ir_reconstruct)
 //   reconstruct/src/reconstruct.cu:799-954    XTT::reconstruct            (orc_xtt_reconstruct)
 //   reconstruct/src/reconstruct.cu:1651-1724  FDK::reconstruct            (orc_fdk_reconstruct)
+//   reconstruct/src/reconstruct.cu:1726-1793, fdk.cu:298-375
+//                                             FDK::hilbertReconstruct     (orc_fdk_reconstruct, kind 2)
 //   reconstruct/include/geometry.h:11-14      voxSize = sod*detSize/sdd
 //   reconstruct/include/vec.h:93-109          normalize (eps 1e-8) / norm2
 //   direction/src/pca.cu:12-65                calcEigenVector             (orc_pca)
@@ -552,6 +554,51 @@ template <class T> void fdk_taps(const orc_geometry& g, int kind, T* h) {
     }
 }
 
+// cuFFTtoProjection + hilbertFiltering (fdk.cu:298-375) for one view: every
+// row x of U samples goes through a forward real-to-complex DFT
+// X[k] = sum_j x[j] e^{-2 pi i j k / U} (k = 0 .. U/2), each bin times
+// -i / (U/2 + 1), and the unnormalised complex-to-real inverse
+// y[n] = Re Y[0] + 2 sum_{0<k<U/2} Re(Y[k] e^{2 pi i k n / U}) + Re Y[U/2] (-1)^n
+// (the last term for even U only), which is what cuFFT's C2R computes from a
+// half spectrum.  Written as explicit DFT sums (no FFT library); the rows are
+// not cosine weighted (hilbertReconstruct leaves calcWeight commented out,
+// reconstruct.cu:1763).
+template <class T> void hilbert_filter_view(const orc_geometry& g, const float* src, T* dst) {
+    const int U = g.n_det_u, V = g.n_det_v, K = U / 2;
+    const T c = (T)(-1.0) / (T)(U / 2 + 1);
+    std::vector<double> cs((size_t)U), sn((size_t)U);
+    for (int m = 0; m < U; m++) {
+        cs[m] = std::cos(2.0 * M_PI * (double)m / (double)U);
+        sn[m] = std::sin(2.0 * M_PI * (double)m / (double)U);
+    }
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < V; v++) {
+        const float* x = src + (int64_t)U * v;
+        std::vector<T> re((size_t)K + 1), im((size_t)K + 1);
+        for (int k = 0; k <= K; k++) {
+            T a = 0, b = 0;
+            for (int j = 0; j < U; j++) {
+                const int m = (int)(((int64_t)j * k) % U);
+                a += (T)x[j] * (T)cs[m];
+                b -= (T)x[j] * (T)sn[m];
+            }
+            // (a + i b) * (i c) = -c b + i c a
+            re[k] = -c * b;
+            im[k] = c * a;
+        }
+        for (int n = 0; n < U; n++) {
+            T y = re[0];
+            const int top = (U % 2 == 0) ? K - 1 : K;
+            for (int k = 1; k <= top; k++) {
+                const int m = (int)(((int64_t)k * n) % U);
+                y += (T)2 * (re[k] * (T)cs[m] - im[k] * (T)sn[m]);
+            }
+            if (U % 2 == 0) y += (n % 2 == 0) ? re[K] : -re[K];
+            dst[n + (int64_t)U * v] = y;
+        }
+    }
+}
+
 // projConv (fdk.cu:28-43) for one view: dst[u,v] = sum_j h[|u-j|] src[j,v] w[j,v].
 template <class T> void fdk_filter_view(const orc_geometry& g, const T* h, const T* w, const float* src, T* dst) {
     const int U = g.n_det_u, V = g.n_det_v;
@@ -777,10 +824,21 @@ void orc_fdk_taps(int prec, const orc_geometry* g, int kind, double* out) {
     }
 }
 
-// a16-a18 for one view: weighted, filtered detector image (U*V doubles).
+// a16-a18 for one view: weighted, filtered detector image (U*V doubles);
+// kind 2: the Hilbert filter of hilbertReconstruct (unweighted rows).
 void orc_fdk_filter_view(int prec, const orc_geometry* g, int kind, const float* src_view, double* out) {
     const int U = g->n_det_u;
     const int64_t n = (int64_t)U * g->n_det_v;
+    if (kind == 2) {
+        if (prec == 0) {
+            std::vector<float> d(n);
+            hilbert_filter_view<float>(*g, src_view, d.data());
+            for (int64_t i = 0; i < n; i++) out[i] = d[i];
+        } else {
+            hilbert_filter_view<double>(*g, src_view, out);
+        }
+        return;
+    }
     if (prec == 0) {
         std::vector<float> h(U), w(n), d(n);
         fdk_taps<float>(*g, kind, h.data());

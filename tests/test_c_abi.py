@@ -43,6 +43,20 @@ def test_filter_taps_match_oracle(xr, orc):
     assert np.allclose(xr.fdk_taps(g, 1), case.fdk_taps(1), rtol=1e-6, atol=0)
 
 
+@pytest.mark.parametrize("U", [96, 95])
+def test_hilbert_taps_reproduce_the_oracle_filter(xr, orc, U):
+    # the library's closed-form circular taps (a sine sum) applied as a circular
+    # convolution equal the oracle's DFT restatement of the FFT chain
+    g = xr.make_geometry(1000.0, 500.0, 0.4, 16, U, 8)
+    h = xr.fdk_taps(g, xr.FILTER_HILBERT).astype(np.float64)
+    case = orc.Case(1000.0, 500.0, 0.4, 16, U, 8, V=3)
+    rows = np.random.default_rng(7).standard_normal((3, U)).astype(np.float32)
+    want = case.fdk_filter_view(rows, kind=2).reshape(3, U)
+    m = (np.arange(U)[:, None] - np.arange(U)[None, :]) % U
+    got = rows.astype(np.float64) @ h[m].T
+    assert np.allclose(got, want, rtol=0, atol=1e-6 * np.abs(want).max())
+
+
 def test_settings_sorting_and_area_translation(xr):
     st, g, scan = xr.load_settings(os.path.join(GOLDEN, "settings_fixture.json"), "bench_tilted")
     assert (st.n_cond, st.n_basis, st.n_vox, st.n_proj, st.n_det_u, st.n_det_v) == (2, 3, 32, 60, 64, 48)

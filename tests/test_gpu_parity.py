@@ -328,8 +328,9 @@ def test_xtt_solver_matches_oracle(orc, xr, ctx):
     assert rel_l2(np.stack(md)[:, valid], omd[:, valid]) <= 1e-4
 
 
-@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 2])
 def test_fdk_matches_oracle(orc, xr, ctx, torch_dev, kind):
+    # kind 2: FDK::hilbertReconstruct (Hilbert-filtered raw rows, FDK backprojection)
     st = Setup(orc, xr, N=40, U=64, P=48, vs=0.5)
     truth, _ = ellipsoid_phantom(orc, xr, st.N, st.vs, seed=50)
     sino = np.stack([st.case.forward_view(truth, n)[0] for n in range(st.P)]).astype(np.float32)

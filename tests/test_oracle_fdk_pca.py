@@ -149,3 +149,42 @@ def test_fdk_distance_weight_closed_form(orc):
         flipped = c * (sod / (sod - yv)) ** 2 * 2 * np.pi / P
         assert np.abs(vol - flipped).max() > 1e-3 * np.abs(want).max()
         assert np.abs(vol - c * 2 * np.pi / P).max() > 1e-3 * np.abs(want).max()
+
+
+def hilbert_numpy(rows):
+    """The reference's Hilbert chain with numpy's FFT: rfft, every half-spectrum
+    bin times -i/(U/2+1), irfft (which drops the imaginary parts of the DC and
+    Nyquist bins as cuFFT's C2R does) times U (cuFFT's inverse is unnormalised)."""
+    U = rows.shape[-1]
+    X = np.fft.rfft(rows.astype(np.float64), axis=-1)
+    return np.fft.irfft(X * (-1j / (U // 2 + 1)), n=U, axis=-1) * U
+
+
+@pytest.mark.parametrize("U", [64, 63, 96])
+def test_hilbert_filter_matches_numpy_fft(orc, U):
+    # the oracle's explicit-DFT restatement (fdk.cu:298-375) against numpy's FFT
+    V = 5
+    case = orc.Case(600.0, 300.0, 0.8, 8, U, 4, V=V)
+    rng = np.random.default_rng(U)
+    view = rng.standard_normal((V, U)).astype(np.float32)
+    got = case.fdk_filter_view(view, kind=2).reshape(V, U)
+    want = hilbert_numpy(view)
+    assert rel_l2(got, want) <= 1e-12
+    got32 = case.fdk_filter_view(view, kind=2, prec=0).reshape(V, U)
+    assert rel_l2(got32, want) <= 1e-5
+
+
+def test_hilbert_filter_closed_forms(orc):
+    # the Hilbert transform of a discrete cosine is the sine (times U/(U/2+1) here);
+    # DC and the Nyquist row go to zero
+    U, V = 64, 4
+    case = orc.Case(600.0, 300.0, 0.8, 8, U, 4, V=V)
+    u = np.arange(U)
+    rows = np.stack([np.cos(2 * np.pi * 3 * u / U), np.cos(2 * np.pi * 11 * u / U), np.ones(U),
+                     np.cos(np.pi * u)]).astype(np.float32)
+    got = case.fdk_filter_view(rows, kind=2).reshape(V, U)
+    s = U / (U // 2 + 1)
+    assert np.allclose(got[0], s * np.sin(2 * np.pi * 3 * u / U), atol=1e-6)
+    assert np.allclose(got[1], s * np.sin(2 * np.pi * 11 * u / U), atol=1e-6)
+    assert np.abs(got[2]).max() < 1e-6 and np.abs(got[3]).max() < 1e-5
+    # a sign slip (+i) or the 1/U normalisation would fail the first two rows

@@ -20,7 +20,7 @@ XR_OK = 0
 CONV_IR, CONV_FDK = 0, 1
 CW, CCW = 0, 1
 METHOD_XTT, METHOD_MLEM, METHOD_ART, METHOD_FDK = 0, 1, 2, 3
-FILTER_REFERENCE, FILTER_RAMLAK = 0, 1
+FILTER_REFERENCE, FILTER_RAMLAK, FILTER_HILBERT = 0, 1, 2
 OUT_RAW, OUT_NORMALISED, OUT_RATIO, OUT_RESIDUAL = 0, 1, 2, 3
 TIMING_FORWARD, TIMING_BACKWARD, TIMING_FILTER, TIMING_OTHER = 0, 1, 2, 3
 BATCH_GRAPH, BATCH_AUTOTUNE = 1, 2

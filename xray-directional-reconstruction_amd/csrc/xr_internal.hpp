@@ -193,7 +193,7 @@ int absmax_blocks(int64_t max_len, int n_seg);
 hipError_t launch_absmax_seg(const float* const* segs, const int64_t* seg_len, float* out, int n_seg,
                              int blocks_per_seg, hipStream_t stream);
 hipError_t launch_fdk_filter(const float* src, float* dst, const float* weight, const float* taps, int U, int V,
-                             const int32_t* views, int n_views, int code_base, hipStream_t stream);
+                             const int32_t* views, int n_views, int code_base, int circular, hipStream_t stream);
 hipError_t launch_fdk_weight(float* weight, int U, int V, float sdd, float det, hipStream_t stream);
 hipError_t launch_to_internal(const float* ref, float* internal, int N, int channels, hipStream_t stream);
 hipError_t launch_from_internal(const float* internal, float* ref, int N, int channels, hipStream_t stream);

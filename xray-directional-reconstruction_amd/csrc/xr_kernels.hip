@@ -2216,17 +2216,20 @@ template <int R>
 __global__ __launch_bounds__(1024) void k_fdk_filter(const float* __restrict__ src, float* __restrict__ dst,
                                                     const float* __restrict__ weight, const float* __restrict__ taps,
                                                     int U, int V, int Upad, const int32_t* views,
-                                                    int code_base) {
+                                                    int code_base, int circular) {
     extern __shared__ float sm[];
-    float* s = sm;         // Upad weighted samples (zero padded)
-    float* hx = sm + Upad;  // 2*Upad taps: hx[m + Upad] = h[|m|]
+    float* s = sm;         // Upad (weighted) samples, zero padded
+    float* hx = sm + Upad;  // 2*Upad taps: hx[m + Upad] = h(m), m = u - j
     const int v = blockIdx.x;
     const int code = views[blockIdx.y];
     const int64_t row = ((int64_t)(code - code_base) * V + v) * U;
-    for (int j = threadIdx.x; j < Upad; j += blockDim.x) s[j] = (j < U) ? src[row + j] * weight[(int64_t)v * U + j] : 0.f;
+    for (int j = threadIdx.x; j < Upad; j += blockDim.x)
+        s[j] = (j < U) ? (weight ? src[row + j] * weight[(int64_t)v * U + j] : src[row + j]) : 0.f;
+    // symmetric taps h(m) = taps[|m|] (a linear convolution), or circular taps
+    // h(m) = taps[m mod U] (a circular convolution over the row)
     for (int m = threadIdx.x; m < 2 * Upad; m += blockDim.x) {
-        const int am = abs(m - Upad);
-        hx[m] = (am < U) ? taps[am] : 0.f;
+        const int dm = m - Upad, am = abs(dm);
+        hx[m] = (am < U) ? taps[circular ? (dm + U) % U : am] : 0.f;
     }
     __syncthreads();
     const int u0 = threadIdx.x * R;
@@ -2868,14 +2871,14 @@ hipError_t launch_absmax_seg(const float* const* segs, const int64_t* seg_len, f
 }
 
 hipError_t launch_fdk_filter(const float* src, float* dst, const float* weight, const float* taps, int U, int V,
-                             const int32_t* views, int n_views, int code_base, hipStream_t stream) {
+                             const int32_t* views, int n_views, int code_base, int circular, hipStream_t stream) {
     constexpr int R = 8;
     const int Upad = ((U + R - 1) / R) * R;
     int threads = ((Upad / R + 63) / 64) * 64;
     if (threads > 1024 || threads < 64) return hipErrorInvalidValue;
     const size_t lds = (size_t)3 * Upad * sizeof(float);
     hipLaunchKernelGGL((k_fdk_filter<R>), dim3(V, n_views), dim3(threads), lds, stream, src, dst, weight, taps, U, V,
-                       Upad, views, code_base);
+                       Upad, views, code_base, circular);
     return hipGetLastError();
 }
 

@@ -924,6 +924,18 @@ int xr_fdk_taps(const xr_geometry* g, int filter, float* taps) {
                 taps[m] = (float)(-1.0 / (2.0 * M_PI * M_PI * (double)m * (double)m * (double)d));
             else
                 taps[m] = 0.0f;
+        } else if (filter == XR_FILTER_HILBERT) {
+            // cuFFTtoProjection + hilbertFiltering (fdk.cu:298-375): a forward
+            // real-to-complex FFT of the row, every half-spectrum bin times
+            // -i/(U/2+1), an unnormalised complex-to-real inverse.  The inverse
+            // drops the imaginary parts of the DC and (even U) Nyquist bins, so the
+            // row is circularly convolved with h(m) = sum_{k=1}^{(U-1)/2}
+            // 2 sin(2 pi k m / U) / (U/2+1), evaluated here in FP64.
+            const int U = g->n_det_u;
+            const double scale = 2.0 / (double)(U / 2 + 1);
+            double h = 0.0;
+            for (int k = 1; k <= (U - 1) / 2; k++) h += std::sin(2.0 * M_PI * (double)((int64_t)k * m % U) / (double)U);
+            taps[m] = (float)(scale * h);
         } else {
             return XR_ERR_INVALID;
         }
@@ -943,10 +955,15 @@ static int fdk_filter(xr_context* ctx, xr_plan* plan, int filter, const float* s
     XR_HIP(ctx, dw.alloc((size_t)g.n_det_u * g.n_det_v));
     XR_HIP(ctx, dt.alloc(taps.size()));
     XR_HIP(ctx, hipMemcpyAsync(dt.p, taps.data(), taps.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    XR_TRY(timed(ctx, 3, [&] { return launch_fdk_weight(dw.p, g.n_det_u, g.n_det_v, g.sdd, g.det_size, ctx->stream); }));
+    // the Hilbert path filters the raw rows (its calcWeight is commented out,
+    // reconstruct.cu:1763); the FDK filters take the cosine-weighted rows
+    const bool hilbert = filter == XR_FILTER_HILBERT;
+    if (!hilbert)
+        XR_TRY(timed(ctx, 3,
+                     [&] { return launch_fdk_weight(dw.p, g.n_det_u, g.n_det_v, g.sdd, g.det_size, ctx->stream); }));
     XR_TRY(timed(ctx, 2, [&] {
-        return launch_fdk_filter(sino, filt, dw.p, dt.p, g.n_det_u, g.n_det_v, ctx->d_codes, n_views, code_base,
-                                 ctx->stream);
+        return launch_fdk_filter(sino, filt, hilbert ? nullptr : dw.p, dt.p, g.n_det_u, g.n_det_v, ctx->d_codes,
+                                 n_views, code_base, hilbert ? 1 : 0, ctx->stream);
     }));
     XR_HIP(ctx, hipStreamSynchronize(ctx->stream));  // dw / dt are freed on return
     return XR_OK;
