@@ -236,6 +236,52 @@ int xr_fdk_backproject(xr_context* ctx, xr_plan* plan, const float* filt, const 
  * symmetric filters, h[m mod U] of the circular Hilbert filter. */
 int xr_fdk_taps(const xr_geometry* g, int filter, float* taps_out);
 
+/* Orientation-aware XTT (forwardOrth / backwardOrth, ir.cu:295-423): with an
+ * orientation field set, the plan's XTT projections (xr_project / xr_residual
+ * with xtt, xr_backproject, xr_backproject_update) rotate the basis per voxel
+ * by rodriguesRotation (ir.cu:597-621) about (cos phi, sin phi, 0) with
+ * cos theta: coef holds phi then cos theta, each N^3 floats in the column
+ * layout (device, read at every launch; the caller keeps it alive).  NULL
+ * returns to the plan's fixed basis. */
+int xr_plan_set_orientation(xr_plan* plan, const float* coef);
+
+/* Orientation helpers of the orientation-aware XTT loop (orthTwiceReconstruct,
+ * reconstruct.cu:956-1239), per voxel on the device, channels nvox floats
+ * apart (any voxel order; the mean filter takes the column layout):
+ * vol 3 channels (the three-direction coefficients), coef 2 channels (phi,
+ * cos theta), est 2 channels (best cosine, iteration), md 3 channels;
+ * basis9: the three basis vectors S0, S1, S2 (host, 9 floats).
+ * xr_orth_normal: calcNormalVectorThreeDirec (ir.cu:856-974, sign choice rot
+ * 0-3) or, with est, calcNormalVectorThreeDirecWithEst (ir.cu:976-1071);
+ * updates coef in place, loss = |R e_z x normal| per voxel.
+ * xr_orth_update_estimate: updateEstimationByCoef (ir.cu:1073-1118).
+ * xr_orth_md: convertNormVector (ir.cu:1448-1480), md = mean(vol) R e_z.
+ * xr_orth_mean_filter: meanFiltFiber (ir.cu:1231-1303) over the interior
+ * voxels of an n_vox^3 volume; boundary voxels copy the source. */
+int xr_orth_normal(xr_context* ctx, const float* basis9, const float* vol, int64_t nvox, float* coef, float* loss,
+                   int rot, const float* est);
+int xr_orth_update_estimate(xr_context* ctx, const float* basis9, const float* vol, int64_t nvox, float* est,
+                            int iter);
+int xr_orth_md(xr_context* ctx, const float* vol, const float* coef, int64_t nvox, float* md);
+int xr_orth_mean_filter(xr_context* ctx, const float* coef_src, const float* vol, int n_vox, float weight,
+                        float* coef_dst);
+
+/* XTT::orthTwiceReconstruct (reconstruct.cu:956-1239): the orientation-aware
+ * XTT loop over the three-direction model (n_basis must be 3).  sino[c]:
+ * host sinograms; vol[3]: host volumes (reference layout), initial guess in,
+ * the kept volume out; md[3]: host main directions out.  iter1 outer
+ * iterations of five passes (iter2 epochs, then four of iter2/4), batch
+ * subsets, method XR_METHOD_MLEM / XR_METHOD_ART.  seed: the subset-order
+ * engine (std::mt19937; the reference seeds it from std::random_device).
+ * volume_path / direction_path (either may be NULL): per outer iteration the
+ * kept volume and the main directions are written as raw float32 files
+ * path + "_sequence<outer>d<i>_NxNxN.raw" (reconstruct.cu:1203-1216).
+ * The recorded-estimate buffer starts at zero (the reference leaves it
+ * uninitialised). */
+int xr_orth_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, float* const* sino, float* const* vol,
+                        float* const* md, int iter1, int iter2, int batch, int direction, int method, float lambda,
+                        uint32_t seed, const char* volume_path, const char* direction_path);
+
 /* Multi-resolution helpers of the learned warm start (no counterpart in the
  * reference), single-channel column-layout volumes on the device.
  * xr_volume_downsample: coarse (edge n_vox/factor) = mean of each

@@ -24,6 +24,12 @@
 //   reconstruct/src/ir.cu:1707-1717   voxelSqrt                           (orc_xtt_reconstruct)
 //   reconstruct/src/ir.cu:1720-1751   backwardonDevice                    (orc_backward_view)
 //   reconstruct/src/ir.cu:1753-1850   forward/backwardXTTonDevice         (orc_forward_view, orc_backward_view)
+//   reconstruct/src/ir.cu:295-423     forwardOrth / backwardOrth          (orc_forward_view_orth, orc_backward_view_orth)
+//   reconstruct/src/ir.cu:597-621     rodriguesRotation                   (orc_rodrigues)
+//   reconstruct/src/ir.cu:856-1118    calcNormalVectorThreeDirec(WithEst), updateEstimationByCoef
+//                                                                         (orc_orth_normal, orc_orth_update_estimate)
+//   reconstruct/src/ir.cu:1231-1303   meanFiltFiber                       (orc_orth_mean_filter)
+//   reconstruct/src/ir.cu:1448-1480   convertNormVector                   (orc_orth_md)
 //   reconstruct/src/ir.cu:1852-1902   rayCasting                          (raycast)
 //   reconstruct/src/fdk.cu:13-43      calcWeight / projConv               (orc_fdk_*)
 //   reconstruct/src/fdk.cu:62-170     FDK backwardonDevice                (orc_fdk_backproject_view)
@@ -225,6 +231,30 @@ template <class T> inline V3<T> basis_vec(const orc_scan& s, int k) {
     return {(T)s.basis[3 * k + 0], (T)s.basis[3 * k + 1], (T)s.basis[3 * k + 2]};
 }
 
+// rodriguesRotation (ir.cu:597-621): rotation about the axis (x, y, z) with the
+// given cosine and sine, (1 - cos) n n^T + [[cos, -n_z sin, n_y sin],
+// [n_z sin, cos, -n_x sin], [-n_y sin, n_x sin, cos]]; identity for a zero axis.
+template <class T> void rodrigues(T x, T y, T z, T c, T sn, T R[9]) {
+    const T len = std::sqrt(x * x + y * y + z * z);
+    if (len < (T)1e-8f) {
+        const T I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        std::memcpy(R, I, sizeof(I));
+        return;
+    }
+    const T nx = x / len, ny = y / len, nz = z / len;
+    const T r1[9] = {nx * nx, nx * ny, nx * nz, nx * ny, ny * ny, ny * nz, nx * nz, ny * nz, nz * nz};
+    const T r2[9] = {c, -nz * sn, ny * sn, nz * sn, c, -nx * sn, -ny * sn, nx * sn, c};
+    for (int i = 0; i < 9; i++) R[i] = ((T)1.0f - c) * r1[i] + r2[i];
+}
+
+// The per-voxel rotation of forwardOrth / backwardOrth (ir.cu:316-328, 390-400):
+// coefficient channel 0 holds phi, channel 1 cos(theta); the axis is
+// (cos phi, sin phi, 0) and the angle's sine sqrt(1 - cos^2).
+template <class T> void orth_rotation(const float* coef, int64_t idx, int64_t lenV, T R[9]) {
+    const T phi = (T)coef[idx], cc = (T)coef[idx + lenV];
+    rodrigues<T>(std::cos(phi), std::sin(phi), (T)0, cc, std::sqrt((T)1.0f - cc * cc), R);
+}
+
 int n_threads() {
 #ifdef _OPENMP
     return omp_get_max_threads();
@@ -245,7 +275,7 @@ int thread_id() {
 // vkm^2*vs*ratio*mu_k (XTT); factor += c_k.  Accumulates into proj/factor (T).
 template <class T, class VT>
 void forward_raw(const orc_geometry& g, const orc_scan& s, int cond, int n, const VT* vol, int xtt, T* proj,
-                 T* factor) {
+                 T* factor, const float* coef = nullptr) {
     const int N = g.n_vox, U = g.n_det_u, V = g.n_det_v;
     const int64_t lenV = (int64_t)N * N * N;
     const Frame<T> f = make_frame<T>(g, s, cond, n, 0);
@@ -269,9 +299,13 @@ void forward_raw(const orc_geometry& g, const orc_scan& s, int cond, int n, cons
                 if (!xtt) {
                     val = ratio * (T)vol[idx];
                 } else {
+                    // forwardOrth (ir.cu:295-362): the basis rotated per voxel
+                    T R[9];
+                    if (coef) orth_rotation<T>(coef, idx, lenV, R);
                     val = 0;
                     for (int k = 0; k < K; k++) {
-                        const T w2 = xtt_w2(B, basis_vec<T>(s, k), f.G);
+                        const V3<T> S = coef ? mv(R, basis_vec<T>(s, k)) : basis_vec<T>(s, k);
+                        const T w2 = xtt_w2(B, S, f.G);
                         val += w2 * (T)g.vox_size * ratio * (T)vol[idx + k * lenV];
                     }
                 }
@@ -350,7 +384,8 @@ template <class T> void normalise_view(const orc_geometry& g, const Frame<T>& f,
 // ir.cu:1807-1850): tmp += sum c_k p_k, factor += 1 (plain) or per channel
 // tmp_k += vkm^2 sum c p, factor_k += vkm^2 (XTT).  Volume-sized accumulators.
 template <class T>
-void backward_raw(const orc_geometry& g, const orc_scan& s, int cond, int n, const T* proj, int xtt, T* tmp, T* factor) {
+void backward_raw(const orc_geometry& g, const orc_scan& s, int cond, int n, const T* proj, int xtt, T* tmp, T* factor,
+                  const float* coef = nullptr) {
     const int N = g.n_vox, U = g.n_det_u, V = g.n_det_v;
     const int64_t lenV = (int64_t)N * N * N;
     const Frame<T> f = make_frame<T>(g, s, cond, n, 0);
@@ -372,8 +407,12 @@ void backward_raw(const orc_geometry& g, const orc_scan& s, int cond, int n, con
                     factor[idx] += (T)1.0f;
                     tmp[idx] += numBack;
                 } else {
+                    // backwardOrth (ir.cu:365-423): the basis rotated per voxel
+                    T R[9];
+                    if (coef) orth_rotation<T>(coef, idx, lenV, R);
                     for (int k = 0; k < K; k++) {
-                        const T w2 = xtt_w2(B, basis_vec<T>(s, k), f.G);
+                        const V3<T> S = coef ? mv(R, basis_vec<T>(s, k)) : basis_vec<T>(s, k);
+                        const T w2 = xtt_w2(B, S, f.G);
                         const T bf = w2 * w.c1 * p1 + w2 * w.c2 * p2 + w2 * w.c3 * p3 + w2 * w.c4 * p4;
                         factor[idx + k * lenV] += w2;
                         tmp[idx + k * lenV] += bf;
@@ -683,6 +722,128 @@ void jacobi3(const double A_in[9], double evals[3], double evecs[9]) {
 
 }  // namespace
 
+// Normal of a voxel's three-direction scattering (ir.cu:912-922, 1042-1052):
+// (mu0 S0 - m1 S1) x (mu0 S0 - m2 S2), normalised.
+template <class T> V3<T> three_direc_normal(const orc_scan& s, T mu0, T m1, T m2) {
+    const V3<T> S0 = basis_vec<T>(s, 0), S1 = basis_vec<T>(s, 1), S2 = basis_vec<T>(s, 2);
+    const V3<T> v1 = {mu0 * S0.x - m1 * S1.x, mu0 * S0.y - m1 * S1.y, mu0 * S0.z - m1 * S1.z};
+    const V3<T> v2 = {mu0 * S0.x - m2 * S2.x, mu0 * S0.y - m2 * S2.y, mu0 * S0.z - m2 * S2.z};
+    return normalized(cross(v1, v2));
+}
+
+// calcNormalVectorThreeDirec (ir.cu:856-974; rot 0-3 picks the signs of mu1,
+// mu2) and calcNormalVectorThreeDirecWithEst (ir.cu:976-1071; the signs from
+// est channel 1: < 0.5, < 1.5, < 2.5, else).  Per voxel: the normal rotated by
+// the voxel's current rotation, flipped into z >= 0, normalised; loss = |R e_z
+// x normal|; coef <- (atan2 of e_z x normal, e_z . normal).  vol: 3 channels,
+// coef: 2 channels, est: 2 channels (or null), lenV voxels each.
+template <class T>
+void orth_normal(const orc_scan& s, int64_t lenV, const float* vol, float* coef, float* loss, int rot, const float* est) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < lenV; i++) {
+        T R[9];
+        orth_rotation<T>(coef, i, lenV, R);
+        const T mu0 = (T)vol[i], mu1 = (T)vol[i + lenV], mu2 = (T)vol[i + 2 * lenV];
+        int r = rot;
+        if (est) {
+            const T e = (T)est[i + lenV];
+            r = e < (T)0.5f ? 0 : (e < (T)1.5f ? 1 : (e < (T)2.5f ? 2 : 3));
+        }
+        T m1 = mu1, m2 = mu2;
+        if (r == 1) {
+            m1 = -mu1;
+        } else if (r == 2) {
+            m1 = -mu1;
+            m2 = -mu2;
+        } else if (r == 3) {
+            m2 = -mu2;
+        }
+        V3<T> nrm = mv(R, three_direc_normal<T>(s, mu0, m1, m2));
+        if (nrm.z < (T)0) nrm = {-nrm.x, -nrm.y, -nrm.z};
+        nrm = normalized(nrm);
+        const V3<T> ez = {0, 0, 1};
+        const V3<T> Rez = mv(R, ez);
+        const V3<T> diff = cross(Rez, nrm);
+        const V3<T> axis = cross(ez, nrm);
+        loss[i] = (float)norm2(diff);
+        coef[i] = (float)std::atan2(axis.y, axis.x);
+        coef[i + lenV] = (float)dot(ez, nrm);
+    }
+}
+
+// updateEstimationByCoef (ir.cu:1073-1118): the unrotated normal, flipped into
+// z >= 0; where its cosine to e_z exceeds est channel 0, est <- (cos, iter).
+template <class T> void orth_update_estimate(const orc_scan& s, int64_t lenV, const float* vol, float* est, int iter) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < lenV; i++) {
+        V3<T> nrm = three_direc_normal<T>(s, (T)vol[i], (T)vol[i + lenV], (T)vol[i + 2 * lenV]);
+        if (nrm.z < (T)0) nrm = {-nrm.x, -nrm.y, -nrm.z};
+        nrm = normalized(nrm);
+        const T c = nrm.z;
+        if (c > (T)est[i]) {
+            est[i] = (float)c;
+            est[i + lenV] = (float)iter;
+        }
+    }
+}
+
+// convertNormVector (ir.cu:1448-1480): md_i = mean(mu) (R e_z)_i.
+template <class T> void orth_md(int64_t lenV, const float* vol, const float* coef, float* md) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < lenV; i++) {
+        const T mu = ((T)vol[i] + (T)vol[i + lenV] + (T)vol[i + 2 * lenV]) / (T)3.0f;
+        T R[9];
+        orth_rotation<T>(coef, i, lenV, R);
+        const V3<T> n = mv(R, V3<T>{0, 0, 1});
+        for (int c = 0; c < 3; c++) md[i + c * lenV] = (float)(mu * n[c]);
+    }
+}
+
+// meanFiltFiber (ir.cu:1231-1303) on the interior voxels (reference layout, x
+// fastest): the 27 neighbours' (mu-weighted) orientation vectors, those within
+// |cos| > 0.5 of the centre's added with weight `w` (sign-aligned), the sum
+// normalised and flipped into z >= 0, stored as (atan2, cos).  The boundary
+// voxels of dst are left as they are.
+template <class T> void orth_mean_filter(int N, const float* src, const float* vol, float w, float* dst) {
+    const int64_t lenV = (int64_t)N * N * N;
+#pragma omp parallel for schedule(static)
+    for (int z = 1; z < N - 1; z++)
+        for (int y = 1; y < N - 1; y++)
+            for (int x = 1; x < N - 1; x++) {
+                V3<T> nb[27];
+                int cnt = 0;
+                for (int i = -1; i < 2; i++)
+                    for (int j = -1; j < 2; j++)
+                        for (int k = -1; k < 2; k++) {
+                            const int64_t q = (x - i) + (int64_t)N * (y - j) + (int64_t)N * N * (z - k);
+                            const T ct = (T)src[q + lenV];
+                            const T st = std::sqrt((T)1.0f - ct * ct);
+                            const T phi = -(T)kPIf / (T)2.0f + (T)src[q];
+                            const T mu = (T)vol[q] + (T)vol[q + lenV] + (T)vol[q + 2 * lenV];
+                            nb[cnt++] = {mu * (st * std::cos(phi)), mu * (st * std::sin(phi)), mu * ct};
+                        }
+                V3<T> c = nb[13];
+                const T cut = (T)0.5f;
+                for (int i = 0; i < 13; i++) {
+                    for (int side = 0; side < 2; side++) {
+                        const V3<T>& o = side ? nb[26 - i] : nb[i];
+                        const T d = dot(o, nb[13]);
+                        if (d > cut)
+                            c = {c.x + (T)w * o.x, c.y + (T)w * o.y, c.z + (T)w * o.z};
+                        else if (d < -cut)
+                            c = {c.x - (T)w * o.x, c.y - (T)w * o.y, c.z - (T)w * o.z};
+                    }
+                }
+                c = normalized(c);
+                if (c.z < (T)0) c = {-c.x, -c.y, -c.z};
+                const V3<T> ez = {0, 0, 1};
+                const V3<T> axis = cross(ez, c);
+                const int64_t q = x + (int64_t)N * y + (int64_t)N * N * z;
+                dst[q] = (float)std::atan2(axis.y, axis.x);
+                dst[q + lenV] = (float)dot(ez, c);
+            }
+}
+
 extern "C" {
 
 // Reference-preset voxel size (geometry.h:13): sod * detSize / sdd in FP32.
@@ -731,6 +892,75 @@ void orc_forward_view(int prec, const orc_geometry* g, const orc_scan* s, int co
         if (out_factor) std::memcpy(out_factor, f.data(), lenP * sizeof(double));
     }
 }
+
+// f3 forwardOrth (+a4 when normalise): one view of the orientation-aware XTT
+// operator; coef: 2*N^3 floats (phi, cos theta) in the reference layout.
+void orc_forward_view_orth(int prec, const orc_geometry* g, const orc_scan* s, int cond, int n, const float* vol,
+                           const float* coef, int normalise, double* out_proj, double* out_factor) {
+    const int64_t lenP = (int64_t)g->n_det_u * g->n_det_v;
+    if (prec == 0) {
+        std::vector<float> p(lenP, 0.f), f(lenP, 0.f);
+        forward_raw<float, float>(*g, *s, cond, n, vol, 1, p.data(), f.data(), coef);
+        if (normalise) normalise_view(*g, make_frame<float>(*g, *s, cond, n, 0), p.data(), f.data());
+        for (int64_t i = 0; i < lenP; i++) {
+            out_proj[i] = p[i];
+            if (out_factor) out_factor[i] = f[i];
+        }
+    } else {
+        std::vector<double> p(lenP, 0.0), f(lenP, 0.0);
+        forward_raw<double, float>(*g, *s, cond, n, vol, 1, p.data(), f.data(), coef);
+        if (normalise) normalise_view(*g, make_frame<double>(*g, *s, cond, n, 0), p.data(), f.data());
+        std::memcpy(out_proj, p.data(), lenP * sizeof(double));
+        if (out_factor) std::memcpy(out_factor, f.data(), lenP * sizeof(double));
+    }
+}
+
+// f3 backwardOrth of one view into K*N^3 accumulators (doubles, accumulated).
+void orc_backward_view_orth(int prec, const orc_geometry* g, const orc_scan* s, int cond, int n,
+                            const float* proj_view, const float* coef, double* tmp, double* factor) {
+    const int64_t lenP = (int64_t)g->n_det_u * g->n_det_v;
+    const int64_t lenV = (int64_t)g->n_vox * g->n_vox * g->n_vox * s->n_basis;
+    if (prec == 0) {
+        std::vector<float> p(proj_view, proj_view + lenP), t(lenV, 0.f), f(lenV, 0.f);
+        backward_raw<float>(*g, *s, cond, n, p.data(), 1, t.data(), f.data(), coef);
+        for (int64_t i = 0; i < lenV; i++) {
+            tmp[i] += t[i];
+            factor[i] += f[i];
+        }
+    } else {
+        std::vector<double> p(proj_view, proj_view + lenP);
+        backward_raw<double>(*g, *s, cond, n, p.data(), 1, tmp, factor, coef);
+    }
+}
+
+void orc_orth_normal(int prec, const orc_scan* s, int64_t lenV, const float* vol, float* coef, float* loss, int rot,
+                     const float* est) {
+    if (prec == 0)
+        orth_normal<float>(*s, lenV, vol, coef, loss, rot, est);
+    else
+        orth_normal<double>(*s, lenV, vol, coef, loss, rot, est);
+}
+void orc_orth_update_estimate(int prec, const orc_scan* s, int64_t lenV, const float* vol, float* est, int iter) {
+    if (prec == 0)
+        orth_update_estimate<float>(*s, lenV, vol, est, iter);
+    else
+        orth_update_estimate<double>(*s, lenV, vol, est, iter);
+}
+void orc_orth_md(int prec, int64_t lenV, const float* vol, const float* coef, float* md) {
+    if (prec == 0)
+        orth_md<float>(lenV, vol, coef, md);
+    else
+        orth_md<double>(lenV, vol, coef, md);
+}
+void orc_orth_mean_filter(int prec, int N, const float* src, const float* vol, float w, float* dst) {
+    if (prec == 0)
+        orth_mean_filter<float>(N, src, vol, w, dst);
+    else
+        orth_mean_filter<double>(N, src, vol, w, dst);
+}
+
+// rodriguesRotation (ir.cu:597-621) in FP64: R (9 doubles, row major).
+void orc_rodrigues(double x, double y, double z, double c, double sn, double* R) { rodrigues<double>(x, y, z, c, sn, R); }
 
 // a4's L for every pixel of view n.
 void orc_chord_length(int prec, const orc_geometry* g, const orc_scan* s, int cond, int n, double* out) {

@@ -46,6 +46,13 @@ def _load():
         "orc_forward_view": (None, [I, G, S, I, I, _f, I, I, _d, _d]),
         "orc_chord_length": (None, [I, G, S, I, I, _d]),
         "orc_backward_view": (None, [I, G, S, I, I, _f, I, _d, _d]),
+        "orc_forward_view_orth": (None, [I, G, S, I, I, _f, _f, I, _d, _d]),
+        "orc_backward_view_orth": (None, [I, G, S, I, I, _f, _f, _d, _d]),
+        "orc_rodrigues": (None, [ctypes.c_double] * 5 + [_d]),
+        "orc_orth_normal": (None, [I, S, ctypes.c_int64, _f, _f, _f, I, _f]),
+        "orc_orth_update_estimate": (None, [I, S, ctypes.c_int64, _f, _f, I]),
+        "orc_orth_md": (None, [I, ctypes.c_int64, _f, _f, _f]),
+        "orc_orth_mean_filter": (None, [I, I, _f, _f, F, _f]),
         "orc_subset_orders": (None, [I, I, I, _i32]),
         "orc_ratio_step": (ctypes.c_double, [I, ctypes.c_int64, _f, _d, I]),
         "orc_ir_reconstruct": (None, [I, G, S, _f, _f, I, I, I, I, F, _d, _d]),
@@ -155,6 +162,38 @@ class Case:
         pv = np.ascontiguousarray(proj_view, dtype=np.float32).reshape(-1)
         _lib.orc_backward_view(prec, self.g, self.s, cond, n, _fp(pv), int(xtt), _dp(tmp), _dp(factor))
 
+    def forward_view_orth(self, vol, coef, n, cond=0, normalise=True, prec=F64):
+        """forwardOrth: the XTT operator with the basis rotated per voxel by the
+        Rodrigues rotation of (phi, cos theta) = coef[0], coef[1] (reference layout)."""
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
+        coef = np.ascontiguousarray(coef, dtype=np.float32)
+        p = np.zeros(self.U * self.V)
+        f = np.zeros(self.U * self.V)
+        _lib.orc_forward_view_orth(prec, self.g, self.s, cond, n, _fp(vol), _fp(coef), int(normalise), _dp(p), _dp(f))
+        return p.reshape(self.V, self.U), f.reshape(self.V, self.U)
+
+    def backward_view_orth(self, proj_view, coef, n, tmp, factor, cond=0, prec=F64):
+        pv = np.ascontiguousarray(proj_view, dtype=np.float32).reshape(-1)
+        coef = np.ascontiguousarray(coef, dtype=np.float32)
+        _lib.orc_backward_view_orth(prec, self.g, self.s, cond, n, _fp(pv), _fp(coef), _dp(tmp), _dp(factor))
+
+    def orth_normal(self, vol, coef, rot=0, est=None, prec=F64):
+        """calcNormalVectorThreeDirec (rot) / ...WithEst (est): returns the new
+        coef (2 channels) and the per-voxel loss; vol 3 channels."""
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
+        c = np.array(coef, dtype=np.float32, copy=True)
+        n = vol.size // 3
+        loss = np.zeros(n, np.float32)
+        e = None if est is None else np.ascontiguousarray(est, dtype=np.float32)
+        _lib.orc_orth_normal(prec, self.s, n, _fp(vol), _fp(c), _fp(loss), rot, None if e is None else _fp(e))
+        return c, loss
+
+    def orth_update_estimate(self, vol, est, it, prec=F64):
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
+        e = np.array(est, dtype=np.float32, copy=True)
+        _lib.orc_orth_update_estimate(prec, self.s, vol.size // 3, _fp(vol), _fp(e), it)
+        return e
+
     def ir_reconstruct(self, sino, vol0, epochs, batch, direction=0, art=False, lam=1e-2, prec=F64):
         sino = np.ascontiguousarray(sino, dtype=np.float32)
         vol0 = np.ascontiguousarray(vol0, dtype=np.float32)
@@ -200,6 +239,34 @@ class Case:
         filt = np.zeros(sino.size)
         _lib.orc_fdk_reconstruct(prec, self.g, self.s, _fp(sino), direction, kind, _dp(vol), _dp(filt))
         return vol.reshape(self.n_cond, self.N, self.N, self.N), filt.reshape(sino.shape)
+
+
+def rodrigues(x, y, z, c, sn):
+    """rodriguesRotation (ir.cu:597-621) in FP64: 3x3 rotation about axis
+    (x, y, z) with cosine c and sine sn."""
+    R = np.zeros(9)
+    _lib.orc_rodrigues(float(x), float(y), float(z), float(c), float(sn), _dp(R))
+    return R.reshape(3, 3)
+
+
+def orth_md(vol, coef, prec=F64):
+    """convertNormVector: md (3 channels) = mean(mu) * R e_z."""
+    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    coef = np.ascontiguousarray(coef, dtype=np.float32)
+    md = np.zeros(vol.size, np.float32)
+    _lib.orc_orth_md(prec, vol.size // 3, _fp(vol), _fp(coef), _fp(md))
+    return md.reshape(vol.shape)
+
+
+def orth_mean_filter(src, vol, w, dst=None, prec=F64):
+    """meanFiltFiber over the interior voxels (reference layout); dst's boundary
+    voxels are kept (default: a copy of src)."""
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    N = round((src.size // 2) ** (1.0 / 3.0))
+    out = np.array(src if dst is None else dst, dtype=np.float32, copy=True)
+    _lib.orc_orth_mean_filter(prec, N, _fp(src), _fp(vol), w, _fp(out))
+    return out
 
 
 def theta_f32(n, P):

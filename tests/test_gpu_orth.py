@@ -1,0 +1,146 @@
+"""Orientation-aware XTT (forwardOrth / backwardOrth, ir.cu:295-423) on the GPU
+against the FP64 oracle: the plan's XTT projections with a per-voxel
+orientation field (phi, cos theta) rotating the basis by rodriguesRotation
+(ir.cu:597-621), untilted and tilted conditions, two grating orientations."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from test_gpu_parity import Setup, columns, grating, proj_error, record, rot_x, to_dev
+
+pytestmark = pytest.mark.gpu
+
+BASIS3 = np.array([[0.666667, -0.666667, -0.333333], [-0.333333, -0.666667, 0.666667],
+                   [0.666667, 0.333333, 0.666667]], np.float32)
+
+
+@pytest.mark.parametrize("tilt", [False, True])
+def test_orth_operators_match_oracle(orc, xr, ctx, torch_dev, tilt):
+    import torch
+
+    cond_R = np.concatenate([np.eye(3).reshape(-1), (rot_x(15.0) if tilt else np.eye(3)).reshape(-1)])
+    st = Setup(orc, xr, N=24, U=40, P=16, vs=0.6, cond_R=cond_R, basis=BASIS3, grating=grating([0, 60]))
+    K, N = 3, 24
+    rng = np.random.default_rng(21)
+    vol = (rng.random((K, N, N, N)) * 0.02).astype(np.float32)
+    coef = np.stack([rng.uniform(-np.pi, np.pi, (N, N, N)), rng.uniform(-1.0, 1.0, (N, N, N))]).astype(np.float32)
+    coef[:, :4] = np.array([0.0, 1.0], np.float32)[:, None, None, None]  # unrotated slab
+    views = [0, 5, 11]
+    codes = np.concatenate([np.array(views) + c * st.P for c in range(2)])
+    plan = ctx.plan(st.geom, st.scan)
+    vcol = columns(ctx, vol, N, K, torch_dev)
+    ccol = columns(ctx, coef, N, 2, torch_dev)
+    plan.set_orientation(ccol)
+    n = 2 * st.P * st.V * st.U
+    proj = torch.zeros(n, device=torch_dev)
+    plan.project(vcol, proj, codes, out_mode=xr.OUT_NORMALISED, xtt=True)
+    got = proj.cpu().numpy().reshape(2, st.P, st.V, st.U)
+    worst = 0.0
+    for c in range(2):
+        for v in views:
+            want, fac = st.case.forward_view_orth(vol, coef, v, cond=c)
+            want32, _ = st.case.forward_view_orth(vol, coef, v, cond=c, prec=0)
+            worst = max(worst, proj_error(got[c, v], want, want32, fac))
+    record("orth_forward_f64/tilt=%d" % tilt, worst)
+    assert worst <= 1e-5
+    # without the field the plan is the fixed-basis operator again
+    plan.set_orientation(None)
+    proj.zero_()
+    plan.project(vcol, proj, codes, out_mode=xr.OUT_NORMALISED, xtt=True)
+    want0, _ = st.case.forward_view(vol, views[1], cond=1, xtt=True)
+    assert rel_l2(proj.cpu().numpy().reshape(2, st.P, st.V, st.U)[1, views[1]], want0) <= 1e-5
+    # raw backprojection per channel
+    plan.set_orientation(ccol)
+    img = rng.random((2, st.P, st.V, st.U)).astype(np.float32)
+    N3 = N ** 3
+    tmp = torch.zeros(K * N3, device=torch_dev)
+    fac = torch.zeros(K * N3, device=torch_dev)
+    plan.backproject(to_dev(img, torch_dev), codes, tmp, fac, xtt=True)
+    tref, fref = torch.empty_like(tmp), torch.empty_like(fac)
+    ctx.from_columns(tmp, tref, N, K)
+    ctx.from_columns(fac, fref, N, K)
+    wt = np.zeros(K * N3)
+    wf = np.zeros(K * N3)
+    for c in range(2):
+        for v in views:
+            st.case.backward_view_orth(img[c, v], coef, v, wt, wf, cond=c)
+    et, ef = rel_l2(tref.cpu().numpy(), wt), rel_l2(fref.cpu().numpy(), wf)
+    record("orth_backward_tmp/tilt=%d" % tilt, et)
+    record("orth_backward_factor/tilt=%d" % tilt, ef)
+    assert et <= 1e-5 and ef <= 1e-5
+    plan.close()
+
+
+def angle_diff(a, b):
+    return np.abs(np.angle(np.exp(1j * (a.astype(np.float64) - b.astype(np.float64)))))
+
+
+def test_orth_helpers_match_oracle(orc, xr, ctx, torch_dev):
+    """calcNormalVectorThreeDirec (every sign choice and the estimate form),
+    updateEstimationByCoef and convertNormVector against the oracle: per voxel,
+    elementwise, so the voxel order is free.  Angles are compared where the
+    normal is not within 1e-3 of the z axis (atan2 of a vanishing axis)."""
+    import torch
+
+    case = orc.Case(600.0, 300.0, 0.8, 8, 16, 4, basis=BASIS3)
+    rng = np.random.default_rng(31)
+    n = 40000
+    vol = rng.uniform(0.05, 1.0, (3, n)).astype(np.float32)
+    coef = np.stack([rng.uniform(-np.pi, np.pi, n), rng.uniform(-1.0, 1.0, n)]).astype(np.float32)
+    vd = to_dev(vol, torch_dev)
+    for rot in range(4):
+        cd = to_dev(coef, torch_dev)
+        ld = torch.zeros(n, device=torch_dev)
+        ctx.orth_normal(BASIS3, vd, n, cd, ld, rot=rot)
+        want, wloss = case.orth_normal(vol, coef, rot)
+        got = cd.cpu().numpy().reshape(2, n)
+        assert np.abs(ld.cpu().numpy() - wloss).max() <= 1e-5
+        assert np.abs(got[1] - want[1]).max() <= 1e-5
+        ok = np.sqrt(np.maximum(1 - want[1].astype(np.float64) ** 2, 0)) > 1e-3
+        assert angle_diff(got[0][ok], want[0][ok]).max() <= 1e-3
+    # the estimate form picks the signs from est channel 1
+    est = np.stack([rng.uniform(0, 1, n), rng.integers(0, 4, n).astype(np.float64)]).astype(np.float32)
+    cd = to_dev(coef, torch_dev)
+    ld = torch.zeros(n, device=torch_dev)
+    ctx.orth_normal(BASIS3, vd, n, cd, ld, est=to_dev(est, torch_dev))
+    want, _ = case.orth_normal(vol, coef, 0, est=est)
+    assert np.abs(cd.cpu().numpy().reshape(2, n)[1] - want[1]).max() <= 1e-5
+    # updateEstimationByCoef: a voxel whose cosine ties the estimate within rounding may go either way
+    ed = to_dev(est, torch_dev)
+    ctx.orth_update_estimate(BASIS3, vd, n, ed, 4)
+    got_e = ed.cpu().numpy().reshape(2, n)
+    want_e = case.orth_update_estimate(vol, est, 4)
+    same = (got_e[1] == want_e[1])
+    assert same.mean() > 0.999 and np.abs(got_e[0][same] - want_e[0][same]).max() <= 1e-5
+    # convertNormVector
+    md = torch.zeros(3 * n, device=torch_dev)
+    ctx.orth_md(vd, to_dev(coef, torch_dev), n, md)
+    assert rel_l2(md.cpu().numpy().reshape(3, n), orc.orth_md(vol, coef)) <= 1e-5
+
+
+def test_orth_mean_filter_matches_oracle(orc, xr, ctx, torch_dev):
+    import torch
+
+    N = 20
+    rng = np.random.default_rng(32)
+    src = np.stack([rng.uniform(-np.pi, np.pi, (N, N, N)), rng.uniform(0.0, 1.0, (N, N, N))]).astype(np.float32)
+    vol = rng.uniform(0.2, 1.0, (3, N, N, N)).astype(np.float32)
+    # FP32 restatement: the |dot| > 0.5 selection may flip where host and device
+    # round a neighbour's vector differently, so a few voxels may differ
+    want = orc.orth_mean_filter(src, vol, 1.0, prec=0)
+    sc = columns(ctx, src, N, 2, torch_dev)
+    vc = columns(ctx, vol, N, 3, torch_dev)
+    dst = torch.empty_like(sc)
+    ctx.orth_mean_filter(sc, vc, N, 1.0, dst)
+    back = torch.empty_like(dst)
+    ctx.from_columns(dst, back, N, 2)
+    got = back.cpu().numpy().reshape(2, N, N, N)
+    inner = (slice(1, -1),) * 3
+    g0, g1, w0, w1 = got[0][inner], got[1][inner], want[0][inner], want[1][inner]
+    ok = np.sqrt(np.maximum(1 - w1.astype(np.float64) ** 2, 0)) > 1e-3
+    bad = (np.abs(g1 - w1) > 1e-5) | (ok & (angle_diff(g0, w0) > 1e-3))
+    assert bad.mean() < 0.005, bad.mean()
+    # boundary voxels keep the source
+    edge = np.ones((N, N, N), bool)
+    edge[inner] = False
+    assert np.array_equal(got[:, edge], src[:, edge])

@@ -113,6 +113,10 @@ def _load():
         "xr_fdk_taps": (I, [ctypes.POINTER(Geometry), I, _c_float_p]),
         "xr_volume_sqrt": (I, [P, P, ctypes.c_int64]),
         "xr_volume_downsample": (I, [P, P, I, I, P]),
+        "xr_orth_normal": (I, [P, _c_float_p, P, ctypes.c_int64, P, P, I, P]),
+        "xr_orth_update_estimate": (I, [P, _c_float_p, P, ctypes.c_int64, P, I]),
+        "xr_orth_md": (I, [P, P, P, ctypes.c_int64, P]),
+        "xr_orth_mean_filter": (I, [P, P, P, I, ctypes.c_float, P]),
         "xr_volume_upsample_add": (I, [P, P, P, I, I, P, ctypes.c_float]),
         "xr_pca": (I, [P, P, I, _c_float_p, ctypes.c_int64, ctypes.c_int64, P, P]),
         "xr_ir_reconstruct": (I, [P, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct),
@@ -144,6 +148,7 @@ def _load():
         "xr_philox4x32": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
                                  ctypes.POINTER(ctypes.c_uint32)]),
         "xr_plan_set_tiles": (I, [P, ctypes.POINTER(Tiles)]),
+        "xr_plan_set_orientation": (I, [P, P]),
         "xr_plan_get_tiles": (I, [P, ctypes.POINTER(Tiles)]),
         "xr_autotune": (I, [P, P, P, P, ctypes.POINTER(Tiles), _c_double_p]),
         "xr_batch_create": (I, [P, I, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct), _c_int32_p, I, I, F,
@@ -308,6 +313,22 @@ class Context:
 
     def from_columns(self, col, ref, n_vox, channels=1):
         self.check(_lib.xr_volume_from_columns(self.h, _dptr(col), _dptr(ref), n_vox, channels))
+
+    def orth_normal(self, basis, vol, nvox, coef, loss, rot=0, est=None):
+        """calcNormalVectorThreeDirec (rot) / ...WithEst (est): coef updated in place."""
+        b = np.ascontiguousarray(basis, dtype=np.float32).reshape(-1)
+        self.check(_lib.xr_orth_normal(self.h, _fptr(b), _dptr(vol), nvox, _dptr(coef), _dptr(loss), rot,
+                                       _dptr(est)))
+
+    def orth_update_estimate(self, basis, vol, nvox, est, iteration):
+        b = np.ascontiguousarray(basis, dtype=np.float32).reshape(-1)
+        self.check(_lib.xr_orth_update_estimate(self.h, _fptr(b), _dptr(vol), nvox, _dptr(est), iteration))
+
+    def orth_md(self, vol, coef, nvox, md):
+        self.check(_lib.xr_orth_md(self.h, _dptr(vol), _dptr(coef), nvox, _dptr(md)))
+
+    def orth_mean_filter(self, coef_src, vol, n_vox, weight, coef_dst):
+        self.check(_lib.xr_orth_mean_filter(self.h, _dptr(coef_src), _dptr(vol), n_vox, weight, _dptr(coef_dst)))
 
     def volume_downsample(self, fine, n_vox, factor, coarse):
         """coarse = mean of each factor^3 block of fine (column layout)."""
@@ -497,6 +518,13 @@ class Plan:
         c = np.ascontiguousarray(codes, dtype=np.int32)
         self.ctx.check(_lib.xr_backproject_update(self.ctx.h, self.h, _dptr(proj), c.ctypes.data_as(_c_int32_p),
                                                   len(c), int(xtt), method, lam, subset_size, _dptr(vol)))
+
+    def set_orientation(self, coef):
+        """Orientation field of the XTT projections (forwardOrth / backwardOrth):
+        a device tensor of 2*N^3 floats, phi then cos theta, column layout; None
+        returns to the fixed basis.  The plan keeps a reference to the tensor."""
+        self._orient = coef
+        self.ctx.check(_lib.xr_plan_set_orientation(self.h, None if coef is None else _dptr(coef)))
 
     def set_tiles(self, bwd_zpl=0, bwd_gx=0, fwd_rows=0, fwd_scan=0, fwd_split=0):
         t = Tiles(bwd_zpl, bwd_gx, fwd_rows, fwd_scan, fwd_split)

@@ -50,6 +50,8 @@ struct xr_plan {
     xr::Dims dims{};
     xr::Basis bas{};
     float xtt_wsum = 0.f;  // sum_k |S_k|^2 max_view (S_k.G)^2: XTT splat value per unit |mu|
+    float xtt_wsum_orth = 0.f;  // sum_k |S_k|^4 max_view |G|^2: the same bound under any rotation of the basis
+    const float* d_orient = nullptr;  // caller's orientation field (xr_plan_set_orientation), or null
     xr_tiles tiles{};      // projector tile shape (zeros: defaults)
     float max_factor = 0.f;     // largest per-pixel splat weight sum (set with d_norm)
     float* d_fmax = nullptr;    // device scalar receiving it during the weight pass

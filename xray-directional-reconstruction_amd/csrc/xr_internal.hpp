@@ -120,6 +120,7 @@ struct FwdArgs {
     int n_sub;               // 1, 2, 4: a work item owns kBandU / n_sub detector columns of its band
     int axis;                // batched launches: 1 -> every view is axis aligned (ViewGeom::axis)
     const int32_t* bands;    // untilted views: [code][band][y] columns xs | xe << 16 to scan, or null (all)
+    const float* coef;       // XTT orientation field (forwardOrth): phi, cos theta per voxel, chan_stride apart, or null
 };
 
 struct BwdArgs {
@@ -145,6 +146,7 @@ struct BwdArgs {
     int tiles_x, tiles_y, tiles_z, n_tiles, per_xcd;
     int axis;                 // batched launches: 1 -> every view is axis aligned
     unsigned* vmax_out;       // batched MLEM / ART: atomic max of |updated voxel| (float bits), or null
+    const float* coef;        // XTT orientation field (backwardOrth): phi, cos theta per voxel, or null
 };
 
 // Tile shape of the plain backprojection (CPW = 2 columns per wave side).
@@ -199,6 +201,11 @@ hipError_t launch_to_internal(const float* ref, float* internal, int N, int chan
 hipError_t launch_from_internal(const float* internal, float* ref, int N, int channels, hipStream_t stream);
 hipError_t launch_sqrt_abs(float* vol, int64_t n, hipStream_t stream);
 hipError_t launch_vol_down(const float* fine, float* coarse, int Nc, int f, hipStream_t stream);
+hipError_t launch_orth_normal(const float* vol, float* coef, float* loss, int64_t n, int rot, const float* est,
+                              const Basis& b, hipStream_t stream);
+hipError_t launch_orth_update_est(const float* vol, float* est, int64_t n, int iter, const Basis& b, hipStream_t stream);
+hipError_t launch_orth_md(const float* vol, const float* coef, float* md, int64_t n, hipStream_t stream);
+hipError_t launch_orth_mean_filter(const float* src, const float* vol, int N, float w, float* dst, hipStream_t stream);
 hipError_t launch_vol_up_add(const float* a, const float* b, int Nc, int f, float* fine, float lo, hipStream_t stream);
 hipError_t launch_pca(const float* mu, int K, const Basis& b, int64_t nvox, int64_t chan_stride, float* md,
                       float* evalue, hipStream_t stream);

@@ -410,6 +410,63 @@ __device__ __forceinline__ float xtt_value(const FwdArgs& a, const FwdCol& f, co
     return s * (d.vs * d.ratio);
 }
 
+// The per-voxel basis rotation of forwardOrth / backwardOrth (ir.cu:316-328,
+// 390-400): rodriguesRotation (ir.cu:597-621) about the axis (cos phi, sin phi,
+// 0) with cosine cc and sine sqrt(1 - cc^2), (1 - cc) n n^T + [[cc, -n_z s,
+// n_y s], [n_z s, cc, -n_x s], [-n_y s, n_x s, cc]] with n_z = 0.
+__device__ __forceinline__ void orth_rotation(float phi, float cc, float R[9]) {
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    const float sn = sqrtf(1.0f - cc * cc);
+    const float len = sqrtf(cp * cp + sp * sp);
+    if (len < 1e-8f) {
+        const float I[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+#pragma unroll
+        for (int i = 0; i < 9; i++) R[i] = I[i];
+        return;
+    }
+    const float nx = cp / len, ny = sp / len, om = 1.0f - cc;
+    R[0] = om * nx * nx + cc;
+    R[1] = om * nx * ny;
+    R[2] = ny * sn;
+    R[3] = om * nx * ny;
+    R[4] = om * ny * ny + cc;
+    R[5] = -nx * sn;
+    R[6] = -ny * sn;
+    R[7] = nx * sn;
+    R[8] = cc;
+}
+
+// Orientation-aware XTT weight of channel k at a voxel: S' = R S_k, w =
+// max(|S'|^2 - (d.S')^2 / |d|^2, 0) (S'.G)^2, i.e. |B x S'|^2 (S'.G)^2
+// (ir.cu:352-359, 410-417).
+__device__ __forceinline__ float orth_weight(const float R[9], const Basis& b, int k, float dx, float dy, float dz,
+                                             float id2, const ViewGeom& g) {
+    const float s0 = fmaf(R[0], b.S[k][0], fmaf(R[1], b.S[k][1], R[2] * b.S[k][2]));
+    const float s1 = fmaf(R[3], b.S[k][0], fmaf(R[4], b.S[k][1], R[5] * b.S[k][2]));
+    const float s2 = fmaf(R[6], b.S[k][0], fmaf(R[7], b.S[k][1], R[8] * b.S[k][2]));
+    const float ds = fmaf(dz, s2, fmaf(dy, s1, dx * s0));
+    const float ss = fmaf(s2, s2, fmaf(s1, s1, s0 * s0));
+    const float gs = fmaf(g.G[2], s2, fmaf(g.G[1], s1, g.G[0] * s0));
+    return fmaxf(fmaf(-ds * ds, id2, ss), 0.f) * (gs * gs);
+}
+
+// forwardOrth voxel value: sum_k |B x R S_k|^2 (R S_k . G)^2 mu_k * vs * ratio.
+__device__ __forceinline__ float orth_value(const FwdArgs& a, const FwdCol& f, const ViewGeom& g, int z,
+                                            const float* mu, float phi, float cc) {
+    const Dims& d = a.d;
+    const float zp = (float)z - d.xc;
+    const float dz = fmaf(d.vs, zp, g.d0[2]);
+    const float id2 = rcpf(fmaf(dz, dz, fmaf(f.dy, f.dy, f.dx * f.dx)));
+    float R[9];
+    orth_rotation(phi, cc, R);
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < kMaxBasis; k++)
+        if (k < d.K) s = fmaf(orth_weight(R, a.b, k, f.dx, f.dy, dz, id2, g), mu[k], s);
+    return s * (d.vs * d.ratio);
+}
+
 // Untilted view, volume row y' = yp: u(x') = hu + (au0 x' + cu) / (ad0 x' + cd)
 // is linear-fractional in x', hence monotone while the denominator is
 // positive, so the columns whose u falls in [ua, ub] form an interval.  The
@@ -718,7 +775,15 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
 #pragma unroll
                         for (int k = 0; k < kMaxBasis; k++)
                             mu[k] = (k < d.K && z < f.z_end) ? vol[f.base + z + k * a.chan_stride] : 0.f;
-                        const float val = xtt_value(a, f, g, z, g2, mu);
+                        float val;
+                        if (a.coef) {
+                            const bool in = z < f.z_end;
+                            const float phi = in ? a.coef[f.base + z] : 0.f;
+                            const float cc = in ? a.coef[f.base + z + a.chan_stride] : 1.f;
+                            val = orth_value(a, f, g, z, mu, phi, cc);
+                        } else {
+                            val = xtt_value(a, f, g, z, g2, mu);
+                        }
                         fwd_splat<AXIS, AccT, AccS>(a, g, f, z, val, scale, acc, u0, v0, vb);
                     }
                 }
@@ -2075,6 +2140,21 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
                             const float dz = fmaf(d.vs, zp, g.d0[2]);
                             const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                             const float id2 = rcpf(d2);
+                            if (a.coef) {
+                                // backwardOrth: the basis rotated per voxel
+                                const int64_t vi = ((int64_t)y * d.N + x) * d.N + z;
+                                float R[9];
+                                orth_rotation(a.coef[vi], a.coef[vi + a.chan_stride], R);
+#pragma unroll
+                                for (int k = 0; k < KA; k++) {
+                                    if (k < d.K) {
+                                        const float w = orth_weight(R, a.b, k, dx, dy, dz, id2, g);
+                                        tmp[cidx][q][k] = fmaf(w, nb, tmp[cidx][q][k]);
+                                        fac[cidx][q][k] += w;
+                                    }
+                                }
+                                continue;
+                            }
 #pragma unroll
                             for (int k = 0; k < KA; k++) {
                                 if (k < d.K) {
@@ -2290,6 +2370,157 @@ __global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in,
                 dst[b + (int64_t)N * y + N2 * a] = t[threadIdx.x][r];  // ref[x=b, y, z=a]
         }
     }
+}
+
+// ---- orientation helpers of the orientation-aware XTT loop ------------------
+// Per voxel, channels nvox apart.  Normal of the three-direction scattering
+// (ir.cu:912-922): (mu0 S0 - m1 S1) x (mu0 S0 - m2 S2), normalised (zero if
+// its length is at most 1e-8, vec.h:94-104).
+__device__ __forceinline__ void three_direc_normal(const Basis& b, float mu0, float m1, float m2, float n[3]) {
+    float v1[3], v2[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        v1[i] = mu0 * b.S[0][i] - m1 * b.S[1][i];
+        v2[i] = mu0 * b.S[0][i] - m2 * b.S[2][i];
+    }
+    n[0] = v1[1] * v2[2] - v1[2] * v2[1];
+    n[1] = v1[2] * v2[0] - v1[0] * v2[2];
+    n[2] = v1[0] * v2[1] - v1[1] * v2[0];
+    const float len = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) n[i] = len > 1e-8f ? n[i] / len : 0.f;
+}
+
+// calcNormalVectorThreeDirec (ir.cu:856-974; rot picks the signs of mu1, mu2)
+// and calcNormalVectorThreeDirecWithEst (ir.cu:976-1071; the signs from est
+// channel 1).  The normal is rotated by the voxel's current rotation, flipped
+// into z >= 0 and normalised; loss = |R e_z x n|; coef <- (atan2(n_x, -n_y),
+// n_z), the rotation that carries e_z onto n.
+__global__ __launch_bounds__(256) void k_orth_normal(const float* __restrict__ vol, float* __restrict__ coef,
+                                                     float* __restrict__ loss, int64_t n, int rot,
+                                                     const float* __restrict__ est, Basis b) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float R[9];
+    orth_rotation(coef[i], coef[i + n], R);
+    const float mu0 = vol[i], mu1 = vol[i + n], mu2 = vol[i + 2 * n];
+    int r = rot;
+    if (est) {
+        const float e = est[i + n];
+        r = e < 0.5f ? 0 : (e < 1.5f ? 1 : (e < 2.5f ? 2 : 3));
+    }
+    const float m1 = (r == 1 || r == 2) ? -mu1 : mu1;
+    const float m2 = (r == 2 || r == 3) ? -mu2 : mu2;
+    float n0[3], nr[3];
+    three_direc_normal(b, mu0, m1, m2, n0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) nr[k] = R[3 * k] * n0[0] + R[3 * k + 1] * n0[1] + R[3 * k + 2] * n0[2];
+    if (nr[2] < 0.f) {
+        nr[0] = -nr[0];
+        nr[1] = -nr[1];
+        nr[2] = -nr[2];
+    }
+    const float len = sqrtf(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) nr[k] = len > 1e-8f ? nr[k] / len : 0.f;
+    // (R e_z) x n
+    const float dx = R[5] * nr[2] - R[8] * nr[1], dy = R[8] * nr[0] - R[2] * nr[2], dz = R[2] * nr[1] - R[5] * nr[0];
+    loss[i] = sqrtf(dx * dx + dy * dy + dz * dz);
+    coef[i] = atan2f(nr[0], -nr[1]);  // e_z x n = (-n_y, n_x, 0)
+    coef[i + n] = nr[2];
+}
+
+// updateEstimationByCoef (ir.cu:1073-1118).
+__global__ __launch_bounds__(256) void k_orth_update_est(const float* __restrict__ vol, float* __restrict__ est,
+                                                         int64_t n, int iter, Basis b) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float nr[3];
+    three_direc_normal(b, vol[i], vol[i + n], vol[i + 2 * n], nr);
+    if (nr[2] < 0.f) {
+        nr[0] = -nr[0];
+        nr[1] = -nr[1];
+        nr[2] = -nr[2];
+    }
+    const float len = sqrtf(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+    const float c = len > 1e-8f ? nr[2] / len : 0.f;
+    if (c > est[i]) {
+        est[i] = c;
+        est[i + n] = (float)iter;
+    }
+}
+
+// convertNormVector (ir.cu:1448-1480): md = mean(mu) R e_z.
+__global__ __launch_bounds__(256) void k_orth_md(const float* __restrict__ vol, const float* __restrict__ coef,
+                                                 float* __restrict__ md, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float mu = (vol[i] + vol[i + n] + vol[i + 2 * n]) / 3.0f;
+    float R[9];
+    orth_rotation(coef[i], coef[i + n], R);
+    md[i] = mu * R[2];
+    md[i + n] = mu * R[5];
+    md[i + 2 * n] = mu * R[8];
+}
+
+// meanFiltFiber (ir.cu:1231-1303) on a column-layout volume of edge N: the
+// 27 neighbours' mu-weighted orientation vectors (sin t cos(phi - pi/2),
+// sin t sin(phi - pi/2), cos t), those with |dot| > 0.5 against the centre
+// added with weight w (sign aligned), normalised, flipped into z >= 0, stored
+// as (atan2, cos).  Boundary voxels copy the source.
+__global__ __launch_bounds__(256) void k_orth_mean_filter(const float* __restrict__ src, const float* __restrict__ vol,
+                                                          int N, float w, float* __restrict__ dst) {
+    const int64_t n = (int64_t)N * N * N;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int z = (int)(i % N), x = (int)((i / N) % N), y = (int)(i / ((int64_t)N * N));
+    if (x < 1 || y < 1 || z < 1 || x > N - 2 || y > N - 2 || z > N - 2) {
+        dst[i] = src[i];
+        dst[i + n] = src[i + n];
+        return;
+    }
+    auto vec = [&](int di, int dj, int dk, float v[3]) {
+        const int64_t q = ((int64_t)(y - dj) * N + (x - di)) * N + (z - dk);
+        const float ct = src[q + n];
+        const float st = sqrtf(1.0f - ct * ct);
+        float sp, cp;
+        sincosf(-(float)M_PI / 2.0f + src[q], &sp, &cp);
+        const float mu = vol[q] + vol[q + n] + vol[q + 2 * n];
+        v[0] = mu * (st * cp);
+        v[1] = mu * (st * sp);
+        v[2] = mu * ct;
+    };
+    float c[3], m[3];
+    vec(0, 0, 0, m);
+    c[0] = m[0];
+    c[1] = m[1];
+    c[2] = m[2];
+    // the reference's order: neighbour cnt and 26 - cnt for cnt = 0 .. 12
+    for (int cnt = 0; cnt < 13; cnt++) {
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int q = side ? 26 - cnt : cnt;
+            float o[3];
+            vec(q / 9 - 1, (q / 3) % 3 - 1, q % 3 - 1, o);
+            const float d = o[0] * m[0] + o[1] * m[1] + o[2] * m[2];
+            const float sgn = d > 0.5f ? 1.f : (d < -0.5f ? -1.f : 0.f);
+            if (sgn != 0.f) {
+                c[0] += sgn * w * o[0];
+                c[1] += sgn * w * o[1];
+                c[2] += sgn * w * o[2];
+            }
+        }
+    }
+    const float len = sqrtf(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) c[k] = len > 1e-8f ? c[k] / len : 0.f;
+    if (c[2] < 0.f) {
+        c[0] = -c[0];
+        c[1] = -c[1];
+        c[2] = -c[2];
+    }
+    dst[i] = atan2f(c[0], -c[1]);
+    dst[i + n] = c[2];
 }
 
 // Mean over f x f x f blocks of a column-layout volume of edge N = Nc * f
@@ -2896,6 +3127,32 @@ hipError_t launch_to_internal(const float* ref, float* internal, int N, int chan
 hipError_t launch_from_internal(const float* internal, float* ref, int N, int channels, hipStream_t stream) {
     hipLaunchKernelGGL((k_transpose<false>), dim3(grid1(N, 32), grid1(N, 32), (unsigned)N * channels), dim3(32, 8), 0,
                        stream, internal, ref, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_orth_normal(const float* vol, float* coef, float* loss, int64_t n, int rot, const float* est,
+                              const Basis& b, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_orth_normal, dim3(grid1(n, 256)), dim3(256), 0, stream, vol, coef, loss, n, rot, est, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_orth_update_est(const float* vol, float* est, int64_t n, int iter, const Basis& b,
+                                  hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_orth_update_est, dim3(grid1(n, 256)), dim3(256), 0, stream, vol, est, n, iter, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_orth_md(const float* vol, const float* coef, float* md, int64_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_orth_md, dim3(grid1(n, 256)), dim3(256), 0, stream, vol, coef, md, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_orth_mean_filter(const float* src, const float* vol, int N, float w, float* dst, hipStream_t stream) {
+    const int64_t n = (int64_t)N * N * N;
+    hipLaunchKernelGGL(k_orth_mean_filter, dim3(grid1(n, 256)), dim3(256), 0, stream, src, vol, N, w, dst);
     return hipGetLastError();
 }
 

@@ -254,11 +254,16 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
             }
             a.vmax = vb;
         }
-        a.wmul = (kind == FWD_XTT) ? p->dims.vs * p->dims.ratio * p->xtt_wsum * 1.0001f : p->dims.ratio * 1.0001f;
+        const float wsum = p->d_orient ? p->xtt_wsum_orth : p->xtt_wsum;
+        a.wmul = (kind == FWD_XTT) ? p->dims.vs * p->dims.ratio * wsum * 1.0001f : p->dims.ratio * 1.0001f;
     }
     // All conditions in one launch share the axis specialisation only if all are axis-aligned.
     bool axis = true;
     for (int c = 0; c < p->n_cond; c++) axis = axis && p->axis_cond[c];
+    if (kind == FWD_XTT && p->d_orient) {
+        a.coef = p->d_orient;  // forwardOrth: per-voxel rotation, general path
+        axis = false;
+    }
     // Detector rows in equal chunks, all in one launch (untilted scans walk
     // only the z range that reaches a chunk's rows).
     const int V = p->g.n_det_v;
@@ -320,6 +325,10 @@ int backward_launch(xr_plan* p, int kind, const float* proj, int code_base, cons
     a.gx = p->tiles.bwd_gx;
     bool axis = true;
     for (int c = 0; c < p->n_cond; c++) axis = axis && p->axis_cond[c];
+    if ((kind == BWD_XTT_RAW || kind == BWD_XTT_MLEM || kind == BWD_XTT_ART) && p->d_orient) {
+        a.coef = p->d_orient;  // backwardOrth: per-voxel rotation, general path
+        axis = false;
+    }
     return timed(ctx, 1, [&] { return launch_backward(a, kind, axis, ctx->stream); });
 }
 
@@ -721,6 +730,11 @@ int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int 
         const double s2 = (double)p->basis[3 * k] * p->basis[3 * k] + (double)p->basis[3 * k + 1] * p->basis[3 * k + 1] +
                           (double)p->basis[3 * k + 2] * p->basis[3 * k + 2];
         p->xtt_wsum += (float)(s2 * g2max);
+        double gg = 0.0;
+        for (size_t i = 0; i < (size_t)C * P; i++)
+            gg = std::max(gg, (double)vg[i].G[0] * vg[i].G[0] + (double)vg[i].G[1] * vg[i].G[1] +
+                                  (double)vg[i].G[2] * vg[i].G[2]);
+        p->xtt_wsum_orth += (float)(s2 * s2 * gg);
     }
     for (int k = 0; k < s->n_basis; k++) {
         for (int i = 0; i < 3; i++) p->bas.S[k][i] = p->basis[3 * k + i];
@@ -762,6 +776,46 @@ int xr_volume_to_columns(xr_context* ctx, const float* ref_dev, float* col_dev, 
 int xr_volume_from_columns(xr_context* ctx, const float* col_dev, float* ref_dev, int n_vox, int channels) {
     if (!ctx || !ref_dev || !col_dev || n_vox <= 0 || channels <= 0) return fail(ctx, XR_ERR_INVALID, "bad args");
     return timed(ctx, 3, [&] { return launch_from_internal(col_dev, ref_dev, n_vox, channels, ctx->stream); });
+}
+
+int xr_plan_set_orientation(xr_plan* plan, const float* coef) {
+    if (!plan) return XR_ERR_INVALID;
+    if (coef && plan->n_basis <= 0) return fail(plan->ctx, XR_ERR_INVALID, "orientation needs a plan with basis vectors");
+    plan->d_orient = coef;
+    return XR_OK;
+}
+
+static Basis basis3(const float* basis9) {
+    Basis b{};
+    for (int k = 0; k < 3; k++)
+        for (int i = 0; i < 3; i++) b.S[k][i] = basis9[3 * k + i];
+    return b;
+}
+
+int xr_orth_normal(xr_context* ctx, const float* basis9, const float* vol, int64_t nvox, float* coef, float* loss,
+                   int rot, const float* est) {
+    if (!ctx || !basis9 || !vol || !coef || !loss || nvox < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    const Basis b = basis3(basis9);
+    return timed(ctx, 3, [&] { return launch_orth_normal(vol, coef, loss, nvox, rot, est, b, ctx->stream); });
+}
+
+int xr_orth_update_estimate(xr_context* ctx, const float* basis9, const float* vol, int64_t nvox, float* est,
+                            int iter) {
+    if (!ctx || !basis9 || !vol || !est || nvox < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    const Basis b = basis3(basis9);
+    return timed(ctx, 3, [&] { return launch_orth_update_est(vol, est, nvox, iter, b, ctx->stream); });
+}
+
+int xr_orth_md(xr_context* ctx, const float* vol, const float* coef, int64_t nvox, float* md) {
+    if (!ctx || !vol || !coef || !md || nvox < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
+    return timed(ctx, 3, [&] { return launch_orth_md(vol, coef, md, nvox, ctx->stream); });
+}
+
+int xr_orth_mean_filter(xr_context* ctx, const float* coef_src, const float* vol, int n_vox, float weight,
+                        float* coef_dst) {
+    if (!ctx || !coef_src || !vol || !coef_dst || n_vox <= 0 || coef_src == coef_dst)
+        return fail(ctx, XR_ERR_INVALID, "bad args (the filter needs distinct source and destination)");
+    return timed(ctx, 3, [&] { return launch_orth_mean_filter(coef_src, vol, n_vox, weight, coef_dst, ctx->stream); });
 }
 
 int xr_volume_downsample(xr_context* ctx, const float* fine, int n_vox, int factor, float* coarse) {
@@ -1254,6 +1308,152 @@ int xr_xtt_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, 
         }
         XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
+    return XR_OK;
+}
+
+// XTT::orthTwiceReconstruct (reconstruct.cu:956-1239).  Per outer iteration
+// five passes: pass 0 runs iter2 epochs of the orientation-aware operator from
+// the stored orientation and keeps the result as the volume; passes 1-4 first
+// re-estimate the orientation from that volume with sign choice pass-1
+// (calcNormalVectorThreeDirec), reset channel 0 to 0.1 (fillVolume touches
+// channel 0 only), run iter2/4 epochs and record the best normal per voxel
+// (updateEstimationByCoef).  Every pass ends with voxelSqrt, the main
+// directions from the kept volume and stored orientation (convertNormVector)
+// and the device orientation reset to the stored one.  After the passes the
+// orientation is re-estimated with the recorded sign choices
+// (calcNormalVectorThreeDirecWithEst) and stored.  One std::mt19937 shuffles
+// the subset order for the whole call (seeded here: the reference seeds it
+// from std::random_device); the estimate starts at zero (the reference leaves
+// it uninitialised).
+int xr_orth_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, float* const* sino, float* const* vol,
+                        float* const* md, int iter1, int iter2, int batch, int direction, int method, float lambda,
+                        uint32_t seed, const char* volume_path, const char* direction_path) {
+    if (!ctx) return XR_ERR_INVALID;
+    XR_TRY(check_geometry(ctx, g));
+    XR_TRY(check_scan(ctx, s));
+    if (s->n_basis != 3) return fail(ctx, XR_ERR_INVALID, "the orientation-aware model has three basis vectors");
+    if (!sino || !vol || !md) return fail(ctx, XR_ERR_INVALID, "sinogram / volume / direction arrays missing");
+    if (iter1 < 0 || iter2 < 0 || batch <= 0 || batch > g->n_proj) return fail(ctx, XR_ERR_INVALID, "bad iterations / batch");
+    if (method != XR_METHOD_MLEM && method != XR_METHOD_ART) return fail(ctx, XR_ERR_INVALID, "method must be MLEM or ART");
+    xr_plan* plan = nullptr;
+    XR_TRY(xr_plan_create(ctx, g, s, XR_CONV_IR, direction, &plan));
+    std::unique_ptr<xr_plan, void (*)(xr_plan*)> guard(plan, xr_plan_destroy);
+    XR_TRY(ensure_norm(plan));
+    const int N = g->n_vox, P = g->n_proj, C = s->n_cond;
+    const int64_t N3 = (int64_t)N * N * N, lenD = (int64_t)g->n_det_u * g->n_det_v * P;
+    const int ss = (P + batch - 1) / batch;
+    DevBuf<float> dref, dvol, dkept, dsino, dproj, dcoef, dstore, dest, dloss, dmd;
+    DevBuf<int32_t> dcodes;
+    DevBuf<uint8_t> dflags;
+    XR_HIP(ctx, dref.alloc(N3 * 3));
+    XR_HIP(ctx, dvol.alloc(N3 * 3));
+    XR_HIP(ctx, dkept.alloc(N3 * 3));
+    XR_HIP(ctx, dsino.alloc(lenD * C));
+    XR_HIP(ctx, dproj.alloc(lenD * C));
+    XR_HIP(ctx, dcoef.alloc(N3 * 2));
+    XR_HIP(ctx, dstore.alloc(N3 * 2));
+    XR_HIP(ctx, dest.alloc(N3 * 2));
+    XR_HIP(ctx, dloss.alloc(N3));
+    XR_HIP(ctx, dmd.alloc(N3 * 3));
+    const size_t per_epoch = (size_t)batch * C * ss;
+    XR_HIP(ctx, dcodes.alloc(per_epoch));
+    XR_HIP(ctx, dflags.alloc(per_epoch));
+    for (int k = 0; k < 3; k++)
+        XR_HIP(ctx, hipMemcpyAsync(dref.p + k * N3, vol[k], sizeof(float) * N3, hipMemcpyHostToDevice, ctx->stream));
+    XR_TRY(xr_volume_to_columns(ctx, dref.p, dvol.p, N, 3));
+    XR_HIP(ctx, hipMemcpyAsync(dkept.p, dvol.p, sizeof(float) * N3 * 3, hipMemcpyDeviceToDevice, ctx->stream));
+    for (int c = 0; c < C; c++)
+        XR_HIP(ctx, hipMemcpyAsync(dsino.p + c * lenD, sino[c], sizeof(float) * lenD, hipMemcpyHostToDevice, ctx->stream));
+    // stored orientation (phi 0, cos theta 1: no rotation) and the estimate
+    XR_HIP(ctx, launch_fill(dstore.p, N3, 0.0f, ctx->stream));
+    XR_HIP(ctx, launch_fill(dstore.p + N3, N3, 1.0f, ctx->stream));
+    XR_HIP(ctx, hipMemcpyAsync(dcoef.p, dstore.p, sizeof(float) * N3 * 2, hipMemcpyDeviceToDevice, ctx->stream));
+    XR_HIP(ctx, launch_fill(dest.p, N3 * 2, 0.0f, ctx->stream));
+    XR_TRY(xr_plan_set_orientation(plan, dcoef.p));
+    const float alpha = (method == XR_METHOD_ART) ? lambda / (float)ss : 0.f;
+    const int kind = (method == XR_METHOD_MLEM) ? BWD_XTT_MLEM : BWD_XTT_ART;
+    const int out_mode = (method == XR_METHOD_MLEM) ? OUT_RATIO : OUT_RESIDUAL;
+    std::mt19937 engine(seed);
+    std::vector<int> order(batch), sv;
+    for (int i = 0; i < batch; i++) order[i] = i;
+    std::vector<int32_t> codes(per_epoch);
+    std::vector<uint8_t> flags(per_epoch);
+    auto save = [&](const char* base, int outer, const float* dev_cols, int ch) -> int {
+        // raw float32, reference layout: base + "_sequence<outer>d<i>_NxNxN.raw" (reconstruct.cu:1203-1216)
+        XR_TRY(xr_volume_from_columns(ctx, dev_cols, dref.p, N, ch));
+        std::vector<float> h(N3);
+        for (int i = 0; i < ch; i++) {
+            XR_HIP(ctx, hipMemcpyAsync(h.data(), dref.p + i * N3, sizeof(float) * N3, hipMemcpyDeviceToHost, ctx->stream));
+            XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            char name[4096];
+            snprintf(name, sizeof(name), "%s_sequence%dd%d_%dx%dx%d.raw", base, outer + 1, i + 1, N, N, N);
+            FILE* f = fopen(name, "wb");
+            if (!f) return fail(ctx, XR_ERR_IO, "cannot write %s", name);
+            const size_t w = fwrite(h.data(), sizeof(float), (size_t)N3, f);
+            fclose(f);
+            if (w != (size_t)N3) return fail(ctx, XR_ERR_IO, "short write %s", name);
+        }
+        return XR_OK;
+    };
+    for (int outer = 0; outer < iter1; outer++) {
+        for (int ep1 = 0; ep1 < 5; ep1++) {
+            if (ep1 != 0) {
+                XR_HIP(ctx, hipMemcpyAsync(dvol.p, dkept.p, sizeof(float) * N3 * 3, hipMemcpyDeviceToDevice, ctx->stream));
+                XR_TRY(xr_orth_normal(ctx, s->basis, dvol.p, N3, dcoef.p, dloss.p, ep1 - 1, nullptr));
+            }
+            XR_HIP(ctx, launch_fill(dvol.p, N3, 0.1f, ctx->stream));  // channel 0 only
+            const int epochs = (ep1 == 0) ? iter2 : iter2 / 4;
+            for (int ep2 = 0; ep2 < epochs; ep2++) {
+                std::shuffle(order.begin(), order.end(), engine);
+                std::vector<char> visited((size_t)C * P, 0);
+                for (int pos = 0; pos < batch; pos++) {
+                    subset_views(order[pos], batch, P, sv);
+                    for (int c = 0; c < C; c++)
+                        for (int j = 0; j < ss; j++) {
+                            const size_t i = (size_t)pos * C * ss + (size_t)c * ss + j;
+                            const int code = c * P + sv[j];
+                            codes[i] = code;
+                            flags[i] = visited[code];
+                            visited[code] = 1;
+                        }
+                }
+                XR_HIP(ctx, hipMemcpyAsync(dcodes.p, codes.data(), codes.size() * sizeof(int32_t),
+                                           hipMemcpyHostToDevice, ctx->stream));
+                XR_HIP(ctx, hipMemcpyAsync(dflags.p, flags.data(), flags.size(), hipMemcpyHostToDevice, ctx->stream));
+                for (int pos = 0; pos < batch; pos++) {
+                    const size_t off = (size_t)pos * C * ss;
+                    XR_TRY(forward_launch(plan, FWD_XTT, dvol.p, 0, dproj.p, dsino.p, 0, dcodes.p + off, dflags.p + off,
+                                          C * ss, out_mode, ctx->d_loss));
+                    XR_TRY(backward_launch(plan, kind, dproj.p, 0, dcodes.p + off, C * ss, dvol.p, nullptr, nullptr,
+                                           alpha, 0));
+                }
+                // the host copy of the subset lists is rewritten next epoch
+                XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            }
+            XR_TRY(xr_volume_sqrt(ctx, dvol.p, N3 * 3));
+            if (ep1 != 0)
+                XR_TRY(xr_orth_update_estimate(ctx, s->basis, dvol.p, N3, dest.p, ep1 - 1));
+            else
+                XR_HIP(ctx, hipMemcpyAsync(dkept.p, dvol.p, sizeof(float) * N3 * 3, hipMemcpyDeviceToDevice, ctx->stream));
+            XR_TRY(xr_orth_md(ctx, dkept.p, dstore.p, N3, dmd.p));
+            XR_HIP(ctx, hipMemcpyAsync(dcoef.p, dstore.p, sizeof(float) * N3 * 2, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        XR_HIP(ctx, hipMemcpyAsync(dvol.p, dkept.p, sizeof(float) * N3 * 3, hipMemcpyDeviceToDevice, ctx->stream));
+        XR_TRY(xr_orth_normal(ctx, s->basis, dvol.p, N3, dcoef.p, dloss.p, 0, dest.p));
+        XR_HIP(ctx, hipMemcpyAsync(dstore.p, dcoef.p, sizeof(float) * N3 * 2, hipMemcpyDeviceToDevice, ctx->stream));
+        XR_TRY(xr_orth_md(ctx, dkept.p, dstore.p, N3, dmd.p));
+        if (direction_path) XR_TRY(save(direction_path, outer, dmd.p, 3));
+        if (volume_path) XR_TRY(save(volume_path, outer, dkept.p, 3));
+    }
+    XR_TRY(xr_plan_set_orientation(plan, nullptr));
+    // results: the kept volume, its main directions, the final ratio / residual images
+    XR_TRY(xr_volume_from_columns(ctx, dkept.p, dref.p, N, 3));
+    for (int k = 0; k < 3; k++)
+        XR_HIP(ctx, hipMemcpyAsync(vol[k], dref.p + k * N3, sizeof(float) * N3, hipMemcpyDeviceToHost, ctx->stream));
+    XR_TRY(xr_volume_from_columns(ctx, dmd.p, dref.p, N, 3));
+    for (int k = 0; k < 3; k++)
+        XR_HIP(ctx, hipMemcpyAsync(md[k], dref.p + k * N3, sizeof(float) * N3, hipMemcpyDeviceToHost, ctx->stream));
+    XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return XR_OK;
 }
 
