@@ -31,6 +31,7 @@
 #include <iostream>
 #include <memory>
 #include <stdexcept>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -317,6 +318,35 @@ inline void reconstruct(Volume<float>* sinogram, Volume<float>* voxel, Volume<fl
                                         method == Method::ART ? XR_METHOD_ART : XR_METHOD_MLEM, lambda, &stats),
                      "XTT::reconstruct");
     xr_detail::write_loss(losses, (double)g.n_det_u * g.n_det_v * g.n_proj);
+}
+
+// XTT::orthTwiceReconstruct (reconstruct.cu:956-1239): the orientation-aware
+// three-direction model.  voxel[3] is the initial guess in and the kept volume
+// out, md[3] the main directions; per outer iteration the directions and
+// volumes are written to DIRECTION_PATH / VOLUME_PATH + "_sequence<outer>d<i>
+// _NxNxN.raw" as the reference does.  The subset order is shuffled by a
+// std::mt19937 seeded from std::random_device, as in the reference (pass a
+// seed to xr_orth_reconstruct for a reproducible run).
+inline void orthTwiceReconstruct(Volume<float>* sinogram, Volume<float> voxel[3], Volume<float> md[3],
+                                 const Geometry& geom, int iter1, int iter2, int batch, Rotate dir, Method method,
+                                 float lambda) {
+    const int C = NUM_PROJ_COND;
+    if (NUM_BASIS_VECTOR != 3) throw std::runtime_error("XTT::orthTwiceReconstruct: needs three basis vectors");
+    auto st = xr_detail::scan_tables(C, 3);
+    xr_geometry g = xr_detail::geometry(geom, sinogram[0].x(), sinogram[0].y(), sinogram[0].z(), voxel[0].x());
+    std::vector<float*> s(C), v(3), m(3);
+    for (int c = 0; c < C; c++) s[c] = sinogram[c].get();
+    for (int k = 0; k < 3; k++) {
+        v[k] = voxel[k].get();
+        if (md[k].x() != voxel[0].x()) md[k] = Volume<float>(voxel[0].x(), voxel[0].y(), voxel[0].z());
+        m[k] = md[k].get();
+    }
+    std::random_device seed_gen;
+    xr_detail::check(xr_orth_reconstruct(xr_detail::context(), &g, &st.scan, s.data(), v.data(), m.data(), iter1,
+                                         iter2, batch, dir == Rotate::CW ? XR_CW : XR_CCW,
+                                         method == Method::ART ? XR_METHOD_ART : XR_METHOD_MLEM, lambda, seed_gen(),
+                                         VOLUME_PATH.c_str(), DIRECTION_PATH.c_str()),
+                     "XTT::orthTwiceReconstruct");
 }
 }  // namespace XTT
 

@@ -30,6 +30,7 @@
 //                                                                         (orc_orth_normal, orc_orth_update_estimate)
 //   reconstruct/src/ir.cu:1231-1303   meanFiltFiber                       (orc_orth_mean_filter)
 //   reconstruct/src/ir.cu:1448-1480   convertNormVector                   (orc_orth_md)
+//   reconstruct/src/reconstruct.cu:956-1239   XTT::orthTwiceReconstruct  (orc_orth_reconstruct)
 //   reconstruct/src/ir.cu:1852-1902   rayCasting                          (raycast)
 //   reconstruct/src/fdk.cu:13-43      calcWeight / projConv               (orc_fdk_*)
 //   reconstruct/src/fdk.cu:62-170     FDK backwardonDevice                (orc_fdk_backproject_view)
@@ -844,6 +845,94 @@ template <class T> void orth_mean_filter(int N, const float* src, const float* v
             }
 }
 
+// One epoch of the orientation-aware operator over every subset (the subset
+// loop of orthTwiceReconstruct, reconstruct.cu:1076-1125): forwardOrth +
+// normalisation + ratio per (condition, view), backwardOrth, the update.
+template <class T>
+void orth_epoch(const orc_geometry& g, const orc_scan& s, const float* sino, const float* coef, int dir, int art,
+                float lambda, const std::vector<int>& order, std::vector<T>& vol) {
+    const int U = g.n_det_u, V = g.n_det_v, P = g.n_proj, C = s.n_cond, batch = (int)order.size();
+    const int64_t lenV = (int64_t)g.n_vox * g.n_vox * g.n_vox, lenP = (int64_t)U * V, lenD = lenP * P;
+    const int rotation = dir == 0 ? 1 : -1;
+    const int subsetSize = (P + batch - 1) / batch;
+    std::vector<T> proj(lenD * C, (T)0), factorP(lenP), tmp(lenV * 3), factorV(lenV * 3);
+    for (int sub : order) {
+        for (int cond = 0; cond < C; cond++)
+            for (int so = 0; so < subsetSize; so++) {
+                std::fill(factorP.begin(), factorP.end(), (T)0);
+                const int n = rotation * ((sub + batch * so) % P);
+                T* pv = &proj[cond * lenD + (int64_t)std::abs(n) * lenP];
+                forward_raw<T, T>(g, s, cond, n, vol.data(), 1, pv, factorP.data(), coef);
+                normalise_view(g, make_frame<T>(g, s, cond, n, 0), pv, factorP.data());
+                ratio_step(lenP, sino + cond * lenD + (int64_t)std::abs(n) * lenP, pv, art);
+            }
+        std::fill(tmp.begin(), tmp.end(), (T)0);
+        std::fill(factorV.begin(), factorV.end(), (T)0);
+        for (int cond = 0; cond < C; cond++)
+            for (int so = 0; so < subsetSize; so++) {
+                const int n = rotation * ((sub + batch * so) % P);
+                backward_raw<T>(g, s, cond, n, &proj[cond * lenD + (int64_t)std::abs(n) * lenP], 1, tmp.data(),
+                                factorV.data(), coef);
+            }
+        update_step<T>(lenV * 3, vol.data(), tmp.data(), factorV.data(), art, (T)(lambda / (float)subsetSize));
+    }
+}
+
+// XTT::orthTwiceReconstruct (reconstruct.cu:956-1239) with a seeded engine and
+// a zeroed estimate (see the library's xr_orth_reconstruct for the pass order).
+template <class T>
+void orth_reconstruct(const orc_geometry& g, const orc_scan& s, const float* sino, const float* vol_in, int iter1,
+                      int iter2, int batch, int dir, int art, float lambda, uint32_t seed, double* vol_out,
+                      double* md_out) {
+    const int64_t lenV = (int64_t)g.n_vox * g.n_vox * g.n_vox;
+    std::vector<T> vol(vol_in, vol_in + lenV * 3), kept(vol);
+    std::vector<float> store(lenV * 2), coef(lenV * 2), est(lenV * 2, 0.f), loss(lenV), md(lenV * 3), tmpf(lenV * 3);
+    for (int64_t i = 0; i < lenV; i++) {
+        store[i] = 0.f;
+        store[i + lenV] = 1.f;
+    }
+    coef = store;
+    std::mt19937 engine(seed);
+    std::vector<int> order(batch);
+    for (int i = 0; i < batch; i++) order[i] = i;
+    auto as_float = [&](const std::vector<T>& v) {
+        for (int64_t i = 0; i < lenV * 3; i++) tmpf[i] = (float)v[i];
+        return tmpf.data();
+    };
+    for (int outer = 0; outer < iter1; outer++) {
+        for (int ep1 = 0; ep1 < 5; ep1++) {
+            if (ep1 != 0) {
+                vol = kept;
+                orth_normal<T>(s, lenV, as_float(vol), coef.data(), loss.data(), ep1 - 1, nullptr);
+            }
+            for (int64_t i = 0; i < lenV; i++) vol[i] = (T)0.1f;  // fillVolume: channel 0
+            const int epochs = (ep1 == 0) ? iter2 : iter2 / 4;
+            for (int ep2 = 0; ep2 < epochs; ep2++) {
+                std::shuffle(order.begin(), order.end(), engine);
+                orth_epoch<T>(g, s, sino, coef.data(), dir, art, lambda, order, vol);
+            }
+            for (int64_t i = 0; i < lenV * 3; i++) {
+                const T m = vol[i];
+                vol[i] = (m < (T)0) ? std::sqrt(-m) : std::sqrt(m);  // voxelSqrt
+            }
+            if (ep1 != 0)
+                orth_update_estimate<T>(s, lenV, as_float(vol), est.data(), ep1 - 1);
+            else
+                kept = vol;
+            orth_md<T>(lenV, as_float(kept), store.data(), md.data());
+            coef = store;
+        }
+        vol = kept;
+        orth_normal<T>(s, lenV, as_float(vol), coef.data(), loss.data(), 0, est.data());
+        store = coef;
+        orth_md<T>(lenV, as_float(kept), store.data(), md.data());
+    }
+    for (int64_t i = 0; i < lenV * 3; i++) {
+        vol_out[i] = (double)kept[i];
+        md_out[i] = (double)md[i];
+    }
+}
+
 extern "C" {
 
 // Reference-preset voxel size (geometry.h:13): sod * detSize / sdd in FP32.
@@ -957,6 +1046,17 @@ void orc_orth_mean_filter(int prec, int N, const float* src, const float* vol, f
         orth_mean_filter<float>(N, src, vol, w, dst);
     else
         orth_mean_filter<double>(N, src, vol, w, dst);
+}
+
+// f3 XTT::orthTwiceReconstruct.  sino: n_cond*U*V*P floats; vol_in: 3*N^3
+// floats (reference layout); vol_out, md_out: 3*N^3 doubles.
+void orc_orth_reconstruct(int prec, const orc_geometry* g, const orc_scan* s, const float* sino, const float* vol_in,
+                          int iter1, int iter2, int batch, int dir, int art, float lambda, uint32_t seed,
+                          double* vol_out, double* md_out) {
+    if (prec == 0)
+        orth_reconstruct<float>(*g, *s, sino, vol_in, iter1, iter2, batch, dir, art, lambda, seed, vol_out, md_out);
+    else
+        orth_reconstruct<double>(*g, *s, sino, vol_in, iter1, iter2, batch, dir, art, lambda, seed, vol_out, md_out);
 }
 
 // rodriguesRotation (ir.cu:597-621) in FP64: R (9 doubles, row major).

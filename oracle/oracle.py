@@ -53,6 +53,7 @@ def _load():
         "orc_orth_update_estimate": (None, [I, S, ctypes.c_int64, _f, _f, I]),
         "orc_orth_md": (None, [I, ctypes.c_int64, _f, _f, _f]),
         "orc_orth_mean_filter": (None, [I, I, _f, _f, F, _f]),
+        "orc_orth_reconstruct": (None, [I, G, S, _f, _f, I, I, I, I, I, F, ctypes.c_uint32, _d, _d]),
         "orc_subset_orders": (None, [I, I, I, _i32]),
         "orc_ratio_step": (ctypes.c_double, [I, ctypes.c_int64, _f, _d, I]),
         "orc_ir_reconstruct": (None, [I, G, S, _f, _f, I, I, I, I, F, _d, _d]),
@@ -193,6 +194,16 @@ class Case:
         e = np.array(est, dtype=np.float32, copy=True)
         _lib.orc_orth_update_estimate(prec, self.s, vol.size // 3, _fp(vol), _fp(e), it)
         return e
+
+    def orth_reconstruct(self, sino, vol0, iter1, iter2, batch, direction=0, art=False, lam=1e-2, seed=0, prec=F64):
+        """XTT::orthTwiceReconstruct: (kept volume, main directions), 3 channels each."""
+        sino = np.ascontiguousarray(sino, dtype=np.float32)
+        vol0 = np.ascontiguousarray(vol0, dtype=np.float32)
+        vol = np.zeros(vol0.size)
+        md = np.zeros(vol0.size)
+        _lib.orc_orth_reconstruct(prec, self.g, self.s, _fp(sino), _fp(vol0), iter1, iter2, batch, direction, int(art),
+                                  lam, seed, _dp(vol), _dp(md))
+        return vol.reshape(vol0.shape), md.reshape(vol0.shape)
 
     def ir_reconstruct(self, sino, vol0, epochs, batch, direction=0, art=False, lam=1e-2, prec=F64):
         sino = np.ascontiguousarray(sino, dtype=np.float32)

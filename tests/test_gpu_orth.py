@@ -144,3 +144,46 @@ def test_orth_mean_filter_matches_oracle(orc, xr, ctx, torch_dev):
     edge = np.ones((N, N, N), bool)
     edge[inner] = False
     assert np.array_equal(got[:, edge], src[:, edge])
+
+
+@pytest.mark.parametrize("method", ["mlem", "art"])
+def test_orth_reconstruct_matches_oracle(orc, xr, ctx, method, tmp_path):
+    """XTT::orthTwiceReconstruct (reconstruct.cu:956-1239) end to end at a small
+    size: one outer iteration of five passes (4 epochs, then four of 1), two
+    grating orientations, against the FP64 oracle with the same seed; the
+    per-outer raw files carry the returned volume and directions."""
+    N, U, P = 16, 28, 8
+    st = Setup(orc, xr, N=N, U=U, P=P, vs=0.6, cond_R=np.tile(np.eye(3).reshape(-1), 2), basis=BASIS3,
+               grating=grating([0, 90]))
+    rng = np.random.default_rng(41)
+    truth = (rng.random((3, N, N, N)) * 0.02 + 0.005).astype(np.float32)
+    sino = np.stack([np.stack([st.case.forward_view(truth, n, cond=c, xtt=True)[0] for n in range(P)])
+                     for c in range(2)]).astype(np.float32)
+    vol0 = np.full((3, N, N, N), 0.01, np.float32)
+    art = method == "art"
+    want_v, want_md = st.case.orth_reconstruct(sino, vol0, 1, 4, 2, art=art, lam=1e-3, seed=7)
+    vol = [vol0[k].reshape(-1).copy() for k in range(3)]
+    md = [np.zeros(N ** 3, np.float32) for _ in range(3)]
+    base = str(tmp_path / "orth")
+    ctx.orth_reconstruct(st.geom, st.scan, [s.reshape(-1).copy() for s in sino], vol, md, 1, 4, 2,
+                         method=xr.METHOD_ART if art else xr.METHOD_MLEM, lam=1e-3, seed=7,
+                         volume_path=base + "_vol", direction_path=base + "_md")
+    ev = rel_l2(np.stack(vol), want_v.reshape(3, -1))
+    record("orth_reconstruct_%s_volume" % method, ev)
+    assert ev <= 1e-4
+    # Directions: a voxel whose three-direction normal is ill-conditioned (the
+    # two difference vectors nearly parallel, or its z component near zero,
+    # where the sign flip decides) may come out flipped or turned between FP32
+    # and FP64 -- ART's signed iterates make such voxels more common.  Those
+    # stay a small fraction; every other voxel agrees to 1e-4.
+    got_md, w_md = np.stack(md).astype(np.float64), want_md.reshape(3, -1)
+    dev = np.linalg.norm(got_md - w_md, axis=0) / np.maximum(np.linalg.norm(w_md, axis=0), 1e-12)
+    good = dev <= 1e-4
+    record("orth_reconstruct_%s_md_voxels_off" % method, 1.0 - good.mean())
+    assert good.mean() >= 0.99
+    assert rel_l2(got_md[:, good], w_md[:, good]) <= 1e-4
+    for i in range(3):
+        f = np.fromfile(base + "_vol_sequence1d%d_%dx%dx%d.raw" % (i + 1, N, N, N), np.float32)
+        assert np.array_equal(f, vol[i])
+        f = np.fromfile(base + "_md_sequence1d%d_%dx%dx%d.raw" % (i + 1, N, N, N), np.float32)
+        assert np.array_equal(f, md[i])

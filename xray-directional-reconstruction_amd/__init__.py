@@ -114,6 +114,9 @@ def _load():
         "xr_volume_sqrt": (I, [P, P, ctypes.c_int64]),
         "xr_volume_downsample": (I, [P, P, I, I, P]),
         "xr_orth_normal": (I, [P, _c_float_p, P, ctypes.c_int64, P, P, I, P]),
+        "xr_orth_reconstruct": (I, [P, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct),
+                                    ctypes.POINTER(_c_float_p), ctypes.POINTER(_c_float_p), ctypes.POINTER(_c_float_p),
+                                    I, I, I, I, I, ctypes.c_float, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_char_p]),
         "xr_orth_update_estimate": (I, [P, _c_float_p, P, ctypes.c_int64, P, I]),
         "xr_orth_md": (I, [P, P, P, ctypes.c_int64, P]),
         "xr_orth_mean_filter": (I, [P, P, P, I, ctypes.c_float, P]),
@@ -422,6 +425,14 @@ class Context:
         self.check(_lib.xr_xtt_reconstruct(self.h, ctypes.byref(geom), ctypes.byref(scan.struct), sp, vp, mp, ep,
                                            epochs, batch, direction, method, lam, ctypes.byref(st)))
         return losses[:epochs]
+
+    def orth_reconstruct(self, geom, scan, sino, vol, md, iter1, iter2, batch, direction=CW, method=METHOD_MLEM,
+                         lam=1e-2, seed=0, volume_path=None, direction_path=None):
+        """XTT::orthTwiceReconstruct: sino[c], vol[3] (in/out), md[3] (out) float32 numpy arrays."""
+        enc = lambda p: None if p is None else p.encode()  # noqa: E731
+        self.check(_lib.xr_orth_reconstruct(self.h, ctypes.byref(geom), ctypes.byref(scan.struct), _ptr_list(sino),
+                                            _ptr_list(vol), _ptr_list(md), iter1, iter2, batch, direction, method, lam,
+                                            seed, enc(volume_path), enc(direction_path)))
 
     def batch(self, geoms, batch=None, direction=CW, method=METHOD_MLEM, lam=1e-2, flags=BATCH_GRAPH, scans=None):
         return Batch(self, geoms, batch, direction, method, lam, flags, scans)
