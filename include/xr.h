@@ -282,6 +282,13 @@ int xr_orth_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s,
                         float* const* md, int iter1, int iter2, int batch, int direction, int method, float lambda,
                         uint32_t seed, const char* volume_path, const char* direction_path);
 
+/* Test hook (no counterpart in the reference): after `after` further device
+ * allocations, every allocation of the library fails with
+ * hipErrorOutOfMemory (the entry point returns XR_ERR_HIP with the message
+ * in xr_last_error) until the hook is reset with a negative value.
+ * Process wide; for testing error paths from the host. */
+void xr_test_fail_allocations(int after);
+
 /* Multi-resolution helpers of the learned warm start (no counterpart in the
  * reference), single-channel column-layout volumes on the device.
  * xr_volume_downsample: coarse (edge n_vox/factor) = mean of each

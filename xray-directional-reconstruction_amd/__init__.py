@@ -152,6 +152,7 @@ def _load():
                                  ctypes.POINTER(ctypes.c_uint32)]),
         "xr_plan_set_tiles": (I, [P, ctypes.POINTER(Tiles)]),
         "xr_plan_set_orientation": (I, [P, P]),
+        "xr_test_fail_allocations": (V, [I]),
         "xr_plan_get_tiles": (I, [P, ctypes.POINTER(Tiles)]),
         "xr_autotune": (I, [P, P, P, P, ctypes.POINTER(Tiles), _c_double_p]),
         "xr_batch_create": (I, [P, I, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct), _c_int32_p, I, I, F,
@@ -751,3 +752,9 @@ def ellipsoid_params(seed, count, fov_radius, centre_frac=0.6, axis_lo=0.1, axis
     if rc != XR_OK:
         raise XRError("xr_phantom_ellipsoid_params failed (%d)" % rc)
     return out[: 16 * count].reshape(count, 16)
+
+
+def test_fail_allocations(after):
+    """Test hook: device allocations of the library fail after `after` more
+    (negative: off)."""
+    _lib.xr_test_fail_allocations(int(after))

@@ -234,18 +234,19 @@ int ensure_epochs(xr_batch* b, int epochs) {
     xr_context* ctx = b->ctx;
     if (epochs <= b->epoch_cap) return XR_OK;
     b->free_tables();
+    b->epoch_cap = 0;  // stays empty if an allocation below fails
     for (auto& p : b->pr) {
         if (p.codes) (void)hipFree(p.codes);
         if (p.flags) (void)hipFree(p.flags);
         p.codes = nullptr;
         p.flags = nullptr;
         const size_t cnt = (size_t)epochs * b->maxb * p.C * p.ss;
-        XR_HIP(ctx, hipMalloc((void**)&p.codes, cnt * sizeof(int32_t)));
-        XR_HIP(ctx, hipMalloc((void**)&p.flags, cnt));
+        XR_HIP(ctx, device_malloc((void**)&p.codes, cnt * sizeof(int32_t)));
+        XR_HIP(ctx, device_malloc((void**)&p.flags, cnt));
     }
     if (b->d_loss) (void)hipFree(b->d_loss);
     b->d_loss = nullptr;
-    XR_HIP(ctx, hipMalloc((void**)&b->d_loss, sizeof(double) * (size_t)epochs * b->n));
+    XR_HIP(ctx, device_malloc((void**)&b->d_loss, sizeof(double) * (size_t)epochs * b->n));
     b->epoch_cap = epochs;
     return XR_OK;
 }
@@ -430,7 +431,7 @@ int build_tables(xr_batch* b) {
         auto upload = [&](auto*& dst, const auto& src) -> int {
             using T = typename std::remove_reference<decltype(src)>::type::value_type;
             if (src.empty()) return XR_OK;
-            XR_HIP(ctx, hipMalloc((void**)&dst, sizeof(T) * src.size()));
+            XR_HIP(ctx, device_malloc((void**)&dst, sizeof(T) * src.size()));
             XR_HIP(ctx, hipMemcpy(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
             return XR_OK;
         };
@@ -673,9 +674,9 @@ int xr_batch_create(xr_context* ctx, int n, const xr_geometry* geoms, const xr_s
         XR_TRY(xr_plan_create(ctx, &geoms[i], s, XR_CONV_IR, direction, &p.plan));
         XR_TRY(ensure_norm(p.plan));
         const int64_t nv = p.vol_len(), lenD = (int64_t)p.U * p.V * p.P * p.C;
-        XR_HIP(ctx, hipMalloc((void**)&p.vol, sizeof(float) * nv));
-        XR_HIP(ctx, hipMalloc((void**)&p.sino, sizeof(float) * lenD));
-        XR_HIP(ctx, hipMalloc((void**)&p.proj, sizeof(float) * lenD));
+        XR_HIP(ctx, device_malloc((void**)&p.vol, sizeof(float) * nv));
+        XR_HIP(ctx, device_malloc((void**)&p.sino, sizeof(float) * lenD));
+        XR_HIP(ctx, device_malloc((void**)&p.proj, sizeof(float) * lenD));
         XR_HIP(ctx, hipMemsetAsync(p.vol, 0, sizeof(float) * nv, ctx->stream));
         XR_HIP(ctx, hipMemsetAsync(p.sino, 0, sizeof(float) * lenD, ctx->stream));
     }
@@ -687,8 +688,8 @@ int xr_batch_create(xr_context* ctx, int n, const xr_geometry* geoms, const xr_s
         const auto& c = b->pr[y];
         return (double)a.vol_len() * a.C * a.ss > (double)c.vol_len() * c.C * c.ss;
     });
-    XR_HIP(ctx, hipMalloc((void**)&b->d_vmax, sizeof(float) * n));
-    XR_HIP(ctx, hipMalloc((void**)&b->d_queue, sizeof(unsigned) * 4));
+    XR_HIP(ctx, device_malloc((void**)&b->d_vmax, sizeof(float) * n));
+    XR_HIP(ctx, device_malloc((void**)&b->d_queue, sizeof(unsigned) * 4));
     XR_HIP(ctx, hipMemsetAsync(b->d_queue, 0, sizeof(unsigned) * 4, ctx->stream));
     XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *out = b.release();

@@ -74,6 +74,12 @@ int fail(xr_context* ctx, int code, const char* fmt, ...);
         if (r_ != XR_OK) return r_; \
     } while (0)
 
+// Every device allocation of the library goes through device_malloc, so a
+// test can make allocations fail from the host (xr_test_fail_allocations)
+// and check that each entry point reports the error and leaves the library
+// usable.
+hipError_t device_malloc(void** p, size_t bytes);
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -87,7 +93,7 @@ struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         if (count == 0) return hipSuccess;
-        return hipMalloc((void**)&p, count * sizeof(T));
+        return device_malloc((void**)&p, count * sizeof(T));
     }
 };
 

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <random>
 #include <string>
@@ -61,6 +62,21 @@ unsigned char* scratch_of(int device) {
 
 namespace xr {
 namespace host {
+
+std::atomic<int> g_alloc_budget{-1};  // allocations left before forced failures, or -1 (off)
+
+hipError_t device_malloc(void** p, size_t bytes) {
+    int left = g_alloc_budget.load();
+    while (left >= 0) {
+        if (left == 0) {
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+        if (g_alloc_budget.compare_exchange_weak(left, left - 1)) break;
+    }
+    return hipMalloc(p, bytes);
+}
+
 
 int fail(xr_context* ctx, int code, const char* fmt, ...) {
     char buf[1024];
@@ -171,7 +187,8 @@ int ensure_codes(xr_context* ctx, size_t n) {
     if (n <= ctx->codes_cap) return XR_OK;
     if (ctx->d_codes) (void)hipFree(ctx->d_codes);
     ctx->d_codes = nullptr;
-    XR_HIP(ctx, hipMalloc((void**)&ctx->d_codes, n * sizeof(int32_t)));
+    ctx->codes_cap = 0;  // stays empty if the allocation fails
+    XR_HIP(ctx, device_malloc((void**)&ctx->d_codes, n * sizeof(int32_t)));
     ctx->codes_cap = n;
     return XR_OK;
 }
@@ -179,7 +196,8 @@ int ensure_flags(xr_context* ctx, size_t n) {
     if (n <= ctx->flags_cap) return XR_OK;
     if (ctx->d_flags) (void)hipFree(ctx->d_flags);
     ctx->d_flags = nullptr;
-    XR_HIP(ctx, hipMalloc((void**)&ctx->d_flags, n));
+    ctx->flags_cap = 0;
+    XR_HIP(ctx, device_malloc((void**)&ctx->d_flags, n));
     ctx->flags_cap = n;
     return XR_OK;
 }
@@ -337,13 +355,19 @@ int ensure_norm(xr_plan* p) {
     xr_context* ctx = p->ctx;
     const int P = p->g.n_proj;
     const int64_t lenD = (int64_t)p->g.n_det_u * p->g.n_det_v * P;
+    // Both tables are built into buffers of their own and handed to the plan
+    // only when complete: after a failure the plan holds neither, and the next
+    // call builds them again.
     if (!p->d_bands && p->g.n_vox <= 32767) {  // column bounds packed in 16 bits
         const int n_bands = (p->g.n_det_u + kBandU - 1) / kBandU;
-        XR_HIP(ctx, hipMalloc((void**)&p->d_bands,
-                              sizeof(int32_t) * (size_t)p->n_cond * P * n_bands * p->g.n_vox));
-        XR_HIP(ctx, launch_band_rows(p->d_vg, p->dims, p->n_cond * P, n_bands, p->d_bands, ctx->stream));
+        DevBuf<int32_t> bands;
+        XR_HIP(ctx, bands.alloc((size_t)p->n_cond * P * n_bands * p->g.n_vox));
+        XR_HIP(ctx, launch_band_rows(p->d_vg, p->dims, p->n_cond * P, n_bands, bands.p, ctx->stream));
+        p->d_bands = bands.p;
+        bands.p = nullptr;
     }
-    XR_HIP(ctx, hipMalloc((void**)&p->d_norm, sizeof(float) * lenD * p->n_cond));
+    DevBuf<float> norm;
+    XR_HIP(ctx, norm.alloc((size_t)lenD * p->n_cond));
     std::vector<int32_t> codes((size_t)p->n_cond * P);
     for (size_t i = 0; i < codes.size(); i++) codes[i] = (int32_t)i;
     DevBuf<int32_t> dc;
@@ -355,7 +379,7 @@ int ensure_norm(xr_plan* p) {
     XR_HIP(ctx, fmax.alloc(1));
     XR_HIP(ctx, hipMemsetAsync(fmax.p, 0, sizeof(float), ctx->stream));
     p->d_fmax = fmax.p;
-    const int rc = forward_launch(p, FWD_WEIGHT, nullptr, 0, p->d_norm, nullptr, 0, dc.p, nullptr, (int)codes.size(),
+    const int rc = forward_launch(p, FWD_WEIGHT, nullptr, 0, norm.p, nullptr, 0, dc.p, nullptr, (int)codes.size(),
                                   OUT_NORM_TABLE, nullptr);
     p->d_fmax = nullptr;
     XR_TRY(rc);
@@ -363,6 +387,8 @@ int ensure_norm(xr_plan* p) {
     XR_HIP(ctx, hipMemcpyAsync(&mf, fmax.p, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     p->max_factor = mf;
+    p->d_norm = norm.p;
+    norm.p = nullptr;
     return XR_OK;
 }
 
@@ -561,8 +587,8 @@ int xr_context_create(xr_context** out, int device) {
     auto* c = new (std::nothrow) xr_context();
     if (!c) return XR_ERR_ALLOC;
     c->device = device;
-    if (hipMalloc((void**)&c->d_loss, 2 * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&c->d_scal, 2 * sizeof(float)) != hipSuccess) {
+    if (device_malloc((void**)&c->d_loss, 2 * sizeof(double)) != hipSuccess ||
+        device_malloc((void**)&c->d_scal, 2 * sizeof(float)) != hipSuccess) {
         if (c->d_loss) (void)hipFree(c->d_loss);
         delete c;
         return XR_ERR_ALLOC;
@@ -742,10 +768,10 @@ int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int 
                        p->bas.S[k][2] * p->bas.S[k][2];
     }
     XR_HIP(ctx, hipSetDevice(ctx->device));
-    XR_HIP(ctx, hipMalloc((void**)&p->d_vg, vg.size() * sizeof(ViewGeom)));
-    XR_HIP(ctx, hipMalloc((void**)&p->d_cg, cg.size() * sizeof(ChordGeom)));
-    XR_HIP(ctx, hipMalloc((void**)&p->d_g2, g2.size() * sizeof(float)));
-    XR_HIP(ctx, hipMalloc((void**)&p->d_xw, xw.size() * sizeof(float)));
+    XR_HIP(ctx, device_malloc((void**)&p->d_vg, vg.size() * sizeof(ViewGeom)));
+    XR_HIP(ctx, device_malloc((void**)&p->d_cg, cg.size() * sizeof(ChordGeom)));
+    XR_HIP(ctx, device_malloc((void**)&p->d_g2, g2.size() * sizeof(float)));
+    XR_HIP(ctx, device_malloc((void**)&p->d_xw, xw.size() * sizeof(float)));
     XR_HIP(ctx, hipMemcpy(p->d_xw, xw.data(), xw.size() * sizeof(float), hipMemcpyHostToDevice));
     XR_HIP(ctx, hipMemcpy(p->d_vg, vg.data(), vg.size() * sizeof(ViewGeom), hipMemcpyHostToDevice));
     XR_HIP(ctx, hipMemcpy(p->d_cg, cg.data(), cg.size() * sizeof(ChordGeom), hipMemcpyHostToDevice));
@@ -777,6 +803,8 @@ int xr_volume_from_columns(xr_context* ctx, const float* col_dev, float* ref_dev
     if (!ctx || !ref_dev || !col_dev || n_vox <= 0 || channels <= 0) return fail(ctx, XR_ERR_INVALID, "bad args");
     return timed(ctx, 3, [&] { return launch_from_internal(col_dev, ref_dev, n_vox, channels, ctx->stream); });
 }
+
+void xr_test_fail_allocations(int after) { xr::host::g_alloc_budget.store(after < 0 ? -1 : after); }
 
 int xr_plan_set_orientation(xr_plan* plan, const float* coef) {
     if (!plan) return XR_ERR_INVALID;

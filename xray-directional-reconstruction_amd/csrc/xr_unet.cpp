@@ -57,7 +57,7 @@ struct Bf16Buf {
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
-        hipError_t e = hipMalloc((void**)&p, count * sizeof(__bf16));
+        hipError_t e = device_malloc((void**)&p, count * sizeof(__bf16));
         if (e == hipSuccess) n = count;
         return e;
     }
@@ -151,7 +151,7 @@ int ensure_buffers(xr_unet* n, int64_t V1, bool training) {
         if (n->d_ghead) (void)hipFree(n->d_ghead);
         n->d_ghead = nullptr;
         n->ghead_n = 0;
-        XR_HIP(ctx, hipMalloc((void**)&n->d_ghead, sizeof(float) * V1));
+        XR_HIP(ctx, device_malloc((void**)&n->d_ghead, sizeof(float) * V1));
         n->ghead_n = V1;
     }
     return XR_OK;
@@ -341,12 +341,12 @@ int xr_unet_create(xr_context* ctx, int base_channels, uint64_t seed, xr_unet** 
     n->n_params = off;
     for (int i = 0; i < kLayers; i++) {
         Layer& l = n->L[i];
-        XR_HIP(ctx, hipMalloc((void**)&l.wf, sizeof(__bf16) * (size_t)round_up(l.Cout, 16) * l.Kpad));
-        XR_HIP(ctx, hipMalloc((void**)&l.wd, sizeof(__bf16) * (size_t)round_up(l.Cin, 16) * l.Kdpad));
-        XR_HIP(ctx, hipMalloc((void**)&l.wt, sizeof(__bf16) * (size_t)round_up(l.Cout, 16) * l.Ktpad));
+        XR_HIP(ctx, device_malloc((void**)&l.wf, sizeof(__bf16) * (size_t)round_up(l.Cout, 16) * l.Kpad));
+        XR_HIP(ctx, device_malloc((void**)&l.wd, sizeof(__bf16) * (size_t)round_up(l.Cin, 16) * l.Kdpad));
+        XR_HIP(ctx, device_malloc((void**)&l.wt, sizeof(__bf16) * (size_t)round_up(l.Cout, 16) * l.Ktpad));
     }
-    for (float** p : {&n->d_p, &n->d_g, &n->d_m, &n->d_v}) XR_HIP(ctx, hipMalloc((void**)p, sizeof(float) * off));
-    XR_HIP(ctx, hipMalloc((void**)&n->d_red, sizeof(double) * 2));
+    for (float** p : {&n->d_p, &n->d_g, &n->d_m, &n->d_v}) XR_HIP(ctx, device_malloc((void**)p, sizeof(float) * off));
+    XR_HIP(ctx, device_malloc((void**)&n->d_red, sizeof(double) * 2));
     // He-normal weights from a Philox stream of `seed` (Box-Muller), zero
     // biases; layer 0 only for its real input channel; zero head.
     std::vector<float> h(off, 0.0f);
