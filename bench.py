@@ -395,6 +395,56 @@ def fdk_art_measure(ctx, xr, geom, sino, art_iters=50, lam=1e-3):
                     "loss_first_last": [losses[0], losses[-1]]}}
 
 
+def grad_measure(ctx, xr, geom, sino, lam=1e-3):
+    """The two GPU stages of IR::gradReconstruct (reconstruct.cu:158-328) at
+    this geometry: one OS-ART epoch (batch 1) of one gradient slot on the
+    (N+1)^3 gradient grid (XR_CONV_GRAD plan: forwardProjGrad + projSubtract,
+    backwardProjGrad + voxelPlus), and the Poisson integration of the N^3
+    result (poissonSolveLDLT, poisson_cpu.cpp:15-112) by four FP64 sine-
+    transform products per z slice: 8 (N-2)^3 flops per slice."""
+    import torch
+
+    N, U, V, P = geom.n_vox, geom.n_det_u, geom.n_det_v, geom.n_proj
+    N1 = N + 1
+    g1 = xr.make_geometry(geom.sdd, geom.sod, geom.det_size, N1, U, P, vox_size=geom.vox_size, n_det_v=V)
+    codes = np.arange(P, dtype=np.int32)
+    plan = ctx.plan(g1, xr.Scan(), convention=xr.CONV_GRAD)
+    vol = torch.full((N1 ** 3,), 0.01, device=sino.device)
+    resid = torch.empty_like(sino)
+    plan.project(vol, resid, codes, out_mode=xr.OUT_RESIDUAL, sino=sino)  # tables, warm-up
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    ctx.timing_reset()
+    plan.project(vol, resid, codes, out_mode=xr.OUT_RESIDUAL, sino=sino)
+    plan.backproject_update(resid, codes, vol, method=xr.METHOD_ART, lam=lam, subset_size=P)
+    torch.cuda.synchronize()
+    f_ms, _ = ctx.timing_read(xr.TIMING_FORWARD)
+    b_ms, _ = ctx.timing_read(xr.TIMING_BACKWARD)
+    plan.close()
+    del resid
+    gx = torch.randn(N1 ** 3, device=sino.device)
+    gy = torch.randn(N1 ** 3, device=sino.device)
+    out = torch.empty(N ** 3, device=sino.device)
+    ctx.poisson_solve(N, gx, gy, out)  # warm-up
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    ctx.poisson_solve(N, gx, gy, out)
+    call_s = time.perf_counter() - t0
+    p_ms, _ = ctx.timing_read(xr.TIMING_OTHER)
+    ctx.timing(False)
+    del gx, gy, out, vol
+    torch.cuda.empty_cache()
+    vv = float(N1) ** 3 * P
+    flops = 8.0 * (N - 2) ** 3 * N
+    return {"workload": "gradient slot on the %d^3 grid (%d views, %dx%d detector) + Poisson integration of %d^3"
+                        % (N1, P, U, V, N),
+            "slot_epoch_ms": round(f_ms + b_ms, 3), "forward_ms": round(f_ms, 3), "backward_ms": round(b_ms, 3),
+            "slot_gups": round(2.0 * vv / ((f_ms + b_ms) * 1e-3) / 1e9, 2),
+            "poisson_kernels_ms": round(p_ms, 3), "poisson_call_ms": round(call_s * 1e3, 3),
+            "poisson_tflops_f64": round(flops / (p_ms * 1e-3) / 1e12, 2),
+            "poisson_frac_f64_peak": round(flops / (p_ms * 1e-3) / 78.6e12, 4)}
+
+
 def measured_traffic(kernel, N, P):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy), if one
@@ -549,6 +599,7 @@ def main():
                                              long_epochs=args.long_epochs, pres=pres, model=args.model or MODEL)
     if not args.no_extras:
         out["fdk_art"] = fdk_art_measure(ctx, xr, geom, sino)
+        out["grad"] = grad_measure(ctx, xr, geom, sino)
         del sino
         torch.cuda.empty_cache()
         out["xtt"] = xtt_measure(xr)

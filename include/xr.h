@@ -41,6 +41,12 @@ extern "C" {
 /* Projector conventions (parity-critical, see the reference files cited). */
 #define XR_CONV_IR 0   /* source R(0,+sod,0), voxel centre x-(N+1)/2 (ir.cu:1874-1884) */
 #define XR_CONV_FDK 1  /* source R(0,-sod,0), voxel centre x-(N-1)/2 (fdk.cu:125-135) */
+/* The gradient grid of IR::gradReconstruct: a plan of side N (the reference's
+ * geom.voxel + 1) with the source at R(0,-sod,0) and voxel centre x - N/2
+ * (forwardProjGrad / backwardProjGrad, ir.cu:447-462).  Its splat is
+ * vox_size*ratio*mu (ir.cu:489-491), XR_OUT_RAW included, with no chord
+ * normalisation: XR_OUT_NORMALISED equals XR_OUT_RAW.  Scalar volumes only. */
+#define XR_CONV_GRAD 2
 
 /* Rotate (reconstruct.cuh:19-24): CW -> view n at +2 pi n/P, CCW -> -2 pi n/P. */
 #define XR_CW 0
@@ -354,6 +360,36 @@ int xr_xtt_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, 
  * is overwritten with the filtered sinogram, as in the reference. */
 int xr_fdk_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, float* const* sino, float* const* vol,
                        int direction, int filter);
+
+/* IR::gradReconstruct (reconstruct.cu:158-328), gradient-domain phase
+ * contrast.  g: the result's geometry (n_vox = N); sino: ONE U*V*P host
+ * sinogram (the reference copies sinogram[0] into every slot, :198-199).  For
+ * each condition c and d in {x, y, z}, slot 3c+d reconstructs an (N+1)^3
+ * gradient volume (initial value 0.01) by OS-ART (or OS-EM) from the sinogram
+ * scaled per view by {cos, sin, 1}[d] of 2 pi n/P (sinogramGradientCoef,
+ * ir.cu:585-594) with the XR_CONV_GRAD operator; all slots share the subset
+ * order of one default-seeded mt19937_64 (:221, 226).  The x and y slots of
+ * condition 0 are then integrated per z slice by xr_poisson_solve into vol
+ * (N^3, host, reference layout).  block_size: the reference's blockSize
+ * (settings recon_variable.blockSize): its launches cover x, z <
+ * min(N+1, ceil(N/blockSize)*blockSize) only (:203-204), and the layers beyond
+ * keep 0.01.  grad: NULL, or 3*n_cond host arrays of (N+1)^3 floats that
+ * receive the gradient volumes.  stats->losses: 3*n_cond*epochs sums of |y - p|,
+ * slot-major.  CCW: the reference's forwardProjGrad writes view images at the
+ * signed index (out of its buffer, ir.cu:496-499) and backwardProjGrad turns
+ * by |n| (ir.cu:516); here both use the signed angle and image |n|. */
+int xr_grad_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s, const float* sino, float* vol,
+                        float* const* grad, int epochs, int batch, int direction, int method, float lambda,
+                        int block_size, xr_stats* stats);
+
+/* poissonSolveLDLT (poisson_cpu.cpp:15-112): per z slice of the n^3 result,
+ * the n*n system with identity rows on the slice border and the 5-point
+ * Laplacian inside, right side gx(x,y,z) - gx(x+1,y,z) + gy(x,y,z) - gy(x,y+1,z)
+ * (FP32, as the reference) on the interior.  gx, gy: (n+1)^3 device arrays,
+ * vol: n^3 device array, reference layouts; n >= 3.  Solved exactly by type-I
+ * sine transforms in FP64 (the reference factorises the matrix with Eigen's
+ * SparseLU in FP32), result rounded to FP32. */
+int xr_poisson_solve(xr_context* ctx, int n, const float* gx, const float* gy, float* vol);
 
 /* ---- batched multi-problem solver ---------------------------------------- */
 

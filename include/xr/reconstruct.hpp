@@ -21,6 +21,7 @@
 // Header only; C++17.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -240,6 +241,12 @@ inline std::string volume_save_path(int k) {
 inline std::string direction_save_path(int i) {
     return DIRECTION_PATH + std::to_string(i + 1) + xr_detail::raw_suffix(NUM_VOXEL, NUM_VOXEL, NUM_VOXEL);
 }
+// Gradient volume i of IR::gradReconstruct: VOLUME_PATH + "grad" + (i+1) + "_(N+1)^3.raw"
+// (reconstruct.cu:316-319).
+inline std::string grad_save_path(int i) {
+    return VOLUME_PATH + "grad" + std::to_string(i + 1) +
+           xr_detail::raw_suffix(NUM_VOXEL + 1, NUM_VOXEL + 1, NUM_VOXEL + 1);
+}
 // Angle volume i (0: phi, 1: theta): DIRECTION_PATH + "_phi_" / "_theta_" + "NxNxN.raw".
 inline std::string angle_save_path(int i) {
     const std::string n = std::to_string(NUM_VOXEL);
@@ -294,6 +301,34 @@ inline void reconstruct(Volume<float>* sinogram, Volume<float>* voxel, const Geo
     // loss.csv holds the last condition's epochs (the reference rewrites it per call)
     std::vector<double> last(losses.end() - epoch, losses.end());
     xr_detail::write_loss(last, (double)g.n_det_u * g.n_det_v * g.n_proj);
+}
+
+// IR::gradReconstruct (reconstruct.cu:158-328): gradient-domain phase contrast
+// from sinogram[0].  Like the reference it advances geom.voxel by one, writes
+// the Poisson-integrated volume to voxel[0] and saves the gradient volumes to
+// grad_save_path(i) for i < 3*min(NUM_BASIS_VECTOR, NUM_PROJ_COND) (the
+// reference's loop runs to 3*NUM_BASIS_VECTOR over 3*NUM_PROJ_COND volumes).
+// The launch extent follows BLOCK_SIZE as the reference's does.
+inline void gradReconstruct(Volume<float>* sinogram, Volume<float>* voxel, Geometry& geom, int epoch, int batch,
+                            Rotate dir, Method method, float lambda) {
+    geom.voxel = geom.voxel + 1;
+    const int C = NUM_PROJ_COND;
+    auto st = xr_detail::scan_tables(C, 0);
+    const int64_t N = voxel[0].x();
+    xr_geometry g = xr_detail::geometry(geom, sinogram[0].x(), sinogram[0].y(), sinogram[0].z(), N);
+    std::vector<Volume<float>> grad(3 * C);
+    std::vector<float*> gp(3 * C);
+    for (int i = 0; i < 3 * C; i++) {
+        grad[i] = Volume<float>(N + 1, N + 1, N + 1);
+        gp[i] = grad[i].get();
+    }
+    xr_detail::check(xr_grad_reconstruct(xr_detail::context(), &g, &st.scan, sinogram[0].get(), voxel[0].get(),
+                                         gp.data(), epoch, batch, dir == Rotate::CW ? XR_CW : XR_CCW,
+                                         method == Method::ART ? XR_METHOD_ART : XR_METHOD_MLEM, lambda, BLOCK_SIZE,
+                                         nullptr),
+                     "IR::gradReconstruct");
+    const int n_save = 3 * std::min(NUM_BASIS_VECTOR, C);
+    for (int i = 0; i < n_save; i++) grad[i].save(xr::grad_save_path(i));
 }
 }  // namespace IR
 

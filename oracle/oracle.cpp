@@ -39,6 +39,11 @@
 //   reconstruct/src/reconstruct.cu:1651-1724  FDK::reconstruct            (orc_fdk_reconstruct)
 //   reconstruct/src/reconstruct.cu:1726-1793, fdk.cu:298-375
 //                                             FDK::hilbertReconstruct     (orc_fdk_reconstruct, kind 2)
+//   reconstruct/src/ir.cu:425-594     forwardProjGrad / backwardProjGrad / sinogramGradientCoef
+//                                             (orc_grad_forward_view, orc_grad_backward_view, orc_grad_coef)
+//   reconstruct/src/reconstruct.cu:158-300    IR::gradReconstruct up to the Poisson solve (orc_grad_reconstruct);
+//                                             the Poisson solve itself (poisson_cpu.cpp:15-112) is
+//                                             oracle.poisson_solve in oracle.py
 //   reconstruct/include/geometry.h:11-14      voxSize = sod*detSize/sdd
 //   reconstruct/include/vec.h:93-109          normalize (eps 1e-8) / norm2
 //   direction/src/pca.cu:12-65                calcEigenVector             (orc_pca)
@@ -169,8 +174,12 @@ Frame<T> make_frame(const orc_geometry& g, const orc_scan& s, int cond, int n, i
 
 // Voxel centre relative to the rotation centre: ir.cu:1881-1884 (IR, (x-(N+1)/2)*vs)
 // and fdk.cu:132-135 (FDK, (x-(N-1)/2)*vs), both minus the rotated offset and t.
+// conv 2 is the gradient grid of forwardProjGrad (ir.cu:459-462): side N+1,
+// ((2(x - 0.5) - N + 1)/2)*vs with N the gradient volume's side, and the
+// source at R(0,-sod,0) (ir.cu:447) as for FDK.
 template <class T> inline T centre_coord(int i, int N, T vs, int conv) {
     if (conv == 0) return ((T)2.0f * (T)i - (T)N - (T)1.0f) * (T)0.5f * vs;
+    if (conv == 2) return ((T)2.0f * ((T)i - (T)0.5f) - (T)N + (T)1.0f) * (T)0.5f * vs;
     return ((T)2.0f * (T)i - (T)N + (T)1.0f) * (T)0.5f * vs;
 }
 
@@ -933,6 +942,135 @@ void orth_reconstruct(const orc_geometry& g, const orc_scan& s, const float* sin
     }
 }
 
+// ---- gradient-domain phase-contrast reconstruction (IR::gradReconstruct) ----
+// g here is the gradient volume's geometry: n_vox = N+1 for an N^3 result.
+// ext: the x and z extent the reference's launches cover, min(N+1, ceil(N/bs)*bs)
+// for blockSize bs (gridV is sized from the N^3 result, reconstruct.cu:203-204,
+// the kernels test against N+1, ir.cu:429).
+
+// forwardProjGrad (ir.cu:425-503): bilinear splat of vs*ratio*voxel, no chord
+// normalisation; accumulates into proj (U*V).
+template <class T>
+void grad_forward_raw(const orc_geometry& g, const orc_scan& s, int cond, int n, const T* vol, int ext, T* proj) {
+    const int N = g.n_vox, U = g.n_det_u, V = g.n_det_v;
+    const Frame<T> f = make_frame<T>(g, s, cond, n, 2);
+    const T ratio = proj_ratio<T>(g);
+    for (int y = 0; y < N; y++)
+        for (int z = 0; z < ext; z++)
+            for (int x = 0; x < ext; x++) {
+                T u, v;
+                V3<T> B;
+                raycast(g, f, x, y, z, 2, u, v, B);
+                Splat<T> w;
+                if (!splat_weights(u, v, U, V, w)) continue;
+                const int64_t idx = x + (int64_t)N * y + (int64_t)N * N * z;
+                const T pr = (T)g.vox_size * ratio * vol[idx];
+                proj[w.iu + (int64_t)U * (w.iv + 1)] += w.c1 * pr;
+                proj[(w.iu + 1) + (int64_t)U * (w.iv + 1)] += w.c2 * pr;
+                proj[(w.iu + 1) + (int64_t)U * w.iv] += w.c3 * pr;
+                proj[w.iu + (int64_t)U * w.iv] += w.c4 * pr;
+            }
+}
+
+// backwardProjGrad (ir.cu:505-583): bilinear gather, +1 per view that lands.
+// The angle is that of the signed view n, as in forwardProjGrad.  (For CCW the
+// reference's pair is inconsistent: forwardProjGrad writes image n < 0 out of
+// its slot, ir.cu:496-499, and backwardProjGrad turns with |n|, ir.cu:516; the
+// pair here follows IR::reconstruct: signed angle, image |n|.)
+template <class T>
+void grad_backward_raw(const orc_geometry& g, const orc_scan& s, int cond, int n, const T* proj, int ext, T* tmp,
+                       T* factor) {
+    const int N = g.n_vox, U = g.n_det_u, V = g.n_det_v;
+    const Frame<T> f = make_frame<T>(g, s, cond, n, 2);
+    for (int y = 0; y < N; y++)
+        for (int z = 0; z < ext; z++)
+            for (int x = 0; x < ext; x++) {
+                T u, v;
+                V3<T> B;
+                raycast(g, f, x, y, z, 2, u, v, B);
+                Splat<T> w;
+                if (!splat_weights(u, v, U, V, w)) continue;
+                const T p1 = proj[w.iu + (int64_t)U * (w.iv + 1)], p2 = proj[(w.iu + 1) + (int64_t)U * (w.iv + 1)];
+                const T p3 = proj[(w.iu + 1) + (int64_t)U * w.iv], p4 = proj[w.iu + (int64_t)U * w.iv];
+                const int64_t idx = x + (int64_t)N * y + (int64_t)N * N * z;
+                factor[idx] += (T)1.0f;
+                tmp[idx] += w.c1 * p1 + w.c2 * p2 + w.c3 * p3 + w.c4 * p4;
+            }
+}
+
+// sinogramGradientCoef (ir.cu:585-594): {cos theta, sin theta, 1}[diff].
+template <class T> T grad_coef(int n, int P, int diff) {
+    const T th = (T)theta_f32(n, P);
+    return diff == 0 ? tcos<T>(th) : diff == 1 ? tsin<T>(th) : (T)1.0f;
+}
+
+// IR::gradReconstruct (reconstruct.cu:158-300) up to the Poisson solve.  g:
+// the result's geometry (n_vox = N); sino0: one U*V*P sinogram (the reference
+// copies sinogram[0] into every slot, :173-174, 198-199).  grads_out: 3*n_cond
+// (N+1)^3 doubles, slot 3*cond + diff; losses: epochs sums of |y - p| over all
+// slots (the reference's loss1, :232-233, which it never reads back).
+template <class T>
+void grad_reconstruct(const orc_geometry& g0, const orc_scan& s, const float* sino0, int epochs, int batch, int dir,
+                      int art, float lambda, int block_size, double* grads_out, double* losses) {
+    orc_geometry g = g0;
+    g.n_vox = g0.n_vox + 1;  // geom.voxel + 1 (:161)
+    const int N1 = g.n_vox, U = g.n_det_u, V = g.n_det_v, P = g.n_proj, C = s.n_cond;
+    const int ext = std::min(N1, (g0.n_vox + block_size - 1) / block_size * block_size);
+    const int64_t lenV = (int64_t)N1 * N1 * N1, lenP = (int64_t)U * V, lenD = lenP * P;
+    const int rotation = dir == 0 ? 1 : -1;
+    const int ss = (P + batch - 1) / batch;
+    std::vector<T> vol((size_t)3 * C * lenV, (T)0.01f), sino((size_t)3 * C * lenD), proj((size_t)3 * C * lenD);
+    for (int i = 0; i < 3 * C; i++)
+        for (int n = 0; n < P; n++) {
+            const T c = grad_coef<T>(n, P, i % 3);
+            for (int64_t k = 0; k < lenP; k++) sino[i * lenD + n * lenP + k] = c * (T)sino0[n * lenP + k];
+        }
+    // the reference multiplies in FP32; the slots feed ratio_step as floats
+    std::vector<float> sino_f(sino.size());
+    for (size_t i = 0; i < sino.size(); i++) sino_f[i] = (float)sino[i];
+    std::vector<int> order(batch);
+    for (int i = 0; i < batch; i++) order[i] = i;
+    std::mt19937_64 rng;
+    std::vector<T> tmp(lenV), factor(lenV);
+    for (int ep = 0; ep < epochs; ep++) {
+        std::shuffle(order.begin(), order.end(), rng);
+        double loss = 0;
+        std::fill(proj.begin(), proj.end(), (T)0);
+        for (int sub : order) {
+            for (int i = 0; i < 3 * C; i++)
+                for (int so = 0; so < ss; so++) {
+                    const int n = rotation * ((sub + batch * so) % P);
+                    T* pv = &proj[i * lenD + (int64_t)std::abs(n) * lenP];
+                    grad_forward_raw<T>(g, s, i / 3, n, &vol[i * lenV], ext, pv);
+                    loss += ratio_step(lenP, &sino_f[i * lenD + (int64_t)std::abs(n) * lenP], pv, art);
+                }
+            for (int i = 0; i < 3 * C; i++) {
+                std::fill(tmp.begin(), tmp.end(), (T)0);
+                std::fill(factor.begin(), factor.end(), (T)0);
+                for (int so = 0; so < ss; so++) {
+                    const int n = rotation * ((sub + batch * so) % P);
+                    grad_backward_raw<T>(g, s, i / 3, n, &proj[i * lenD + (int64_t)std::abs(n) * lenP], ext,
+                                         tmp.data(), factor.data());
+                }
+                // voxelPlus / voxelProduct over the launched x, z range only
+                const T alpha = (T)(lambda / (float)ss);
+                T* v = &vol[i * lenV];
+                for (int z = 0; z < ext; z++)
+                    for (int y = 0; y < N1; y++)
+                        for (int x = 0; x < ext; x++) {
+                            const int64_t idx = x + (int64_t)N1 * y + (int64_t)N1 * N1 * z;
+                            if (art)
+                                v[idx] = v[idx] + alpha * tmp[idx];
+                            else
+                                v[idx] = (factor[idx] == (T)0) ? (T)0 : v[idx] * tmp[idx] / factor[idx];
+                        }
+            }
+        }
+        if (losses) losses[ep] = loss;
+    }
+    for (size_t i = 0; i < vol.size(); i++) grads_out[i] = (double)vol[i];
+}
+
 extern "C" {
 
 // Reference-preset voxel size (geometry.h:13): sod * detSize / sdd in FP32.
@@ -1119,6 +1257,54 @@ void orc_ir_reconstruct(int prec, const orc_geometry* g, const orc_scan* s, cons
         ir_reconstruct<float>(*g, *s, sino, vol_in, epochs, batch, dir, art, lambda, vol_out, losses);
     else
         ir_reconstruct<double>(*g, *s, sino, vol_in, epochs, batch, dir, art, lambda, vol_out, losses);
+}
+
+// f4 gradient path.  g: the gradient volume's geometry (n_vox = N+1); vol and
+// the accumulators: (N+1)^3 in the reference layout; ext as in grad_forward_raw.
+void orc_grad_forward_view(int prec, const orc_geometry* g, const orc_scan* s, int cond, int n, const float* vol,
+                           int ext, double* out_proj) {
+    const int64_t lenP = (int64_t)g->n_det_u * g->n_det_v, lenV = (int64_t)g->n_vox * g->n_vox * g->n_vox;
+    if (prec == 0) {
+        std::vector<float> p(lenP, 0.f), v(vol, vol + lenV);
+        grad_forward_raw<float>(*g, *s, cond, n, v.data(), ext, p.data());
+        for (int64_t i = 0; i < lenP; i++) out_proj[i] = p[i];
+    } else {
+        std::vector<double> v(vol, vol + lenV);
+        std::fill(out_proj, out_proj + lenP, 0.0);
+        grad_forward_raw<double>(*g, *s, cond, n, v.data(), ext, out_proj);
+    }
+}
+
+void orc_grad_backward_view(int prec, const orc_geometry* g, const orc_scan* s, int cond, int n,
+                            const float* proj_view, int ext, double* tmp, double* factor) {
+    const int64_t lenP = (int64_t)g->n_det_u * g->n_det_v, lenV = (int64_t)g->n_vox * g->n_vox * g->n_vox;
+    if (prec == 0) {
+        std::vector<float> p(proj_view, proj_view + lenP), t(lenV, 0.f), f(lenV, 0.f);
+        grad_backward_raw<float>(*g, *s, cond, n, p.data(), ext, t.data(), f.data());
+        for (int64_t i = 0; i < lenV; i++) {
+            tmp[i] += t[i];
+            factor[i] += f[i];
+        }
+    } else {
+        std::vector<double> p(proj_view, proj_view + lenP);
+        grad_backward_raw<double>(*g, *s, cond, n, p.data(), ext, tmp, factor);
+    }
+}
+
+double orc_grad_coef(int prec, int n, int P, int diff) {
+    return prec == 0 ? (double)grad_coef<float>(n, P, diff) : grad_coef<double>(n, P, diff);
+}
+
+// IR::gradReconstruct up to the Poisson solve.  g: the result's geometry
+// (n_vox = N); sino0: U*V*P floats; grads_out: 3*n_cond*(N+1)^3 doubles;
+// losses: epochs doubles.  dir 0 = CW, 1 = CCW; art 0 = MLEM, 1 = ART.
+void orc_grad_reconstruct(int prec, const orc_geometry* g, const orc_scan* s, const float* sino0, int epochs,
+                          int batch, int dir, int art, float lambda, int block_size, double* grads_out,
+                          double* losses) {
+    if (prec == 0)
+        grad_reconstruct<float>(*g, *s, sino0, epochs, batch, dir, art, lambda, block_size, grads_out, losses);
+    else
+        grad_reconstruct<double>(*g, *s, sino0, epochs, batch, dir, art, lambda, block_size, grads_out, losses);
 }
 
 // a14 (+a13): XTT::reconstruct.  sino: n_cond*U*V*P floats; vol_in: K*N^3 floats.

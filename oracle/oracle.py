@@ -57,6 +57,10 @@ def _load():
         "orc_subset_orders": (None, [I, I, I, _i32]),
         "orc_ratio_step": (ctypes.c_double, [I, ctypes.c_int64, _f, _d, I]),
         "orc_ir_reconstruct": (None, [I, G, S, _f, _f, I, I, I, I, F, _d, _d]),
+        "orc_grad_forward_view": (None, [I, G, S, I, I, _f, I, _d]),
+        "orc_grad_backward_view": (None, [I, G, S, I, I, _f, I, _d, _d]),
+        "orc_grad_coef": (ctypes.c_double, [I, I, I, I]),
+        "orc_grad_reconstruct": (None, [I, G, S, _f, I, I, I, I, F, I, _d, _d]),
         "orc_unet_num_params": (ctypes.c_int64, [I]),
         "orc_conv3": (None, [_f, I, I, I, I, _f, _f, I, I, _d]),
         "orc_unet_forward": (None, [_f, I, _f, I, I, I, F, I, _f]),
@@ -213,6 +217,45 @@ class Case:
         _lib.orc_ir_reconstruct(prec, self.g, self.s, _fp(sino), _fp(vol0), epochs, batch, direction, int(art), lam,
                                 _dp(out), _dp(losses))
         return out.reshape(vol0.shape), losses[: self.n_cond * epochs]
+
+    def _grad_geometry(self):
+        g = OrcGeometry()
+        for name, _ in OrcGeometry._fields_:
+            setattr(g, name, getattr(self.g, name))
+        g.n_vox = self.g.n_vox + 1
+        return g
+
+    def grad_extent(self, block_size):
+        """x and z extent of the gradient volume that the reference's launches
+        cover: min(N+1, ceil(N/blockSize)*blockSize)."""
+        N = self.N
+        return min(N + 1, -(-N // block_size) * block_size)
+
+    def grad_forward_view(self, vol, n, ext=None, cond=0, prec=F64):
+        """forwardProjGrad of one view: vol is the (N+1)^3 gradient volume
+        ([z][y][x]); the result is the un-normalised splat of vs*ratio*voxel."""
+        g1 = self._grad_geometry()
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
+        p = np.zeros(self.U * self.V)
+        _lib.orc_grad_forward_view(prec, g1, self.s, cond, n, _fp(vol), g1.n_vox if ext is None else ext, _dp(p))
+        return p.reshape(self.V, self.U)
+
+    def grad_backward_view(self, proj_view, n, tmp, factor, ext=None, cond=0, prec=F64):
+        g1 = self._grad_geometry()
+        pv = np.ascontiguousarray(proj_view, dtype=np.float32).reshape(-1)
+        _lib.orc_grad_backward_view(prec, g1, self.s, cond, n, _fp(pv), g1.n_vox if ext is None else ext, _dp(tmp),
+                                    _dp(factor))
+
+    def grad_reconstruct(self, sino0, epochs, batch, direction=0, art=True, lam=1e-2, block_size=32, prec=F64):
+        """IR::gradReconstruct up to the Poisson solve: (3*n_cond gradient
+        volumes of side N+1, slot 3*cond + diff; per-epoch loss sums)."""
+        sino0 = np.ascontiguousarray(sino0, dtype=np.float32)
+        N1 = self.N + 1
+        grads = np.zeros(3 * self.n_cond * N1 ** 3)
+        losses = np.zeros(max(1, epochs))
+        _lib.orc_grad_reconstruct(prec, self.g, self.s, _fp(sino0), epochs, batch, direction, int(art), lam,
+                                  block_size, _dp(grads), _dp(losses))
+        return grads.reshape(3 * self.n_cond, N1, N1, N1), losses[:epochs]
 
     def xtt_reconstruct(self, sino, vol0, epochs, batch, direction=0, art=False, lam=1e-2, prec=F64):
         sino = np.ascontiguousarray(sino, dtype=np.float32)
@@ -412,3 +455,54 @@ def conv3(x, W, b, relu=False):
     out = np.zeros((d0, d1, d2, cout))
     _lib.orc_conv3(_fp(x), cin, d0, d1, d2, _fp(W), _fp(b), cout, int(relu), _dp(out))
     return out
+
+
+def grad_coef(n, P, diff, prec=F64):
+    """sinogramGradientCoef's factor {cos theta, sin theta, 1}[diff] (ir.cu:585-594)."""
+    return _lib.orc_grad_coef(prec, n, P, diff)
+
+
+def poisson_matrix(n):
+    """The n^2 x n^2 system of poissonSolveLDLT (poisson_cpu.cpp:77-90): identity
+    rows on the slice border, the 5-point Laplacian (4, four -1) inside, row and
+    column index x + n*y."""
+    import scipy.sparse as sp
+
+    rows, cols, vals = [], [], []
+    for x in range(n):
+        for y in range(n):
+            r = x + n * y
+            if x == 0 or x == n - 1 or y == 0 or y == n - 1:
+                rows.append(r), cols.append(r), vals.append(1.0)
+                continue
+            for dx, dy, v in ((0, 0, 4.0), (1, 0, -1.0), (-1, 0, -1.0), (0, 1, -1.0), (0, -1, -1.0)):
+                rows.append(r), cols.append(x + dx + n * (y + dy)), vals.append(v)
+    return sp.csc_matrix((vals, (rows, cols)), shape=(n * n, n * n))
+
+
+def poisson_rhs(src0, src1, n):
+    """Right sides of poissonSolveLDLT (poisson_cpu.cpp:98-105), in FP32 as the
+    reference: src0(x,y,z) - src0(x+1,y,z) + src1(x,y,z) - src1(x,y+1,z) for
+    1 <= x, y <= n-2, zero on the border.  src0/src1: (n+1)^3 float32 [z][y][x];
+    returns [z][y][x] float32 of side n."""
+    s0 = np.asarray(src0, dtype=np.float32)
+    s1 = np.asarray(src1, dtype=np.float32)
+    b = np.zeros((n, n, n), np.float32)
+    i = slice(1, n - 1)
+    b[:, i, i] = ((s0[:n, 1:n - 1, 1:n - 1] - s0[:n, 1:n - 1, 2:n]) + s1[:n, 1:n - 1, 1:n - 1]) - s1[:n, 2:n, 1:n - 1]
+    return b
+
+
+def poisson_solve(src0, src1, n):
+    """poissonSolveLDLT (poisson_cpu.cpp:15-112): per z slice, solve the system
+    of poisson_matrix(n) for poisson_rhs by sparse LU — Eigen's SparseLU in FP32
+    there, SciPy's SuperLU in FP64 here.  Returns [z][y][x] float64 of side n."""
+    import scipy.sparse.linalg as spl
+
+    lu = spl.splu(poisson_matrix(n))
+    b = poisson_rhs(src0, src1, n).astype(np.float64)
+    out = np.zeros((n, n, n))
+    for z in range(n):
+        out[z] = lu.solve(b[z].reshape(-1)).reshape(n, n)
+    return out
+

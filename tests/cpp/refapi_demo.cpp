@@ -1,6 +1,6 @@
 // Drop-in check of the reference-compatible C++ API (include/xr/reconstruct.hpp):
 // the same calls the reference's main.cu makes (init_params, Volume, Geometry,
-// IR / FDK / XTT ::reconstruct), on a small synthetic problem.  Prints one
+// IR / FDK / XTT ::reconstruct, IR::gradReconstruct), on a small synthetic problem.  Prints one
 // JSON line with checksums for the Python test to compare.
 #include <cstdio>
 
@@ -45,6 +45,20 @@ int main(int argc, char** argv) {
     double s_fdk = 0;
     for (int c = 0; c < C; c++)
         for (int64_t i = 0; i < (int64_t)N * N * N; i++) s_fdk += fv[c].get()[i];
-    std::printf("{\"ir\": %.9g, \"xtt\": %.9g, \"md\": %.9g, \"fdk\": %.9g}\n", s_ir, s_xtt, s_md, s_fdk);
+    // IR::gradReconstruct from sinogram[0] as FDK left it; its gradient volumes
+    // are saved under VOLUME_PATH, taken from argv[3] when given
+    double s_grad = 0;
+    int voxel_after = 0;
+    if (argc > 3) {
+        VOLUME_PATH = argv[3];
+        Geometry g2 = geom;
+        std::vector<Volume<float>> gv(1);
+        gv[0] = Volume<float>(N, N, N);
+        IR::gradReconstruct(sino.data(), gv.data(), g2, 1, 4, Rotate::CW, Method::ART, 1e-2f);
+        for (int64_t i = 0; i < (int64_t)N * N * N; i++) s_grad += gv[0].get()[i];
+        voxel_after = g2.voxel;
+    }
+    std::printf("{\"ir\": %.9g, \"xtt\": %.9g, \"md\": %.9g, \"fdk\": %.9g, \"grad\": %.9g, \"voxel_after\": %d}\n",
+                s_ir, s_xtt, s_md, s_fdk, s_grad, voxel_after);
     return 0;
 }

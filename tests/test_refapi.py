@@ -21,17 +21,20 @@ def test_refapi_demo_builds():
 
 
 def synthetic_sino(U, V, P, c):
+    """The demo's data, in its FP32 arithmetic (0.05f + 0.01f * k)."""
     n, v, u = np.meshgrid(np.arange(P), np.arange(V), np.arange(U), indexing="ij")
-    return (0.05 + 0.01 * ((u * 7 + v * 3 + n + c) % 11)).astype(np.float32)
+    k = ((u * 7 + v * 3 + n + c) % 11).astype(np.float32)
+    return np.float32(0.05) + np.float32(0.01) * k
 
 
 @pytest.mark.gpu
-def test_refapi_demo_matches_python_binding(xr, ctx):
+def test_refapi_demo_matches_python_binding(xr, ctx, tmp_path):
     exe = os.path.join(ROOT, "tests", "cpp", "refapi_demo")
     assert os.path.exists(exe), "build() compiles tests/cpp/refapi_demo"
     env = dict(os.environ, XR_LOSS_CSV="")  # no loss file from this run
-    r = subprocess.run([exe, FIXTURE, "bench_tilted"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
-                       timeout=120, env=env)
+    vol_path = str(tmp_path / "ct")
+    r = subprocess.run([exe, FIXTURE, "bench_tilted", vol_path], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=120, env=env)
     assert r.returncode == 0, r.stderr
     got = json.loads(r.stdout.strip().splitlines()[-1])
     st, g, scan = xr.load_settings(FIXTURE, "bench_tilted")
@@ -45,7 +48,21 @@ def test_refapi_demo_matches_python_binding(xr, ctx):
     ctx.xtt_reconstruct(g, scan, s2, ct, md, 1, 4)
     fv = [np.zeros(N ** 3, np.float32) for _ in range(C)]
     ctx.fdk_reconstruct(g, scan, s2, fv)  # FDK reads the sinogram XTT overwrote, as the demo does
-    want = {"ir": sum(float(v.astype(np.float64).sum()) for v in vol),
+    # gradReconstruct reads sinogram[0] as FDK left it; blockSize 32 = N, so
+    # the x = N and z = N layers are skipped as in the reference's launches
+    gv = np.zeros(N ** 3, np.float32)
+    grads = [np.zeros((N + 1) ** 3, np.float32) for _ in range(3 * C)]
+    ctx.grad_reconstruct(g, scan, s2[0], gv, 1, 4, method=xr.METHOD_ART, lam=1e-2, block_size=32, grads=grads)
+    assert got["voxel_after"] == N + 1  # geom.voxel + 1, as the reference leaves it
+    # 3*min(NUM_BASIS_VECTOR, NUM_PROJ_COND) gradient volumes saved
+    for i in range(3 * min(K, C)):
+        path = vol_path + "grad%d_%dx%dx%d.raw" % (i + 1, N + 1, N + 1, N + 1)
+        saved = np.fromfile(path, dtype=np.float32)
+        assert saved.size == (N + 1) ** 3
+        assert np.linalg.norm(saved - grads[i]) <= 1e-6 * np.linalg.norm(grads[i])
+    assert not os.path.exists(vol_path + "grad%d_%dx%dx%d.raw" % (3 * min(K, C) + 1, N + 1, N + 1, N + 1))
+    want = {"grad": float(gv.astype(np.float64).sum()),
+            "ir": sum(float(v.astype(np.float64).sum()) for v in vol),
             "xtt": sum(float(v.astype(np.float64).sum()) for v in ct),
             "md": sum(float(np.abs(v).astype(np.float64).sum()) for v in md),
             "fdk": sum(float(v.astype(np.float64).sum()) for v in fv)}

@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxr.so")
 
 XR_OK = 0
-CONV_IR, CONV_FDK = 0, 1
+CONV_IR, CONV_FDK, CONV_GRAD = 0, 1, 2
 CW, CCW = 0, 1
 METHOD_XTT, METHOD_MLEM, METHOD_ART, METHOD_FDK = 0, 1, 2, 3
 FILTER_REFERENCE, FILTER_RAMLAK, FILTER_HILBERT = 0, 1, 2
@@ -131,6 +131,10 @@ def _load():
                                    ctypes.POINTER(Stats)]),
         "xr_fdk_reconstruct": (I, [P, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct),
                                    ctypes.POINTER(_c_float_p), ctypes.POINTER(_c_float_p), I, I]),
+        "xr_grad_reconstruct": (I, [P, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct), _c_float_p,
+                                    _c_float_p, ctypes.POINTER(_c_float_p), I, I, I, I, F, I,
+                                    ctypes.POINTER(Stats)]),
+        "xr_poisson_solve": (I, [P, I, P, P, P]),
         "xr_load_settings": (I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Settings)]),
         "xr_settings_geometry": (I, [ctypes.POINTER(Settings), ctypes.POINTER(Geometry),
                                      ctypes.POINTER(ScanStruct)]),
@@ -456,6 +460,31 @@ class Context:
         sp, vp = _ptr_list(sino), _ptr_list(vol)
         self.check(_lib.xr_fdk_reconstruct(self.h, ctypes.byref(geom), ctypes.byref(scan.struct), sp, vp, direction,
                                            filter))
+
+
+    def grad_reconstruct(self, geom, scan, sino, vol, epochs, batch, direction=CW, method=METHOD_ART, lam=1e-2,
+                         block_size=32, grads=None):
+        """IR::gradReconstruct: sino one float32 U*V*P sinogram, vol the N^3
+        float32 result (out); grads: None or 3*n_cond float32 arrays of
+        (N+1)^3 that receive the gradient volumes (slot 3*cond + diff).
+        Returns the [slot][epoch] raw loss sums."""
+        for a in (sino, vol):
+            if a.dtype != np.float32 or not a.flags.c_contiguous:
+                raise XRError("expected contiguous float32 arrays")
+        slots = 3 * scan.n_cond
+        gp = None if grads is None else _ptr_list(grads)
+        if grads is not None and len(grads) != slots:
+            raise XRError("grads needs %d arrays" % slots)
+        losses = np.zeros(max(1, slots * epochs), dtype=np.float64)
+        st = Stats(losses.ctypes.data_as(_c_double_p))
+        self.check(_lib.xr_grad_reconstruct(self.h, ctypes.byref(geom), ctypes.byref(scan.struct), _fptr(sino),
+                                            _fptr(vol), gp, epochs, batch, direction, method, lam, block_size,
+                                            ctypes.byref(st)))
+        return losses[: slots * epochs].reshape(slots, epochs)
+
+    def poisson_solve(self, n, gx, gy, vol):
+        """poissonSolveLDLT on device tensors: gx, gy (n+1)^3, vol n^3 (reference layouts)."""
+        self.check(_lib.xr_poisson_solve(self.h, n, _dptr(gx), _dptr(gy), _dptr(vol)))
 
 
 def _ptr_list(arrs):

@@ -21,7 +21,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["xr_kernels.hip", "xr_runtime.cpp", "xr_batch.cpp", "xr_unet_kernels.hip", "xr_unet.cpp"]
+SOURCES = ["xr_kernels.hip", "xr_runtime.cpp", "xr_batch.cpp", "xr_unet_kernels.hip", "xr_unet.cpp", "xr_grad_kernels.hip",
+           "xr_grad.cpp"]
 
 
 def _run(cmd):

@@ -61,6 +61,8 @@ namespace xr {
 namespace host {
 
 int fail(xr_context* ctx, int code, const char* fmt, ...);
+// 2.0f * (float)M_PI * (float)n / (float)P in FP32, step by step (ir.cu:1860)
+float theta_f32(int n, int P);
 
 #define XR_HIP(ctx, call)                                                                                      \
     do {                                                                                                       \

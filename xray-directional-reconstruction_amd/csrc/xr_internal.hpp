@@ -121,6 +121,7 @@ struct FwdArgs {
     int axis;                // batched launches: 1 -> every view is axis aligned (ViewGeom::axis)
     const int32_t* bands;    // untilted views: [code][band][y] columns xs | xe << 16 to scan, or null (all)
     const float* coef;       // XTT orientation field (forwardOrth): phi, cos theta per voxel, chan_stride apart, or null
+    float out_scale = 1.0f;  // factor on each new pixel sum before it joins proj's contents (gradient grid: vs)
 };
 
 struct BwdArgs {
@@ -210,6 +211,13 @@ hipError_t launch_vol_up_add(const float* a, const float* b, int Nc, int f, floa
 hipError_t launch_pca(const float* mu, int K, const Basis& b, int64_t nvox, int64_t chan_stride, float* md,
                       float* evalue, hipStream_t stream);
 hipError_t launch_fill(float* p, int64_t n, float value, hipStream_t stream);
+// gradient-domain phase contrast (xr_grad_kernels.hip)
+hipError_t launch_scale_views(const float* src, float* dst, const float* coef, int64_t len_view, int n_views,
+                              hipStream_t stream);
+hipError_t launch_grad_faces(float* vol, int N1, int ext, float value, hipStream_t stream);
+hipError_t launch_poisson_rhs(const float* g0, const float* g1, double* B, int n, hipStream_t stream);
+hipError_t launch_poisson_dst(const double* S, const double* e, double* B, double* T, float* vol, int n,
+                              hipStream_t stream);
 // Direction post-processing on 3 channels of n floats (pca.cu:67-106).
 hipError_t launch_md_angles(const float* md, float* angle, int64_t n, hipStream_t stream);
 hipError_t launch_md_normalise(float* md, int64_t n, float eps, hipStream_t stream);

@@ -592,7 +592,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
         unscale = ldexpf(1.0f, -S);
     } else {
         scale = fminf(fmaxf(2145386496.0f / bound, 0x1p-100f), 0x1p100f);
-        unscale = 1.0f / scale;
+        unscale = a.out_scale / scale;
     }
     __syncthreads();
 

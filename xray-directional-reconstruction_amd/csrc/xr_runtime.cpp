@@ -228,6 +228,9 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
     a.views = d_codes;
     a.accum = d_flags;
     a.vol = vol;
+    // forwardProjGrad splats vs*ratio*voxel (ir.cu:489-491), and a repeated view
+    // adds that to the image it already holds (no normalisation afterwards)
+    if (p->conv == XR_CONV_GRAD) a.out_scale = p->g.vox_size;
     const int64_t N3 = (int64_t)p->g.n_vox * p->g.n_vox * p->g.n_vox;
     a.vol_cond_stride = vol_cond_stride;
     a.chan_stride = N3;
@@ -387,6 +390,9 @@ int ensure_norm(xr_plan* p) {
     XR_HIP(ctx, hipMemcpyAsync(&mf, fmax.p, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     XR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     p->max_factor = mf;
+    // The gradient operator has no chord normalisation (its splat carries the
+    // factor vs instead, FwdArgs::out_scale): a table of ones.
+    if (p->conv == XR_CONV_GRAD) XR_HIP(ctx, launch_fill(norm.p, lenD * p->n_cond, 1.0f, ctx->stream));
     p->d_norm = norm.p;
     norm.p = nullptr;
     return XR_OK;
@@ -687,7 +693,8 @@ int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int 
     *out = nullptr;
     XR_TRY(check_geometry(ctx, g));
     XR_TRY(check_scan(ctx, s));
-    if (convention != XR_CONV_IR && convention != XR_CONV_FDK) return fail(ctx, XR_ERR_INVALID, "bad convention");
+    if (convention != XR_CONV_IR && convention != XR_CONV_FDK && convention != XR_CONV_GRAD)
+        return fail(ctx, XR_ERR_INVALID, "bad convention");
     if (direction != XR_CW && direction != XR_CCW) return fail(ctx, XR_ERR_INVALID, "bad direction");
     std::unique_ptr<xr_plan> p(new (std::nothrow) xr_plan());
     if (!p) return fail(ctx, XR_ERR_ALLOC, "plan allocation failed");
@@ -736,7 +743,11 @@ int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int 
     d.V = g->n_det_v;
     d.P = P;
     d.K = s->n_basis;
-    d.xc = (convention == XR_CONV_IR) ? (float)(g->n_vox + 1) * 0.5f : (float)(g->n_vox - 1) * 0.5f;
+    // voxel centre x - xc: (N+1)/2 (ir.cu:1881), (N-1)/2 (fdk.cu:132), and N/2 on
+    // the gradient grid of side N (ir.cu:459: (2(x - 0.5) - N + 1)/2)
+    d.xc = (convention == XR_CONV_IR)    ? (float)(g->n_vox + 1) * 0.5f
+           : (convention == XR_CONV_FDK) ? (float)(g->n_vox - 1) * 0.5f
+                                         : (float)g->n_vox * 0.5f;
     d.hu = 0.5f * (float)g->n_det_u;
     d.hv = 0.5f * (float)g->n_det_v;
     d.umax = (float)g->n_det_u - 0.55f;
@@ -876,6 +887,14 @@ static int upload_views(xr_plan* p, const int32_t* views, int n_views, const uin
     return XR_OK;
 }
 
+// Plans the voxel-driven projectors serve: the IR convention, and the
+// gradient grid of IR::gradReconstruct (scalar volumes only).
+static int check_projecting(xr_context* ctx, const xr_plan* plan, int xtt) {
+    if (plan->conv == XR_CONV_IR) return XR_OK;
+    if (plan->conv == XR_CONV_GRAD && !xtt) return XR_OK;
+    return fail(ctx, XR_ERR_INVALID, "projection needs an XR_CONV_IR plan (or XR_CONV_GRAD without XTT)");
+}
+
 int xr_project(xr_context* ctx, xr_plan* plan, const float* vol, float* proj, const float* sino,
                const int32_t* views, int n_views, const uint8_t* accumulate, int xtt, int out_mode, double* loss) {
     if (!ctx || !plan || !vol || !proj || !views || n_views < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
@@ -883,7 +902,7 @@ int xr_project(xr_context* ctx, xr_plan* plan, const float* vol, float* proj, co
     if ((out_mode == XR_OUT_RATIO || out_mode == XR_OUT_RESIDUAL) && !sino)
         return fail(ctx, XR_ERR_INVALID, "sino required for ratio/residual");
     if (xtt && plan->n_basis <= 0) return fail(ctx, XR_ERR_INVALID, "XTT projection needs basis vectors");
-    if (plan->conv != XR_CONV_IR) return fail(ctx, XR_ERR_INVALID, "forward projection uses the IR convention");
+    XR_TRY(check_projecting(ctx, plan, xtt));
     if (n_views == 0) {
         if (loss) *loss = 0;
         return XR_OK;
@@ -946,7 +965,7 @@ int xr_residual(xr_context* ctx, xr_plan* plan, const float* vol, float* resid, 
                 const int32_t* views, int n_views, int xtt, double* l1, double* l2) {
     if (!ctx || !plan || !vol || !resid || !sino || !views || n_views < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
     if (xtt && plan->n_basis <= 0) return fail(ctx, XR_ERR_INVALID, "XTT projection needs basis vectors");
-    if (plan->conv != XR_CONV_IR) return fail(ctx, XR_ERR_INVALID, "forward projection uses the IR convention");
+    XR_TRY(check_projecting(ctx, plan, xtt));
     if (l1) *l1 = 0;
     if (l2) *l2 = 0;
     if (n_views == 0) return XR_OK;
@@ -968,7 +987,7 @@ int xr_backproject(xr_context* ctx, xr_plan* plan, const float* proj, const int3
                    float* tmp, float* factor) {
     if (!ctx || !plan || !proj || !views || !tmp || !factor || n_views < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
     if (xtt && plan->n_basis <= 0) return fail(ctx, XR_ERR_INVALID, "XTT backprojection needs basis vectors");
-    if (plan->conv != XR_CONV_IR) return fail(ctx, XR_ERR_INVALID, "backprojection uses the IR convention");
+    XR_TRY(check_projecting(ctx, plan, xtt));
     if (n_views == 0) return XR_OK;
     XR_TRY(upload_views(plan, views, n_views, nullptr));
     return backward_launch(plan, xtt ? BWD_XTT_RAW : BWD_RAW, proj, 0, ctx->d_codes, n_views, nullptr, tmp, factor,
@@ -980,7 +999,7 @@ int xr_backproject_update(xr_context* ctx, xr_plan* plan, const float* proj, con
     if (!ctx || !plan || !proj || !views || !vol || n_views < 0) return fail(ctx, XR_ERR_INVALID, "bad args");
     if (method != XR_METHOD_MLEM && method != XR_METHOD_ART) return fail(ctx, XR_ERR_INVALID, "method must be MLEM or ART");
     if (xtt && plan->n_basis <= 0) return fail(ctx, XR_ERR_INVALID, "XTT update needs basis vectors");
-    if (plan->conv != XR_CONV_IR) return fail(ctx, XR_ERR_INVALID, "backprojection uses the IR convention");
+    XR_TRY(check_projecting(ctx, plan, xtt));
     if (subset_size <= 0) return fail(ctx, XR_ERR_INVALID, "subset_size must be positive");
     XR_TRY(upload_views(plan, views, n_views, nullptr));
     int kind;
