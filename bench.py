@@ -481,7 +481,8 @@ def main():
     ap.add_argument("--long-epochs", type=int, default=200)
     ap.add_argument("--pres", default="5")
     ap.add_argument("--model", default="")
-    ap.add_argument("--workload", choices=["mlem", "batch", "heldout", "xtt", "ttr", "fdkart"], default="mlem",
+    ap.add_argument("--workload", choices=["mlem", "batch", "heldout", "xtt", "ttr", "fdkart", "grad"],
+                    default="mlem",
                     help="mlem: one MLEM iteration at 512^3/720 views (default); batch: 48 uneven MLEM problems, "
                          "30 epochs in persistent launches (HIP graph, autotuned tiles); heldout: time to "
                          "residual, plain vs warm start, on three phantoms outside the training draws")
@@ -512,6 +513,15 @@ def main():
         sino = c2_noisy_sinogram(ctx, xr, geom, plan, N, geom.vox_size, photons=args.photons)
         plan.close()
         print(json.dumps(fdk_art_measure(ctx, xr, geom, sino)))
+        return
+    if args.workload == "grad":
+        N, U, P = args.n_vox, args.n_det, args.n_proj
+        geom = xr.make_geometry(1000.0, 500.0, 0.4, N, U, P)
+        ctx = xr.Context(0)
+        plan = ctx.plan(geom, xr.Scan())
+        sino = c2_noisy_sinogram(ctx, xr, geom, plan, N, geom.vox_size, photons=args.photons)
+        plan.close()
+        print(json.dumps(grad_measure(ctx, xr, geom, sino)))
         return
     if args.workload == "ttr":
         print(json.dumps(ttr_workload(xr, args)))
