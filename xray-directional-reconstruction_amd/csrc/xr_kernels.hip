@@ -2289,49 +2289,90 @@ __global__ __launch_bounds__(256) void k_absmax_seg(const float* const* __restri
 }
 
 // ---------------------------------------------------------------------------
-// FDK row filter (projConv, fdk.cu:28-43): dst[u] = sum_j h[|u-j|] src[j] w[j],
-// register-blocked Toeplitz product, R consecutive outputs per thread.
+// FDK row filter (projConv, fdk.cu:28-43): dst[u] = sum_j h(u - j) w[j] src[j]
+// for every row (view, v) of the launch, as one FP32 matrix product on the
+// matrix cores: C[row][u] = sum_j A[row][j] T[j][u], A the weighted rows and
+// T[j][u] = h(u - j) taken from a tap window in LDS (the Toeplitz operand is
+// never stored).  h(m) = taps[|m|] (a linear convolution) or taps[m mod U]
+// (circular), zero for |m| >= U.  Block tile 64 rows x 128 columns, four
+// waves of 32 x 64, K steps of 32 samples.  v_mfma_f32_16x16x4f32 operands:
+// A from lane (row lane&15, k lane>>4), B from lane (k lane>>4, column
+// lane&15); the result lane holds column lane&15, rows 4*(lane>>4) + r.
 // ---------------------------------------------------------------------------
-template <int R>
-__global__ __launch_bounds__(1024) void k_fdk_filter(const float* __restrict__ src, float* __restrict__ dst,
-                                                    const float* __restrict__ weight, const float* __restrict__ taps,
-                                                    int U, int V, int Upad, const int32_t* views,
-                                                    int code_base, int circular) {
-    extern __shared__ float sm[];
-    float* s = sm;         // Upad (weighted) samples, zero padded
-    float* hx = sm + Upad;  // 2*Upad taps: hx[m + Upad] = h(m), m = u - j
-    const int v = blockIdx.x;
-    const int code = views[blockIdx.y];
-    const int64_t row = ((int64_t)(code - code_base) * V + v) * U;
-    for (int j = threadIdx.x; j < Upad; j += blockDim.x)
-        s[j] = (j < U) ? (weight ? src[row + j] * weight[(int64_t)v * U + j] : src[row + j]) : 0.f;
-    // symmetric taps h(m) = taps[|m|] (a linear convolution), or circular taps
-    // h(m) = taps[m mod U] (a circular convolution over the row)
-    for (int m = threadIdx.x; m < 2 * Upad; m += blockDim.x) {
-        const int dm = m - Upad, am = abs(dm);
-        hx[m] = (am < U) ? taps[circular ? (dm + U) % U : am] : 0.f;
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+constexpr int kFiltBM = 64, kFiltBN = 128, kFiltBK = 32, kFiltAStride = kFiltBK + 1;
+constexpr int kFiltWin = kFiltBN + kFiltBK - 1;  // taps one K step of one block touches
+
+__global__ __launch_bounds__(256) void k_fdk_filter(const float* __restrict__ src, float* __restrict__ dst,
+                                                   const float* __restrict__ weight, const float* __restrict__ taps,
+                                                   int U, int V, const int32_t* __restrict__ views, int n_rows,
+                                                   int row_base, int code_base, int circular) {
+    __shared__ float As[kFiltBM * kFiltAStride];
+    __shared__ float Hs[kFiltWin + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int row0 = row_base + blockIdx.y * kFiltBM, col0 = blockIdx.x * kFiltBN;
+    // this thread loads row lr, samples lj .. lj+7 of each K step
+    const int lr = tid >> 2, lj = (tid & 3) * 8;
+    int64_t roff = -1;
+    int wv = 0;
+    if (row0 + lr < n_rows) {
+        const int slot = (row0 + lr) / V;
+        wv = row0 + lr - slot * V;
+        roff = ((int64_t)(views[slot] - code_base) * V + wv) * U;
     }
-    __syncthreads();
-    const int u0 = threadIdx.x * R;
-    if (u0 >= Upad) return;
-    float acc[R];
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;  // the wave's 32 x 64 sub-tile
+    const int li = lane & 15, lk = lane >> 4;
+    f32x4_t acc[2][4];
 #pragma unroll
-    for (int i = 0; i < R; i++) acc[i] = 0.f;
-    for (int j0 = 0; j0 < Upad; j0 += R) {
-        float sj[R], hw[2 * R - 1];
+    for (int mt = 0; mt < 2; mt++)
 #pragma unroll
-        for (int r = 0; r < R; r++) sj[r] = s[j0 + r];
-        const int base = u0 - j0 - (R - 1) + Upad;
+        for (int nt = 0; nt < 4; nt++) acc[mt][nt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (int j0 = 0; j0 < U; j0 += kFiltBK) {
+        __syncthreads();  // the previous K step is done with As and Hs
 #pragma unroll
-        for (int q = 0; q < 2 * R - 1; q++) hw[q] = hx[base + q];
+        for (int q = 0; q < 8; q++) {
+            const int j = j0 + lj + q;
+            float x = 0.f;
+            if (roff >= 0 && j < U) {
+                x = src[roff + j];
+                if (weight) x *= weight[(int64_t)wv * U + j];
+            }
+            As[lr * kFiltAStride + lj + q] = x;
+        }
+        // Hs[t] = h(m), m = col0 - j0 - (BK - 1) + t
+        if (tid < kFiltWin) {
+            const int m = col0 - j0 - (kFiltBK - 1) + tid, am = abs(m);
+            Hs[tid] = (am < U) ? taps[circular ? ((m % U) + U) % U : am] : 0.f;
+        }
+        __syncthreads();
 #pragma unroll
-        for (int r = 0; r < R; r++)
+        for (int kk = 0; kk < kFiltBK; kk += 4) {
+            float a[2], b[4];
 #pragma unroll
-            for (int i = 0; i < R; i++) acc[i] = fmaf(hw[i - r + R - 1], sj[r], acc[i]);
+            for (int mt = 0; mt < 2; mt++) a[mt] = As[(wm + mt * 16 + li) * kFiltAStride + kk + lk];
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) b[nt] = Hs[wn + nt * 16 + li - kk - lk + (kFiltBK - 1)];
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                for (int nt = 0; nt < 4; nt++)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+        }
     }
 #pragma unroll
-    for (int i = 0; i < R; i++)
-        if (u0 + i < U) dst[row + u0 + i] = acc[i];
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = row0 + wm + mt * 16 + 4 * lk + r;
+            if (row >= n_rows) continue;
+            const int slot = row / V, v = row - slot * V;
+            const int64_t off = ((int64_t)(views[slot] - code_base) * V + v) * U;
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) {
+                const int col = col0 + wn + nt * 16 + li;
+                if (col < U) dst[off + col] = acc[mt][nt][r];
+            }
+        }
 }
 
 // calcWeight (fdk.cu:13-26).
@@ -3103,14 +3144,19 @@ hipError_t launch_absmax_seg(const float* const* segs, const int64_t* seg_len, f
 
 hipError_t launch_fdk_filter(const float* src, float* dst, const float* weight, const float* taps, int U, int V,
                              const int32_t* views, int n_views, int code_base, int circular, hipStream_t stream) {
-    constexpr int R = 8;
-    const int Upad = ((U + R - 1) / R) * R;
-    int threads = ((Upad / R + 63) / 64) * 64;
-    if (threads > 1024 || threads < 64) return hipErrorInvalidValue;
-    const size_t lds = (size_t)3 * Upad * sizeof(float);
-    hipLaunchKernelGGL((k_fdk_filter<R>), dim3(V, n_views), dim3(threads), lds, stream, src, dst, weight, taps, U, V,
-                       Upad, views, code_base, circular);
-    return hipGetLastError();
+    const int64_t n_rows = (int64_t)n_views * V;
+    if (n_rows > INT32_MAX - kFiltBM) return hipErrorInvalidValue;
+    // row tiles in launches of at most 65535 tiles (the grid's y limit)
+    const int64_t per_launch = (int64_t)65535 * kFiltBM;
+    for (int64_t base = 0; base < n_rows; base += per_launch) {
+        const int64_t rows = std::min(per_launch, n_rows - base);
+        const dim3 grid((U + kFiltBN - 1) / kFiltBN, (unsigned)((rows + kFiltBM - 1) / kFiltBM));
+        hipLaunchKernelGGL(k_fdk_filter, grid, dim3(256), 0, stream, src, dst, weight, taps, U, V, views, (int)n_rows,
+                           (int)base, code_base, circular);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_fdk_weight(float* weight, int U, int V, float sdd, float det, hipStream_t stream) {
