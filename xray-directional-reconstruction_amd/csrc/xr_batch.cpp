@@ -339,6 +339,7 @@ int build_tables(xr_batch* b) {
             a.out_mode = out_mode;
             a.n_bands = (p.U + kBandU - 1) / kBandU;
             a.n_slots = p.C * p.ss;
+            a.view_group = kFwdViewGroup;
             a.axis = 1;
             for (int c = 0; c < p.C; c++) a.axis = a.axis && pl->axis_cond[c];
             const int rows = forward_rows(pl, xtt ? FWD_XTT : FWD_PLAIN);

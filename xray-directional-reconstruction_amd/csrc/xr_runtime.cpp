@@ -231,6 +231,7 @@ int forward_launch(xr_plan* p, int kind, const float* vol, int64_t vol_cond_stri
     // forwardProjGrad splats vs*ratio*voxel (ir.cu:489-491), and a repeated view
     // adds that to the image it already holds (no normalisation afterwards)
     if (p->conv == XR_CONV_GRAD) a.out_scale = p->g.vox_size;
+    a.view_group = kFwdViewGroup;
     const int64_t N3 = (int64_t)p->g.n_vox * p->g.n_vox * p->g.n_vox;
     a.vol_cond_stride = vol_cond_stride;
     a.chan_stride = N3;
