@@ -345,6 +345,23 @@ def test_fdk_matches_oracle(orc, xr, ctx, torch_dev, kind):
     assert ef <= 1e-5 and ev <= 1e-5
 
 
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_fdk_filter_partial_tiles(orc, xr, ctx, torch_dev, kind):
+    """The row filter is one matrix product over 64-row x 128-column tiles with
+    32-sample K steps: U = 150 leaves a partial column tile and a partial K
+    step, V x P = 37 x 7 = 259 rows a partial row tile."""
+    st = Setup(orc, xr, N=24, U=150, V=37, P=7, vs=0.5)
+    rng = np.random.default_rng(51 + kind)
+    sino = rng.random((st.P, st.V, st.U)).astype(np.float32)
+    _, want_filt = st.case.fdk_reconstruct(sino[None], kind=kind)
+    v = [np.zeros(st.N ** 3, np.float32)]
+    s = [sino.reshape(-1).copy()]
+    ctx.fdk_reconstruct(st.geom, st.scan, s, v, filter=kind)
+    ef = rel_l2(s[0], want_filt.reshape(-1))
+    record("fdk_filter_partial/%d" % kind, ef)
+    assert ef <= 1e-5, ef
+
+
 def test_pca_kernel_matches_oracle(orc, xr, ctx, torch_dev):
     import torch
 
