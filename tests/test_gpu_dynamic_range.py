@@ -205,7 +205,7 @@ def test_c1_mlem_20_iterations(orc, xr, ctx, c1_data, batch):
     losses = ctx.ir_reconstruct(st.geom, st.scan, [y.copy()], v, 20, batch)
     err = rel_l2(v[0], want.reshape(-1))
     record("c1_mlem20_batch%d" % batch, err)
-    print("config 1, 20 MLEM iterations batch %d: rel L2 vs FP64 oracle %.3g; loss %.4g -> %.4g"
+    print("64^3 / 90 views, 20 MLEM iterations batch %d: rel L2 vs FP64 oracle %.3g; loss %.4g -> %.4g"
           % (batch, err, losses[0], losses[-1]))
     assert err <= 1e-4
     assert np.allclose(losses, wloss, rtol=1e-4)
