@@ -205,7 +205,8 @@ int xr_residual(xr_context* ctx, xr_plan* plan, const float* vol, float* resid, 
 /* xr_project without any allocation.  No counterpart in the reference: it is
  * for callers that cannot allocate (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
- * and results as xr_project; the view list, accumulate flags, FP64 loss sum
+ * and results as xr_project on XR_CONV_IR plans (other conventions return
+ * XR_ERR_INVALID); the view list, accumulate flags, FP64 loss sum
  * and max |voxel| of the call live in a fixed 80.25 KiB device array per GPU
  * (the library's `scratch`), held until the process ends, instead of buffers
  * the library allocates.  Limits: at most 16384 views per call (larger lists
