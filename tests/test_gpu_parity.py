@@ -171,6 +171,31 @@ def test_forward_is_bitwise_deterministic(orc, xr, ctx, torch_dev):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("xtt", [False, True])
+def test_forward_item_order_does_not_change_results(orc, xr, ctx, torch_dev, xtt):
+    """One launch over 90 views x 16 bands walks its items in groups of 16
+    views; one launch per view splits each band into sub-band items instead.
+    Every item writes only its own pixels and the sums are exact fixed point,
+    so the images are bitwise the same either way."""
+    import torch
+
+    basis = XTT_BASIS13[:3] if xtt else None
+    st = Setup(orc, xr, N=64, U=256, V=96, P=90, vs=0.5, basis=basis)
+    K = 3 if xtt else 1
+    rng = np.random.default_rng(17)
+    vol = rng.random((K, st.N, st.N, st.N)).astype(np.float32)
+    plan = ctx.plan(st.geom, st.scan)
+    vcol = columns(ctx, vol, st.N, K, torch_dev)
+    n = st.P * st.V * st.U
+    together = torch.zeros(n, device=torch_dev)
+    plan.project(vcol, together, plan.codes(range(st.P)), out_mode=xr.OUT_NORMALISED, xtt=xtt)
+    apart = torch.zeros(n, device=torch_dev)
+    for v in range(st.P):
+        plan.project(vcol, apart, plan.codes([v]), out_mode=xr.OUT_NORMALISED, xtt=xtt)
+    ctx.synchronize()
+    assert np.array_equal(together.cpu().numpy(), apart.cpu().numpy())
+
+
 def test_forward_accumulate_flag(orc, xr, ctx, torch_dev):
     st = Setup(orc, xr, N=32, U=48, P=12)
     vol, _ = ellipsoid_phantom(orc, xr, st.N, st.vs, seed=5)
