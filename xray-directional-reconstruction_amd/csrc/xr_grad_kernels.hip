@@ -10,8 +10,8 @@
 // S[j][k] = sin(pi (j+1)(k+1) / (M+1)), M = n-2, diagonalises it exactly:
 //   F = (2/(M+1))^2 S ((S B S) ./ L) S,  L[k][l] = e[k] + e[l],
 //   e[k] = 2 - 2 cos(pi (k+1) / (M+1)),
-// so a slice costs four M x M matrix products.  They run in FP64 (LDS-tiled
-// batched GEMM, 64x64 tiles, 4x4 per lane), every slice of the volume in one
+// so a slice costs four M x M matrix products.  They run in FP64 on the
+// matrix cores (batched GEMM, 64x64 tiles), every slice of the volume in one
 // launch per product; the right sides are formed in FP32 as the reference
 // forms them, and the result is rounded to FP32 once.
 #include <hip/hip_runtime.h>
@@ -77,6 +77,7 @@ __global__ void k_poisson_rhs(const float* __restrict__ g0, const float* __restr
 }
 
 constexpr int kTile = 64, kTk = 16, kGemmThreads = 256;
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
 
 enum GemmOut { GEMM_STORE = 0, GEMM_SPECTRAL = 1, GEMM_VOLUME = 2 };
 
@@ -84,30 +85,39 @@ enum GemmOut { GEMM_STORE = 0, GEMM_SPECTRAL = 1, GEMM_VOLUME = 2 };
 // of 0 shares one matrix over the batch.  OUT: GEMM_STORE writes C;
 // GEMM_SPECTRAL writes C[i][j] * scale / (e[i] + e[j]); GEMM_VOLUME writes
 // (float)C[y][x] to vol[(x+1) + n*((y+1) + n*b)], n = M+2.
+// On the matrix cores: a 64 x 64 block tile, four waves of 32 x 32, each a
+// 2 x 2 set of v_mfma_f64_16x16x4f64 tiles; A from lane (row lane&15, k
+// lane>>4), B from lane (k lane>>4, column lane&15); result register r of a
+// lane holds column lane&15, row 4*r + (lane>>4) (unlike the FP32 and BF16
+// 16x16 forms, whose rows are 4*(lane>>4) + r).  K steps of 16 staged
+// through LDS.
 template <int OUT>
 __global__ void __launch_bounds__(kGemmThreads) k_gemm64(const double* __restrict__ A, int64_t sa,
                                                          const double* __restrict__ Bm, int64_t sb,
                                                          double* __restrict__ C, float* __restrict__ vol,
                                                          const double* __restrict__ e, double scale, int M) {
-    __shared__ double As[kTk][kTile + 1];  // As[k][row]
+    __shared__ double As[kTile][kTk + 1];  // As[row][k]
     __shared__ double Bs[kTk][kTile + 1];  // Bs[k][col]
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int row0 = blockIdx.y * kTile, col0 = blockIdx.x * kTile;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
     const int64_t b = blockIdx.z;
     const double* Ab = A + b * sa;
     const double* Bb = Bm + b * sb;
-    double acc[4][4];
+    f64x4_t acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int mt = 0; mt < 2; mt++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+        for (int nt = 0; nt < 2; nt++) acc[mt][nt] = f64x4_t{0.0, 0.0, 0.0, 0.0};
     for (int k0 = 0; k0 < M; k0 += kTk) {
+        __syncthreads();  // the previous K step is done with As and Bs
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             // A: 64 rows x 16 k, k fastest in memory
             const int m = (tid >> 4) + 16 * r, kk = tid & 15;
             const int gr = row0 + m, gk = k0 + kk;
-            As[kk][m] = (gr < M && gk < M) ? Ab[(int64_t)gr * M + gk] : 0.0;
+            As[m][kk] = (gr < M && gk < M) ? Ab[(int64_t)gr * M + gk] : 0.0;
             // B: 16 k x 64 columns, columns fastest
             const int nn = tid & 63, kb = (tid >> 6) + 4 * r;
             const int gc = col0 + nn, gkb = k0 + kb;
@@ -115,37 +125,40 @@ __global__ void __launch_bounds__(kGemmThreads) k_gemm64(const double* __restric
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kTk; k++) {
-            double a[4], bv[4];
+        for (int kk = 0; kk < kTk; kk += 4) {
+            double a[2], bv[2];
 #pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = As[k][ty + 16 * i];
+            for (int mt = 0; mt < 2; mt++) a[mt] = As[wm + mt * 16 + li][kk + lk];
 #pragma unroll
-            for (int j = 0; j < 4; j++) bv[j] = Bs[k][tx + 16 * j];
+            for (int nt = 0; nt < 2; nt++) bv[nt] = Bs[kk + lk][wn + nt * 16 + li];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int mt = 0; mt < 2; mt++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], bv[j], acc[i][j]);
+                for (int nt = 0; nt < 2; nt++)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bv[nt], acc[mt][nt], 0, 0, 0);
         }
-        __syncthreads();
     }
     const int n = M + 2;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int r = row0 + ty + 16 * i;
-        if (r >= M) continue;
+    for (int mt = 0; mt < 2; mt++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int c = col0 + tx + 16 * j;
-            if (c >= M) continue;
-            if (OUT == GEMM_STORE) {
-                C[b * (int64_t)M * M + (int64_t)r * M + c] = acc[i][j];
-            } else if (OUT == GEMM_SPECTRAL) {
-                C[b * (int64_t)M * M + (int64_t)r * M + c] = acc[i][j] * scale / (e[r] + e[c]);
-            } else {
-                vol[(c + 1) + (int64_t)n * ((r + 1) + (int64_t)n * b)] = (float)acc[i][j];
+        for (int r = 0; r < 4; r++) {
+            const int row = row0 + wm + mt * 16 + 4 * r + lk;
+            if (row >= M) continue;
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) {
+                const int c = col0 + wn + nt * 16 + li;
+                if (c >= M) continue;
+                const double v = acc[mt][nt][r];
+                if (OUT == GEMM_STORE) {
+                    C[b * (int64_t)M * M + (int64_t)row * M + c] = v;
+                } else if (OUT == GEMM_SPECTRAL) {
+                    C[b * (int64_t)M * M + (int64_t)row * M + c] = v * scale / (e[row] + e[c]);
+                } else {
+                    vol[(c + 1) + (int64_t)n * ((row + 1) + (int64_t)n * b)] = (float)v;
+                }
             }
         }
-    }
 }
 
 inline int grid_1d(int64_t total, int threads) {
