@@ -1442,7 +1442,7 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
     int pidx[kStagePer];
     auto fetch = [&](const Foot& f, int code) {
         const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
-        int r0 = (int)((float)threadIdx.x * (1.0f / (float)f.W));
+        int r0 = (int)((float)threadIdx.x * rcpf((float)f.W));  // within 1 of t / W: corrected below
         int c = (int)threadIdx.x - r0 * f.W;
         if (c < 0) {
             r0--;
@@ -1453,13 +1453,16 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
         }
         const bool act = r0 < f.R;
         const auto src = img + (int64_t)(f.iv_lo + r0) * d.U + f.iu_lo + c;
+        // texel (iu_lo, iv_lo) of the footprint, inside the image: the address a
+        // lane outside the footprint loads from, so that no load is predicated
+        const auto safe = img + (int64_t)f.iv_lo * d.U + f.iu_lo;
         const int64_t rstep = (int64_t)f.R * d.U;
 #pragma unroll
         for (int k = 0; k < kStagePer; k++) {
             const int r = r0 + k * f.R;
             const bool in = act && r < f.H;
             pidx[k] = in ? r * f.Ws + c : -1;
-            pre[k] = in ? src[k * rstep] : 0.f;
+            pre[k] = *(in ? src + k * rstep : safe);  // not stored unless in
         }
     };
     auto store = [&](lds_float* buf) {
@@ -1744,7 +1747,7 @@ __device__ __forceinline__ float backward_item_untilted_xtt(const BwdArgs& a, in
     int pidx[kXttStagePer];
     auto fetch = [&](const Foot& f, int code) {
         const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
-        int r0 = (int)((float)threadIdx.x * (1.0f / (float)f.W));
+        int r0 = (int)((float)threadIdx.x * rcpf((float)f.W));  // within 1 of t / W: corrected below
         int c = (int)threadIdx.x - r0 * f.W;
         if (c < 0) {
             r0--;
@@ -1943,7 +1946,7 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
         // rows of the footprint in passes of R = kBwdThreads / W rows: thread t
         // owns column t % W of rows t / W + k*R (one division per view)
         const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
-        int r0 = (int)((float)threadIdx.x * (1.0f / (float)f.W));
+        int r0 = (int)((float)threadIdx.x * rcpf((float)f.W));  // within 1 of t / W: corrected below
         int c = (int)threadIdx.x - r0 * f.W;
         if (c < 0) {
             r0--;
@@ -1954,13 +1957,16 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
         }
         const bool act = r0 < f.R;
         const auto src = img + (int64_t)(f.iv_lo + r0) * d.U + f.iu_lo + c;
+        // texel (iu_lo, iv_lo) of the footprint, inside the image: the address a
+        // lane outside the footprint loads from, so that no load is predicated
+        const auto safe = img + (int64_t)f.iv_lo * d.U + f.iu_lo;
         const int64_t step = (int64_t)f.R * d.U;
 #pragma unroll
         for (int k = 0; k < kStagePer; k++) {
             const int r = r0 + k * f.R;
             const bool in = act && r < f.H;
             pidx[k] = in ? r * f.Ws + c : -1;
-            pre[k] = in ? src[k * step] : 0.f;
+            pre[k] = *(in ? src + k * step : safe);  // not stored unless in
         }
     };
     auto store = [&](lds_float* buf) {
