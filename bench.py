@@ -445,6 +445,56 @@ def grad_measure(ctx, xr, geom, sino, lam=1e-3):
             "poisson_frac_f64_peak": round(flops / (p_ms * 1e-3) / 78.6e12, 4)}
 
 
+def unet_flops_per_voxel(C):
+    """Algorithmic FLOP per full-resolution voxel of one U-Net inference: 2 x 27
+    x Cin x Cout per convolution output voxel at its level (1, 1/8, 1/64 of the
+    voxels), plus the 1x1x1 head; the MFMA padding of Cout to 16 is not
+    counted."""
+    lv0 = [(8, C), (C, C), (3 * C, C), (C, C)]
+    lv1 = [(C, 2 * C), (2 * C, 2 * C), (6 * C, 2 * C), (2 * C, 2 * C)]
+    lv2 = [(2 * C, 4 * C), (4 * C, 4 * C)]
+    f = lambda layers: sum(54.0 * ci * co for ci, co in layers)
+    return f(lv0) + f(lv1) / 8 + f(lv2) / 64 + 2.0 * C
+
+
+def unet_measure(xr, N=512, reps=5):
+    """One U-Net inference (the warm-start network, base width 24) on an N^3
+    volume: decoder upsampling fused into the next convolution's fetch, and
+    materialised (as training runs it).  Wall time per inference between
+    stream synchronisations; FLOP/s against the 2.5 PFLOP/s BF16 dense peak."""
+    import torch
+
+    ctx = xr.Context(0)
+    net = xr.UNet(ctx, 24)
+    if os.path.exists(MODEL):
+        net.load(MODEL)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.rand(N ** 3, device="cuda", generator=g) * 0.05
+    y = torch.empty_like(x)
+    flop = unet_flops_per_voxel(24) * float(N) ** 3
+    out = {"workload": "U-Net inference (base width 24, 3 levels, BF16 MFMA) on a %d^3 volume" % N,
+           "gflop": round(flop / 1e9, 1)}
+    outs = {}
+    for fused in (False, True):
+        net.set_fused_upsampling(fused)
+        net.forward(x, y, (N, N, N), scale=0.05)  # warm-up (buffers sized on first use)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            net.forward(x, y, (N, N, N), scale=0.05)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        key = "fused" if fused else "materialised"
+        out[key + "_ms"] = round(ms, 3)
+        out[key + "_tflops"] = round(flop / (ms * 1e-3) / 1e12, 1)
+        out[key + "_frac_bf16_peak"] = round(flop / (ms * 1e-3) / 2.5e15, 4)
+        outs[key] = y.clone()
+    out["bitwise_equal"] = bool(torch.equal(outs["fused"], outs["materialised"]))
+    net.close()
+    ctx.close()
+    return out
+
+
 def measured_traffic(kernel, N, P):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy), if one
@@ -481,7 +531,7 @@ def main():
     ap.add_argument("--long-epochs", type=int, default=200)
     ap.add_argument("--pres", default="5")
     ap.add_argument("--model", default="")
-    ap.add_argument("--workload", choices=["mlem", "batch", "heldout", "xtt", "ttr", "fdkart", "grad"],
+    ap.add_argument("--workload", choices=["mlem", "batch", "heldout", "xtt", "ttr", "fdkart", "grad", "unet"],
                     default="mlem",
                     help="mlem: one MLEM iteration at 512^3/720 views (default); batch: 48 uneven MLEM problems, "
                          "30 epochs in persistent launches (HIP graph, autotuned tiles); heldout: time to "
@@ -525,6 +575,9 @@ def main():
         return
     if args.workload == "ttr":
         print(json.dumps(ttr_workload(xr, args)))
+        return
+    if args.workload == "unet":
+        print(json.dumps(unet_measure(xr, N=args.n_vox, reps=max(args.steps, 1))))
         return
     if args.workload == "xtt":
         print(json.dumps(xtt_measure(xr, steps=args.steps, warmup=args.warmup)))

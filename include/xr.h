@@ -475,6 +475,12 @@ int xr_unet_loss_grad(xr_unet* net, const float* x_dev, const float* t_dev, int 
 int xr_unet_adam_step(xr_unet* net, float lr, float beta1, float beta2, float eps);
 /* Scale s used by the last forward / loss_grad call. */
 float xr_unet_last_scale(const xr_unet* net);
+/* Inference fuses the decoder's nearest x2 upsampling into the following
+ * convolution's input fetch (default on), so the upsampled half of each skip
+ * concatenation is never written; with enable = 0 it is materialised first,
+ * as training does.  The output is bitwise the same either way.  No
+ * counterpart in the reference. */
+int xr_unet_set_fused_upsampling(xr_unet* net, int enable);
 /* Raw parameter file: 16-byte header ("XUNT", version 1, C, count) + FP32. */
 int xr_unet_save(xr_unet* net, const char* path);
 int xr_unet_load(xr_unet* net, const char* path);

@@ -79,6 +79,39 @@ def test_forward_matches_oracle(orc, xr, ctx, torch_dev, C, dims):
     net.close()
 
 
+# (4, 8, 32): upsampling fused at the first level only; (8, 8, 64) and
+# (8, 16, 128): at both decoder levels; (8, 12, 16): at neither
+@pytest.mark.parametrize("C,dims", [(8, (4, 8, 32)), (16, (8, 8, 64)), (24, (8, 16, 128)), (24, (8, 12, 16))])
+def test_fused_upsampling_is_bitwise_the_materialised_one(orc, xr, ctx, torch_dev, C, dims):
+    """Inference with the decoder's nearest x2 upsampling read in the next
+    convolution's halo fetch equals inference with the upsampled channels
+    written into the concatenation first, bit for bit; also after a training
+    step has left other values in the concatenation buffers."""
+    import torch
+
+    net = xr.UNet(ctx, C, seed=2)
+    net.set_params(random_params(orc, C, 60 + C))
+    rng = np.random.default_rng(7 + C)
+    x = dev(rng.uniform(0.0, 0.05, dims), torch_dev)
+    t = dev(rng.uniform(0.0, 0.05, dims), torch_dev)
+    outs = []
+    for fused in (False, True, False):
+        net.set_fused_upsampling(fused)
+        y = torch.empty_like(x)
+        net.forward(x, y, dims)
+        outs.append(y)
+    net.loss_grad(x, t, dims)
+    net.set_fused_upsampling(True)
+    y = torch.empty_like(x)
+    net.forward(x, y, dims)
+    outs.append(y)
+    torch.cuda.synchronize()
+    assert float((outs[0] - x).abs().max()) > 0  # the network part is not zero
+    for y in outs[1:]:
+        assert torch.equal(y, outs[0])
+    net.close()
+
+
 @pytest.mark.parametrize("C,dims", [(8, (8, 8, 8)), (24, (8, 12, 16)), (8, (4, 4, 32))])
 def test_gradients_match_oracle(orc, xr, ctx, torch_dev, C, dims):
     net = xr.UNet(ctx, C, seed=1)

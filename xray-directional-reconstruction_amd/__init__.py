@@ -178,6 +178,7 @@ def _load():
         "xr_unet_loss_grad": (I, [P, P, P, I, I, I, F, _c_double_p]),
         "xr_unet_adam_step": (I, [P, F, F, F, F]),
         "xr_unet_last_scale": (F, [P]),
+        "xr_unet_set_fused_upsampling": (I, [P, I]),
         "xr_unet_save": (I, [P, ctypes.c_char_p]),
         "xr_unet_load": (I, [P, ctypes.c_char_p]),
         "xr_batch_reconstruct": (I, [P, I, ctypes.POINTER(Geometry), ctypes.POINTER(ScanStruct), _c_int32_p,
@@ -720,6 +721,11 @@ class UNet:
 
     def last_scale(self):
         return float(_lib.xr_unet_last_scale(self.h))
+
+    def set_fused_upsampling(self, enable):
+        """Inference with the decoder's upsampling fused into the next
+        convolution's fetch (True, the default) or materialised (False)."""
+        self.ctx.check(_lib.xr_unet_set_fused_upsampling(self.h, 1 if enable else 0))
 
     def save(self, path):
         self.ctx.check(_lib.xr_unet_save(self.h, path.encode()))

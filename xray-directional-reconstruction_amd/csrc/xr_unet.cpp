@@ -77,6 +77,7 @@ struct xr_unet {
     int64_t step = 0;
     double* d_red = nullptr;  // [0] sum (scale), [1] loss
     float last_scale = 1.f;
+    bool fuse_upsampling = true;  // inference only (training reads the concatenations)
     // activations (forward) and gradients (backward), sized for the last dims
     Bf16Buf x8, a1, cat1, p1, b1, cat2, p2, c1, c2, d1, d2, e1, e2;
     Bf16Buf g_e2, g_e1, g_cat1, g_d2, g_d1, g_cat2, g_c2, g_c1, g_p2, g_b1, g_p1, g_a1;
@@ -164,8 +165,8 @@ int ensure_buffers(xr_unet* n, int64_t V1, bool training) {
     } while (0)
 
 // Forward convolution of layer i: in (ld, off) at dims -> out (ld, off), ReLU.
-int conv_fwd(xr_unet* n, int i, const __bf16* in, int in_ld, int in_off, __bf16* out, int out_ld, int out_off,
-             int D0, int D1, int D2) {
+ConvArgs conv_args(xr_unet* n, int i, const __bf16* in, int in_ld, int in_off, __bf16* out, int out_ld,
+                   int out_off, int D0, int D1, int D2) {
     const Layer& l = n->L[i];
     ConvArgs a{};
     a.in = in;
@@ -185,6 +186,34 @@ int conv_fwd(xr_unet* n, int i, const __bf16* in, int in_ld, int in_off, __bf16*
     a.D1 = D1;
     a.D2 = D2;
     a.relu = 1;
+    return a;
+}
+
+int conv_fwd(xr_unet* n, int i, const __bf16* in, int in_ld, int in_off, __bf16* out, int out_ld, int out_off,
+             int D0, int D1, int D2) {
+    XR_K(n->ctx, launch_conv3(conv_args(n, i, in, in_ld, in_off, out, out_ld, out_off, D0, D1, D2), n->ctx->stream));
+    return XR_OK;
+}
+
+// Decoder convolution i over the concatenation [upsample(low) | skip] (low:
+// half-resolution, up_C channels).  fused: the concatenation is read in place,
+// the nearest x2 upsampling in the convolution's halo fetch (the caller has
+// checked that the tiled kernel takes the dims); otherwise the upsampled
+// channels are first written into cat, whose channels [up_C, ...) hold the
+// skip (training: the weight gradient reads cat).  The output is the same.
+int conv_up(xr_unet* n, int i, bool fused, const __bf16* low, int up_C, const __bf16* skip, int skip_ld,
+            __bf16* cat, int cat_ld, __bf16* out, int out_ld, int D0, int D1, int D2) {
+    ConvArgs a;
+    if (fused) {
+        a = conv_args(n, i, skip, skip_ld, 0, out, out_ld, 0, D0, D1, D2);
+        a.up = low;
+        a.up_ld = up_C;
+        a.up_off = 0;
+        a.up_chunks = up_C / 8;
+    } else {
+        XR_K(n->ctx, launch_upsample(low, up_C, D0, D1, D2, cat, cat_ld, 0, n->ctx->stream));
+        a = conv_args(n, i, cat, cat_ld, 0, out, out_ld, 0, D0, D1, D2);
+    }
     XR_K(n->ctx, launch_conv3(a, n->ctx->stream));
     return XR_OK;
 }
@@ -253,19 +282,28 @@ int forward(xr_unet* n, const float* x, float* y, const float* t, int D0, int D1
     }
     n->last_scale = scale;
     XR_K(ctx, launch_input8(x, V1, 1.0f / scale, n->x8.p, ctx->stream));
+    // Inference reads each decoder concatenation in place (fused): the skips
+    // go to compact buffers that are free until the decoder's last
+    // convolution of their level (e2 at level 0, d2 at level 1); training
+    // writes them into the concatenations cat1 / cat2, which it keeps.
+    const bool fuse = !t && n->fuse_upsampling;
+    const bool fuse0 = fuse && conv3_tiled_ok(conv_args(n, 8, nullptr, 3 * C, 0, nullptr, C, 0, D0, D1, D2));
+    const bool fuse1 = fuse && conv3_tiled_ok(conv_args(n, 6, nullptr, 6 * C, 0, nullptr, 2 * C, 0, E0, E1, E2));
+    __bf16* const skip0 = fuse0 ? n->e2.p : n->cat1.p;
+    const int s0_ld = fuse0 ? C : 3 * C, s0_off = fuse0 ? 0 : 2 * C;
+    __bf16* const skip1 = fuse1 ? n->d2.p : n->cat2.p;
+    const int s1_ld = fuse1 ? 2 * C : 6 * C, s1_off = fuse1 ? 0 : 4 * C;
     XR_TRY(conv_fwd(n, 0, n->x8.p, 8, 0, n->a1.p, C, 0, D0, D1, D2));
-    XR_TRY(conv_fwd(n, 1, n->a1.p, C, 0, n->cat1.p, 3 * C, 2 * C, D0, D1, D2));
-    XR_K(ctx, launch_pool(n->cat1.p, 3 * C, 2 * C, C, D0, D1, D2, n->p1.p, ctx->stream));
+    XR_TRY(conv_fwd(n, 1, n->a1.p, C, 0, skip0, s0_ld, s0_off, D0, D1, D2));
+    XR_K(ctx, launch_pool(skip0, s0_ld, s0_off, C, D0, D1, D2, n->p1.p, ctx->stream));
     XR_TRY(conv_fwd(n, 2, n->p1.p, C, 0, n->b1.p, 2 * C, 0, E0, E1, E2));
-    XR_TRY(conv_fwd(n, 3, n->b1.p, 2 * C, 0, n->cat2.p, 6 * C, 4 * C, E0, E1, E2));
-    XR_K(ctx, launch_pool(n->cat2.p, 6 * C, 4 * C, 2 * C, E0, E1, E2, n->p2.p, ctx->stream));
+    XR_TRY(conv_fwd(n, 3, n->b1.p, 2 * C, 0, skip1, s1_ld, s1_off, E0, E1, E2));
+    XR_K(ctx, launch_pool(skip1, s1_ld, s1_off, 2 * C, E0, E1, E2, n->p2.p, ctx->stream));
     XR_TRY(conv_fwd(n, 4, n->p2.p, 2 * C, 0, n->c1.p, 4 * C, 0, F0, F1, F2));
     XR_TRY(conv_fwd(n, 5, n->c1.p, 4 * C, 0, n->c2.p, 4 * C, 0, F0, F1, F2));
-    XR_K(ctx, launch_upsample(n->c2.p, 4 * C, E0, E1, E2, n->cat2.p, 6 * C, 0, ctx->stream));
-    XR_TRY(conv_fwd(n, 6, n->cat2.p, 6 * C, 0, n->d1.p, 2 * C, 0, E0, E1, E2));
+    XR_TRY(conv_up(n, 6, fuse1, n->c2.p, 4 * C, skip1, s1_ld, n->cat2.p, 6 * C, n->d1.p, 2 * C, E0, E1, E2));
     XR_TRY(conv_fwd(n, 7, n->d1.p, 2 * C, 0, n->d2.p, 2 * C, 0, E0, E1, E2));
-    XR_K(ctx, launch_upsample(n->d2.p, 2 * C, D0, D1, D2, n->cat1.p, 3 * C, 0, ctx->stream));
-    XR_TRY(conv_fwd(n, 8, n->cat1.p, 3 * C, 0, n->e1.p, C, 0, D0, D1, D2));
+    XR_TRY(conv_up(n, 8, fuse0, n->d2.p, 2 * C, skip0, s0_ld, n->cat1.p, 3 * C, n->e1.p, C, D0, D1, D2));
     XR_TRY(conv_fwd(n, 9, n->e1.p, C, 0, n->e2.p, C, 0, D0, D1, D2));
     if (t) XR_K(ctx, launch_fill(reinterpret_cast<float*>(n->d_red + 1), 2, 0.0f, ctx->stream));
     XR_K(ctx, launch_head(n->e2.p, C, V1, n->d_p + n->head_off, x, scale, y, t, t ? n->d_ghead : nullptr,
@@ -437,6 +475,12 @@ int xr_unet_adam_step(xr_unet* n, float lr, float beta1, float beta2, float eps)
 }
 
 float xr_unet_last_scale(const xr_unet* n) { return n ? n->last_scale : 0.f; }
+
+int xr_unet_set_fused_upsampling(xr_unet* n, int enable) {
+    if (!n) return XR_ERR_INVALID;
+    n->fuse_upsampling = enable != 0;
+    return XR_OK;
+}
 
 int xr_unet_save(xr_unet* n, const char* path) {
     if (!n || !path) return XR_ERR_INVALID;

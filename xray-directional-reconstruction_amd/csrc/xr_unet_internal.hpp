@@ -28,7 +28,18 @@ struct ConvArgs {
     // LDS-tiled kernel (used when the dims are multiples of its tile)
     const __bf16* wt;
     int Ktpad;
+    // optional (LDS-tiled kernel only): a concatenation [upsample(up) | in]
+    // read in place, with the nearest x2 upsampling fused into the halo fetch.
+    // Input channels ci < 8*up_chunks are read from the half-resolution tensor
+    // up[(v0/2, v1/2, v2/2) * up_ld + up_off + ci] (dims D0/2 x D1/2 x D2/2),
+    // the others from in[v * in_ld + in_off + ci - 8*up_chunks].
+    const __bf16* up;
+    int up_ld, up_off, up_chunks;
 };
+
+// True when launch_conv3 runs a on the LDS-tiled kernel (the only one that
+// takes a fused upsampling source).
+bool conv3_tiled_ok(const ConvArgs& a);
 
 // dw[co * ldw + k] += sum_v gout[v][g_off + co] im2col(in)[v][k]  (k < 27*Cin).
 struct WgradArgs {

@@ -144,7 +144,11 @@ constexpr int kTapsPad = 28;
 // concatenations, 96-byte voxel records) stage two chunks at once, so that
 // both 16-byte pieces of a record are fetched together while its cache line
 // is resident, instead of the line being fetched again for the next chunk.
-template <int NT, int CPS>
+// UP: the first a.up_chunks chunks come from the half-resolution tensor a.up
+// at the halo voxel's parent (nearest x2 upsampling), the rest from a.in, so
+// the decoder's concatenation is never written; the halo in LDS, and hence
+// the result, is the same as with the concatenation materialised.
+template <int NT, int CPS, bool UP>
 __global__ __launch_bounds__(256) void k_conv3_tiled(const ConvArgs a) {
     __shared__ bf16x8 halo[CPS * kCHalo];
     const lds_bf16x8* const hl = (const lds_bf16x8*)halo;
@@ -171,7 +175,7 @@ __global__ __launch_bounds__(256) void k_conv3_tiled(const ConvArgs a) {
     // halo voxels of this thread: i = threadIdx.x + 256 j; their global offsets
     // (-1: outside the volume, zero padding) are the same for every chunk
     constexpr int kPer = (kCHalo + 255) / 256;
-    int64_t hoff[kPer];
+    int64_t hoff[kPer], uoff[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
         const int i = threadIdx.x + 256 * j;
@@ -180,7 +184,10 @@ __global__ __launch_bounds__(256) void k_conv3_tiled(const ConvArgs a) {
         const bool ok = i < kCHalo && (unsigned)g0 < (unsigned)a.D0 && (unsigned)g1 < (unsigned)a.D1 &&
                         (unsigned)g2 < (unsigned)a.D2;
         hoff[j] = ok ? (((int64_t)g0 * a.D1 + g1) * a.D2 + g2) * a.in_ld : -1;
+        if (UP)
+            uoff[j] = ok ? (((int64_t)(g0 >> 1) * (a.D1 >> 1) + (g1 >> 1)) * (a.D2 >> 1) + (g2 >> 1)) * a.up_ld : -1;
     }
+    const __bf16* __restrict__ up = UP ? a.up + a.up_off : nullptr;
     // stage st holds chunks st*CPS .. st*CPS + CPS-1 (chunks past the last are zero)
     bf16x8 pre[CPS][kPer];
     auto fetch = [&](int st) {
@@ -188,8 +195,11 @@ __global__ __launch_bounds__(256) void k_conv3_tiled(const ConvArgs a) {
         for (int c = 0; c < CPS; c++) {
             const int cc = st * CPS + c;
 #pragma unroll
-            for (int j = 0; j < kPer; j++)
-                pre[c][j] = (cc < nchunk && hoff[j] >= 0) ? *(const bf16x8*)(in + hoff[j] + cc * 8) : zero8();
+            for (int j = 0; j < kPer; j++) {
+                const __bf16* src = (UP && cc < a.up_chunks) ? up + uoff[j] + cc * 8
+                                                             : in + hoff[j] + (cc - (UP ? a.up_chunks : 0)) * 8;
+                pre[c][j] = (cc < nchunk && hoff[j] >= 0) ? *(const bf16x8*)src : zero8();
+            }
         }
     };
     auto store = [&] {
@@ -577,23 +587,39 @@ inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs)
 
 }  // namespace
 
+bool conv3_tiled_ok(const ConvArgs& a) {
+    const int nt = (a.Cout + 15) / 16;
+    return a.wt && a.D0 % kCT0 == 0 && a.D1 % kCT1 == 0 && a.D2 % kCT2 == 0 &&
+           a.Ktpad == a.Cin / 8 * kTapsPad * 8 && nt <= 6;
+}
+
 hipError_t launch_conv3(const ConvArgs& a, hipStream_t s) {
     const int64_t V = (int64_t)a.D0 * a.D1 * a.D2;
     if (V == 0) return hipSuccess;
     if (a.Cin % 8 != 0 || a.in_ld % 8 != 0 || a.in_off % 8 != 0 || a.Kpad % 32 != 0 || a.Kpad < 27 * a.Cin)
         return hipErrorInvalidValue;
     const int nt = (a.Cout + 15) / 16;
-    if (a.wt && a.D0 % kCT0 == 0 && a.D1 % kCT1 == 0 && a.D2 % kCT2 == 0 && a.Ktpad == a.Cin / 8 * kTapsPad * 8 &&
-        nt <= 6) {
+    const bool up = a.up_chunks > 0;
+    if (up && (!conv3_tiled_ok(a) || !a.up || a.up_chunks > a.Cin / 8 || a.up_ld % 8 != 0 || a.up_off % 8 != 0 ||
+               a.up_off + 8 * a.up_chunks > a.up_ld))
+        return hipErrorInvalidValue;
+    if (conv3_tiled_ok(a)) {
         const dim3 tgrid((unsigned)(V / (kCT0 * kCT1 * kCT2)));
         // several chunks per stage for inputs of 32 or more channels
         const int cps = (a.Cin >= 32 && nt <= 3) ? (nt <= 2 ? 3 : 2) : 1;
         switch (nt) {
-#define XR_CONVT(N)                                                                          \
-    case N:                                                                                  \
-        if (cps == 3) hipLaunchKernelGGL((k_conv3_tiled<N, 3>), tgrid, dim3(256), 0, s, a);     \
-        else if (cps == 2) hipLaunchKernelGGL((k_conv3_tiled<N, 2>), tgrid, dim3(256), 0, s, a); \
-        else hipLaunchKernelGGL((k_conv3_tiled<N, 1>), tgrid, dim3(256), 0, s, a);              \
+#define XR_CONVT_UP(N, C)                                                                  \
+    if (up) hipLaunchKernelGGL((k_conv3_tiled<N, C, true>), tgrid, dim3(256), 0, s, a); \
+    else hipLaunchKernelGGL((k_conv3_tiled<N, C, false>), tgrid, dim3(256), 0, s, a);
+#define XR_CONVT(N)                 \
+    case N:                         \
+        if (cps == 3) {             \
+            XR_CONVT_UP(N, 3)       \
+        } else if (cps == 2) {      \
+            XR_CONVT_UP(N, 2)       \
+        } else {                    \
+            XR_CONVT_UP(N, 1)       \
+        }                           \
         break;
             XR_CONVT(1)
             XR_CONVT(2)
@@ -602,6 +628,7 @@ hipError_t launch_conv3(const ConvArgs& a, hipStream_t s) {
             XR_CONVT(5)
             XR_CONVT(6)
 #undef XR_CONVT
+#undef XR_CONVT_UP
         }
         return hipGetLastError();
     }
