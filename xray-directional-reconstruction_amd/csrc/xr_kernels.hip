@@ -1221,6 +1221,79 @@ __device__ __forceinline__ Foot foot_from_extent(float umn, float umx, float vmn
     return f;
 }
 
+// Word 3 of a raw buffer descriptor on gfx9-family GPUs (32-bit data, no
+// swizzle, stride 0): the range check is on the byte offset alone.
+constexpr int kBufferWord3 = 0x00020000;
+
+// Staging of one view's footprint (W x H texels at (iu_lo, iv_lo) of the
+// view's U-wide image) into an LDS buffer of row stride Ws, in passes of
+// R = threads / W rows: thread t owns column c = t % W of rows r0 + k R,
+// r0 = t / W (one division per view).  Only the passes the footprint needs
+// are issued and stored (uniform tests); only the last needs a per-lane row
+// test.  The LDS contents are those of a plain row-by-row copy.
+// BUF: a pass is one raw buffer load at a per-lane byte offset whose range is
+// the view's image, so a row below the image's last reads 0 (it lies outside
+// the footprint and is never stored): no address select, no predication, one
+// add per pass.  Otherwise (kernels whose scalar registers are already
+// scarce: the buffer descriptor takes four) a global load per pass, lanes
+// outside the footprint reading its first texel.
+template <int NP, bool BUF>
+struct FootStage {
+    float pre[NP];
+    int r0, lds0;  // lds0 = r0 Ws + c; r0 >= R: the thread stages nothing
+
+    __device__ __forceinline__ void fetch(const float* img, int img_bytes, const Foot& f, int U) {
+        r0 = (int)((float)threadIdx.x * rcpf((float)f.W));  // within 1 of t / W: corrected below
+        int c = (int)threadIdx.x - r0 * f.W;
+        if (c < 0) {
+            r0--;
+            c += f.W;
+        } else if (c >= f.W) {
+            r0++;
+            c -= f.W;
+        }
+        lds0 = r0 * f.Ws + c;
+        if constexpr (BUF) {
+            // the image (one view's, the same for the whole block) in scalar
+            // registers, as a buffer descriptor must be
+            const uint64_t ip = (uint64_t)(uintptr_t)img;
+            const uint64_t up = ((uint64_t)(unsigned)uniformi((int)(ip >> 32)) << 32) | (unsigned)uniformi((int)ip);
+            const auto rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)(uintptr_t)up, (short)0, uniformi(img_bytes),
+                                                                kBufferWord3);
+            unsigned voff = ((unsigned)(f.iv_lo + r0) * (unsigned)U + (unsigned)(f.iu_lo + c)) * 4u;
+            // opaque to the optimiser, so that pass k adds the uniform k * step
+            // (one add) instead of recomputing (row + k R) * U (a quarter-rate multiply)
+            asm volatile("" : "+v"(voff));
+            const unsigned step = (unsigned)f.R * (unsigned)U * 4u;
+#pragma unroll
+            for (int k = 0; k < NP; k++)
+                if (k * f.R < f.H)
+                    pre[k] =
+                        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff + k * step, 0, 0));
+        } else {
+            const auto g = to_global(img);
+            const bool act = r0 < f.R;
+            const auto src = g + (int64_t)(f.iv_lo + r0) * U + f.iu_lo + c;
+            const auto safe = g + (int64_t)f.iv_lo * U + f.iu_lo;  // inside the image
+            const int64_t step = (int64_t)f.R * U;
+#pragma unroll
+            for (int k = 0; k < NP; k++)
+                if (k * f.R < f.H) pre[k] = *((act && r0 + k * f.R < f.H) ? src + k * step : safe);
+        }
+    }
+    __device__ __forceinline__ void store(lds_float* buf, const Foot& f) const {
+        if (r0 >= f.R) return;
+        const int rs = f.R * f.Ws;
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            if ((k + 1) * f.R <= f.H)
+                buf[lds0 + k * rs] = pre[k];
+            else if (k * f.R < f.H && r0 + k * f.R < f.H)
+                buf[lds0 + k * rs] = pre[k];
+        }
+    }
+};
+
 // Fused update of a backprojection tile: every voxel of the lane's columns is
 // read and written once (raw accumulate, MLEM ratio, ART step or FDK sum).
 // Invariant relied on by the solvers: the MLEM / ART tiles of a launch cover
@@ -1438,37 +1511,10 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
     };
 
     // staging of one view's footprint: prefetch into registers, store later
-    float pre[kStagePer];
-    int pidx[kStagePer];
+    FootStage<kStagePer, true> st;
+    const int img_bytes = d.V * d.U * 4;
     auto fetch = [&](const Foot& f, int code) {
-        const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
-        int r0 = (int)((float)threadIdx.x * rcpf((float)f.W));  // within 1 of t / W: corrected below
-        int c = (int)threadIdx.x - r0 * f.W;
-        if (c < 0) {
-            r0--;
-            c += f.W;
-        } else if (c >= f.W) {
-            r0++;
-            c -= f.W;
-        }
-        const bool act = r0 < f.R;
-        const auto src = img + (int64_t)(f.iv_lo + r0) * d.U + f.iu_lo + c;
-        // texel (iu_lo, iv_lo) of the footprint, inside the image: the address a
-        // lane outside the footprint loads from, so that no load is predicated
-        const auto safe = img + (int64_t)f.iv_lo * d.U + f.iu_lo;
-        const int64_t rstep = (int64_t)f.R * d.U;
-#pragma unroll
-        for (int k = 0; k < kStagePer; k++) {
-            const int r = r0 + k * f.R;
-            const bool in = act && r < f.H;
-            pidx[k] = in ? r * f.Ws + c : -1;
-            pre[k] = *(in ? src + k * rstep : safe);  // not stored unless in
-        }
-    };
-    auto store = [&](lds_float* buf) {
-#pragma unroll
-        for (int k = 0; k < kStagePer; k++)
-            if (pidx[k] >= 0) buf[pidx[k]] = pre[k];
+        st.fetch(a.proj + (int64_t)(code - a.code_base) * d.V * d.U, img_bytes, f, d.U);
     };
 
     if (wave == 0) foot_batch(0, 0);
@@ -1478,7 +1524,7 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
     Foot f = load_foot(0, 0);
     if (f.staged) {
         fetch(f, code);
-        store(lds);
+        st.store(lds, f);
     }
     __syncthreads();
 
@@ -1577,7 +1623,7 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
             }
         }
         if (s + 1 < nv) {
-            if (nf.staged) store(nxt);
+            if (nf.staged) st.store(nxt, nf);
             code = ncode;
             f = nf;
         }
@@ -1940,39 +1986,10 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
     for (int c = 0; c < NC; c++) nwhole[c] = 0;
 
     // staging of one view's footprint: prefetch into registers, store later
-    float pre[kStagePer];
-    int pidx[kStagePer];
+    FootStage<kStagePer, !XTT> st;
+    const int img_bytes = d.V * d.U * 4;
     auto fetch = [&](const Foot& f, int code) {
-        // rows of the footprint in passes of R = kBwdThreads / W rows: thread t
-        // owns column t % W of rows t / W + k*R (one division per view)
-        const auto img = to_global(a.proj + (int64_t)(code - a.code_base) * d.V * d.U);
-        int r0 = (int)((float)threadIdx.x * rcpf((float)f.W));  // within 1 of t / W: corrected below
-        int c = (int)threadIdx.x - r0 * f.W;
-        if (c < 0) {
-            r0--;
-            c += f.W;
-        } else if (c >= f.W) {
-            r0++;
-            c -= f.W;
-        }
-        const bool act = r0 < f.R;
-        const auto src = img + (int64_t)(f.iv_lo + r0) * d.U + f.iu_lo + c;
-        // texel (iu_lo, iv_lo) of the footprint, inside the image: the address a
-        // lane outside the footprint loads from, so that no load is predicated
-        const auto safe = img + (int64_t)f.iv_lo * d.U + f.iu_lo;
-        const int64_t step = (int64_t)f.R * d.U;
-#pragma unroll
-        for (int k = 0; k < kStagePer; k++) {
-            const int r = r0 + k * f.R;
-            const bool in = act && r < f.H;
-            pidx[k] = in ? r * f.Ws + c : -1;
-            pre[k] = *(in ? src + k * step : safe);  // not stored unless in
-        }
-    };
-    auto store = [&](lds_float* buf) {
-#pragma unroll
-        for (int k = 0; k < kStagePer; k++)
-            if (pidx[k] >= 0) buf[pidx[k]] = pre[k];
+        st.fetch(a.proj + (int64_t)(code - a.code_base) * d.V * d.U, img_bytes, f, d.U);
     };
 
     if (a.n_views == 0) return 0.f;
@@ -1986,7 +2003,7 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
     lds_float* const lds = (lds_float*)stage;
     if (f.staged) {
         fetch(f, code);
-        store(lds);
+        st.store(lds, f);
     }
     __syncthreads();
 
@@ -2202,7 +2219,7 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
             }
         }
         if (s + 1 < a.n_views) {
-            if (nf.staged) store(nxt);
+            if (nf.staged) st.store(nxt, nf);
             code = ncode;
             g = ng;
             f = nf;
