@@ -1382,7 +1382,8 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
     const int gx = a.gx;
     const int bcx = CPW * gx, bcy = CPW * a.gy;  // block columns along x, y
     const int tx = o % a.tiles_x, ty = (o / a.tiles_x) % a.tiles_y, tz = o / (a.tiles_x * a.tiles_y);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index as a wave-uniform value: the column tests and counts below stay scalar
+    const int lane = threadIdx.x & 63, wave = uniformi(threadIdx.x >> 6);
     const int xb = tx * bcx + (wave % gx) * CPW, yb = ty * bcy + (wave / gx) * CPW;
     const int z0 = tz * 64 * ZPL;
     const float bx0 = (float)(tx * bcx) - d.xc, bx1 = (float)min(tx * bcx + bcx - 1, d.N - 1) - d.xc;
@@ -1554,6 +1555,10 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
                 const float Bc = tp[0], Ac = tp[1], fu = tp[2], hi = tp[3], wcol = tp[4];
                 const int lu = ti[5], iu = ti[6], boff = ti[7];
                 const float a0 = 1.0f - fu;
+                // the column's samples; both branches leave them in nbq and the
+                // accumulation after them is shared, so the accumulators are
+                // updated in place (no copies where the branches meet)
+                float nbq[ZPL];
                 if (uniformi(__float_as_int(hi)) == 0x7f800000) {
                     // all reads of the column first (LDS latency), then the sums
                     const unsigned ws4 = (unsigned)f.Ws * 4u;
@@ -1575,50 +1580,48 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
                     for (int q = 0; q < ZPL; q++) {
                         const float top = fmaf(fu, r[q][1], a0 * r[q][0]);
                         const float bot = fmaf(fu, r[q][3], a0 * r[q][2]);
-                        const float nb = fmaf(fv[q], bot - top, top);
-                        if (KIND == BWD_FDK)
-                            tmp[cidx][q][0] = fmaf(wcol, nb, tmp[cidx][q][0]);
-                        else
-                            tmp[cidx][q][0] += nb;
+                        nbq[q] = fmaf(fv[q], bot - top, top);
                     }
                     if (KIND != BWD_FDK) nwhole[cidx]++;
-                    continue;
+                } else {
+                    // Window test per voxel: 0.55 < v < V - 0.55 becomes 0.05 < t < hi
+                    // (hi < 0 rejects the column).  Branch free: a rejected voxel reads
+                    // an in-bounds texel and adds zero.  Lanes with z >= N accumulate
+                    // but are never stored.
+#pragma unroll
+                    for (int q = 0; q < ZPL; q++) {
+                        const float t = fmaf(Bc, zpq[q], Ac);
+                        const bool ok = t > 0.05f && t < hi;
+                        const float fl = floorf(t);
+                        const float fv = t - fl;
+                        const int iv = (int)fl;
+                        float r00, r01, r10, r11;
+                        if (f.staged) {
+                            const int qi = (ok ? iv - f.iv_lo : 0) * f.Ws + lu;
+                            r00 = cur[qi];
+                            r01 = cur[qi + 1];
+                            r10 = cur[qi + f.Ws];
+                            r11 = cur[qi + f.Ws + 1];
+                        } else {
+                            // direct reads: a distinct (nontemporal) load form next to the LDS reads
+                            const int64_t qi = (int64_t)min(max(iv, 0), d.V - 2) * d.U + min(max(iu, 0), d.U - 2);
+                            r00 = __builtin_nontemporal_load(img + qi);
+                            r01 = __builtin_nontemporal_load(img + qi + 1);
+                            r10 = __builtin_nontemporal_load(img + qi + d.U);
+                            r11 = __builtin_nontemporal_load(img + qi + d.U + 1);
+                        }
+                        const float top = fmaf(fu, r01, a0 * r00);
+                        const float bot = fmaf(fu, r11, a0 * r10);
+                        nbq[q] = ok ? fmaf(fv, bot - top, top) : 0.0f;
+                        if (KIND != BWD_FDK) fac[cidx][q][0] += ok ? 1.0f : 0.0f;
+                    }
                 }
-                // Window test per voxel: 0.55 < v < V - 0.55 becomes 0.05 < t < hi
-                // (hi < 0 rejects the column).  Branch free: a rejected voxel reads
-                // an in-bounds texel and adds zero.  Lanes with z >= N accumulate
-                // but are never stored.
 #pragma unroll
                 for (int q = 0; q < ZPL; q++) {
-                    const float t = fmaf(Bc, zpq[q], Ac);
-                    const bool ok = t > 0.05f && t < hi;
-                    const float fl = floorf(t);
-                    const float fv = t - fl;
-                    const int iv = (int)fl;
-                    float r00, r01, r10, r11;
-                    if (f.staged) {
-                        const int qi = (ok ? iv - f.iv_lo : 0) * f.Ws + lu;
-                        r00 = cur[qi];
-                        r01 = cur[qi + 1];
-                        r10 = cur[qi + f.Ws];
-                        r11 = cur[qi + f.Ws + 1];
-                    } else {
-                        // direct reads: a distinct (nontemporal) load form next to the LDS reads
-                        const int64_t qi = (int64_t)min(max(iv, 0), d.V - 2) * d.U + min(max(iu, 0), d.U - 2);
-                        r00 = __builtin_nontemporal_load(img + qi);
-                        r01 = __builtin_nontemporal_load(img + qi + 1);
-                        r10 = __builtin_nontemporal_load(img + qi + d.U);
-                        r11 = __builtin_nontemporal_load(img + qi + d.U + 1);
-                    }
-                    const float top = fmaf(fu, r01, a0 * r00);
-                    const float bot = fmaf(fu, r11, a0 * r10);
-                    const float nb = ok ? fmaf(fv, bot - top, top) : 0.0f;
-                    if (KIND == BWD_FDK) {
-                        tmp[cidx][q][0] = fmaf(wcol, nb, tmp[cidx][q][0]);
-                    } else {
-                        tmp[cidx][q][0] += nb;
-                        fac[cidx][q][0] += ok ? 1.0f : 0.0f;
-                    }
+                    if (KIND == BWD_FDK)
+                        tmp[cidx][q][0] = fmaf(wcol, nbq[q], tmp[cidx][q][0]);
+                    else
+                        tmp[cidx][q][0] += nbq[q];
                 }
             }
         }
