@@ -194,7 +194,7 @@ inline int forward_split(int64_t items, int64_t slots, int tile) {
     return s;
 }
 int batch_blocks_per_cu_backward(int kind, int zpl, int kmax = 0);
-size_t backward_batch_lds_bytes();
+size_t backward_batch_lds_bytes(int kind);
 // out[i] = max |segs[i][0, seg_len[i])| (out zeroed beforehand).
 // blocks per segment for k_absmax_seg over n_seg segments of at most max_len floats
 int absmax_blocks(int64_t max_len, int n_seg);

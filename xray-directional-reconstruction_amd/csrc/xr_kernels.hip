@@ -1148,7 +1148,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) __attribute__((amdgpu_waves_per_eu(
 }
 
 // ---------------------------------------------------------------------------
-// Backprojection gather with fused update.  Block = kBwdWaves waves laid out
+// Backprojection gather with fused update.  Block = BwdBlock waves laid out
 // as a gx x gy grid of CPW x CPW column groups; every lane owns ZPL voxels
 // (z = z0 + lane + 64 q) in each of its CPW^2 columns, so the per-column
 // geometry of a view is computed once per ZPL voxels.  The block's detector
@@ -1157,14 +1157,24 @@ __global__ __launch_bounds__(64 * kFwdWaves) __attribute__((amdgpu_waves_per_eu(
 // one barrier per view.  The update (MLEM / ART / FDK / raw) is applied at the
 // end: each voxel is read and written once per subset.
 // ---------------------------------------------------------------------------
-constexpr int kBwdWaves = 16;
-constexpr int kBwdThreads = 64 * kBwdWaves;
-constexpr int kStageBuf = 8192;                    // floats per staging buffer
-constexpr int kStagePer = kStageBuf / kBwdThreads;  // staged floats per thread
-
 template <int KIND>
 struct IsXtt {
     static constexpr bool value = (KIND == BWD_XTT_RAW || KIND == BWD_XTT_MLEM || KIND == BWD_XTT_ART);
+};
+
+// Block shape per kind.  The plain kinds (about 110 VGPRs, 4 waves per SIMD)
+// run one 16-wave block per CU; XTT runs 8-wave blocks, two per CU, so one
+// block accumulates while the other waits at its per-view barrier (its
+// footprints are small: 8 columns x 64 z).  Measured: XTT 150 against 167 ms
+// per backprojection at 256^3 x 13 coefficients, 4 x 360 views; the plain
+// kinds in 8-wave blocks (half the tile, 6144-float staging) 70 against 67 ms
+// at 512^3 / 720 views.
+template <int KIND>
+struct BwdBlock {
+    static constexpr int waves = IsXtt<KIND>::value ? 8 : 16;
+    static constexpr int threads = 64 * waves;
+    static constexpr int stage_buf = IsXtt<KIND>::value ? 6144 : 8192;  // floats per staging buffer
+    static constexpr int stage_per = stage_buf / threads;              // staged floats per thread
 };
 
 struct Foot {
@@ -1172,10 +1182,12 @@ struct Foot {
     bool staged, empty;
 };
 
+template <int KIND>
 __device__ __forceinline__ Foot foot_from_extent(float umn, float umx, float vmn, float vmx, int bad, const Dims& d);
 
 // Footprint of the block box on view g: the projective map's extremes over a box
 // are at its corners while den > 0 there (all lanes compute it identically).
+template <int KIND>
 __device__ __forceinline__ Foot footprint(const ViewGeom& g, const Dims& d, float bx0, float bx1, float by0, float by1,
                                           float bz0, float bz1, int lane) {
     const int ci = lane & 7;
@@ -1195,11 +1207,12 @@ __device__ __forceinline__ Foot footprint(const ViewGeom& g, const Dims& d, floa
         vmx = fmaxf(vmx, __shfl_xor(vmx, off));
         bad |= __shfl_xor(bad, off);
     }
-    return foot_from_extent(umn, umx, vmn, vmx, bad, d);
+    return foot_from_extent<KIND>(umn, umx, vmn, vmx, bad, d);
 }
 
 // Footprint from the extremes of the projected corners (bad: a corner behind
 // the source or far off the detector, so the extremes bound nothing).
+template <int KIND>
 __device__ __forceinline__ Foot foot_from_extent(float umn, float umx, float vmn, float vmx, int bad, const Dims& d) {
     Foot f{0, 0, 0, 0, 0, 0, false, false};
     if (bad) return f;  // direct (unstaged) reads
@@ -1214,10 +1227,10 @@ __device__ __forceinline__ Foot foot_from_extent(float umn, float umx, float vmn
         return f;
     }
     f.Ws = f.W | 1;
-    f.R = kBwdThreads / f.W;
+    f.R = BwdBlock<KIND>::threads / f.W;
     // the last word of the second buffer is reserved (work-queue ticket of the
-    // persistent launch), so a staged footprint holds at most kStageBuf-1 floats
-    f.staged = (f.Ws * f.H <= kStageBuf - 1) && (f.H <= kStagePer * f.R);
+    // persistent launch), so a staged footprint holds at most stage_buf-1 floats
+    f.staged = (f.Ws * f.H <= BwdBlock<KIND>::stage_buf - 1) && (f.H <= BwdBlock<KIND>::stage_per * f.R);
     return f;
 }
 
@@ -1367,7 +1380,10 @@ constexpr int kTermWords = 8;    // Bc, Ac, fu, hi (+inf: whole z extent accepte
 constexpr int kTermCols = 4;     // columns per wave (CPW = 2)
 constexpr int kBwdFootArea = 2 * kBwdBatch * kFootWords;          // words, block-wide, two batches
 constexpr int kBwdTermArea = kBwdBatch * kTermCols * kTermWords;  // words per wave
-constexpr int kBwdExtraWords = kBwdFootArea + kBwdWaves * kBwdTermArea;
+template <int KIND>
+constexpr int bwd_extra_words() {
+    return kBwdFootArea + BwdBlock<KIND>::waves * kBwdTermArea;
+}
 static_assert(64 % kTermCols == 0 && 64 / kTermCols >= kBwdBatch, "one lane per (view, column) of a batch");
 
 typedef __attribute__((address_space(3))) int lds_int;
@@ -1408,8 +1424,8 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
     for (int q = 0; q < ZPL; q++) zpq[q] = (float)(z0 + lane + 64 * q) - d.xc;
 
     lds_float* const lds = (lds_float*)stage;
-    lds_int* const foot = (lds_int*)(stage + 2 * kStageBuf);
-    lds_float* const term = (lds_float*)(stage + 2 * kStageBuf + kBwdFootArea + wave * kBwdTermArea);
+    lds_int* const foot = (lds_int*)(stage + 2 * BwdBlock<KIND>::stage_buf);
+    lds_float* const term = (lds_float*)(stage + 2 * BwdBlock<KIND>::stage_buf + kBwdFootArea + wave * kBwdTermArea);
 
     // footprints of views b0 .. b0+15 into buffer buf: lane = 4 x view + corner
     // pair (x, y corner; both z corners), extremes over the 4 lanes of a view
@@ -1438,7 +1454,7 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
             vmx = fmaxf(vmx, __shfl_xor(vmx, off));
             bad |= __shfl_xor(bad, off);
         }
-        const Foot f = foot_from_extent(umn, umx, vmn, vmx, bad, d);
+        const Foot f = foot_from_extent<KIND>(umn, umx, vmn, vmx, bad, d);
         if (c == 0) {
             lds_int* p = foot + (buf * kBwdBatch + j) * kFootWords;
             p[0] = f.iu_lo;
@@ -1512,7 +1528,7 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
     };
 
     // staging of one view's footprint: prefetch into registers, store later
-    FootStage<kStagePer, true> st;
+    FootStage<BwdBlock<KIND>::stage_per, true> st;
     const int img_bytes = d.V * d.U * 4;
     auto fetch = [&](const Foot& f, int code) {
         st.fetch(a.proj + (int64_t)(code - a.code_base) * d.V * d.U, img_bytes, f, d.U);
@@ -1535,8 +1551,8 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
         wave_sync();
         // the next batch's footprints, read from the next iteration on
         if (wave == 0 && j == kBwdBatch - 2 && s + 2 < nv) foot_batch(s + 2, ((s + 2) / kBwdBatch) & 1);
-        lds_float* cur = lds + (s & 1) * kStageBuf;
-        lds_float* nxt = lds + ((s + 1) & 1) * kStageBuf;
+        lds_float* cur = lds + (s & 1) * BwdBlock<KIND>::stage_buf;
+        lds_float* nxt = lds + ((s + 1) & 1) * BwdBlock<KIND>::stage_buf;
         int ncode = 0;
         Foot nf{0, 0, 0, 0, 0, 0, false, false};
         if (s + 1 < nv) {
@@ -1689,8 +1705,8 @@ __device__ __forceinline__ float backward_item_untilted_xtt(const BwdArgs& a, in
     for (int q = 0; q < ZPL; q++) zpq[q] = (float)(z0 + lane + 64 * q) - d.xc;
 
     lds_float* const lds = (lds_float*)stage;
-    lds_int* const foot = (lds_int*)(stage + 2 * kStageBuf);
-    lds_float* const term = (lds_float*)(stage + 2 * kStageBuf + kBwdFootArea + wave * kBwdTermArea);
+    lds_int* const foot = (lds_int*)(stage + 2 * BwdBlock<KIND>::stage_buf);
+    lds_float* const term = (lds_float*)(stage + 2 * BwdBlock<KIND>::stage_buf + kBwdFootArea + wave * kBwdTermArea);
     // staged only when the footprint fits the fetch passes of this path
     auto staged_here = [](int fl, int H, int R) { return (fl & 1) != 0 && H <= kXttStagePer * R; };
 
@@ -1719,7 +1735,7 @@ __device__ __forceinline__ float backward_item_untilted_xtt(const BwdArgs& a, in
             vmx = fmaxf(vmx, __shfl_xor(vmx, off));
             bad |= __shfl_xor(bad, off);
         }
-        const Foot f = foot_from_extent(umn, umx, vmn, vmx, bad, d);
+        const Foot f = foot_from_extent<KIND>(umn, umx, vmn, vmx, bad, d);
         if (c == 0) {
             lds_int* p = foot + (buf * kBwdBatch + j) * kFootWords;
             p[0] = f.iu_lo;
@@ -1841,8 +1857,8 @@ __device__ __forceinline__ float backward_item_untilted_xtt(const BwdArgs& a, in
         if (j == 0 && s > 0) term_batch(s, (s / kBwdBatch) & 1);
         wave_sync();
         if (wave == 0 && j == kBwdBatch - 2 && s + 2 < nv) foot_batch(s + 2, ((s + 2) / kBwdBatch) & 1);
-        lds_float* cur = lds + (s & 1) * kStageBuf;
-        lds_float* nxt = lds + ((s + 1) & 1) * kStageBuf;
+        lds_float* cur = lds + (s & 1) * BwdBlock<KIND>::stage_buf;
+        lds_float* nxt = lds + ((s + 1) & 1) * BwdBlock<KIND>::stage_buf;
         int ncode = 0;
         Foot nf{0, 0, 0, 0, 0, 0, false, false};
         if (s + 1 < nv) {
@@ -1944,7 +1960,7 @@ __device__ __forceinline__ float backward_item_untilted_xtt(const BwdArgs& a, in
 }
 
 // One voxel tile o of a backprojection launch; step selects the view list
-// views + step*n_views (a solver's subset step).  stage: 2 x kStageBuf floats.
+// views + step*n_views (a solver's subset step).  stage: 2 x stage_buf floats.
 // Returns the lane's max |updated voxel| (MLEM / ART kinds).
 template <int KIND, bool AXIS, int CPW, int ZPL, int KM>
 __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step, float* stage) {
@@ -1989,7 +2005,7 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
     for (int c = 0; c < NC; c++) nwhole[c] = 0;
 
     // staging of one view's footprint: prefetch into registers, store later
-    FootStage<kStagePer, !XTT> st;
+    FootStage<BwdBlock<KIND>::stage_per, !XTT> st;
     const int img_bytes = d.V * d.U * 4;
     auto fetch = [&](const Foot& f, int code) {
         st.fetch(a.proj + (int64_t)(code - a.code_base) * d.V * d.U, img_bytes, f, d.U);
@@ -2002,7 +2018,7 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
     for (int q = 0; q < ZPL; q++) zpq[q] = (float)(z0 + lane + 64 * q) - d.xc;
     int code = views[0];
     ViewGeom g = vg[code];
-    Foot f = footprint(g, d, bx0, bx1, by0, by1, bz0, bz1, lane);
+    Foot f = footprint<KIND>(g, d, bx0, bx1, by0, by1, bz0, bz1, lane);
     lds_float* const lds = (lds_float*)stage;
     if (f.staged) {
         fetch(f, code);
@@ -2011,8 +2027,8 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
     __syncthreads();
 
     for (int s = 0; s < a.n_views; s++) {
-        lds_float* cur = lds + (s & 1) * kStageBuf;
-        lds_float* nxt = lds + ((s + 1) & 1) * kStageBuf;
+        lds_float* cur = lds + (s & 1) * BwdBlock<KIND>::stage_buf;
+        lds_float* nxt = lds + ((s + 1) & 1) * BwdBlock<KIND>::stage_buf;
         // prefetch the next view's footprint
         int ncode = 0;
         ViewGeom ng;
@@ -2020,7 +2036,7 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
         if (s + 1 < a.n_views) {
             ncode = views[s + 1];
             ng = vg[ncode];
-            nf = footprint(ng, d, bx0, bx1, by0, by1, bz0, bz1, lane);
+            nf = footprint<KIND>(ng, d, bx0, bx1, by0, by1, bz0, bz1, lane);
             if (nf.staged) fetch(nf, ncode);
         }
         if (!f.empty) {
@@ -2233,21 +2249,24 @@ __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step
 }
 
 template <int KIND, bool AXIS, int CPW, int ZPL, int KM>
-__global__ __launch_bounds__(kBwdThreads) void k_backward(const BwdArgs a) {
-    extern __shared__ float stage[];  // 2 x kStageBuf
+// at least 4 waves per SIMD (at most 128 VGPRs): two 8-wave blocks, or one
+// 16-wave block, per CU
+__global__ __launch_bounds__(BwdBlock<KIND>::threads) __attribute__((amdgpu_waves_per_eu(4))) void k_backward(
+    const BwdArgs a) {
+    extern __shared__ float stage[];  // 2 x stage_buf
     const int o = (blockIdx.x % 8) * a.per_xcd + blockIdx.x / 8;
     if (o >= a.n_tiles) return;
     float m = backward_item<KIND, AXIS, CPW, ZPL, KM>(a, o, 0, stage);
     if constexpr (KIND == BWD_MLEM || KIND == BWD_ART) {
         if (a.vmax_out) {
             // the tile's max |updated voxel|: one atomic per block
-            lds_float* const wmax = (lds_float*)(stage + 2 * kStageBuf + kBwdExtraWords);
+            lds_float* const wmax = (lds_float*)(stage + 2 * BwdBlock<KIND>::stage_buf + bwd_extra_words<KIND>());
             for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
             if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
             __syncthreads();
             if (threadIdx.x == 0) {
                 float t = 0.f;
-                for (int w = 0; w < kBwdWaves; w++) t = fmaxf(t, wmax[w]);
+                for (int w = 0; w < BwdBlock<KIND>::waves; w++) t = fmaxf(t, wmax[w]);
                 __hip_atomic_fetch_max(a.vmax_out, __float_as_uint(t), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
@@ -2258,15 +2277,16 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward(const BwdArgs a) {
 // same work queue as k_forward_batch; items[i] is one problem with
 // item_end[i] - item_end[i-1] tiles, and its own tile shape (zpl, gx).
 template <int KIND, int ZPL, int CPW = 2, int KM = 1>
-__global__ __launch_bounds__(kBwdThreads) void k_backward_batch(const BwdArgs* __restrict__ items,
+__global__ __launch_bounds__(BwdBlock<KIND>::threads) __attribute__((amdgpu_waves_per_eu(4))) void k_backward_batch(
+    const BwdArgs* __restrict__ items,
                                                                 const int* __restrict__ item_end, int n_items,
                                                                 unsigned* queue, int step) {
-    extern __shared__ float stage[];  // 2 x kStageBuf; the last word holds the ticket
+    extern __shared__ float stage[];  // 2 x stage_buf; the last word holds the ticket
     typedef __attribute__((address_space(3))) unsigned lds_uint;
-    lds_uint* const tk = (lds_uint*)stage + (2 * kStageBuf - 1);
+    lds_uint* const tk = (lds_uint*)stage + (2 * BwdBlock<KIND>::stage_buf - 1);
     // past the staging and term areas: the last tile's per-wave max |voxel| and its item
-    lds_float* const wmax = (lds_float*)(stage + 2 * kStageBuf + kBwdExtraWords);
-    lds_int* const witem = (lds_int*)(wmax + kBwdWaves);
+    lds_float* const wmax = (lds_float*)(stage + 2 * BwdBlock<KIND>::stage_buf + bwd_extra_words<KIND>());
+    lds_int* const witem = (lds_int*)(wmax + BwdBlock<KIND>::waves);
     const int total = item_end[n_items - 1];
     if (threadIdx.x == 0) *witem = -1;
     for (;;) {
@@ -2277,7 +2297,7 @@ __global__ __launch_bounds__(kBwdThreads) void k_backward_batch(const BwdArgs* _
             unsigned* const dst = pi >= 0 ? items[pi].vmax_out : nullptr;
             if (dst) {
                 float m = 0.f;
-                for (int w = 0; w < kBwdWaves; w++) m = fmaxf(m, wmax[w]);
+                for (int w = 0; w < BwdBlock<KIND>::waves; w++) m = fmaxf(m, wmax[w]);
                 __hip_atomic_fetch_max(dst, __float_as_uint(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             *witem = -1;  // flushed: only a tile that sets it again sends a max
@@ -2985,11 +3005,29 @@ hipError_t launch_forward(const FwdArgs& a, int kind, bool axis, hipStream_t str
 
 int default_zpl(int N) { return N > 128 ? 4 : (N > 64 ? 2 : 1); }
 
+// staging buffers, per-view footprint and term areas, per-wave max slots (and
+// the persistent launch's item slot)
+template <int KD>
+constexpr size_t backward_lds_bytes(bool batch) {
+    return (2 * BwdBlock<KD>::stage_buf + bwd_extra_words<KD>() + BwdBlock<KD>::waves + (batch ? 1 : 0)) *
+           sizeof(float);
+}
+
+inline bool bwd_xtt(int kind) { return kind == BWD_XTT_RAW || kind == BWD_XTT_MLEM || kind == BWD_XTT_ART; }
+
+int backward_threads(int kind) {
+    return bwd_xtt(kind) ? BwdBlock<BWD_XTT_MLEM>::threads : BwdBlock<BWD_MLEM>::threads;
+}
+
+size_t backward_batch_lds_bytes(int kind) {
+    return bwd_xtt(kind) ? backward_lds_bytes<BWD_XTT_MLEM>(true) : backward_lds_bytes<BWD_MLEM>(true);
+}
+
 bool backward_shape(BwdArgs& a, int zpl, int gx, bool xtt) {
     const int N = a.d.N;
     // Plain / FDK: 2x2 columns x zpl z per lane, 16 waves as gx x gy column
     // groups (a (2gx) x (2gy)-column tile, 64*zpl z).  XTT (2K accumulators per
-    // voxel): one column x zpl (1 or 2) z per lane.
+    // voxel): one column x zpl (1 or 2) z per lane, 8 waves.
     const int CPW = xtt ? 1 : 2;
     if (xtt && zpl == 0) zpl = 1;
     if (xtt && zpl > 2) return false;
@@ -2997,9 +3035,11 @@ bool backward_shape(BwdArgs& a, int zpl, int gx, bool xtt) {
     if (gx == 0) gx = 4;
     if (!(zpl == 1 || zpl == 2 || zpl == 4)) return false;
     if (!(gx == 1 || gx == 2 || gx == 4 || gx == 8 || gx == 16)) return false;
+    const int waves = xtt ? BwdBlock<BWD_XTT_MLEM>::waves : BwdBlock<BWD_MLEM>::waves;
+    gx = std::min(gx, waves);  // XTT blocks have 8 waves: gx 16 is taken as 8
     a.zpl = zpl;
     a.gx = gx;
-    a.gy = kBwdWaves / gx;
+    a.gy = waves / gx;
     a.tiles_x = (N + CPW * a.gx - 1) / (CPW * a.gx);
     a.tiles_y = (N + CPW * a.gy - 1) / (CPW * a.gy);
     a.tiles_z = (N + 64 * zpl - 1) / (64 * zpl);
@@ -3013,10 +3053,9 @@ hipError_t launch_backward(BwdArgs a, int kind, bool axis, hipStream_t stream) {
     if (!backward_shape(a, a.zpl, a.gx, xtt)) return hipErrorInvalidValue;
     const int zpl = a.zpl;
     const unsigned grid = (unsigned)(8 * a.per_xcd);
-    // staging buffers, per-view footprint and term areas, per-wave max slots
-    const size_t lds = (2 * kStageBuf + kBwdExtraWords + kBwdWaves) * sizeof(float);
-#define XR_BWD(KD, A, C, Z, KM) \
-    hipLaunchKernelGGL((k_backward<KD, A, C, Z, KM>), dim3(grid), dim3(kBwdThreads), lds, stream, a)
+#define XR_BWD(KD, A, C, Z, KM)                                                                          \
+    hipLaunchKernelGGL((k_backward<KD, A, C, Z, KM>), dim3(grid), dim3(BwdBlock<KD>::threads),          \
+                       backward_lds_bytes<KD>(false), stream, a)
 #define XR_BWD_AX(KD, C, Z, KM)         \
     do {                                \
         if (axis)                       \
@@ -3134,10 +3173,8 @@ hipError_t launch_backward_batch(const BwdArgs* items, const int* item_end, int 
     const void* k = backward_batch_kernel(kind, zpl, kmax);
     if (!k) return hipErrorInvalidValue;
     void* args[] = {(void*)&items, (void*)&item_end, (void*)&n_items, (void*)&queue, (void*)&step};
-    return hipLaunchKernel(k, dim3(grid), dim3(kBwdThreads), args, backward_batch_lds_bytes(), stream);
+    return hipLaunchKernel(k, dim3(grid), dim3(backward_threads(kind)), args, backward_batch_lds_bytes(kind), stream);
 }
-
-size_t backward_batch_lds_bytes() { return (2 * kStageBuf + kBwdExtraWords + kBwdWaves + 1) * sizeof(float); }
 
 int batch_blocks_per_cu_forward(int kind, size_t lds_bytes, bool axis_all, int kmax) {
     int n = 0;
@@ -3175,7 +3212,8 @@ int batch_blocks_per_cu_backward(int kind, int zpl, int kmax) {
     int n = 0;
     const void* k = backward_batch_kernel(kind, zpl, kmax);
     if (!k) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBwdThreads, backward_batch_lds_bytes()) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, backward_threads(kind), backward_batch_lds_bytes(kind)) !=
+        hipSuccess)
         return 0;
     return n;
 }
