@@ -401,7 +401,9 @@ def grad_measure(ctx, xr, geom, sino, lam=1e-3):
     (N+1)^3 gradient grid (XR_CONV_GRAD plan: forwardProjGrad + projSubtract,
     backwardProjGrad + voxelPlus), and the Poisson integration of the N^3
     result (poissonSolveLDLT, poisson_cpu.cpp:15-112) by four FP64 sine-
-    transform products per z slice: 8 (N-2)^3 flops per slice."""
+    transform products per z slice: 8 (N-2)^3 flops per slice; then the whole
+    entry point (three slots of one epoch and the integration) on the same
+    data."""
     import torch
 
     N, U, V, P = geom.n_vox, geom.n_det_u, geom.n_det_v, geom.n_proj
@@ -434,12 +436,36 @@ def grad_measure(ctx, xr, geom, sino, lam=1e-3):
     ctx.timing(False)
     del gx, gy, out, vol
     torch.cuda.empty_cache()
+    # the entry point itself (xr_grad_reconstruct): three slots of one epoch
+    # (batch 1) and the Poisson integration; with a block size (32) dividing N
+    # the layers z >= N are held at zero and neither projector visits them
+    sino_h = sino.cpu().numpy()
+    rec = {}
+    for epochs in (1, 2):  # the difference is one epoch of the three slots
+        vol_h = np.zeros(N ** 3, np.float32)
+        ctx.timing(True)
+        ctx.timing_reset()
+        t0 = time.perf_counter()
+        ctx.grad_reconstruct(geom, xr.Scan(), sino_h, vol_h, epochs, 1, method=xr.METHOD_ART, lam=lam,
+                             block_size=32)
+        rec[epochs] = (time.perf_counter() - t0, ctx.timing_read(xr.TIMING_FORWARD)[0],
+                       ctx.timing_read(xr.TIMING_BACKWARD)[0])
+        ctx.timing(False)
+    del sino_h, vol_h
+    rec_s = rec[1][0]
+    rf_ms = (rec[2][1] - rec[1][1]) / 3
+    rb_ms = (rec[2][2] - rec[1][2]) / 3
+    tables_ms = rec[1][1] - 3 * rf_ms  # the plan's weight pass (fixed-point bound of the forward)
     vv = float(N1) ** 3 * P
     flops = 8.0 * (N - 2) ** 3 * N
     return {"workload": "gradient slot on the %d^3 grid (%d views, %dx%d detector) + Poisson integration of %d^3"
                         % (N1, P, U, V, N),
             "slot_epoch_ms": round(f_ms + b_ms, 3), "forward_ms": round(f_ms, 3), "backward_ms": round(b_ms, 3),
             "slot_gups": round(2.0 * vv / ((f_ms + b_ms) * 1e-3) / 1e9, 2),
+            "reconstruct": {"epochs": 1, "batch": 1, "block_size": 32, "wall_s": round(rec_s, 3),
+                            "slot_epoch_ms": round(rf_ms + rb_ms, 3), "forward_ms": round(rf_ms, 3),
+                            "backward_ms": round(rb_ms, 3), "plan_tables_ms": round(tables_ms, 3),
+                            "slot_gups": round(2.0 * vv / ((rf_ms + rb_ms) * 1e-3) / 1e9, 2)},
             "poisson_kernels_ms": round(p_ms, 3), "poisson_call_ms": round(call_s * 1e3, 3),
             "poisson_tflops_f64": round(flops / (p_ms * 1e-3) / 1e12, 2),
             "poisson_frac_f64_peak": round(flops / (p_ms * 1e-3) / 78.6e12, 4)}

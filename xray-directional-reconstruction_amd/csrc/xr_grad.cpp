@@ -80,8 +80,12 @@ int xr_grad_reconstruct(xr_context* ctx, const xr_geometry* g, const xr_scan* s,
     xr_plan* plan = nullptr;
     XR_TRY(xr_plan_create(ctx, &g1, s, XR_CONV_GRAD, direction, &plan));
     std::unique_ptr<xr_plan, void (*)(xr_plan*)> guard(plan, xr_plan_destroy);
-    XR_TRY(ensure_norm(plan));
     const int ext = (int)std::min<int64_t>(N1, ((int64_t)N + block_size - 1) / block_size * block_size);
+    // the layers z >= ext are held at zero while projecting: no projector pass
+    // visits them, the weight pass of the plan's tables included (a 513-voxel
+    // column is then 8 chunks of 64, not 9)
+    plan->dims.nz = ext;
+    XR_TRY(ensure_norm(plan));
     const int64_t V1 = (int64_t)N1 * N1 * N1, lenP = (int64_t)g->n_det_u * g->n_det_v, lenD = lenP * P;
     const int ss = (P + batch - 1) / batch;
     const size_t per_epoch = (size_t)batch * ss;

@@ -75,6 +75,7 @@ enum BwdKind { BWD_RAW = 0, BWD_MLEM = 1, BWD_ART = 2, BWD_FDK = 3, BWD_XTT_RAW 
 
 struct Dims {
     int N, U, V, P, K;
+    int nz;          // z extent the projectors visit (N; less where the layers above are known zero)
     float xc;        // voxel index of the centred origin ((N+1)/2 IR, (N-1)/2 FDK)
     float hu, hv;    // 0.5f*U, 0.5f*V
     float umax, vmax;  // (float)U - 0.55f, (float)V - 0.55f

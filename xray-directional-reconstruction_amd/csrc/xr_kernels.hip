@@ -224,7 +224,7 @@ struct FwdCol {
     float inv, u;
     int64_t base;  // column base index in the column layout
     float dx, dy;  // XTT source->voxel vector (x, y parts)
-    int z_beg, z_end;  // z range that can reach this launch's rows (AXIS), else [0, N)
+    int z_beg, z_end;  // z range that can reach this launch's rows (AXIS), else [0, nz)
 };
 
 // Value of x in lane `l` (wave-uniform l), into a scalar register.
@@ -237,11 +237,11 @@ __device__ __forceinline__ int lane_i(int x, int l) { return __builtin_amdgcn_re
 // v(z') = hv + (nv + av2 z') inv is linear in z'; a voxel reaches those rows
 // only if v in [v0 - 0.5, v0 + vb + 0.5).  The range is widened by two voxels
 // and every voxel is still tested exactly in the splat.  A chunk of all rows
-// keeps [0, N): the splat's window test rejects voxels that miss the detector.
+// keeps [0, nz): the splat's window test rejects voxels that miss the detector.
 __device__ __forceinline__ void col_zrange(const ViewGeom& g, const ColForms& c, float inv, const Dims& d, int v0,
                                            int vb, int& z_beg, int& z_end) {
     z_beg = 0;
-    z_end = d.N;
+    z_end = d.nz;
     if (v0 <= 0 && v0 + vb >= d.V) return;
     const float slope = g.av[2] * inv;
     if (fabsf(slope) > 1e-6f) {
@@ -249,10 +249,10 @@ __device__ __forceinline__ void col_zrange(const ViewGeom& g, const ColForms& c,
         const float zb = ((float)(v0 + vb) + 0.5f - d.hv - c.nv * inv) / slope;
         const float lo = fminf(za, zb) + d.xc - 2.0f, hi = fmaxf(za, zb) + d.xc + 2.0f;
         z_beg = (int)fmaxf(0.0f, floorf(lo));
-        z_end = (int)fminf((float)d.N, ceilf(hi) + 1.0f);
+        z_end = (int)fminf((float)d.nz, ceilf(hi) + 1.0f);
         if (!(lo == lo) || !(hi == hi)) {
             z_beg = 0;
-            z_end = d.N;
+            z_end = d.nz;
         }
     }
 }
@@ -264,16 +264,16 @@ __device__ __forceinline__ void col_zrange(const ViewGeom& g, const ColForms& c,
 __device__ __forceinline__ void col_zrange_t(float B, float A, const Dims& d, int v0, int vb, int& z_beg,
                                              int& z_end) {
     z_beg = 0;
-    z_end = d.N;
+    z_end = d.nz;
     if (fabsf(B) > 1e-6f) {
         const float za = ((float)(v0 - 1) - A) / B;
         const float zb = ((float)(v0 + vb) - A) / B;
         const float lo = fminf(za, zb) + d.xc - 2.0f, hi = fmaxf(za, zb) + d.xc + 2.0f;
         z_beg = (int)fmaxf(0.0f, floorf(lo));
-        z_end = (int)fminf((float)d.N, ceilf(hi) + 1.0f);
+        z_end = (int)fminf((float)d.nz, ceilf(hi) + 1.0f);
         if (!(lo == lo) || !(hi == hi)) {
             z_beg = 0;
-            z_end = d.N;
+            z_end = d.nz;
         }
     }
 }
@@ -290,7 +290,7 @@ __device__ __forceinline__ FwdCol fwd_col(const FwdArgs& a, const ViewGeom& g, i
     f.dx = (KIND == FWD_XTT) ? fmaf(d.vs, xp, g.d0[0]) : 0.f;
     f.dy = (KIND == FWD_XTT) ? fmaf(d.vs, yp, g.d0[1]) : 0.f;
     f.z_beg = 0;
-    f.z_end = d.N;
+    f.z_end = d.nz;
     if (AXIS) col_zrange(g, f.c, f.inv, d, v0, vb, f.z_beg, f.z_end);
     f.z_beg = uniformi(f.z_beg);
     f.z_end = uniformi(f.z_end);
@@ -696,20 +696,20 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                         const float w0 = lane_f(sw0, b), w1 = lane_f(sw1, b);
                         const float cxy = lane_f(xcxy, b);
                         const lds_ptr<AccT> cp = acc + lane_i(scp, b);
-                        int zb = 0, ze = d.N;
+                        int zb = 0, ze = d.nz;
                         if (!rows_all) col_zrange_t(Bc, Ac, d, v0, vb, zb, ze);
                         const int64_t cbase = ((int64_t)y * d.N + x0 + b) * d.N;
                         const unsigned row0 =
                             (unsigned)(uintptr_t)cp + (unsigned)((1 - v0) * kBandStride * (int)sizeof(AccT));
                         const lds_ptr<AccT> spare = cp + spare_row * kBandStride;
-                        const bool whole = (d.N & 63) == 0 && zb == 0 && ze == d.N;
+                        const bool whole = (d.nz & 63) == 0 && zb == 0 && ze == d.nz;
                         // every voxel of the column inside the window (t monotone in z)
-                        const float ta = fmaf(Bc, -d.xc, Ac), tb = fmaf(Bc, (float)(d.N - 1) - d.xc, Ac);
+                        const float ta = fmaf(Bc, -d.xc, Ac), tb = fmaf(Bc, (float)(d.nz - 1) - d.xc, Ac);
                         const bool inside = whole && fminf(ta, tb) > t_lo && fmaxf(ta, tb) < t_hi;
                         const auto col = vol + cbase + lane;
                         for (int zg = zb & ~63; zg < ze; zg += 64 * kXttChunks) {
                             float mu[kXttChunks][KM];
-                            if (EXACT && whole && zg + 64 * kXttChunks <= d.N) {
+                            if (EXACT && whole && zg + 64 * kXttChunks <= d.nz) {
                                 // every lane in range: unmasked loads, one address per channel
                                 // walked from the previous one (the chunks are immediate offsets)
 #pragma unroll
@@ -721,7 +721,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
 #pragma unroll
                                 for (int c = 0; c < kXttChunks; c++) {
                                     const int z = zg + c * 64 + lane;
-                                    const bool in = whole ? (zg + c * 64 < d.N) : (z >= zb && z < ze);
+                                    const bool in = whole ? (zg + c * 64 < d.nz) : (z >= zb && z < ze);
 #pragma unroll
                                     for (int k = 0; k < KM; k++)
                                         mu[c][k] = (in && (EXACT || k < d.K))
@@ -801,7 +801,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                         const float w00 = lane_f(sw0, b0), w01 = lane_f(sw1, b0);
                         const float w10 = lane_f(sw0, b1), w11 = lane_f(sw1, b1);
                         const lds_ptr<AccT> cp0 = acc + lane_i(scp, b0), cp1 = acc + lane_i(scp, b1);
-                        int zb0 = 0, ze0 = d.N, zb1 = 0, ze1 = d.N;
+                        int zb0 = 0, ze0 = d.nz, zb1 = 0, ze1 = d.nz;
                         if (!rows_all) {
                             col_zrange_t(B0, A0, d, v0, vb, zb0, ze0);
                             col_zrange_t(B1, A1, d, v0, vb, zb1, ze1);
@@ -823,14 +823,14 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                         // Whole columns (both z ranges are [0, N), N a multiple of 64): every
                         // loaded z is in range, so the loads need no masks and share one
                         // address per column.  Wave-uniform.
-                        const bool whole = KIND != FWD_WEIGHT && (d.N & 63) == 0 && zb0 == 0 && ze0 == d.N &&
-                                           zb1 == 0 && ze1 == d.N;
+                        const bool whole = KIND != FWD_WEIGHT && (d.nz & 63) == 0 && zb0 == 0 && ze0 == d.nz &&
+                                           zb1 == 0 && ze1 == d.nz;
                         const auto col0 = vol + base0 + lane, col1 = vol + base1 + lane;
                         // Inside: every voxel of a whole column lies in the window (t at the
                         // first and last z bound every lane's t: z - xc is exact and fmaf is
                         // monotone in it), so the splat needs no window test.
                         auto inside_col = [&](float B, float A) {
-                            const float ta = fmaf(B, -d.xc, A), tb = fmaf(B, (float)(d.N - 1) - d.xc, A);
+                            const float ta = fmaf(B, -d.xc, A), tb = fmaf(B, (float)(d.nz - 1) - d.xc, A);
                             return fminf(ta, tb) > t_lo && fmaxf(ta, tb) < t_hi;
                         };
                         const bool inside = whole && inside_col(B0, A0) && (!two || inside_col(B1, A1));
@@ -845,7 +845,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                             // not depend on where the row chunk clips the column.
                             for (int zg = zb & ~63; zg < ze; zg += 64 * kFwdChunks) {
                                 float m0[kFwdChunks], m1[kFwdChunks];
-                                if (WHOLE && zg + 64 * kFwdChunks <= d.N) {
+                                if (WHOLE && zg + 64 * kFwdChunks <= d.nz) {
                                     // all chunks in range: plain loads at immediate offsets
 #pragma unroll
                                     for (int c = 0; c < kFwdChunks; c++) {
@@ -857,7 +857,7 @@ __device__ __forceinline__ void forward_item(const FwdArgs& a, int o, int step, 
                                 for (int c = 0; c < kFwdChunks; c++) {
                                     const int z = zg + c * 64 + lane;
                                     if (WHOLE) {
-                                        const bool in = zg + c * 64 < d.N;
+                                        const bool in = zg + c * 64 < d.nz;
                                         m0[c] = in ? col0[zg + c * 64] : 0.f;
                                         if (TWO) m1[c] = in ? col1[zg + c * 64] : 0.f;
                                     } else if (KIND == FWD_WEIGHT) {
@@ -1313,7 +1313,8 @@ struct FootStage {
 // and write every voxel of the volume, so the maximum of the values written
 // here is the volume's max |voxel| exactly, and the next forward takes its
 // fixed-point scale from it.  A tile skipped for any reason (say, outside the
-// field of view) would leave that bound too small.
+// field of view) would leave that bound too small.  (Layers z >= nz are not
+// visited; a plan sets nz < N only where those layers are held at zero.)
 template <int KIND, int CPW, int ZPL, int KA>
 __device__ __forceinline__ float backward_update(const BwdArgs& a, const float (&tmp)[CPW * CPW][ZPL][KA],
                                                  const float (&fac)[CPW * CPW][ZPL][KA], const int (&nwhole)[CPW * CPW],
@@ -3042,7 +3043,7 @@ bool backward_shape(BwdArgs& a, int zpl, int gx, bool xtt) {
     a.gy = waves / gx;
     a.tiles_x = (N + CPW * a.gx - 1) / (CPW * a.gx);
     a.tiles_y = (N + CPW * a.gy - 1) / (CPW * a.gy);
-    a.tiles_z = (N + 64 * zpl - 1) / (64 * zpl);
+    a.tiles_z = (a.d.nz + 64 * zpl - 1) / (64 * zpl);  // z >= nz: never visited
     a.n_tiles = a.tiles_x * a.tiles_y * a.tiles_z;
     a.per_xcd = (a.n_tiles + 7) / 8;
     return true;

@@ -740,6 +740,7 @@ int xr_plan_create(xr_context* ctx, const xr_geometry* g, const xr_scan* s, int 
         }
     Dims& d = p->dims;
     d.N = g->n_vox;
+    d.nz = g->n_vox;
     d.U = g->n_det_u;
     d.V = g->n_det_v;
     d.P = P;
