@@ -122,6 +122,11 @@ FWD_CASES = {
     "offsets": dict(N=48, U=72, P=40, vs=0.5, cond_t=[0.7, -0.4, 0.3], cond_offset=[0.2, 0.5, -0.6]),
     "tilt20": dict(N=40, U=64, P=24, vs=0.5, cond_R=rot_x(20.0)),
     "tilt90": dict(N=40, U=64, P=24, vs=0.5, cond_R=rot_x(90.0) @ rot_y(5.0)),
+    # the volume's shadow is twice the detector's height: every footprint is
+    # clipped at the first or last image row, and the staging passes of the
+    # last views read past the end of the image buffer (rows that are never
+    # stored)
+    "clipped_v40": dict(N=64, U=96, V=40, P=30, vs=0.5),
 }
 
 
@@ -210,13 +215,13 @@ def test_forward_accumulate_flag(orc, xr, ctx, torch_dev):
     assert rel_l2(got, want) < 1e-6
 
 
-@pytest.mark.parametrize("name", ["c1_64", "odd_37_U50_V44", "tilt90"])
+@pytest.mark.parametrize("name", ["c1_64", "odd_37_U50_V44", "tilt90", "offsets", "clipped_v40"])
 def test_backward_matches_oracle(orc, xr, ctx, torch_dev, name):
     import torch
 
     st = Setup(orc, xr, **FWD_CASES[name])
     rng = np.random.default_rng(9)
-    views = sorted({0, 3, st.P // 2, st.P - 2})
+    views = sorted({0, 3, st.P // 2, st.P - 2, st.P - 1})
     img = rng.random((st.P, st.V, st.U)).astype(np.float32)
     plan = ctx.plan(st.geom, st.scan)
     N3 = st.N ** 3
