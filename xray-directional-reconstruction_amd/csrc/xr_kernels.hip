@@ -1169,11 +1169,15 @@ struct IsXtt {
 // per backprojection at 256^3 x 13 coefficients, 4 x 360 views; the plain
 // kinds in 8-wave blocks (half the tile, 6144-float staging) 70 against 67 ms
 // at 512^3 / 720 views.
+// The plain kinds' untilted path stages each view two views ahead in four
+// buffers, so that its block meets a barrier every second view; the other
+// paths double buffer in the first two.
 template <int KIND>
 struct BwdBlock {
     static constexpr int waves = IsXtt<KIND>::value ? 8 : 16;
     static constexpr int threads = 64 * waves;
-    static constexpr int stage_buf = IsXtt<KIND>::value ? 6144 : 8192;  // floats per staging buffer
+    static constexpr int nbuf = IsXtt<KIND>::value ? 2 : 4;              // staging buffers
+    static constexpr int stage_buf = IsXtt<KIND>::value ? 6144 : 8000;  // floats per staging buffer
     static constexpr int stage_per = stage_buf / threads;              // staged floats per thread
 };
 
@@ -1425,8 +1429,9 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
     for (int q = 0; q < ZPL; q++) zpq[q] = (float)(z0 + lane + 64 * q) - d.xc;
 
     lds_float* const lds = (lds_float*)stage;
-    lds_int* const foot = (lds_int*)(stage + 2 * BwdBlock<KIND>::stage_buf);
-    lds_float* const term = (lds_float*)(stage + 2 * BwdBlock<KIND>::stage_buf + kBwdFootArea + wave * kBwdTermArea);
+    lds_int* const foot = (lds_int*)(stage + BwdBlock<KIND>::nbuf * BwdBlock<KIND>::stage_buf);
+    lds_float* const term =
+        (lds_float*)(stage + BwdBlock<KIND>::nbuf * BwdBlock<KIND>::stage_buf + kBwdFootArea + wave * kBwdTermArea);
 
     // footprints of views b0 .. b0+15 into buffer buf: lane = 4 x view + corner
     // pair (x, y corner; both z corners), extremes over the 4 lanes of a view
@@ -1535,14 +1540,29 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
         st.fetch(a.proj + (int64_t)(code - a.code_base) * d.V * d.U, img_bytes, f, d.U);
     };
 
+    // View v is staged in buffer v % 4 two views ahead (fetched into registers
+    // during view v-2, stored after it) and the block meets a barrier after
+    // every odd view.  A store into buffer v % 4 then follows, across at least
+    // one barrier, the last reads of view v-4 there, and precedes the reads of
+    // view v across one; footprint batches are written four views ahead.
+    static_assert(BwdBlock<KIND>::nbuf == 4, "four staging buffers");
+    constexpr int SB = BwdBlock<KIND>::stage_buf;
     if (wave == 0) foot_batch(0, 0);
     __syncthreads();
     term_batch(0, 0);
-    int code = views[0];
-    Foot f = load_foot(0, 0);
+    int code = views[0], code1 = 0;
+    Foot f = load_foot(0, 0), f1{0, 0, 0, 0, 0, 0, false, false};
     if (f.staged) {
         fetch(f, code);
         st.store(lds, f);
+    }
+    if (nv > 1) {
+        code1 = views[1];
+        f1 = load_foot(0, 1);
+        if (f1.staged) {
+            fetch(f1, code1);
+            st.store(lds + SB, f1);
+        }
     }
     __syncthreads();
 
@@ -1550,15 +1570,15 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
         const int j = s & (kBwdBatch - 1);
         if (j == 0 && s > 0) term_batch(s, (s / kBwdBatch) & 1);
         wave_sync();
-        // the next batch's footprints, read from the next iteration on
-        if (wave == 0 && j == kBwdBatch - 2 && s + 2 < nv) foot_batch(s + 2, ((s + 2) / kBwdBatch) & 1);
-        lds_float* cur = lds + (s & 1) * BwdBlock<KIND>::stage_buf;
-        lds_float* nxt = lds + ((s + 1) & 1) * BwdBlock<KIND>::stage_buf;
+        // the footprints of the batch after this one: read from view s + 2 on
+        if (wave == 0 && j == kBwdBatch - 4 && s + 4 < nv) foot_batch(s + 4, ((s + 4) / kBwdBatch) & 1);
+        lds_float* cur = lds + (s & 3) * SB;
+        lds_float* nxt = lds + ((s + 2) & 3) * SB;
         int ncode = 0;
         Foot nf{0, 0, 0, 0, 0, 0, false, false};
-        if (s + 1 < nv) {
-            ncode = views[s + 1];
-            nf = load_foot(((s + 1) / kBwdBatch) & 1, (s + 1) & (kBwdBatch - 1));
+        if (s + 2 < nv) {
+            ncode = views[s + 2];
+            nf = load_foot(((s + 2) / kBwdBatch) & 1, (s + 2) & (kBwdBatch - 1));
             if (nf.staged) fetch(nf, ncode);
         }
         if (!f.empty) {
@@ -1642,12 +1662,12 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
                 }
             }
         }
-        if (s + 1 < nv) {
-            if (nf.staged) st.store(nxt, nf);
-            code = ncode;
-            f = nf;
-        }
-        __syncthreads();
+        if (s + 2 < nv && nf.staged) st.store(nxt, nf);
+        code = code1;
+        f = f1;
+        code1 = ncode;
+        f1 = nf;
+        if (s & 1) __syncthreads();
     }
     return backward_update<KIND, CPW, ZPL, 1>(a, tmp, fac, nwhole, xb, yb, z0, lane);
 }
@@ -1706,8 +1726,9 @@ __device__ __forceinline__ float backward_item_untilted_xtt(const BwdArgs& a, in
     for (int q = 0; q < ZPL; q++) zpq[q] = (float)(z0 + lane + 64 * q) - d.xc;
 
     lds_float* const lds = (lds_float*)stage;
-    lds_int* const foot = (lds_int*)(stage + 2 * BwdBlock<KIND>::stage_buf);
-    lds_float* const term = (lds_float*)(stage + 2 * BwdBlock<KIND>::stage_buf + kBwdFootArea + wave * kBwdTermArea);
+    lds_int* const foot = (lds_int*)(stage + BwdBlock<KIND>::nbuf * BwdBlock<KIND>::stage_buf);
+    lds_float* const term =
+        (lds_float*)(stage + BwdBlock<KIND>::nbuf * BwdBlock<KIND>::stage_buf + kBwdFootArea + wave * kBwdTermArea);
     // staged only when the footprint fits the fetch passes of this path
     auto staged_here = [](int fl, int H, int R) { return (fl & 1) != 0 && H <= kXttStagePer * R; };
 
@@ -2261,7 +2282,8 @@ __global__ __launch_bounds__(BwdBlock<KIND>::threads) __attribute__((amdgpu_wave
     if constexpr (KIND == BWD_MLEM || KIND == BWD_ART) {
         if (a.vmax_out) {
             // the tile's max |updated voxel|: one atomic per block
-            lds_float* const wmax = (lds_float*)(stage + 2 * BwdBlock<KIND>::stage_buf + bwd_extra_words<KIND>());
+            lds_float* const wmax =
+                (lds_float*)(stage + BwdBlock<KIND>::nbuf * BwdBlock<KIND>::stage_buf + bwd_extra_words<KIND>());
             for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
             if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
             __syncthreads();
@@ -2286,7 +2308,8 @@ __global__ __launch_bounds__(BwdBlock<KIND>::threads) __attribute__((amdgpu_wave
     typedef __attribute__((address_space(3))) unsigned lds_uint;
     lds_uint* const tk = (lds_uint*)stage + (2 * BwdBlock<KIND>::stage_buf - 1);
     // past the staging and term areas: the last tile's per-wave max |voxel| and its item
-    lds_float* const wmax = (lds_float*)(stage + 2 * BwdBlock<KIND>::stage_buf + bwd_extra_words<KIND>());
+    lds_float* const wmax =
+        (lds_float*)(stage + BwdBlock<KIND>::nbuf * BwdBlock<KIND>::stage_buf + bwd_extra_words<KIND>());
     lds_int* const witem = (lds_int*)(wmax + BwdBlock<KIND>::waves);
     const int total = item_end[n_items - 1];
     if (threadIdx.x == 0) *witem = -1;
@@ -3010,7 +3033,8 @@ int default_zpl(int N) { return N > 128 ? 4 : (N > 64 ? 2 : 1); }
 // the persistent launch's item slot)
 template <int KD>
 constexpr size_t backward_lds_bytes(bool batch) {
-    return (2 * BwdBlock<KD>::stage_buf + bwd_extra_words<KD>() + BwdBlock<KD>::waves + (batch ? 1 : 0)) *
+    return (BwdBlock<KD>::nbuf * BwdBlock<KD>::stage_buf + bwd_extra_words<KD>() + BwdBlock<KD>::waves +
+            (batch ? 1 : 0)) *
            sizeof(float);
 }
 
