@@ -1169,15 +1169,15 @@ struct IsXtt {
 // per backprojection at 256^3 x 13 coefficients, 4 x 360 views; the plain
 // kinds in 8-wave blocks (half the tile, 6144-float staging) 70 against 67 ms
 // at 512^3 / 720 views.
-// The plain kinds' untilted path stages each view two views ahead in four
-// buffers, so that its block meets a barrier every second view; the other
-// paths double buffer in the first two.
+// The untilted paths stage each view two views ahead in four buffers, so
+// that a block meets a barrier every second view; the general (tilted) path
+// double buffers in the first two.
 template <int KIND>
 struct BwdBlock {
     static constexpr int waves = IsXtt<KIND>::value ? 8 : 16;
     static constexpr int threads = 64 * waves;
-    static constexpr int nbuf = IsXtt<KIND>::value ? 2 : 4;              // staging buffers
-    static constexpr int stage_buf = IsXtt<KIND>::value ? 6144 : 8000;  // floats per staging buffer
+    static constexpr int nbuf = 4;                                      // staging buffers
+    static constexpr int stage_buf = IsXtt<KIND>::value ? 4000 : 8000;  // floats per staging buffer
     static constexpr int stage_per = stage_buf / threads;              // staged floats per thread
 };
 
@@ -1863,14 +1863,27 @@ __device__ __forceinline__ float backward_item_untilted_xtt(const BwdArgs& a, in
             if (k * f.R < f.H && pidx[k] >= 0) buf[pidx[k]] = pre[k];
     };
 
+    // the staging schedule of the plain untilted path: view v in buffer v % 4,
+    // fetched during view v-2, a barrier after every odd view, footprint
+    // batches written four views ahead
+    static_assert(BwdBlock<KIND>::nbuf == 4, "four staging buffers");
+    constexpr int SB = BwdBlock<KIND>::stage_buf;
     if (wave == 0) foot_batch(0, 0);
     __syncthreads();
     term_batch(0, 0);
-    int code = views[0];
-    Foot f = load_foot(0, 0);
+    int code = views[0], code1 = 0;
+    Foot f = load_foot(0, 0), f1{0, 0, 0, 0, 0, 0, false, false};
     if (f.staged) {
         fetch(f, code);
         store(f, lds);
+    }
+    if (nv > 1) {
+        code1 = views[1];
+        f1 = load_foot(0, 1);
+        if (f1.staged) {
+            fetch(f1, code1);
+            store(f1, lds + SB);
+        }
     }
     __syncthreads();
 
@@ -1878,14 +1891,14 @@ __device__ __forceinline__ float backward_item_untilted_xtt(const BwdArgs& a, in
         const int j = s & (kBwdBatch - 1);
         if (j == 0 && s > 0) term_batch(s, (s / kBwdBatch) & 1);
         wave_sync();
-        if (wave == 0 && j == kBwdBatch - 2 && s + 2 < nv) foot_batch(s + 2, ((s + 2) / kBwdBatch) & 1);
-        lds_float* cur = lds + (s & 1) * BwdBlock<KIND>::stage_buf;
-        lds_float* nxt = lds + ((s + 1) & 1) * BwdBlock<KIND>::stage_buf;
+        if (wave == 0 && j == kBwdBatch - 4 && s + 4 < nv) foot_batch(s + 4, ((s + 4) / kBwdBatch) & 1);
+        lds_float* cur = lds + (s & 3) * SB;
+        lds_float* nxt = lds + ((s + 2) & 3) * SB;
         int ncode = 0;
         Foot nf{0, 0, 0, 0, 0, 0, false, false};
-        if (s + 1 < nv) {
-            ncode = views[s + 1];
-            nf = load_foot(((s + 1) / kBwdBatch) & 1, (s + 1) & (kBwdBatch - 1));
+        if (s + 2 < nv) {
+            ncode = views[s + 2];
+            nf = load_foot(((s + 2) / kBwdBatch) & 1, (s + 2) & (kBwdBatch - 1));
             if (nf.staged) fetch(nf, ncode);
         }
         if (!f.empty && col_in) {
@@ -1971,12 +1984,12 @@ __device__ __forceinline__ float backward_item_untilted_xtt(const BwdArgs& a, in
                 }
             }
         }
-        if (s + 1 < nv) {
-            if (nf.staged) store(nf, nxt);
-            code = ncode;
-            f = nf;
-        }
-        __syncthreads();
+        if (s + 2 < nv && nf.staged) store(nf, nxt);
+        code = code1;
+        f = f1;
+        code1 = ncode;
+        f1 = nf;
+        if (s & 1) __syncthreads();
     }
     return backward_update<KIND, 1, ZPL, KM>(a, tmp, fac, nwhole, xb, yb, z0, lane);
 }
