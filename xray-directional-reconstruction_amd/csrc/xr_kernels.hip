@@ -1995,7 +1995,7 @@ __device__ __forceinline__ float backward_item_untilted_xtt(const BwdArgs& a, in
 }
 
 // One voxel tile o of a backprojection launch; step selects the view list
-// views + step*n_views (a solver's subset step).  stage: 2 x stage_buf floats.
+// views + step*n_views (a solver's subset step).  stage: nbuf x stage_buf floats, then the footprint and term areas.
 // Returns the lane's max |updated voxel| (MLEM / ART kinds).
 template <int KIND, bool AXIS, int CPW, int ZPL, int KM>
 __device__ __forceinline__ float backward_item(const BwdArgs& a, int o, int step, float* stage) {
@@ -2288,7 +2288,7 @@ template <int KIND, bool AXIS, int CPW, int ZPL, int KM>
 // 16-wave block, per CU
 __global__ __launch_bounds__(BwdBlock<KIND>::threads) __attribute__((amdgpu_waves_per_eu(4))) void k_backward(
     const BwdArgs a) {
-    extern __shared__ float stage[];  // 2 x stage_buf
+    extern __shared__ float stage[];  // nbuf x stage_buf staging, footprint and term areas
     const int o = (blockIdx.x % 8) * a.per_xcd + blockIdx.x / 8;
     if (o >= a.n_tiles) return;
     float m = backward_item<KIND, AXIS, CPW, ZPL, KM>(a, o, 0, stage);
@@ -2317,7 +2317,7 @@ __global__ __launch_bounds__(BwdBlock<KIND>::threads) __attribute__((amdgpu_wave
     const BwdArgs* __restrict__ items,
                                                                 const int* __restrict__ item_end, int n_items,
                                                                 unsigned* queue, int step) {
-    extern __shared__ float stage[];  // 2 x stage_buf; the last word holds the ticket
+    extern __shared__ float stage[];  // nbuf x stage_buf; the last word of buffer 1 holds the ticket
     typedef __attribute__((address_space(3))) unsigned lds_uint;
     lds_uint* const tk = (lds_uint*)stage + (2 * BwdBlock<KIND>::stage_buf - 1);
     // past the staging and term areas: the last tile's per-wave max |voxel| and its item
