@@ -1152,10 +1152,11 @@ __global__ __launch_bounds__(64 * kFwdWaves) __attribute__((amdgpu_waves_per_eu(
 // as a gx x gy grid of CPW x CPW column groups; every lane owns ZPL voxels
 // (z = z0 + lane + 64 q) in each of its CPW^2 columns, so the per-column
 // geometry of a view is computed once per ZPL voxels.  The block's detector
-// footprint for each view is staged in LDS, double buffered: the footprint of
-// view s+1 is fetched into registers while view s is accumulated, so there is
-// one barrier per view.  The update (MLEM / ART / FDK / raw) is applied at the
-// end: each voxel is read and written once per subset.
+// footprint for each view is staged in LDS: the untilted paths fetch view s+2
+// into registers while view s is accumulated (four buffers, a barrier every
+// second view), the general path view s+1 (two buffers, a barrier per view).
+// The update (MLEM / ART / FDK / raw) is applied at the end: each voxel is
+// read and written once per subset.
 // ---------------------------------------------------------------------------
 template <int KIND>
 struct IsXtt {
@@ -1684,8 +1685,9 @@ __device__ __forceinline__ float backward_item_untilted(const BwdArgs& a, int o,
 // its own column (lane = view x channel quarter), all into LDS; the view loop
 // reads them back with broadcast LDS loads.  The view's sig_k S_k,z and g2s_k
 // are scalar loads of the plan's weight table.  Footprints are staged in
-// LDS, double buffered, with as many fetch passes as the footprint needs
-// (at most kXttStagePer; a taller one is read from global memory directly).
+// LDS two views ahead in four buffers, with as many fetch passes as the
+// footprint needs (at most kXttStagePer; a taller one is read from global
+// memory directly).
 // ---------------------------------------------------------------------------
 constexpr int kXttStagePer = 4;
 
