@@ -143,7 +143,7 @@ float xr_default_vox_size(float sdd, float sod, float det_size);
  *   bwd_zpl:  z voxels per lane in the backprojection (1, 2, 4; default by N)
  *   bwd_gx:   column groups along x in a backprojection block of 16 waves
  *             (1, 2, 4, 8, 16; gy = 16/gx; default 4); XTT kinds run 8-wave
- *             blocks, where a gx of 16 is taken as 8 (gy = 8/gx)
+ *             blocks, where a gx of 16 is taken as 8 (gy = 8/gx; default 2)
  *   fwd_rows: cap on the detector rows of one forward launch (default: the
  *             most that fit 64 KB of LDS)
  *   fwd_scan: 0 (default): the forward of an untilted view scans, per volume
